@@ -1,0 +1,165 @@
+/*
+ * nfx.h -- C ABI of libnfx.so, the MI355X (gfx950) implementation of noFluidX3D's explicit
+ * tetrahedral-FEM time step.
+ *
+ * This boundary replaces PyOpenCL, nofluidx3d/openCL.py, nofluidx3d/KernelBuilder.py and the
+ * OpenCL kernels under nofluidx3d/kernels/ (reference paths).  The Python classes Simulation,
+ * TetraCell, Interaction, Tetra, Sphere, Substrate, Plane, VelocityVerlet and Bridge keep their
+ * API and bind these entry points with ctypes.
+ *
+ * Conventions
+ * - Every array argument is a DEVICE pointer owned by the caller (torch tensors in the Python
+ *   layer) unless the name ends in _host.  The library never frees caller memory.
+ * - Layouts are the reference's structure-of-arrays: component c of node n at c*N + n
+ *   (Simulation.py:82-84), corner a of tetra t at a*T + t (Simulation.py:85-87), matrix entry
+ *   (i,j) of tetra t at (3*i+j)*T + t (Tetra.py:53-58).
+ * - State is FP32 on the device; the one-time inverse-shape-matrix/volume precompute and the
+ *   tip-force reduction are FP64.
+ * - All calls are asynchronous on the context's stream; nfx_stream_sync waits.  One host thread
+ *   drives one context (the reference drives one in-order queue, openCL.py:34).
+ * - Return value 0 on success, a negative NFX_ERR_* code otherwise; nfx_last_error() then holds
+ *   a thread-local message.  The reference's hard exits (openCL.py:30-32, util/IO.py:17-28) are
+ *   reproduced by the Python layer only.
+ */
+#ifndef NFX_H
+#define NFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NFX_OK 0
+#define NFX_ERR_ARG (-1)
+#define NFX_ERR_DEVICE (-2)
+#define NFX_ERR_HIP (-3)
+#define NFX_ERR_RANGE (-4)
+
+/* material models: Tetra.materialModel (interactions/Tetra.py:25-33); NEO_HOOKEAN is MOONEY_RIVLIN
+ * with ratio 1 (Tetra.py:77-78, tetra.cl:115) */
+#define NFX_SVKK 0
+#define NFX_SVK 1
+#define NFX_MR 2
+#define NFX_VISCOPLASTIC 3
+
+#define NFX_SCATTER_ATOMIC 0 /* float atomics, as the reference's atomicAdd (atomicAdd.cl:1-11) */
+#define NFX_SCATTER_CSR 1    /* deterministic node<-corner gather, fixed summation order */
+
+#define NFX_INDENTER_NONE 0
+#define NFX_INDENTER_SPHERE 1 /* Sphere.cl */
+#define NFX_INDENTER_PLANE 2  /* Plane.cl */
+
+typedef struct nfx_ctx nfx_ctx;
+
+/* Elastic constants in simulation units: the #defines of interactions/Tetra.py:66-79. */
+typedef struct {
+    int32_t model; /* NFX_SVKK, NFX_SVK or NFX_MR */
+    double E, nu, ratio;
+} nfx_elastic;
+
+/* Finite-strain viscoplastic parameters (uniform per tetra; Interactions.py:270-277 builds eight
+ * uniform per-tetra arrays, SimulationLegacy.py:842-882 sets their values). */
+typedef struct {
+    double mu1, mu2, kappa;
+    double flow_exponent;       /* m */
+    double yield_stress;        /* Y: sigma0 = Y (1 + epsbar/eps0)^n */
+    double flow_rate;           /* eps0dot per step */
+    double hardening_threshold; /* eps0 */
+    double hardening_exponent;  /* n */
+    double yield_offset;        /* sigma_y >= 0; rate uses max(svm - sigma_y, 0); 0 = reference law */
+} nfx_viscoplastic;
+
+/* Everything one fused step of one cell touches. */
+typedef struct {
+    int64_t N, T;
+    const uint32_t* tetras; /* 4T  SoA */
+    const float* dminv;     /* 9T  SoA, inverse reference shape matrix */
+    const float* v0;        /* T   reference volume */
+    float* points;          /* 3N  SoA */
+    float* vel;             /* 3N */
+    float* force;           /* 3N  (atomic scatter target; cleared by the node update) */
+    float* fold;            /* 3N */
+    int32_t model;          /* NFX_SVKK / NFX_SVK / NFX_MR / NFX_VISCOPLASTIC */
+    double E, nu, ratio;
+    nfx_viscoplastic vp;
+    float* Fp;        /* 9T, model NFX_VISCOPLASTIC */
+    float* epsbar;    /* T */
+    float* von_mises; /* T, nullable: written by the viscoplastic force when non-null */
+    float* pressure;  /* T, nullable */
+    int32_t scatter;  /* NFX_SCATTER_ATOMIC or NFX_SCATTER_CSR */
+    const uint32_t* csr_offsets; /* N+1, incidences of node n at [off[n], off[n+1]) */
+    const uint32_t* csr_slots;   /* 4T, slot = 4*t + corner, ascending per node */
+    float* corners;              /* 12T scratch: force of corner a of tetra t at 12*t + 3*a + c */
+    int32_t substrate;
+    double substrate_k, wall;
+    int32_t indenter; /* NFX_INDENTER_* */
+    double ind_k, ind_radius, ind_start, ind_speed, ind_move_steps;
+    int32_t fix_top_bottom;
+} nfx_plan;
+
+/* ---- device and queue: replaces openCL.py:17-50 -------------------------------------------- */
+int nfx_device_count(int* count);                        /* openCL.py:20-28 device walk */
+int nfx_init(int device_id, void* stream, nfx_ctx** ctx); /* openCL.py:17-34 initializeOpenCLObjects;
+                                                             stream: hipStream_t to use, or NULL for a
+                                                             library-owned default-priority stream */
+int nfx_destroy(nfx_ctx* ctx);
+int nfx_stream_sync(nfx_ctx* ctx); /* getCommandQueue().finish(), Simulation.py:266 */
+const char* nfx_last_error(void);
+int nfx_device_name(nfx_ctx* ctx, char* buf, int32_t len);
+
+/* ---- one-time FP64 precompute: the inversion of Dm = [R1|R2|R3] that tetra.cl:47-53 repeats
+ *      every step.  Inputs are Tetra's buffers (interactions/Tetra.py:53-58): edge_vectors 9T FP64
+ *      SoA (TetraCell.py:26-30) and volumes T FP64 (TetraCell.py:40-41).  Outputs FP32. ---------- */
+int nfx_precompute_f64(nfx_ctx* ctx, const double* edge_vectors, const double* volumes, int64_t T, float* dminv,
+                       float* v0);
+
+/* ---- per-plugin kernels, one per reference kernel ------------------------------------------ */
+/* tetra.cl:14-182 calcTetraForce with the atomic node scatter (force += ...) */
+int nfx_tetra_force(nfx_ctx* ctx, const nfx_elastic* mat, const float* points, const uint32_t* tetras,
+                    const float* dminv, const float* v0, int64_t N, int64_t T, float* force);
+/* same forces written per corner (no scatter): corners[12*t + 3*a + c] */
+int nfx_tetra_corner_forces(nfx_ctx* ctx, const nfx_elastic* mat, const float* points, const uint32_t* tetras,
+                            const float* dminv, const float* v0, int64_t N, int64_t T, float* corners);
+/* InteractionFiniteStrainViscoplastic.cl:1-220 (node-0 edge base); atomic scatter when corners
+ * is NULL, per-corner output otherwise */
+int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float* points, const uint32_t* tetras,
+                           const float* dminv, int64_t N, int64_t T, float* Fp, float* epsbar, float* von_mises,
+                           float* pressure, float* force, float* corners);
+/* deterministic gather: force[c*N+n] += sum over slots of node n, in slot order */
+int nfx_gather_corners(nfx_ctx* ctx, const uint32_t* csr_offsets, const uint32_t* csr_slots, const float* corners,
+                       int64_t N, float* force);
+/* host-side CSR build (host pointers): offsets N+1, slots 4T ascending per node */
+int nfx_build_node_csr_host(const uint32_t* tetras_host, int64_t N, int64_t T, uint32_t* offsets_host,
+                            uint32_t* slots_host);
+int nfx_sphere_force(nfx_ctx* ctx, float* force, const float* points, int64_t N, double k, double radius,
+                     double sphere_pos);                                                       /* Sphere.cl */
+int nfx_substrate_force(nfx_ctx* ctx, float* force, const float* points, int64_t N, double k, double wall_pos); /* Substrate.cl */
+int nfx_plane_force(nfx_ctx* ctx, float* force, const float* points, int64_t N, double k, double wall_pos);     /* Plane.cl */
+int nfx_velocity_verlet(nfx_ctx* ctx, float* points, float* vel, float* force, float* fold, int64_t N,
+                        int32_t fix_top_bottom); /* VelocityVerlet.cl */
+/* Simulation.py:146-153 / 198-212 forceSI and "Max dis" in simulation units, reduced in FP64 on the
+ * device: out2[0] = indenter force, out2[1] = max penetration (device pointer to 2 doubles). */
+int nfx_tip_force(nfx_ctx* ctx, const float* points, int64_t N, int32_t indenter, double k, double radius,
+                  double pos, double* out2);
+
+/* ---- fused stepping: Simulation.timeStep (Simulation.py:232-237) for steps
+ *      [step0, step0 + nsteps): force pass, then one node pass that applies the substrate and
+ *      indenter terms at the device-computed indenter position (Simulation.py:117-132, 173-183)
+ *      and the damped velocity-Verlet update.  tip_hist (device, nullable, nsteps doubles)
+ *      receives the FP64 indenter force of each step.  use_graph != 0 replays the steps from a
+ *      captured hipGraph of up to 100 steps. ------------------------------------------------- */
+int nfx_run_steps(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int64_t nsteps, double* tip_hist,
+                  int32_t use_graph);
+/* the node pass alone (for callers that enqueue the force pass themselves) */
+int nfx_node_update(nfx_ctx* ctx, const nfx_plan* plan, int64_t step, double* tip_out);
+
+/* ---- kernel timing for the benchmark: average device time of the force pass and of the node
+ *      pass over nrep steps, measured with HIP events on the context's stream (ms). ---------- */
+int nfx_time_passes(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int32_t nrep, double* ms_force,
+                    double* ms_node);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NFX_H */

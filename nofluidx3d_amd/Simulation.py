@@ -1,0 +1,361 @@
+"""AFM indentation simulation of one tetrahedral cell (reference ``nofluidx3d/Simulation.py``).
+
+Same constructor, setters, ``register``, ``timeStep`` and ``run`` as the reference, the same JSON
+config keys and unit system (``Simulation.py:42-56``), and the same recorded quantities
+(time, indentation [m], tip force [N]) written to ``simdata.dat``.
+
+Stepping.  When the registered plugins form the reference's standard chain -- one force plugin
+(Tetra or FiniteStrainViscoplastic), optionally a Substrate and one indenter (Sphere or Plane)
+whose position law is known, and VelocityVerlet -- ``timeStep`` and ``run`` advance with the
+fused device step ``nfx_run_steps`` (a force pass and one node pass per step, indenter position
+computed on the device, replayed from a captured hipGraph of up to 100 steps).  Any other chain
+runs plugin by plugin, exactly like the reference's loop (``Simulation.py:232-237``).
+
+Extensions (keyword arguments, defaults = reference behaviour where one exists):
+``scatter`` "atomic" (float atomics, as the reference) or "csr" (deterministic gather);
+``fused`` and ``use_graph`` to switch the fused step and graph replay off.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+
+from nofluidx3d_amd import _lib, openCL
+from nofluidx3d_amd.interactions import FiniteStrainViscoplastic, Plane, Sphere, Substrate, Tetra, VelocityVerlet
+from nofluidx3d_amd.openCL import getCommandQueue, initializeOpenCLObjects
+from nofluidx3d_amd.TetraCell import TetraCell
+from nofluidx3d_amd.util.Bridge import Bridge
+from nofluidx3d_amd.util.IO import writeVTK
+
+
+def _law(fn, start, speed, move_steps):
+    fn.law = (start, speed, move_steps)
+    return fn
+
+
+class Simulation:
+    def __init__(self, con, scatter="atomic", fused=True, use_graph=True, mesh=None, verbose=True):
+        if isinstance(con, dict):
+            self.parameters = dict(con)
+        else:
+            with open(con) as configfile:
+                self.parameters = json.load(configfile)
+        if scatter not in ("atomic", "csr"):
+            raise ValueError(f"scatter must be 'atomic' or 'csr', not {scatter!r}")
+        self.scatter = scatter
+        self.fused = fused
+        self.use_graph = use_graph
+        self.verbose = verbose
+
+        initializeOpenCLObjects(self.parameters["gpu"])
+
+        # Calculate all conversion constants (Simulation.py:42-56)
+        rho = 1.0
+        self.rho0 = self.parameters["RhoSI"] / rho
+        self.L0 = self.parameters["CELL"]["RadiusSI"] / self.parameters["CELL"]["RadiusSIM"]
+
+        E = 1e-2
+        self.p0 = self.parameters["CELL"]["YoungsModulusSI"] / E
+
+        self.V0 = np.sqrt(self.p0 / self.rho0)
+        self.T0 = self.L0 / self.V0
+
+        self._print(f"V0={self.V0}")
+        self._print(f"T0={self.T0}")
+
+        self.Nsteps = int(round((self.parameters["SimulationTimeSI"]) / self.T0))
+        self.VTKInterval = int(round(self.Nsteps / self.parameters["VTKFramesTotal"]))
+        self.vtkfilespath = self.parameters["OutputDir"] + "vtkfiles/frame"
+        os.makedirs(self.parameters["OutputDir"] + "vtkfiles", exist_ok=True)
+
+        self.cell = TetraCell(
+            mesh if mesh is not None else self.parameters["CELL"]["InitVTK"],
+            radius=self.parameters["CELL"]["RadiusSIM"],
+            recenter=True,
+        )
+
+        self.numPoints = self.cell.points[:, 0].size
+        self.numTetra = self.cell.tetras[:, 0].size
+        # the kernels index node arrays with these ids without a bounds check
+        if self.numTetra == 0 or self.cell.tetras.min() < 0 or self.cell.tetras.max() >= self.numPoints:
+            raise ValueError("tetra connectivity references nodes outside the mesh")
+
+        self._print(f"Particle mesh consists of {self.numPoints} vertices and {self.numTetra} tetrahedrons")
+        self.interactions = []
+        self.time = 0
+
+        # device buffers shared by several kernels (Simulation.py:82-90), FP32 state on the device
+        self.points = Bridge(self.cell.points.reshape(3 * self.numPoints, order="F"), datatype=np.float64)
+        self.tetras = Bridge(self.cell.tetras.reshape(4 * self.numTetra, order="F"), datatype=np.uint32)
+        self.force = Bridge(np.zeros(self.numPoints * 3), datatype=np.float64)
+        self.vonMises = Bridge(np.zeros(self.numTetra), datatype=np.float64)
+        self.pressure = Bridge(np.zeros(self.numTetra), datatype=np.float64)
+
+        self.recordedQuantities = []
+
+        def currentTime():
+            return self.time
+
+        self.recordedQuantities.append(currentTime)
+        self._plan = None
+        self._plan_key = None
+        self._csr = None
+        self._tip = None
+
+    def _print(self, *a, **k):
+        if self.verbose:
+            print(*a, **k)
+
+    # ---------------------------------------------------------------------------------------------
+    # setters (Simulation.py:102-230)
+    # ---------------------------------------------------------------------------------------------
+    def register(self, interaction):
+        interaction.register(self)
+        self.interactions.append(interaction)
+        self._plan = None
+
+    def setInteractionSubstrate(self):
+        # the reference places the wall at -(RadiusSIM + InitialDistance) (Simulation.py:107); an
+        # optional "SubstrateDistance" key sets the wall offset independently of the tip start
+        dist = self.parameters.get("SubstrateDistance", self.parameters["InitialDistance"])
+        wallPosition = -(self.parameters["CELL"]["RadiusSIM"] + dist)
+        forceConst = 0.1
+        self.register(Substrate(wallPosition, forceConst))
+
+    def _indenter_law(self, start):
+        speed = self.parameters["VelocitySI"] / self.V0
+        move = self.parameters["MoveTimeSI"] / self.T0
+
+        def pos(time):
+            if time <= move:
+                return start - speed * time
+            return start - speed * move
+
+        return _law(pos, start, speed, move)
+
+    def setInteractionPlane(self, record=True):
+        wallFunc = self._indenter_law(self.parameters["CELL"]["RadiusSIM"] + self.parameters["InitialDistance"])
+        forceConst = 0.1
+        self.register(Plane(wallFunc, forceConst))
+
+        def indentationSI():
+            return (wallFunc(0) - wallFunc(self.time)) * self.L0
+
+        def forceSI():
+            f, dis = self.tip_force(_lib.INDENTER_PLANE, forceConst, 0.0, wallFunc(self.time))
+            self._print(f"Max dis: {dis}")
+            return f * (self.p0 * self.L0**2)
+
+        if record:
+            self.recordedQuantities.append(indentationSI)
+            self.recordedQuantities.append(forceSI)
+
+    def setInteractionSphere(self, record=True, isSpherical=True):
+        radius = self.parameters["SphereRadiusSI"] / self.L0
+        if isSpherical:
+            sphereStartingPos = self.parameters["CELL"]["RadiusSIM"] + radius + self.parameters["InitialDistance"]
+        else:
+            sphereStartingPos = self.cell.mesh.points[3][1] + radius + self.parameters["InitialDistance"]
+        self._print(f"DEBUG: sphereStartingPos = {sphereStartingPos}, lowest point = {sphereStartingPos - radius}")
+        sphereFunc = self._indenter_law(sphereStartingPos)
+        forceConst = 0.1
+        self.register(Sphere(radius, sphereFunc, forceConst))
+
+        def indentationSI():
+            return (sphereFunc(0) - sphereFunc(self.time)) * self.L0
+
+        def forceSI():
+            f, dis = self.tip_force(_lib.INDENTER_SPHERE, forceConst, radius, sphereFunc(self.time))
+            self._print(f"Max dis: {dis}")
+            return f * (self.p0 * self.L0**2)
+
+        if record:
+            self.recordedQuantities.append(indentationSI)
+            self.recordedQuantities.append(forceSI)
+
+    def setInteractionTetra(self):
+        youngsModulusSI = self.parameters["CELL"]["YoungsModulusSI"]
+        poissonRatio = self.parameters["CELL"]["PoissonRatio"]
+        mooneyRivlinRatio = self.parameters["CELL"].get("MooneyRivlinRatio", 1)
+        youngsModulus = youngsModulusSI / self.p0
+        model = Tetra.materialModel[self.parameters["CELL"]["model"]]
+        self.register(Tetra(self.cell, youngsModulus, poissonRatio, mooneyRivlinRatio, model))
+
+    def setInteractionFiniteStrainViscoplastic(self):
+        """Legacy parameter mapping of SimulationLegacy.py:842-882 (ReynoldScaling and YoungsScaling
+        default to 1 when absent).  Config-2 style keys ``ViscositySI``/``YieldStressSI`` in CELL
+        select the Maxwell-type mapping m = 1, n = 0, Y = 3*eta*(1/s), rate = T0 [1/s per step],
+        yield offset = YieldStressSI."""
+        c = self.parameters["CELL"]
+        rs = self.parameters.get("ReynoldScaling", 1.0)
+        ys = c.get("YoungsScaling", 1.0)
+        youngsModulus = c["YoungsModulusSI"] / rs / ys / self.p0
+        nu = c["PoissonRatio"]
+        ratio = c.get("MooneyRivlinRatio", 1.0)
+        shear = youngsModulus / (2 * (1 + nu))
+        bulk = youngsModulus / (3 * (1 - 2 * nu))
+        if "ViscositySI" in c:
+            flowExponent, hardeningThreshhold, hardeningExponent = 1.0, 1.0, 0.0
+            flowRate = self.T0
+            yieldStress = 3.0 * c["ViscositySI"] * 1.0 / self.p0
+            yieldOffset = c.get("YieldStressSI", 0.0) / self.p0
+        else:
+            flowExponent = c["ViscoplasticFlowExponent"]
+            hardeningThreshhold = c["HardeningThreshhold"]
+            hardeningExponent = c["HardeningExponent"]
+            flowRate = self.T0
+            yieldStress = c["ViscoplasticTimeConstantSI"] ** (1.0 / flowExponent) / ys
+            yieldOffset = 0.0
+        self.register(FiniteStrainViscoplastic(self.cell, ratio * shear, (1 - ratio) * shear, bulk, flowExponent,
+                                               yieldStress, flowRate, hardeningThreshhold, hardeningExponent,
+                                               yieldOffset))
+
+    # Velocity Verlet Interaction, always add time integration interactions as last element
+    def setInteractionVelocityVerlet(self, fixTopBottom=False):
+        self.register(VelocityVerlet(self.numPoints, fixTopBottom))
+
+    # ---------------------------------------------------------------------------------------------
+    # stepping
+    # ---------------------------------------------------------------------------------------------
+    def node_csr(self):
+        """Deterministic-gather buffers: CSR offsets/slots of node->corner incidences and the
+        per-corner force scratch (12 floats per tetra)."""
+        if self._csr is None:
+            import ctypes
+
+            import torch
+
+            tets = np.ascontiguousarray(self.tetras.arrayC, dtype=np.uint32)
+            off = np.zeros(self.numPoints + 1, dtype=np.uint32)
+            slots = np.zeros(4 * self.numTetra, dtype=np.uint32)
+            _lib.call("nfx_build_node_csr_host", tets.ctypes.data_as(ctypes.c_void_p), self.numPoints, self.numTetra,
+                      off.ctypes.data_as(ctypes.c_void_p), slots.ctypes.data_as(ctypes.c_void_p))
+            dev = self.points.buf.device
+            self._csr = (
+                torch.from_numpy(off.view(np.int32)).to(dev),
+                torch.from_numpy(slots.view(np.int32)).to(dev),
+                torch.empty(12 * self.numTetra, dtype=torch.float32, device=dev),
+            )
+            torch.cuda.synchronize(dev)
+        return self._csr
+
+    def plan(self):
+        """The fused-step description of the registered plugin chain, or None if it cannot fuse."""
+        if not self.fused:
+            return None
+        key = tuple(id(i) for i in self.interactions) + (self.scatter,)
+        if self._plan is not None and self._plan_key == key:
+            return self._plan
+        p = _lib.Plan()
+        p.N, p.T = self.numPoints, self.numTetra
+        p.tetras, p.points, p.force = self.tetras.ptr, self.points.ptr, self.force.ptr
+        ok = len(self.interactions) >= 2 and isinstance(self.interactions[-1], VelocityVerlet)
+        for inter in self.interactions:
+            ok = ok and inter.contribute(p)
+        if not ok or not p.dminv or not p.vel:
+            self._plan, self._plan_key = None, key
+            return None
+        if self.scatter == "csr":
+            off, slots, corners = self.node_csr()
+            p.scatter = _lib.SCATTER_CSR
+            p.csr_offsets, p.csr_slots, p.corners = off.data_ptr(), slots.data_ptr(), corners.data_ptr()
+        else:
+            p.scatter = _lib.SCATTER_ATOMIC
+        self._plan, self._plan_key = p, key
+        return p
+
+    def advance(self, nsteps, tip_hist=None):
+        """Advance ``nsteps`` steps (fused when possible).  ``tip_hist``: optional float64 device
+        tensor receiving the indenter force (simulation units) of each step."""
+        p = self.plan()
+        if p is None:
+            if tip_hist is not None:
+                raise _lib.NfxError("per-step tip history needs the fused step")
+            for _ in range(nsteps):
+                self.timeStep()
+            return
+        _lib.call("nfx_run_steps", openCL.handle(), p, self.time, nsteps,
+                  None if tip_hist is None else tip_hist.data_ptr(), 1 if self.use_graph else 0)
+        self.time += nsteps
+
+    def timeStep(self):
+        p = self.plan()
+        if p is not None:
+            _lib.call("nfx_run_steps", openCL.handle(), p, self.time, 1, None, 0)
+            self.time += 1
+            return
+        for interaction in self.interactions:
+            interaction.beforeTimeStep(self.time)
+        for interaction in self.interactions:
+            interaction.enqueue()
+        self.time += 1
+
+    def tip_force(self, indenter, k, radius, pos):
+        """FP64 device reduction of the indenter force at the current positions (simulation
+        units) and the maximum penetration."""
+        import torch
+
+        if self._tip is None:
+            self._tip = torch.zeros(2, dtype=torch.float64, device=self.points.buf.device)
+        _lib.call("nfx_tip_force", openCL.handle(), self.points.ptr, self.numPoints, indenter, float(k), float(radius),
+                  float(pos), self._tip.data_ptr())
+        getCommandQueue().finish()
+        out = self._tip.cpu().numpy()
+        return float(out[0]), float(out[1])
+
+    def positions(self) -> np.ndarray:
+        """Current node positions (N, 3) float64."""
+        return self.points.readFromGPU().reshape(3, self.numPoints).T.copy()
+
+    def run(self, writeStress=True, writePressure=True, writeVolumeChange=False):
+        self._print("\n\n")
+        self._print("+------------------------------------+")
+        self._print("|         Starting Simulation        |")
+        self._print("+------------------------------------+")
+        self._print("\n\n")
+        self._print(f"VTKInterval: {self.VTKInterval}")
+        for inter in self.interactions:
+            if isinstance(inter, FiniteStrainViscoplastic) and (writeStress or writePressure):
+                inter.write_stress = True
+                inter.setArgs()
+                self._plan = None
+        starttimes = []
+        endtimes = []
+        numberCycles = self.parameters["VTKFramesTotal"]
+
+        writeVTK(self, writeStress=writeStress, writePressure=writePressure, writeVolumeChange=writeVolumeChange)
+        newDataPoints = []
+        for quant in self.recordedQuantities:
+            newDataPoints.append(quant())
+        dataPoints = np.array(newDataPoints)
+        np.savetxt(self.parameters["OutputDir"] + "simdata.dat", dataPoints)
+
+        for i in range(numberCycles):
+            starttimes.append(time.time())
+            self.advance(self.VTKInterval)
+            getCommandQueue().finish()
+            anatime1 = time.time()
+            writeVTK(self, writeStress=writeStress, writePressure=writePressure, writeVolumeChange=writeVolumeChange)
+            newDataPoints = []
+            for quant in self.recordedQuantities:
+                newDataPoints.append(quant())
+            dataPoints = np.vstack((dataPoints, newDataPoints))
+            np.savetxt(self.parameters["OutputDir"] + "simdata.dat", dataPoints)
+            anatime2 = time.time()
+            endtimes.append(time.time())
+            cycleTime = np.average(np.array(endtimes) - np.array(starttimes))
+            vtktime = anatime2 - anatime1
+            self._print(
+                f"| Average # of simulation steps per second: {np.round(self.VTKInterval / cycleTime, 1)} "
+                f"| VTK cycle took {np.round(vtktime, 1)} seconds | estimate of time remaining: "
+                f"{np.round(((numberCycles - i) * (cycleTime + vtktime)) / 60, 1)} min",
+                end="\r",
+            )
+        self._print("\n\n")
+        self._print("+------------------------------------+")
+        self._print("|  Simulation finished successfully  |")
+        self._print("+------------------------------------+")

@@ -1,0 +1,152 @@
+"""ctypes binding of libnfx.so (declarations in include/nfx.h).
+
+The product path has no fallback: if the library is missing, or no HIP device is present, the
+calls raise.  Device buffers are torch tensors; only their ``data_ptr()`` crosses the boundary.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnfx.so")
+
+NFX_SVKK, NFX_SVK, NFX_MR, NFX_VISCOPLASTIC = 0, 1, 2, 3
+SCATTER_ATOMIC, SCATTER_CSR = 0, 1
+INDENTER_NONE, INDENTER_SPHERE, INDENTER_PLANE = 0, 1, 2
+
+c_i32, c_i64, c_f64, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+
+class NfxError(RuntimeError):
+    pass
+
+
+class Elastic(ctypes.Structure):
+    _fields_ = [("model", c_i32), ("E", c_f64), ("nu", c_f64), ("ratio", c_f64)]
+
+
+class Viscoplastic(ctypes.Structure):
+    _fields_ = [
+        ("mu1", c_f64),
+        ("mu2", c_f64),
+        ("kappa", c_f64),
+        ("flow_exponent", c_f64),
+        ("yield_stress", c_f64),
+        ("flow_rate", c_f64),
+        ("hardening_threshold", c_f64),
+        ("hardening_exponent", c_f64),
+        ("yield_offset", c_f64),
+    ]
+
+
+class Plan(ctypes.Structure):
+    _fields_ = [
+        ("N", c_i64),
+        ("T", c_i64),
+        ("tetras", c_vp),
+        ("dminv", c_vp),
+        ("v0", c_vp),
+        ("points", c_vp),
+        ("vel", c_vp),
+        ("force", c_vp),
+        ("fold", c_vp),
+        ("model", c_i32),
+        ("E", c_f64),
+        ("nu", c_f64),
+        ("ratio", c_f64),
+        ("vp", Viscoplastic),
+        ("Fp", c_vp),
+        ("epsbar", c_vp),
+        ("von_mises", c_vp),
+        ("pressure", c_vp),
+        ("scatter", c_i32),
+        ("csr_offsets", c_vp),
+        ("csr_slots", c_vp),
+        ("corners", c_vp),
+        ("substrate", c_i32),
+        ("substrate_k", c_f64),
+        ("wall", c_f64),
+        ("indenter", c_i32),
+        ("ind_k", c_f64),
+        ("ind_radius", c_f64),
+        ("ind_start", c_f64),
+        ("ind_speed", c_f64),
+        ("ind_move_steps", c_f64),
+        ("fix_top_bottom", c_i32),
+    ]
+
+
+# name -> (restype, argtypes); every symbol include/nfx.h declares
+SIGNATURES = {
+    "nfx_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "nfx_init": (ctypes.c_int, [ctypes.c_int, c_vp, ctypes.POINTER(c_vp)]),
+    "nfx_destroy": (ctypes.c_int, [c_vp]),
+    "nfx_stream_sync": (ctypes.c_int, [c_vp]),
+    "nfx_last_error": (ctypes.c_char_p, []),
+    "nfx_device_name": (ctypes.c_int, [c_vp, ctypes.c_char_p, c_i32]),
+    "nfx_precompute_f64": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp]),
+    "nfx_tetra_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Elastic), c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
+    "nfx_tetra_corner_forces": (ctypes.c_int,
+                                [c_vp, ctypes.POINTER(Elastic), c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
+    "nfx_viscoplastic_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Viscoplastic), c_vp, c_vp, c_vp, c_i64, c_i64,
+                                              c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_build_node_csr_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "nfx_sphere_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64, c_f64]),
+    "nfx_substrate_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
+    "nfx_plane_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
+    "nfx_velocity_verlet": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i32]),
+    "nfx_tip_force": (ctypes.c_int, [c_vp, c_vp, c_i64, c_i32, c_f64, c_f64, c_f64, c_vp]),
+    "nfx_run_steps": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i64, c_vp, c_i32]),
+    "nfx_node_update": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_vp]),
+    "nfx_time_passes": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i32, ctypes.POINTER(c_f64),
+                                       ctypes.POINTER(c_f64)]),
+}
+
+_lib = None
+_extra_signatures = []
+
+
+def register_signatures(sigs: dict):
+    """Other modules (batched stepper, surrogate) add their entry points here."""
+    SIGNATURES.update(sigs)
+    if _lib is not None:
+        _bind(_lib, sigs)
+
+
+def _bind(L, sigs):
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def lib():
+    """Load libnfx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NfxError(f"{LIB_PATH} is missing: build it with `python -m nofluidx3d_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        _bind(L, SIGNATURES)
+        _lib = L
+    return _lib
+
+
+def check(rc: int, what: str = ""):
+    if rc != 0:
+        msg = lib().nfx_last_error().decode(errors="replace")
+        raise NfxError(f"{what}: {msg} (code {rc})" if what else f"{msg} (code {rc})")
+
+
+def call(name: str, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+def ptr(t) -> int:
+    """Device address of a torch tensor (or 0 for None)."""
+    if t is None:
+        return None
+    return t.data_ptr()

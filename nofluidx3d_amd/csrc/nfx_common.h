@@ -1,0 +1,66 @@
+// Host-side plumbing shared by the libnfx translation units: error state, context, launch checks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "../../include/nfx.h"
+
+namespace nfx {
+
+void set_error(const char* fmt, ...);
+
+#define NFX_HIP(call)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ::nfx::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                     \
+            return NFX_ERR_HIP;                                                             \
+        }                                                                                   \
+    } while (0)
+
+#define NFX_CHECK_LAUNCH()                                                                   \
+    do {                                                                                     \
+        hipError_t e_ = hipGetLastError();                                                   \
+        if (e_ != hipSuccess) {                                                              \
+            ::nfx::set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(e_), __FILE__, \
+                             __LINE__);                                                      \
+            return NFX_ERR_HIP;                                                              \
+        }                                                                                    \
+    } while (0)
+
+#define NFX_REQUIRE(cond, ...)            \
+    do {                                  \
+        if (!(cond)) {                    \
+            ::nfx::set_error(__VA_ARGS__); \
+            return NFX_ERR_ARG;           \
+        }                                 \
+    } while (0)
+
+// reduction scratch: per-block FP64 partials and a completion counter, allocated once per context
+constexpr int kMaxReduceBlocks = 2048;
+constexpr int kGraphBlock = 100;  // steps per captured graph
+
+}  // namespace nfx
+
+struct nfx_ctx {
+    int device = 0;
+    int num_cu = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double* partials = nullptr;       // kMaxReduceBlocks * 2 doubles
+    unsigned int* counter = nullptr;  // 1 counter (reset by the last block of each reduction)
+    int64_t* step_dev = nullptr;      // device step counter for graph replays
+    double* hist_dev = nullptr;       // kGraphBlock doubles, tip force of the replayed block
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // captured graph cache
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    nfx_plan graph_plan{};
+    int64_t graph_steps = 0;
+    bool graph_hist = false;
+};

@@ -1,0 +1,356 @@
+// Device-side constitutive and contact math shared by the plain and the batched kernels.
+//
+// All per-tetra functions take the four gathered node positions and the tetra's inverse
+// reference shape matrix D = Dm^-1 (row-major, Dm = [R1 | R2 | R3] as columns, TetraCell.py:33-38)
+// and return the four corner forces f0..f3.  They restate the reference's laws in an algebraically
+// equivalent form that avoids the two per-step 3x3 inversions of tetra.cl:53,159:
+//
+//   reference (tetra.cl:159-168):  rows f1..f3 = -V0 * J * r^-1 * sigma,  J = det(r)/(6 V0)
+//   here:                          f_k = -(1/6) * sigma * c_k,  c_k = r_{k+1} x r_{k+2}
+//
+// because V0 * J * r^-1 = det(r)/6 * adj(r)/det(r) and the rows of adj(r) are c_1, c_2, c_3;
+// sigma is symmetric.  f0 = -(f1 + f2 + f3) (tetra.cl:168).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nfx {
+
+struct V3 {
+    float x, y, z;
+};
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// symmetric 3x3: xx yy zz xy yz xz
+struct S6 {
+    float xx, yy, zz, xy, yz, xz;
+};
+__device__ __forceinline__ V3 smul(const S6& s, V3 v) {
+    return V3{s.xx * v.x + s.xy * v.y + s.xz * v.z, s.xy * v.x + s.yy * v.y + s.yz * v.z,
+              s.xz * v.x + s.yz * v.y + s.zz * v.z};
+}
+
+// general 3x3, row-major m[i][j]
+struct M9 {
+    float m[3][3];
+};
+
+// F = r * D where r has columns r1, r2, r3
+__device__ __forceinline__ M9 deformation_gradient(V3 r1, V3 r2, V3 r3, const float* D) {
+    M9 F;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        F.m[0][j] = r1.x * D[0 * 3 + j] + r2.x * D[1 * 3 + j] + r3.x * D[2 * 3 + j];
+        F.m[1][j] = r1.y * D[0 * 3 + j] + r2.y * D[1 * 3 + j] + r3.y * D[2 * 3 + j];
+        F.m[2][j] = r1.z * D[0 * 3 + j] + r2.z * D[1 * 3 + j] + r3.z * D[2 * 3 + j];
+    }
+    return F;
+}
+
+// A * A^T (symmetric)
+__device__ __forceinline__ S6 aat(const M9& A) {
+    S6 s;
+    s.xx = A.m[0][0] * A.m[0][0] + A.m[0][1] * A.m[0][1] + A.m[0][2] * A.m[0][2];
+    s.yy = A.m[1][0] * A.m[1][0] + A.m[1][1] * A.m[1][1] + A.m[1][2] * A.m[1][2];
+    s.zz = A.m[2][0] * A.m[2][0] + A.m[2][1] * A.m[2][1] + A.m[2][2] * A.m[2][2];
+    s.xy = A.m[0][0] * A.m[1][0] + A.m[0][1] * A.m[1][1] + A.m[0][2] * A.m[1][2];
+    s.yz = A.m[1][0] * A.m[2][0] + A.m[1][1] * A.m[2][1] + A.m[1][2] * A.m[2][2];
+    s.xz = A.m[0][0] * A.m[2][0] + A.m[0][1] * A.m[2][1] + A.m[0][2] * A.m[2][2];
+    return s;
+}
+
+__device__ __forceinline__ float det9(const M9& A) {
+    return A.m[0][0] * (A.m[1][1] * A.m[2][2] - A.m[1][2] * A.m[2][1]) -
+           A.m[0][1] * (A.m[1][0] * A.m[2][2] - A.m[1][2] * A.m[2][0]) +
+           A.m[0][2] * (A.m[1][0] * A.m[2][1] - A.m[1][1] * A.m[2][0]);
+}
+
+// inverse by cofactors / det
+__device__ __forceinline__ M9 inv9(const M9& A) {
+    M9 R;
+    const float c00 = A.m[1][1] * A.m[2][2] - A.m[1][2] * A.m[2][1];
+    const float c01 = A.m[1][2] * A.m[2][0] - A.m[1][0] * A.m[2][2];
+    const float c02 = A.m[1][0] * A.m[2][1] - A.m[1][1] * A.m[2][0];
+    const float d = A.m[0][0] * c00 + A.m[0][1] * c01 + A.m[0][2] * c02;
+    const float id = 1.0f / d;
+    R.m[0][0] = c00 * id;
+    R.m[1][0] = c01 * id;
+    R.m[2][0] = c02 * id;
+    R.m[0][1] = (A.m[0][2] * A.m[2][1] - A.m[0][1] * A.m[2][2]) * id;
+    R.m[1][1] = (A.m[0][0] * A.m[2][2] - A.m[0][2] * A.m[2][0]) * id;
+    R.m[2][1] = (A.m[0][1] * A.m[2][0] - A.m[0][0] * A.m[2][1]) * id;
+    R.m[0][2] = (A.m[0][1] * A.m[1][2] - A.m[0][2] * A.m[1][1]) * id;
+    R.m[1][2] = (A.m[0][2] * A.m[1][0] - A.m[0][0] * A.m[1][2]) * id;
+    R.m[2][2] = (A.m[0][0] * A.m[1][1] - A.m[0][1] * A.m[1][0]) * id;
+    return R;
+}
+
+__device__ __forceinline__ M9 mul9(const M9& A, const M9& B) {
+    M9 C;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C.m[i][j] = A.m[i][0] * B.m[0][j] + A.m[i][1] * B.m[1][j] + A.m[i][2] * B.m[2][j];
+    return C;
+}
+
+// compressible Mooney-Rivlin Cauchy stress (tetra.cl:118-156; NH when mu2 = 0) from the left
+// Cauchy-Green tensor B and J.  J^-5/3 and J^-7/3 come from one cube root t = J^-1/3, taken as
+// exp2(-log2(J)/3) so that J <= 0 yields NaN/Inf as pow() does in the reference.
+__device__ __forceinline__ S6 mr_stress(const S6& B, float J, float mu1, float mu2, float kappa) {
+    const float trB = B.xx + B.yy + B.zz;
+    // B^2 (symmetric)
+    S6 B2;
+    B2.xx = B.xx * B.xx + B.xy * B.xy + B.xz * B.xz;
+    B2.yy = B.xy * B.xy + B.yy * B.yy + B.yz * B.yz;
+    B2.zz = B.xz * B.xz + B.yz * B.yz + B.zz * B.zz;
+    B2.xy = B.xx * B.xy + B.xy * B.yy + B.xz * B.yz;
+    B2.yz = B.xy * B.xz + B.yy * B.yz + B.yz * B.zz;
+    B2.xz = B.xx * B.xz + B.xy * B.yz + B.xz * B.zz;
+    const float trB2 = B2.xx + B2.yy + B2.zz;
+    const float t = exp2f(-log2f(J) * (1.0f / 3.0f));
+    const float t2 = t * t;
+    const float t5 = t2 * t2 * t;
+    const float a = mu1 * t5;        // dU/dI1
+    const float k = mu2 * t5 * t2;   // dU/dK
+    const float iso = kappa * (J - 1.0f) - trB * (1.0f / 3.0f) * a + k * (trB2 - trB * trB) * (1.0f / 3.0f);
+    const float bc = a + k * trB;    // coefficient of B
+    S6 s;
+    s.xx = bc * B.xx - k * B2.xx + iso;
+    s.yy = bc * B.yy - k * B2.yy + iso;
+    s.zz = bc * B.zz - k * B2.zz + iso;
+    s.xy = bc * B.xy - k * B2.xy;
+    s.yz = bc * B.yz - k * B2.yz;
+    s.xz = bc * B.xz - k * B2.xz;
+    return s;
+}
+
+struct Corners {
+    V3 f0, f1, f2, f3;
+};
+
+// f_k = -(sgn/6) sigma c_k
+__device__ __forceinline__ Corners cofactor_forces(const S6& sig, V3 r1, V3 r2, V3 r3, float scale) {
+    Corners c;
+    c.f1 = smul(sig, cross(r2, r3)) * scale;
+    c.f2 = smul(sig, cross(r3, r1)) * scale;
+    c.f3 = smul(sig, cross(r1, r2)) * scale;
+    c.f0 = (c.f1 + c.f2 + c.f3) * -1.0f;
+    return c;
+}
+
+// Mooney-Rivlin / Neo-Hookean branch of tetra.cl:115-168.
+__device__ __forceinline__ Corners mr_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const float* D, float v0, float mu1,
+                                            float mu2, float kappa) {
+    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
+    const M9 F = deformation_gradient(r1, r2, r3, D);
+    const S6 B = aat(F);
+    const float J = dot(r1, cross(r2, r3)) / (6.0f * v0);  // tetra.cl:132
+    const S6 sig = mr_stress(B, J, mu1, mu2, kappa);
+    return cofactor_forces(sig, r1, r2, r3, -1.0f / 6.0f);
+}
+
+// rows f1..f3 of  -V0 mu * D * M * F^T  (tetra.cl:93 and :114 share this shape), M symmetric or not
+__device__ __forceinline__ Corners dmf_rows(const float* D, const M9& M, const M9& F, float scale) {
+    // G = D * M
+    M9 Dm;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Dm.m[i][j] = D[i * 3 + j];
+    const M9 G = mul9(Dm, M);
+    V3 rows[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        // row i of G * F^T: (G F^T)_{il} = sum_j G_ij F_lj
+        rows[i] = mk(G.m[i][0] * F.m[0][0] + G.m[i][1] * F.m[0][1] + G.m[i][2] * F.m[0][2],
+                     G.m[i][0] * F.m[1][0] + G.m[i][1] * F.m[1][1] + G.m[i][2] * F.m[1][2],
+                     G.m[i][0] * F.m[2][0] + G.m[i][1] * F.m[2][1] + G.m[i][2] * F.m[2][2]) *
+                  scale;
+    }
+    Corners c;
+    c.f1 = rows[0];
+    c.f2 = rows[1];
+    c.f3 = rows[2];
+    c.f0 = (c.f1 + c.f2 + c.f3) * -1.0f;
+    return c;
+}
+
+// St-Venant-Kirchhoff branch (tetra.cl:94-114): rows = -V0 mu * D F^T (kappa tr(B-I) I + (B-I)),
+// kappa = nu/(1-2nu) (the ratio lambda/(2 mu)).
+__device__ __forceinline__ Corners svk_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const float* D, float v0, float mu,
+                                             float kappa) {
+    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
+    const M9 F = deformation_gradient(r1, r2, r3, D);
+    const S6 B = aat(F);
+    const float trBP = B.xx + B.yy + B.zz - 3.0f;
+    // H = D F^T; Force = H * (kappa trBP I + BP) * (-V0 mu)  ==  D * F^T * M
+    M9 FT;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) FT.m[i][j] = F.m[j][i];
+    M9 M;
+    M.m[0][0] = kappa * trBP + B.xx - 1.0f;
+    M.m[1][1] = kappa * trBP + B.yy - 1.0f;
+    M.m[2][2] = kappa * trBP + B.zz - 1.0f;
+    M.m[0][1] = M.m[1][0] = B.xy;
+    M.m[1][2] = M.m[2][1] = B.yz;
+    M.m[0][2] = M.m[2][0] = B.xz;
+    const M9 FM = mul9(FT, M);
+    // rows of D * FM
+    Corners c;
+    V3 rows[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        rows[i] = mk(D[i * 3 + 0] * FM.m[0][0] + D[i * 3 + 1] * FM.m[1][0] + D[i * 3 + 2] * FM.m[2][0],
+                     D[i * 3 + 0] * FM.m[0][1] + D[i * 3 + 1] * FM.m[1][1] + D[i * 3 + 2] * FM.m[2][1],
+                     D[i * 3 + 0] * FM.m[0][2] + D[i * 3 + 1] * FM.m[1][2] + D[i * 3 + 2] * FM.m[2][2]) *
+                  (-v0 * mu);
+    c.f1 = rows[0];
+    c.f2 = rows[1];
+    c.f3 = rows[2];
+    c.f0 = (c.f1 + c.f2 + c.f3) * -1.0f;
+    return c;
+}
+
+// "Kellnberger" St-Venant-Kirchhoff variant (tetra.cl:76-93): S = polar stretch of B by ten Newton
+// steps X <- (X + X^-T)/2 (tetra.cl:5-12), inner = I + S^-1 (kappa (tr S - 3) - 1),
+// rows = -V0 mu * D * inner * F^T.
+__device__ __forceinline__ Corners svkk_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const float* D, float v0, float mu,
+                                              float kappa) {
+    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
+    const M9 F = deformation_gradient(r1, r2, r3, D);
+    const S6 Bs = aat(F);
+    M9 B;
+    B.m[0][0] = Bs.xx; B.m[1][1] = Bs.yy; B.m[2][2] = Bs.zz;
+    B.m[0][1] = B.m[1][0] = Bs.xy;
+    B.m[1][2] = B.m[2][1] = Bs.yz;
+    B.m[0][2] = B.m[2][0] = Bs.xz;
+    M9 X = B;
+    for (int it = 0; it < 10; ++it) {
+        const M9 Xi = inv9(X);  // (X^T)^-1 = (X^-1)^T
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) X.m[i][j] = 0.5f * (X.m[i][j] + Xi.m[j][i]);
+    }
+    M9 XT;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) XT.m[i][j] = X.m[j][i];
+    const M9 S = mul9(XT, B);
+    const M9 Si = inv9(S);
+    const float sf = kappa * (S.m[0][0] + S.m[1][1] + S.m[2][2] - 3.0f) - 1.0f;
+    M9 inner;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) inner.m[i][j] = (i == j ? 1.0f : 0.0f) + Si.m[i][j] * sf;
+    return dmf_rows(D, inner, F, -v0 * mu);
+}
+
+struct VPOut {
+    Corners f;
+    float Fp[9];
+    float epsbar;
+    float von_mises, pressure;
+};
+
+struct VPConst {
+    float mu1, mu2, kappa, m, Y, rate, eps0, n, sy;
+};
+
+// Finite-strain viscoplastic law (InteractionFiniteStrainViscoplastic.cl:57-206) with node 0 as
+// the edge base.  The face-area nodal forces of :110-141 equal -(sgn(det r)/6) sigma c_k for a
+// closed tetra surface (both are 1/3 of sigma times the outward area vector of the opposite face).
+__device__ __forceinline__ VPOut vp_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const float* D, const float* Fp, float epsbar,
+                                          const VPConst& P) {
+    VPOut o;
+    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
+    const M9 F = deformation_gradient(r1, r2, r3, D);
+    M9 Fpm;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fpm.m[i][j] = Fp[i * 3 + j];
+    const M9 Fe = mul9(F, inv9(Fpm));
+    const S6 B = aat(Fe);
+    // J = sqrt(det B) (:87)
+    const float detB = B.xx * (B.yy * B.zz - B.yz * B.yz) - B.xy * (B.xy * B.zz - B.yz * B.xz) +
+                       B.xz * (B.xy * B.yz - B.yy * B.xz);
+    const float J = sqrtf(detB);
+    const S6 sig = mr_stress(B, J, P.mu1, P.mu2, P.kappa);
+    const float detr = dot(r1, cross(r2, r3));
+    o.f = cofactor_forces(sig, r1, r2, r3, detr < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
+    // :144-169
+    const float tr3 = (sig.xx + sig.yy + sig.zz) * (1.0f / 3.0f);
+    S6 dv = sig;
+    dv.xx -= tr3;
+    dv.yy -= tr3;
+    dv.zz -= tr3;
+    const float dd = dv.xx * dv.xx + dv.yy * dv.yy + dv.zz * dv.zz + 2.0f * (dv.xy * dv.xy + dv.yz * dv.yz + dv.xz * dv.xz);
+    const float svm = sqrtf(1.5f * dd);
+    const float sigma0 = P.Y * powf(1.0f + epsbar / P.eps0, P.n);
+    o.von_mises = svm;
+    o.pressure = tr3;
+    const float over = P.sy > 0.0f ? fmaxf(svm - P.sy, 0.0f) : svm;
+    const float rate = P.rate * powf(over / sigma0, P.m);
+    o.epsbar = epsbar + rate;
+    const float g = (svm != 0.0f) ? rate * 1.5f / svm : 0.0f;
+    // Fp += Fe^-1 * (g dev sigma) * F
+    M9 Dp;
+    Dp.m[0][0] = g * dv.xx; Dp.m[1][1] = g * dv.yy; Dp.m[2][2] = g * dv.zz;
+    Dp.m[0][1] = Dp.m[1][0] = g * dv.xy;
+    Dp.m[1][2] = Dp.m[2][1] = g * dv.yz;
+    Dp.m[0][2] = Dp.m[2][0] = g * dv.xz;
+    const M9 dF = mul9(mul9(inv9(Fe), Dp), F);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) o.Fp[i * 3 + j] = Fp[i * 3 + j] + dF.m[i][j];
+    return o;
+}
+
+// ---- node terms (per node, FP32 positions) -------------------------------------------------
+
+// Sphere.cl:9-16; returns the force added to the node
+__device__ __forceinline__ V3 sphere_term(V3 x, float k, float radius, float ypos) {
+    const V3 r = mk(x.x, x.y - ypos, x.z);
+    const float len = sqrtf(dot(r, r));
+    const float dis = fmaxf(radius - len, 0.0f);
+    if (dis <= 0.0f || len <= 0.0f) return mk(0.f, 0.f, 0.f);
+    return r * (k * dis / len);
+}
+
+// Substrate.cl:4-5 (y component)
+__device__ __forceinline__ float substrate_term(float y, float k, float wall) { return k * fmaxf(wall - y, 0.0f); }
+
+// Plane.cl:4-5 (y component)
+__device__ __forceinline__ float plane_term(float y, float k, float wall) { return -k * fmaxf(y - wall, 0.0f); }
+
+// VelocityVerlet.cl:11-15 with xi = 0.005, unit mass, dt = 1
+struct VV {
+    static constexpr float a = 1.0f - 0.5f * 0.005f;
+    static constexpr float inv_b = 1.0f / (1.0f + 0.5f * 0.005f);
+};
+__device__ __forceinline__ void vv_update(V3& x, V3& v, V3 f, V3 fo) {
+    v = (v * VV::a + (f + fo) * 0.5f) * VV::inv_b;
+    x = x + v * VV::a + f * 0.5f;
+}
+
+// Simulation.py:173-183 / 117-132, in double
+__device__ __forceinline__ double indenter_pos(double start, double speed, double move_steps, int64_t time) {
+    const double t = (double)time;
+    return (t <= move_steps) ? start - speed * t : start - speed * move_steps;
+}
+
+}  // namespace nfx

@@ -1,0 +1,875 @@
+// libnfx: plain (single-cell) path of the explicit tetrahedral-FEM step on gfx950.
+//
+// Passes per step (Simulation.timeStep, Simulation.py:232-237):
+//   force pass  one thread per tetra: gather 4 nodes, F = r * Dm^-1, stress, 4 corner forces;
+//               either float atomics into the node force (the reference's scatter,
+//               atomicAdd.cl:1-11) or a 12-float corner record per tetra (deterministic mode).
+//   node pass   one node per thread (atomic mode) or 4 lanes per node that sum the node's corner
+//               records in CSR order and combine them with a fixed xor butterfly (deterministic
+//               mode); then substrate + indenter terms at the device-computed indenter position,
+//               the damped velocity-Verlet update, and the FP64 indenter-force reduction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "nfx_common.h"
+#include "nfx_device.h"
+
+using namespace nfx;
+
+namespace nfx {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+constexpr int kBlock = 256;
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
+// logical blocks so that tetras sharing nodes share an L2.
+__device__ __forceinline__ int64_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+    return (int64_t)x * q + (x < r ? x : r) + i;
+}
+
+struct TetArgs {
+    int64_t N, T;
+    const uint32_t* __restrict__ tetras;
+    const float* __restrict__ dminv;
+    const float* __restrict__ v0;
+    const float* __restrict__ x;
+    float* force;
+    float* corners;
+};
+
+struct ElasticDev {
+    float mu1, mu2, kappa;  // MR
+    float mu, kappa_svk;    // SVK / SVKK (kappa = nu / (1 - 2 nu))
+};
+
+static ElasticDev elastic_consts(double E, double nu, double ratio) {
+    const double mu = E / (2 * (1 + nu));  // tetra.cl:118-121 (FP64 on the host)
+    ElasticDev e;
+    e.mu1 = (float)(ratio * mu);
+    e.mu2 = (float)((1 - ratio) * mu);
+    e.kappa = (float)(E / (3 * (1 - 2 * nu)));
+    e.mu = (float)mu;
+    e.kappa_svk = (float)(nu / (1 - 2 * nu));
+    return e;
+}
+
+static VPConst vp_consts(const nfx_viscoplastic& v) {
+    VPConst c;
+    c.mu1 = (float)v.mu1;
+    c.mu2 = (float)v.mu2;
+    c.kappa = (float)v.kappa;
+    c.m = (float)v.flow_exponent;
+    c.Y = (float)v.yield_stress;
+    c.rate = (float)v.flow_rate;
+    c.eps0 = (float)v.hardening_threshold;
+    c.n = (float)v.hardening_exponent;
+    c.sy = (float)v.yield_offset;
+    return c;
+}
+
+__device__ __forceinline__ V3 gather3(const float* __restrict__ x, int64_t N, uint32_t n) {
+    return mk(x[n], x[N + n], x[2 * N + n]);
+}
+
+__device__ __forceinline__ void scatter_atomic(float* f, int64_t N, uint32_t n, V3 v) {
+    unsafeAtomicAdd(&f[n], v.x);
+    unsafeAtomicAdd(&f[N + n], v.y);
+    unsafeAtomicAdd(&f[2 * N + n], v.z);
+}
+
+__device__ __forceinline__ void store_corners(float* __restrict__ c, int64_t t, const Corners& f) {
+    float* p = c + 12 * t;
+    p[0] = f.f0.x; p[1] = f.f0.y; p[2] = f.f0.z;
+    p[3] = f.f1.x; p[4] = f.f1.y; p[5] = f.f1.z;
+    p[6] = f.f2.x; p[7] = f.f2.y; p[8] = f.f2.z;
+    p[9] = f.f3.x; p[10] = f.f3.y; p[11] = f.f3.z;
+}
+
+// ---- one-time FP64 precompute -----------------------------------------------------------------
+// Inputs are the reference's Tetra buffers (Tetra.py:53-58): edge vectors R1..R3 as 9 SoA FP64
+// arrays and the signed volumes.  Dm = [R1 | R2 | R3] (tetra.cl:40-47) is inverted once in FP64
+// (the reference re-inverts it every step, tetra.cl:53); rows of Dm^-1 are (R2xR3, R3xR1, R1xR2)/det.
+__global__ __launch_bounds__(kBlock) void k_precompute_f64(const double* __restrict__ ev, const double* __restrict__ vol,
+                                                           int64_t T, float* __restrict__ dminv, float* __restrict__ v0) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    double R[3][3];  // R[k] = column k+1 of Dm
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R[k][c] = ev[(3 * k + c) * T + t];
+    double cr[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double* a = R[(k + 1) % 3];
+        const double* b = R[(k + 2) % 3];
+        cr[k][0] = a[1] * b[2] - a[2] * b[1];
+        cr[k][1] = a[2] * b[0] - a[0] * b[2];
+        cr[k][2] = a[0] * b[1] - a[1] * b[0];
+    }
+    const double det = R[0][0] * cr[0][0] + R[0][1] * cr[0][1] + R[0][2] * cr[0][2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dminv[(3 * i + j) * T + t] = (float)(cr[i][j] / det);
+    v0[t] = (float)vol[t];
+}
+
+// ---- force pass, elastic models -------------------------------------------------------------
+template <int MODEL, bool CORNERS>
+__global__ __launch_bounds__(kBlock) void k_tetra(TetArgs a, ElasticDev e) {
+    const int64_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (t >= a.T) return;
+    const int64_t T = a.T, N = a.N;
+    const uint32_t n0 = a.tetras[t], n1 = a.tetras[T + t], n2 = a.tetras[2 * T + t], n3 = a.tetras[3 * T + t];
+    float D[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) D[k] = a.dminv[k * T + t];
+    const float v0 = a.v0[t];
+    const V3 p0 = gather3(a.x, N, n0), p1 = gather3(a.x, N, n1), p2 = gather3(a.x, N, n2), p3 = gather3(a.x, N, n3);
+    Corners c;
+    if constexpr (MODEL == NFX_MR)
+        c = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
+    else if constexpr (MODEL == NFX_SVK)
+        c = svk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+    else
+        c = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+    if constexpr (CORNERS) {
+        store_corners(a.corners, t, c);
+    } else {
+        scatter_atomic(a.force, N, n0, c.f0);
+        scatter_atomic(a.force, N, n1, c.f1);
+        scatter_atomic(a.force, N, n2, c.f2);
+        scatter_atomic(a.force, N, n3, c.f3);
+    }
+}
+
+// ---- force pass, viscoplastic -----------------------------------------------------------------
+template <bool CORNERS, bool STRESS>
+__global__ __launch_bounds__(kBlock) void k_vp(TetArgs a, VPConst P, float* __restrict__ Fp, float* __restrict__ eps,
+                                               float* __restrict__ vm, float* __restrict__ pr) {
+    const int64_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    if (t >= a.T) return;
+    const int64_t T = a.T, N = a.N;
+    const uint32_t n0 = a.tetras[t], n1 = a.tetras[T + t], n2 = a.tetras[2 * T + t], n3 = a.tetras[3 * T + t];
+    float D[9], fp[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) D[k] = a.dminv[k * T + t];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) fp[k] = Fp[k * T + t];
+    const float e0 = eps[t];
+    const V3 p0 = gather3(a.x, N, n0), p1 = gather3(a.x, N, n1), p2 = gather3(a.x, N, n2), p3 = gather3(a.x, N, n3);
+    const VPOut o = vp_tetra(p0, p1, p2, p3, D, fp, e0, P);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Fp[k * T + t] = o.Fp[k];
+    eps[t] = o.epsbar;
+    if constexpr (STRESS) {
+        vm[t] = o.von_mises;
+        pr[t] = o.pressure;
+    }
+    if constexpr (CORNERS) {
+        store_corners(a.corners, t, o.f);
+    } else {
+        scatter_atomic(a.force, N, n0, o.f.f0);
+        scatter_atomic(a.force, N, n1, o.f.f1);
+        scatter_atomic(a.force, N, n2, o.f.f2);
+        scatter_atomic(a.force, N, n3, o.f.f3);
+    }
+}
+
+// ---- deterministic block reduction (sum, max) finished by the last block --------------------
+__device__ __forceinline__ void wave_sum_max(double& s, double& m) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        m = fmax(m, __shfl_xor(m, off));
+    }
+}
+
+__device__ void reduce_finish(double s, double m, double* partials, unsigned int* counter, double* out_sum,
+                              double* out_max) {
+    __shared__ double ss[kBlock / 64], sm[kBlock / 64];
+    __shared__ bool last;
+    wave_sum_max(s, m);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        ss[wid] = s;
+        sm[wid] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            a += ss[w];
+            b = fmax(b, sm[w]);
+        }
+        partials[2 * blockIdx.x] = a;
+        partials[2 * blockIdx.x + 1] = b;
+        __threadfence();
+        const unsigned int prev = atomicAdd(counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    double a = 0.0, b = 0.0;
+    for (unsigned int i = threadIdx.x; i < gridDim.x; i += kBlock) {
+        a += __hip_atomic_load(&partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        b = fmax(b, __hip_atomic_load(&partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    wave_sum_max(a, b);
+    __syncthreads();
+    if (lane == 0) {
+        ss[wid] = a;
+        sm[wid] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0, y = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) {
+            x += ss[w];
+            y = fmax(y, sm[w]);
+        }
+        if (out_sum) *out_sum = x;
+        if (out_max) *out_max = y;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---- node pass ------------------------------------------------------------------------------
+struct NodeArgs {
+    int64_t N;
+    float* x;
+    float* v;
+    float* force;
+    float* fold;
+    const uint32_t* __restrict__ off;
+    const uint32_t* __restrict__ slots;
+    const float* __restrict__ corners;
+    float sub_k, wall;
+    int32_t indenter;
+    float ind_k, ind_radius;
+    double ind_start, ind_speed, ind_move;
+};
+
+// Sum of node n's corner records over the G lanes of its group, combined by a fixed butterfly.
+template <int G>
+__device__ __forceinline__ V3 gather_node(const NodeArgs& a, int64_t n, int sub) {
+    V3 f = mk(0.f, 0.f, 0.f);
+    const uint32_t b = a.off[n], e = a.off[n + 1];
+    for (uint32_t j = b + sub; j < e; j += G) {
+        const uint32_t s = a.slots[j];
+        const float* p = a.corners + 12 * (int64_t)(s >> 2) + 3 * (s & 3);
+        f = f + mk(p[0], p[1], p[2]);
+    }
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        f.x += __shfl_xor(f.x, o);
+        f.y += __shfl_xor(f.y, o);
+        f.z += __shfl_xor(f.z, o);
+    }
+    return f;
+}
+
+template <bool CSR, bool SUB, int IND, bool FIX, bool TIP>
+__global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step, const int64_t* step_dev, int32_t step_off,
+                                                        double* partials, unsigned int* counter, double* tip_sum,
+                                                        double* tip_max) {
+    constexpr int G = CSR ? 4 : 1;
+    const int64_t time = step_dev ? (*step_dev + step_off) : step;
+    const float pos = IND ? (float)indenter_pos(a.ind_start, a.ind_speed, a.ind_move, time) : 0.0f;
+    const int64_t N = a.N;
+    const int sub = threadIdx.x % G;
+    double tsum = 0.0, tmax = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
+    for (int64_t n = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G; n < N; n += stride) {
+        V3 f;
+        if constexpr (CSR) {
+            f = gather_node<G>(a, n, sub);
+        } else {
+            f = mk(a.force[n], a.force[N + n], a.force[2 * N + n]);
+        }
+        if (sub != 0) continue;
+        V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
+        if constexpr (SUB) f.y += substrate_term(x.y, a.sub_k, a.wall);
+        if constexpr (IND == NFX_INDENTER_SPHERE) {
+            const V3 fs = sphere_term(x, a.ind_k, a.ind_radius, pos);
+            f = f + fs;
+            if constexpr (TIP) {
+                tsum -= (double)fs.y;
+                const float r = sqrtf(x.x * x.x + (x.y - pos) * (x.y - pos) + x.z * x.z);
+                tmax = fmax(tmax, (double)fmaxf(a.ind_radius - r, 0.0f));
+            }
+        } else if constexpr (IND == NFX_INDENTER_PLANE) {
+            const float fp = plane_term(x.y, a.ind_k, pos);
+            f.y += fp;
+            if constexpr (TIP) {
+                tsum -= (double)fp;
+                tmax = fmax(tmax, (double)fmaxf(x.y - pos, 0.0f));
+            }
+        }
+        V3 v = mk(a.v[n], a.v[N + n], a.v[2 * N + n]);
+        const V3 fo = mk(a.fold[n], a.fold[N + n], a.fold[2 * N + n]);
+        vv_update(x, v, f, fo);
+        if constexpr (FIX) {
+            if (n == 3 || n == 4) {
+                x.x = 0.0f;
+                x.z = 0.0f;
+            }
+        }
+        a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
+        a.v[n] = v.x; a.v[N + n] = v.y; a.v[2 * N + n] = v.z;
+        a.fold[n] = f.x; a.fold[N + n] = f.y; a.fold[2 * N + n] = f.z;
+        if constexpr (!CSR) {
+            a.force[n] = 0.0f; a.force[N + n] = 0.0f; a.force[2 * N + n] = 0.0f;
+        }
+    }
+    if constexpr (TIP) reduce_finish(tsum, tmax, partials, counter, tip_sum, tip_max);
+}
+
+// ---- standalone per-plugin node kernels --------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_gather(NodeArgs a, float* __restrict__ force) {
+    constexpr int G = 4;
+    const int sub = threadIdx.x % G;
+    const int64_t N = a.N;
+    const int64_t n = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G;
+    if (n >= N) return;
+    const V3 f = gather_node<G>(a, n, sub);
+    if (sub == 0) {
+        force[n] += f.x;
+        force[N + n] += f.y;
+        force[2 * N + n] += f.z;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sphere(float* f, const float* __restrict__ x, int64_t N, float k, float R,
+                                                   float pos) {
+    const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    const V3 fs = sphere_term(mk(x[n], x[N + n], x[2 * N + n]), k, R, pos);
+    f[n] += fs.x;
+    f[N + n] += fs.y;
+    f[2 * N + n] += fs.z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_substrate(float* f, const float* __restrict__ x, int64_t N, float k, float w) {
+    const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    f[N + n] += substrate_term(x[N + n], k, w);
+}
+
+__global__ __launch_bounds__(kBlock) void k_plane(float* f, const float* __restrict__ x, int64_t N, float k, float w) {
+    const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    f[N + n] += plane_term(x[N + n], k, w);
+}
+
+__global__ __launch_bounds__(kBlock) void k_vv(float* x, float* v, float* f, float* fo, int64_t N, int32_t fix) {
+    const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    V3 xx = mk(x[n], x[N + n], x[2 * N + n]);
+    V3 vv = mk(v[n], v[N + n], v[2 * N + n]);
+    const V3 ff = mk(f[n], f[N + n], f[2 * N + n]);
+    const V3 oo = mk(fo[n], fo[N + n], fo[2 * N + n]);
+    vv_update(xx, vv, ff, oo);
+    if (fix && (n == 3 || n == 4)) {
+        xx.x = 0.0f;
+        xx.z = 0.0f;
+    }
+    x[n] = xx.x; x[N + n] = xx.y; x[2 * N + n] = xx.z;
+    v[n] = vv.x; v[N + n] = vv.y; v[2 * N + n] = vv.z;
+    fo[n] = ff.x; fo[N + n] = ff.y; fo[2 * N + n] = ff.z;
+    f[n] = 0.0f; f[N + n] = 0.0f; f[2 * N + n] = 0.0f;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tip(const float* __restrict__ x, int64_t N, int32_t ind, float k, float R,
+                                                float pos, double* partials, unsigned int* counter, double* out_sum,
+                                                double* out_max) {
+    double s = 0.0, m = 0.0;
+    for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += (int64_t)gridDim.x * kBlock) {
+        const V3 p = mk(x[n], x[N + n], x[2 * N + n]);
+        if (ind == NFX_INDENTER_SPHERE) {
+            const V3 fs = sphere_term(p, k, R, pos);
+            s -= (double)fs.y;
+            const float r = sqrtf(p.x * p.x + (p.y - pos) * (p.y - pos) + p.z * p.z);
+            m = fmax(m, (double)fmaxf(R - r, 0.0f));
+        } else {
+            s += (double)(k * fmaxf(p.y - pos, 0.0f));
+            m = fmax(m, (double)fmaxf(p.y - pos, 0.0f));
+        }
+    }
+    reduce_finish(s, m, partials, counter, out_sum, out_max);
+}
+
+__global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
+
+// ---- host helpers -----------------------------------------------------------------------------
+static inline unsigned int nblocks(int64_t n, int per = kBlock) { return (unsigned int)((n + per - 1) / per); }
+
+static int check_sizes(int64_t N, int64_t T) {
+    NFX_REQUIRE(N > 0 && N < (int64_t(1) << 31), "node count %lld out of range", (long long)N);
+    NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
+    return NFX_OK;
+}
+
+static TetArgs tet_args(const nfx_plan& p) {
+    return TetArgs{p.N, p.T, p.tetras, p.dminv, p.v0, p.points, p.force, p.corners};
+}
+
+static NodeArgs node_args(const nfx_plan& p) {
+    NodeArgs a;
+    a.N = p.N;
+    a.x = p.points;
+    a.v = p.vel;
+    a.force = p.force;
+    a.fold = p.fold;
+    a.off = p.csr_offsets;
+    a.slots = p.csr_slots;
+    a.corners = p.corners;
+    a.sub_k = (float)p.substrate_k;
+    a.wall = (float)p.wall;
+    a.indenter = p.indenter;
+    a.ind_k = (float)p.ind_k;
+    a.ind_radius = (float)p.ind_radius;
+    a.ind_start = p.ind_start;
+    a.ind_speed = p.ind_speed;
+    a.ind_move = p.ind_move_steps;
+    return a;
+}
+
+static int validate_plan(const nfx_plan* p) {
+    NFX_REQUIRE(p != nullptr, "plan is NULL");
+    int rc = check_sizes(p->N, p->T);
+    if (rc) return rc;
+    NFX_REQUIRE(p->tetras && p->dminv && p->points && p->vel && p->fold, "plan: NULL mesh/state pointer");
+    NFX_REQUIRE(p->model >= NFX_SVKK && p->model <= NFX_VISCOPLASTIC, "plan: unknown model %d", p->model);
+    NFX_REQUIRE(p->model == NFX_VISCOPLASTIC || p->v0, "plan: NULL v0");
+    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || (p->Fp && p->epsbar), "plan: viscoplastic state missing");
+    NFX_REQUIRE(p->scatter == NFX_SCATTER_ATOMIC || p->scatter == NFX_SCATTER_CSR, "plan: unknown scatter %d",
+                p->scatter);
+    NFX_REQUIRE(p->scatter != NFX_SCATTER_ATOMIC || p->force, "plan: atomic scatter needs force");
+    NFX_REQUIRE(p->scatter != NFX_SCATTER_CSR || (p->csr_offsets && p->csr_slots && p->corners),
+                "plan: CSR scatter needs offsets, slots and corners");
+    NFX_REQUIRE(p->indenter >= 0 && p->indenter <= 2, "plan: unknown indenter %d", p->indenter);
+    return NFX_OK;
+}
+
+static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
+    TetArgs a = tet_args(p);
+    const unsigned int nb = nblocks(p.T);
+    const bool csr = p.scatter == NFX_SCATTER_CSR;
+    hipStream_t s = ctx->stream;
+    if (p.model == NFX_VISCOPLASTIC) {
+        const VPConst c = vp_consts(p.vp);
+        const bool st = stress && p.von_mises && p.pressure;
+        if (csr && st) k_vp<true, true><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
+        else if (csr) k_vp<true, false><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, nullptr, nullptr);
+        else if (st) k_vp<false, true><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
+        else k_vp<false, false><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, nullptr, nullptr);
+    } else {
+        const ElasticDev e = elastic_consts(p.E, p.nu, p.ratio);
+        switch (p.model * 2 + (csr ? 1 : 0)) {
+            case NFX_MR * 2: k_tetra<NFX_MR, false><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_MR * 2 + 1: k_tetra<NFX_MR, true><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_SVK * 2: k_tetra<NFX_SVK, false><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_SVK * 2 + 1: k_tetra<NFX_SVK, true><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_SVKK * 2: k_tetra<NFX_SVKK, false><<<nb, kBlock, 0, s>>>(a, e); break;
+            default: k_tetra<NFX_SVKK, true><<<nb, kBlock, 0, s>>>(a, e); break;
+        }
+    }
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+template <bool CSR, bool SUB, int IND, bool FIX>
+static void launch_node_t(nfx_ctx* ctx, const NodeArgs& a, int64_t step, const int64_t* step_dev, int32_t off,
+                          double* tip_sum, double* tip_max) {
+    constexpr int G = CSR ? 4 : 1;
+    const bool tip = (tip_sum || tip_max) && IND != NFX_INDENTER_NONE;
+    unsigned int nb = nblocks(a.N * G);
+    if (tip) {
+        nb = std::min<unsigned int>(nb, kMaxReduceBlocks);
+        k_node_update<CSR, SUB, IND, FIX, (IND != NFX_INDENTER_NONE)><<<nb, kBlock, 0, ctx->stream>>>(
+            a, step, step_dev, off, ctx->partials, ctx->counter, tip_sum, tip_max);
+    } else {
+        k_node_update<CSR, SUB, IND, FIX, false><<<nb, kBlock, 0, ctx->stream>>>(a, step, step_dev, off, nullptr,
+                                                                                nullptr, nullptr, nullptr);
+    }
+}
+
+template <bool CSR, bool SUB>
+static void launch_node_ind(nfx_ctx* ctx, const nfx_plan& p, const NodeArgs& a, int64_t step, const int64_t* sd,
+                            int32_t off, double* ts, double* tm) {
+    const bool fix = p.fix_top_bottom != 0;
+    switch (p.indenter * 2 + (fix ? 1 : 0)) {
+        case 0: launch_node_t<CSR, SUB, 0, false>(ctx, a, step, sd, off, ts, tm); break;
+        case 1: launch_node_t<CSR, SUB, 0, true>(ctx, a, step, sd, off, ts, tm); break;
+        case 2: launch_node_t<CSR, SUB, 1, false>(ctx, a, step, sd, off, ts, tm); break;
+        case 3: launch_node_t<CSR, SUB, 1, true>(ctx, a, step, sd, off, ts, tm); break;
+        case 4: launch_node_t<CSR, SUB, 2, false>(ctx, a, step, sd, off, ts, tm); break;
+        default: launch_node_t<CSR, SUB, 2, true>(ctx, a, step, sd, off, ts, tm); break;
+    }
+}
+
+static int launch_node_pass(nfx_ctx* ctx, const nfx_plan& p, int64_t step, const int64_t* step_dev, int32_t off,
+                            double* tip_sum, double* tip_max) {
+    const NodeArgs a = node_args(p);
+    const bool csr = p.scatter == NFX_SCATTER_CSR, sub = p.substrate != 0;
+    if (csr && sub) launch_node_ind<true, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max);
+    else if (csr) launch_node_ind<true, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max);
+    else if (sub) launch_node_ind<false, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max);
+    else launch_node_ind<false, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+static void drop_graph(nfx_ctx* ctx) {
+    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
+    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+    ctx->graph_exec = nullptr;
+    ctx->graph = nullptr;
+    ctx->graph_steps = 0;
+}
+
+static int capture_graph(nfx_ctx* ctx, const nfx_plan& p, int64_t steps, bool hist) {
+    drop_graph(ctx);
+    NFX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeGlobal));
+    int rc = NFX_OK;
+    for (int64_t i = 0; i < steps && rc == NFX_OK; ++i) {
+        rc = launch_force_pass(ctx, p, false);
+        if (rc == NFX_OK)
+            rc = launch_node_pass(ctx, p, 0, ctx->step_dev, (int32_t)i, hist ? ctx->hist_dev + i : nullptr, nullptr);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != NFX_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        return NFX_ERR_HIP;
+    }
+    ctx->graph = g;
+    NFX_HIP(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
+    ctx->graph_plan = p;
+    ctx->graph_steps = steps;
+    ctx->graph_hist = hist;
+    return NFX_OK;
+}
+
+}  // namespace nfx
+
+// =================================================================================================
+// C ABI
+// =================================================================================================
+extern "C" {
+
+const char* nfx_last_error(void) { return nfx::g_err.c_str(); }
+
+int nfx_device_count(int* count) {
+    NFX_REQUIRE(count, "count is NULL");
+    int c = 0;
+    const hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) c = 0;
+    *count = c;
+    return NFX_OK;
+}
+
+int nfx_init(int device_id, void* stream, nfx_ctx** out) {
+    NFX_REQUIRE(out, "ctx out-pointer is NULL");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("no HIP device available");
+        return NFX_ERR_DEVICE;
+    }
+    if (device_id < 0 || device_id >= count) {
+        set_error("device %d not found (%d HIP devices)", device_id, count);
+        return NFX_ERR_DEVICE;
+    }
+    NFX_HIP(hipSetDevice(device_id));
+    nfx_ctx* c = new nfx_ctx();
+    c->device = device_id;
+    hipDeviceProp_t prop;
+    NFX_HIP(hipGetDeviceProperties(&prop, device_id));
+    c->num_cu = prop.multiProcessorCount;
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+    } else {
+        NFX_HIP(hipStreamCreate(&c->stream));
+        c->own_stream = true;
+    }
+    NFX_HIP(hipMalloc(&c->partials, sizeof(double) * 2 * kMaxReduceBlocks));
+    NFX_HIP(hipMalloc(&c->counter, sizeof(unsigned int) * 4));
+    NFX_HIP(hipMalloc(&c->step_dev, sizeof(int64_t) * 2));
+    NFX_HIP(hipMalloc(&c->hist_dev, sizeof(double) * (kGraphBlock + 2)));
+    NFX_HIP(hipMemset(c->counter, 0, sizeof(unsigned int) * 4));
+    NFX_HIP(hipMemset(c->step_dev, 0, sizeof(int64_t) * 2));
+    NFX_HIP(hipEventCreate(&c->ev0));
+    NFX_HIP(hipEventCreate(&c->ev1));
+    NFX_HIP(hipDeviceSynchronize());
+    *out = c;
+    return NFX_OK;
+}
+
+int nfx_destroy(nfx_ctx* c) {
+    if (!c) return NFX_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    nfx::drop_graph(c);
+    (void)hipFree(c->partials);
+    (void)hipFree(c->counter);
+    (void)hipFree(c->step_dev);
+    (void)hipFree(c->hist_dev);
+    (void)hipEventDestroy(c->ev0);
+    (void)hipEventDestroy(c->ev1);
+    if (c->own_stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NFX_OK;
+}
+
+int nfx_stream_sync(nfx_ctx* c) {
+    NFX_REQUIRE(c, "ctx is NULL");
+    NFX_HIP(hipStreamSynchronize(c->stream));
+    return NFX_OK;
+}
+
+int nfx_device_name(nfx_ctx* c, char* buf, int32_t len) {
+    NFX_REQUIRE(c && buf && len > 0, "bad arguments");
+    hipDeviceProp_t prop;
+    NFX_HIP(hipGetDeviceProperties(&prop, c->device));
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return NFX_OK;
+}
+
+int nfx_precompute_f64(nfx_ctx* c, const double* edge_vectors, const double* volumes, int64_t T, float* dminv,
+                       float* v0) {
+    NFX_REQUIRE(c && edge_vectors && volumes && dminv && v0, "NULL argument");
+    NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
+    k_precompute_f64<<<nblocks(T), kBlock, 0, c->stream>>>(edge_vectors, volumes, T, dminv, v0);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+static int tetra_common(nfx_ctx* c, const nfx_elastic* m, const float* points, const uint32_t* tetras,
+                        const float* dminv, const float* v0, int64_t N, int64_t T, float* force, float* corners) {
+    NFX_REQUIRE(c && m && points && tetras && dminv && v0 && (force || corners), "NULL argument");
+    NFX_REQUIRE(m->model >= NFX_SVKK && m->model <= NFX_MR, "unknown elastic model %d", m->model);
+    int rc = check_sizes(N, T);
+    if (rc) return rc;
+    nfx_plan p{};
+    p.N = N;
+    p.T = T;
+    p.tetras = tetras;
+    p.dminv = dminv;
+    p.v0 = v0;
+    p.points = const_cast<float*>(points);
+    p.force = force;
+    p.corners = corners;
+    p.model = m->model;
+    p.E = m->E;
+    p.nu = m->nu;
+    p.ratio = m->ratio;
+    p.scatter = corners ? NFX_SCATTER_CSR : NFX_SCATTER_ATOMIC;
+    return launch_force_pass(c, p, false);
+}
+
+int nfx_tetra_force(nfx_ctx* c, const nfx_elastic* m, const float* points, const uint32_t* tetras, const float* dminv,
+                    const float* v0, int64_t N, int64_t T, float* force) {
+    NFX_REQUIRE(force, "force is NULL");
+    return tetra_common(c, m, points, tetras, dminv, v0, N, T, force, nullptr);
+}
+
+int nfx_tetra_corner_forces(nfx_ctx* c, const nfx_elastic* m, const float* points, const uint32_t* tetras,
+                            const float* dminv, const float* v0, int64_t N, int64_t T, float* corners) {
+    NFX_REQUIRE(corners, "corners is NULL");
+    return tetra_common(c, m, points, tetras, dminv, v0, N, T, nullptr, corners);
+}
+
+int nfx_viscoplastic_force(nfx_ctx* c, const nfx_viscoplastic* vp, const float* points, const uint32_t* tetras,
+                           const float* dminv, int64_t N, int64_t T, float* Fp, float* epsbar, float* von_mises,
+                           float* pressure, float* force, float* corners) {
+    NFX_REQUIRE(c && vp && points && tetras && dminv && Fp && epsbar && (force || corners), "NULL argument");
+    int rc = check_sizes(N, T);
+    if (rc) return rc;
+    nfx_plan p{};
+    p.N = N;
+    p.T = T;
+    p.tetras = tetras;
+    p.dminv = dminv;
+    p.points = const_cast<float*>(points);
+    p.force = force;
+    p.corners = corners;
+    p.model = NFX_VISCOPLASTIC;
+    p.vp = *vp;
+    p.Fp = Fp;
+    p.epsbar = epsbar;
+    p.von_mises = von_mises;
+    p.pressure = pressure;
+    p.scatter = corners ? NFX_SCATTER_CSR : NFX_SCATTER_ATOMIC;
+    return launch_force_pass(c, p, true);
+}
+
+int nfx_gather_corners(nfx_ctx* c, const uint32_t* off, const uint32_t* slots, const float* corners, int64_t N,
+                       float* force) {
+    NFX_REQUIRE(c && off && slots && corners && force, "NULL argument");
+    NFX_REQUIRE(N > 0 && N < (int64_t(1) << 31), "node count out of range");
+    NodeArgs a{};
+    a.N = N;
+    a.off = off;
+    a.slots = slots;
+    a.corners = corners;
+    k_gather<<<nblocks(N * 4), kBlock, 0, c->stream>>>(a, force);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_build_node_csr_host(const uint32_t* tet, int64_t N, int64_t T, uint32_t* off, uint32_t* slots) {
+    NFX_REQUIRE(tet && off && slots, "NULL argument");
+    int rc = check_sizes(N, T);
+    if (rc) return rc;
+    std::vector<uint32_t> cnt((size_t)N + 1, 0);
+    for (int64_t a = 0; a < 4; ++a)
+        for (int64_t t = 0; t < T; ++t) {
+            const uint32_t n = tet[a * T + t];
+            NFX_REQUIRE(n < (uint64_t)N, "tetra %lld corner %lld references node %u >= N", (long long)t, (long long)a, n);
+            cnt[n + 1]++;
+        }
+    for (int64_t n = 0; n < N; ++n) cnt[n + 1] += cnt[n];
+    std::memcpy(off, cnt.data(), sizeof(uint32_t) * (size_t)(N + 1));
+    // ascending slot order per node: iterate slots in increasing order (t major, corner minor)
+    for (int64_t t = 0; t < T; ++t)
+        for (int64_t a = 0; a < 4; ++a) {
+            const uint32_t n = tet[a * T + t];
+            slots[cnt[n]++] = (uint32_t)(4 * t + a);
+        }
+    return NFX_OK;
+}
+
+int nfx_sphere_force(nfx_ctx* c, float* force, const float* points, int64_t N, double k, double radius, double pos) {
+    NFX_REQUIRE(c && force && points && N > 0, "bad argument");
+    k_sphere<<<nblocks(N), kBlock, 0, c->stream>>>(force, points, N, (float)k, (float)radius, (float)pos);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_substrate_force(nfx_ctx* c, float* force, const float* points, int64_t N, double k, double wall) {
+    NFX_REQUIRE(c && force && points && N > 0, "bad argument");
+    k_substrate<<<nblocks(N), kBlock, 0, c->stream>>>(force, points, N, (float)k, (float)wall);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_plane_force(nfx_ctx* c, float* force, const float* points, int64_t N, double k, double wall) {
+    NFX_REQUIRE(c && force && points && N > 0, "bad argument");
+    k_plane<<<nblocks(N), kBlock, 0, c->stream>>>(force, points, N, (float)k, (float)wall);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_velocity_verlet(nfx_ctx* c, float* points, float* vel, float* force, float* fold, int64_t N, int32_t fix) {
+    NFX_REQUIRE(c && points && vel && force && fold && N > 0, "bad argument");
+    k_vv<<<nblocks(N), kBlock, 0, c->stream>>>(points, vel, force, fold, N, fix);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_tip_force(nfx_ctx* c, const float* points, int64_t N, int32_t ind, double k, double radius, double pos,
+                  double* out2) {
+    NFX_REQUIRE(c && points && out2 && N > 0, "bad argument");
+    NFX_REQUIRE(ind == NFX_INDENTER_SPHERE || ind == NFX_INDENTER_PLANE, "unknown indenter %d", ind);
+    const unsigned int nb = std::min<unsigned int>(nblocks(N), kMaxReduceBlocks);
+    k_tip<<<nb, kBlock, 0, c->stream>>>(points, N, ind, (float)k, (float)radius, (float)pos, c->partials, c->counter,
+                                        out2, out2 + 1);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_node_update(nfx_ctx* c, const nfx_plan* p, int64_t step, double* tip_out) {
+    NFX_REQUIRE(c, "ctx is NULL");
+    int rc = validate_plan(p);
+    if (rc) return rc;
+    NFX_REQUIRE(!tip_out || p->indenter != NFX_INDENTER_NONE, "tip force output needs an indenter");
+    return launch_node_pass(c, *p, step, nullptr, 0, tip_out, tip_out ? tip_out + 1 : nullptr);
+}
+
+int nfx_run_steps(nfx_ctx* c, const nfx_plan* p, int64_t step0, int64_t nsteps, double* tip_hist, int32_t use_graph) {
+    NFX_REQUIRE(c, "ctx is NULL");
+    int rc = validate_plan(p);
+    if (rc) return rc;
+    NFX_REQUIRE(nsteps >= 0, "nsteps < 0");
+    NFX_REQUIRE(!tip_hist || p->indenter != NFX_INDENTER_NONE, "tip force history needs an indenter");
+    int64_t done = 0;
+    if (use_graph && nsteps >= 2) {
+        const int64_t B = std::min<int64_t>(nsteps, kGraphBlock);
+        const bool hist = tip_hist != nullptr;
+        if (!c->graph_exec || c->graph_steps != B || c->graph_hist != hist ||
+            std::memcmp(&c->graph_plan, p, sizeof(nfx_plan)) != 0) {
+            rc = capture_graph(c, *p, B, hist);
+            if (rc) return rc;
+        }
+        while (nsteps - done >= B) {
+            k_set_i64<<<1, 1, 0, c->stream>>>(c->step_dev, step0 + done);
+            NFX_CHECK_LAUNCH();
+            NFX_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+            if (hist)
+                NFX_HIP(hipMemcpyAsync(tip_hist + done, c->hist_dev, sizeof(double) * B, hipMemcpyDeviceToDevice,
+                                       c->stream));
+            done += B;
+        }
+    }
+    for (; done < nsteps; ++done) {
+        rc = launch_force_pass(c, *p, false);
+        if (rc) return rc;
+        rc = launch_node_pass(c, *p, step0 + done, nullptr, 0, tip_hist ? tip_hist + done : nullptr, nullptr);
+        if (rc) return rc;
+    }
+    return NFX_OK;
+}
+
+int nfx_time_passes(nfx_ctx* c, const nfx_plan* p, int64_t step0, int32_t nrep, double* ms_force, double* ms_node) {
+    NFX_REQUIRE(c && ms_force && ms_node && nrep > 0, "bad argument");
+    int rc = validate_plan(p);
+    if (rc) return rc;
+    std::vector<hipEvent_t> ev(3 * (size_t)nrep);
+    for (auto& e : ev) NFX_HIP(hipEventCreate(&e));
+    for (int i = 0; i < nrep && rc == NFX_OK; ++i) {
+        NFX_HIP(hipEventRecord(ev[3 * i], c->stream));
+        rc = launch_force_pass(c, *p, false);
+        if (rc) break;
+        NFX_HIP(hipEventRecord(ev[3 * i + 1], c->stream));
+        rc = launch_node_pass(c, *p, step0 + i, nullptr, 0, nullptr, nullptr);
+        NFX_HIP(hipEventRecord(ev[3 * i + 2], c->stream));
+    }
+    if (rc == NFX_OK) {
+        NFX_HIP(hipEventSynchronize(ev[3 * (size_t)nrep - 1]));
+        double sf = 0, sn = 0;
+        for (int i = 0; i < nrep; ++i) {
+            float a = 0, b = 0;
+            NFX_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
+            NFX_HIP(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
+            sf += a;
+            sn += b;
+        }
+        *ms_force = sf / nrep;
+        *ms_node = sn / nrep;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,97 @@
+"""Finite-strain viscoplastic material (multiplicative Fe Fp split, power-law flow with hardening).
+
+Restates the legacy ``Interaction_FiniteStrainViscoplastic`` (reference
+``kernels/InteractionFiniteStrainViscoplastic.cl:1-220``, wrapper ``Interactions.py:248-349``)
+behind the current plugin protocol, with node 0 as the edge base (the convention of
+``TetraCell.py:33-38``; the legacy kernel used node 3).  Parameters are uniform over the mesh, as
+the legacy wrapper's eight per-tetra arrays are (``Interactions.py:270-277``).  ``yieldOffset``
+(default 0 = the reference law) makes the flow rate use max(svm - yieldOffset, 0).
+
+State per tetra: the plastic deformation gradient Fp (9 SoA, starts at I) and the accumulated
+plastic strain; ``vonMises`` and ``pressure`` are written when the plugin's ``write_stress`` is on.
+"""
+from pathlib import Path
+
+import numpy as np
+
+from nofluidx3d_amd import _lib
+from nofluidx3d_amd.interactions.Interaction import Interaction
+from nofluidx3d_amd.interactions.Tetra import precompute_reference
+from nofluidx3d_amd.KernelBuilder import KernelBuilder
+
+kernels = Path(__file__).parents[1] / "kernels"
+
+
+class FiniteStrainViscoplastic(Interaction):
+    def __init__(
+        self,
+        referenceCell,
+        shearModulus1,
+        shearModulus2,
+        bulkModulus,
+        viscoplasticFlowExponent,
+        viscoplasticYieldStress,
+        viscoplasticFlowRate,
+        hardeningThreshhold,
+        hardeningExponent,
+        yieldOffset=0.0,
+    ):
+        self.referenceCell = referenceCell
+        self.params = _lib.Viscoplastic(
+            float(shearModulus1), float(shearModulus2), float(bulkModulus), float(viscoplasticFlowExponent),
+            float(viscoplasticYieldStress), float(viscoplasticFlowRate), float(hardeningThreshhold),
+            float(hardeningExponent), float(yieldOffset))
+        self.write_stress = False
+
+    def createBuffers(self):
+        import torch
+
+        self.dminvB, self.volumesB = precompute_reference(self.referenceCell, self.simulation)
+        self.referenceCell = None
+        T = self.simulation.numTetra
+        dev = self.dminvB.device
+        self.accumulatedStrainB = torch.zeros(T, dtype=torch.float32, device=dev)
+        # identity in the SoA layout: entry k of tetra t at k*T + t (Interactions.py:279-281)
+        fp = np.repeat(np.identity(3).reshape(9), T).astype(np.float32)
+        self.plasticDeformationTensorB = torch.from_numpy(fp).to(dev)
+
+    def build(self):
+        self.scope = self.simulation.numTetra
+        KernelBuilder.define(INSERT_NUM_POINTS=self.simulation.numPoints, INSERT_NUM_TETRAS=self.simulation.numTetra)
+        self.knl = KernelBuilder.build(kernels / "InteractionFiniteStrainViscoplastic.cl",
+                                       "Interaction_FiniteStrainViscoplastic")
+
+    def setArgs(self):
+        sim = self.simulation
+        self.knl.set_args(sim.force.buf, sim.points.buf, sim.tetras.buf, self.dminvB, self.params,
+                          self.accumulatedStrainB, self.plasticDeformationTensorB,
+                          sim.vonMises.buf if self.write_stress else None,
+                          sim.pressure.buf if self.write_stress else None)
+
+    def enqueue(self):
+        sim = self.simulation
+        if getattr(sim, "scatter", "atomic") == "csr":
+            from nofluidx3d_amd import openCL
+
+            off, slots, corners = sim.node_csr()
+            h = openCL.handle()
+            vm = sim.vonMises.ptr if self.write_stress else None
+            pr = sim.pressure.ptr if self.write_stress else None
+            _lib.call("nfx_viscoplastic_force", h, self.params, sim.points.ptr, sim.tetras.ptr,
+                      self.dminvB.data_ptr(), sim.numPoints, sim.numTetra, self.plasticDeformationTensorB.data_ptr(),
+                      self.accumulatedStrainB.data_ptr(), vm, pr, None, corners.data_ptr())
+            _lib.call("nfx_gather_corners", h, off.data_ptr(), slots.data_ptr(), corners.data_ptr(), sim.numPoints,
+                      sim.force.ptr)
+        else:
+            self.knl()
+
+    def contribute(self, plan) -> bool:
+        if plan.dminv:
+            return False
+        plan.model = _lib.NFX_VISCOPLASTIC
+        plan.vp = self.params
+        plan.dminv = self.dminvB.data_ptr()
+        plan.v0 = self.volumesB.data_ptr()
+        plan.Fp = self.plasticDeformationTensorB.data_ptr()
+        plan.epsbar = self.accumulatedStrainB.data_ptr()
+        return True

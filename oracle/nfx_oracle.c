@@ -1,0 +1,518 @@
+/*
+ * nfx_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * FP64 CPU restatement of noFluidX3D's explicit tetrahedral-FEM time step.  It is the checker
+ * for the HIP kernels (tests, smoke check) and the timed CPU baseline of bench.py; nothing in
+ * the product links or calls it.  Each function follows the reference's arithmetic in the
+ * order the OpenCL source writes it (same buffers, same SoA layouts, FP64 throughout):
+ *
+ *   orc_tetra_force      kernels/Interactions/tetra.cl:14-182 (SVKK, SVK, MR/NH branches)
+ *                        with kernels/matrix.cl:190-290 (Double3x3 algebra, 1-based rows)
+ *   orc_viscoplastic     kernels/InteractionFiniteStrainViscoplastic.cl:1-220, restated with
+ *                        node 0 as the edge base (the legacy kernel used node 3; TetraCell.py:33-38
+ *                        uses node 0), plus an optional yield offset (0 = reference law)
+ *   orc_sphere           kernels/Interactions/Sphere.cl:1-17
+ *   orc_substrate        kernels/Interactions/Substrate.cl:1-7
+ *   orc_plane            kernels/Interactions/Plane.cl:1-7
+ *   orc_velocity_verlet  kernels/Interactions/VelocityVerlet.cl:1-36
+ *   orc_tip_force        Simulation.py:198-212 (forceSI, without the p0*L0^2 unit factor)
+ *   orc_run              Simulation.py:232-237 timeStep in registration order
+ *                        (Tetra, Substrate, Sphere|Plane, VelocityVerlet)
+ *
+ * The node scatter (kernels/atomicAdd.cl:1-11 CAS loop) is restated as a plain sequential add
+ * when nthreads == 1 (deterministic order t = 0..T-1) and as an OpenMP atomic add otherwise.
+ * Parity of each function is pinned by tests/test_oracle.py (closed forms, energy consistency by
+ * central differences, objectivity, momentum balance, exact integrator recurrences).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { double x, y, z; } d3;
+typedef struct { double m11, m12, m13, m21, m22, m23, m31, m32, m33; } M3;
+
+static inline d3 v3(double x, double y, double z) { d3 r = {x, y, z}; return r; }
+static inline d3 vadd(d3 a, d3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline d3 vsub(d3 a, d3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline d3 vscale(d3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
+static inline double vdot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline d3 vcross(d3 a, d3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline double vlen(d3 a) { return sqrt(vdot(a, a)); }
+static inline d3 vdivs(d3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
+
+/* matrix.cl:190-204 */
+static inline M3 from_rows(d3 r1, d3 r2, d3 r3) {
+    M3 m = {r1.x, r1.y, r1.z, r2.x, r2.y, r2.z, r3.x, r3.y, r3.z};
+    return m;
+}
+static inline M3 from_cols(d3 c1, d3 c2, d3 c3) {
+    M3 m = {c1.x, c2.x, c3.x, c1.y, c2.y, c3.y, c1.z, c2.z, c3.z};
+    return m;
+}
+/* matrix.cl:214-232 */
+static inline d3 row(M3 m, int i) {
+    return i == 1 ? v3(m.m11, m.m12, m.m13) : i == 2 ? v3(m.m21, m.m22, m.m23) : v3(m.m31, m.m32, m.m33);
+}
+static inline d3 col(M3 m, int j) {
+    return j == 1 ? v3(m.m11, m.m21, m.m31) : j == 2 ? v3(m.m12, m.m22, m.m32) : v3(m.m13, m.m23, m.m33);
+}
+/* matrix.cl:234-248 */
+static inline double mtr(M3 m) { return m.m11 + m.m22 + m.m33; }
+static inline double mdet(M3 m) { return vdot(col(m, 1), vcross(col(m, 2), col(m, 3))); }
+static inline M3 mT(M3 m) {
+    M3 r = {m.m11, m.m21, m.m31, m.m12, m.m22, m.m32, m.m13, m.m23, m.m33};
+    return r;
+}
+/* matrix.cl:280-290 */
+static inline M3 mscale(M3 m, double s) {
+    M3 r = {m.m11 * s, m.m12 * s, m.m13 * s, m.m21 * s, m.m22 * s, m.m23 * s, m.m31 * s, m.m32 * s, m.m33 * s};
+    return r;
+}
+static inline M3 mdiv(M3 m, double s) { return mscale(m, 1.0 / s); }
+/* matrix.cl:250-253: cofactor rows divided by det, no singularity check */
+static inline M3 minv(M3 m) {
+    return mdiv(from_rows(vcross(col(m, 2), col(m, 3)), vcross(col(m, 3), col(m, 1)), vcross(col(m, 1), col(m, 2))),
+                mdet(m));
+}
+/* matrix.cl:255-265 */
+static inline M3 madd(M3 a, M3 b) {
+    return from_rows(vadd(row(a, 1), row(b, 1)), vadd(row(a, 2), row(b, 2)), vadd(row(a, 3), row(b, 3)));
+}
+static inline M3 msub(M3 a, M3 b) {
+    return from_rows(vsub(row(a, 1), row(b, 1)), vsub(row(a, 2), row(b, 2)), vsub(row(a, 3), row(b, 3)));
+}
+/* matrix.cl:267-273 */
+static inline M3 mmul(M3 a, M3 b) {
+    M3 r = {vdot(row(a, 1), col(b, 1)), vdot(row(a, 1), col(b, 2)), vdot(row(a, 1), col(b, 3)),
+            vdot(row(a, 2), col(b, 1)), vdot(row(a, 2), col(b, 2)), vdot(row(a, 2), col(b, 3)),
+            vdot(row(a, 3), col(b, 1)), vdot(row(a, 3), col(b, 2)), vdot(row(a, 3), col(b, 3))};
+    return r;
+}
+static inline M3 munit(void) {
+    M3 r = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    return r;
+}
+
+/* tetra.cl:5-12: ten Newton steps X <- (X + X^-T)/2, then X^T * initial */
+static M3 approximate_root(M3 initial) {
+    M3 it = initial;
+    for (int i = 0; i < 10; i++) it = mdiv(madd(it, minv(mT(it))), 2.0);
+    return mmul(mT(it), initial);
+}
+
+/* exported for the SVKK pin: approximateRoot of a row-major 3x3 */
+void orc_polar_stretch(const double* m9, double* out9) {
+    M3 m = {m9[0], m9[1], m9[2], m9[3], m9[4], m9[5], m9[6], m9[7], m9[8]};
+    M3 r = approximate_root(m);
+    out9[0] = r.m11; out9[1] = r.m12; out9[2] = r.m13;
+    out9[3] = r.m21; out9[4] = r.m22; out9[5] = r.m23;
+    out9[6] = r.m31; out9[7] = r.m32; out9[8] = r.m33;
+}
+
+enum { ORC_SVKK = 0, ORC_SVK = 1, ORC_MR = 2 };
+
+static inline void scatter(double* f, int64_t idx, double v, int atomic) {
+    if (atomic) {
+#pragma omp atomic
+        f[idx] += v;
+    } else {
+        f[idx] += v;
+    }
+}
+
+/* kernels/Interactions/tetra.cl:14-182 for tetra t. */
+static void tetra_one(int64_t t, int model, double E, double nu, double ratio, int64_t N, int64_t T,
+                      const double* points, const uint32_t* tetras, const double* ev, const double* vol,
+                      double* force, int atomic) {
+    const uint32_t i0 = tetras[t], i1 = tetras[T + t], i2 = tetras[2 * T + t], i3 = tetras[3 * T + t];
+    const d3 p0 = v3(points[i0], points[N + i0], points[2 * N + i0]);
+    const d3 p1 = v3(points[i1], points[N + i1], points[2 * N + i1]);
+    const d3 p2 = v3(points[i2], points[N + i2], points[2 * N + i2]);
+    const d3 p3 = v3(points[i3], points[N + i3], points[2 * N + i3]);
+    const d3 r1 = vsub(p1, p0), r2 = vsub(p2, p0), r3 = vsub(p3, p0); /* :31-35 */
+    const M3 rT = from_cols(r1, r2, r3);                                /* :37 */
+    const d3 R1 = v3(ev[t], ev[T + t], ev[2 * T + t]);                  /* :40-43 */
+    const d3 R2 = v3(ev[3 * T + t], ev[4 * T + t], ev[5 * T + t]);
+    const d3 R3 = v3(ev[6 * T + t], ev[7 * T + t], ev[8 * T + t]);
+    const double V0 = vol[t];
+    const M3 J = from_cols(R1, R2, R3); /* :47 */
+    const M3 invJ = minv(J);            /* :53 */
+    const M3 F = mmul(rT, invJ);        /* :63 */
+    const M3 FT = mT(F);
+    const M3 B = mmul(F, FT); /* :68 */
+    const M3 uT = munit();
+    M3 Force;
+    if (model == ORC_SVKK) { /* :76-93 */
+        const double kappa = nu / (1 - 2 * nu);
+        const double mu = E / (2 * (1 + nu));
+        const M3 S = approximate_root(B);
+        const M3 invS = minv(S);
+        const double sf = kappa * (mtr(S) - 3) - 1;
+        const M3 inner = madd(uT, mscale(invS, sf));
+        Force = mscale(mmul(mmul(invJ, inner), FT), -V0 * mu);
+    } else if (model == ORC_SVK) { /* :94-114 */
+        const double kappa = nu / (1 - 2 * nu);
+        const double mu = E / (2 * (1 + nu));
+        const M3 BP = msub(B, uT);
+        const M3 H = mmul(invJ, FT);
+        const M3 term1 = mscale(H, kappa * mtr(BP));
+        const M3 term2 = mmul(H, BP);
+        Force = mscale(madd(term1, term2), -V0 * mu);
+    } else { /* :115-163, MR and NH */
+        const double mu = E / (2 * (1 + nu));
+        const double mu1 = ratio * mu;
+        const double mu2 = (1 - ratio) * mu;
+        const double kappa = E / (3 * (1 - 2 * nu));
+        const M3 Bsq = mmul(B, B);
+        const double trB = mtr(B);
+        const double trBsq = mtr(Bsq);
+        const double Jr = mdet(rT) / (6 * V0);
+        const double dUdI1 = mu1 / pow(Jr, 5 / 3.0);
+        const double dUdK = mu2 / pow(Jr, 7 / 3.0);
+        const double dUdJ = kappa * (Jr - 1);
+        const double dI1dJ = -trB / 3;
+        const M3 term1 = mscale(B, dUdI1);
+        const M3 term2 = mscale(uT, dUdJ + dI1dJ * dUdI1);
+        const M3 inner1 = msub(mscale(B, trB), Bsq);
+        const M3 inner2 = mscale(uT, (trBsq - trB * trB) / 3);
+        const M3 term3 = mscale(madd(inner1, inner2), dUdK);
+        const M3 sigma = madd(madd(term1, term2), term3);
+        const M3 helper = mmul(minv(rT), sigma); /* :159 */
+        Force = mscale(helper, -V0 * Jr);        /* :162 */
+    }
+    const d3 f1 = row(Force, 1), f2 = row(Force, 2), f3 = row(Force, 3);
+    const d3 f0 = vscale(vadd(vadd(f1, f2), f3), -1.0); /* :168 */
+    scatter(force, i0, f0.x, atomic);                    /* :170-181 */
+    scatter(force, N + i0, f0.y, atomic);
+    scatter(force, 2 * N + i0, f0.z, atomic);
+    scatter(force, i1, f1.x, atomic);
+    scatter(force, N + i1, f1.y, atomic);
+    scatter(force, 2 * N + i1, f1.z, atomic);
+    scatter(force, i2, f2.x, atomic);
+    scatter(force, N + i2, f2.y, atomic);
+    scatter(force, 2 * N + i2, f2.z, atomic);
+    scatter(force, i3, f3.x, atomic);
+    scatter(force, N + i3, f3.y, atomic);
+    scatter(force, 2 * N + i3, f3.z, atomic);
+}
+
+static int set_threads(int nthreads) {
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    return nthreads;
+#else
+    (void)nthreads;
+    return 1;
+#endif
+}
+
+void orc_tetra_force(int model, double E, double nu, double ratio, int64_t N, int64_t T, const double* points,
+                     const uint32_t* tetras, const double* ev, const double* vol, double* force, int nthreads) {
+    nthreads = set_threads(nthreads);
+    if (nthreads == 1) {
+        for (int64_t t = 0; t < T; t++) tetra_one(t, model, E, nu, ratio, N, T, points, tetras, ev, vol, force, 0);
+        return;
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t t = 0; t < T; t++) tetra_one(t, model, E, nu, ratio, N, T, points, tetras, ev, vol, force, 1);
+}
+
+/* Viscoplastic material parameters; uniform over tetra as the reference wrapper builds them
+ * (Interactions.py:270-277, values from SimulationLegacy.py:842-882). */
+typedef struct {
+    double mu1, mu2, kappa;      /* shear moduli and bulk modulus */
+    double flow_exponent;        /* m   */
+    double yield_stress;         /* Y   (sigma0 at zero accumulated strain) */
+    double flow_rate;            /* eps0dot, per step */
+    double hardening_threshold;  /* eps0 */
+    double hardening_exponent;   /* n   */
+    double yield_offset;         /* sigma_y: rate uses max(svm - sigma_y, 0); 0 = reference law */
+} orc_vp_params;
+
+/* InteractionFiniteStrainViscoplastic.cl:1-220 with the edge base moved to node 0: the legacy
+ * nodes (p1, p2, p3, p4) are (tetras[1], tetras[2], tetras[3], tetras[0]). */
+static void vp_one(int64_t t, const orc_vp_params* P, int64_t N, int64_t T, const double* points,
+                   const uint32_t* tetras, const double* ev, const double* vol, double* force, double* acc,
+                   double* Fp, double* vonMises, double* pressure, int atomic) {
+    const uint32_t id1 = tetras[T + t], id2 = tetras[2 * T + t], id3 = tetras[3 * T + t], id4 = tetras[t];
+    const d3 p1 = v3(points[id1], points[N + id1], points[2 * N + id1]);
+    const d3 p2 = v3(points[id2], points[N + id2], points[2 * N + id2]);
+    const d3 p3 = v3(points[id3], points[N + id3], points[2 * N + id3]);
+    const d3 p4 = v3(points[id4], points[N + id4], points[2 * N + id4]);
+    const d3 r1 = vsub(p1, p4), r2 = vsub(p2, p4), r3 = vsub(p3, p4); /* :18-22 */
+    const d3 R1 = v3(ev[t], ev[T + t], ev[2 * T + t]);                  /* :25-28 */
+    const d3 R2 = v3(ev[3 * T + t], ev[4 * T + t], ev[5 * T + t]);
+    const d3 R3 = v3(ev[6 * T + t], ev[7 * T + t], ev[8 * T + t]);
+    (void)vol;
+    const d3 u1 = vsub(r1, R1), u2 = vsub(r2, R2), u3 = vsub(r3, R3); /* :32-34 */
+    const double divisor = vdot(vcross(R1, R2), R3);                  /* :38-42 */
+    const d3 invJ1 = vdivs(vcross(R2, R3), divisor);
+    const d3 invJ2 = vdivs(vcross(R3, R1), divisor);
+    const d3 invJ3 = vdivs(vcross(R1, R2), divisor);
+    /* :47-55 */
+    const d3 du1 = vadd(vadd(vscale(invJ1, u1.x), vscale(invJ2, u2.x)), vscale(invJ3, u3.x));
+    const d3 du2 = vadd(vadd(vscale(invJ1, u1.y), vscale(invJ2, u2.y)), vscale(invJ3, u3.y));
+    const d3 du3 = vadd(vadd(vscale(invJ1, u1.z), vscale(invJ2, u2.z)), vscale(invJ3, u3.z));
+    const d3 F1 = vadd(du1, v3(1, 0, 0)), F2 = vadd(du2, v3(0, 1, 0)), F3 = vadd(du3, v3(0, 0, 1));
+    /* :59-75 */
+    double a = Fp[0 * T + t], b = Fp[1 * T + t], c = Fp[2 * T + t];
+    double d = Fp[3 * T + t], e = Fp[4 * T + t], f = Fp[5 * T + t];
+    double g = Fp[6 * T + t], h = Fp[7 * T + t], i = Fp[8 * T + t];
+    const double detFp = vdot(vcross(v3(a, b, c), v3(d, e, f)), v3(g, h, i));
+    const d3 iF1 = vscale(v3(e * i - f * h, c * h - b * i, b * f - c * e), 1.0 / detFp);
+    const d3 iF2 = vscale(v3(f * g - d * i, a * i - c * g, c * d - a * f), 1.0 / detFp);
+    const d3 iF3 = vscale(v3(d * h - e * g, b * g - a * h, a * e - b * d), 1.0 / detFp);
+    /* :77-79  Fe = F * Fp^-1 */
+    const d3 Fe1 = v3(F1.x * iF1.x + F1.y * iF2.x + F1.z * iF3.x, F1.x * iF1.y + F1.y * iF2.y + F1.z * iF3.y,
+                      F1.x * iF1.z + F1.y * iF2.z + F1.z * iF3.z);
+    const d3 Fe2 = v3(F2.x * iF1.x + F2.y * iF2.x + F2.z * iF3.x, F2.x * iF1.y + F2.y * iF2.y + F2.z * iF3.y,
+                      F2.x * iF1.z + F2.y * iF2.z + F2.z * iF3.z);
+    const d3 Fe3 = v3(F3.x * iF1.x + F3.y * iF2.x + F3.z * iF3.x, F3.x * iF1.y + F3.y * iF2.y + F3.z * iF3.y,
+                      F3.x * iF1.z + F3.y * iF2.z + F3.z * iF3.z);
+    /* :82-96 */
+    const d3 B1 = v3(vdot(Fe1, Fe1), vdot(Fe1, Fe2), vdot(Fe1, Fe3));
+    const d3 B2 = v3(vdot(Fe2, Fe1), vdot(Fe2, Fe2), vdot(Fe2, Fe3));
+    const d3 B3 = v3(vdot(Fe3, Fe1), vdot(Fe3, Fe2), vdot(Fe3, Fe3));
+    const double J = sqrt(vdot(vcross(B1, B2), B3));
+    const d3 Bsq1 = v3(B1.x * B1.x + B1.y * B2.x + B1.z * B3.x, B1.x * B1.y + B1.y * B2.y + B1.z * B3.y,
+                       B1.x * B1.z + B1.y * B2.z + B1.z * B3.z);
+    const d3 Bsq2 = v3(B2.x * B1.x + B2.y * B2.x + B2.z * B3.x, B2.x * B1.y + B2.y * B2.y + B2.z * B3.y,
+                       B2.x * B1.z + B2.y * B2.z + B2.z * B3.z);
+    const d3 Bsq3 = v3(B3.x * B1.x + B3.y * B2.x + B3.z * B3.x, B3.x * B1.y + B3.y * B2.y + B3.z * B3.y,
+                       B3.x * B1.z + B3.y * B2.z + B3.z * B3.z);
+    const double trB = B1.x + B2.y + B3.z;
+    const double trBsq = Bsq1.x + Bsq2.y + Bsq3.z;
+    const d3 E1 = v3(1, 0, 0), E2 = v3(0, 1, 0), E3 = v3(0, 0, 1);
+    /* :105-107 */
+    const double j53 = pow(J, -5.0 / 3.0), j73 = pow(J, -7.0 / 3.0);
+    const d3 s1 = vadd(vadd(vscale(vsub(B1, vscale(E1, 1. / 3. * trB)), P->mu1 * j53), vscale(E1, P->kappa * (J - 1))),
+                       vscale(vadd(vsub(vsub(vscale(B1, trB), vscale(E1, 1.0 / 3.0 * trB * trB)), Bsq1),
+                                   vscale(E1, 1.0 / 3.0 * trBsq)),
+                              P->mu2 * j73));
+    const d3 s2 = vadd(vadd(vscale(vsub(B2, vscale(E2, 1. / 3. * trB)), P->mu1 * j53), vscale(E2, P->kappa * (J - 1))),
+                       vscale(vadd(vsub(vsub(vscale(B2, trB), vscale(E2, 1.0 / 3.0 * trB * trB)), Bsq2),
+                                   vscale(E2, 1.0 / 3.0 * trBsq)),
+                              P->mu2 * j73));
+    const d3 s3 = vadd(vadd(vscale(vsub(B3, vscale(E3, 1. / 3. * trB)), P->mu1 * j53), vscale(E3, P->kappa * (J - 1))),
+                       vscale(vadd(vsub(vsub(vscale(B3, trB), vscale(E3, 1.0 / 3.0 * trB * trB)), Bsq3),
+                                   vscale(E3, 1.0 / 3.0 * trBsq)),
+                              P->mu2 * j73));
+    /* :110-123 outward unit normals */
+    d3 n1 = vcross(r2, r3), n2 = vcross(r3, r1), n3 = vcross(r1, r2), n4 = vcross(vsub(r3, r1), vsub(r3, r2));
+    n1 = vdivs(n1, vlen(n1));
+    n2 = vdivs(n2, vlen(n2));
+    n3 = vdivs(n3, vlen(n3));
+    n4 = vdivs(n4, vlen(n4));
+    if (vdot(n1, vscale(r1, -1.0)) < 0) n1 = vscale(n1, -1.0);
+    if (vdot(n2, vscale(r2, -1.0)) < 0) n2 = vscale(n2, -1.0);
+    if (vdot(n3, vscale(r3, -1.0)) < 0) n3 = vscale(n3, -1.0);
+    if (vdot(n4, r1) < 0) n4 = vscale(n4, -1.0);
+    /* :126-141 */
+    const double A1 = 0.5 * vlen(vcross(r2, r3)), A2 = 0.5 * vlen(vcross(r3, r1)), A3 = 0.5 * vlen(vcross(r1, r2));
+    const double A4 = 0.5 * vlen(vcross(vsub(r3, r1), vsub(r3, r2)));
+    const d3 fA1 = vscale(v3(vdot(s1, n1), vdot(s2, n1), vdot(s3, n1)), A1);
+    const d3 fA2 = vscale(v3(vdot(s1, n2), vdot(s2, n2), vdot(s3, n2)), A2);
+    const d3 fA3 = vscale(v3(vdot(s1, n3), vdot(s2, n3), vdot(s3, n3)), A3);
+    const d3 fA4 = vscale(v3(vdot(s1, n4), vdot(s2, n4), vdot(s3, n4)), A4);
+    const d3 f1 = vscale(vadd(vadd(fA2, fA3), fA4), -1.0 / 3.0);
+    const d3 f2 = vscale(vadd(vadd(fA3, fA4), fA1), -1.0 / 3.0);
+    const d3 f3 = vscale(vadd(vadd(fA4, fA1), fA2), -1.0 / 3.0);
+    const d3 f4 = vscale(vadd(vadd(fA1, fA2), fA3), -1.0 / 3.0);
+    /* :144-156 */
+    const double trS = s1.x + s2.y + s3.z;
+    const d3 ds1 = vsub(s1, vscale(E1, trS / 3.0)), ds2 = vsub(s2, vscale(E2, trS / 3.0)),
+             ds3 = vsub(s3, vscale(E3, trS / 3.0));
+    const double svm = sqrt(3. / 2. * (vdot(ds1, ds1) + vdot(ds2, ds2) + vdot(ds3, ds3)));
+    const double sigma0 = P->yield_stress * pow(1 + acc[t] / P->hardening_threshold, P->hardening_exponent);
+    if (vonMises) vonMises[t] = svm;
+    if (pressure) pressure[t] = trS / 3.0;
+    const double over = P->yield_offset > 0 ? fmax(svm - P->yield_offset, 0.0) : svm;
+    const double rate = P->flow_rate * pow(over / sigma0, P->flow_exponent);
+    acc[t] += rate;
+    /* :160-169 */
+    d3 Dp1, Dp2, Dp3;
+    if (svm != 0) {
+        Dp1 = vdivs(vscale(ds1, rate * 3. / 2.), svm);
+        Dp2 = vdivs(vscale(ds2, rate * 3. / 2.), svm);
+        Dp3 = vdivs(vscale(ds3, rate * 3. / 2.), svm);
+    } else {
+        Dp1 = Dp2 = Dp3 = v3(0, 0, 0);
+    }
+    /* :173-206  Fp += Fe^-1 * Dp * F */
+    a = Fe1.x; b = Fe1.y; c = Fe1.z; d = Fe2.x; e = Fe2.y; f = Fe2.z; g = Fe3.x; h = Fe3.y; i = Fe3.z;
+    const double detFe = vdot(vcross(Fe1, Fe2), Fe3);
+    const d3 iE1 = vscale(v3(e * i - f * h, c * h - b * i, b * f - c * e), 1. / detFe);
+    const d3 iE2 = vscale(v3(f * g - d * i, a * i - c * g, c * d - a * f), 1. / detFe);
+    const d3 iE3 = vscale(v3(d * h - e * g, b * g - a * h, a * e - b * d), 1. / detFe);
+    const d3 tm1 = v3(iE1.x * Dp1.x + iE1.y * Dp2.x + iE1.z * Dp3.x, iE1.x * Dp1.y + iE1.y * Dp2.y + iE1.z * Dp3.y,
+                      iE1.x * Dp1.z + iE1.y * Dp2.z + iE1.z * Dp3.z);
+    const d3 tm2 = v3(iE2.x * Dp1.x + iE2.y * Dp2.x + iE2.z * Dp3.x, iE2.x * Dp1.y + iE2.y * Dp2.y + iE2.z * Dp3.y,
+                      iE2.x * Dp1.z + iE2.y * Dp2.z + iE2.z * Dp3.z);
+    const d3 tm3 = v3(iE3.x * Dp1.x + iE3.y * Dp2.x + iE3.z * Dp3.x, iE3.x * Dp1.y + iE3.y * Dp2.y + iE3.z * Dp3.y,
+                      iE3.x * Dp1.z + iE3.y * Dp2.z + iE3.z * Dp3.z);
+    const d3 dF1 = v3(tm1.x * F1.x + tm1.y * F2.x + tm1.z * F3.x, tm1.x * F1.y + tm1.y * F2.y + tm1.z * F3.y,
+                      tm1.x * F1.z + tm1.y * F2.z + tm1.z * F3.z);
+    const d3 dF2 = v3(tm2.x * F1.x + tm2.y * F2.x + tm2.z * F3.x, tm2.x * F1.y + tm2.y * F2.y + tm2.z * F3.y,
+                      tm2.x * F1.z + tm2.y * F2.z + tm2.z * F3.z);
+    const d3 dF3 = v3(tm3.x * F1.x + tm3.y * F2.x + tm3.z * F3.x, tm3.x * F1.y + tm3.y * F2.y + tm3.z * F3.y,
+                      tm3.x * F1.z + tm3.y * F2.z + tm3.z * F3.z);
+    Fp[0 * T + t] += dF1.x; Fp[1 * T + t] += dF1.y; Fp[2 * T + t] += dF1.z;
+    Fp[3 * T + t] += dF2.x; Fp[4 * T + t] += dF2.y; Fp[5 * T + t] += dF2.z;
+    Fp[6 * T + t] += dF3.x; Fp[7 * T + t] += dF3.y; Fp[8 * T + t] += dF3.z;
+    /* :209-220 (legacy p1..p4 -> node ids id1..id4) */
+    scatter(force, id1, f1.x, atomic); scatter(force, N + id1, f1.y, atomic); scatter(force, 2 * N + id1, f1.z, atomic);
+    scatter(force, id2, f2.x, atomic); scatter(force, N + id2, f2.y, atomic); scatter(force, 2 * N + id2, f2.z, atomic);
+    scatter(force, id3, f3.x, atomic); scatter(force, N + id3, f3.y, atomic); scatter(force, 2 * N + id3, f3.z, atomic);
+    scatter(force, id4, f4.x, atomic); scatter(force, N + id4, f4.y, atomic); scatter(force, 2 * N + id4, f4.z, atomic);
+}
+
+void orc_viscoplastic(const orc_vp_params* P, int64_t N, int64_t T, const double* points, const uint32_t* tetras,
+                      const double* ev, const double* vol, double* force, double* acc, double* Fp, double* vonMises,
+                      double* pressure, int nthreads) {
+    nthreads = set_threads(nthreads);
+    if (nthreads == 1) {
+        for (int64_t t = 0; t < T; t++)
+            vp_one(t, P, N, T, points, tetras, ev, vol, force, acc, Fp, vonMises, pressure, 0);
+        return;
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t t = 0; t < T; t++) vp_one(t, P, N, T, points, tetras, ev, vol, force, acc, Fp, vonMises, pressure, 1);
+}
+
+/* Sphere.cl:1-17.  OpenCL normalize() of a zero vector returns the zero vector. */
+void orc_sphere(int64_t N, double k, double* force, const double* points, double radius, double spherePos) {
+    for (int64_t p = 0; p < N; p++) {
+        const d3 pos = v3(points[p], points[N + p], points[2 * N + p]);
+        const d3 r = vsub(pos, v3(0, spherePos, 0));
+        const double len = vlen(r);
+        const d3 n = len > 0 ? vscale(r, 1.0 / len) : v3(0, 0, 0);
+        const double dis = fmax(radius - len, 0.);
+        const d3 fc = vscale(n, k * dis);
+        force[p] += fc.x;
+        force[N + p] += fc.y;
+        force[2 * N + p] += fc.z;
+    }
+}
+
+/* Substrate.cl:1-7 */
+void orc_substrate(int64_t N, double k, double* force, const double* points, double wallPos) {
+    for (int64_t p = 0; p < N; p++) {
+        const double dis = fmax(wallPos - points[N + p], 0.);
+        force[N + p] += k * dis;
+    }
+}
+
+/* Plane.cl:1-7 */
+void orc_plane(int64_t N, double k, double* force, const double* points, double wallPos) {
+    for (int64_t p = 0; p < N; p++) {
+        const double dis = fmax(points[N + p] - wallPos, 0.);
+        force[N + p] += -k * dis;
+    }
+}
+
+/* VelocityVerlet.cl:1-36 (xi = 0.005, unit mass, dt = 1) */
+void orc_velocity_verlet(int64_t N, int fixTopBottom, double* points, double* vel, double* force, double* fold) {
+    const double xi = 0.005;
+    for (int64_t p = 0; p < N; p++) {
+        d3 x = v3(points[p], points[N + p], points[2 * N + p]);
+        d3 v = v3(vel[p], vel[N + p], vel[2 * N + p]);
+        const d3 f = v3(force[p], force[N + p], force[2 * N + p]);
+        const d3 fo = v3(fold[p], fold[N + p], fold[2 * N + p]);
+        v = vscale(vadd(vscale(v, 1 - 0.5 * xi), vscale(vadd(f, fo), 0.5)), 1.0 / (1 + 0.5 * xi));
+        x = vadd(vadd(x, vscale(v, 1 - 0.5 * xi)), vscale(f, 0.5));
+        if (fixTopBottom && (p == 3 || p == 4)) {
+            x.x = 0.0;
+            x.z = 0.0;
+        }
+        points[p] = x.x; points[N + p] = x.y; points[2 * N + p] = x.z;
+        vel[p] = v.x; vel[N + p] = v.y; vel[2 * N + p] = v.z;
+        fold[p] = f.x; fold[N + p] = f.y; fold[2 * N + p] = f.z;
+        force[p] = 0.0; force[N + p] = 0.0; force[2 * N + p] = 0.0;
+    }
+}
+
+/* Simulation.py:198-212 in simulation units: -sum_p (k * max(R - |x-c|, 0) * n)_y and the
+ * maximum penetration.  indenter 1 = sphere at (0, pos, 0); 2 = plane at y = pos
+ * (Simulation.py:146-153, which returns +sum(k * max(y - pos, 0))). */
+void orc_tip_force(int64_t N, const double* points, int indenter, double k, double radius, double pos,
+                   double* out_force, double* out_maxdis) {
+    double s = 0, mx = 0;
+    for (int64_t p = 0; p < N; p++) {
+        if (indenter == 1) {
+            const d3 r = vsub(v3(points[p], points[N + p], points[2 * N + p]), v3(0, pos, 0));
+            const double len = vlen(r);
+            const double dis = fmax(radius - len, 0.);
+            const double ny = len > 0 ? r.y / len : 0.0;
+            s += -(k * dis * ny);
+            if (dis > mx) mx = dis;
+        } else {
+            const double dis = fmax(points[N + p] - pos, 0.);
+            s += k * dis;
+            if (dis > mx) mx = dis;
+        }
+    }
+    *out_force = s;
+    *out_maxdis = mx;
+}
+
+/* One simulation run with the plugin sequence Tetra|Viscoplastic, Substrate, Sphere|Plane,
+ * VelocityVerlet (Simulation.py:232-237), with the indenter law of Simulation.py:117-132 /
+ * 173-183.  tip_hist[s] receives the indenter force of step s (computed from the positions at
+ * the start of step s, i.e. forceSI at time s). */
+typedef struct {
+    int model;              /* ORC_SVKK / ORC_SVK / ORC_MR, or 3 = viscoplastic */
+    double E, nu, ratio;    /* elastic constants in simulation units */
+    orc_vp_params vp;       /* model 3 */
+    int substrate;          /* 1 = on */
+    double substrate_k, wall;
+    int indenter;           /* 0 none, 1 sphere, 2 plane */
+    double ind_k, ind_radius, ind_start, ind_speed, ind_move_steps;
+    int fix_top_bottom;
+    int nthreads;
+} orc_run_params;
+
+static double indenter_pos(const orc_run_params* R, int64_t time) {
+    /* Simulation.py:173-183 (sphere) and :117-132 (plane) */
+    if ((double)time <= R->ind_move_steps) return R->ind_start - R->ind_speed * (double)time;
+    return R->ind_start - R->ind_speed * R->ind_move_steps;
+}
+
+void orc_run(const orc_run_params* R, int64_t N, int64_t T, double* points, const uint32_t* tetras, const double* ev,
+             const double* vol, double* vel, double* force, double* fold, double* acc, double* Fp, int64_t step0,
+             int64_t nsteps, double* tip_hist) {
+    for (int64_t s = 0; s < nsteps; s++) {
+        const int64_t time = step0 + s;
+        if (R->model == 3)
+            orc_viscoplastic(&R->vp, N, T, points, tetras, ev, vol, force, acc, Fp, NULL, NULL, R->nthreads);
+        else
+            orc_tetra_force(R->model, R->E, R->nu, R->ratio, N, T, points, tetras, ev, vol, force, R->nthreads);
+        if (R->substrate) orc_substrate(N, R->substrate_k, force, points, R->wall);
+        if (R->indenter) {
+            const double pos = indenter_pos(R, time);
+            if (tip_hist) {
+                double fz, mx;
+                orc_tip_force(N, points, R->indenter, R->ind_k, R->ind_radius, pos, &fz, &mx);
+                tip_hist[s] = fz;
+            }
+            if (R->indenter == 1)
+                orc_sphere(N, R->ind_k, force, points, R->ind_radius, pos);
+            else
+                orc_plane(N, R->ind_k, force, points, pos);
+        }
+        orc_velocity_verlet(N, R->fix_top_bottom, points, vel, force, fold);
+    }
+}
+
+int orc_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,377 @@
+"""GPU parity of the plain path: each libnfx kernel through the C ABI against the FP64 oracle on
+the same seeded inputs, then fused multi-step runs.
+
+Tolerances (FP32 device state vs FP64 oracle):
+- single force evaluation: max |f_gpu - f_oracle| <= 2e-5 * max |f_oracle|;
+- multi-step runs: positions within 1e-4 * RadiusSIM and the indenter force curve within
+  1e-4 * max |F| (the curve error is normalised by the curve maximum because the force is zero
+  before contact, so a pointwise relative error is undefined);
+- deterministic (CSR) mode: bitwise identical across repeated runs and across graph/eager.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(2024)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from nofluidx3d_amd import openCL
+
+    openCL.initializeOpenCLObjects(0)
+    return openCL.handle()
+
+
+def dev(a, dtype=None):
+    import torch
+
+    a = np.ascontiguousarray(a)
+    if a.dtype == np.uint32:
+        a = a.view(np.int32)
+    t = torch.from_numpy(a if dtype is None else a.astype(dtype)).to("cuda:0")
+    torch.cuda.synchronize()
+    return t
+
+
+def host(t, dtype=np.float64):
+    import torch
+
+    from nofluidx3d_amd import _lib, openCL
+
+    _lib.call("nfx_stream_sync", openCL.handle())
+    torch.cuda.synchronize()
+    return t.cpu().numpy().astype(dtype)
+
+
+def call(name, *args):
+    from nofluidx3d_amd import _lib
+
+    _lib.call(name, *args)
+
+
+def small_mesh(radius=3.0, spacing=0.6, seed=3, scale=5.0):
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    P, T = generate_sphere_mesh(radius, spacing, seed)
+    return P * scale, T
+
+
+def gpu_precompute(ctx, P, T):
+    import torch
+
+    ev, vol = O.edge_vectors_volumes(P, T)
+    dminv = torch.empty(9 * len(T), dtype=torch.float32, device="cuda:0")
+    v0 = torch.empty(len(T), dtype=torch.float32, device="cuda:0")
+    evd, vold = dev(O.soa(ev)), dev(vol)
+    call("nfx_precompute_f64", ctx, evd.data_ptr(), vold.data_ptr(), len(T), dminv.data_ptr(), v0.data_ptr())
+    return dminv, v0, ev, vol
+
+
+# ------------------------------------------------------------------------------------------------
+# single kernels
+# ------------------------------------------------------------------------------------------------
+
+
+def test_precompute_inverse_fp64(ctx):
+    P, T = small_mesh()
+    dminv, v0, ev, vol = gpu_precompute(ctx, P, T)
+    D = host(dminv).reshape(9, -1).T.reshape(-1, 3, 3)
+    Dm = ev.reshape(-1, 3, 3).transpose(0, 2, 1)  # columns R1, R2, R3
+    ref = np.linalg.inv(Dm)
+    assert np.max(np.abs(D - ref) / np.abs(ref).max(axis=(1, 2))[:, None, None]) < 2e-7
+    assert np.array_equal(host(v0, np.float32), vol.astype(np.float32))
+
+
+@pytest.mark.parametrize("model,ratio", [(O.MR, 1.0), (O.MR, 0.75), (O.SVK, 1.0), (O.SVKK, 1.0)])
+def test_tetra_force_parity(ctx, model, ratio):
+    import torch
+
+    from nofluidx3d_amd import _lib
+
+    P, T = small_mesh()
+    N, nT = len(P), len(T)
+    dminv, v0, ev, vol = gpu_precompute(ctx, P, T)
+    x = P + 0.15 * RNG.normal(size=P.shape)
+    x32 = x.astype(np.float32).astype(np.float64)
+    E, nu = 0.01, 0.45
+    f_ref = np.zeros(3 * N)
+    O.tetra_force(model, E, nu, ratio, O.soa(x32), O.soa(T, np.uint32), O.soa(ev), np.ascontiguousarray(vol), f_ref)
+    xd, td = dev(O.soa(x32), np.float32), dev(O.soa(T, np.uint32))
+    f = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
+    mat = _lib.Elastic(model, E, nu, ratio)
+    call("nfx_tetra_force", ctx, mat, xd.data_ptr(), td.data_ptr(), dminv.data_ptr(), v0.data_ptr(), N, nT, f.data_ptr())
+    err = np.max(np.abs(host(f) - f_ref)) / np.max(np.abs(f_ref))
+    assert err < 2e-5, err
+    # per-corner records + deterministic gather give the same node forces
+    corners = torch.zeros(12 * nT, dtype=torch.float32, device="cuda:0")
+    call("nfx_tetra_corner_forces", ctx, mat, xd.data_ptr(), td.data_ptr(), dminv.data_ptr(), v0.data_ptr(), N, nT,
+         corners.data_ptr())
+    c = host(corners).reshape(nT, 4, 3)
+    assert np.max(np.abs(c.sum(1))) < 1e-6 * np.max(np.abs(c))  # f0 = -(f1+f2+f3)
+    off, slots = csr(T, N)
+    g = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
+    call("nfx_gather_corners", ctx, off.data_ptr(), slots.data_ptr(), corners.data_ptr(), N, g.data_ptr())
+    err = np.max(np.abs(host(g) - f_ref)) / np.max(np.abs(f_ref))
+    assert err < 2e-5, err
+
+
+def csr(T, N):
+    off = np.zeros(N + 1, np.uint32)
+    slots = np.zeros(4 * len(T), np.uint32)
+    tets = O.soa(T, np.uint32)
+    call("nfx_build_node_csr_host", tets.ctypes.data_as(ctypes.c_void_p), N, len(T),
+         off.ctypes.data_as(ctypes.c_void_p), slots.ctypes.data_as(ctypes.c_void_p))
+    return dev(off), dev(slots)
+
+
+def test_node_kernels_parity(ctx):
+    import torch
+
+    N = 5000
+    x = RNG.normal(scale=4.0, size=(N, 3))
+    x32 = x.astype(np.float32).astype(np.float64)
+    f0 = RNG.normal(scale=1e-3, size=3 * N).astype(np.float32).astype(np.float64)
+    xs = O.soa(x32)
+    # sphere + substrate + plane
+    for name, fn, args in (("nfx_sphere_force", O.sphere, (0.1, 3.0, 1.5)), ("nfx_substrate_force", O.substrate,
+                                                                            (0.1, -2.0)),
+                           ("nfx_plane_force", O.plane, (0.1, 2.5))):
+        fr = f0.copy()
+        fn(args[0], fr, xs, *args[1:])
+        fd = dev(f0, np.float32)
+        call(name, ctx, fd.data_ptr(), dev(xs, np.float32).data_ptr(), N, *args)
+        assert np.max(np.abs(host(fd) - fr)) < 1e-6, name
+    # velocity Verlet, with fixTopBottom
+    v = RNG.normal(scale=1e-2, size=3 * N).astype(np.float32).astype(np.float64)
+    fo = RNG.normal(scale=1e-3, size=3 * N).astype(np.float32).astype(np.float64)
+    for fix in (0, 1):
+        xr, vr, fr, for_ = xs.copy(), v.copy(), f0.copy(), fo.copy()
+        O.velocity_verlet(xr, vr, fr, for_, fix_top_bottom=bool(fix))
+        xd, vd, fd, fod = (dev(a, np.float32) for a in (xs, v, f0, fo))
+        call("nfx_velocity_verlet", ctx, xd.data_ptr(), vd.data_ptr(), fd.data_ptr(), fod.data_ptr(), N, fix)
+        assert np.max(np.abs(host(xd) - xr)) < 4e-6
+        assert np.max(np.abs(host(vd) - vr)) < 1e-7
+        assert np.array_equal(host(fod), for_) and np.all(host(fd) == 0)
+    # FP64 tip-force reduction vs oracle sum on the same FP32 positions
+    out = torch.zeros(2, dtype=torch.float64, device="cuda:0")
+    for ind, R, pos in ((1, 3.0, 2.0), (2, 0.0, 1.0)):
+        call("nfx_tip_force", ctx, dev(xs, np.float32).data_ptr(), N, ind, 0.1, R, pos, out.data_ptr())
+        f, m = host(out)
+        fr, mr = O.tip_force(xs, ind, 0.1, R, pos)
+        assert f == pytest.approx(fr, rel=1e-5) and m == pytest.approx(mr, rel=1e-5)
+
+
+def test_viscoplastic_kernel_parity(ctx):
+    import torch
+
+    from nofluidx3d_amd import _lib
+
+    P, T = small_mesh()
+    N, nT = len(P), len(T)
+    dminv, v0, ev, vol = gpu_precompute(ctx, P, T)
+    x = P + 0.3 * RNG.normal(size=P.shape)
+    x32 = O.soa(x.astype(np.float32).astype(np.float64))
+    E, nu, ratio = 0.01, 0.46, 0.75
+    mu = E / (2 * (1 + nu))
+    prm = dict(mu1=ratio * mu, mu2=(1 - ratio) * mu, kappa=E / (3 * (1 - 2 * nu)), m=1.0, Y=0.002, rate=5e-3,
+               eps0=1.0, n=0.0, sy=0.0005)
+    vp_o = O.VPParams(*prm.values())
+    vp_g = _lib.Viscoplastic(*prm.values())
+    Fp_r = O.soa(np.tile(np.eye(3).reshape(1, 9), (nT, 1)))
+    acc_r = np.zeros(nT)
+    Fp_d = dev(Fp_r, np.float32)
+    acc_d = torch.zeros(nT, dtype=torch.float32, device="cuda:0")
+    vm_d = torch.zeros(nT, dtype=torch.float32, device="cuda:0")
+    pr_d = torch.zeros(nT, dtype=torch.float32, device="cuda:0")
+    xd, td = dev(x32, np.float32), dev(O.soa(T, np.uint32))
+    for it in range(5):
+        f_r, vm_r, pr_r = np.zeros(3 * N), np.zeros(nT), np.zeros(nT)
+        O.viscoplastic(vp_o, x32, O.soa(T, np.uint32), O.soa(ev), np.ascontiguousarray(vol), f_r, acc_r, Fp_r, vm_r,
+                       pr_r)
+        f_d = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
+        call("nfx_viscoplastic_force", ctx, vp_g, xd.data_ptr(), td.data_ptr(), dminv.data_ptr(), N, nT,
+             Fp_d.data_ptr(), acc_d.data_ptr(), vm_d.data_ptr(), pr_d.data_ptr(), f_d.data_ptr(), None)
+        assert np.max(np.abs(host(f_d) - f_r)) < 3e-5 * np.max(np.abs(f_r)), it
+        assert np.max(np.abs(host(vm_d) - vm_r)) < 3e-5 * np.max(vm_r)
+        assert np.max(np.abs(host(pr_d) - pr_r)) < 3e-5 * np.max(np.abs(pr_r))
+        assert np.max(np.abs(host(acc_d) - acc_r)) < 3e-5 * np.max(acc_r) + 1e-12
+        assert np.max(np.abs(host(Fp_d) - Fp_r)) < 1e-6
+    assert acc_r.max() > 0  # flow happened
+
+
+# ------------------------------------------------------------------------------------------------
+# fused multi-step runs through the Simulation API
+# ------------------------------------------------------------------------------------------------
+
+
+def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, mesh=(4.0, 0.8, 1), material="NH",
+              indenter="sphere", steps=400):
+    from nofluidx3d_amd import configs
+    from nofluidx3d_amd.Simulation import Simulation
+    from nofluidx3d_amd.util import vtk
+
+    cfg = configs.small_cell(outdir="/tmp/nfx_test/", steps=steps, frames=4)
+    cfg["MESH"] = {"radius_um": mesh[0], "spacing_um": mesh[1], "seed": mesh[2]}
+    cfg["CELL"]["RadiusSI"] = mesh[0] * 1e-6
+    cfg["CELL"]["RadiusSIM"] = 2.5 * mesh[0]
+    cfg["SphereRadiusSI"] = 0.4 * mesh[0] * 1e-6
+    cfg["InitialDistance"] = -0.5
+    cfg.update(cfg_overrides or {})
+    if material == "VP":
+        cfg["CELL"].update({"MooneyRivlinRatio": 0.75, "model": "MOONEY_RIVLIN", "ViscositySI": 0.05,
+                            "YieldStressSI": 20.0})
+    P, T = configs.make_mesh(cfg)
+    sim = Simulation(cfg, scatter=scatter, fused=fused, use_graph=use_graph, mesh=vtk.from_tetra(P, T),
+                     verbose=False)
+    sim.setInteractionSubstrate()
+    if indenter == "sphere":
+        sim.setInteractionSphere()
+    else:
+        sim.setInteractionPlane()
+    if material == "VP":
+        sim.setInteractionFiniteStrainViscoplastic()
+    else:
+        sim.setInteractionTetra()
+    sim.setInteractionVelocityVerlet()
+    return sim
+
+
+def oracle_for(sim, nthreads=1):
+    from nofluidx3d_amd.interactions import FiniteStrainViscoplastic, Sphere, Substrate, Tetra
+
+    prm = O.RunParams()
+    prm.nthreads = nthreads
+    for inter in sim.interactions:
+        if isinstance(inter, Tetra):
+            v = inter.knl.values
+            prm.model = O.MR if inter.model_code() == 2 else inter.model_code()
+            prm.E, prm.nu, prm.ratio = (float(v[k]) for k in ("def_tetraYoungsModulus", "def_tetraPoissonRatio",
+                                                                "def_tetraMooneyRivlinRatio"))
+        elif isinstance(inter, FiniteStrainViscoplastic):
+            prm.model = O.VISCOPLASTIC
+            p = inter.params
+            prm.vp = O.VPParams(p.mu1, p.mu2, p.kappa, p.flow_exponent, p.yield_stress, p.flow_rate,
+                                p.hardening_threshold, p.hardening_exponent, p.yield_offset)
+        elif isinstance(inter, Substrate):
+            prm.substrate, prm.substrate_k, prm.wall = 1, inter.forceConst, inter.wallPosition
+        elif isinstance(inter, Sphere):
+            prm.indenter, prm.ind_k, prm.ind_radius = 1, inter.forceConst, inter.radius
+            prm.ind_start, prm.ind_speed, prm.ind_move_steps = inter.sphereFunc.law
+        elif hasattr(inter, "wallFunc"):
+            prm.indenter, prm.ind_k, prm.ind_radius = 2, inter.forceConst, 0.0
+            prm.ind_start, prm.ind_speed, prm.ind_move_steps = inter.wallFunc.law
+    return O.OracleState(sim.cell.points, sim.cell.tetras, prm)
+
+
+def run_gpu(sim, nsteps):
+    import torch
+
+    hist = torch.zeros(nsteps, dtype=torch.float64, device="cuda:0")
+    sim.advance(nsteps, tip_hist=hist)
+    return sim.positions(), host(hist)
+
+
+def assert_parity(sim, gx, gf, ox, of):
+    R = sim.parameters["CELL"]["RadiusSIM"]
+    pos_err = np.max(np.abs(gx - ox)) / R
+    f_err = np.max(np.abs(gf - of)) / np.max(np.abs(of))
+    assert np.max(np.abs(of)) > 0
+    assert pos_err < 1e-4, pos_err
+    assert f_err < 1e-4, f_err
+    return pos_err, f_err
+
+
+@pytest.mark.parametrize("scatter", ["atomic", "csr"])
+@pytest.mark.parametrize("indenter", ["sphere", "plane"])
+def test_fused_run_matches_oracle(ctx, scatter, indenter):
+    sim = build_sim(scatter=scatter, indenter=indenter)
+    gx, gf = run_gpu(sim, 600)
+    ref = oracle_for(sim)
+    of = ref.run(600)
+    assert_parity(sim, gx, gf, ref.positions(), of)
+
+
+def test_viscoplastic_fused_run_matches_oracle(ctx):
+    sim = build_sim(material="VP", scatter="csr")
+    gx, gf = run_gpu(sim, 600)
+    ref = oracle_for(sim)
+    of = ref.run(600)
+    assert ref.acc.max() > 0
+    assert_parity(sim, gx, gf, ref.positions(), of)
+
+
+def test_csr_is_bitwise_deterministic_and_graph_equals_eager(ctx):
+    a = build_sim(scatter="csr", use_graph=True)
+    xa, fa = run_gpu(a, 250)
+    b = build_sim(scatter="csr", use_graph=True)
+    xb, fb = run_gpu(b, 250)
+    c = build_sim(scatter="csr", use_graph=False)
+    xc, fc = run_gpu(c, 250)
+    assert np.array_equal(xa, xb) and np.array_equal(fa, fb)
+    assert np.array_equal(xa, xc) and np.array_equal(fa, fc)
+
+
+def test_per_plugin_path_matches_fused(ctx):
+    """timeStep() plugin by plugin (the reference's loop) vs the fused device step."""
+    a = build_sim(scatter="csr", fused=False)
+    for _ in range(60):
+        a.timeStep()
+    b = build_sim(scatter="csr", fused=True)
+    b.advance(60)
+    xa, xb = a.positions(), b.positions()
+    assert np.max(np.abs(xa - xb)) < 1e-5 * a.parameters["CELL"]["RadiusSIM"]
+
+
+def test_simulation_run_writes_frames_and_simdata(ctx, tmp_path):
+    from nofluidx3d_amd.util import vtk
+
+    sim = build_sim(cfg_overrides={"OutputDir": str(tmp_path) + "/"}, steps=80)
+    sim.parameters["VTKFramesTotal"] = 4
+    sim.run()
+    data = np.loadtxt(str(tmp_path / "simdata.dat"))
+    assert data.shape == (5, 3)
+    assert np.array_equal(data[:, 0], [0, 20, 40, 60, 80])
+    # the recorded force equals the oracle's indenter force at the same times
+    ref = oracle_for(build_sim(steps=80))
+    of = ref.run(81)
+    scale = sim.p0 * sim.L0**2
+    assert np.max(np.abs(data[:, 2] / scale - of[[0, 20, 40, 60, 80]])) < 1e-4 * np.max(np.abs(of))
+    g = vtk.read(str(tmp_path / "vtkfiles" / "frame80.vtk"))
+    assert np.allclose(g.points, sim.positions())
+
+
+def test_small_cell_full_run_matches_oracle(ctx):
+    """The plain small cell (R 8 um, 0.8 um lattice, seed 1; ~25k tetra) over its full 20 000
+    steps: positions and the 201-sample tip-force curve against the FP64 oracle."""
+    import torch
+
+    from nofluidx3d_amd import configs
+    from nofluidx3d_amd.Simulation import Simulation
+    from nofluidx3d_amd.util import vtk
+
+    cfg = configs.small_cell(outdir="/tmp/nfx_c1/")
+    P, T = configs.make_mesh(cfg)
+    sim = Simulation(cfg, scatter="csr", mesh=vtk.from_tetra(P, T), verbose=False)
+    sim.setInteractionSubstrate()
+    sim.setInteractionSphere()
+    sim.setInteractionTetra()
+    sim.setInteractionVelocityVerlet()
+    n = sim.Nsteps
+    hist = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+    sim.advance(n, tip_hist=hist)
+    gx, gf = sim.positions(), host(hist)
+    ref = oracle_for(sim, nthreads=O.max_threads())
+    of = ref.run(n)
+    pe, fe = assert_parity(sim, gx, gf[::100], ref.positions(), of[::100])
+    print(f"small cell 20k steps: pos_err {pe:.2e} force_err {fe:.2e} max F {np.max(np.abs(of)):.3e}")

@@ -1,0 +1,206 @@
+"""CPU tests: the C-ABI library's exports, host-side helpers, mesh generator, VTK I/O, TetraCell,
+KernelBuilder and the config unit mappings.  No GPU compute is called."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols(header="nfx.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(nfx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from nofluidx3d_amd import _lib
+
+    L = _lib.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(L, s), f"libnfx.so does not export {s}"
+        assert s in _lib.SIGNATURES, f"no ctypes signature for {s}"
+
+
+def test_header_structs_match_ctypes():
+    """field order of nfx_plan in include/nfx.h equals the ctypes Plan."""
+    from nofluidx3d_amd import _lib
+
+    text = open(os.path.join(ROOT, "include", "nfx.h")).read()
+    body = text[text.index("typedef struct {\n    int64_t N, T;"):text.index("} nfx_plan;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip().replace("typedef struct {", "").strip()
+        if not decl:
+            continue
+        parts = [w for w in decl.replace("*", " ").split() if w != "const"]
+        for n in " ".join(parts[1:]).split(","):
+            names.append(n.strip())
+    assert names == [f[0] for f in _lib.Plan._fields_]
+
+
+def test_node_csr_host_matches_numpy():
+    from nofluidx3d_amd import _lib
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    P, T = generate_sphere_mesh(3.0, 0.75, seed=5)
+    N, nT = len(P), len(T)
+    tets = np.ascontiguousarray(T.reshape(-1, order="F").astype(np.uint32))
+    off = np.zeros(N + 1, np.uint32)
+    slots = np.zeros(4 * nT, np.uint32)
+    _lib.call("nfx_build_node_csr_host", tets.ctypes.data_as(ctypes.c_void_p), N, nT,
+              off.ctypes.data_as(ctypes.c_void_p), slots.ctypes.data_as(ctypes.c_void_p))
+    # brute force: for node n, the ascending list of 4*t + a with T[t, a] == n
+    flat_slot = (4 * np.arange(nT)[:, None] + np.arange(4)[None, :]).ravel()
+    flat_node = T.ravel()
+    for n in range(0, N, 7):
+        exp = np.sort(flat_slot[flat_node == n])
+        assert np.array_equal(slots[off[n]:off[n + 1]], exp)
+    assert off[-1] == 4 * nT
+    # a corner index out of range is rejected, not written
+    bad = tets.copy()
+    bad[5] = N + 3
+    rc = _lib.lib().nfx_build_node_csr_host(bad.ctypes.data_as(ctypes.c_void_p), N, nT,
+                                            off.ctypes.data_as(ctypes.c_void_p),
+                                            slots.ctypes.data_as(ctypes.c_void_p))
+    assert rc != 0 and b"references node" in _lib.lib().nfx_last_error()
+
+
+def test_null_arguments_are_errors_not_crashes():
+    from nofluidx3d_amd import _lib
+
+    L = _lib.lib()
+    assert L.nfx_stream_sync(None) == -1
+    assert L.nfx_tip_force(None, None, 10, 1, 0.1, 1.0, 0.0, None) == -1
+    assert L.nfx_run_steps(None, None, 0, 1, None, 0) == -1
+
+
+def test_mesh_generator_invariants():
+    from nofluidx3d_amd.mesh import DEFAULT_MIN_QUALITY, generate_sphere_mesh, signed_volumes, tetra_quality
+
+    P, T = generate_sphere_mesh(8.0, 0.8, seed=1)
+    assert (len(P), len(T)) == (4169, 25342)  # the seeded small cell
+    assert np.array_equal(P[0], [0, 0, 0]) and np.array_equal(P[1], [8.0, 0, 0])
+    assert np.all(signed_volumes(P, T) > 0)
+    assert tetra_quality(P, T).min() >= DEFAULT_MIN_QUALITY
+    assert len(np.unique(T)) == len(P)  # every node used
+    # boundary faces (in exactly one tetra) form closed surfaces: every edge lies in an even number
+    # of them (the boundary of a boundary is empty mod 2)
+    faces = np.sort(np.concatenate([T[:, [1, 2, 3]], T[:, [0, 2, 3]], T[:, [0, 1, 3]], T[:, [0, 1, 2]]]), 1)
+    uf, cnt = np.unique(faces, axis=0, return_counts=True)
+    assert cnt.max() == 2
+    bnd = uf[cnt == 1]
+    edges = np.sort(np.concatenate([bnd[:, [0, 1]], bnd[:, [1, 2]], bnd[:, [0, 2]]]), 1)
+    _, ecnt = np.unique(edges, axis=0, return_counts=True)
+    assert np.all(ecnt % 2 == 0)
+    # deterministic from the seed
+    P2, T2 = generate_sphere_mesh(8.0, 0.8, seed=1)
+    assert np.array_equal(P, P2) and np.array_equal(T, T2)
+
+
+@pytest.mark.parametrize("binary", [True, False])
+def test_vtk_roundtrip(tmp_path, binary):
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+    from nofluidx3d_amd.util import vtk
+
+    P, T = generate_sphere_mesh(2.0, 0.5, seed=3)
+    g = vtk.from_tetra(P, T)
+    g.cell_data.set_array(np.arange(len(T), dtype=np.float64) * 0.5, "vonMises")
+    path = tmp_path / "m.vtk"
+    g.save(str(path), binary=binary)
+    h = vtk.read(str(path))
+    assert np.array_equal(h.points, P)
+    assert np.array_equal(h.cells_dict[10], T)
+    assert np.array_equal(h.cell_data["vonMises"], g.cell_data["vonMises"])
+
+
+def test_vtk_reads_version_51_ascii(tmp_path):
+    from nofluidx3d_amd.util import vtk
+
+    text = """# vtk DataFile Version 5.1
+cell
+ASCII
+DATASET UNSTRUCTURED_GRID
+POINTS 5 float
+0 0 0 1 0 0 0 1 0
+0 0 1 1 1 1
+CELLS 3 8
+OFFSETS vtktypeint64
+0 4 8
+CONNECTIVITY vtktypeint64
+0 1 2 3 1 2 3 4
+CELL_TYPES 2
+10
+10
+CELL_DATA 2
+SCALARS p double 1
+LOOKUP_TABLE default
+1.5 2.5
+"""
+    path = tmp_path / "m51.vtk"
+    path.write_text(text)
+    g = vtk.read(str(path))
+    assert g.n_points == 5 and g.n_cells == 2
+    assert np.array_equal(g.cells_dict[10], [[0, 1, 2, 3], [1, 2, 3, 4]])
+    assert np.allclose(g.cell_data["p"], [1.5, 2.5])
+
+
+def test_tetracell_recenter_scale_and_reference_quantities(tmp_path):
+    import oracle as O
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+    from nofluidx3d_amd.TetraCell import TetraCell
+    from nofluidx3d_amd.util import vtk
+
+    P, T = generate_sphere_mesh(3.0, 0.6, seed=2)
+    shifted = P * 1e-6 + np.array([1e-6, -2e-6, 3e-6])
+    path = tmp_path / "cell.vtk"
+    vtk.from_tetra(shifted, T).save(str(path))
+    c = TetraCell(str(path), radius=15.0)
+    assert np.allclose(c.points[0], 0)
+    assert abs(c.points[1][0]) == pytest.approx(15.0)
+    assert np.allclose(c.points, P * 5.0, atol=1e-9)
+    ev, vol = O.edge_vectors_volumes(c.points, c.tetras)
+    assert np.allclose(c.edgeVectors, ev) and np.allclose(c.volumes, vol)
+    c.addTetraData("vonMises", np.ones(len(T)))
+    assert c.mesh.cell_data["vonMises"].shape == (len(T),)
+    c.mesh.points = c.points * 1.1
+    c.calcVolumeChange()
+    assert np.allclose(c.mesh.cell_data["volumeChange"], 1.1**3 - 1)
+
+
+def test_kernel_builder_defines_render_like_the_reference():
+    from nofluidx3d_amd.KernelBuilder import KernelBuilder
+
+    KernelBuilder.define("FLAG_X", a=3, b=True, c=0.5)
+    s = KernelBuilder.getDefines()
+    assert "#define FLAG_X\n" in s and "#define a 3\n" in s and "#define b true\n" in s and "#define c 0.5\n" in s
+    k = KernelBuilder.build("kernels/Interactions/tetra.cl", "calcTetraForce")
+    assert k.values["a"] == 3 and "FLAG_X" in k.defines
+    with pytest.raises(Exception):
+        KernelBuilder.build("kernels/CalcCOM.cl", "CalcCOM")
+
+
+def test_config_units():
+    import oracle as O
+    from nofluidx3d_amd import configs
+
+    c = configs.small_cell()
+    u = O.units(c["CELL"]["RadiusSI"], c["CELL"]["RadiusSIM"], c["CELL"]["YoungsModulusSI"], c["RhoSI"])
+    assert round(c["SimulationTimeSI"] / u["T0"]) == 20000
+    assert round(round(c["SimulationTimeSI"] / u["T0"]) / c["VTKFramesTotal"]) == 100
+    # tip travels 2 um in the move time
+    assert c["VelocitySI"] * c["MoveTimeSI"] == pytest.approx(2e-6)
+    L = configs.large_cell()
+    cc = L["CELL"]
+    mu = cc["YoungsModulusSI"] / (2 * (1 + cc["PoissonRatio"]))
+    K = cc["YoungsModulusSI"] / (3 * (1 - 2 * cc["PoissonRatio"]))
+    assert mu * cc["MooneyRivlinRatio"] == pytest.approx(300.0)
+    assert mu * (1 - cc["MooneyRivlinRatio"]) == pytest.approx(100.0)
+    assert K == pytest.approx(5000.0)

@@ -1,0 +1,428 @@
+"""Pins of the FP64 oracle against mathematics (the reference has no tests or golden vectors).
+
+Each check is chosen so that a plausible restatement mistake (a dropped term, a wrong sign, a
+transposed operand, a wrong SoA index) fails it:
+- hyperelastic forces: energy consistency f = -dE/dx against an independently written strain
+  energy (central differences), textbook closed forms (hydrostatic, uniaxial NH, simple-shear MR
+  including the normal-stress differences), rest state, rigid motions, momentum balance;
+- SVK: energy consistency with Psi = lambda/2 tr(E)^2 + mu tr(E^2); SVKK's polar root against
+  scipy.linalg.polar; SVKK rest state and hydrostatic closed form of its own stress law;
+- viscoplastic: equals the MR force when Fp = I, one-step rate known answer from the shear
+  closed form, plastic incompressibility, Maxwell relaxation time;
+- contacts: closed forms per node; integrator: the 3-term damped Stormer recurrence and the
+  terminal velocity f/xi; tip force: brute force against a numpy sum;
+- scatter: sequential vs OpenMP-atomic equality to rounding.
+"""
+
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle as O
+
+RNG = np.random.default_rng(12345)
+
+
+def one_tet(points4, tets=None):
+    P = np.asarray(points4, np.float64)
+    t = np.array([[0, 1, 2, 3]]) if tets is None else tets
+    ev, vol = O.edge_vectors_volumes(P, t)
+    return O.soa(P), O.soa(t, np.uint32), O.soa(ev), np.ascontiguousarray(vol)
+
+
+REF_TET = np.array([[0.1, -0.2, 0.05], [1.3, 0.1, -0.1], [0.2, 1.1, 0.3], [-0.1, 0.25, 1.2]])
+
+
+def forces_of(model, E, nu, ratio, X, x, tets=None):
+    """oracle nodal forces (N,3) for current positions x, reference X."""
+    _, t, ev, vol = one_tet(X, tets)
+    pts = O.soa(x)
+    f = np.zeros_like(pts)
+    O.tetra_force(model, E, nu, ratio, pts, t, ev, vol, f)
+    return f.reshape(3, -1).T
+
+
+def psi_mr(F, E, nu, ratio):
+    """compressible Mooney-Rivlin strain energy (isochoric invariants), written from C = F^T F."""
+    mu = E / (2 * (1 + nu))
+    mu1, mu2 = ratio * mu, (1 - ratio) * mu
+    kappa = E / (3 * (1 - 2 * nu))
+    J = np.linalg.det(F)
+    C = F.T @ F
+    I1 = np.trace(C)
+    I2 = 0.5 * (I1**2 - np.trace(C @ C))
+    return 0.5 * mu1 * (J ** (-2 / 3) * I1 - 3) + 0.5 * mu2 * (J ** (-4 / 3) * I2 - 3) + 0.5 * kappa * (J - 1) ** 2
+
+
+def psi_svk(F, E, nu):
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    Eg = 0.5 * (F.T @ F - np.eye(3))
+    return 0.5 * lam * np.trace(Eg) ** 2 + mu * np.trace(Eg @ Eg)
+
+
+def energy_forces(psi, X, x, h=1e-6):
+    X = np.asarray(X)
+    Dm = (X[1:] - X[0]).T
+    V0 = np.linalg.det(Dm) / 6
+    Dinv = np.linalg.inv(Dm)
+
+    def energy(xx):
+        Ds = (xx[1:] - xx[0]).T
+        return V0 * psi(Ds @ Dinv)
+
+    f = np.zeros_like(x)
+    for a in range(4):
+        for c in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[a, c] += h
+            xm[a, c] -= h
+            f[a, c] = -(energy(xp) - energy(xm)) / (2 * h)
+    return f
+
+
+def virial_stress(X, x, f):
+    """sigma = -(1/v) sum_a f_a x_a^T for the nodal forces of one homogeneously strained tetra."""
+    v = np.linalg.det((x[1:] - x[0]).T) / 6
+    return -sum(np.outer(f[a], x[a]) for a in range(4)) / v
+
+
+def deform(F, X, t=np.zeros(3)):
+    return X @ F.T + t
+
+
+def random_F(scale=0.15):
+    return np.eye(3) + scale * RNG.uniform(-1, 1, (3, 3))
+
+
+# --------------------------------------------------------------------------------------------
+# precompute (TetraCell.py:26-41)
+# --------------------------------------------------------------------------------------------
+
+
+def test_edge_vectors_layout_and_unit_volume():
+    P = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]])
+    ev, vol = O.edge_vectors_volumes(P, np.array([[0, 1, 2, 3]]))
+    assert np.allclose(ev[0], np.eye(3).ravel())
+    assert vol[0] == pytest.approx(1 / 6)
+    P2 = P[[0, 2, 1, 3]]  # swapped orientation: signed volume flips
+    assert O.edge_vectors_volumes(P2, np.array([[0, 1, 2, 3]]))[1][0] == pytest.approx(-1 / 6)
+    # SoA: element k of tetra t at k*T + t (Tetra.py:53-58)
+    T = 5
+    ev5 = np.arange(T * 9, dtype=np.float64).reshape(T, 9)
+    s = O.soa(ev5)
+    for t in range(T):
+        for k in range(9):
+            assert s[k * T + t] == ev5[t, k]
+
+
+def test_volume_sum_is_hull_volume():
+    from scipy.spatial import ConvexHull
+
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    P, T = generate_sphere_mesh(3.0, 0.6, seed=4, min_quality=0.0, morton=False)
+    _, vol = O.edge_vectors_volumes(P, T)
+    assert np.all(vol > 0)
+    assert vol.sum() == pytest.approx(ConvexHull(P).volume, rel=1e-10)
+
+
+# --------------------------------------------------------------------------------------------
+# hyperelastic forces (tetra.cl)
+# --------------------------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("ratio", [1.0, 0.75, 0.3])
+def test_mr_energy_consistency(ratio):
+    E, nu = 0.01, 0.45
+    for _ in range(5):
+        F = random_F(0.2)
+        x = deform(F, REF_TET, RNG.normal(size=3))
+        x += 0.05 * RNG.normal(size=x.shape)
+        f = forces_of(O.MR, E, nu, ratio, REF_TET, x)
+        fe = energy_forces(lambda G: psi_mr(G, E, nu, ratio), REF_TET, x)
+        assert np.max(np.abs(f - fe)) < 1e-7 * np.max(np.abs(fe))
+
+
+def test_svk_energy_consistency():
+    E, nu = 0.02, 0.3
+    for _ in range(5):
+        x = deform(random_F(0.2), REF_TET, RNG.normal(size=3))
+        f = forces_of(O.SVK, E, nu, 1.0, REF_TET, x)
+        fe = energy_forces(lambda G: psi_svk(G, E, nu), REF_TET, x)
+        assert np.max(np.abs(f - fe)) < 1e-7 * np.max(np.abs(fe))
+
+
+@pytest.mark.parametrize("model", [O.MR, O.SVK, O.SVKK])
+def test_rest_and_rigid_motion_give_zero_force(model):
+    E, nu = 0.01, 0.45
+    f = forces_of(model, E, nu, 0.8, REF_TET, REF_TET.copy())
+    assert np.max(np.abs(f)) < 1e-15
+    Q = scipy.linalg.expm(np.array([[0, -0.7, 0.2], [0.7, 0, -0.4], [-0.2, 0.4, 0]]))
+    x = deform(Q, REF_TET, np.array([3.0, -1.0, 2.0]))
+    f = forces_of(model, E, nu, 0.8, REF_TET, x)
+    assert np.max(np.abs(f)) < 1e-12
+
+
+@pytest.mark.parametrize("model", [O.MR, O.SVK])
+def test_momentum_balance(model):
+    x = deform(random_F(0.25), REF_TET) + 0.05 * RNG.normal(size=(4, 3))
+    f = forces_of(model, 0.01, 0.4, 0.6, REF_TET, x)
+    assert np.max(np.abs(f.sum(0))) < 1e-15
+    torque = sum(np.cross(x[a], f[a]) for a in range(4))
+    assert np.max(np.abs(torque)) < 1e-14
+
+
+def test_hydrostatic_closed_form():
+    E, nu = 0.01, 0.45
+    kappa = E / (3 * (1 - 2 * nu))
+    for ratio in (1.0, 0.5):
+        for s in (0.9, 1.1):
+            x = deform(s * np.eye(3), REF_TET)
+            sig = virial_stress(REF_TET, x, forces_of(O.MR, E, nu, ratio, REF_TET, x))
+            assert np.allclose(sig, kappa * (s**3 - 1) * np.eye(3), rtol=1e-10, atol=1e-15)
+
+
+def test_uniaxial_neo_hookean_closed_form():
+    E, nu = 0.01, 0.45
+    mu = E / (2 * (1 + nu))
+    kappa = E / (3 * (1 - 2 * nu))
+    lam = 1.2
+    x = deform(np.diag([lam, 1, 1]), REF_TET)
+    sig = virial_stress(REF_TET, x, forces_of(O.MR, E, nu, 1.0, REF_TET, x))
+    dev = mu * lam ** (-5 / 3) * (lam**2 - 1)
+    assert sig[0, 0] == pytest.approx(2 / 3 * dev + kappa * (lam - 1), rel=1e-10)
+    assert sig[1, 1] == pytest.approx(-1 / 3 * dev + kappa * (lam - 1), rel=1e-10)
+    assert sig[2, 2] == pytest.approx(sig[1, 1], rel=1e-12)
+    assert abs(sig[0, 1]) < 1e-15
+
+
+def test_simple_shear_mooney_rivlin_closed_form():
+    """Classic MR simple-shear results: sigma12 = (mu1+mu2) g, sigma11 - sigma22 = (mu1+mu2) g^2,
+    sigma22 - sigma33 = -mu2 g^2 (Rivlin)."""
+    E, nu, ratio = 0.01, 0.45, 0.6
+    mu = E / (2 * (1 + nu))
+    mu1, mu2 = ratio * mu, (1 - ratio) * mu
+    g = 0.3
+    x = deform(np.array([[1, g, 0], [0, 1, 0], [0, 0, 1.0]]), REF_TET)
+    sig = virial_stress(REF_TET, x, forces_of(O.MR, E, nu, ratio, REF_TET, x))
+    assert sig[0, 1] == pytest.approx((mu1 + mu2) * g, rel=1e-10)
+    assert sig[0, 0] - sig[1, 1] == pytest.approx((mu1 + mu2) * g**2, rel=1e-9)
+    assert sig[1, 1] - sig[2, 2] == pytest.approx(-mu2 * g**2, rel=1e-9)
+
+
+def test_svkk_polar_root_and_hydrostatic():
+    # approximateRoot(M) returns the symmetric polar factor P of M = U P (tetra.cl:5-12)
+    import ctypes
+
+    L = O.lib()
+    if not hasattr(L, "orc_polar_stretch"):
+        pytest.skip("oracle built without orc_polar_stretch")
+    for _ in range(5):
+        M = random_F(0.4)
+        out = np.zeros(9)
+        L.orc_polar_stretch(np.ascontiguousarray(M.ravel()).ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        assert np.allclose(out.reshape(3, 3), scipy.linalg.polar(M)[1], atol=1e-12)
+    # SVKK hydrostatic F = sI: S = s^2 I, rows of Force = -V0 mu (1 + (kappa(3s^2-3)-1)/s^2) s Dm^-1
+    E, nu, s = 0.01, 0.3, 1.1
+    mu = E / (2 * (1 + nu))
+    kappa = nu / (1 - 2 * nu)
+    x = deform(s * np.eye(3), REF_TET)
+    f = forces_of(O.SVKK, E, nu, 1.0, REF_TET, x)
+    Dm = (REF_TET[1:] - REF_TET[0]).T
+    V0 = np.linalg.det(Dm) / 6
+    rows = -V0 * mu * (1 + (kappa * (3 * s * s - 3) - 1) / (s * s)) * s * np.linalg.inv(Dm)
+    assert np.allclose(f[1:], rows, rtol=1e-10, atol=1e-16)
+
+
+def test_sequential_and_atomic_scatter_agree():
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    P, T = generate_sphere_mesh(4.0, 0.8, seed=7)
+    ev, vol = O.edge_vectors_volumes(P, T)
+    x = P + 0.02 * RNG.normal(size=P.shape)
+    fs, fp = np.zeros(3 * len(P)), np.zeros(3 * len(P))
+    args = (O.MR, 0.01, 0.45, 0.8, O.soa(x), O.soa(T, np.uint32), O.soa(ev), np.ascontiguousarray(vol))
+    O.tetra_force(*args, fs, nthreads=1)
+    O.tetra_force(*args, fp, nthreads=4)
+    assert np.allclose(fs, fp, rtol=1e-12, atol=1e-15)
+    # momentum balance over the whole mesh
+    assert np.max(np.abs(fs.reshape(3, -1).sum(1))) < 1e-13
+
+
+# --------------------------------------------------------------------------------------------
+# viscoplastic (InteractionFiniteStrainViscoplastic.cl restated with node-0 base)
+# --------------------------------------------------------------------------------------------
+
+
+def vp_params(E=0.01, nu=0.45, ratio=0.75, m=1.0, Y=0.002, rate=1e-3, eps0=1.0, n=0.0, sy=0.0):
+    mu = E / (2 * (1 + nu))
+    return O.VPParams(ratio * mu, (1 - ratio) * mu, E / (3 * (1 - 2 * nu)), m, Y, rate, eps0, n, sy)
+
+
+def vp_step(P, X, x, Fp=None, acc=None, tets=None):
+    _, t, ev, vol = one_tet(X, tets)
+    pts = O.soa(x)
+    f = np.zeros_like(pts)
+    acc = np.zeros(1) if acc is None else acc
+    Fp = O.soa(np.eye(3).reshape(1, 9)) if Fp is None else Fp
+    vm, pr = np.zeros(1), np.zeros(1)
+    O.viscoplastic(P, pts, t, ev, vol, f, acc, Fp, vm, pr)
+    return f.reshape(3, -1).T, acc, Fp, vm[0], pr[0]
+
+
+def test_vp_matches_mr_force_with_identity_Fp():
+    E, nu, ratio = 0.01, 0.45, 0.75
+    P = vp_params(E, nu, ratio, rate=0.0)
+    for _ in range(5):
+        x = deform(random_F(0.2), REF_TET) + 0.03 * RNG.normal(size=(4, 3))
+        f_vp, acc, Fp, _, _ = vp_step(P, REF_TET, x)
+        f_mr = forces_of(O.MR, E, nu, ratio, REF_TET, x)
+        assert np.allclose(f_vp, f_mr, rtol=1e-9, atol=1e-15)
+        assert acc[0] == 0 and np.allclose(Fp, O.soa(np.eye(3).reshape(1, 9)))
+
+
+def test_vp_rest_no_flow_and_orientation_independent():
+    P = vp_params()
+    f, acc, Fp, vm, pr = vp_step(P, REF_TET, REF_TET.copy())
+    assert np.max(np.abs(f)) < 1e-15 and acc[0] == 0 and vm == 0 and pr == 0
+    # swapped node order (negative orientation): the face-area force law is orientation free
+    x = deform(random_F(0.2), REF_TET)
+    f1 = vp_step(P, REF_TET, x)[0]
+    sw = np.array([[0, 2, 1, 3]])
+    Xs, xs = REF_TET, x
+    f2 = vp_step(P, Xs, xs, tets=sw)[0]
+    assert np.allclose(f1, f2, rtol=1e-10, atol=1e-16)
+
+
+def test_vp_one_step_rate_known_answer():
+    E, nu, ratio = 0.01, 0.45, 0.6
+    mu = E / (2 * (1 + nu))
+    mu1, mu2 = ratio * mu, (1 - ratio) * mu
+    g, m, Y, rate = 0.2, 2.0, 0.003, 1e-3
+    P = vp_params(E, nu, ratio, m=m, Y=Y, rate=rate)
+    x = deform(np.array([[1, g, 0], [0, 1, 0], [0, 0, 1.0]]), REF_TET)
+    _, acc, Fp, vm, pr = vp_step(P, REF_TET, x)
+    s12 = (mu1 + mu2) * g
+    d12, d23 = (mu1 + mu2) * g**2, -mu2 * g**2
+    d31 = -(d12 + d23)
+    svm = np.sqrt(0.5 * (d12**2 + d23**2 + d31**2) + 3 * s12**2)
+    assert vm == pytest.approx(svm, rel=1e-9)
+    assert acc[0] == pytest.approx(rate * (svm / Y) ** m, rel=1e-9)
+    Fpm = Fp.reshape(9, 1)[:, 0].reshape(3, 3)
+    assert abs(np.linalg.det(Fpm) - 1) < 1e-3 * acc[0]
+
+
+def test_vp_yield_offset_blocks_flow_below_threshold():
+    P = vp_params(sy=1.0)  # threshold far above any stress here
+    x = deform(np.array([[1, 0.2, 0], [0, 1, 0], [0, 0, 1.0]]), REF_TET)
+    _, acc, _, vm, _ = vp_step(P, REF_TET, x)
+    assert vm > 0 and acc[0] == 0
+
+
+def test_vp_maxwell_relaxation_time():
+    """m = 1, n = 0, held small shear: sigma12 decays like exp(-s/tau), tau = Y/(3 G eps0dot)."""
+    E, nu, ratio = 0.01, 0.45, 1.0
+    G = E / (2 * (1 + nu))
+    Y, rate = 0.01, 2e-4
+    P = vp_params(E, nu, ratio, m=1.0, Y=Y, rate=rate)
+    x = deform(np.array([[1, 0.01, 0], [0, 1, 0], [0, 0, 1.0]]), REF_TET)
+    acc = np.zeros(1)
+    Fp = O.soa(np.eye(3).reshape(1, 9))
+    vms = []
+    for _ in range(400):
+        _, acc, Fp, vm, _ = vp_step(P, REF_TET, x, Fp, acc)
+        vms.append(vm)
+    tau = Y / (3 * G * rate)
+    measured = -399 / np.log(vms[-1] / vms[0])
+    assert measured == pytest.approx(tau, rel=0.02)
+
+
+# --------------------------------------------------------------------------------------------
+# contacts, integrator, tip force
+# --------------------------------------------------------------------------------------------
+
+
+def test_contacts_closed_form():
+    k, R, yc = 0.1, 2.0, 5.0
+    x = np.array([[0.0, 4.0, 0.0], [0.3, 10.0, 0.0], [1.0, 5.0, 1.0], [0.0, -3.0, 0.0]])
+    pts = O.soa(x)
+    f = np.zeros_like(pts)
+    O.sphere(k, f, pts, R, yc)
+    f = f.reshape(3, -1).T
+    assert np.allclose(f[0], [0, -k * 1.0, 0])  # 1 deep, pushed away from the tip centre (down)
+    assert np.allclose(f[1], 0)  # outside
+    d = np.sqrt(2.0)
+    assert np.allclose(f[2], k * (R - d) * np.array([1, 0, 1]) / d)
+    f = np.zeros_like(pts)
+    O.substrate(k, f, pts, -2.0)
+    assert np.allclose(f.reshape(3, -1).T[:, 1], [0, 0, 0, k * 1.0])
+    f = np.zeros_like(pts)
+    O.plane(k, f, pts, 4.5)
+    assert np.allclose(f.reshape(3, -1).T[:, 1], [0, -k * 5.5, -k * 0.5, 0])
+
+
+def test_velocity_verlet_recurrence_and_terminal_velocity():
+    xi = 0.005
+    a, b = 1 - xi / 2, 1 + xi / 2
+    rho = a / b
+    # spring f = -k x on one node: x_{n+1} = (1+rho) x_n - rho x_{n-1} + f_n / b for n >= 1
+    kk = 0.3
+    x = np.array([1.0, 0.0, 0.0, 0.0, 0.0, 0.0])  # N = 2, SoA
+    v, f, fo = np.zeros(6), np.zeros(6), np.zeros(6)
+    xs = []
+    for _ in range(50):
+        f[:] = -kk * x
+        xs.append(x[0])
+        O.velocity_verlet(x, v, f, fo)
+    xs = np.array(xs)
+    for n in range(1, 48):
+        assert xs[n + 1] == pytest.approx((1 + rho) * xs[n] - rho * xs[n - 1] - kk * xs[n] / b, abs=1e-14)
+    # constant force: velocity tends to f / xi
+    x, v, fo = np.zeros(6), np.zeros(6), np.zeros(6)
+    for _ in range(20000):
+        f = np.full(6, 1e-4)
+        O.velocity_verlet(x, v, f, fo)
+        assert np.all(f == 0)  # force buffer cleared (VelocityVerlet.cl:246-248)
+    assert v[0] == pytest.approx(1e-4 / xi, rel=1e-6)
+
+
+def test_fix_top_bottom():
+    N = 6
+    x = np.arange(3 * N, dtype=np.float64) + 1
+    v, f, fo = np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N)
+    O.velocity_verlet(x, v, f, fo, fix_top_bottom=True)
+    X = x.reshape(3, N).T
+    assert X[3, 0] == 0 and X[3, 2] == 0 and X[4, 0] == 0 and X[4, 2] == 0 and X[3, 1] != 0 and X[2, 0] != 0
+
+
+def test_tip_force_brute_force():
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    P, _ = generate_sphere_mesh(4.0, 0.8, seed=3)
+    R, k, yc = 1.5, 0.1, 4.5
+    fz, mx = O.tip_force(O.soa(P), 1, k, R, yc)
+    ref = O.force_si_sphere(P, yc, R, k, 1.0, 1.0)
+    assert fz == pytest.approx(ref, rel=1e-12)
+    assert fz > 0  # cell pushes the tip up
+    d = np.maximum(R - np.linalg.norm(P - [0, yc, 0], axis=1), 0).max()
+    assert mx == pytest.approx(d)
+
+
+def test_run_momentum_decay_without_contacts():
+    """no external force: total momentum obeys sum v_{n+1} = rho * sum v_n."""
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    P, T = generate_sphere_mesh(3.0, 0.75, seed=11)
+    prm = O.RunParams()
+    prm.model, prm.E, prm.nu, prm.ratio, prm.nthreads = O.MR, 0.01, 0.45, 1.0, 1
+    st = O.OracleState(P * 2.0, T, prm)  # reference from unscaled P: prestressed start
+    st.ev[:] = O.soa(O.edge_vectors_volumes(P * 1.9, T)[0])
+    st.vol[:] = O.edge_vectors_volumes(P * 1.9, T)[1]
+    st.vel[:] = 1e-3
+    p_prev = st.vel.reshape(3, -1).sum(1)
+    st.run(1, record=False)
+    rho = (1 - 0.0025) / (1 + 0.0025)
+    p1 = st.vel.reshape(3, -1).sum(1)
+    assert np.allclose(p1, rho * p_prev, rtol=1e-9)
