@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit tetrahedral-FEM step on one MI355X.
+
+Workload (default): the large single cell -- R 10 um, lattice 0.2 um, seed 2 (~0.52M nodes,
+~3.3M tetra), Mooney-Rivlin + viscoplastic, substrate and spherical tip, FP32 state.  A step is
+one Simulation.timeStep of the whole cell (force pass + node pass); steps are replayed from a
+captured hipGraph.  Inputs are resident in HBM before the timed region starts.
+
+Printed JSON line (one):
+- value = tetra-updates/s = T * steps / wall seconds of the timed region;
+- hbm_gbs = algorithmic bytes per step / ms_per_step (132 T + 108 N for the viscoplastic step,
+  56 T + 108 N for the elastic one);
+- roofline: the dominant kernel (the force pass) -- algorithmic bytes per launch (132 T + 24 N)
+  over its HIP-event average duration, against 8 TB/s; ``traffic`` = measured HBM bytes per
+  launch from rocprofv3 FETCH_SIZE + WRITE_SIZE when ``--traffic-json`` names a file holding them;
+- cpu_baseline: the FP64 C oracle (OpenMP, all allowed host threads) on a bounded step sample
+  of the same cell;
+- plain_small: the small cell (R 8 um, 0.8 um lattice, ~25k tetra) fused step, for comparison.
+
+``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cached_mesh(cfg):
+    from nofluidx3d_amd import configs
+
+    m = cfg["MESH"]
+    path = f"/tmp/nfx_mesh_{m['radius_um']}_{m['spacing_um']}_{m['seed']}.npz"
+    if os.path.exists(path):
+        d = np.load(path)
+        return d["P"], d["T"]
+    t0 = time.time()
+    P, T = configs.make_mesh(cfg)
+    np.savez(path, P=P, T=T)
+    log(f"mesh {m}: {len(P)} nodes {len(T)} tetra in {time.time() - t0:.1f}s")
+    return P, T
+
+
+def make_sim(cfg, P, T, scatter, material):
+    from nofluidx3d_amd.Simulation import Simulation
+    from nofluidx3d_amd.util import vtk
+
+    sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False)
+    sim.setInteractionSubstrate()
+    sim.setInteractionSphere()
+    if material == "viscoplastic":
+        sim.setInteractionFiniteStrainViscoplastic()
+    else:
+        sim.setInteractionTetra()
+    sim.setInteractionVelocityVerlet()
+    return sim
+
+
+def sync(sim):
+    import torch
+
+    from nofluidx3d_amd import _lib, openCL
+
+    _lib.call("nfx_stream_sync", openCL.handle())
+    torch.cuda.synchronize()
+
+
+def time_steps(sim, warmup, steps):
+    sim.advance(100)  # capture and instantiate the 100-step graph outside the timed region
+    sim.advance(warmup)
+    sync(sim)
+    t0 = time.perf_counter()
+    sim.advance(steps)
+    sync(sim)
+    return (time.perf_counter() - t0) / steps
+
+
+def pass_times(sim, nrep):
+    import ctypes
+
+    from nofluidx3d_amd import _lib, openCL
+
+    mf, mn = ctypes.c_double(), ctypes.c_double()
+    _lib.call("nfx_time_passes", openCL.handle(), sim.plan(), sim.time, nrep, ctypes.byref(mf), ctypes.byref(mn))
+    sim.time += nrep
+    sync(sim)
+    return mf.value, mn.value
+
+
+def cpu_baseline(P, T, cfg, material, target_s=12.0):
+    import oracle as O
+    from nofluidx3d_amd.TetraCell import TetraCell
+
+    c = TetraCell.from_arrays(P, T, radius=cfg["CELL"]["RadiusSIM"])
+    u = O.units(cfg["CELL"]["RadiusSI"], cfg["CELL"]["RadiusSIM"], cfg["CELL"]["YoungsModulusSI"], cfg["RhoSI"])
+    prm = O.RunParams()
+    E, nu = 0.01, cfg["CELL"]["PoissonRatio"]
+    ratio = cfg["CELL"].get("MooneyRivlinRatio", 1.0)
+    mu = E / (2 * (1 + nu))
+    if material == "viscoplastic":
+        prm.model = O.VISCOPLASTIC
+        cc = cfg["CELL"]
+        prm.vp = O.VPParams(ratio * mu, (1 - ratio) * mu, E / (3 * (1 - 2 * nu)), 1.0, 3 * cc["ViscositySI"] / u["p0"],
+                            u["T0"], 1.0, 0.0, cc["YieldStressSI"] / u["p0"])
+    else:
+        prm.model, prm.E, prm.nu, prm.ratio = O.MR, E, nu, ratio
+    prm.substrate, prm.substrate_k = 1, 0.1
+    prm.wall = -(cfg["CELL"]["RadiusSIM"] + cfg["SubstrateDistance"])
+    R = cfg["SphereRadiusSI"] / u["L0"]
+    prm.indenter, prm.ind_k, prm.ind_radius = 1, 0.1, R
+    prm.ind_start = cfg["CELL"]["RadiusSIM"] + R + cfg["InitialDistance"]
+    prm.ind_speed, prm.ind_move_steps = cfg["VelocitySI"] / u["V0"], cfg["MoveTimeSI"] / u["T0"]
+    threads = O.max_threads()
+    prm.nthreads = threads
+    st = O.OracleState(c.points, c.tetras, prm)
+    t0 = time.perf_counter()
+    st.run(1, record=False)
+    t1 = time.perf_counter() - t0
+    n = int(max(1, min(200, target_s / max(t1, 1e-6))))
+    t0 = time.perf_counter()
+    st.run(n, record=False)
+    dt = time.perf_counter() - t0
+    rate = len(T) * n / dt
+    return {
+        "value": rate,
+        "unit": "tetra-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} FP64 steps of the same cell ({len(T)} tetra, {material}) with the C oracle, "
+                  f"OpenMP {threads} threads, atomic scatter, {dt:.1f}s",
+        "cpu_model": _cpu_model(),
+        "nproc": os.cpu_count(),
+        "hbm_equiv_gbs": None,
+    }
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="large", choices=["large", "small"])
+    ap.add_argument("--material", default=None, choices=[None, "viscoplastic", "elastic"])
+    ap.add_argument("--scatter", default="tiled", choices=["tiled", "csr", "atomic"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--traffic-json", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only "
+              f"in this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from nofluidx3d_amd import configs
+
+    assert torch.cuda.is_available(), "no HIP device"
+    if args.workload == "large":
+        cfg = configs.large_cell()
+        material = args.material or "viscoplastic"
+    else:
+        cfg = configs.small_cell()
+        material = args.material or "elastic"
+    P, T = cached_mesh(cfg)
+    N, nT = len(P), len(T)
+    sim = make_sim(cfg, P, T, args.scatter, material)
+    assert sim.plan() is not None
+    ms = time_steps(sim, args.warmup, args.steps) * 1e3
+    ms_force, ms_node = pass_times(sim, min(args.steps, 100))
+    vp = material == "viscoplastic"
+    force_bytes = (132 if vp else 56) * nT + 24 * N
+    step_bytes = (132 if vp else 56) * nT + 108 * N
+    achieved = force_bytes / (ms_force * 1e-3) / 1e9
+    traffic = None
+    if args.traffic_json and os.path.exists(args.traffic_json):
+        traffic = json.load(open(args.traffic_json)).get("force_pass_bytes")
+    out = {
+        "metric": "tetra-updates/s",
+        "value": nT * 1e3 / ms,
+        "unit": "tetra-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}_cell: R {cfg['MESH']['radius_um']} um, lattice {cfg['MESH']['spacing_um']} "
+                        f"um, seed {cfg['MESH']['seed']}, {N} nodes, {nT} tetra, "
+                        f"{'Mooney-Rivlin + viscoplastic' if vp else cfg['CELL']['model']}, substrate + sphere tip, "
+                        f"fused step, {args.scatter} scatter, hipGraph replay",
+            "nodes": N,
+            "tetra": nT,
+            "scatter": args.scatter,
+            "parallelism": "single GPU",
+        },
+        "hbm_gbs": step_bytes / (ms * 1e-3) / 1e9,
+        "hbm_frac": step_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "ms_force_pass": ms_force,
+        "ms_node_pass": ms_node,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "force pass (k_vp / k_tetra)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": force_bytes,
+        },
+    }
+    del sim
+    torch.cuda.empty_cache()
+    if not args.no_extra:
+        out["other_scatter"] = []
+        for other in ("tiled", "csr", "atomic"):
+            if other == args.scatter:
+                continue
+            s2 = make_sim(cfg, P, T, other, material)
+            ms2 = time_steps(s2, args.warmup, min(args.steps, 200)) * 1e3
+            f2, n2 = pass_times(s2, 50)
+            out["other_scatter"].append({"scatter": other, "ms_per_step": ms2, "value": nT * 1e3 / ms2,
+                                         "ms_force_pass": f2, "ms_node_pass": n2})
+            del s2
+        small = configs.small_cell()
+        Ps, Ts = cached_mesh(small)
+        s3 = make_sim(small, Ps, Ts, "tiled", "elastic")
+        ms3 = time_steps(s3, 200, 2000) * 1e3
+        out["plain_small"] = {"workload": f"small_cell {len(Ps)} nodes {len(Ts)} tetra NH", "ms_per_step": ms3,
+                              "value": len(Ts) * 1e3 / ms3, "unit": "tetra-updates/s"}
+        del s3
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(P, T, cfg, material)
+        out["cpu_baseline"]["hbm_equiv_gbs"] = out["cpu_baseline"]["value"] / nT * step_bytes / 1e9
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

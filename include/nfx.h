@@ -45,6 +45,7 @@ extern "C" {
 
 #define NFX_SCATTER_ATOMIC 0 /* float atomics, as the reference's atomicAdd (atomicAdd.cl:1-11) */
 #define NFX_SCATTER_CSR 1    /* deterministic node<-corner gather, fixed summation order */
+#define NFX_SCATTER_TILED 2  /* deterministic two-level gather: per-tile LDS sums, then per-node sums */
 
 #define NFX_INDENTER_NONE 0
 #define NFX_INDENTER_SPHERE 1 /* Sphere.cl */
@@ -91,6 +92,15 @@ typedef struct {
     const uint32_t* csr_offsets; /* N+1, incidences of node n at [off[n], off[n+1]) */
     const uint32_t* csr_slots;   /* 4T, slot = 4*t + corner, ascending per node */
     float* corners;              /* 12T scratch: force of corner a of tetra t at 12*t + 3*a + c */
+    int32_t tile_k;              /* NFX_SCATTER_TILED: tetra per tile (256, 512 or 1024) */
+    int32_t tile_max_nodes;      /* largest number of distinct nodes in one tile */
+    const uint32_t* tile_off;    /* ntiles+1: tile c owns tile-nodes [tile_off[c], tile_off[c+1]) */
+    const uint32_t* tile_nodes;  /* M: global node id of each tile-node, ascending within a tile */
+    const uint16_t* tile_lend;   /* M: end of the tile-node's slot run in the tile's slot list */
+    const uint16_t* tile_slots;  /* 4*tile_k*ntiles: local slots 4*(t-t0)+a grouped by tile-node */
+    const uint32_t* node_off;    /* N+1: node n's tile-nodes at node_parts[node_off[n] .. node_off[n+1]) */
+    const uint32_t* node_parts;  /* M: tile-node indices, ascending */
+    float* partials;             /* 3M scratch: per-tile-node force sums */
     int32_t substrate;
     double substrate_k, wall;
     int32_t indenter; /* NFX_INDENTER_* */
@@ -129,6 +139,11 @@ int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float
 /* deterministic gather: force[c*N+n] += sum over slots of node n, in slot order */
 int nfx_gather_corners(nfx_ctx* ctx, const uint32_t* csr_offsets, const uint32_t* csr_slots, const float* corners,
                        int64_t N, float* force);
+/* host-side tiling for NFX_SCATTER_TILED (host pointers).  Call once with tile_nodes == NULL to get
+ * *m_total (the number of tile-nodes) and *max_nodes, then with arrays sized from it. */
+int nfx_build_tiles_host(const uint32_t* tetras_host, int64_t N, int64_t T, int32_t tile_k, uint32_t* tile_off,
+                         uint32_t* tile_nodes, uint16_t* tile_lend, uint16_t* tile_slots, uint32_t* node_off,
+                         uint32_t* node_parts, int64_t* m_total, int32_t* max_nodes);
 /* host-side CSR build (host pointers): offsets N+1, slots 4T ascending per node */
 int nfx_build_node_csr_host(const uint32_t* tetras_host, int64_t N, int64_t T, uint32_t* offsets_host,
                             uint32_t* slots_host);

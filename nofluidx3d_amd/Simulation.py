@@ -11,9 +11,14 @@ fused device step ``nfx_run_steps`` (a force pass and one node pass per step, in
 computed on the device, replayed from a captured hipGraph of up to 100 steps).  Any other chain
 runs plugin by plugin, exactly like the reference's loop (``Simulation.py:232-237``).
 
-Extensions (keyword arguments, defaults = reference behaviour where one exists):
-``scatter`` "atomic" (float atomics, as the reference) or "csr" (deterministic gather);
-``fused`` and ``use_graph`` to switch the fused step and graph replay off.
+Extensions (keyword arguments):
+``scatter`` -- how the per-tetra corner forces reach the nodes:
+  "tiled" (default) deterministic two-level gather (per-tile sums in LDS, then per-node sums);
+  "csr" deterministic gather of per-tetra corner records through the node CSR;
+  "atomic" float atomics in arbitrary order, as the reference's atomicAdd.
+  The physics is the same; only the floating-point summation order differs.
+``tile_k`` tetra per tile for "tiled" (256 or 512);
+``fused`` and ``use_graph`` switch the fused step and the graph replay off.
 """
 
 from __future__ import annotations
@@ -38,15 +43,16 @@ def _law(fn, start, speed, move_steps):
 
 
 class Simulation:
-    def __init__(self, con, scatter="atomic", fused=True, use_graph=True, mesh=None, verbose=True):
+    def __init__(self, con, scatter="tiled", fused=True, use_graph=True, mesh=None, verbose=True, tile_k=512):
         if isinstance(con, dict):
             self.parameters = dict(con)
         else:
             with open(con) as configfile:
                 self.parameters = json.load(configfile)
-        if scatter not in ("atomic", "csr"):
-            raise ValueError(f"scatter must be 'atomic' or 'csr', not {scatter!r}")
+        if scatter not in ("atomic", "csr", "tiled"):
+            raise ValueError(f"scatter must be 'tiled', 'csr' or 'atomic', not {scatter!r}")
         self.scatter = scatter
+        self.tile_k = tile_k
         self.fused = fused
         self.use_graph = use_graph
         self.verbose = verbose
@@ -104,6 +110,7 @@ class Simulation:
         self._plan = None
         self._plan_key = None
         self._csr = None
+        self._tiles = None
         self._tip = None
 
     def _print(self, *a, **k):
@@ -243,6 +250,48 @@ class Simulation:
             torch.cuda.synchronize(dev)
         return self._csr
 
+    def node_tiles(self):
+        """Tiled-gather arrays (see nfx_build_tiles_host) on the device."""
+        if self._tiles is None:
+            import ctypes
+
+            import torch
+
+            tets = np.ascontiguousarray(self.tetras.arrayC, dtype=np.uint32)
+            K = int(self.tile_k)
+            ntiles = (self.numTetra + K - 1) // K
+            tile_off = np.zeros(ntiles + 1, dtype=np.uint32)
+            m, mx = ctypes.c_int64(), ctypes.c_int32()
+            vp = ctypes.c_void_p
+            _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), self.numPoints, self.numTetra, K,
+                      tile_off.ctypes.data_as(vp), None, None, None, None, None, ctypes.byref(m), ctypes.byref(mx))
+            M = m.value
+            tile_nodes = np.zeros(M, np.uint32)
+            tile_lend = np.zeros(M, np.uint16)
+            tile_slots = np.zeros(4 * K * ntiles, np.uint16)
+            node_off = np.zeros(self.numPoints + 1, np.uint32)
+            node_parts = np.zeros(M, np.uint32)
+            _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), self.numPoints, self.numTetra, K,
+                      tile_off.ctypes.data_as(vp), tile_nodes.ctypes.data_as(vp), tile_lend.ctypes.data_as(vp),
+                      tile_slots.ctypes.data_as(vp), node_off.ctypes.data_as(vp), node_parts.ctypes.data_as(vp),
+                      ctypes.byref(m), ctypes.byref(mx))
+            dev = self.points.buf.device
+            to = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a.view(np.int16)).to(dev)
+            self._tiles = {
+                "tile_k": K,
+                "max_nodes": mx.value,
+                "m_total": M,
+                "tile_off": to(tile_off),
+                "tile_nodes": to(tile_nodes),
+                "tile_lend": to(tile_lend),
+                "tile_slots": to(tile_slots),
+                "node_off": to(node_off),
+                "node_parts": to(node_parts),
+                "partials": torch.zeros(3 * M, dtype=torch.float32, device=dev),
+            }
+            torch.cuda.synchronize(dev)
+        return self._tiles
+
     def plan(self):
         """The fused-step description of the registered plugin chain, or None if it cannot fuse."""
         if not self.fused:
@@ -263,6 +312,12 @@ class Simulation:
             off, slots, corners = self.node_csr()
             p.scatter = _lib.SCATTER_CSR
             p.csr_offsets, p.csr_slots, p.corners = off.data_ptr(), slots.data_ptr(), corners.data_ptr()
+        elif self.scatter == "tiled":
+            t = self.node_tiles()
+            p.scatter = _lib.SCATTER_TILED
+            p.tile_k, p.tile_max_nodes = t["tile_k"], t["max_nodes"]
+            for k in ("tile_off", "tile_nodes", "tile_lend", "tile_slots", "node_off", "node_parts", "partials"):
+                setattr(p, k, t[k].data_ptr())
         else:
             p.scatter = _lib.SCATTER_ATOMIC
         self._plan, self._plan_key = p, key

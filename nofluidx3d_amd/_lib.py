@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnfx.so")
 
 NFX_SVKK, NFX_SVK, NFX_MR, NFX_VISCOPLASTIC = 0, 1, 2, 3
-SCATTER_ATOMIC, SCATTER_CSR = 0, 1
+SCATTER_ATOMIC, SCATTER_CSR, SCATTER_TILED = 0, 1, 2
 INDENTER_NONE, INDENTER_SPHERE, INDENTER_PLANE = 0, 1, 2
 
 c_i32, c_i64, c_f64, c_vp = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
@@ -65,6 +65,15 @@ class Plan(ctypes.Structure):
         ("csr_offsets", c_vp),
         ("csr_slots", c_vp),
         ("corners", c_vp),
+        ("tile_k", c_i32),
+        ("tile_max_nodes", c_i32),
+        ("tile_off", c_vp),
+        ("tile_nodes", c_vp),
+        ("tile_lend", c_vp),
+        ("tile_slots", c_vp),
+        ("node_off", c_vp),
+        ("node_parts", c_vp),
+        ("partials", c_vp),
         ("substrate", c_i32),
         ("substrate_k", c_f64),
         ("wall", c_f64),
@@ -94,6 +103,8 @@ SIGNATURES = {
                                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_build_node_csr_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp]),
+    "nfx_build_tiles_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                            ctypes.POINTER(c_i64), ctypes.POINTER(c_i32)]),
     "nfx_sphere_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64, c_f64]),
     "nfx_substrate_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
     "nfx_plane_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),

@@ -191,6 +191,125 @@ __global__ __launch_bounds__(kBlock) void k_vp(TetArgs a, VPConst P, float* __re
     }
 }
 
+// ---- force pass, tiled deterministic scatter ---------------------------------------------------
+// One workgroup per tile of K consecutive (Morton-ordered) tetra.  The tile's distinct nodes are
+// listed on the host (tile_nodes, ascending) with, for each, the run of the tile's corner slots
+// that touch it (tile_slots, grouped by node; tile_lend = run ends).  Phases:
+//   1  slot list and node positions -> LDS
+//   2  invert the slot runs into per-corner local node ids (conn)
+//   3  one thread per tetra: material law on LDS positions, corner forces -> LDS
+//   4  one thread per tile-node: sum its slot run in list order -> partials (global)
+// The node pass then sums each node's partials in tile order: no atomics, fixed summation order.
+struct TileArgs {
+    int64_t N, T;
+    const float* __restrict__ dminv;
+    const float* __restrict__ v0;
+    const float* __restrict__ x;
+    const uint32_t* __restrict__ tile_off;
+    const uint32_t* __restrict__ tile_nodes;
+    const uint16_t* __restrict__ tile_lend;
+    const uint16_t* __restrict__ tile_slots;
+    float* __restrict__ partials;
+    int32_t max_nodes;
+};
+
+template <int K>
+__host__ __device__ constexpr size_t tile_lds_bytes_fixed() {
+    return 2 * 4 * K + 2 * 4 * K + 4 * 12 * K;
+}
+
+template <int MODEL, int K, bool STRESS>
+__global__ __launch_bounds__(kBlock) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Fp,
+                                                       float* __restrict__ eps, float* __restrict__ vm,
+                                                       float* __restrict__ pr) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint16_t* s_slots = reinterpret_cast<uint16_t*>(smem);
+    uint16_t* s_conn = s_slots + 4 * K;
+    float* s_cf = reinterpret_cast<float*>(s_conn + 4 * K);
+    float* s_x = s_cf + 12 * K;
+    const int MX = a.max_nodes;
+
+    const int64_t c = xcd_block(blockIdx.x, gridDim.x);
+    const int64_t T = a.T, N = a.N;
+    const int64_t t0 = c * K;
+    const int nt = (int)((T - t0) < K ? (T - t0) : K);
+    const uint32_t j0 = a.tile_off[c];
+    const int M = (int)(a.tile_off[c + 1] - j0);
+
+    for (int i = threadIdx.x; i < 4 * nt; i += kBlock) s_slots[i] = a.tile_slots[4 * K * c + i];
+    for (int i = threadIdx.x; i < M; i += kBlock) {
+        const uint32_t n = a.tile_nodes[j0 + i];
+        s_x[i] = a.x[n];
+        s_x[MX + i] = a.x[N + n];
+        s_x[2 * MX + i] = a.x[2 * N + n];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < M; i += kBlock) {
+        const int b = i ? a.tile_lend[j0 + i - 1] : 0;
+        const int en = a.tile_lend[j0 + i];
+        for (int q = b; q < en; ++q) {
+            const int sl = s_slots[q];
+            s_conn[(sl & 3) * K + (sl >> 2)] = (uint16_t)i;
+        }
+    }
+    __syncthreads();
+    for (int tl = threadIdx.x; tl < nt; tl += kBlock) {
+        const int64_t t = t0 + tl;
+        const int l0 = s_conn[tl], l1 = s_conn[K + tl], l2 = s_conn[2 * K + tl], l3 = s_conn[3 * K + tl];
+        const V3 p0 = mk(s_x[l0], s_x[MX + l0], s_x[2 * MX + l0]);
+        const V3 p1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
+        const V3 p2 = mk(s_x[l2], s_x[MX + l2], s_x[2 * MX + l2]);
+        const V3 p3 = mk(s_x[l3], s_x[MX + l3], s_x[2 * MX + l3]);
+        float D[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) D[k] = a.dminv[k * T + t];
+        Corners f;
+        if constexpr (MODEL == NFX_VISCOPLASTIC) {
+            float fp[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) fp[k] = Fp[k * T + t];
+            const VPOut o = vp_tetra(p0, p1, p2, p3, D, fp, eps[t], P);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) Fp[k * T + t] = o.Fp[k];
+            eps[t] = o.epsbar;
+            if constexpr (STRESS) {
+                vm[t] = o.von_mises;
+                pr[t] = o.pressure;
+            }
+            f = o.f;
+        } else {
+            const float v0 = a.v0[t];
+            if constexpr (MODEL == NFX_MR)
+                f = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
+            else if constexpr (MODEL == NFX_SVK)
+                f = svk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+            else
+                f = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+        }
+        s_cf[0 * K + tl] = f.f0.x; s_cf[1 * K + tl] = f.f0.y; s_cf[2 * K + tl] = f.f0.z;
+        s_cf[3 * K + tl] = f.f1.x; s_cf[4 * K + tl] = f.f1.y; s_cf[5 * K + tl] = f.f1.z;
+        s_cf[6 * K + tl] = f.f2.x; s_cf[7 * K + tl] = f.f2.y; s_cf[8 * K + tl] = f.f2.z;
+        s_cf[9 * K + tl] = f.f3.x; s_cf[10 * K + tl] = f.f3.y; s_cf[11 * K + tl] = f.f3.z;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < M; i += kBlock) {
+        const int b = i ? a.tile_lend[j0 + i - 1] : 0;
+        const int en = a.tile_lend[j0 + i];
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        for (int q = b; q < en; ++q) {
+            const int sl = s_slots[q];
+            const int base = 3 * (sl & 3) * K + (sl >> 2);
+            fx += s_cf[base];
+            fy += s_cf[base + K];
+            fz += s_cf[base + 2 * K];
+        }
+        float* o = a.partials + 3 * (int64_t)(j0 + i);
+        o[0] = fx;
+        o[1] = fy;
+        o[2] = fz;
+    }
+}
+
 // ---- deterministic block reduction (sum, max) finished by the last block --------------------
 __device__ __forceinline__ void wave_sum_max(double& s, double& m) {
 #pragma unroll
@@ -260,6 +379,9 @@ struct NodeArgs {
     const uint32_t* __restrict__ off;
     const uint32_t* __restrict__ slots;
     const float* __restrict__ corners;
+    const uint32_t* __restrict__ node_off;
+    const uint32_t* __restrict__ node_parts;
+    const float* __restrict__ partials;
     float sub_k, wall;
     int32_t indenter;
     float ind_k, ind_radius;
@@ -285,10 +407,13 @@ __device__ __forceinline__ V3 gather_node(const NodeArgs& a, int64_t n, int sub)
     return f;
 }
 
-template <bool CSR, bool SUB, int IND, bool FIX, bool TIP>
+// GATHER: 0 = read the atomically accumulated force (and clear it), 1 = corner records via the
+// node CSR (4 lanes per node), 2 = tile partials via node_parts (one lane per node)
+template <int GATHER, bool SUB, int IND, bool FIX, bool TIP>
 __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step, const int64_t* step_dev, int32_t step_off,
                                                         double* partials, unsigned int* counter, double* tip_sum,
                                                         double* tip_max) {
+    constexpr bool CSR = GATHER == 1;
     constexpr int G = CSR ? 4 : 1;
     const int64_t time = step_dev ? (*step_dev + step_off) : step;
     const float pos = IND ? (float)indenter_pos(a.ind_start, a.ind_speed, a.ind_move, time) : 0.0f;
@@ -298,8 +423,15 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
     const int64_t stride = (int64_t)gridDim.x * (kBlock / G);
     for (int64_t n = ((int64_t)blockIdx.x * kBlock + threadIdx.x) / G; n < N; n += stride) {
         V3 f;
-        if constexpr (CSR) {
+        if constexpr (GATHER == 1) {
             f = gather_node<G>(a, n, sub);
+        } else if constexpr (GATHER == 2) {
+            f = mk(0.f, 0.f, 0.f);
+            const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
+            for (uint32_t j = b; j < e; ++j) {
+                const float* p = a.partials + 3 * (int64_t)a.node_parts[j];
+                f = f + mk(p[0], p[1], p[2]);
+            }
         } else {
             f = mk(a.force[n], a.force[N + n], a.force[2 * N + n]);
         }
@@ -334,7 +466,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
         a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
         a.v[n] = v.x; a.v[N + n] = v.y; a.v[2 * N + n] = v.z;
         a.fold[n] = f.x; a.fold[N + n] = f.y; a.fold[2 * N + n] = f.z;
-        if constexpr (!CSR) {
+        if constexpr (GATHER == 0) {
             a.force[n] = 0.0f; a.force[N + n] = 0.0f; a.force[2 * N + n] = 0.0f;
         }
     }
@@ -440,6 +572,9 @@ static NodeArgs node_args(const nfx_plan& p) {
     a.off = p.csr_offsets;
     a.slots = p.csr_slots;
     a.corners = p.corners;
+    a.node_off = p.node_off;
+    a.node_parts = p.node_parts;
+    a.partials = p.partials;
     a.sub_k = (float)p.substrate_k;
     a.wall = (float)p.wall;
     a.indenter = p.indenter;
@@ -459,8 +594,18 @@ static int validate_plan(const nfx_plan* p) {
     NFX_REQUIRE(p->model >= NFX_SVKK && p->model <= NFX_VISCOPLASTIC, "plan: unknown model %d", p->model);
     NFX_REQUIRE(p->model == NFX_VISCOPLASTIC || p->v0, "plan: NULL v0");
     NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || (p->Fp && p->epsbar), "plan: viscoplastic state missing");
-    NFX_REQUIRE(p->scatter == NFX_SCATTER_ATOMIC || p->scatter == NFX_SCATTER_CSR, "plan: unknown scatter %d",
+    NFX_REQUIRE(p->scatter >= NFX_SCATTER_ATOMIC && p->scatter <= NFX_SCATTER_TILED, "plan: unknown scatter %d",
                 p->scatter);
+    if (p->scatter == NFX_SCATTER_TILED) {
+        NFX_REQUIRE(p->tile_k == 256 || p->tile_k == 512 || p->tile_k == 1024, "plan: tile_k must be 256, 512 or 1024");
+        NFX_REQUIRE(p->tile_off && p->tile_nodes && p->tile_lend && p->tile_slots && p->node_off && p->node_parts &&
+                        p->partials,
+                    "plan: tiled scatter needs the tile arrays");
+        NFX_REQUIRE(p->tile_max_nodes > 0 && p->tile_max_nodes <= 4 * p->tile_k, "plan: bad tile_max_nodes %d",
+                    p->tile_max_nodes);
+        const size_t lds = (size_t)p->tile_k * 64 + 12 * (size_t)p->tile_max_nodes;
+        NFX_REQUIRE(lds <= 65536, "plan: tile needs %zu B of LDS (max 65536); use a smaller tile_k", lds);
+    }
     NFX_REQUIRE(p->scatter != NFX_SCATTER_ATOMIC || p->force, "plan: atomic scatter needs force");
     NFX_REQUIRE(p->scatter != NFX_SCATTER_CSR || (p->csr_offsets && p->csr_slots && p->corners),
                 "plan: CSR scatter needs offsets, slots and corners");
@@ -468,7 +613,36 @@ static int validate_plan(const nfx_plan* p) {
     return NFX_OK;
 }
 
+template <int K>
+static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
+    TileArgs a{p.N, p.T, p.dminv, p.v0, p.points, p.tile_off, p.tile_nodes, p.tile_lend, p.tile_slots, p.partials,
+               p.tile_max_nodes};
+    const unsigned int nb = (unsigned int)((p.T + K - 1) / K);
+    const size_t lds = tile_lds_bytes_fixed<K>() + 12 * (size_t)p.tile_max_nodes;
+    const ElasticDev e = elastic_consts(p.E, p.nu, p.ratio);
+    const VPConst c = vp_consts(p.vp);
+    hipStream_t s = ctx->stream;
+    switch (p.model) {
+        case NFX_MR: k_tile_force<NFX_MR, K, false><<<nb, kBlock, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_SVK: k_tile_force<NFX_SVK, K, false><<<nb, kBlock, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_SVKK: k_tile_force<NFX_SVKK, K, false><<<nb, kBlock, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        default:
+            if (st)
+                k_tile_force<NFX_VISCOPLASTIC, K, true><<<nb, kBlock, lds, s>>>(a, e, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
+            else
+                k_tile_force<NFX_VISCOPLASTIC, K, false><<<nb, kBlock, lds, s>>>(a, e, c, p.Fp, p.epsbar, nullptr, nullptr);
+    }
+}
+
 static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
+    if (p.scatter == NFX_SCATTER_TILED) {
+        const bool st = stress && p.von_mises && p.pressure;
+        if (p.tile_k == 256) launch_tiled_k<256>(ctx, p, st);
+        else if (p.tile_k == 512) launch_tiled_k<512>(ctx, p, st);
+        else launch_tiled_k<1024>(ctx, p, st);
+        NFX_CHECK_LAUNCH();
+        return NFX_OK;
+    }
     TetArgs a = tet_args(p);
     const unsigned int nb = nblocks(p.T);
     const bool csr = p.scatter == NFX_SCATTER_CSR;
@@ -495,10 +669,10 @@ static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
     return NFX_OK;
 }
 
-template <bool CSR, bool SUB, int IND, bool FIX>
+template <int CSR, bool SUB, int IND, bool FIX>
 static void launch_node_t(nfx_ctx* ctx, const NodeArgs& a, int64_t step, const int64_t* step_dev, int32_t off,
                           double* tip_sum, double* tip_max) {
-    constexpr int G = CSR ? 4 : 1;
+    constexpr int G = CSR == 1 ? 4 : 1;
     const bool tip = (tip_sum || tip_max) && IND != NFX_INDENTER_NONE;
     unsigned int nb = nblocks(a.N * G);
     if (tip) {
@@ -511,7 +685,7 @@ static void launch_node_t(nfx_ctx* ctx, const NodeArgs& a, int64_t step, const i
     }
 }
 
-template <bool CSR, bool SUB>
+template <int CSR, bool SUB>
 static void launch_node_ind(nfx_ctx* ctx, const nfx_plan& p, const NodeArgs& a, int64_t step, const int64_t* sd,
                             int32_t off, double* ts, double* tm) {
     const bool fix = p.fix_top_bottom != 0;
@@ -528,11 +702,15 @@ static void launch_node_ind(nfx_ctx* ctx, const nfx_plan& p, const NodeArgs& a, 
 static int launch_node_pass(nfx_ctx* ctx, const nfx_plan& p, int64_t step, const int64_t* step_dev, int32_t off,
                             double* tip_sum, double* tip_max) {
     const NodeArgs a = node_args(p);
-    const bool csr = p.scatter == NFX_SCATTER_CSR, sub = p.substrate != 0;
-    if (csr && sub) launch_node_ind<true, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max);
-    else if (csr) launch_node_ind<true, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max);
-    else if (sub) launch_node_ind<false, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max);
-    else launch_node_ind<false, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max);
+    const bool sub = p.substrate != 0;
+    switch (p.scatter * 2 + (sub ? 1 : 0)) {
+        case 0: launch_node_ind<0, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
+        case 1: launch_node_ind<0, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
+        case 2: launch_node_ind<1, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
+        case 3: launch_node_ind<1, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
+        case 4: launch_node_ind<2, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
+        default: launch_node_ind<2, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
+    }
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }
@@ -757,6 +935,62 @@ int nfx_build_node_csr_host(const uint32_t* tet, int64_t N, int64_t T, uint32_t*
             const uint32_t n = tet[a * T + t];
             slots[cnt[n]++] = (uint32_t)(4 * t + a);
         }
+    return NFX_OK;
+}
+
+int nfx_build_tiles_host(const uint32_t* tet, int64_t N, int64_t T, int32_t K, uint32_t* tile_off, uint32_t* tile_nodes,
+                         uint16_t* tile_lend, uint16_t* tile_slots, uint32_t* node_off, uint32_t* node_parts,
+                         int64_t* m_total, int32_t* max_nodes) {
+    NFX_REQUIRE(tet && tile_off && m_total && max_nodes, "NULL argument");
+    NFX_REQUIRE(K == 256 || K == 512 || K == 1024, "tile_k must be 256, 512 or 1024");
+    int rc = check_sizes(N, T);
+    if (rc) return rc;
+    const bool fill = tile_nodes != nullptr;
+    NFX_REQUIRE(!fill || (tile_lend && tile_slots && node_off && node_parts), "NULL output array");
+    const int64_t ntiles = (T + K - 1) / K;
+    std::vector<uint64_t> key((size_t)4 * K);
+    uint64_t total = 0;
+    int32_t mx = 0;
+    tile_off[0] = 0;
+    for (int64_t c = 0; c < ntiles; ++c) {
+        const int64_t t0 = c * K;
+        const int nt = (int)std::min<int64_t>(K, T - t0);
+        for (int tl = 0; tl < nt; ++tl)
+            for (int a = 0; a < 4; ++a) {
+                const uint32_t n = tet[(int64_t)a * T + t0 + tl];
+                NFX_REQUIRE(n < (uint64_t)N, "tetra %lld references node %u >= N", (long long)(t0 + tl), n);
+                key[(size_t)(4 * tl + a)] = ((uint64_t)n << 12) | (uint64_t)(4 * tl + a);
+            }
+        std::sort(key.begin(), key.begin() + 4 * nt);
+        int M = 0;
+        for (int q = 0; q < 4 * nt; ++q) {
+            const uint32_t n = (uint32_t)(key[(size_t)q] >> 12);
+            const bool first = q == 0 || (uint32_t)(key[(size_t)q - 1] >> 12) != n;
+            if (first) {
+                if (fill) tile_nodes[total + M] = n;
+                ++M;
+            }
+            if (fill) {
+                tile_slots[(size_t)4 * K * c + q] = (uint16_t)(key[(size_t)q] & 0xFFF);
+                tile_lend[total + M - 1] = (uint16_t)(q + 1);
+            }
+        }
+        if (fill)
+            for (int q = 4 * nt; q < 4 * K; ++q) tile_slots[(size_t)4 * K * c + q] = 0;
+        total += (uint64_t)M;
+        NFX_REQUIRE(total < (uint64_t(1) << 32), "too many tile-nodes");
+        tile_off[c + 1] = (uint32_t)total;
+        mx = std::max(mx, M);
+    }
+    *m_total = (int64_t)total;
+    *max_nodes = mx;
+    if (!fill) return NFX_OK;
+    NFX_REQUIRE(*m_total == (int64_t)tile_off[ntiles], "inconsistent tile count");
+    std::vector<uint32_t> cnt((size_t)N + 1, 0);
+    for (uint64_t j = 0; j < total; ++j) cnt[tile_nodes[j] + 1]++;
+    for (int64_t n = 0; n < N; ++n) cnt[n + 1] += cnt[n];
+    std::memcpy(node_off, cnt.data(), sizeof(uint32_t) * (size_t)(N + 1));
+    for (uint64_t j = 0; j < total; ++j) node_parts[cnt[tile_nodes[j]]++] = (uint32_t)j;
     return NFX_OK;
 }
 

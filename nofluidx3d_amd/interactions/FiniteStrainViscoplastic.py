@@ -70,7 +70,7 @@ class FiniteStrainViscoplastic(Interaction):
 
     def enqueue(self):
         sim = self.simulation
-        if getattr(sim, "scatter", "atomic") == "csr":
+        if getattr(sim, "scatter", "atomic") in ("csr", "tiled"):
             from nofluidx3d_amd import openCL
 
             off, slots, corners = sim.node_csr()

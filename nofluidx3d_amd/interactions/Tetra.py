@@ -6,8 +6,9 @@ FP32 inverse and volume.  The model flags are the reference's ``#define``s (``Te
 NEO_HOOKEAN and MOONEY_RIVLIN share one branch.
 
 Node scatter: ``simulation.scatter == "atomic"`` adds the corner forces with float atomics (the
-reference's ``atomicAdd``); ``"csr"`` writes one corner record per tetra and gathers them per node
-in a fixed order, which makes the force sum bitwise reproducible.
+reference's ``atomicAdd``); ``"csr"`` (and ``"tiled"``, whose two-level gather exists only in the
+fused step) writes one corner record per tetra and gathers them per node in a fixed order, which
+makes the force sum bitwise reproducible.
 """
 from enum import Enum, auto
 from pathlib import Path
@@ -86,7 +87,7 @@ class Tetra(Interaction):
 
     def enqueue(self):
         sim = self.simulation
-        if getattr(sim, "scatter", "atomic") == "csr":
+        if getattr(sim, "scatter", "atomic") in ("csr", "tiled"):
             off, slots, corners = sim.node_csr()
             v = self.knl.values
             mat = _lib.Elastic(self.model_code(), float(v["def_tetraYoungsModulus"]),

@@ -240,8 +240,10 @@ def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, 
         cfg["CELL"].update({"MooneyRivlinRatio": 0.75, "model": "MOONEY_RIVLIN", "ViscositySI": 10.0,
                             "YieldStressSI": 5.0})
     P, T = configs.make_mesh(cfg)
+    tile_k = 256 if scatter == "tiled256" else 512
+    scatter = "tiled" if scatter == "tiled256" else scatter
     sim = Simulation(cfg, scatter=scatter, fused=fused, use_graph=use_graph, mesh=vtk.from_tetra(P, T),
-                     verbose=False)
+                     verbose=False, tile_k=tile_k)
     sim.setInteractionSubstrate()
     if indenter == "sphere":
         sim.setInteractionSphere()
@@ -300,7 +302,7 @@ def assert_parity(sim, gx, gf, ox, of):
     return pos_err, f_err
 
 
-@pytest.mark.parametrize("scatter", ["atomic", "csr"])
+@pytest.mark.parametrize("scatter", ["atomic", "csr", "tiled", "tiled256"])
 @pytest.mark.parametrize("indenter", ["sphere", "plane"])
 def test_fused_run_matches_oracle(ctx, scatter, indenter):
     sim = build_sim(scatter=scatter, indenter=indenter)
@@ -310,8 +312,9 @@ def test_fused_run_matches_oracle(ctx, scatter, indenter):
     assert_parity(sim, gx, gf, ref.positions(), of)
 
 
-def test_viscoplastic_fused_run_matches_oracle(ctx):
-    sim = build_sim(material="VP", scatter="csr")
+@pytest.mark.parametrize("scatter", ["csr", "tiled"])
+def test_viscoplastic_fused_run_matches_oracle(ctx, scatter):
+    sim = build_sim(material="VP", scatter=scatter)
     gx, gf = run_gpu(sim, 600)
     ref = oracle_for(sim)
     of = ref.run(600)
@@ -319,12 +322,13 @@ def test_viscoplastic_fused_run_matches_oracle(ctx):
     assert_parity(sim, gx, gf, ref.positions(), of)
 
 
-def test_csr_is_bitwise_deterministic_and_graph_equals_eager(ctx):
-    a = build_sim(scatter="csr", use_graph=True)
+@pytest.mark.parametrize("scatter", ["csr", "tiled"])
+def test_deterministic_modes_are_bitwise_reproducible_and_graph_equals_eager(ctx, scatter):
+    a = build_sim(scatter=scatter, use_graph=True)
     xa, fa = run_gpu(a, 250)
-    b = build_sim(scatter="csr", use_graph=True)
+    b = build_sim(scatter=scatter, use_graph=True)
     xb, fb = run_gpu(b, 250)
-    c = build_sim(scatter="csr", use_graph=False)
+    c = build_sim(scatter=scatter, use_graph=False)
     xc, fc = run_gpu(c, 250)
     assert np.array_equal(xa, xb) and np.array_equal(fa, fb)
     assert np.array_equal(xa, xc) and np.array_equal(fa, fc)
@@ -370,7 +374,7 @@ def test_small_cell_full_run_matches_oracle(ctx):
 
     cfg = configs.small_cell(outdir="/tmp/nfx_c1/")
     P, T = configs.make_mesh(cfg)
-    sim = Simulation(cfg, scatter="csr", mesh=vtk.from_tetra(P, T), verbose=False)
+    sim = Simulation(cfg, scatter="tiled", mesh=vtk.from_tetra(P, T), verbose=False)
     sim.setInteractionSubstrate()
     sim.setInteractionSphere()
     sim.setInteractionTetra()

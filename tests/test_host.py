@@ -204,3 +204,47 @@ def test_config_units():
     assert mu * cc["MooneyRivlinRatio"] == pytest.approx(300.0)
     assert mu * (1 - cc["MooneyRivlinRatio"]) == pytest.approx(100.0)
     assert K == pytest.approx(5000.0)
+
+
+@pytest.mark.parametrize("K", [256, 512])
+def test_tile_builder_invariants(K):
+    from nofluidx3d_amd import _lib
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    P, T = generate_sphere_mesh(3.0, 0.5, seed=6)
+    N, nT = len(P), len(T)
+    tets = np.ascontiguousarray(T.reshape(-1, order="F").astype(np.uint32))
+    vp = ctypes.c_void_p
+    ntiles = (nT + K - 1) // K
+    tile_off = np.zeros(ntiles + 1, np.uint32)
+    m, mx = ctypes.c_int64(), ctypes.c_int32()
+    _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), N, nT, K, tile_off.ctypes.data_as(vp), None, None,
+              None, None, None, ctypes.byref(m), ctypes.byref(mx))
+    M = m.value
+    tn, tl, ts = np.zeros(M, np.uint32), np.zeros(M, np.uint16), np.zeros(4 * K * ntiles, np.uint16)
+    no, npart = np.zeros(N + 1, np.uint32), np.zeros(M, np.uint32)
+    _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), N, nT, K, tile_off.ctypes.data_as(vp),
+              tn.ctypes.data_as(vp), tl.ctypes.data_as(vp), ts.ctypes.data_as(vp), no.ctypes.data_as(vp),
+              npart.ctypes.data_as(vp), ctypes.byref(m), ctypes.byref(mx))
+    sizes = np.diff(tile_off.astype(np.int64))
+    assert mx.value == sizes.max() and tile_off[-1] == M
+    for c in range(ntiles):
+        t0, nt = c * K, min(K, nT - c * K)
+        sub = T[t0:t0 + nt]
+        nodes = tn[tile_off[c]:tile_off[c + 1]]
+        assert np.array_equal(nodes, np.unique(sub))  # ascending unique nodes of the tile
+        ends = tl[tile_off[c]:tile_off[c + 1]].astype(np.int64)
+        starts = np.concatenate([[0], ends[:-1]])
+        assert ends[-1] == 4 * nt
+        seen = []
+        for i, (b, e) in enumerate(zip(starts, ends)):
+            run = ts[4 * K * c + b:4 * K * c + e].astype(np.int64)
+            assert np.all(np.diff(run) > 0)  # ascending slots within the run
+            assert np.all(sub[run >> 2, run & 3] == nodes[i])
+            seen.extend(run)
+        assert sorted(seen) == list(range(4 * nt))
+    # node -> tile-node lists, ascending, and consistent with tile_nodes
+    for n in range(0, N, 5):
+        parts = npart[no[n]:no[n + 1]]
+        assert np.all(np.diff(parts.astype(np.int64)) > 0) and np.all(tn[parts] == n)
+    assert no[-1] == M
