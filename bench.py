@@ -53,11 +53,11 @@ def cached_mesh(cfg):
     return P, T
 
 
-def make_sim(cfg, P, T, scatter, material):
+def make_sim(cfg, P, T, scatter, material, tile_k=512):
     from nofluidx3d_amd.Simulation import Simulation
     from nofluidx3d_amd.util import vtk
 
-    sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False)
+    sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False, tile_k=tile_k)
     sim.setInteractionSubstrate()
     sim.setInteractionSphere()
     if material == "viscoplastic":
@@ -166,7 +166,10 @@ def main():
     ap.add_argument("--scatter", default="tiled", choices=["tiled", "csr", "atomic"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
-    ap.add_argument("--traffic-json", default=None)
+    ap.add_argument("--tile-k", type=int, default=512, choices=[256, 512])
+    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_large_cell.json"))
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="also run one device-to-device copy of BYTES (HBM-counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only "
@@ -186,8 +189,15 @@ def main():
         material = args.material or "elastic"
     P, T = cached_mesh(cfg)
     N, nT = len(P), len(T)
-    sim = make_sim(cfg, P, T, args.scatter, material)
+    sim = make_sim(cfg, P, T, args.scatter, material, args.tile_k)
     assert sim.plan() is not None
+    if args.calibrate_copy:
+        a = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     ms = time_steps(sim, args.warmup, args.steps) * 1e3
     ms_force, ms_node = pass_times(sim, min(args.steps, 100))
     vp = material == "viscoplastic"
@@ -196,7 +206,10 @@ def main():
     achieved = force_bytes / (ms_force * 1e-3) / 1e9
     traffic = None
     if args.traffic_json and os.path.exists(args.traffic_json):
-        traffic = json.load(open(args.traffic_json)).get("force_pass_bytes")
+        tj = json.load(open(args.traffic_json))
+        key = f"{material}_{args.scatter}_{args.tile_k}"
+        if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}):
+            traffic = tj["force_pass_bytes"][key]
     out = {
         "metric": "tetra-updates/s",
         "value": nT * 1e3 / ms,
@@ -218,6 +231,7 @@ def main():
             "nodes": N,
             "tetra": nT,
             "scatter": args.scatter,
+            "tile_k": args.tile_k,
             "parallelism": "single GPU",
         },
         "hbm_gbs": step_bytes / (ms * 1e-3) / 1e9,

@@ -92,15 +92,15 @@ typedef struct {
     const uint32_t* csr_offsets; /* N+1, incidences of node n at [off[n], off[n+1]) */
     const uint32_t* csr_slots;   /* 4T, slot = 4*t + corner, ascending per node */
     float* corners;              /* 12T scratch: force of corner a of tetra t at 12*t + 3*a + c */
-    int32_t tile_k;              /* NFX_SCATTER_TILED: tetra per tile (256, 512 or 1024) */
-    int32_t tile_max_nodes;      /* largest number of distinct nodes in one tile */
-    const uint32_t* tile_off;    /* ntiles+1: tile c owns tile-nodes [tile_off[c], tile_off[c+1]) */
-    const uint32_t* tile_nodes;  /* M: global node id of each tile-node, ascending within a tile */
-    const uint16_t* tile_lend;   /* M: end of the tile-node's slot run in the tile's slot list */
+    int32_t tile_k;              /* NFX_SCATTER_TILED: tetra per tile (256 or 512) */
+    int32_t tile_max_nodes;      /* MX: tile-node capacity per tile (largest distinct-node count) */
+    const uint32_t* tile_count;  /* ntiles: distinct nodes of tile c (<= MX) */
+    const uint32_t* tile_nodes;  /* ntiles*MX: node ids, ascending within a tile, padded */
+    const uint16_t* tile_lend;   /* ntiles*MX: end of each tile-node's run in the tile's slot list */
     const uint16_t* tile_slots;  /* 4*tile_k*ntiles: local slots 4*(t-t0)+a grouped by tile-node */
-    const uint32_t* node_off;    /* N+1: node n's tile-nodes at node_parts[node_off[n] .. node_off[n+1]) */
-    const uint32_t* node_parts;  /* M: tile-node indices, ascending */
-    float* partials;             /* 3M scratch: per-tile-node force sums */
+    const uint32_t* tile_dst;    /* ntiles*MX: position of each tile-node's partial in node order */
+    const uint32_t* node_off;    /* N+1: partials of node n at [node_off[n], node_off[n+1]) */
+    float* partials;             /* 3*M scratch, node order (tile order within a node) */
     int32_t substrate;
     double substrate_k, wall;
     int32_t indenter; /* NFX_INDENTER_* */
@@ -140,10 +140,11 @@ int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float
 int nfx_gather_corners(nfx_ctx* ctx, const uint32_t* csr_offsets, const uint32_t* csr_slots, const float* corners,
                        int64_t N, float* force);
 /* host-side tiling for NFX_SCATTER_TILED (host pointers).  Call once with tile_nodes == NULL to get
- * *m_total (the number of tile-nodes) and *max_nodes, then with arrays sized from it. */
-int nfx_build_tiles_host(const uint32_t* tetras_host, int64_t N, int64_t T, int32_t tile_k, uint32_t* tile_off,
-                         uint32_t* tile_nodes, uint16_t* tile_lend, uint16_t* tile_slots, uint32_t* node_off,
-                         uint32_t* node_parts, int64_t* m_total, int32_t* max_nodes);
+ * *max_nodes (MX), *m_total and tile_count, then again with arrays of ntiles*MX entries
+ * (tile_nodes, tile_lend, tile_dst), 4*tile_k*ntiles (tile_slots) and N+1 (node_off). */
+int nfx_build_tiles_host(const uint32_t* tetras_host, int64_t N, int64_t T, int32_t tile_k, int32_t* max_nodes,
+                         int64_t* m_total, uint32_t* tile_count, uint32_t* tile_nodes, uint16_t* tile_lend,
+                         uint16_t* tile_slots, uint32_t* tile_dst, uint32_t* node_off);
 /* host-side CSR build (host pointers): offsets N+1, slots 4T ascending per node */
 int nfx_build_node_csr_host(const uint32_t* tetras_host, int64_t N, int64_t T, uint32_t* offsets_host,
                             uint32_t* slots_host);

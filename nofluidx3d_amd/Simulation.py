@@ -260,33 +260,33 @@ class Simulation:
             tets = np.ascontiguousarray(self.tetras.arrayC, dtype=np.uint32)
             K = int(self.tile_k)
             ntiles = (self.numTetra + K - 1) // K
-            tile_off = np.zeros(ntiles + 1, dtype=np.uint32)
+            count = np.zeros(ntiles, dtype=np.uint32)
             m, mx = ctypes.c_int64(), ctypes.c_int32()
             vp = ctypes.c_void_p
             _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), self.numPoints, self.numTetra, K,
-                      tile_off.ctypes.data_as(vp), None, None, None, None, None, ctypes.byref(m), ctypes.byref(mx))
-            M = m.value
-            tile_nodes = np.zeros(M, np.uint32)
-            tile_lend = np.zeros(M, np.uint16)
+                      ctypes.byref(mx), ctypes.byref(m), count.ctypes.data_as(vp), None, None, None, None, None)
+            MX, M = mx.value, m.value
+            tile_nodes = np.zeros(ntiles * MX, np.uint32)
+            tile_lend = np.zeros(ntiles * MX, np.uint16)
             tile_slots = np.zeros(4 * K * ntiles, np.uint16)
+            tile_dst = np.zeros(ntiles * MX, np.uint32)
             node_off = np.zeros(self.numPoints + 1, np.uint32)
-            node_parts = np.zeros(M, np.uint32)
             _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), self.numPoints, self.numTetra, K,
-                      tile_off.ctypes.data_as(vp), tile_nodes.ctypes.data_as(vp), tile_lend.ctypes.data_as(vp),
-                      tile_slots.ctypes.data_as(vp), node_off.ctypes.data_as(vp), node_parts.ctypes.data_as(vp),
-                      ctypes.byref(m), ctypes.byref(mx))
+                      ctypes.byref(mx), ctypes.byref(m), count.ctypes.data_as(vp), tile_nodes.ctypes.data_as(vp),
+                      tile_lend.ctypes.data_as(vp), tile_slots.ctypes.data_as(vp), tile_dst.ctypes.data_as(vp),
+                      node_off.ctypes.data_as(vp))
             dev = self.points.buf.device
             to = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a.view(np.int16)).to(dev)
             self._tiles = {
                 "tile_k": K,
-                "max_nodes": mx.value,
+                "max_nodes": MX,
                 "m_total": M,
-                "tile_off": to(tile_off),
+                "tile_count": to(count),
                 "tile_nodes": to(tile_nodes),
                 "tile_lend": to(tile_lend),
                 "tile_slots": to(tile_slots),
+                "tile_dst": to(tile_dst),
                 "node_off": to(node_off),
-                "node_parts": to(node_parts),
                 "partials": torch.zeros(3 * M, dtype=torch.float32, device=dev),
             }
             torch.cuda.synchronize(dev)
@@ -312,11 +312,15 @@ class Simulation:
             off, slots, corners = self.node_csr()
             p.scatter = _lib.SCATTER_CSR
             p.csr_offsets, p.csr_slots, p.corners = off.data_ptr(), slots.data_ptr(), corners.data_ptr()
+        elif self.scatter == "tiled" and self.node_tiles()["max_nodes"] > self.node_tiles()["tile_k"]:
+            # a tile touching more distinct nodes than it has threads: use the CSR gather instead
+            self.scatter = "csr"
+            return self.plan()
         elif self.scatter == "tiled":
             t = self.node_tiles()
             p.scatter = _lib.SCATTER_TILED
             p.tile_k, p.tile_max_nodes = t["tile_k"], t["max_nodes"]
-            for k in ("tile_off", "tile_nodes", "tile_lend", "tile_slots", "node_off", "node_parts", "partials"):
+            for k in ("tile_count", "tile_nodes", "tile_lend", "tile_slots", "tile_dst", "node_off", "partials"):
                 setattr(p, k, t[k].data_ptr())
         else:
             p.scatter = _lib.SCATTER_ATOMIC

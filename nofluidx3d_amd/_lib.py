@@ -67,12 +67,12 @@ class Plan(ctypes.Structure):
         ("corners", c_vp),
         ("tile_k", c_i32),
         ("tile_max_nodes", c_i32),
-        ("tile_off", c_vp),
+        ("tile_count", c_vp),
         ("tile_nodes", c_vp),
         ("tile_lend", c_vp),
         ("tile_slots", c_vp),
+        ("tile_dst", c_vp),
         ("node_off", c_vp),
-        ("node_parts", c_vp),
         ("partials", c_vp),
         ("substrate", c_i32),
         ("substrate_k", c_f64),
@@ -103,8 +103,8 @@ SIGNATURES = {
                                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_build_node_csr_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp]),
-    "nfx_build_tiles_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
-                                            ctypes.POINTER(c_i64), ctypes.POINTER(c_i32)]),
+    "nfx_build_tiles_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
+                                            c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "nfx_sphere_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64, c_f64]),
     "nfx_substrate_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
     "nfx_plane_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),

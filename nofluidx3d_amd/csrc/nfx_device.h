@@ -16,6 +16,14 @@
 
 namespace nfx {
 
+// Hardware transcendental/division approximations (v_rcp/v_sqrt/v_exp/v_log, about 1 ulp, no
+// denormal rescaling).  Every operand they see here is a normal number or a case whose IEEE
+// result they also give (log of a negative -> NaN, log 0 -> -inf, exp2 of +inf -> +inf).
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
+
 struct V3 {
     float x, y, z;
 };
@@ -79,7 +87,7 @@ __device__ __forceinline__ M9 inv9(const M9& A) {
     const float c01 = A.m[1][2] * A.m[2][0] - A.m[1][0] * A.m[2][2];
     const float c02 = A.m[1][0] * A.m[2][1] - A.m[1][1] * A.m[2][0];
     const float d = A.m[0][0] * c00 + A.m[0][1] * c01 + A.m[0][2] * c02;
-    const float id = 1.0f / d;
+    const float id = frcp(d);
     R.m[0][0] = c00 * id;
     R.m[1][0] = c01 * id;
     R.m[2][0] = c02 * id;
@@ -103,7 +111,7 @@ __device__ __forceinline__ M9 mul9(const M9& A, const M9& B) {
 
 // compressible Mooney-Rivlin Cauchy stress (tetra.cl:118-156; NH when mu2 = 0) from the left
 // Cauchy-Green tensor B and J.  J^-5/3 and J^-7/3 come from one cube root t = J^-1/3, taken as
-// exp2(-log2(J)/3) so that J <= 0 yields NaN/Inf as pow() does in the reference.
+// exp2(-log2(J)/3) so that J < 0 yields NaN and J = 0 yields Inf, as pow() does in the reference.
 __device__ __forceinline__ S6 mr_stress(const S6& B, float J, float mu1, float mu2, float kappa) {
     const float trB = B.xx + B.yy + B.zz;
     // B^2 (symmetric)
@@ -115,7 +123,7 @@ __device__ __forceinline__ S6 mr_stress(const S6& B, float J, float mu1, float m
     B2.yz = B.xy * B.xz + B.yy * B.yz + B.yz * B.zz;
     B2.xz = B.xx * B.xz + B.xy * B.yz + B.xz * B.zz;
     const float trB2 = B2.xx + B2.yy + B2.zz;
-    const float t = exp2f(-log2f(J) * (1.0f / 3.0f));
+    const float t = fexp2(flog2(J) * (-1.0f / 3.0f));
     const float t2 = t * t;
     const float t5 = t2 * t2 * t;
     const float a = mu1 * t5;        // dU/dI1
@@ -152,7 +160,7 @@ __device__ __forceinline__ Corners mr_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const fl
     const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
     const M9 F = deformation_gradient(r1, r2, r3, D);
     const S6 B = aat(F);
-    const float J = dot(r1, cross(r2, r3)) / (6.0f * v0);  // tetra.cl:132
+    const float J = dot(r1, cross(r2, r3)) * frcp(6.0f * v0);  // tetra.cl:132
     const S6 sig = mr_stress(B, J, mu1, mu2, kappa);
     return cofactor_forces(sig, r1, r2, r3, -1.0f / 6.0f);
 }
@@ -267,6 +275,7 @@ struct VPOut {
 
 struct VPConst {
     float mu1, mu2, kappa, m, Y, rate, eps0, n, sy;
+    int linear;  // m == 1 and n == 0
 };
 
 // Finite-strain viscoplastic law (InteractionFiniteStrainViscoplastic.cl:57-206) with node 0 as
@@ -287,7 +296,7 @@ __device__ __forceinline__ VPOut vp_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const floa
     // J = sqrt(det B) (:87)
     const float detB = B.xx * (B.yy * B.zz - B.yz * B.yz) - B.xy * (B.xy * B.zz - B.yz * B.xz) +
                        B.xz * (B.xy * B.yz - B.yy * B.xz);
-    const float J = sqrtf(detB);
+    const float J = fsqrt(detB);
     const S6 sig = mr_stress(B, J, P.mu1, P.mu2, P.kappa);
     const float detr = dot(r1, cross(r2, r3));
     o.f = cofactor_forces(sig, r1, r2, r3, detr < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
@@ -298,14 +307,19 @@ __device__ __forceinline__ VPOut vp_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const floa
     dv.yy -= tr3;
     dv.zz -= tr3;
     const float dd = dv.xx * dv.xx + dv.yy * dv.yy + dv.zz * dv.zz + 2.0f * (dv.xy * dv.xy + dv.yz * dv.yz + dv.xz * dv.xz);
-    const float svm = sqrtf(1.5f * dd);
-    const float sigma0 = P.Y * powf(1.0f + epsbar / P.eps0, P.n);
+    const float svm = fsqrt(1.5f * dd);
     o.von_mises = svm;
     o.pressure = tr3;
     const float over = P.sy > 0.0f ? fmaxf(svm - P.sy, 0.0f) : svm;
-    const float rate = P.rate * powf(over / sigma0, P.m);
+    float rate;
+    if (P.linear) {  // m = 1, n = 0: sigma0 = Y, rate linear in the overstress (uniform branch)
+        rate = P.rate * over * frcp(P.Y);
+    } else {
+        const float sigma0 = P.Y * powf(1.0f + epsbar / P.eps0, P.n);
+        rate = P.rate * powf(over / sigma0, P.m);
+    }
     o.epsbar = epsbar + rate;
-    const float g = (svm != 0.0f) ? rate * 1.5f / svm : 0.0f;
+    const float g = (svm != 0.0f) ? rate * 1.5f * frcp(svm) : 0.0f;
     // Fp += Fe^-1 * (g dev sigma) * F
     M9 Dp;
     Dp.m[0][0] = g * dv.xx; Dp.m[1][1] = g * dv.yy; Dp.m[2][2] = g * dv.zz;

@@ -78,6 +78,7 @@ static VPConst vp_consts(const nfx_viscoplastic& v) {
     c.eps0 = (float)v.hardening_threshold;
     c.n = (float)v.hardening_exponent;
     c.sy = (float)v.yield_offset;
+    c.linear = (v.flow_exponent == 1.0 && v.hardening_exponent == 0.0) ? 1 : 0;
     return c;
 }
 
@@ -192,23 +193,26 @@ __global__ __launch_bounds__(kBlock) void k_vp(TetArgs a, VPConst P, float* __re
 }
 
 // ---- force pass, tiled deterministic scatter ---------------------------------------------------
-// One workgroup per tile of K consecutive (Morton-ordered) tetra.  The tile's distinct nodes are
-// listed on the host (tile_nodes, ascending) with, for each, the run of the tile's corner slots
-// that touch it (tile_slots, grouped by node; tile_lend = run ends).  Phases:
+// One workgroup per tile of K consecutive (Morton-ordered) tetra, one thread per tetra.  The
+// tile's distinct nodes are listed on the host (tile_nodes, ascending, MX entries per tile) with,
+// for each, the run of the tile's corner slots that touch it (tile_slots grouped by node; tile_lend
+// = run ends) and the position of its partial sum in node order (tile_dst).  Phases:
+//   0  issue the tetra's streaming loads (Dm^-1, V0 or Fp, epsbar) and the tile metadata
 //   1  slot list and node positions -> LDS
 //   2  invert the slot runs into per-corner local node ids (conn)
-//   3  one thread per tetra: material law on LDS positions, corner forces -> LDS
-//   4  one thread per tile-node: sum its slot run in list order -> partials (global)
-// The node pass then sums each node's partials in tile order: no atomics, fixed summation order.
+//   3  material law on LDS positions, corner forces -> LDS
+//   4  one thread per tile-node: sum its slot run in list order -> partials[tile_dst]
+// The node pass sums each node's contiguous partials in tile order: no atomics, fixed order.
 struct TileArgs {
     int64_t N, T;
     const float* __restrict__ dminv;
     const float* __restrict__ v0;
     const float* __restrict__ x;
-    const uint32_t* __restrict__ tile_off;
+    const uint32_t* __restrict__ tile_count;
     const uint32_t* __restrict__ tile_nodes;
     const uint16_t* __restrict__ tile_lend;
     const uint16_t* __restrict__ tile_slots;
+    const uint32_t* __restrict__ tile_dst;
     float* __restrict__ partials;
     int32_t max_nodes;
 };
@@ -219,9 +223,10 @@ __host__ __device__ constexpr size_t tile_lds_bytes_fixed() {
 }
 
 template <int MODEL, int K, bool STRESS>
-__global__ __launch_bounds__(kBlock) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Fp,
-                                                       float* __restrict__ eps, float* __restrict__ vm,
-                                                       float* __restrict__ pr) {
+__global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Fp,
+                                                  float* __restrict__ eps, float* __restrict__ vm,
+                                                  float* __restrict__ pr) {
+    // one thread per tetra of the tile (blockDim.x == K)
     extern __shared__ __align__(16) unsigned char smem[];
     uint16_t* s_slots = reinterpret_cast<uint16_t*>(smem);
     uint16_t* s_conn = s_slots + 4 * K;
@@ -233,58 +238,77 @@ __global__ __launch_bounds__(kBlock) void k_tile_force(TileArgs a, ElasticDev e,
     const int64_t T = a.T, N = a.N;
     const int64_t t0 = c * K;
     const int nt = (int)((T - t0) < K ? (T - t0) : K);
-    const uint32_t j0 = a.tile_off[c];
-    const int M = (int)(a.tile_off[c + 1] - j0);
+    const int tl = threadIdx.x;
+    const bool active = tl < nt;
+    const uint32_t u = (uint32_t)tl;
 
-    for (int i = threadIdx.x; i < 4 * nt; i += kBlock) s_slots[i] = a.tile_slots[4 * K * c + i];
-    for (int i = threadIdx.x; i < M; i += kBlock) {
-        const uint32_t n = a.tile_nodes[j0 + i];
-        s_x[i] = a.x[n];
-        s_x[MX + i] = a.x[N + n];
-        s_x[2 * MX + i] = a.x[2 * N + n];
+    // issue this tetra's streaming loads first: they overlap the staging phases below
+    const float* __restrict__ Dt = a.dminv + t0;
+    float D[9], fp[9], e0 = 0.f, v0 = 0.f;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) D[k] = (Dt + k * T)[u];
+        if constexpr (MODEL == NFX_VISCOPLASTIC) {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) fp[k] = (Fp + t0 + k * T)[u];
+            e0 = (eps + t0)[u];
+        } else {
+            v0 = (a.v0 + t0)[u];
+        }
+    }
+
+    // tile metadata: thread tl owns tile-node tl (MX <= K, checked on the host)
+    const int64_t j0 = c * (int64_t)MX;
+    const int M = (int)a.tile_count[c];
+    const bool owner = tl < MX;
+    uint32_t nid = 0, dst = 0;
+    int lb = 0, le = 0;
+    if (owner) {
+        nid = a.tile_nodes[j0 + tl];
+        lb = tl ? a.tile_lend[j0 + tl - 1] : 0;
+        le = a.tile_lend[j0 + tl];
+        dst = a.tile_dst[j0 + tl];
+    }
+    if (active) {
+        // the tile's 4 slots of this tetra position, as one 8-byte load
+        reinterpret_cast<uint2*>(s_slots)[tl] = reinterpret_cast<const uint2*>(a.tile_slots + 4 * K * c)[tl];
+    }
+    if (owner) {
+        s_x[tl] = a.x[nid];
+        s_x[MX + tl] = a.x[N + nid];
+        s_x[2 * MX + tl] = a.x[2 * N + nid];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < M; i += kBlock) {
-        const int b = i ? a.tile_lend[j0 + i - 1] : 0;
-        const int en = a.tile_lend[j0 + i];
-        for (int q = b; q < en; ++q) {
+    if (tl < M) {
+        for (int q = lb; q < le; ++q) {
             const int sl = s_slots[q];
-            s_conn[(sl & 3) * K + (sl >> 2)] = (uint16_t)i;
+            s_conn[(sl & 3) * K + (sl >> 2)] = (uint16_t)tl;
         }
     }
     __syncthreads();
-    for (int tl = threadIdx.x; tl < nt; tl += kBlock) {
-        const int64_t t = t0 + tl;
+    if (active) {
         const int l0 = s_conn[tl], l1 = s_conn[K + tl], l2 = s_conn[2 * K + tl], l3 = s_conn[3 * K + tl];
         const V3 p0 = mk(s_x[l0], s_x[MX + l0], s_x[2 * MX + l0]);
         const V3 p1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
         const V3 p2 = mk(s_x[l2], s_x[MX + l2], s_x[2 * MX + l2]);
         const V3 p3 = mk(s_x[l3], s_x[MX + l3], s_x[2 * MX + l3]);
-        float D[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) D[k] = a.dminv[k * T + t];
         Corners f;
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
-            float fp[9];
+            const VPOut o = vp_tetra(p0, p1, p2, p3, D, fp, e0, P);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) fp[k] = Fp[k * T + t];
-            const VPOut o = vp_tetra(p0, p1, p2, p3, D, fp, eps[t], P);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) Fp[k * T + t] = o.Fp[k];
-            eps[t] = o.epsbar;
+            for (int k = 0; k < 9; ++k) (Fp + t0 + k * T)[u] = o.Fp[k];
+            (eps + t0)[u] = o.epsbar;
             if constexpr (STRESS) {
-                vm[t] = o.von_mises;
-                pr[t] = o.pressure;
+                (vm + t0)[u] = o.von_mises;
+                (pr + t0)[u] = o.pressure;
             }
             f = o.f;
+        } else if constexpr (MODEL == NFX_MR) {
+            f = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
+        } else if constexpr (MODEL == NFX_SVK) {
+            f = svk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
         } else {
-            const float v0 = a.v0[t];
-            if constexpr (MODEL == NFX_MR)
-                f = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
-            else if constexpr (MODEL == NFX_SVK)
-                f = svk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
-            else
-                f = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+            f = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
         }
         s_cf[0 * K + tl] = f.f0.x; s_cf[1 * K + tl] = f.f0.y; s_cf[2 * K + tl] = f.f0.z;
         s_cf[3 * K + tl] = f.f1.x; s_cf[4 * K + tl] = f.f1.y; s_cf[5 * K + tl] = f.f1.z;
@@ -292,18 +316,16 @@ __global__ __launch_bounds__(kBlock) void k_tile_force(TileArgs a, ElasticDev e,
         s_cf[9 * K + tl] = f.f3.x; s_cf[10 * K + tl] = f.f3.y; s_cf[11 * K + tl] = f.f3.z;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < M; i += kBlock) {
-        const int b = i ? a.tile_lend[j0 + i - 1] : 0;
-        const int en = a.tile_lend[j0 + i];
+    if (tl < M) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (int q = b; q < en; ++q) {
+        for (int q = lb; q < le; ++q) {
             const int sl = s_slots[q];
             const int base = 3 * (sl & 3) * K + (sl >> 2);
             fx += s_cf[base];
             fy += s_cf[base + K];
             fz += s_cf[base + 2 * K];
         }
-        float* o = a.partials + 3 * (int64_t)(j0 + i);
+        float* o = a.partials + 3 * (int64_t)dst;
         o[0] = fx;
         o[1] = fy;
         o[2] = fz;
@@ -380,7 +402,6 @@ struct NodeArgs {
     const uint32_t* __restrict__ slots;
     const float* __restrict__ corners;
     const uint32_t* __restrict__ node_off;
-    const uint32_t* __restrict__ node_parts;
     const float* __restrict__ partials;
     float sub_k, wall;
     int32_t indenter;
@@ -429,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
             f = mk(0.f, 0.f, 0.f);
             const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
             for (uint32_t j = b; j < e; ++j) {
-                const float* p = a.partials + 3 * (int64_t)a.node_parts[j];
+                const float* p = a.partials + 3 * (int64_t)j;
                 f = f + mk(p[0], p[1], p[2]);
             }
         } else {
@@ -573,7 +594,6 @@ static NodeArgs node_args(const nfx_plan& p) {
     a.slots = p.csr_slots;
     a.corners = p.corners;
     a.node_off = p.node_off;
-    a.node_parts = p.node_parts;
     a.partials = p.partials;
     a.sub_k = (float)p.substrate_k;
     a.wall = (float)p.wall;
@@ -597,12 +617,13 @@ static int validate_plan(const nfx_plan* p) {
     NFX_REQUIRE(p->scatter >= NFX_SCATTER_ATOMIC && p->scatter <= NFX_SCATTER_TILED, "plan: unknown scatter %d",
                 p->scatter);
     if (p->scatter == NFX_SCATTER_TILED) {
-        NFX_REQUIRE(p->tile_k == 256 || p->tile_k == 512 || p->tile_k == 1024, "plan: tile_k must be 256, 512 or 1024");
-        NFX_REQUIRE(p->tile_off && p->tile_nodes && p->tile_lend && p->tile_slots && p->node_off && p->node_parts &&
+        NFX_REQUIRE(p->tile_k == 256 || p->tile_k == 512, "plan: tile_k must be 256 or 512");
+        NFX_REQUIRE(p->tile_count && p->tile_nodes && p->tile_lend && p->tile_slots && p->tile_dst && p->node_off &&
                         p->partials,
                     "plan: tiled scatter needs the tile arrays");
-        NFX_REQUIRE(p->tile_max_nodes > 0 && p->tile_max_nodes <= 4 * p->tile_k, "plan: bad tile_max_nodes %d",
-                    p->tile_max_nodes);
+        NFX_REQUIRE(p->tile_max_nodes > 0 && p->tile_max_nodes <= p->tile_k,
+                    "plan: tile_max_nodes %d exceeds tile_k %d (use a larger tile or another scatter mode)",
+                    p->tile_max_nodes, p->tile_k);
         const size_t lds = (size_t)p->tile_k * 64 + 12 * (size_t)p->tile_max_nodes;
         NFX_REQUIRE(lds <= 65536, "plan: tile needs %zu B of LDS (max 65536); use a smaller tile_k", lds);
     }
@@ -615,22 +636,22 @@ static int validate_plan(const nfx_plan* p) {
 
 template <int K>
 static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
-    TileArgs a{p.N, p.T, p.dminv, p.v0, p.points, p.tile_off, p.tile_nodes, p.tile_lend, p.tile_slots, p.partials,
-               p.tile_max_nodes};
+    TileArgs a{p.N, p.T, p.dminv, p.v0, p.points, p.tile_count, p.tile_nodes, p.tile_lend, p.tile_slots, p.tile_dst,
+               p.partials, p.tile_max_nodes};
     const unsigned int nb = (unsigned int)((p.T + K - 1) / K);
     const size_t lds = tile_lds_bytes_fixed<K>() + 12 * (size_t)p.tile_max_nodes;
     const ElasticDev e = elastic_consts(p.E, p.nu, p.ratio);
     const VPConst c = vp_consts(p.vp);
     hipStream_t s = ctx->stream;
     switch (p.model) {
-        case NFX_MR: k_tile_force<NFX_MR, K, false><<<nb, kBlock, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
-        case NFX_SVK: k_tile_force<NFX_SVK, K, false><<<nb, kBlock, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
-        case NFX_SVKK: k_tile_force<NFX_SVKK, K, false><<<nb, kBlock, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_MR: k_tile_force<NFX_MR, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_SVK: k_tile_force<NFX_SVK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_SVKK: k_tile_force<NFX_SVKK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
         default:
             if (st)
-                k_tile_force<NFX_VISCOPLASTIC, K, true><<<nb, kBlock, lds, s>>>(a, e, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
+                k_tile_force<NFX_VISCOPLASTIC, K, true><<<nb, K, lds, s>>>(a, e, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
             else
-                k_tile_force<NFX_VISCOPLASTIC, K, false><<<nb, kBlock, lds, s>>>(a, e, c, p.Fp, p.epsbar, nullptr, nullptr);
+                k_tile_force<NFX_VISCOPLASTIC, K, false><<<nb, K, lds, s>>>(a, e, c, p.Fp, p.epsbar, nullptr, nullptr);
     }
 }
 
@@ -638,8 +659,7 @@ static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
     if (p.scatter == NFX_SCATTER_TILED) {
         const bool st = stress && p.von_mises && p.pressure;
         if (p.tile_k == 256) launch_tiled_k<256>(ctx, p, st);
-        else if (p.tile_k == 512) launch_tiled_k<512>(ctx, p, st);
-        else launch_tiled_k<1024>(ctx, p, st);
+        else launch_tiled_k<512>(ctx, p, st);
         NFX_CHECK_LAUNCH();
         return NFX_OK;
     }
@@ -938,20 +958,22 @@ int nfx_build_node_csr_host(const uint32_t* tet, int64_t N, int64_t T, uint32_t*
     return NFX_OK;
 }
 
-int nfx_build_tiles_host(const uint32_t* tet, int64_t N, int64_t T, int32_t K, uint32_t* tile_off, uint32_t* tile_nodes,
-                         uint16_t* tile_lend, uint16_t* tile_slots, uint32_t* node_off, uint32_t* node_parts,
-                         int64_t* m_total, int32_t* max_nodes) {
-    NFX_REQUIRE(tet && tile_off && m_total && max_nodes, "NULL argument");
-    NFX_REQUIRE(K == 256 || K == 512 || K == 1024, "tile_k must be 256, 512 or 1024");
+int nfx_build_tiles_host(const uint32_t* tet, int64_t N, int64_t T, int32_t K, int32_t* max_nodes, int64_t* m_total,
+                         uint32_t* tile_count, uint32_t* tile_nodes, uint16_t* tile_lend, uint16_t* tile_slots,
+                         uint32_t* tile_dst, uint32_t* node_off) {
+    NFX_REQUIRE(tet && max_nodes && m_total && tile_count, "NULL argument");
+    NFX_REQUIRE(K == 256 || K == 512, "tile_k must be 256 or 512");
     int rc = check_sizes(N, T);
     if (rc) return rc;
     const bool fill = tile_nodes != nullptr;
-    NFX_REQUIRE(!fill || (tile_lend && tile_slots && node_off && node_parts), "NULL output array");
+    NFX_REQUIRE(!fill || (tile_lend && tile_slots && tile_dst && node_off), "NULL output array");
     const int64_t ntiles = (T + K - 1) / K;
     std::vector<uint64_t> key((size_t)4 * K);
+    const int32_t MX = fill ? *max_nodes : 0;
     uint64_t total = 0;
     int32_t mx = 0;
-    tile_off[0] = 0;
+    std::vector<uint32_t> cnt;
+    if (fill) cnt.assign((size_t)N + 1, 0);
     for (int64_t c = 0; c < ntiles; ++c) {
         const int64_t t0 = c * K;
         const int nt = (int)std::min<int64_t>(K, T - t0);
@@ -967,30 +989,41 @@ int nfx_build_tiles_host(const uint32_t* tet, int64_t N, int64_t T, int32_t K, u
             const uint32_t n = (uint32_t)(key[(size_t)q] >> 12);
             const bool first = q == 0 || (uint32_t)(key[(size_t)q - 1] >> 12) != n;
             if (first) {
-                if (fill) tile_nodes[total + M] = n;
+                if (fill) {
+                    NFX_REQUIRE(M < MX, "tile %lld has more than max_nodes = %d nodes", (long long)c, MX);
+                    tile_nodes[c * MX + M] = n;
+                    cnt[n + 1]++;
+                }
                 ++M;
             }
             if (fill) {
                 tile_slots[(size_t)4 * K * c + q] = (uint16_t)(key[(size_t)q] & 0xFFF);
-                tile_lend[total + M - 1] = (uint16_t)(q + 1);
+                tile_lend[c * MX + M - 1] = (uint16_t)(q + 1);
             }
         }
-        if (fill)
+        if (fill) {
             for (int q = 4 * nt; q < 4 * K; ++q) tile_slots[(size_t)4 * K * c + q] = 0;
+            for (int i = M; i < MX; ++i) {  // padding: a valid node id, an empty run, never read
+                tile_nodes[c * MX + i] = tile_nodes[c * MX];
+                tile_lend[c * MX + i] = (uint16_t)(4 * nt);
+                tile_dst[c * MX + i] = 0;
+            }
+        }
+        tile_count[c] = (uint32_t)M;
         total += (uint64_t)M;
         NFX_REQUIRE(total < (uint64_t(1) << 32), "too many tile-nodes");
-        tile_off[c + 1] = (uint32_t)total;
         mx = std::max(mx, M);
     }
     *m_total = (int64_t)total;
-    *max_nodes = mx;
-    if (!fill) return NFX_OK;
-    NFX_REQUIRE(*m_total == (int64_t)tile_off[ntiles], "inconsistent tile count");
-    std::vector<uint32_t> cnt((size_t)N + 1, 0);
-    for (uint64_t j = 0; j < total; ++j) cnt[tile_nodes[j] + 1]++;
+    if (!fill) {
+        *max_nodes = mx;
+        return NFX_OK;
+    }
+    // node-ordered partial positions: node n's tile-nodes in ascending tile order
     for (int64_t n = 0; n < N; ++n) cnt[n + 1] += cnt[n];
     std::memcpy(node_off, cnt.data(), sizeof(uint32_t) * (size_t)(N + 1));
-    for (uint64_t j = 0; j < total; ++j) node_parts[cnt[tile_nodes[j]]++] = (uint32_t)j;
+    for (int64_t c = 0; c < ntiles; ++c)
+        for (uint32_t i = 0; i < tile_count[c]; ++i) tile_dst[c * MX + i] = cnt[tile_nodes[c * MX + i]]++;
     return NFX_OK;
 }
 

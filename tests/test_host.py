@@ -216,24 +216,25 @@ def test_tile_builder_invariants(K):
     tets = np.ascontiguousarray(T.reshape(-1, order="F").astype(np.uint32))
     vp = ctypes.c_void_p
     ntiles = (nT + K - 1) // K
-    tile_off = np.zeros(ntiles + 1, np.uint32)
+    count = np.zeros(ntiles, np.uint32)
     m, mx = ctypes.c_int64(), ctypes.c_int32()
-    _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), N, nT, K, tile_off.ctypes.data_as(vp), None, None,
-              None, None, None, ctypes.byref(m), ctypes.byref(mx))
-    M = m.value
-    tn, tl, ts = np.zeros(M, np.uint32), np.zeros(M, np.uint16), np.zeros(4 * K * ntiles, np.uint16)
-    no, npart = np.zeros(N + 1, np.uint32), np.zeros(M, np.uint32)
-    _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), N, nT, K, tile_off.ctypes.data_as(vp),
-              tn.ctypes.data_as(vp), tl.ctypes.data_as(vp), ts.ctypes.data_as(vp), no.ctypes.data_as(vp),
-              npart.ctypes.data_as(vp), ctypes.byref(m), ctypes.byref(mx))
-    sizes = np.diff(tile_off.astype(np.int64))
-    assert mx.value == sizes.max() and tile_off[-1] == M
+    _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), N, nT, K, ctypes.byref(mx), ctypes.byref(m),
+              count.ctypes.data_as(vp), None, None, None, None, None)
+    MX, M = mx.value, m.value
+    assert MX == count.max() and M == count.sum() and MX <= K
+    tn, tl = np.zeros(ntiles * MX, np.uint32), np.zeros(ntiles * MX, np.uint16)
+    ts, td = np.zeros(4 * K * ntiles, np.uint16), np.zeros(ntiles * MX, np.uint32)
+    no = np.zeros(N + 1, np.uint32)
+    _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), N, nT, K, ctypes.byref(mx), ctypes.byref(m),
+              count.ctypes.data_as(vp), tn.ctypes.data_as(vp), tl.ctypes.data_as(vp), ts.ctypes.data_as(vp),
+              td.ctypes.data_as(vp), no.ctypes.data_as(vp))
+    owner = -np.ones(M, np.int64)  # which (tile, tile-node) fills each node-ordered partial position
     for c in range(ntiles):
         t0, nt = c * K, min(K, nT - c * K)
         sub = T[t0:t0 + nt]
-        nodes = tn[tile_off[c]:tile_off[c + 1]]
+        nodes = tn[c * MX:c * MX + count[c]]
         assert np.array_equal(nodes, np.unique(sub))  # ascending unique nodes of the tile
-        ends = tl[tile_off[c]:tile_off[c + 1]].astype(np.int64)
+        ends = tl[c * MX:c * MX + count[c]].astype(np.int64)
         starts = np.concatenate([[0], ends[:-1]])
         assert ends[-1] == 4 * nt
         seen = []
@@ -242,9 +243,13 @@ def test_tile_builder_invariants(K):
             assert np.all(np.diff(run) > 0)  # ascending slots within the run
             assert np.all(sub[run >> 2, run & 3] == nodes[i])
             seen.extend(run)
+            d = td[c * MX + i]
+            assert no[nodes[i]] <= d < no[nodes[i] + 1]
+            assert owner[d] == -1
+            owner[d] = c
         assert sorted(seen) == list(range(4 * nt))
-    # node -> tile-node lists, ascending, and consistent with tile_nodes
-    for n in range(0, N, 5):
-        parts = npart[no[n]:no[n + 1]]
-        assert np.all(np.diff(parts.astype(np.int64)) > 0) and np.all(tn[parts] == n)
-    assert no[-1] == M
+        assert np.all(tn[c * MX + count[c]:(c + 1) * MX] < N)  # padding stays a valid node id
+    assert np.all(owner >= 0) and no[-1] == M
+    # within a node, partial positions follow tile order
+    for n in range(N):
+        assert np.all(np.diff(owner[no[n]:no[n + 1]]) > 0)
