@@ -108,6 +108,31 @@ typedef struct {
     int32_t fix_top_bottom;
 } nfx_plan;
 
+/* Per-cell parameters of a batch, in that cell's simulation units.  The batched material is the
+ * Mooney-Rivlin/Neo-Hookean law, optionally with the Maxwell-type viscoplastic flow (m = 1, n = 0:
+ * rate = vp_rate * max(svm - vp_offset, 0) / vp_yield per step). */
+typedef struct {
+    float mu1, mu2, kappa;
+    float vp_rate, vp_yield, vp_offset;
+    float wall, substrate_k;
+    float tip_radius, tip_k;
+    double tip_start, tip_speed, tip_move_steps;
+} nfx_cell;
+
+/* A batch is the concatenation of many cells' meshes, with tiles aligned to cells (a tile never
+ * spans two cells, so each cell sums its forces in the same order as when it runs alone). */
+typedef struct {
+    int32_t ncells;
+    const uint16_t* tile_ntet;     /* ntiles: tetra in each tile */
+    const uint32_t* tile_cell;     /* ntiles: owning cell of each tile */
+    const uint32_t* node_cell;     /* N: owning cell of each node */
+    const uint32_t* cell_node_off; /* ncells+1: node range of each cell */
+    const nfx_cell* cells;         /* ncells (device) */
+    double* curves;                /* ncells*ncurve FP64 indenter-force samples */
+    int32_t ncurve;
+    int64_t curve_steps; /* sample k is taken at the first step s with floor(s*ncurve/curve_steps) == k */
+} nfx_batch;
+
 /* ---- device and queue: replaces openCL.py:17-50 -------------------------------------------- */
 int nfx_device_count(int* count);                        /* openCL.py:20-28 device walk */
 int nfx_init(int device_id, void* stream, nfx_ctx** ctx); /* openCL.py:17-34 initializeOpenCLObjects;
@@ -169,6 +194,17 @@ int nfx_run_steps(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int64_t nst
                   int32_t use_graph);
 /* the node pass alone (for callers that enqueue the force pass themselves) */
 int nfx_node_update(nfx_ctx* ctx, const nfx_plan* plan, int64_t step, double* tip_out);
+
+/* ---- batched stepping (no reference counterpart; the reference simulates one cell per process):
+ *      all cells of the batch advance together, one force pass + one node pass per step over the
+ *      concatenated meshes, plus a per-cell curve-sampling pass; plan holds the concatenated mesh,
+ *      state and tile arrays (scatter must be NFX_SCATTER_TILED, model NFX_MR or NFX_VISCOPLASTIC,
+ *      the substrate/indenter flags select the terms; their values come from the cell table). ---- */
+int nfx_batch_run_steps(nfx_ctx* ctx, const nfx_plan* plan, const nfx_batch* batch, int64_t step0, int64_t nsteps,
+                        int32_t use_graph);
+/* average device time of the three batched passes over nrep steps (ms), HIP events */
+int nfx_batch_time_passes(nfx_ctx* ctx, const nfx_plan* plan, const nfx_batch* batch, int64_t step0, int32_t nrep,
+                          double* ms_force, double* ms_node, double* ms_curve);
 
 /* ---- kernel timing for the benchmark: average device time of the force pass and of the node
  *      pass over nrep steps, measured with HIP events on the context's stream (ms). ---------- */

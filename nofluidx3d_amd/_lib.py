@@ -87,6 +87,20 @@ class Plan(ctypes.Structure):
     ]
 
 
+class Batch(ctypes.Structure):
+    _fields_ = [
+        ("ncells", c_i32),
+        ("tile_ntet", c_vp),
+        ("tile_cell", c_vp),
+        ("node_cell", c_vp),
+        ("cell_node_off", c_vp),
+        ("cells", c_vp),
+        ("curves", c_vp),
+        ("ncurve", c_i32),
+        ("curve_steps", c_i64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/nfx.h declares
 SIGNATURES = {
     "nfx_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
@@ -114,6 +128,9 @@ SIGNATURES = {
     "nfx_node_update": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_vp]),
     "nfx_time_passes": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i32, ctypes.POINTER(c_f64),
                                        ctypes.POINTER(c_f64)]),
+    "nfx_batch_run_steps": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), ctypes.POINTER(Batch), c_i64, c_i64, c_i32]),
+    "nfx_batch_time_passes": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), ctypes.POINTER(Batch), c_i64, c_i32,
+                                             ctypes.POINTER(c_f64), ctypes.POINTER(c_f64), ctypes.POINTER(c_f64)]),
 }
 
 _lib = None

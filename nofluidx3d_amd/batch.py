@@ -1,0 +1,402 @@
+"""Batched multi-cell AFM indentation: many cells with their own meshes and parameters advance
+together on one GPU.
+
+The reference simulates one cell per process (``Simulation.py``); this module runs a population
+of cells as one concatenated mesh.  Tiles of the tiled scatter are aligned to cells, so every cell
+sums its nodal forces in exactly the order it would use alone, and a cell's batched trajectory
+equals its single-cell run.  Per step: one curve-sampling pass (per-cell FP64 indenter force on
+sampling steps), one force pass over all tiles, one node pass over all nodes; steps replay from a
+captured hipGraph (``nfx_batch_run_steps``).
+
+Cell model (per cell, in that cell's simulation units, ``Simulation.py:42-56``): Neo-Hookean
+elasticity with E_sim = 0.01 and the cell's Poisson ratio, optionally the Maxwell-type
+viscoplastic flow (m = 1, n = 0, Y = 3 eta * 1/s, rate = T0 per step; ``yield_si`` offsets the
+von Mises stress); a substrate wall touching the cell's bottom; a spherical tip touching its top at
+step 0 and moving down at the cell's speed for the whole run, to ``indent_frac * R``.  The lattice
+spacing sets RadiusSIM = (R / spacing) * h with h <= 1 simulation lengths per lattice spacing,
+small enough for explicit stability (``CellSpec.lattice_sim``); mass scaling sets
+T0 = (indentation time) / steps.  The curve is the tip force sampled ``ncurve`` times, normalised by
+E R^2 (F_sim / (0.01 RadiusSIM^2)).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import os
+
+import numpy as np
+
+from nofluidx3d_amd import _lib
+
+E_SIM = 1e-2
+
+
+@dataclasses.dataclass
+class CellSpec:
+    radius_um: float
+    spacing_um: float
+    seed: int
+    E_si: float
+    nu: float
+    viscosity_si: float | None  # None: purely elastic
+    tip_radius_um: float
+    speed_um_s: float
+    yield_si: float = 0.0
+    indent_frac: float = 0.3
+
+    def lattice_sim(self):
+        """Lattice spacing in simulation units.  The explicit step (dt = 1, unit node mass) is stable
+        while the largest stiffness eigenvalue stays below 4; for the seeded meshes (quality >= 0.1)
+        it is about 27 * M * h with M = E (1 - nu) / ((1 + nu)(1 - 2 nu)) the P-wave modulus in
+        simulation units, so h = min(1, 0.075 / M) keeps it at or below 2."""
+        M = E_SIM * (1 - self.nu) / ((1 + self.nu) * (1 - 2 * self.nu))
+        return min(1.0, 0.075 / M)
+
+    def units(self, steps):
+        """Simulation units of this cell: RadiusSIM, L0, p0, T0, V0."""
+        m = max(int(round(self.radius_um / self.spacing_um)), 2)
+        radius_sim = float(m) * self.lattice_sim()
+        L0 = self.radius_um * 1e-6 / radius_sim
+        p0 = self.E_si / E_SIM
+        t_ind = self.indent_frac * self.radius_um / self.speed_um_s
+        T0 = t_ind / steps
+        V0 = L0 / T0
+        return {"RadiusSIM": radius_sim, "L0": L0, "p0": p0, "T0": T0, "V0": V0, "rho": p0 / V0**2}
+
+    def relaxation_per_step(self, steps):
+        """G * T0 / eta: the fraction of deviatoric stress the explicit Maxwell update removes per step."""
+        if self.viscosity_si is None:
+            return 0.0
+        G = self.E_si / (2 * (1 + self.nu))
+        return G * self.units(steps)["T0"] / self.viscosity_si
+
+
+def estimated_tetra(radius_um, spacing_um):
+    """~6 tetra per lattice node for the seeded generator (quality filter included)."""
+    m = max(int(round(radius_um / spacing_um)), 2)
+    return 6.05 * 4.0 / 3.0 * math.pi * m**3
+
+
+def sample_batch(n=512, seed0=1000, steps=10000, t_range=(2000, 60000), max_relax=0.1, viscous=True):
+    """Per-cell seeds seed0 + i.  R ~ U(4, 10) um, spacing ~ U(0.3, 0.6) um, E log-uniform 0.1-10 kPa,
+    viscosity log-uniform 1-1000 Pa s, nu ~ U(0.3, 0.49), tip radius ~ U(1, 5) um, speed log-uniform
+    0.5-50 um/s.  (R, spacing) is redrawn until the cell has ``t_range`` tetra (the stated 2k-60k),
+    and (E, nu, viscosity, speed) until the explicit Maxwell update removes at most ``max_relax`` of
+    the deviatoric stress per step (stability of the reference's one-step explicit flow update)."""
+    specs = []
+    stats = {"redraw_size": 0, "redraw_relax": 0}
+    for i in range(n):
+        rng = np.random.Generator(np.random.PCG64(seed0 + i))
+        while True:
+            R, s = rng.uniform(4.0, 10.0), rng.uniform(0.3, 0.6)
+            if t_range[0] <= estimated_tetra(R, s) <= t_range[1]:
+                break
+            stats["redraw_size"] += 1
+        tip = rng.uniform(1.0, 5.0)
+        while True:
+            E = 10 ** rng.uniform(2.0, 4.0)
+            eta = 10 ** rng.uniform(0.0, 3.0) if viscous else None
+            nu = rng.uniform(0.3, 0.49)
+            v = 10 ** rng.uniform(math.log10(0.5), math.log10(50.0))
+            spec = CellSpec(R, s, seed0 + i, E, nu, eta, tip, v)
+            if spec.relaxation_per_step(steps) <= max_relax:
+                break
+            stats["redraw_relax"] += 1
+        specs.append(spec)
+    return specs, stats
+
+
+def _mesh_job(args):
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    R, s, seed, cache = args
+    path = os.path.join(cache, f"cell_{R:.6f}_{s:.6f}_{seed}.npz") if cache else None
+    if path and os.path.exists(path):
+        d = np.load(path)
+        return d["P"], d["T"]
+    P, T = generate_sphere_mesh(R, s, seed)
+    if path:
+        np.savez(path, P=P, T=T)
+    return P, T
+
+
+def make_meshes(specs, workers=None, cache="/tmp/nfx_cells"):
+    """Seeded meshes of all cells (a spawned process pool; no GPU use in the workers)."""
+    import multiprocessing as mp
+
+    if cache:
+        os.makedirs(cache, exist_ok=True)
+    jobs = [(c.radius_um, c.spacing_um, c.seed, cache) for c in specs]
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(jobs) < 4:
+        return [_mesh_job(j) for j in jobs]
+    with mp.get_context("spawn").Pool(workers) as pool:
+        return pool.map(_mesh_job, jobs, chunksize=4)
+
+
+def sim_points(P_um, radius_sim):
+    """TetraCell's recentre and scale (TetraCell.py:15-20) in FP64."""
+    p = np.asarray(P_um, np.float64) - P_um[0]
+    return p * (radius_sim / abs(p[1, 0]))
+
+
+def build_host_arrays(specs, meshes, steps, tile_k):
+    """All host-side arrays of a batch (numpy), before upload.  Tiles are built per cell with a
+    common capacity MX and concatenated with node/partial offsets."""
+    K = int(tile_k)
+    nc = len(specs)
+    cells = np.zeros(nc, dtype=CELL_DTYPE)
+    pts, evs, vols, tets, tile_meta, units = [], [], [], [], [], []
+    node_off, tet_pad_off = [0], [0]
+    radius_sim = np.zeros(nc)
+    for i, (spec, (P, T)) in enumerate(zip(specs, meshes)):
+        u = spec.units(steps)
+        units.append(u)
+        Rs = u["RadiusSIM"]
+        radius_sim[i] = Rs
+        x = sim_points(P, Rs)
+        p = x[T]
+        R1, R2, R3 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]
+        ev = np.stack([R1, R2, R3], 1).reshape(len(T), 9)
+        vol = np.einsum("ij,ij->i", R1, np.cross(R2, R3)) / 6
+        nT = len(T)
+        nTp = -(-nT // K) * K
+        pad = nTp - nT
+        # padding tetra (never read by the kernels): identity shape, the cell's first node
+        evs.append(np.concatenate([ev, np.tile(np.eye(3).reshape(1, 9), (pad, 1))]))
+        vols.append(np.concatenate([vol, np.full(pad, 1.0 / 6.0)]))
+        tets.append(np.concatenate([T + node_off[-1], np.full((pad, 4), node_off[-1])]))
+        pts.append(x)
+        tile_meta.append((T, len(P), nT))
+        node_off.append(node_off[-1] + len(P))
+        tet_pad_off.append(tet_pad_off[-1] + nTp)
+        # material (E_sim = 0.01, Neo-Hookean): the constants the single-cell path computes
+        mu = E_SIM / (2 * (1 + spec.nu))
+        c = cells[i]
+        c["mu1"], c["mu2"], c["kappa"] = mu, 0.0, E_SIM / (3 * (1 - 2 * spec.nu))
+        if spec.viscosity_si is not None:
+            c["vp_rate"] = u["T0"]
+            c["vp_yield"] = 3.0 * spec.viscosity_si * 1.0 / u["p0"]
+            c["vp_offset"] = spec.yield_si / u["p0"]
+        tipR = spec.tip_radius_um * 1e-6 / u["L0"]
+        c["wall"], c["substrate_k"] = -Rs, 0.1
+        c["tip_radius"], c["tip_k"] = tipR, 0.1
+        c["tip_start"] = Rs + tipR
+        c["tip_speed"] = spec.speed_um_s * 1e-6 / u["V0"]
+        c["tip_move_steps"] = float(steps)
+    N = int(node_off[-1])
+
+    vp = ctypes.c_void_p
+    per = []
+    MX = 0
+    for T, n, nT in tile_meta:
+        tt = np.ascontiguousarray(T.reshape(-1, order="F").astype(np.uint32))
+        ntiles = -(-nT // K)
+        cnt = np.zeros(ntiles, np.uint32)
+        m, mx = ctypes.c_int64(), ctypes.c_int32()
+        _lib.call("nfx_build_tiles_host", tt.ctypes.data_as(vp), n, nT, K, ctypes.byref(mx), ctypes.byref(m),
+                  cnt.ctypes.data_as(vp), None, None, None, None, None)
+        per.append((tt, n, nT, ntiles, cnt, m.value))
+        MX = max(MX, mx.value)
+    if MX > K:
+        raise _lib.NfxError(f"a tile touches {MX} nodes > tile_k {K}; use a larger tile_k")
+    nt_tot = sum(p[3] for p in per)
+    h = {
+        "tile_count": np.zeros(nt_tot, np.uint32),
+        "tile_nodes": np.zeros(nt_tot * MX, np.uint32),
+        "tile_lend": np.zeros(nt_tot * MX, np.uint16),
+        "tile_slots": np.zeros(4 * K * nt_tot, np.uint16),
+        "tile_dst": np.zeros(nt_tot * MX, np.uint32),
+        "tile_ntet": np.zeros(nt_tot, np.uint16),
+        "tile_cell": np.zeros(nt_tot, np.uint32),
+        "node_poff": np.zeros(N + 1, np.uint32),
+    }
+    tpos, ppos = 0, 0
+    for i, (tt, n, nT, ntiles, cnt, M) in enumerate(per):
+        tn = np.zeros(ntiles * MX, np.uint32)
+        tl = np.zeros(ntiles * MX, np.uint16)
+        ts = np.zeros(4 * K * ntiles, np.uint16)
+        td = np.zeros(ntiles * MX, np.uint32)
+        no = np.zeros(n + 1, np.uint32)
+        m, mx = ctypes.c_int64(M), ctypes.c_int32(MX)
+        _lib.call("nfx_build_tiles_host", tt.ctypes.data_as(vp), n, nT, K, ctypes.byref(mx), ctypes.byref(m),
+                  cnt.ctypes.data_as(vp), tn.ctypes.data_as(vp), tl.ctypes.data_as(vp), ts.ctypes.data_as(vp),
+                  td.ctypes.data_as(vp), no.ctypes.data_as(vp))
+        sl = slice(tpos, tpos + ntiles)
+        h["tile_count"][sl] = cnt
+        h["tile_nodes"][tpos * MX:(tpos + ntiles) * MX] = tn + node_off[i]
+        h["tile_lend"][tpos * MX:(tpos + ntiles) * MX] = tl
+        h["tile_slots"][4 * K * tpos:4 * K * (tpos + ntiles)] = ts
+        h["tile_dst"][tpos * MX:(tpos + ntiles) * MX] = td + ppos
+        h["tile_ntet"][sl] = np.minimum(K, nT - K * np.arange(ntiles))
+        h["tile_cell"][sl] = i
+        h["node_poff"][node_off[i]:node_off[i] + n + 1] = no + ppos
+        tpos += ntiles
+        ppos += M
+    h.update({
+        "cells": cells, "units": units, "radius_sim": radius_sim, "max_nodes": MX, "m_total": ppos,
+        "node_off": np.array(node_off, np.int64), "tet_pad_off": np.array(tet_pad_off, np.int64),
+        "N": N, "T": int(tet_pad_off[-1]), "T_real": int(sum(m[2] for m in tile_meta)),
+        "x": np.concatenate(pts), "tets": np.concatenate(tets), "ev": np.concatenate(evs),
+        "vol": np.concatenate(vols),
+        "node_cell": np.repeat(np.arange(nc, dtype=np.uint32), np.diff(np.array(node_off))),
+    })
+    return h
+
+
+class BatchSimulation:
+    """All cells of ``specs`` on the current device.  ``meshes``: list of (points_um, tets)."""
+
+    def __init__(self, specs, meshes, steps=10000, ncurve=256, tile_k=256, use_graph=True, device_id=0):
+        import torch
+
+        from nofluidx3d_amd import openCL
+
+        openCL.initializeOpenCLObjects(device_id)
+        self.specs = list(specs)
+        self.steps = steps
+        self.ncurve = ncurve
+        self.tile_k = K = int(tile_k)
+        self.use_graph = use_graph
+        self.time = 0
+        nc = len(specs)
+        self.ncells = nc
+        viscous = [s.viscosity_si is not None for s in specs]
+        if any(viscous) and not all(viscous):
+            raise ValueError("a batch is either all elastic or all viscoplastic")
+        self.model = _lib.NFX_VISCOPLASTIC if viscous[0] else _lib.NFX_MR
+        h = build_host_arrays(specs, meshes, steps, K)
+        self.cells_host, self.units, self.radius_sim = h["cells"], h["units"], h["radius_sim"]
+        self.node_off, self.tet_pad_off = h["node_off"], h["tet_pad_off"]
+        self.N, self.T, self.T_real, MX = h["N"], h["T"], h["T_real"], h["max_nodes"]
+        self.max_nodes = MX
+
+        dev = torch.device("cuda", device_id)
+        f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)
+        u32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.uint32).view(np.int32)).to(dev)
+        u16 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.uint16).view(np.int16)).to(dev)
+        self.x = f32(h["x"].T.reshape(-1))
+        self.v = torch.zeros(3 * self.N, dtype=torch.float32, device=dev)
+        self.fold = torch.zeros(3 * self.N, dtype=torch.float32, device=dev)
+        self.tetras = u32(h["tets"].T.reshape(-1))
+        ev = torch.from_numpy(np.ascontiguousarray(h["ev"].T.reshape(-1))).to(dev)
+        vol = torch.from_numpy(np.ascontiguousarray(h["vol"])).to(dev)
+        self.dminv = torch.empty(9 * self.T, dtype=torch.float32, device=dev)
+        self.v0 = torch.empty(self.T, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize(dev)
+        hh = openCL.handle()
+        _lib.call("nfx_precompute_f64", hh, ev.data_ptr(), vol.data_ptr(), self.T, self.dminv.data_ptr(),
+                  self.v0.data_ptr())
+        _lib.call("nfx_stream_sync", hh)
+        del ev, vol
+        if self.model == _lib.NFX_VISCOPLASTIC:
+            self.Fp = f32(np.repeat(np.identity(3).reshape(9), self.T))
+            self.eps = torch.zeros(self.T, dtype=torch.float32, device=dev)
+        else:
+            self.Fp = self.eps = None
+        self.t_count, self.t_nodes, self.t_lend = u32(h["tile_count"]), u32(h["tile_nodes"]), u16(h["tile_lend"])
+        self.t_slots, self.t_dst = u16(h["tile_slots"]), u32(h["tile_dst"])
+        self.t_ntet, self.t_cell = u16(h["tile_ntet"]), u32(h["tile_cell"])
+        self.node_poff, self.node_cell = u32(h["node_poff"]), u32(h["node_cell"])
+        self.cell_node_off = u32(self.node_off)
+        self.partials = torch.zeros(3 * h["m_total"], dtype=torch.float32, device=dev)
+        self.cells = torch.from_numpy(self.cells_host.view(np.uint8).copy()).to(dev)
+        self.curves = torch.zeros(nc * ncurve, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize(dev)
+
+        p = _lib.Plan()
+        p.N, p.T = self.N, self.T
+        p.tetras, p.dminv, p.v0 = self.tetras.data_ptr(), self.dminv.data_ptr(), self.v0.data_ptr()
+        p.points, p.vel, p.fold = self.x.data_ptr(), self.v.data_ptr(), self.fold.data_ptr()
+        p.force = 0
+        p.model = self.model
+        if self.Fp is not None:
+            p.Fp, p.epsbar = self.Fp.data_ptr(), self.eps.data_ptr()
+        p.scatter = _lib.SCATTER_TILED
+        p.tile_k, p.tile_max_nodes = K, MX
+        p.tile_count, p.tile_nodes, p.tile_lend = (t.data_ptr() for t in (self.t_count, self.t_nodes, self.t_lend))
+        p.tile_slots, p.tile_dst = self.t_slots.data_ptr(), self.t_dst.data_ptr()
+        p.node_off, p.partials = self.node_poff.data_ptr(), self.partials.data_ptr()
+        p.substrate, p.indenter = 1, _lib.INDENTER_SPHERE
+        self.plan = p
+        b = Batch()
+        b.ncells = nc
+        b.tile_ntet, b.tile_cell = self.t_ntet.data_ptr(), self.t_cell.data_ptr()
+        b.node_cell, b.cell_node_off, b.cells = self.node_cell.data_ptr(), self.cell_node_off.data_ptr(), \
+            self.cells.data_ptr()
+        b.curves, b.ncurve, b.curve_steps = self.curves.data_ptr(), ncurve, steps
+        self.batch = b
+
+    def advance(self, nsteps):
+        from nofluidx3d_amd import openCL
+
+        _lib.call("nfx_batch_run_steps", openCL.handle(), self.plan, self.batch, self.time, nsteps,
+                  1 if self.use_graph else 0)
+        self.time += nsteps
+
+    def sync(self):
+        import torch
+
+        from nofluidx3d_amd import openCL
+
+        _lib.call("nfx_stream_sync", openCL.handle())
+        torch.cuda.synchronize()
+
+    def time_passes(self, nrep=20):
+        from nofluidx3d_amd import openCL
+
+        f, n, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _lib.call("nfx_batch_time_passes", openCL.handle(), self.plan, self.batch, self.time, nrep, ctypes.byref(f),
+                  ctypes.byref(n), ctypes.byref(c))
+        self.time += nrep
+        return f.value, n.value, c.value
+
+    def curves_sim(self):
+        """(ncells, ncurve) indenter force in simulation units."""
+        self.sync()
+        return self.curves.cpu().numpy().reshape(self.ncells, self.ncurve)
+
+    def curves_normalized(self):
+        """(ncells, ncurve) F / (E R^2)."""
+        return self.curves_sim() / (E_SIM * self.radius_sim[:, None] ** 2)
+
+    def positions(self, i):
+        self.sync()
+        X = self.x.cpu().numpy().reshape(3, self.N).T
+        return X[self.node_off[i]:self.node_off[i + 1]].astype(np.float64)
+
+    def single_cell_config(self, i, outdir="/tmp/nfx_batch_cell/"):
+        """The reference-format config of cell i (for the single-cell Simulation path)."""
+        s, u = self.specs[i], self.units[i]
+        cfg = {
+            "gpu": 0,
+            "RhoSI": u["rho"],
+            "SimulationTimeSI": u["T0"] * self.steps,
+            "VTKFramesTotal": 1,
+            "OutputDir": outdir,
+            "InitialDistance": 0.0,
+            "SubstrateDistance": 0.0,
+            "MoveTimeSI": u["T0"] * self.steps,
+            "VelocitySI": s.speed_um_s * 1e-6,
+            "SphereRadiusSI": s.tip_radius_um * 1e-6,
+            "CELL": {"RadiusSI": s.radius_um * 1e-6, "RadiusSIM": u["RadiusSIM"], "YoungsModulusSI": s.E_si,
+                     "PoissonRatio": s.nu, "MooneyRivlinRatio": 1.0, "model": "NEO_HOOKEAN", "InitVTK": ""},
+        }
+        if s.viscosity_si is not None:
+            cfg["CELL"]["ViscositySI"] = s.viscosity_si
+            cfg["CELL"]["YieldStressSI"] = s.yield_si
+        return cfg
+
+
+CELL_DTYPE = np.dtype([
+    ("mu1", np.float32), ("mu2", np.float32), ("kappa", np.float32),
+    ("vp_rate", np.float32), ("vp_yield", np.float32), ("vp_offset", np.float32),
+    ("wall", np.float32), ("substrate_k", np.float32),
+    ("tip_radius", np.float32), ("tip_k", np.float32),
+    ("tip_start", np.float64), ("tip_speed", np.float64), ("tip_move_steps", np.float64),
+], align=True)
+
+
+Batch = _lib.Batch

@@ -63,4 +63,11 @@ struct nfx_ctx {
     nfx_plan graph_plan{};
     int64_t graph_steps = 0;
     bool graph_hist = false;
+    // batched stepping: the batch being enqueued (NULL otherwise) and its graph cache
+    const nfx_batch* batch = nullptr;
+    hipGraphExec_t bgraph_exec = nullptr;
+    hipGraph_t bgraph = nullptr;
+    nfx_plan bgraph_plan{};
+    nfx_batch bgraph_batch{};
+    int64_t bgraph_steps = 0;
 };

@@ -215,6 +215,10 @@ struct TileArgs {
     const uint32_t* __restrict__ tile_dst;
     float* __restrict__ partials;
     int32_t max_nodes;
+    // batches: cell-aligned tiles with per-cell material constants (NULL for a single cell)
+    const uint16_t* __restrict__ tile_ntet;
+    const uint32_t* __restrict__ tile_cell;
+    const nfx_cell* __restrict__ cells;
 };
 
 template <int K>
@@ -237,7 +241,21 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     const int64_t c = xcd_block(blockIdx.x, gridDim.x);
     const int64_t T = a.T, N = a.N;
     const int64_t t0 = c * K;
-    const int nt = (int)((T - t0) < K ? (T - t0) : K);
+    int nt = (int)((T - t0) < K ? (T - t0) : K);
+    if (a.cells) {  // batch: this tile's cell and its constants (uniform over the workgroup)
+        nt = a.tile_ntet[c];
+        const nfx_cell cp = a.cells[a.tile_cell[c]];
+        e.mu1 = cp.mu1;
+        e.mu2 = cp.mu2;
+        e.kappa = cp.kappa;
+        P.mu1 = cp.mu1;
+        P.mu2 = cp.mu2;
+        P.kappa = cp.kappa;
+        P.rate = cp.vp_rate;
+        P.Y = cp.vp_yield;
+        P.sy = cp.vp_offset;
+        P.linear = 1;
+    }
     const int tl = threadIdx.x;
     const bool active = tl < nt;
     const uint32_t u = (uint32_t)tl;
@@ -636,8 +654,15 @@ static int validate_plan(const nfx_plan* p) {
 
 template <int K>
 static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
-    TileArgs a{p.N, p.T, p.dminv, p.v0, p.points, p.tile_count, p.tile_nodes, p.tile_lend, p.tile_slots, p.tile_dst,
-               p.partials, p.tile_max_nodes};
+    // ctx->batch is set only while a batched step is being enqueued
+    TileArgs a{p.N,         p.T,          p.dminv,     p.v0,     p.points,         p.tile_count,
+               p.tile_nodes, p.tile_lend,  p.tile_slots, p.tile_dst, p.partials,     p.tile_max_nodes,
+               nullptr,     nullptr,      nullptr};
+    if (ctx->batch) {
+        a.tile_ntet = ctx->batch->tile_ntet;
+        a.tile_cell = ctx->batch->tile_cell;
+        a.cells = ctx->batch->cells;
+    }
     const unsigned int nb = (unsigned int)((p.T + K - 1) / K);
     const size_t lds = tile_lds_bytes_fixed<K>() + 12 * (size_t)p.tile_max_nodes;
     const ElasticDev e = elastic_consts(p.E, p.nu, p.ratio);
@@ -735,6 +760,165 @@ static int launch_node_pass(nfx_ctx* ctx, const nfx_plan& p, int64_t step, const
     return NFX_OK;
 }
 
+// ---- batched node pass and curve sampling ----------------------------------------------------
+struct BatchNodeArgs {
+    int64_t N;
+    float* x;
+    float* v;
+    float* fold;
+    const uint32_t* __restrict__ node_off;
+    const float* __restrict__ partials;
+    const uint32_t* __restrict__ node_cell;
+    const nfx_cell* __restrict__ cells;
+};
+
+// Node pass of a batch: each node applies its own cell's substrate and indenter (position from
+// the step index with the cell's law), then the damped velocity-Verlet update.
+template <bool SUB, int IND>
+__global__ __launch_bounds__(kBlock) void k_batch_node(BatchNodeArgs a, int64_t step, const int64_t* step_dev,
+                                                       int32_t step_off) {
+    const int64_t time = step_dev ? (*step_dev + step_off) : step;
+    const int64_t N = a.N;
+    const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    V3 f = mk(0.f, 0.f, 0.f);
+    const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
+    for (uint32_t j = b; j < e; ++j) {
+        const float* p = a.partials + 3 * (int64_t)j;
+        f = f + mk(p[0], p[1], p[2]);
+    }
+    const nfx_cell* cp = a.cells + a.node_cell[n];
+    V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
+    if constexpr (SUB) f.y += substrate_term(x.y, cp->substrate_k, cp->wall);
+    if constexpr (IND == NFX_INDENTER_SPHERE) {
+        const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
+        f = f + sphere_term(x, cp->tip_k, cp->tip_radius, pos);
+    } else if constexpr (IND == NFX_INDENTER_PLANE) {
+        const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
+        f.y += plane_term(x.y, cp->tip_k, pos);
+    }
+    V3 v = mk(a.v[n], a.v[N + n], a.v[2 * N + n]);
+    const V3 fo = mk(a.fold[n], a.fold[N + n], a.fold[2 * N + n]);
+    vv_update(x, v, f, fo);
+    a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
+    a.v[n] = v.x; a.v[N + n] = v.y; a.v[2 * N + n] = v.z;
+    a.fold[n] = f.x; a.fold[N + n] = f.y; a.fold[2 * N + n] = f.z;
+}
+
+// One workgroup per cell: on a sampling step, the cell's indenter force (Simulation.py:198-212 in
+// simulation units) from the positions at the start of the step, summed in FP64 in a fixed order.
+template <int IND>
+__global__ __launch_bounds__(kBlock) void k_batch_curves(const float* __restrict__ x, int64_t N,
+                                                         const uint32_t* __restrict__ cell_node_off,
+                                                         const nfx_cell* __restrict__ cells, double* curves,
+                                                         int32_t ncurve, int64_t curve_steps, int64_t step,
+                                                         const int64_t* step_dev, int32_t step_off) {
+    const int64_t time = step_dev ? (*step_dev + step_off) : step;
+    if (time < 0 || time >= curve_steps) return;
+    const int64_t k = time * ncurve / curve_steps;
+    if (time > 0 && (time - 1) * ncurve / curve_steps == k) return;  // not the first step of sample k
+    const int cell = blockIdx.x;
+    const nfx_cell cp = cells[cell];
+    const float pos = (float)indenter_pos(cp.tip_start, cp.tip_speed, cp.tip_move_steps, time);
+    double s = 0.0;
+    for (int64_t n = cell_node_off[cell] + threadIdx.x; n < cell_node_off[cell + 1]; n += kBlock) {
+        const V3 p = mk(x[n], x[N + n], x[2 * N + n]);
+        if constexpr (IND == NFX_INDENTER_SPHERE) {
+            s -= (double)sphere_term(p, cp.tip_k, cp.tip_radius, pos).y;
+        } else {
+            s -= (double)plane_term(p.y, cp.tip_k, pos);
+        }
+    }
+    __shared__ double ws[kBlock / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += ws[w];
+        curves[(int64_t)cell * ncurve + k] = t;
+    }
+}
+
+static int validate_batch(const nfx_plan* p, const nfx_batch* b) {
+    NFX_REQUIRE(b != nullptr, "batch is NULL");
+    NFX_REQUIRE(p->scatter == NFX_SCATTER_TILED, "batched stepping needs the tiled scatter");
+    NFX_REQUIRE(p->model == NFX_MR || p->model == NFX_VISCOPLASTIC, "batched model must be MR or viscoplastic");
+    NFX_REQUIRE(b->ncells > 0 && b->tile_ntet && b->tile_cell && b->node_cell && b->cell_node_off && b->cells,
+                "batch: NULL table");
+    NFX_REQUIRE(!b->curves || (b->ncurve > 0 && b->curve_steps > 0 && p->indenter != NFX_INDENTER_NONE),
+                "batch: curves need ncurve, curve_steps and an indenter");
+    NFX_REQUIRE(p->T % p->tile_k == 0, "batch: padded tetra count must be a multiple of tile_k");
+    return NFX_OK;
+}
+
+template <bool SUB>
+static void launch_batch_node_sub(nfx_ctx* ctx, const nfx_plan& p, const BatchNodeArgs& a, int64_t step,
+                                  const int64_t* sd, int32_t off) {
+    const unsigned int nb = nblocks(p.N);
+    if (p.indenter == NFX_INDENTER_SPHERE) k_batch_node<SUB, 1><<<nb, kBlock, 0, ctx->stream>>>(a, step, sd, off);
+    else if (p.indenter == NFX_INDENTER_PLANE) k_batch_node<SUB, 2><<<nb, kBlock, 0, ctx->stream>>>(a, step, sd, off);
+    else k_batch_node<SUB, 0><<<nb, kBlock, 0, ctx->stream>>>(a, step, sd, off);
+}
+
+static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b, int64_t step, const int64_t* sd,
+                             int32_t off, int which = 7) {
+    if ((which & 1) && b.curves) {
+        if (p.indenter == NFX_INDENTER_SPHERE)
+            k_batch_curves<1><<<b.ncells, kBlock, 0, ctx->stream>>>(p.points, p.N, b.cell_node_off, b.cells, b.curves,
+                                                                    b.ncurve, b.curve_steps, step, sd, off);
+        else
+            k_batch_curves<2><<<b.ncells, kBlock, 0, ctx->stream>>>(p.points, p.N, b.cell_node_off, b.cells, b.curves,
+                                                                    b.ncurve, b.curve_steps, step, sd, off);
+        NFX_CHECK_LAUNCH();
+    }
+    if (which & 2) {
+        ctx->batch = &b;
+        const int rc = launch_force_pass(ctx, p, false);
+        ctx->batch = nullptr;
+        if (rc) return rc;
+    }
+    if (which & 4) {
+        BatchNodeArgs a{p.N, p.points, p.vel, p.fold, p.node_off, p.partials, b.node_cell, b.cells};
+        if (p.substrate) launch_batch_node_sub<true>(ctx, p, a, step, sd, off);
+        else launch_batch_node_sub<false>(ctx, p, a, step, sd, off);
+        NFX_CHECK_LAUNCH();
+    }
+    return NFX_OK;
+}
+
+static void drop_bgraph(nfx_ctx* ctx) {
+    if (ctx->bgraph_exec) (void)hipGraphExecDestroy(ctx->bgraph_exec);
+    if (ctx->bgraph) (void)hipGraphDestroy(ctx->bgraph);
+    ctx->bgraph_exec = nullptr;
+    ctx->bgraph = nullptr;
+    ctx->bgraph_steps = 0;
+}
+
+static int capture_bgraph(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b, int64_t steps) {
+    drop_bgraph(ctx);
+    NFX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeGlobal));
+    int rc = NFX_OK;
+    for (int64_t i = 0; i < steps && rc == NFX_OK; ++i) rc = launch_batch_step(ctx, p, b, 0, ctx->step_dev, (int32_t)i);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc != NFX_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        return NFX_ERR_HIP;
+    }
+    ctx->bgraph = g;
+    NFX_HIP(hipGraphInstantiate(&ctx->bgraph_exec, g, nullptr, nullptr, 0));
+    ctx->bgraph_plan = p;
+    ctx->bgraph_batch = b;
+    ctx->bgraph_steps = steps;
+    return NFX_OK;
+}
+
 static void drop_graph(nfx_ctx* ctx) {
     if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
@@ -829,6 +1013,7 @@ int nfx_destroy(nfx_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     nfx::drop_graph(c);
+    nfx::drop_bgraph(c);
     (void)hipFree(c->partials);
     (void)hipFree(c->counter);
     (void)hipFree(c->step_dev);
@@ -1132,6 +1317,75 @@ int nfx_time_passes(nfx_ctx* c, const nfx_plan* p, int64_t step0, int32_t nrep, 
             sf += a;
             sn += b;
         }
+        *ms_force = sf / nrep;
+        *ms_node = sn / nrep;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+
+int nfx_batch_run_steps(nfx_ctx* c, const nfx_plan* p, const nfx_batch* b, int64_t step0, int64_t nsteps,
+                        int32_t use_graph) {
+    NFX_REQUIRE(c, "ctx is NULL");
+    int rc = validate_plan(p);
+    if (rc) return rc;
+    rc = validate_batch(p, b);
+    if (rc) return rc;
+    NFX_REQUIRE(nsteps >= 0, "nsteps < 0");
+    int64_t done = 0;
+    if (use_graph && nsteps >= 2) {
+        const int64_t B = std::min<int64_t>(nsteps, kGraphBlock);
+        if (!c->bgraph_exec || c->bgraph_steps != B || std::memcmp(&c->bgraph_plan, p, sizeof(nfx_plan)) != 0 ||
+            std::memcmp(&c->bgraph_batch, b, sizeof(nfx_batch)) != 0) {
+            rc = capture_bgraph(c, *p, *b, B);
+            if (rc) return rc;
+        }
+        while (nsteps - done >= B) {
+            k_set_i64<<<1, 1, 0, c->stream>>>(c->step_dev, step0 + done);
+            NFX_CHECK_LAUNCH();
+            NFX_HIP(hipGraphLaunch(c->bgraph_exec, c->stream));
+            done += B;
+        }
+    }
+    for (; done < nsteps; ++done) {
+        rc = launch_batch_step(c, *p, *b, step0 + done, nullptr, 0);
+        if (rc) return rc;
+    }
+    return NFX_OK;
+}
+
+int nfx_batch_time_passes(nfx_ctx* c, const nfx_plan* p, const nfx_batch* b, int64_t step0, int32_t nrep,
+                          double* ms_force, double* ms_node, double* ms_curve) {
+    NFX_REQUIRE(c && ms_force && ms_node && ms_curve && nrep > 0, "bad argument");
+    int rc = validate_plan(p);
+    if (rc) return rc;
+    rc = validate_batch(p, b);
+    if (rc) return rc;
+    std::vector<hipEvent_t> ev(4 * (size_t)nrep);
+    for (auto& e : ev) NFX_HIP(hipEventCreate(&e));
+    for (int i = 0; i < nrep && rc == NFX_OK; ++i) {
+        NFX_HIP(hipEventRecord(ev[4 * i], c->stream));
+        rc = launch_batch_step(c, *p, *b, step0 + i, nullptr, 0, 1);
+        NFX_HIP(hipEventRecord(ev[4 * i + 1], c->stream));
+        if (rc == NFX_OK) rc = launch_batch_step(c, *p, *b, step0 + i, nullptr, 0, 2);
+        NFX_HIP(hipEventRecord(ev[4 * i + 2], c->stream));
+        if (rc == NFX_OK) rc = launch_batch_step(c, *p, *b, step0 + i, nullptr, 0, 4);
+        NFX_HIP(hipEventRecord(ev[4 * i + 3], c->stream));
+    }
+    if (rc == NFX_OK) {
+        NFX_HIP(hipEventSynchronize(ev[4 * (size_t)nrep - 1]));
+        double sc = 0, sf = 0, sn = 0;
+        for (int i = 0; i < nrep; ++i) {
+            float a = 0, bb = 0, cc = 0;
+            NFX_HIP(hipEventElapsedTime(&a, ev[4 * i], ev[4 * i + 1]));
+            NFX_HIP(hipEventElapsedTime(&bb, ev[4 * i + 1], ev[4 * i + 2]));
+            NFX_HIP(hipEventElapsedTime(&cc, ev[4 * i + 2], ev[4 * i + 3]));
+            sc += a;
+            sf += bb;
+            sn += cc;
+        }
+        *ms_curve = sc / nrep;
         *ms_force = sf / nrep;
         *ms_node = sn / nrep;
     }

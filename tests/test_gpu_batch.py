@@ -1,0 +1,150 @@
+"""GPU tests of the batched stepper (cells of different meshes and parameters in one launch per pass).
+
+- every cell of a batch matches its own single-cell Simulation run (same kernels, cell-aligned
+  tiles): positions within 1e-5 * RadiusSIM, curve within 1e-5 * max|F| (rounding-level, from the
+  cell-constant conversions and the FP64 curve reduction order);
+- one cell against the FP64 oracle: 1e-4 (the plain-path tolerance);
+- bitwise: repeated runs, graph vs eager;
+- tile size (an autotuned choice) changes only the rounding: 1e-4.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 800
+NCURVE = 32
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from nofluidx3d_amd import openCL
+
+    openCL.initializeOpenCLObjects(0)
+    return openCL.handle()
+
+
+def specs_for(viscous):
+    from nofluidx3d_amd.batch import CellSpec
+
+    geo = [(3.0, 0.6), (3.5, 0.7), (2.5, 0.5), (4.0, 0.8), (3.2, 0.8)]
+    E = [300.0, 1000.0, 3000.0, 150.0, 800.0]
+    nu = [0.45, 0.3, 0.49, 0.4, 0.35]
+    eta = [40.0, 200.0, 900.0, 20.0, 100.0]
+    tip = [1.0, 2.0, 1.5, 3.0, 1.2]
+    v = [5.0, 0.8, 20.0, 2.0, 10.0]
+    out = []
+    for i in range(5):
+        s = CellSpec(geo[i][0], geo[i][1], 3000 + i, E[i], nu[i], eta[i] if viscous else None, tip[i], v[i])
+        assert s.relaxation_per_step(STEPS) <= 0.1
+        out.append(s)
+    return out
+
+
+def make_batch(viscous, tile_k=256, use_graph=True):
+    from nofluidx3d_amd.batch import BatchSimulation, make_meshes
+
+    specs = specs_for(viscous)
+    meshes = make_meshes(specs, workers=1, cache=None)
+    return BatchSimulation(specs, meshes, steps=STEPS, ncurve=NCURVE, tile_k=tile_k, use_graph=use_graph), meshes
+
+
+def sample_steps():
+    s = np.arange(STEPS)
+    k = s * NCURVE // STEPS
+    first = np.concatenate([[True], k[1:] != k[:-1]])
+    return s[first]
+
+
+def single_cell(b, meshes, i):
+    import torch
+
+    from nofluidx3d_amd.Simulation import Simulation
+    from nofluidx3d_amd.util import vtk
+
+    cfg = b.single_cell_config(i)
+    P, T = meshes[i]
+    sim = Simulation(cfg, mesh=vtk.from_tetra(P, T), verbose=False, tile_k=b.tile_k)
+    sim.setInteractionSubstrate()
+    sim.setInteractionSphere()
+    if b.specs[i].viscosity_si is not None:
+        sim.setInteractionFiniteStrainViscoplastic()
+    else:
+        sim.setInteractionTetra()
+    sim.setInteractionVelocityVerlet()
+    hist = torch.zeros(STEPS, dtype=torch.float64, device="cuda:0")
+    sim.advance(STEPS, tip_hist=hist)
+    return sim, sim.positions(), hist.cpu().numpy()
+
+
+@pytest.mark.parametrize("viscous", [False, True])
+def test_batch_cells_match_single_cell_runs(ctx, viscous):
+    b, meshes = make_batch(viscous)
+    b.advance(STEPS)
+    curves = b.curves_sim()
+    ks = sample_steps()
+    for i in range(b.ncells):
+        sim, x, hist = single_cell(b, meshes, i)
+        R = b.radius_sim[i]
+        assert np.max(np.abs(b.positions(i) - x)) < 1e-5 * R, i
+        ref = hist[ks]
+        assert np.max(np.abs(ref)) > 0, i
+        assert np.max(np.abs(curves[i] - ref)) < 1e-5 * np.max(np.abs(ref)), i
+
+
+def test_batch_cell_matches_oracle(ctx):
+    b, meshes = make_batch(True)
+    b.advance(STEPS)
+    i = 1
+    s, c = b.specs[i], b.cells_host[i]
+    prm = O.RunParams()
+    prm.model = O.VISCOPLASTIC
+    prm.vp = O.VPParams(float(c["mu1"]), float(c["mu2"]), float(c["kappa"]), 1.0, float(c["vp_yield"]),
+                        float(c["vp_rate"]), 1.0, 0.0, float(c["vp_offset"]))
+    prm.vp.mu1 = 0.01 / (2 * (1 + s.nu))
+    prm.vp.kappa = 0.01 / (3 * (1 - 2 * s.nu))
+    prm.vp.flow_rate = b.units[i]["T0"]
+    prm.vp.yield_stress = 3.0 * s.viscosity_si / b.units[i]["p0"]
+    prm.substrate, prm.substrate_k, prm.wall = 1, 0.1, float(c["wall"])
+    prm.indenter, prm.ind_k, prm.ind_radius = 1, 0.1, s.tip_radius_um * 1e-6 / b.units[i]["L0"]
+    prm.ind_start, prm.ind_speed, prm.ind_move_steps = float(c["tip_start"]), float(c["tip_speed"]), float(STEPS)
+    prm.nthreads = 1
+    from nofluidx3d_amd.batch import sim_points
+
+    P, T = meshes[i]
+    st = O.OracleState(sim_points(P, b.radius_sim[i]), T, prm)
+    hist = st.run(STEPS)
+    ref = hist[sample_steps()]
+    R = b.radius_sim[i]
+    assert np.max(np.abs(b.positions(i) - st.positions())) < 1e-4 * R
+    assert np.max(np.abs(b.curves_sim()[i] - ref)) < 1e-4 * np.max(np.abs(ref))
+
+
+def test_batch_bitwise_reproducible_graph_and_eager(ctx):
+    a, _ = make_batch(True, use_graph=True)
+    a.advance(300)
+    b, _ = make_batch(True, use_graph=True)
+    b.advance(300)
+    c, _ = make_batch(True, use_graph=False)
+    c.advance(300)
+    xa, xb, xc = (z.x.cpu().numpy() for z in (a, b, c))
+    assert np.array_equal(xa, xb) and np.array_equal(xa, xc)
+    assert np.array_equal(a.curves_sim(), b.curves_sim()) and np.array_equal(a.curves_sim(), c.curves_sim())
+
+
+def test_tile_size_changes_only_rounding(ctx):
+    a, _ = make_batch(False, tile_k=256)
+    a.advance(STEPS)
+    b, _ = make_batch(False, tile_k=512)
+    b.advance(STEPS)
+    ca, cb = a.curves_sim(), b.curves_sim()
+    assert np.max(np.abs(ca - cb)) < 1e-4 * np.max(np.abs(ca))
+    for i in range(a.ncells):
+        assert np.max(np.abs(a.positions(i) - b.positions(i))) < 1e-4 * a.radius_sim[i]

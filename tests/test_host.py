@@ -253,3 +253,62 @@ def test_tile_builder_invariants(K):
     # within a node, partial positions follow tile order
     for n in range(N):
         assert np.all(np.diff(owner[no[n]:no[n + 1]]) > 0)
+
+
+def test_cell_table_layout_matches_header():
+    from nofluidx3d_amd import _lib
+    from nofluidx3d_amd.batch import CELL_DTYPE
+
+    text = open(os.path.join(ROOT, "include", "nfx.h")).read()
+    body = text[text.index("typedef struct {\n    float mu1, mu2, kappa;"):text.index("} nfx_cell;")]
+    names = re.findall(r"\b([a-z_0-9]+)\s*[,;]", body.replace("typedef struct {", ""))
+    assert list(CELL_DTYPE.names) == names
+    assert CELL_DTYPE.itemsize == 64 and CELL_DTYPE.fields["tip_start"][1] == 40
+    body = text[text.index("typedef struct {\n    int32_t ncells;"):text.index("} nfx_batch;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = [d.replace("*", " ").split()[-1] for d in body.replace("typedef struct {", "").split(";") if d.strip()]
+    assert names == [f[0] for f in _lib.Batch._fields_]
+
+
+def test_batch_sampling_respects_ranges():
+    from nofluidx3d_amd.batch import estimated_tetra, sample_batch
+
+    specs, stats = sample_batch(64, seed0=1000)
+    assert len(specs) == 64 and len({s.seed for s in specs}) == 64
+    for s in specs:
+        assert 4 <= s.radius_um <= 10 and 0.3 <= s.spacing_um <= 0.6
+        assert 2000 <= estimated_tetra(s.radius_um, s.spacing_um) <= 60000
+        assert 100 <= s.E_si <= 1e4 and 0.3 <= s.nu <= 0.49 and 1 <= s.viscosity_si <= 1000
+        assert 1 <= s.tip_radius_um <= 5 and 0.5 <= s.speed_um_s <= 50
+        assert s.relaxation_per_step(10000) <= 0.1
+    again, _ = sample_batch(64, seed0=1000)
+    assert again == specs  # seeded
+
+
+def test_batch_host_arrays_indexing_is_in_bounds():
+    """Every index the batched kernels read from the host-built arrays stays inside its array."""
+    from nofluidx3d_amd.batch import build_host_arrays, make_meshes, sample_batch
+
+    specs, _ = sample_batch(6, seed0=77, t_range=(2000, 9000))
+    meshes = make_meshes(specs, workers=1, cache=None)
+    for K in (256, 512):
+        h = build_host_arrays(specs, meshes, 1000, K)
+        N, T, MX, M = h["N"], h["T"], h["max_nodes"], h["m_total"]
+        ntiles = len(h["tile_count"])
+        assert T == ntiles * K and MX <= K
+        assert h["tile_nodes"].max() < N and h["tile_dst"].max() < M
+        assert np.all(h["tile_ntet"] >= 1) and np.all(h["tile_ntet"] <= K)
+        assert np.all(h["tile_count"] <= MX) and h["tile_count"].sum() == M
+        assert h["node_poff"][-1] == M and np.all(np.diff(h["node_poff"].astype(np.int64)) >= 1)
+        assert h["tets"].max() < N and len(h["tets"]) == T
+        # every tile's node ids lie inside its own cell's node range
+        for c in range(ntiles):
+            cell = h["tile_cell"][c]
+            nodes = h["tile_nodes"][c * MX:c * MX + h["tile_count"][c]]
+            assert np.all((nodes >= h["node_off"][cell]) & (nodes < h["node_off"][cell + 1]))
+            lend = h["tile_lend"][c * MX:(c + 1) * MX]
+            assert lend.max() <= 4 * h["tile_ntet"][c]
+        # tile-node partial positions are a permutation of 0..M-1
+        used = np.concatenate([h["tile_dst"][c * MX:c * MX + h["tile_count"][c]] for c in range(ntiles)])
+        assert np.array_equal(np.sort(used), np.arange(M))
+        assert len(h["node_cell"]) == N and h["node_cell"].max() == len(specs) - 1

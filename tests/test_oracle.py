@@ -426,3 +426,31 @@ def test_run_momentum_decay_without_contacts():
     rho = (1 - 0.0025) / (1 + 0.0025)
     p1 = st.vel.reshape(3, -1).sum(1)
     assert np.allclose(p1, rho * p_prev, rtol=1e-9)
+
+
+def test_explicit_stability_bound_of_the_cell_units():
+    """The batched cells' unit choice keeps the largest stiffness eigenvalue (unit node mass) below
+    the explicit limit 4: power iteration on the oracle's force Jacobian for the stiffest sampled
+    material (nu = 0.49) at the chosen lattice size."""
+    from nofluidx3d_amd.batch import CellSpec, sim_points
+    from nofluidx3d_amd.mesh import generate_sphere_mesh
+
+    s = CellSpec(2.5, 0.5, 3002, 3000.0, 0.49, None, 1.5, 20.0)
+    P, T = generate_sphere_mesh(s.radius_um, s.spacing_um, s.seed)
+    x = sim_points(P, s.units(800)["RadiusSIM"])
+    ev, vol = O.edge_vectors_volumes(x, T)
+    args = (O.soa(T, np.uint32), O.soa(ev), np.ascontiguousarray(vol))
+
+    def forces(X):
+        f = np.zeros(3 * len(X))
+        O.tetra_force(O.MR, 0.01, s.nu, 1.0, O.soa(X), *args, f)
+        return f.reshape(3, -1).T
+
+    v = np.random.default_rng(0).normal(size=x.shape)
+    lam = 0.0
+    for _ in range(60):
+        v /= np.linalg.norm(v)
+        Kv = -(forces(x + 1e-6 * v) - forces(x - 1e-6 * v)) / 2e-6
+        lam = float((v * Kv).sum())
+        v = Kv
+    assert lam < 2.5, lam
