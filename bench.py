@@ -146,6 +146,54 @@ def cpu_baseline(P, T, cfg, material, target_s=12.0):
     }
 
 
+def bench_batched(steps=10000, ncells=512, log=log):
+    """The batched workload: 512 seeded cells (seeds 1000+i), 10k steps each, one 256-point curve per
+    cell; tile size from the autotuner.  Returns the JSON block."""
+    from nofluidx3d_amd.autotune import tune_batch
+    from nofluidx3d_amd.batch import BatchSimulation, make_meshes, sample_batch
+
+    specs, stats = sample_batch(ncells, seed0=1000, steps=steps)
+    t0 = time.time()
+    meshes = make_meshes(specs)
+    tmesh = time.time() - t0
+    K, tuned = tune_batch(specs, meshes, steps, use_cache=False, log=log)
+    b = BatchSimulation(specs, meshes, steps=steps, ncurve=256, tile_k=K)
+    b.advance(100)  # includes the graph capture
+    b.sync()
+    t0 = time.perf_counter()
+    b.advance(steps - 100)
+    b.sync()
+    dt = time.perf_counter() - t0
+    ms = dt / (steps - 100) * 1e3
+    f, n, c = b.time_passes(20)
+    curves = b.curves_normalized()
+    ok = np.isfinite(curves).all(axis=1)
+    tet = np.array([len(m[1]) for m in meshes])
+    step_bytes = 132 * b.T_real + 108 * b.N
+    return {
+        "workload": f"{ncells} cells (seeds 1000+i), {b.T_real} tetra, {b.N} nodes, NH + Maxwell viscoplastic, "
+                    f"{steps} steps each, 256-point curve per cell",
+        "cells": ncells,
+        "tetra": int(b.T_real),
+        "nodes": int(b.N),
+        "tetra_per_cell": {"min": int(tet.min()), "median": int(np.median(tet)), "max": int(tet.max())},
+        "sampling_redraws": stats,
+        "tile_k": K,
+        "autotune_ms_per_step": tuned,
+        "ms_per_step": ms,
+        "value": b.T_real * 1e3 / ms,
+        "unit": "tetra-updates/s",
+        "hbm_gbs": step_bytes / (ms * 1e-3) / 1e9,
+        "ms_force_pass": f,
+        "ms_node_pass": n,
+        "ms_curve_pass": c,
+        "full_run_s": dt * steps / (steps - 100),
+        "cell_curves_per_s": ncells / (dt * steps / (steps - 100)),
+        "cells_finite": int(ok.sum()),
+        "mesh_generation_s": tmesh,
+    }
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -166,7 +214,8 @@ def main():
     ap.add_argument("--scatter", default="tiled", choices=["tiled", "csr", "atomic"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
-    ap.add_argument("--tile-k", type=int, default=512, choices=[256, 512])
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched 512-cell measurement")
+    ap.add_argument("--tile-k", type=int, default=0, choices=[0, 256, 512], help="0: autotune")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_large_cell.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (HBM-counter calibration under rocprofv3)")
@@ -189,8 +238,13 @@ def main():
         material = args.material or "elastic"
     P, T = cached_mesh(cfg)
     N, nT = len(P), len(T)
-    sim = make_sim(cfg, P, T, args.scatter, material, args.tile_k)
+    sim = make_sim(cfg, P, T, args.scatter, material, args.tile_k or 512)
     assert sim.plan() is not None
+    tuned = None
+    if args.scatter == "tiled" and args.tile_k == 0:
+        from nofluidx3d_amd.autotune import tune_simulation
+
+        _, tuned = tune_simulation(sim, log=log)
     if args.calibrate_copy:
         a = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device="cuda:0")
         b = torch.empty_like(a)
@@ -207,7 +261,7 @@ def main():
     traffic = None
     if args.traffic_json and os.path.exists(args.traffic_json):
         tj = json.load(open(args.traffic_json))
-        key = f"{material}_{args.scatter}_{args.tile_k}"
+        key = f"{material}_{args.scatter}_{sim.tile_k}"
         if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}):
             traffic = tj["force_pass_bytes"][key]
     out = {
@@ -231,7 +285,8 @@ def main():
             "nodes": N,
             "tetra": nT,
             "scatter": args.scatter,
-            "tile_k": args.tile_k,
+            "tile_k": sim.tile_k,
+            "autotune_ms_per_step": tuned,
             "parallelism": "single GPU",
         },
         "hbm_gbs": step_bytes / (ms * 1e-3) / 1e9,
@@ -269,6 +324,10 @@ def main():
         out["plain_small"] = {"workload": f"small_cell {len(Ps)} nodes {len(Ts)} tetra NH", "ms_per_step": ms3,
                               "value": len(Ts) * 1e3 / ms3, "unit": "tetra-updates/s"}
         del s3
+    if not args.no_batched and not args.no_extra:
+        out["batched"] = bench_batched()
+        if "plain_small" in out:
+            out["batched"]["x_plain_small"] = out["batched"]["value"] / out["plain_small"]["value"]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P, T, cfg, material)
         out["cpu_baseline"]["hbm_equiv_gbs"] = out["cpu_baseline"]["value"] / nT * step_bytes / 1e9

@@ -366,6 +366,16 @@ class Simulation:
         out = self._tip.cpu().numpy()
         return float(out[0]), float(out[1])
 
+    def state_tensors(self):
+        """Device tensors that carry the simulation state (positions, force, plugin state)."""
+        out = [self.points.buf, self.force.buf, self.vonMises.buf, self.pressure.buf]
+        for inter in self.interactions:
+            for name in ("velocityB", "forceOldB", "plasticDeformationTensorB", "accumulatedStrainB"):
+                t = getattr(inter, name, None)
+                if t is not None:
+                    out.append(t)
+        return out
+
     def positions(self) -> np.ndarray:
         """Current node positions (N, 3) float64."""
         return self.points.readFromGPU().reshape(3, self.numPoints).T.copy()
