@@ -194,6 +194,24 @@ def bench_batched(steps=10000, ncells=512, log=log):
     }
 
 
+def stream_copy_gbs(nbytes=1 << 30, nrep=20):
+    """Measured copy bandwidth (read + write bytes / time) with the library's vectorized copy."""
+    import ctypes
+
+    import torch
+
+    from nofluidx3d_amd import _lib, openCL
+
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    ms = ctypes.c_double()
+    _lib.call("nfx_stream_copy", openCL.handle(), a.data_ptr(), b.data_ptr(), nbytes, nrep, ctypes.byref(ms))
+    ok = bool(torch.equal(a[:1024], b[:1024]))
+    del a, b
+    return {"gbs": 2 * nbytes / (ms.value * 1e-3) / 1e9, "bytes_copied": nbytes, "ms": ms.value, "verified": ok}
+
+
 def _cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -324,6 +342,10 @@ def main():
         out["plain_small"] = {"workload": f"small_cell {len(Ps)} nodes {len(Ts)} tetra NH", "ms_per_step": ms3,
                               "value": len(Ts) * 1e3 / ms3, "unit": "tetra-updates/s"}
         del s3
+    if not args.no_extra:
+        sc = stream_copy_gbs()
+        out["stream_copy"] = sc
+        out["roofline"]["frac_of_stream_copy"] = achieved / sc["gbs"]
     if not args.no_batched and not args.no_extra:
         out["batched"] = bench_batched()
         if "plain_small" in out:

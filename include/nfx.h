@@ -211,6 +211,47 @@ int nfx_batch_time_passes(nfx_ctx* ctx, const nfx_plan* plan, const nfx_batch* b
 int nfx_time_passes(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int32_t nrep, double* ms_force,
                     double* ms_node);
 
+/* ---- MLP surrogate (no reference counterpart): BF16-on-MFMA GEMMs with fused epilogues --------
+ * C[M x N] = A[M x K] * B[N x K]^T, A and B row-major BF16 (K contiguous), FP32 accumulation.
+ * M a multiple of 128, N and K of 32, lda/ldb multiples of 8. */
+#define NFX_MLP_STORE 0    /* C = acc (+ bias if non-NULL) */
+#define NFX_MLP_FWD_GELU 1 /* C (nullable) = Z = acc + bias; H = gelu(Z) BF16 [M x N]; HT (nullable) = H^T */
+#define NFX_MLP_BWD_GELU 2 /* H = acc * gelu'(Z) BF16 [M x N]; HT = H^T */
+#define NFX_MLP_ADAM 3     /* acc = weight gradient: Adam on FP32 master C, BF16 copies H [M x N], HT [N x M] */
+
+typedef struct {
+    int32_t epilogue;
+    int32_t M, N, K;
+    const void* A; /* BF16 */
+    int32_t lda;
+    const void* B; /* BF16 */
+    int32_t ldb;
+    const float* bias; /* N */
+    float* C;          /* FP32 [M x N] */
+    const float* Z;    /* FP32 [M x N], BWD_GELU */
+    void* H;           /* BF16 [M x N] */
+    void* HT;          /* BF16 [N x M] */
+    float* adam_m;     /* ADAM moments, FP32 [M x N] */
+    float* adam_v;
+    float lr_t;        /* lr / (1 - beta1^t) */
+    float bc2;         /* 1 / sqrt(1 - beta2^t): w -= lr_t m / (sqrt(v) bc2 + eps) */
+    float beta1, beta2, eps;
+    float grad_scale;  /* multiplies the gradient before the update */
+} nfx_mlp_gemm_args;
+
+int nfx_mlp_gemm(nfx_ctx* ctx, const nfx_mlp_gemm_args* args);
+/* mean squared error over M x N FP32 (accumulated into *loss_sum, FP64, device) and its gradient
+ * 2 (Y - T) / (M N) as BF16 in [M x N] (dZ) and [N x M] (dZT) */
+int nfx_mlp_mse(nfx_ctx* ctx, const float* Y, const float* target, int32_t M, int32_t N, void* dZ, void* dZT,
+                double* loss_sum);
+/* bias gradient = row sums of dZT [N x M] (BF16), then Adam on the FP32 bias b with moments m, v */
+int nfx_mlp_bias_adam(nfx_ctx* ctx, const void* dZT, int32_t M, int32_t N, float* b, float* m, float* v, float lr_t,
+                      float bc2, float beta1, float beta2, float eps, float grad_scale);
+
+/* ---- measured HBM ceiling: average time of nrep device-to-device copies of `bytes` (a multiple of
+ *      16) with 16-byte vector accesses, HIP events (ms).  No reference counterpart. ----------- */
+int nfx_stream_copy(nfx_ctx* ctx, const void* src, void* dst, int64_t bytes, int32_t nrep, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,6 +101,35 @@ class Batch(ctypes.Structure):
     ]
 
 
+NFX_MLP_STORE, NFX_MLP_FWD_GELU, NFX_MLP_BWD_GELU, NFX_MLP_ADAM = 0, 1, 2, 3
+
+
+class MlpGemm(ctypes.Structure):
+    _fields_ = [
+        ("epilogue", c_i32),
+        ("M", c_i32),
+        ("N", c_i32),
+        ("K", c_i32),
+        ("A", c_vp),
+        ("lda", c_i32),
+        ("B", c_vp),
+        ("ldb", c_i32),
+        ("bias", c_vp),
+        ("C", c_vp),
+        ("Z", c_vp),
+        ("H", c_vp),
+        ("HT", c_vp),
+        ("adam_m", c_vp),
+        ("adam_v", c_vp),
+        ("lr_t", ctypes.c_float),
+        ("bc2", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+        ("grad_scale", ctypes.c_float),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/nfx.h declares
 SIGNATURES = {
     "nfx_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
@@ -128,6 +157,11 @@ SIGNATURES = {
     "nfx_node_update": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_vp]),
     "nfx_time_passes": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i32, ctypes.POINTER(c_f64),
                                        ctypes.POINTER(c_f64)]),
+    "nfx_mlp_gemm": (ctypes.c_int, [c_vp, ctypes.POINTER(MlpGemm)]),
+    "nfx_mlp_mse": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp]),
+    "nfx_mlp_bias_adam": (ctypes.c_int, [c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, ctypes.c_float, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "nfx_stream_copy": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i32, ctypes.POINTER(c_f64)]),
     "nfx_batch_run_steps": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), ctypes.POINTER(Batch), c_i64, c_i64, c_i32]),
     "nfx_batch_time_passes": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), ctypes.POINTER(Batch), c_i64, c_i32,
                                              ctypes.POINTER(c_f64), ctypes.POINTER(c_f64), ctypes.POINTER(c_f64)]),

@@ -588,6 +588,12 @@ __global__ __launch_bounds__(kBlock) void k_tip(const float* __restrict__ x, int
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
+// STREAM-style copy (16-byte vector loads/stores, grid-stride): the measured HBM ceiling
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b,
+                                                        int64_t n4) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += (int64_t)gridDim.x * kBlock) b[i] = a[i];
+}
+
 // ---- host helpers -----------------------------------------------------------------------------
 static inline unsigned int nblocks(int64_t n, int per = kBlock) { return (unsigned int)((n + per - 1) / per); }
 
@@ -1391,6 +1397,24 @@ int nfx_batch_time_passes(nfx_ctx* c, const nfx_plan* p, const nfx_batch* b, int
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
+}
+
+
+int nfx_stream_copy(nfx_ctx* c, const void* src, void* dst, int64_t bytes, int32_t nrep, double* ms) {
+    NFX_REQUIRE(c && src && dst && ms && nrep > 0 && bytes > 0 && bytes % 16 == 0, "bad argument");
+    const int64_t n4 = bytes / 16;
+    const unsigned int nb = (unsigned int)std::min<int64_t>((n4 + kBlock - 1) / kBlock, (int64_t)c->num_cu * 16);
+    k_stream_copy<<<nb, kBlock, 0, c->stream>>>((const float4*)src, (float4*)dst, n4);
+    NFX_CHECK_LAUNCH();
+    NFX_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int i = 0; i < nrep; ++i) k_stream_copy<<<nb, kBlock, 0, c->stream>>>((const float4*)src, (float4*)dst, n4);
+    NFX_CHECK_LAUNCH();
+    NFX_HIP(hipEventRecord(c->ev1, c->stream));
+    NFX_HIP(hipEventSynchronize(c->ev1));
+    float t = 0;
+    NFX_HIP(hipEventElapsedTime(&t, c->ev0, c->ev1));
+    *ms = t / nrep;
+    return NFX_OK;
 }
 
 }  // extern "C"
