@@ -142,9 +142,10 @@ def sim_points(P_um, radius_sim):
     return p * (radius_sim / abs(p[1, 0]))
 
 
-def build_host_arrays(specs, meshes, steps, tile_k):
+def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
     """All host-side arrays of a batch (numpy), before upload.  Tiles are built per cell with a
-    common capacity MX and concatenated with node/partial offsets."""
+    common capacity MX and concatenated with node/partial offsets (skipped with geometry_only:
+    positions, reference shapes and the cell table only)."""
     K = int(tile_k)
     nc = len(specs)
     cells = np.zeros(nc, dtype=CELL_DTYPE)
@@ -187,6 +188,9 @@ def build_host_arrays(specs, meshes, steps, tile_k):
         c["tip_speed"] = spec.speed_um_s * 1e-6 / u["V0"]
         c["tip_move_steps"] = float(steps)
     N = int(node_off[-1])
+    if geometry_only:
+        return {"cells": cells, "units": units, "radius_sim": radius_sim, "x": np.concatenate(pts),
+                "ev": np.concatenate(evs), "vol": np.concatenate(vols)}
 
     vp = ctypes.c_void_p
     per = []
@@ -268,6 +272,7 @@ class BatchSimulation:
             raise ValueError("a batch is either all elastic or all viscoplastic")
         self.model = _lib.NFX_VISCOPLASTIC if viscous[0] else _lib.NFX_MR
         h = build_host_arrays(specs, meshes, steps, K)
+        self._meshes = meshes
         self.cells_host, self.units, self.radius_sim = h["cells"], h["units"], h["radius_sim"]
         self.node_off, self.tet_pad_off = h["node_off"], h["tet_pad_off"]
         self.N, self.T, self.T_real, MX = h["N"], h["T"], h["T_real"], h["max_nodes"]
@@ -328,6 +333,46 @@ class BatchSimulation:
             self.cells.data_ptr()
         b.curves, b.ncurve, b.curve_steps = self.curves.data_ptr(), ncurve, steps
         self.batch = b
+
+    def reset(self, specs):
+        """Start over with new physical parameters on the same meshes: only E, nu, viscosity, yield,
+        tip radius, speed and indentation depth may change.  Topology and tile arrays are kept."""
+        import torch
+
+        from nofluidx3d_amd import openCL
+
+        for a, b in zip(specs, self.specs):
+            if (a.radius_um, a.spacing_um, a.seed) != (b.radius_um, b.spacing_um, b.seed):
+                raise ValueError("reset keeps the meshes: radius, spacing and seed must not change")
+        if (specs[0].viscosity_si is not None) != (self.model == _lib.NFX_VISCOPLASTIC):
+            raise ValueError("reset keeps the material model")
+        self.specs = list(specs)
+        h = build_host_arrays(specs, self._meshes, self.steps, self.tile_k, geometry_only=True)
+        self.cells_host, self.units, self.radius_sim = h["cells"], h["units"], h["radius_sim"]
+        dev = self.x.device
+        self.x.copy_(torch.from_numpy(np.ascontiguousarray(h["x"].T.reshape(-1), np.float32)))
+        self.v.zero_()
+        self.fold.zero_()
+        ev = torch.from_numpy(np.ascontiguousarray(h["ev"].T.reshape(-1))).to(dev)
+        vol = torch.from_numpy(np.ascontiguousarray(h["vol"])).to(dev)
+        if self.Fp is not None:
+            self.Fp.copy_(torch.from_numpy(np.repeat(np.identity(3).reshape(9), self.T).astype(np.float32)))
+            self.eps.zero_()
+        self.cells.copy_(torch.from_numpy(self.cells_host.view(np.uint8).copy()))
+        self.curves.zero_()
+        torch.cuda.synchronize(dev)
+        hh = openCL.handle()
+        _lib.call("nfx_precompute_f64", hh, ev.data_ptr(), vol.data_ptr(), self.T, self.dminv.data_ptr(),
+                  self.v0.data_ptr())
+        _lib.call("nfx_stream_sync", hh)
+        del ev, vol
+        self.time = 0
+
+    def features(self):
+        """Surrogate inputs per cell: log10 E, nu, log10 viscosity, tip radius [um], log10 speed,
+        cell radius [um]."""
+        return np.array([[math.log10(s.E_si), s.nu, math.log10(s.viscosity_si or 1.0), s.tip_radius_um,
+                          math.log10(s.speed_um_s), s.radius_um] for s in self.specs])
 
     def advance(self, nsteps):
         from nofluidx3d_amd import openCL
