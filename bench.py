@@ -8,9 +8,10 @@ captured hipGraph.  Inputs are resident in HBM before the timed region starts.
 
 Printed JSON line (one):
 - value = tetra-updates/s = T * steps / wall seconds of the timed region;
-- hbm_gbs = algorithmic bytes per step / ms_per_step (132 T + 108 N for the viscoplastic step,
-  56 T + 108 N for the elastic one);
-- roofline: the dominant kernel (the force pass) -- algorithmic bytes per launch (132 T + 24 N)
+- hbm_gbs = algorithmic bytes per step / ms_per_step (96 T + 108 N for the viscoplastic step:
+  connectivity 16 B + shape state S = Dm^-1 Fp^-1 read and written 72 B + epsbar read and written
+  8 B per tetra; 56 T + 108 N for the elastic one: connectivity 16 + Dm^-1 36 + V0 4);
+- roofline: the dominant kernel (the force pass) -- algorithmic bytes per launch (96 T + 24 N)
   over its HIP-event average duration, against 8 TB/s; ``traffic`` = measured HBM bytes per
   launch from rocprofv3 FETCH_SIZE + WRITE_SIZE when ``--traffic-json`` names a file holding them;
 - cpu_baseline: the FP64 C oracle (OpenMP, all allowed host threads) on a bounded step sample
@@ -32,6 +33,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+VP_TET_BYTES = 96  # per tetra and step: connectivity 16 + shape state 36 + 36 + epsbar 4 + 4
 
 
 def log(*a):
@@ -48,7 +50,9 @@ def cached_mesh(cfg):
         return d["P"], d["T"]
     t0 = time.time()
     P, T = configs.make_mesh(cfg)
-    np.savez(path, P=P, T=T)
+    tmp = f"{path}.{os.getpid()}.tmp.npz"
+    np.savez(tmp, P=P, T=T)
+    os.replace(tmp, path)
     log(f"mesh {m}: {len(P)} nodes {len(T)} tetra in {time.time() - t0:.1f}s")
     return P, T
 
@@ -169,7 +173,7 @@ def bench_batched(steps=10000, ncells=512, log=log):
     curves = b.curves_normalized()
     ok = np.isfinite(curves).all(axis=1)
     tet = np.array([len(m[1]) for m in meshes])
-    step_bytes = 132 * b.T_real + 108 * b.N
+    step_bytes = VP_TET_BYTES * b.T_real + 108 * b.N
     return {
         "workload": f"{ncells} cells (seeds 1000+i), {b.T_real} tetra, {b.N} nodes, NH + Maxwell viscoplastic, "
                     f"{steps} steps each, 256-point curve per cell",
@@ -273,8 +277,8 @@ def main():
     ms = time_steps(sim, args.warmup, args.steps) * 1e3
     ms_force, ms_node = pass_times(sim, min(args.steps, 100))
     vp = material == "viscoplastic"
-    force_bytes = (132 if vp else 56) * nT + 24 * N
-    step_bytes = (132 if vp else 56) * nT + 108 * N
+    force_bytes = (VP_TET_BYTES if vp else 56) * nT + 24 * N
+    step_bytes = (VP_TET_BYTES if vp else 56) * nT + 108 * N
     achieved = force_bytes / (ms_force * 1e-3) / 1e9
     traffic = None
     if args.traffic_json and os.path.exists(args.traffic_json):

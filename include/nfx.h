@@ -106,6 +106,8 @@ typedef struct {
     int32_t indenter; /* NFX_INDENTER_* */
     double ind_k, ind_radius, ind_start, ind_speed, ind_move_steps;
     int32_t fix_top_bottom;
+    float* vp_shape; /* 9T, model NFX_VISCOPLASTIC with NFX_SCATTER_TILED: the state in shape form
+                        S = Dm^-1 Fp^-1 (replaces Fp and dminv on that path; nfx_vp_shape_from_fp) */
 } nfx_plan;
 
 /* Per-cell parameters of a batch, in that cell's simulation units.  The batched material is the
@@ -161,6 +163,11 @@ int nfx_tetra_corner_forces(nfx_ctx* ctx, const nfx_elastic* mat, const float* p
 int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float* points, const uint32_t* tetras,
                            const float* dminv, int64_t N, int64_t T, float* Fp, float* epsbar, float* von_mises,
                            float* pressure, float* force, float* corners);
+/* The tiled viscoplastic force keeps its state as S = Dm^-1 Fp^-1 (one 9-float tensor instead of
+ * Dm^-1 and Fp; Fe = r S).  Conversions between that form and the reference's Fp
+ * (plasticDeformationTensor, Interactions.py:279-281), 9T SoA FP32 each: */
+int nfx_vp_shape_from_fp(nfx_ctx* ctx, const float* dminv, const float* Fp, int64_t T, float* shape);
+int nfx_vp_fp_from_shape(nfx_ctx* ctx, const float* dminv, const float* shape, int64_t T, float* Fp);
 /* deterministic gather: force[c*N+n] += sum over slots of node n, in slot order */
 int nfx_gather_corners(nfx_ctx* ctx, const uint32_t* csr_offsets, const uint32_t* csr_slots, const float* corners,
                        int64_t N, float* force);

@@ -293,7 +293,24 @@ class Simulation:
         return self._tiles
 
     def plan(self):
-        """The fused-step description of the registered plugin chain, or None if it cannot fuse."""
+        """The fused-step description of the registered plugin chain, or None if it cannot fuse.
+        Plugin state is brought into the form the fused step advances (callers step right after)."""
+        p = self._build_plan()
+        if p is not None:
+            for inter in self.interactions:
+                inter.state_form(shape=p.scatter == _lib.SCATTER_TILED)
+        return p
+
+    def sync_forms(self):
+        """Bring every plugin state representation up to date (before a snapshot of the state)."""
+        for inter in self.interactions:
+            inter.sync_forms()
+
+    def forms_restored(self):
+        for inter in self.interactions:
+            inter.forms_restored()
+
+    def _build_plan(self):
         if not self.fused:
             return None
         key = tuple(id(i) for i in self.interactions) + (self.scatter,)
@@ -315,7 +332,7 @@ class Simulation:
         elif self.scatter == "tiled" and self.node_tiles()["max_nodes"] > self.node_tiles()["tile_k"]:
             # a tile touching more distinct nodes than it has threads: use the CSR gather instead
             self.scatter = "csr"
-            return self.plan()
+            return self._build_plan()
         elif self.scatter == "tiled":
             t = self.node_tiles()
             p.scatter = _lib.SCATTER_TILED
@@ -370,7 +387,7 @@ class Simulation:
         """Device tensors that carry the simulation state (positions, force, plugin state)."""
         out = [self.points.buf, self.force.buf, self.vonMises.buf, self.pressure.buf]
         for inter in self.interactions:
-            for name in ("velocityB", "forceOldB", "plasticDeformationTensorB", "accumulatedStrainB"):
+            for name in ("velocityB", "forceOldB", "plasticDeformationTensorB", "shapeB", "accumulatedStrainB"):
                 t = getattr(inter, name, None)
                 if t is not None:
                     out.append(t)

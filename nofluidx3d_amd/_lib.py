@@ -84,6 +84,7 @@ class Plan(ctypes.Structure):
         ("ind_speed", c_f64),
         ("ind_move_steps", c_f64),
         ("fix_top_bottom", c_i32),
+        ("vp_shape", c_vp),
     ]
 
 
@@ -145,6 +146,8 @@ SIGNATURES = {
     "nfx_viscoplastic_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Viscoplastic), c_vp, c_vp, c_vp, c_i64, c_i64,
                                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_vp_shape_from_fp": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_vp_fp_from_shape": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_build_node_csr_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp]),
     "nfx_build_tiles_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

@@ -78,11 +78,15 @@ def tune_simulation(sim, nrep=20, log=None):
     """Pick ``tile_k`` for a single-cell Simulation with the tiled scatter (mutates ``sim.tile_k``)."""
     import ctypes
 
+    import torch
+
     from nofluidx3d_amd import _lib, openCL
 
     if sim.scatter != "tiled":
         return sim.tile_k, {}
     saved_time = sim.time
+    sim.sync_forms()
+    openCL.getCommandQueue().finish()
     snapshot = [t.clone() for t in sim.state_tensors()]  # timing runs real steps: restore afterwards
     times = {}
     for K in TILE_CANDIDATES:
@@ -99,8 +103,11 @@ def tune_simulation(sim, nrep=20, log=None):
         times[K] = mf.value + mn.value
         if log:
             log(f"tile_k {K}: force {mf.value:.4f} ms, node {mn.value:.4f} ms")
+    openCL.getCommandQueue().finish()
     for t, s0 in zip(sim.state_tensors(), snapshot):
         t.copy_(s0)
+    torch.cuda.synchronize()
+    sim.forms_restored()
     sim.time = saved_time
     best = min(times, key=times.get)
     sim.tile_k = best

@@ -117,8 +117,10 @@ def _mesh_job(args):
         d = np.load(path)
         return d["P"], d["T"]
     P, T = generate_sphere_mesh(R, s, seed)
-    if path:
-        np.savez(path, P=P, T=T)
+    if path:  # write then rename: an interrupted run leaves no truncated cache file
+        tmp = f"{path}.{os.getpid()}.tmp.npz"
+        np.savez(tmp, P=P, T=T)
+        os.replace(tmp, path)
     return P, T
 
 
@@ -297,10 +299,11 @@ class BatchSimulation:
         _lib.call("nfx_stream_sync", hh)
         del ev, vol
         if self.model == _lib.NFX_VISCOPLASTIC:
-            self.Fp = f32(np.repeat(np.identity(3).reshape(9), self.T))
+            # state in shape form S = Dm^-1 Fp^-1 (the tiled force pass's form); S = Dm^-1 at Fp = I
+            self.shape = self.dminv.clone()
             self.eps = torch.zeros(self.T, dtype=torch.float32, device=dev)
         else:
-            self.Fp = self.eps = None
+            self.shape = self.eps = None
         self.t_count, self.t_nodes, self.t_lend = u32(h["tile_count"]), u32(h["tile_nodes"]), u16(h["tile_lend"])
         self.t_slots, self.t_dst = u16(h["tile_slots"]), u32(h["tile_dst"])
         self.t_ntet, self.t_cell = u16(h["tile_ntet"]), u32(h["tile_cell"])
@@ -317,8 +320,8 @@ class BatchSimulation:
         p.points, p.vel, p.fold = self.x.data_ptr(), self.v.data_ptr(), self.fold.data_ptr()
         p.force = 0
         p.model = self.model
-        if self.Fp is not None:
-            p.Fp, p.epsbar = self.Fp.data_ptr(), self.eps.data_ptr()
+        if self.shape is not None:
+            p.vp_shape, p.epsbar = self.shape.data_ptr(), self.eps.data_ptr()
         p.scatter = _lib.SCATTER_TILED
         p.tile_k, p.tile_max_nodes = K, MX
         p.tile_count, p.tile_nodes, p.tile_lend = (t.data_ptr() for t in (self.t_count, self.t_nodes, self.t_lend))
@@ -355,8 +358,7 @@ class BatchSimulation:
         self.fold.zero_()
         ev = torch.from_numpy(np.ascontiguousarray(h["ev"].T.reshape(-1))).to(dev)
         vol = torch.from_numpy(np.ascontiguousarray(h["vol"])).to(dev)
-        if self.Fp is not None:
-            self.Fp.copy_(torch.from_numpy(np.repeat(np.identity(3).reshape(9), self.T).astype(np.float32)))
+        if self.eps is not None:
             self.eps.zero_()
         self.cells.copy_(torch.from_numpy(self.cells_host.view(np.uint8).copy()))
         self.curves.zero_()
@@ -365,6 +367,9 @@ class BatchSimulation:
         _lib.call("nfx_precompute_f64", hh, ev.data_ptr(), vol.data_ptr(), self.T, self.dminv.data_ptr(),
                   self.v0.data_ptr())
         _lib.call("nfx_stream_sync", hh)
+        if self.shape is not None:
+            self.shape.copy_(self.dminv)
+            torch.cuda.synchronize(dev)
         del ev, vol
         self.time = 0
 

@@ -268,7 +268,7 @@ __device__ __forceinline__ Corners svkk_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const 
 
 struct VPOut {
     Corners f;
-    float Fp[9];
+    float Fp[9];  // the updated state: Fp (vp_tetra) or the shape state S (vp_tetra_shape)
     float epsbar;
     float von_mises, pressure;
 };
@@ -281,26 +281,20 @@ struct VPConst {
 // Finite-strain viscoplastic law (InteractionFiniteStrainViscoplastic.cl:57-206) with node 0 as
 // the edge base.  The face-area nodal forces of :110-141 equal -(sgn(det r)/6) sigma c_k for a
 // closed tetra surface (both are 1/3 of sigma times the outward area vector of the opposite face).
-__device__ __forceinline__ VPOut vp_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const float* D, const float* Fp, float epsbar,
-                                          const VPConst& P) {
-    VPOut o;
-    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
-    const M9 F = deformation_gradient(r1, r2, r3, D);
-    M9 Fpm;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) Fpm.m[i][j] = Fp[i * 3 + j];
-    const M9 Fe = mul9(F, inv9(Fpm));
+//
+// vp_stress_flow: from the elastic part Fe, the Mooney-Rivlin stress on Be = Fe Fe^T with
+// J = sqrt(det Be) (:82-107), the corner forces, von Mises stress and pressure (:144-151), the
+// flow rate (:153, optionally with the yield offset) and epsbar += rate (:156).  Returns the
+// plastic stretching Dp = 1.5 rate dev(sigma) / svm (:160-169) as a symmetric tensor.
+__device__ __forceinline__ S6 vp_stress_flow(const M9& Fe, V3 r1, V3 r2, V3 r3, float epsbar, const VPConst& P,
+                                             VPOut& o) {
     const S6 B = aat(Fe);
-    // J = sqrt(det B) (:87)
     const float detB = B.xx * (B.yy * B.zz - B.yz * B.yz) - B.xy * (B.xy * B.zz - B.yz * B.xz) +
                        B.xz * (B.xy * B.yz - B.yy * B.xz);
     const float J = fsqrt(detB);
     const S6 sig = mr_stress(B, J, P.mu1, P.mu2, P.kappa);
     const float detr = dot(r1, cross(r2, r3));
     o.f = cofactor_forces(sig, r1, r2, r3, detr < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
-    // :144-169
     const float tr3 = (sig.xx + sig.yy + sig.zz) * (1.0f / 3.0f);
     S6 dv = sig;
     dv.xx -= tr3;
@@ -320,17 +314,77 @@ __device__ __forceinline__ VPOut vp_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const floa
     }
     o.epsbar = epsbar + rate;
     const float g = (svm != 0.0f) ? rate * 1.5f * frcp(svm) : 0.0f;
-    // Fp += Fe^-1 * (g dev sigma) * F
-    M9 Dp;
-    Dp.m[0][0] = g * dv.xx; Dp.m[1][1] = g * dv.yy; Dp.m[2][2] = g * dv.zz;
-    Dp.m[0][1] = Dp.m[1][0] = g * dv.xy;
-    Dp.m[1][2] = Dp.m[2][1] = g * dv.yz;
-    Dp.m[0][2] = Dp.m[2][0] = g * dv.xz;
-    const M9 dF = mul9(mul9(inv9(Fe), Dp), F);
+    S6 Dp;
+    Dp.xx = g * dv.xx;
+    Dp.yy = g * dv.yy;
+    Dp.zz = g * dv.zz;
+    Dp.xy = g * dv.xy;
+    Dp.yz = g * dv.yz;
+    Dp.xz = g * dv.xz;
+    return Dp;
+}
+
+// State Fp (the reference's plasticDeformationTensor): Fe = F Fp^-1, Fp += Fe^-1 Dp F (:171-206).
+__device__ __forceinline__ VPOut vp_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const float* D, const float* Fp, float epsbar,
+                                          const VPConst& P) {
+    VPOut o;
+    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
+    const M9 F = deformation_gradient(r1, r2, r3, D);
+    M9 Fpm;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Fpm.m[i][j] = Fp[i * 3 + j];
+    const M9 Fe = mul9(F, inv9(Fpm));
+    const S6 Dp = vp_stress_flow(Fe, r1, r2, r3, epsbar, P, o);
+    M9 Dpm;
+    Dpm.m[0][0] = Dp.xx; Dpm.m[1][1] = Dp.yy; Dpm.m[2][2] = Dp.zz;
+    Dpm.m[0][1] = Dpm.m[1][0] = Dp.xy;
+    Dpm.m[1][2] = Dpm.m[2][1] = Dp.yz;
+    Dpm.m[0][2] = Dpm.m[2][0] = Dp.xz;
+    const M9 dF = mul9(mul9(inv9(Fe), Dpm), F);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) o.Fp[i * 3 + j] = Fp[i * 3 + j] + dF.m[i][j];
+    return o;
+}
+
+// The same law on the shape state S = Dm^-1 Fp^-1 (9 floats per tetra, replacing both Dm^-1 and
+// Fp): Fe = r S, one 3x3 product.  The reference's update Fp' = Fp + Fe^-1 Dp F equals
+// Fp (I + F^-1 Dp F), so exactly
+//     S' = Dm^-1 Fp'^-1 = r^-1 (I + Dp)^-1 Fe,   i.e.   S' = S - r^-1 (I + Dp)^-1 Dp Fe,
+// with r^-1 = [c1; c2; c3] / det r (the cofactor rows the forces use).  The increment form is
+// exactly zero when nothing flows, so S carries no rounding drift between flow events.
+__device__ __forceinline__ VPOut vp_tetra_shape(V3 p0, V3 p1, V3 p2, V3 p3, const float* S, float epsbar,
+                                                const VPConst& P) {
+    VPOut o;
+    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
+    const M9 Fe = deformation_gradient(r1, r2, r3, S);
+    const S6 Dp = vp_stress_flow(Fe, r1, r2, r3, epsbar, P, o);
+    // Ai = (I + Dp)^-1 (symmetric, by cofactors)
+    const float axx = 1.0f + Dp.xx, ayy = 1.0f + Dp.yy, azz = 1.0f + Dp.zz;
+    const float cxx = ayy * azz - Dp.yz * Dp.yz, cyy = axx * azz - Dp.xz * Dp.xz, czz = axx * ayy - Dp.xy * Dp.xy;
+    const float cxy = Dp.xz * Dp.yz - Dp.xy * azz, cyz = Dp.xy * Dp.xz - axx * Dp.yz, cxz = Dp.xy * Dp.yz - ayy * Dp.xz;
+    const float ia = frcp(axx * cxx + Dp.xy * cxy + Dp.xz * cxz);
+    const S6 Ai{cxx * ia, cyy * ia, czz * ia, cxy * ia, cyz * ia, cxz * ia};
+    // Nm = Ai Dp (symmetric in exact arithmetic: Ai and Dp commute), full 3x3 here
+    M9 Nm;
+    const V3 d0 = mk(Dp.xx, Dp.xy, Dp.xz), d1 = mk(Dp.xy, Dp.yy, Dp.yz), d2 = mk(Dp.xz, Dp.yz, Dp.zz);
+    const V3 n0 = smul(Ai, d0), n1 = smul(Ai, d1), n2 = smul(Ai, d2);  // columns of Ai Dp
+    Nm.m[0][0] = n0.x; Nm.m[1][0] = n0.y; Nm.m[2][0] = n0.z;
+    Nm.m[0][1] = n1.x; Nm.m[1][1] = n1.y; Nm.m[2][1] = n1.z;
+    Nm.m[0][2] = n2.x; Nm.m[1][2] = n2.y; Nm.m[2][2] = n2.z;
+    const M9 W = mul9(Nm, Fe);
+    const V3 c1 = cross(r2, r3), c2 = cross(r3, r1), c3 = cross(r1, r2);
+    const float idet = -frcp(dot(r1, c1));
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const V3 w = mk(W.m[0][j], W.m[1][j], W.m[2][j]);
+        o.Fp[0 * 3 + j] = S[0 * 3 + j] + dot(c1, w) * idet;
+        o.Fp[1 * 3 + j] = S[1 * 3 + j] + dot(c2, w) * idet;
+        o.Fp[2 * 3 + j] = S[2 * 3 + j] + dot(c3, w) * idet;
+    }
     return o;
 }
 

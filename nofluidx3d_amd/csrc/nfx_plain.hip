@@ -192,12 +192,34 @@ __global__ __launch_bounds__(kBlock) void k_vp(TetArgs a, VPConst P, float* __re
     }
 }
 
+// ---- viscoplastic state conversion: shape state S = Dm^-1 Fp^-1 <-> Fp = S^-1 Dm^-1 -------------
+template <bool TO_SHAPE>
+__global__ __launch_bounds__(kBlock) void k_vp_convert(const float* __restrict__ dminv, const float* __restrict__ in,
+                                                       int64_t T, float* __restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    M9 D, X;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            D.m[i][j] = dminv[(3 * i + j) * T + t];
+            X.m[i][j] = in[(3 * i + j) * T + t];
+        }
+    const M9 Y = TO_SHAPE ? mul9(D, inv9(X)) : mul9(inv9(X), D);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[(3 * i + j) * T + t] = Y.m[i][j];
+}
+
 // ---- force pass, tiled deterministic scatter ---------------------------------------------------
 // One workgroup per tile of K consecutive (Morton-ordered) tetra, one thread per tetra.  The
 // tile's distinct nodes are listed on the host (tile_nodes, ascending, MX entries per tile) with,
 // for each, the run of the tile's corner slots that touch it (tile_slots grouped by node; tile_lend
 // = run ends) and the position of its partial sum in node order (tile_dst).  Phases:
-//   0  issue the tetra's streaming loads (Dm^-1, V0 or Fp, epsbar) and the tile metadata
+//   0  issue the tetra's streaming loads (Dm^-1 and V0, or the viscoplastic shape state and
+//      epsbar) and the tile metadata
 //   1  slot list and node positions -> LDS
 //   2  invert the slot runs into per-corner local node ids (conn)
 //   3  material law on LDS positions, corner forces -> LDS
@@ -227,7 +249,7 @@ __host__ __device__ constexpr size_t tile_lds_bytes_fixed() {
 }
 
 template <int MODEL, int K, bool STRESS>
-__global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Fp,
+__global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
     // one thread per tetra of the tile (blockDim.x == K)
@@ -260,17 +282,17 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     const bool active = tl < nt;
     const uint32_t u = (uint32_t)tl;
 
-    // issue this tetra's streaming loads first: they overlap the staging phases below
-    const float* __restrict__ Dt = a.dminv + t0;
-    float D[9], fp[9], e0 = 0.f, v0 = 0.f;
+    // issue this tetra's streaming loads first: they overlap the staging phases below.  The
+    // viscoplastic law reads its shape state S = Dm^-1 Fp^-1 in place of Dm^-1 and Fp.
+    float D[9], e0 = 0.f, v0 = 0.f;
     if (active) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k) D[k] = (Dt + k * T)[u];
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) fp[k] = (Fp + t0 + k * T)[u];
+            for (int k = 0; k < 9; ++k) D[k] = (Sst + t0 + k * T)[u];
             e0 = (eps + t0)[u];
         } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) D[k] = (a.dminv + t0 + k * T)[u];
             v0 = (a.v0 + t0)[u];
         }
     }
@@ -312,9 +334,9 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
         const V3 p3 = mk(s_x[l3], s_x[MX + l3], s_x[2 * MX + l3]);
         Corners f;
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
-            const VPOut o = vp_tetra(p0, p1, p2, p3, D, fp, e0, P);
+            const VPOut o = vp_tetra_shape(p0, p1, p2, p3, D, e0, P);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) (Fp + t0 + k * T)[u] = o.Fp[k];
+            for (int k = 0; k < 9; ++k) (Sst + t0 + k * T)[u] = o.Fp[k];
             (eps + t0)[u] = o.epsbar;
             if constexpr (STRESS) {
                 (vm + t0)[u] = o.von_mises;
@@ -637,7 +659,11 @@ static int validate_plan(const nfx_plan* p) {
     NFX_REQUIRE(p->tetras && p->dminv && p->points && p->vel && p->fold, "plan: NULL mesh/state pointer");
     NFX_REQUIRE(p->model >= NFX_SVKK && p->model <= NFX_VISCOPLASTIC, "plan: unknown model %d", p->model);
     NFX_REQUIRE(p->model == NFX_VISCOPLASTIC || p->v0, "plan: NULL v0");
-    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || (p->Fp && p->epsbar), "plan: viscoplastic state missing");
+    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->epsbar, "plan: viscoplastic state missing");
+    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->scatter == NFX_SCATTER_TILED || p->Fp,
+                "plan: viscoplastic state Fp missing");
+    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->scatter != NFX_SCATTER_TILED || p->vp_shape,
+                "plan: the tiled viscoplastic force needs the shape state vp_shape");
     NFX_REQUIRE(p->scatter >= NFX_SCATTER_ATOMIC && p->scatter <= NFX_SCATTER_TILED, "plan: unknown scatter %d",
                 p->scatter);
     if (p->scatter == NFX_SCATTER_TILED) {
@@ -680,9 +706,11 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
         case NFX_SVKK: k_tile_force<NFX_SVKK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
         default:
             if (st)
-                k_tile_force<NFX_VISCOPLASTIC, K, true><<<nb, K, lds, s>>>(a, e, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
+                k_tile_force<NFX_VISCOPLASTIC, K, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar, p.von_mises,
+                                                                           p.pressure);
             else
-                k_tile_force<NFX_VISCOPLASTIC, K, false><<<nb, K, lds, s>>>(a, e, c, p.Fp, p.epsbar, nullptr, nullptr);
+                k_tile_force<NFX_VISCOPLASTIC, K, false><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar, nullptr,
+                                                                            nullptr);
     }
 }
 
@@ -1111,6 +1139,22 @@ int nfx_viscoplastic_force(nfx_ctx* c, const nfx_viscoplastic* vp, const float* 
     p.pressure = pressure;
     p.scatter = corners ? NFX_SCATTER_CSR : NFX_SCATTER_ATOMIC;
     return launch_force_pass(c, p, true);
+}
+
+int nfx_vp_shape_from_fp(nfx_ctx* c, const float* dminv, const float* Fp, int64_t T, float* shape) {
+    NFX_REQUIRE(c && dminv && Fp && shape, "NULL argument");
+    NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
+    k_vp_convert<true><<<nblocks(T), kBlock, 0, c->stream>>>(dminv, Fp, T, shape);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_vp_fp_from_shape(nfx_ctx* c, const float* dminv, const float* shape, int64_t T, float* Fp) {
+    NFX_REQUIRE(c && dminv && Fp && shape, "NULL argument");
+    NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
+    k_vp_convert<false><<<nblocks(T), kBlock, 0, c->stream>>>(dminv, shape, T, Fp);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
 }
 
 int nfx_gather_corners(nfx_ctx* c, const uint32_t* off, const uint32_t* slots, const float* corners, int64_t N,

@@ -9,6 +9,9 @@ the legacy wrapper's eight per-tetra arrays are (``Interactions.py:270-277``).  
 
 State per tetra: the plastic deformation gradient Fp (9 SoA, starts at I) and the accumulated
 plastic strain; ``vonMises`` and ``pressure`` are written when the plugin's ``write_stress`` is on.
+The tiled fused step keeps the same state in shape form S = Dm^-1 Fp^-1 (``shapeB``), which
+replaces both Dm^-1 and Fp in its force pass; ``state_form`` converts between the two forms when
+the stepping path changes, and ``plastic_deformation()`` returns an up-to-date Fp.
 """
 from pathlib import Path
 
@@ -54,6 +57,47 @@ class FiniteStrainViscoplastic(Interaction):
         # identity in the SoA layout: entry k of tetra t at k*T + t (Interactions.py:279-281)
         fp = np.repeat(np.identity(3).reshape(9), T).astype(np.float32)
         self.plasticDeformationTensorB = torch.from_numpy(fp).to(dev)
+        self.shapeB = self.dminvB.clone()  # S = Dm^-1 while Fp = I
+        torch.cuda.synchronize(dev)
+        self._current = {"fp", "shape"}
+
+    def state_form(self, shape: bool):
+        """Make Fp (shape=False) or S (shape=True) current before steps that advance it; after
+        them only that form is current."""
+        want = "shape" if shape else "fp"
+        if want not in self._current:
+            self._convert(want)
+        self._current = {want}
+
+    def _convert(self, want):
+        from nofluidx3d_amd import openCL
+
+        T = self.simulation.numTetra
+        if want == "shape":
+            _lib.call("nfx_vp_shape_from_fp", openCL.handle(), self.dminvB.data_ptr(),
+                      self.plasticDeformationTensorB.data_ptr(), T, self.shapeB.data_ptr())
+        else:
+            _lib.call("nfx_vp_fp_from_shape", openCL.handle(), self.dminvB.data_ptr(), self.shapeB.data_ptr(), T,
+                      self.plasticDeformationTensorB.data_ptr())
+        self._current.add(want)
+
+    def sync_forms(self):
+        """Bring both forms up to date (no steps in between)."""
+        for want in ("fp", "shape"):
+            if want not in self._current:
+                self._convert(want)
+
+    def forms_restored(self):
+        """Both forms were just restored from one snapshot taken after ``sync_forms``."""
+        self._current = {"fp", "shape"}
+
+    def plastic_deformation(self):
+        """The plastic deformation gradient Fp (9T SoA device tensor), up to date."""
+        from nofluidx3d_amd import openCL
+
+        self.sync_forms()
+        openCL.getCommandQueue().finish()
+        return self.plasticDeformationTensorB
 
     def build(self):
         self.scope = self.simulation.numTetra
@@ -70,6 +114,7 @@ class FiniteStrainViscoplastic(Interaction):
 
     def enqueue(self):
         sim = self.simulation
+        self.state_form(shape=False)
         if getattr(sim, "scatter", "atomic") in ("csr", "tiled"):
             from nofluidx3d_amd import openCL
 
@@ -93,5 +138,6 @@ class FiniteStrainViscoplastic(Interaction):
         plan.dminv = self.dminvB.data_ptr()
         plan.v0 = self.volumesB.data_ptr()
         plan.Fp = self.plasticDeformationTensorB.data_ptr()
+        plan.vp_shape = self.shapeB.data_ptr()
         plan.epsbar = self.accumulatedStrainB.data_ptr()
         return True

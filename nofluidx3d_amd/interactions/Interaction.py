@@ -38,3 +38,13 @@ class Interaction:
 
     def contribute(self, plan) -> bool:
         return False
+
+    def state_form(self, shape: bool):
+        """Called before steps: ``shape`` is True when the tiled fused step will advance the
+        plugin's state (plugins with a second state representation convert here)."""
+
+    def sync_forms(self):
+        """Bring every state representation up to date (before a snapshot of the state)."""
+
+    def forms_restored(self):
+        """Every state representation was just restored from a snapshot taken after sync_forms."""

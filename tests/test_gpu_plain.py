@@ -218,6 +218,28 @@ def test_viscoplastic_kernel_parity(ctx):
     assert acc_r.max() > 0  # flow happened
 
 
+def test_vp_shape_state_conversions(ctx):
+    """S = Dm^-1 Fp^-1 (the tiled viscoplastic state) and back, against FP64 numpy on random
+    plastic deformations.  Tolerances: FP32 products and cofactor inverses (cond(Dm) <= ~30)."""
+    import torch
+
+    P, T = small_mesh()
+    nT = len(T)
+    dminv, _, _, _ = gpu_precompute(ctx, P, T)
+    D = host(dminv).reshape(9, -1).T.reshape(-1, 3, 3)
+    Fp = (np.eye(3) + 0.05 * RNG.normal(size=(nT, 3, 3))).astype(np.float32).astype(np.float64)
+    Fpd = dev(Fp.reshape(nT, 9).T.reshape(-1), np.float32)
+    S = torch.empty(9 * nT, dtype=torch.float32, device="cuda:0")
+    call("nfx_vp_shape_from_fp", ctx, dminv.data_ptr(), Fpd.data_ptr(), nT, S.data_ptr())
+    Sg = host(S).reshape(9, -1).T.reshape(-1, 3, 3)
+    Sref = D @ np.linalg.inv(Fp)
+    assert np.max(np.abs(Sg - Sref).max(axis=(1, 2)) / np.abs(Sref).max(axis=(1, 2))) < 2e-6
+    back = torch.empty_like(S)
+    call("nfx_vp_fp_from_shape", ctx, dminv.data_ptr(), S.data_ptr(), nT, back.data_ptr())
+    Fb = host(back).reshape(9, -1).T.reshape(-1, 3, 3)
+    assert np.max(np.abs(Fb - Fp)) < 5e-5
+
+
 # ------------------------------------------------------------------------------------------------
 # fused multi-step runs through the Simulation API
 # ------------------------------------------------------------------------------------------------
@@ -320,6 +342,45 @@ def test_viscoplastic_fused_run_matches_oracle(ctx, scatter):
     of = ref.run(600)
     assert ref.acc.max() > 0
     assert_parity(sim, gx, gf, ref.positions(), of)
+
+
+@pytest.mark.parametrize("scatter", ["csr", "tiled"])
+def test_viscoplastic_state_matches_oracle(ctx, scatter):
+    """Per-tetra state after 600 steps: Fp (read back from the shape form on the tiled path) and
+    the accumulated plastic strain, against the FP64 oracle's Fp and epsbar."""
+    from nofluidx3d_amd.interactions import FiniteStrainViscoplastic
+
+    sim = build_sim(material="VP", scatter=scatter)
+    gx, gf = run_gpu(sim, 600)
+    ref = oracle_for(sim)
+    of = ref.run(600)
+    vp = next(i for i in sim.interactions if isinstance(i, FiniteStrainViscoplastic))
+    Fp = host(vp.plastic_deformation())
+    acc = host(vp.accumulatedStrainB)
+    assert np.max(np.abs(Fp - np.eye(3).reshape(9).repeat(len(acc)))) > 1e-4  # plastic flow happened
+    assert np.max(np.abs(Fp - ref.Fp)) < 1e-4
+    assert np.max(np.abs(acc - ref.acc)) < 1e-3 * np.max(ref.acc)
+    assert_parity(sim, gx, gf, ref.positions(), of)
+
+
+def test_viscoplastic_switching_between_fused_and_per_plugin_steps(ctx):
+    """Tiled fused steps (shape-form state), then per-plugin steps (Fp form), then fused again:
+    the state follows each switch and the run matches the oracle."""
+    import torch
+
+    sim = build_sim(material="VP", scatter="tiled")
+    hist = torch.zeros(600, dtype=torch.float64, device="cuda:0")
+    sim.advance(300, tip_hist=hist[:300])
+    sim.fused = False
+    for _ in range(30):
+        sim.timeStep()
+    sim.fused = True
+    sim.advance(270, tip_hist=hist[330:])
+    ref = oracle_for(sim)
+    of = ref.run(600)
+    gf = host(hist)
+    keep = np.r_[0:300, 330:600]
+    assert_parity(sim, sim.positions(), gf[keep], ref.positions(), of[keep])
 
 
 @pytest.mark.parametrize("scatter", ["csr", "tiled"])

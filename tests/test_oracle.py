@@ -339,6 +339,32 @@ def test_vp_maxwell_relaxation_time():
     assert measured == pytest.approx(tau, rel=0.02)
 
 
+def test_vp_shape_form_update_is_the_reference_update():
+    """The tiled GPU path keeps S = Dm^-1 Fp^-1 and updates it as S' = S - r^-1 (I + Dp)^-1 Dp Fe
+    with Fe = r S.  Recovering Dp from the oracle's own update (Fp' - Fp = Fe^-1 Dp F), S' must
+    equal Dm^-1 Fp'^-1: the identity the GPU update rests on."""
+    P = vp_params(rate=5e-2)
+    I = np.eye(3)
+    for _ in range(5):
+        Fp0 = I + 0.05 * RNG.normal(size=(3, 3))
+        x = deform(random_F(0.2), REF_TET) + 0.02 * RNG.normal(size=(4, 3))
+        _, acc, Fp1, _, _ = vp_step(P, REF_TET, x, Fp=O.soa(Fp0.reshape(1, 9)).copy())
+        Fp1 = Fp1.reshape(3, 3)
+        Dm = (REF_TET[1:] - REF_TET[0]).T  # columns R1, R2, R3
+        r = (x[1:] - x[0]).T
+        F = r @ np.linalg.inv(Dm)
+        Fe = F @ np.linalg.inv(Fp0)
+        Dp = Fe @ (Fp1 - Fp0) @ np.linalg.inv(F)
+        nd = np.abs(Dp).max()
+        assert acc[0] > 0 and nd > 1e-6
+        assert np.max(np.abs(Dp - Dp.T)) < 1e-9 * nd and abs(np.trace(Dp)) < 1e-9 * nd  # plastic stretching
+        S = np.linalg.inv(Dm) @ np.linalg.inv(Fp0)
+        assert np.allclose(r @ S, Fe, rtol=0, atol=1e-13)
+        S1 = S - np.linalg.inv(r) @ np.linalg.inv(I + Dp) @ Dp @ Fe
+        ref = np.linalg.inv(Dm) @ np.linalg.inv(Fp1)
+        assert np.max(np.abs(S1 - ref)) < 1e-11 * np.abs(ref).max()
+
+
 # --------------------------------------------------------------------------------------------
 # contacts, integrator, tip force
 # --------------------------------------------------------------------------------------------
