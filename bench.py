@@ -16,7 +16,10 @@ Printed JSON line (one):
   launch from rocprofv3 FETCH_SIZE + WRITE_SIZE when ``--traffic-json`` names a file holding them;
 - cpu_baseline: the FP64 C oracle (OpenMP, all allowed host threads) on a bounded step sample
   of the same cell;
-- plain_small: the small cell (R 8 um, 0.8 um lattice, ~25k tetra) fused step, for comparison.
+- plain_small: the small cell (R 8 um, 0.8 um lattice, ~25k tetra) fused step, for comparison;
+- batched: the 512-cell workload (10k steps each, 256-point curve per cell);
+- surrogate: MLP inference / training-step throughput on the MFMA kernels and a surrogate-assisted
+  fit of 512 new cells confirmed by the plain simulator (when trained weights are present).
 
 ``--gpus N`` with N > 1 exits: this build runs on one GPU.
 """
@@ -198,6 +201,54 @@ def bench_batched(steps=10000, ncells=512, log=log):
     }
 
 
+def bench_surrogate(weights, n_targets=512, log=log):
+    """The learned part: MLP inference and training-step throughput on the MFMA kernels, and a
+    surrogate-assisted fit of ``n_targets`` new cells (batched workload generator, seeds 7000+i,
+    1% noise) confirmed by the plain simulator."""
+    import torch
+
+    from nofluidx3d_amd.surrogate import fit, fit_bench, train
+
+    net, norm = train.load(weights)
+    sizes = net.sizes
+    flops_row = 2 * sum(a * b for a, b in zip(net.in_pad, sizes[1:]))
+    rows = 65536
+    X = np.random.default_rng(0).normal(size=(rows, sizes[0])).astype(np.float32)
+    net.set_input(X)
+    net.forward_dev(rows)
+    net.sync()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        net.forward_dev(rows)
+    net.sync()
+    t_inf = (time.perf_counter() - t0) / 10
+    B = 4096
+    net.set_input(X[:B], train=True)
+    tgt = torch.zeros(B, sizes[-1], dtype=torch.float32, device="cuda:0")
+    snap = net.state()
+    net.train_step(B, tgt)
+    net.sync()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        net.train_step(B, tgt)
+    net.sync()
+    t_train = (time.perf_counter() - t0) / 20
+    net.load_state(snap)
+    out = {"weights": os.path.relpath(weights, ROOT), "sizes": list(sizes),
+           "inference_rows_per_s": rows / t_inf, "inference_tflops": flops_row * rows / t_inf / 1e12,
+           "train_step_ms_batch4096": t_train * 1e3, "train_tflops": 3 * flops_row * B / t_train / 1e12,
+           "mfma_peak_tflops_bf16_dense": 2500.0}
+    cells, meshes, bs, clean, noisy = fit_bench.make_targets(n_targets)
+    r = fit.fit_surrogate(net, norm, noisy, fit_bench.hidden_blind(cells), bs, ntop=2)
+    out["fit"] = {"targets": n_targets, "fits_per_s": n_targets / r["total_s"], "batch_wall_s": r["total_s"],
+                  "simulations_per_target": r["simulations_per_target"],
+                  "confirmed_rel_l2": fit_bench.err_stats(r["sim_err"]),
+                  "noise_floor_rel_l2_median": float(np.median(fit.rel_err(clean, noisy))),
+                  "param_errors": fit_bench.param_errors(r["theta"], fit_bench.theta_true(cells))}
+    del bs
+    return out
+
+
 def stream_copy_gbs(nbytes=1 << 30, nrep=20):
     """Measured copy bandwidth (read + write bytes / time) with the library's vectorized copy."""
     import ctypes
@@ -237,6 +288,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 512-cell measurement")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
     ap.add_argument("--tile-k", type=int, default=0, choices=[0, 256, 512], help="0: autotune")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_large_cell.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
@@ -354,6 +406,9 @@ def main():
         out["batched"] = bench_batched()
         if "plain_small" in out:
             out["batched"]["x_plain_small"] = out["batched"]["value"] / out["plain_small"]["value"]
+    weights = os.path.join(ROOT, "nofluidx3d_amd", "surrogate", "weights", "surrogate.npz")
+    if not args.no_extra and not args.no_surrogate and os.path.exists(weights):
+        out["surrogate"] = bench_surrogate(weights)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(P, T, cfg, material)
         out["cpu_baseline"]["hbm_equiv_gbs"] = out["cpu_baseline"]["value"] / nT * step_bytes / 1e9

@@ -42,6 +42,11 @@ extern "C" {
 #define NFX_SVK 1
 #define NFX_MR 2
 #define NFX_VISCOPLASTIC 3
+/* legacy stress laws (SimulationLegacy.py:696-840; Interactions.py:101-247, 352-408), node-0 base */
+#define NFX_LINEAR_ELASTIC 4     /* InteractionLinearElastic.cl: E, nu */
+#define NFX_LINEAR_DEVIATORIC 5  /* InteractionLinearElasticDeviatoric.cl: shear, bulk */
+#define NFX_NEO_HOOKEAN_2 6      /* InteractionSecondOrderNeoHookean.cl: shear, shear2, bulk */
+#define NFX_PORO_NEO_HOOKEAN 7   /* InteractionPoroNeoHookean.cl: shear, volume_fraction */
 
 #define NFX_SCATTER_ATOMIC 0 /* float atomics, as the reference's atomicAdd (atomicAdd.cl:1-11) */
 #define NFX_SCATTER_CSR 1    /* deterministic node<-corner gather, fixed summation order */
@@ -53,10 +58,12 @@ extern "C" {
 
 typedef struct nfx_ctx nfx_ctx;
 
-/* Elastic constants in simulation units: the #defines of interactions/Tetra.py:66-79. */
+/* Elastic constants in simulation units: the #defines of interactions/Tetra.py:66-79, and the
+ * per-tetra parameter arrays of the legacy laws (uniform here). */
 typedef struct {
-    int32_t model; /* NFX_SVKK, NFX_SVK or NFX_MR */
+    int32_t model; /* NFX_SVKK, NFX_SVK, NFX_MR or a legacy law NFX_LINEAR_ELASTIC..NFX_PORO_NEO_HOOKEAN */
     double E, nu, ratio;
+    double shear, bulk, shear2, volume_fraction; /* legacy laws */
 } nfx_elastic;
 
 /* Finite-strain viscoplastic parameters (uniform per tetra; Interactions.py:270-277 builds eight
@@ -81,12 +88,14 @@ typedef struct {
     float* vel;             /* 3N */
     float* force;           /* 3N  (atomic scatter target; cleared by the node update) */
     float* fold;            /* 3N */
-    int32_t model;          /* NFX_SVKK / NFX_SVK / NFX_MR / NFX_VISCOPLASTIC */
+    int32_t model;          /* NFX_SVKK / NFX_SVK / NFX_MR / NFX_VISCOPLASTIC / legacy laws */
     double E, nu, ratio;
     nfx_viscoplastic vp;
     float* Fp;        /* 9T, model NFX_VISCOPLASTIC */
     float* epsbar;    /* T */
-    float* von_mises; /* T, nullable: written by the viscoplastic force when non-null */
+    float* von_mises; /* T, nullable: when set (with pressure), nfx_run_steps writes the per-tetra von
+                         Mises stress and pressure of the last step of the call (viscoplastic and
+                         legacy laws) */
     float* pressure;  /* T, nullable */
     int32_t scatter;  /* NFX_SCATTER_ATOMIC or NFX_SCATTER_CSR */
     const uint32_t* csr_offsets; /* N+1, incidences of node n at [off[n], off[n+1]) */
@@ -108,6 +117,7 @@ typedef struct {
     int32_t fix_top_bottom;
     float* vp_shape; /* 9T, model NFX_VISCOPLASTIC with NFX_SCATTER_TILED: the state in shape form
                         S = Dm^-1 Fp^-1 (replaces Fp and dminv on that path; nfx_vp_shape_from_fp) */
+    double shear, bulk, shear2, volume_fraction; /* legacy laws (as in nfx_elastic) */
 } nfx_plan;
 
 /* Per-cell parameters of a batch, in that cell's simulation units.  The batched material is the
@@ -158,6 +168,13 @@ int nfx_tetra_force(nfx_ctx* ctx, const nfx_elastic* mat, const float* points, c
 /* same forces written per corner (no scatter): corners[12*t + 3*a + c] */
 int nfx_tetra_corner_forces(nfx_ctx* ctx, const nfx_elastic* mat, const float* points, const uint32_t* tetras,
                             const float* dminv, const float* v0, int64_t N, int64_t T, float* corners);
+/* the legacy laws' kernels (InteractionLinearElastic.cl, InteractionLinearElasticDeviatoric.cl,
+ * InteractionSecondOrderNeoHookean.cl, InteractionPoroNeoHookean.cl: Cauchy stress, face-area forces,
+ * vonMises and pressure), node-0 edge base; atomic scatter when corners is NULL, per-corner output
+ * otherwise; von_mises/pressure nullable */
+int nfx_legacy_force(nfx_ctx* ctx, const nfx_elastic* mat, const float* points, const uint32_t* tetras,
+                     const float* dminv, int64_t N, int64_t T, float* von_mises, float* pressure, float* force,
+                     float* corners);
 /* InteractionFiniteStrainViscoplastic.cl:1-220 (node-0 edge base); atomic scatter when corners
  * is NULL, per-corner output otherwise */
 int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float* points, const uint32_t* tetras,
@@ -247,10 +264,11 @@ typedef struct {
 } nfx_mlp_gemm_args;
 
 int nfx_mlp_gemm(nfx_ctx* ctx, const nfx_mlp_gemm_args* args);
-/* mean squared error over M x N FP32 (accumulated into *loss_sum, FP64, device) and its gradient
- * 2 (Y - T) / (M N) as BF16 in [M x N] (dZ) and [N x M] (dZT) */
-int nfx_mlp_mse(nfx_ctx* ctx, const float* Y, const float* target, int32_t M, int32_t N, void* dZ, void* dZT,
-                double* loss_sum);
+/* row-weighted mean squared error sum_ij w_i (Y - T)^2 / (M N) over M x N FP32 (accumulated into
+ * *loss_sum, FP64, device) and its gradient 2 w_i (Y - T) / (M N) as BF16 in [M x N] (dZ) and
+ * [N x M] (dZT).  row_weight (M FP32) is nullable: w_i = 1. */
+int nfx_mlp_mse(nfx_ctx* ctx, const float* Y, const float* target, const float* row_weight, int32_t M, int32_t N,
+                void* dZ, void* dZT, double* loss_sum);
 /* bias gradient = row sums of dZT [N x M] (BF16), then Adam on the FP32 bias b with moments m, v */
 int nfx_mlp_bias_adam(nfx_ctx* ctx, const void* dZT, int32_t M, int32_t N, float* b, float* m, float* v, float lr_t,
                       float bc2, float beta1, float beta2, float eps, float grad_scale);

@@ -30,7 +30,9 @@ import time
 import numpy as np
 
 from nofluidx3d_amd import _lib, openCL
-from nofluidx3d_amd.interactions import FiniteStrainViscoplastic, Plane, Sphere, Substrate, Tetra, VelocityVerlet
+from nofluidx3d_amd.interactions import (FiniteStrainViscoplastic, LegacyElastic, LinearElastic,
+                                        LinearElasticDeviatoric, Plane, PoroNeoHookean, SecondOrderNeoHookean,
+                                        Sphere, Substrate, Tetra, VelocityVerlet)
 from nofluidx3d_amd.openCL import getCommandQueue, initializeOpenCLObjects
 from nofluidx3d_amd.TetraCell import TetraCell
 from nofluidx3d_amd.util.Bridge import Bridge
@@ -221,6 +223,33 @@ class Simulation:
                                                yieldStress, flowRate, hardeningThreshhold, hardeningExponent,
                                                yieldOffset))
 
+    # legacy stress laws (SimulationLegacy.py:696-840); ReynoldScaling and YoungsScaling default to 1
+    def _legacy_youngs(self):
+        c = self.parameters["CELL"]
+        return c["YoungsModulusSI"] / self.parameters.get("ReynoldScaling", 1.0) / c.get("YoungsScaling", 1.0) / self.p0
+
+    def setInteractionLinearElastic(self):
+        self.register(LinearElastic(self.cell, self._legacy_youngs(), self.parameters["CELL"]["PoissonRatio"]))
+
+    def setInteractionLinearPoroElastic(self):
+        c = self.parameters["CELL"]
+        E, nu = self._legacy_youngs(), c["PoissonRatio"]
+        shear = E / (2 * (1 + nu))
+        bulk = (1 - c["PoroElasticFactor"]) * E / (3 * (1 - 2 * nu))
+        self.register(LinearElasticDeviatoric(self.cell, shear, bulk))
+
+    def setInteractionSecondOrderNeoHookean(self):
+        c = self.parameters["CELL"]
+        E, nu = self._legacy_youngs(), c["PoissonRatio"]
+        shear = E / (2 * (1 + nu))
+        self.register(SecondOrderNeoHookean(self.cell, shear, shear * c["SecondOrderNeoHookeanFactor"],
+                                            E / (3 * (1 - 2 * nu))))
+
+    def setInteractionPoroNeoHookean(self):
+        c = self.parameters["CELL"]
+        shear = self._legacy_youngs() / (2 * (1 + 0.5))
+        self.register(PoroNeoHookean(self.cell, shear, c["PorousVolumeFraction"]))
+
     # Velocity Verlet Interaction, always add time integration interactions as last element
     def setInteractionVelocityVerlet(self, fixTopBottom=False):
         self.register(VelocityVerlet(self.numPoints, fixTopBottom))
@@ -405,7 +434,7 @@ class Simulation:
         self._print("\n\n")
         self._print(f"VTKInterval: {self.VTKInterval}")
         for inter in self.interactions:
-            if isinstance(inter, FiniteStrainViscoplastic) and (writeStress or writePressure):
+            if isinstance(inter, (FiniteStrainViscoplastic, LegacyElastic)) and (writeStress or writePressure):
                 inter.write_stress = True
                 inter.setArgs()
                 self._plan = None

@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnfx.so")
 
 NFX_SVKK, NFX_SVK, NFX_MR, NFX_VISCOPLASTIC = 0, 1, 2, 3
+NFX_LINEAR_ELASTIC, NFX_LINEAR_DEVIATORIC, NFX_NEO_HOOKEAN_2, NFX_PORO_NEO_HOOKEAN = 4, 5, 6, 7
 SCATTER_ATOMIC, SCATTER_CSR, SCATTER_TILED = 0, 1, 2
 INDENTER_NONE, INDENTER_SPHERE, INDENTER_PLANE = 0, 1, 2
 
@@ -24,7 +25,8 @@ class NfxError(RuntimeError):
 
 
 class Elastic(ctypes.Structure):
-    _fields_ = [("model", c_i32), ("E", c_f64), ("nu", c_f64), ("ratio", c_f64)]
+    _fields_ = [("model", c_i32), ("E", c_f64), ("nu", c_f64), ("ratio", c_f64), ("shear", c_f64), ("bulk", c_f64),
+                ("shear2", c_f64), ("volume_fraction", c_f64)]
 
 
 class Viscoplastic(ctypes.Structure):
@@ -85,6 +87,10 @@ class Plan(ctypes.Structure):
         ("ind_move_steps", c_f64),
         ("fix_top_bottom", c_i32),
         ("vp_shape", c_vp),
+        ("shear", c_f64),
+        ("bulk", c_f64),
+        ("shear2", c_f64),
+        ("volume_fraction", c_f64),
     ]
 
 
@@ -147,6 +153,8 @@ SIGNATURES = {
                                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_vp_shape_from_fp": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_legacy_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Elastic), c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp,
+                                         c_vp, c_vp]),
     "nfx_vp_fp_from_shape": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_build_node_csr_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp]),
     "nfx_build_tiles_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
@@ -161,7 +169,7 @@ SIGNATURES = {
     "nfx_time_passes": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i32, ctypes.POINTER(c_f64),
                                        ctypes.POINTER(c_f64)]),
     "nfx_mlp_gemm": (ctypes.c_int, [c_vp, ctypes.POINTER(MlpGemm)]),
-    "nfx_mlp_mse": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp]),
+    "nfx_mlp_mse": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp]),
     "nfx_mlp_bias_adam": (ctypes.c_int, [c_vp, c_vp, c_i32, c_i32, c_vp, c_vp, c_vp, ctypes.c_float, ctypes.c_float,
                                          ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     "nfx_stream_copy": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i32, ctypes.POINTER(c_f64)]),

@@ -73,6 +73,16 @@ class CellSpec:
         return G * self.units(steps)["T0"] / self.viscosity_si
 
 
+def surrogate_inputs(E_si, nu, viscosity_si, tip_radius_um, speed_um_s, radius_um):
+    """The six surrogate inputs (n, 6): log10 E [Pa], nu, log10 viscosity [Pa s], log10 tip radius
+    [um], log10 speed [um/s], log10 cell radius [um].  In logs, the dimensionless groups of the
+    problem (viscosity/E, tip/cell radius, the Deborah number viscosity*speed/(E*radius)) are
+    linear combinations of the inputs."""
+    return np.stack([np.log10(np.asarray(E_si, np.float64)), np.asarray(nu, np.float64),
+                     np.log10(np.asarray(viscosity_si, np.float64)), np.log10(np.asarray(tip_radius_um, np.float64)),
+                     np.log10(np.asarray(speed_um_s, np.float64)), np.log10(np.asarray(radius_um, np.float64))], 1)
+
+
 def estimated_tetra(radius_um, spacing_um):
     """~6 tetra per lattice node for the seeded generator (quality filter included)."""
     m = max(int(round(radius_um / spacing_um)), 2)
@@ -374,10 +384,10 @@ class BatchSimulation:
         self.time = 0
 
     def features(self):
-        """Surrogate inputs per cell: log10 E, nu, log10 viscosity, tip radius [um], log10 speed,
-        cell radius [um]."""
-        return np.array([[math.log10(s.E_si), s.nu, math.log10(s.viscosity_si or 1.0), s.tip_radius_um,
-                          math.log10(s.speed_um_s), s.radius_um] for s in self.specs])
+        """Surrogate inputs of every cell (``surrogate_inputs``)."""
+        return surrogate_inputs([s.E_si for s in self.specs], [s.nu for s in self.specs],
+                                [s.viscosity_si or 1.0 for s in self.specs], [s.tip_radius_um for s in self.specs],
+                                [s.speed_um_s for s in self.specs], [s.radius_um for s in self.specs])
 
     def advance(self, nsteps):
         from nofluidx3d_amd import openCL

@@ -266,6 +266,70 @@ __device__ __forceinline__ Corners svkk_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const 
     return dmf_rows(D, inner, F, -v0 * mu);
 }
 
+// ---- legacy stress laws (kernels/InteractionLinearElastic.cl, InteractionLinearElasticDeviatoric.cl,
+// InteractionSecondOrderNeoHookean.cl, InteractionPoroNeoHookean.cl) with node 0 as the edge base.
+// Each gives a Cauchy stress from F = r D; the nodal forces are the kernels' face-area forces,
+// -(sgn(det r)/6) sigma c_k (orientation free, as the outward normals there are).
+struct LegacyConst {
+    float mu, lam;    // linear elastic: 2 mu eps + lam tr(eps) I
+    float K;          // deviatoric and second-order laws: bulk modulus
+    float mu2;        // second-order coefficient
+    float phi;        // porous volume fraction
+};
+
+template <int LAW>  // 4 linear elastic, 5 linear deviatoric, 6 second-order NH, 7 poro-NH
+__device__ __forceinline__ Corners legacy_tetra(V3 p0, V3 p1, V3 p2, V3 p3, const float* D, const LegacyConst& L,
+                                                float& von_mises, float& pressure) {
+    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
+    const M9 F = deformation_gradient(r1, r2, r3, D);
+    S6 sig;
+    if constexpr (LAW == 4 || LAW == 5) {
+        // small strain eps = sym(F) - I  (InteractionLinearElastic.cl:60-62)
+        const S6 e{F.m[0][0] - 1.0f, F.m[1][1] - 1.0f, F.m[2][2] - 1.0f, 0.5f * (F.m[0][1] + F.m[1][0]),
+                   0.5f * (F.m[1][2] + F.m[2][1]), 0.5f * (F.m[0][2] + F.m[2][0])};
+        const float tr = e.xx + e.yy + e.zz;
+        if constexpr (LAW == 4) {
+            const float a = 2.0f * L.mu, b = L.lam * tr;
+            sig = S6{a * e.xx + b, a * e.yy + b, a * e.zz + b, a * e.xy, a * e.yz, a * e.xz};
+        } else {
+            const float a = 2.0f * L.mu, t3 = tr * (1.0f / 3.0f), b = L.K * tr;
+            sig = S6{a * (e.xx - t3) + b, a * (e.yy - t3) + b, a * (e.zz - t3) + b, a * e.xy, a * e.yz, a * e.xz};
+        }
+    } else {
+        const S6 B = aat(F);
+        const float detB = B.xx * (B.yy * B.zz - B.yz * B.yz) - B.xy * (B.xy * B.zz - B.yz * B.xz) +
+                           B.xz * (B.xy * B.yz - B.yy * B.xz);
+        const float J = fsqrt(detB);
+        const float trB = B.xx + B.yy + B.zz;
+        const float t = fexp2(flog2(J) * (-1.0f / 3.0f));  // J^-1/3
+        if constexpr (LAW == 6) {
+            // a = J^-5/3 (mu + mu2 (trB J^-2/3 - 3)) on dev B, b = K (J - 1)
+            const float t2 = t * t;
+            const float a = t2 * t2 * t * (L.mu + L.mu2 * (trB * t2 - 3.0f));
+            const float b = L.K * (J - 1.0f);
+            const float d3 = trB * (1.0f / 3.0f);
+            sig = S6{a * (B.xx - d3) + b, a * (B.yy - d3) + b, a * (B.zz - d3) + b, a * B.xy, a * B.yz, a * B.xz};
+        } else {
+            // Danielsson-Parks-Boyce porous Neo-Hookean (InteractionPoroNeoHookean.cl:72-78)
+            const float phi = L.phi;
+            const float eta = 1.0f + (J - 1.0f) * frcp(phi);
+            const float ie3 = fexp2(flog2(eta) * (-1.0f / 3.0f));  // eta^-1/3
+            const float iJ = frcp(J);
+            const float t2 = t * t;
+            const float a = 0.5f * L.mu * (4.0f * iJ - 2.0f * iJ * iJ - 2.0f * (phi + 2.0f * (J - 1.0f)) * t2 * t2 * t * ie3);
+            const float q = ((4.0f - phi) * eta + (1.0f - phi)) * frcp(phi * eta * eta + (1.0f - phi) * eta);
+            const float b = 0.5f * L.mu * trB * (iJ * iJ - (1.0f / 3.0f) * t2 * ie3 * q);
+            sig = S6{a * B.xx + b, a * B.yy + b, a * B.zz + b, a * B.xy, a * B.yz, a * B.xz};
+        }
+    }
+    const float tr3 = (sig.xx + sig.yy + sig.zz) * (1.0f / 3.0f);
+    const float dx = sig.xx - tr3, dy = sig.yy - tr3, dz = sig.zz - tr3;
+    von_mises = fsqrt(1.5f * (dx * dx + dy * dy + dz * dz + 2.0f * (sig.xy * sig.xy + sig.yz * sig.yz + sig.xz * sig.xz)));
+    pressure = tr3;
+    const float detr = dot(r1, cross(r2, r3));
+    return cofactor_forces(sig, r1, r2, r3, detr < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
+}
+
 struct VPOut {
     Corners f;
     float Fp[9];  // the updated state: Fp (vp_tetra) or the shape state S (vp_tetra_shape)

@@ -98,17 +98,20 @@ __global__ __launch_bounds__(256) void k_gemm_nt(nfx_mlp_gemm_args g) {
             }
 }
 
-// loss = mean((Y - T)^2) over M x N; dZ = 2 (Y - T) / (M N) as BF16 in [M x N] and [N x M]
-__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Y, const float* __restrict__ Tg, int32_t M,
-                                             int32_t N, __bf16* dZ, __bf16* dZT, double* loss_sum) {
+// loss = sum_ij w_i (Y - T)^2 / (M N) (w_i = 1 without row weights); dZ = 2 w_i (Y - T) / (M N) as
+// BF16 in [M x N] and [N x M]
+__global__ __launch_bounds__(256) void k_mse(const float* __restrict__ Y, const float* __restrict__ Tg,
+                                             const float* __restrict__ w, int32_t M, int32_t N, __bf16* dZ,
+                                             __bf16* dZT, double* loss_sum) {
     const int64_t n = (int64_t)M * N;
     const float sc = 2.0f / (float)n;
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        const float d = Y[i] - Tg[i];
-        s += (double)d * d;
         const int row = (int)(i / N), col = (int)(i % N);
-        const __bf16 b = to_bf16(sc * d);
+        const float wr = w ? w[row] : 1.0f;
+        const float d = Y[i] - Tg[i];
+        s += (double)wr * d * d;
+        const __bf16 b = to_bf16(sc * wr * d);
         dZ[i] = b;
         dZT[(int64_t)col * M + row] = b;
     }
@@ -172,12 +175,12 @@ int nfx_mlp_gemm(nfx_ctx* c, const nfx_mlp_gemm_args* g) {
     return NFX_OK;
 }
 
-int nfx_mlp_mse(nfx_ctx* c, const float* Y, const float* target, int32_t M, int32_t N, void* dZ, void* dZT,
-                double* loss_sum) {
+int nfx_mlp_mse(nfx_ctx* c, const float* Y, const float* target, const float* row_weight, int32_t M, int32_t N,
+                void* dZ, void* dZT, double* loss_sum) {
     NFX_REQUIRE(c && Y && target && dZ && dZT && loss_sum && M > 0 && N > 0, "bad argument");
     const int64_t n = (int64_t)M * N;
     const unsigned int nb = (unsigned int)std::min<int64_t>((n + 255) / 256, 4096);
-    k_mse<<<nb, 256, 0, c->stream>>>(Y, target, M, N, (__bf16*)dZ, (__bf16*)dZT, loss_sum);
+    k_mse<<<nb, 256, 0, c->stream>>>(Y, target, row_weight, M, N, (__bf16*)dZ, (__bf16*)dZT, loss_sum);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }

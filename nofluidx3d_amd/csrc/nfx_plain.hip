@@ -49,14 +49,20 @@ struct TetArgs {
     const float* __restrict__ x;
     float* force;
     float* corners;
+    float* vm;  // legacy laws: per-tetra von Mises stress and pressure (nullable)
+    float* pr;
 };
 
 struct ElasticDev {
     float mu1, mu2, kappa;  // MR
     float mu, kappa_svk;    // SVK / SVKK (kappa = nu / (1 - 2 nu))
+    LegacyConst L;          // legacy laws
 };
 
-static ElasticDev elastic_consts(double E, double nu, double ratio) {
+static bool is_legacy(int model) { return model >= NFX_LINEAR_ELASTIC && model <= NFX_PORO_NEO_HOOKEAN; }
+
+static ElasticDev elastic_consts(const nfx_plan& p) {
+    const double E = p.E, nu = p.nu, ratio = p.ratio;
     const double mu = E / (2 * (1 + nu));  // tetra.cl:118-121 (FP64 on the host)
     ElasticDev e;
     e.mu1 = (float)(ratio * mu);
@@ -64,6 +70,12 @@ static ElasticDev elastic_consts(double E, double nu, double ratio) {
     e.kappa = (float)(E / (3 * (1 - 2 * nu)));
     e.mu = (float)mu;
     e.kappa_svk = (float)(nu / (1 - 2 * nu));
+    // InteractionLinearElastic.cl:66-67: a = 2E/(2+2nu) = 2 mu, b = E nu/((1+nu)(1-2nu)) = lambda
+    e.L.mu = (float)(p.model == NFX_LINEAR_ELASTIC ? mu : p.shear);
+    e.L.lam = (float)(E * nu / ((1 + nu) * (1 - 2 * nu)));
+    e.L.K = (float)p.bulk;
+    e.L.mu2 = (float)p.shear2;
+    e.L.phi = (float)p.volume_fraction;
     return e;
 }
 
@@ -140,15 +152,24 @@ __global__ __launch_bounds__(kBlock) void k_tetra(TetArgs a, ElasticDev e) {
     float D[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) D[k] = a.dminv[k * T + t];
-    const float v0 = a.v0[t];
     const V3 p0 = gather3(a.x, N, n0), p1 = gather3(a.x, N, n1), p2 = gather3(a.x, N, n2), p3 = gather3(a.x, N, n3);
     Corners c;
-    if constexpr (MODEL == NFX_MR)
-        c = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
-    else if constexpr (MODEL == NFX_SVK)
-        c = svk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
-    else
-        c = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+    if constexpr (MODEL >= NFX_LINEAR_ELASTIC) {
+        float vm, pr;
+        c = legacy_tetra<MODEL>(p0, p1, p2, p3, D, e.L, vm, pr);
+        if (a.vm) {
+            a.vm[t] = vm;
+            a.pr[t] = pr;
+        }
+    } else {
+        const float v0 = a.v0[t];
+        if constexpr (MODEL == NFX_MR)
+            c = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
+        else if constexpr (MODEL == NFX_SVK)
+            c = svk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+        else
+            c = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+    }
     if constexpr (CORNERS) {
         store_corners(a.corners, t, c);
     } else {
@@ -293,7 +314,7 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
         } else {
 #pragma unroll
             for (int k = 0; k < 9; ++k) D[k] = (a.dminv + t0 + k * T)[u];
-            v0 = (a.v0 + t0)[u];
+            if constexpr (MODEL < NFX_VISCOPLASTIC) v0 = (a.v0 + t0)[u];
         }
     }
 
@@ -343,6 +364,13 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
                 (pr + t0)[u] = o.pressure;
             }
             f = o.f;
+        } else if constexpr (MODEL >= NFX_LINEAR_ELASTIC) {
+            float svm, spr;
+            f = legacy_tetra<MODEL>(p0, p1, p2, p3, D, e.L, svm, spr);
+            if constexpr (STRESS) {
+                (vm + t0)[u] = svm;
+                (pr + t0)[u] = spr;
+            }
         } else if constexpr (MODEL == NFX_MR) {
             f = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
         } else if constexpr (MODEL == NFX_SVK) {
@@ -626,7 +654,7 @@ static int check_sizes(int64_t N, int64_t T) {
 }
 
 static TetArgs tet_args(const nfx_plan& p) {
-    return TetArgs{p.N, p.T, p.tetras, p.dminv, p.v0, p.points, p.force, p.corners};
+    return TetArgs{p.N, p.T, p.tetras, p.dminv, p.v0, p.points, p.force, p.corners, nullptr, nullptr};
 }
 
 static NodeArgs node_args(const nfx_plan& p) {
@@ -657,8 +685,8 @@ static int validate_plan(const nfx_plan* p) {
     int rc = check_sizes(p->N, p->T);
     if (rc) return rc;
     NFX_REQUIRE(p->tetras && p->dminv && p->points && p->vel && p->fold, "plan: NULL mesh/state pointer");
-    NFX_REQUIRE(p->model >= NFX_SVKK && p->model <= NFX_VISCOPLASTIC, "plan: unknown model %d", p->model);
-    NFX_REQUIRE(p->model == NFX_VISCOPLASTIC || p->v0, "plan: NULL v0");
+    NFX_REQUIRE(p->model >= NFX_SVKK && p->model <= NFX_PORO_NEO_HOOKEAN, "plan: unknown model %d", p->model);
+    NFX_REQUIRE(p->model > NFX_MR || p->v0, "plan: NULL v0");
     NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->epsbar, "plan: viscoplastic state missing");
     NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->scatter == NFX_SCATTER_TILED || p->Fp,
                 "plan: viscoplastic state Fp missing");
@@ -684,6 +712,15 @@ static int validate_plan(const nfx_plan* p) {
     return NFX_OK;
 }
 
+template <int MODEL, int K>
+static void launch_tile_legacy(unsigned int nb, size_t lds, hipStream_t s, const TileArgs& a, const ElasticDev& e,
+                               const VPConst& c, float* vm, float* pr) {
+    if (vm && pr)
+        k_tile_force<MODEL, K, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, vm, pr);
+    else
+        k_tile_force<MODEL, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
+}
+
 template <int K>
 static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     // ctx->batch is set only while a batched step is being enqueued
@@ -697,13 +734,19 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     }
     const unsigned int nb = (unsigned int)((p.T + K - 1) / K);
     const size_t lds = tile_lds_bytes_fixed<K>() + 12 * (size_t)p.tile_max_nodes;
-    const ElasticDev e = elastic_consts(p.E, p.nu, p.ratio);
+    const ElasticDev e = elastic_consts(p);
     const VPConst c = vp_consts(p.vp);
     hipStream_t s = ctx->stream;
+    float* vm = st ? p.von_mises : nullptr;
+    float* pr = st ? p.pressure : nullptr;
     switch (p.model) {
         case NFX_MR: k_tile_force<NFX_MR, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
         case NFX_SVK: k_tile_force<NFX_SVK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
         case NFX_SVKK: k_tile_force<NFX_SVKK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_LINEAR_ELASTIC: launch_tile_legacy<NFX_LINEAR_ELASTIC, K>(nb, lds, s, a, e, c, vm, pr); break;
+        case NFX_LINEAR_DEVIATORIC: launch_tile_legacy<NFX_LINEAR_DEVIATORIC, K>(nb, lds, s, a, e, c, vm, pr); break;
+        case NFX_NEO_HOOKEAN_2: launch_tile_legacy<NFX_NEO_HOOKEAN_2, K>(nb, lds, s, a, e, c, vm, pr); break;
+        case NFX_PORO_NEO_HOOKEAN: launch_tile_legacy<NFX_PORO_NEO_HOOKEAN, K>(nb, lds, s, a, e, c, vm, pr); break;
         default:
             if (st)
                 k_tile_force<NFX_VISCOPLASTIC, K, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar, p.von_mises,
@@ -734,14 +777,26 @@ static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
         else if (st) k_vp<false, true><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
         else k_vp<false, false><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, nullptr, nullptr);
     } else {
-        const ElasticDev e = elastic_consts(p.E, p.nu, p.ratio);
+        const ElasticDev e = elastic_consts(p);
+        if (is_legacy(p.model) && stress && p.von_mises && p.pressure) {
+            a.vm = p.von_mises;
+            a.pr = p.pressure;
+        }
         switch (p.model * 2 + (csr ? 1 : 0)) {
             case NFX_MR * 2: k_tetra<NFX_MR, false><<<nb, kBlock, 0, s>>>(a, e); break;
             case NFX_MR * 2 + 1: k_tetra<NFX_MR, true><<<nb, kBlock, 0, s>>>(a, e); break;
             case NFX_SVK * 2: k_tetra<NFX_SVK, false><<<nb, kBlock, 0, s>>>(a, e); break;
             case NFX_SVK * 2 + 1: k_tetra<NFX_SVK, true><<<nb, kBlock, 0, s>>>(a, e); break;
             case NFX_SVKK * 2: k_tetra<NFX_SVKK, false><<<nb, kBlock, 0, s>>>(a, e); break;
-            default: k_tetra<NFX_SVKK, true><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_SVKK * 2 + 1: k_tetra<NFX_SVKK, true><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_LINEAR_ELASTIC * 2: k_tetra<NFX_LINEAR_ELASTIC, false><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_LINEAR_ELASTIC * 2 + 1: k_tetra<NFX_LINEAR_ELASTIC, true><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_LINEAR_DEVIATORIC * 2: k_tetra<NFX_LINEAR_DEVIATORIC, false><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_LINEAR_DEVIATORIC * 2 + 1: k_tetra<NFX_LINEAR_DEVIATORIC, true><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_NEO_HOOKEAN_2 * 2: k_tetra<NFX_NEO_HOOKEAN_2, false><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_NEO_HOOKEAN_2 * 2 + 1: k_tetra<NFX_NEO_HOOKEAN_2, true><<<nb, kBlock, 0, s>>>(a, e); break;
+            case NFX_PORO_NEO_HOOKEAN * 2: k_tetra<NFX_PORO_NEO_HOOKEAN, false><<<nb, kBlock, 0, s>>>(a, e); break;
+            default: k_tetra<NFX_PORO_NEO_HOOKEAN, true><<<nb, kBlock, 0, s>>>(a, e); break;
         }
     }
     NFX_CHECK_LAUNCH();
@@ -1105,6 +1160,35 @@ static int tetra_common(nfx_ctx* c, const nfx_elastic* m, const float* points, c
     return launch_force_pass(c, p, false);
 }
 
+int nfx_legacy_force(nfx_ctx* c, const nfx_elastic* m, const float* points, const uint32_t* tetras,
+                     const float* dminv, int64_t N, int64_t T, float* von_mises, float* pressure, float* force,
+                     float* corners) {
+    NFX_REQUIRE(c && m && points && tetras && dminv && (force || corners), "NULL argument");
+    NFX_REQUIRE(is_legacy(m->model), "not a legacy law: model %d", m->model);
+    int rc = check_sizes(N, T);
+    if (rc) return rc;
+    nfx_plan p{};
+    p.N = N;
+    p.T = T;
+    p.tetras = tetras;
+    p.dminv = dminv;
+    p.points = const_cast<float*>(points);
+    p.force = force;
+    p.corners = corners;
+    p.model = m->model;
+    p.E = m->E;
+    p.nu = m->nu;
+    p.ratio = m->ratio;
+    p.shear = m->shear;
+    p.bulk = m->bulk;
+    p.shear2 = m->shear2;
+    p.volume_fraction = m->volume_fraction;
+    p.von_mises = von_mises;
+    p.pressure = pressure;
+    p.scatter = corners ? NFX_SCATTER_CSR : NFX_SCATTER_ATOMIC;
+    return launch_force_pass(c, p, true);
+}
+
 int nfx_tetra_force(nfx_ctx* c, const nfx_elastic* m, const float* points, const uint32_t* tetras, const float* dminv,
                     const float* v0, int64_t N, int64_t T, float* force) {
     NFX_REQUIRE(force, "force is NULL");
@@ -1316,15 +1400,19 @@ int nfx_run_steps(nfx_ctx* c, const nfx_plan* p, int64_t step0, int64_t nsteps, 
     NFX_REQUIRE(nsteps >= 0, "nsteps < 0");
     NFX_REQUIRE(!tip_hist || p->indenter != NFX_INDENTER_NONE, "tip force history needs an indenter");
     int64_t done = 0;
-    if (use_graph && nsteps >= 2) {
-        const int64_t B = std::min<int64_t>(nsteps, kGraphBlock);
+    // with stress outputs in the plan, the last step of the call is enqueued eagerly and writes
+    // the per-tetra von Mises stress and pressure (the state a frame written afterwards shows)
+    const bool stress = p->von_mises && p->pressure;
+    const int64_t ngraph = stress ? nsteps - 1 : nsteps;
+    if (use_graph && ngraph >= 2) {
+        const int64_t B = std::min<int64_t>(ngraph, kGraphBlock);
         const bool hist = tip_hist != nullptr;
         if (!c->graph_exec || c->graph_steps != B || c->graph_hist != hist ||
             std::memcmp(&c->graph_plan, p, sizeof(nfx_plan)) != 0) {
             rc = capture_graph(c, *p, B, hist);
             if (rc) return rc;
         }
-        while (nsteps - done >= B) {
+        while (ngraph - done >= B) {
             k_set_i64<<<1, 1, 0, c->stream>>>(c->step_dev, step0 + done);
             NFX_CHECK_LAUNCH();
             NFX_HIP(hipGraphLaunch(c->graph_exec, c->stream));
@@ -1335,7 +1423,7 @@ int nfx_run_steps(nfx_ctx* c, const nfx_plan* p, int64_t step0, int64_t nsteps, 
         }
     }
     for (; done < nsteps; ++done) {
-        rc = launch_force_pass(c, *p, false);
+        rc = launch_force_pass(c, *p, stress && done == nsteps - 1);
         if (rc) return rc;
         rc = launch_node_pass(c, *p, step0 + done, nullptr, 0, tip_hist ? tip_hist + done : nullptr, nullptr);
         if (rc) return rc;

@@ -140,4 +140,6 @@ class FiniteStrainViscoplastic(Interaction):
         plan.Fp = self.plasticDeformationTensorB.data_ptr()
         plan.vp_shape = self.shapeB.data_ptr()
         plan.epsbar = self.accumulatedStrainB.data_ptr()
+        if self.write_stress:
+            plan.von_mises, plan.pressure = self.simulation.vonMises.ptr, self.simulation.pressure.ptr
         return True

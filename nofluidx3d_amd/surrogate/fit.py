@@ -2,31 +2,36 @@
 
 A target is a measured curve F(delta) (FP64, newtons, 256 samples) of a cell whose geometry and
 protocol are known (mesh, cell radius, tip radius, speed, depth).  The surrogate predicts
-F / (E R^2), so a candidate (E, nu, eta) predicts F = E R^2 f(features).
+F / (E R^2), so a candidate (E, nu, eta) predicts F = E R^2 f(inputs).
+
+The simulator has an exact symmetry: (E, eta) -> (a E, a eta) leaves every simulation-unit
+quantity of the cell unchanged, so the simulated curve becomes exactly a F.  Hence for any curve
+shape the best amplitude is a closed form, a = <F, F_t> / <F, F>, and the search runs over the
+shape parameters (nu, log10 eta/E) only.
 
 Surrogate-assisted fit (``fit_surrogate``), for all targets at once:
-1. ``rounds`` rounds of 64 candidates per target, each round one batched inference launch over
-   targets x 64 rows: round 0 samples the prior box (log10 E in [2, 4], nu in [0.3, 0.49],
-   log10 eta in [0, 3]); later rounds sample a Gaussian fitted to the 8 best candidates so far
-   (cross-entropy method), clipped to the box;
-2. the 4 best distinct candidates per target are simulated with the batched simulator (the
-   target's own mesh and protocol) and the best simulated curve is the fit -- the plain simulation
-   judges it.
-Fallback / baseline (``fit_nelder_mead``): Nelder-Mead on (log10 E, nu, log10 eta) with every
-evaluation a simulation (all targets advance in lock-step, one batched simulation per evaluation).
+1. ``rounds`` cross-entropy rounds of ``ncand`` candidates (nu, log10 eta/E) per target, each
+   round one batched inference over targets x ncand rows; each candidate's E comes from the
+   closed-form amplitude against the target (clipped to the sampled E and viscosity ranges);
+2. the ``ntop`` best distinct candidates per target are simulated with the batched simulator on
+   the targets' own meshes and protocols; each simulated curve gets its closed-form amplitude and
+   the best becomes the fit;
+3. the fit's parameters are simulated once more: that plain simulation is the confirmed curve and
+   its relative L2 distance to the target the reported error.
+Baseline (``fit_nelder_mead``): simulation-only Nelder-Mead on (log10 E, nu, log10 eta), all
+targets in lock-step, one batched simulation per evaluation.
 """
 
 from __future__ import annotations
 
 import dataclasses
-import math
 import time
 
 import numpy as np
 
-from nofluidx3d_amd.batch import CellSpec
+from nofluidx3d_amd.batch import surrogate_inputs
 
-LO = np.array([2.0, 0.3, 0.0])
+LO = np.array([2.0, 0.3, 0.0])  # log10 E [Pa], nu, log10 viscosity [Pa s]
 HI = np.array([4.0, 0.49, 3.0])
 
 
@@ -38,8 +43,8 @@ def theta_to_spec(theta, base):
 
 def features(theta, bases):
     """surrogate inputs for parameter rows theta (n, 3) of cells ``bases`` (n)"""
-    return np.stack([theta[:, 0], theta[:, 1], theta[:, 2], [b.tip_radius_um for b in bases],
-                     [math.log10(b.speed_um_s) for b in bases], [b.radius_um for b in bases]], 1)
+    return surrogate_inputs(10 ** theta[:, 0], theta[:, 1], 10 ** theta[:, 2], [b.tip_radius_um for b in bases],
+                            [b.speed_um_s for b in bases], [b.radius_um for b in bases])
 
 
 def curve_force(curve_norm, spec):
@@ -55,36 +60,87 @@ def simulate(bs, specs):
     return np.stack([curve_force(cn[i], s) for i, s in enumerate(specs)])
 
 
+def rel_err(F, targets):
+    return np.linalg.norm(F - targets, axis=-1) / np.linalg.norm(targets, axis=-1)
+
+
+def amplitude(F, targets):
+    """closed-form best scale a of curves F (..., n) against targets (..., n)"""
+    return np.sum(F * targets, axis=-1) / np.maximum(np.sum(F * F, axis=-1), 1e-300)
+
+
+def clip_theta(theta):
+    """Keep log10 E and log10 viscosity inside the box by a common shift (the symmetry direction),
+    then clip every coordinate."""
+    th = np.array(theta, np.float64, copy=True)
+    lo = np.maximum(LO[0] - th[..., 0], LO[2] - th[..., 2])
+    hi = np.minimum(HI[0] - th[..., 0], HI[2] - th[..., 2])
+    shift = np.clip(0.0, lo, np.maximum(lo, hi))
+    th[..., 0] += shift
+    th[..., 2] += shift
+    return np.clip(th, LO, HI)
+
+
+def time_units(bases, steps):
+    """T0 [s per step] of each cell (independent of its material)"""
+    return np.array([b.units(steps)["T0"] for b in bases])
+
+
+def stabilize(theta, T0, max_relax=0.1):
+    """Raise the viscosity where the explicit Maxwell update would remove more than ``max_relax``
+    of the deviatoric stress per step (G T0 / eta > max_relax: unstable flow update), the limit
+    the training data were drawn under.  ``T0``: time unit of each row's cell."""
+    th = np.array(theta, np.float64, copy=True)
+    lim = th[..., 0] + np.log10(np.asarray(T0) / (2.0 * (1.0 + th[..., 1]) * max_relax))
+    th[..., 2] = np.maximum(th[..., 2], lim)
+    return th
+
+
 def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, seed=0, log=None):
     """targets (n, 256) newtons; bases (n) CellSpec with the known geometry/protocol; ``bs`` a
     BatchSimulation over the targets' meshes.  Returns dict with the fitted parameters, the
-    simulated curve of the fit and timings."""
+    confirming simulation's curve and error, and timings."""
     n = len(targets)
     rng = np.random.Generator(np.random.PCG64(seed))
-    scale = np.array([curve_force(1.0, b) / b.E_si for b in bases])  # R^2 per target
+    R2 = np.array([(b.radius_um * 1e-6) ** 2 for b in bases])
+    slo, shi = np.array([LO[1], LO[2] - HI[0]]), np.array([HI[1], HI[2] - LO[0]])  # nu, log10 eta/E
+    rep = [b for b in bases for _ in range(ncand)]
+    T0 = time_units(bases, bs.steps)
+    T0r = np.repeat(T0, ncand)
     t0 = time.perf_counter()
     all_theta = np.zeros((n, 0, 3))
     all_err = np.zeros((n, 0))
     infer_s = 0.0
     for r in range(rounds):
         if r == 0:
-            th = LO + (HI - LO) * rng.random((n, ncand, 3))
+            sh = slo + (shi - slo) * rng.random((n, ncand, 2))
         else:
             order = np.argsort(all_err, axis=1)[:, :8]
             elite = np.take_along_axis(all_theta, order[:, :, None], 1)
-            mu, sd = elite.mean(1), elite.std(1) + 0.02 * (HI - LO)
-            th = np.clip(mu[:, None, :] + sd[:, None, :] * rng.normal(size=(n, ncand, 3)), LO, HI)
-        rows = th.reshape(-1, 3)
-        feats = features(rows, [b for b in bases for _ in range(ncand)])
-        ti = time.perf_counter()
-        pred = net.predict(norm.x(feats)) * norm.y_scale  # one batched inference launch chain
-        infer_s += time.perf_counter() - ti
-        F = pred.reshape(n, ncand, -1) * (10 ** th[:, :, :1]) * scale[:, None, None]
-        err = np.linalg.norm(F - targets[:, None, :], axis=2) / np.linalg.norm(targets, axis=1)[:, None]
+            es = np.stack([elite[..., 1], elite[..., 2] - elite[..., 0]], -1)
+            mu, sd = es.mean(1), es.std(1) + 0.02 * (shi - slo)
+            sh = np.clip(mu[:, None, :] + sd[:, None, :] * rng.normal(size=(n, ncand, 2)), slo, shi)
+        # E from the amplitude of the predicted shape at a reference E (the network is E-invariant
+        # up to its error, so one pass at the reference E and one at the amplitude-corrected E)
+        th = np.zeros((n, ncand, 3))
+        th[..., 1] = sh[..., 0]
+        th[..., 0] = 3.0
+        th[..., 2] = th[..., 0] + sh[..., 1]
+        th = stabilize(clip_theta(th).reshape(-1, 3), T0r).reshape(n, ncand, 3)
+        for _ in range(2):
+            rows = th.reshape(-1, 3)
+            ti = time.perf_counter()
+            pred = net.predict(norm.x(features(rows, rep))) * norm.y_scale
+            infer_s += time.perf_counter() - ti
+            F = pred.reshape(n, ncand, -1) * (10 ** th[:, :, :1]) * R2[:, None, None]
+            a = np.clip(amplitude(F, targets[:, None, :]), 1e-6, 1e6)
+            th[..., 0] += np.log10(a)
+            th[..., 2] += np.log10(a)
+            th = stabilize(clip_theta(th).reshape(-1, 3), T0r).reshape(n, ncand, 3)
+        err = rel_err(F * a[..., None], targets[:, None, :])
         all_theta = np.concatenate([all_theta, th], 1)
         all_err = np.concatenate([all_err, err], 1)
     t_sur = time.perf_counter() - t0
-    # 4 best distinct candidates per target -> plain batched simulation
     order = np.argsort(all_err, axis=1)
     top = np.zeros((n, ntop, 3))
     for i in range(n):
@@ -95,39 +151,60 @@ def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, see
                 picked.append(cand)
             if len(picked) == ntop:
                 break
+        while len(picked) < ntop:
+            picked.append(picked[-1])
         top[i] = np.array(picked)
-    sims = np.zeros((ntop, n, targets.shape[1]))
     t1 = time.perf_counter()
+    best_err = np.full(n, np.inf)
+    top = np.stack([stabilize(top[:, k], T0) for k in range(ntop)], 1)
+    theta = top[:, 0].copy()
     for k in range(ntop):
-        sims[k] = simulate(bs, [theta_to_spec(top[i, k], bases[i]) for i in range(n)])
+        F = simulate(bs, [theta_to_spec(top[i, k], bases[i]) for i in range(n)])
+        a = amplitude(F, targets)
+        e = rel_err(F * a[:, None], targets)
+        cand = stabilize(clip_theta(top[:, k] + np.log10(np.clip(a, 1e-6, 1e6))[:, None] * np.array([1.0, 0.0, 1.0])),
+                         T0)
+        better = e < best_err
+        best_err[better] = e[better]
+        theta[better] = cand[better]
         if log:
             log(f"refine {k + 1}/{ntop}: {time.perf_counter() - t1:.1f}s")
-    t_sim = time.perf_counter() - t1
-    serr = np.linalg.norm(sims - targets[None], axis=2) / np.linalg.norm(targets, axis=1)[None]
-    best = np.argmin(serr, axis=0)
-    theta = top[np.arange(n), best]
-    return {"theta": theta, "curve": sims[best, np.arange(n)], "sim_err": serr[best, np.arange(n)],
-            "surrogate_s": t_sur, "inference_s": infer_s, "simulation_s": t_sim,
-            "total_s": time.perf_counter() - t0, "candidates_per_target": rounds * ncand,
-            "simulations_per_target": ntop}
+    t_ref = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    confirmed = simulate(bs, [theta_to_spec(theta[i], bases[i]) for i in range(n)])
+    t_conf = time.perf_counter() - t2
+    return {"theta": theta, "curve": confirmed, "sim_err": rel_err(confirmed, targets),
+            "predicted_refined_err": best_err, "surrogate_s": t_sur, "inference_s": infer_s,
+            "refine_s": t_ref, "confirm_s": t_conf, "total_s": time.perf_counter() - t0,
+            "candidates_per_target": rounds * ncand, "simulations_per_target": ntop + 1}
 
 
 def fit_nelder_mead(targets, bases, bs, max_evals=60, log=None):
-    """Simulation-only Nelder-Mead (standard coefficients 1, 2, 0.5, 0.5) on (log10 E, nu, log10 eta)
-    in box-normalised coordinates, all targets in lock-step: every evaluation is one batched
-    simulation of all targets."""
+    """Simulation-only Nelder-Mead (coefficients 1, 2, 0.5, 0.5) on (log10 E, nu, log10 eta) in
+    box-normalised coordinates, all targets in lock-step: every evaluation is one batched
+    simulation of all targets.  ``history`` holds the best error of every target after each
+    evaluation."""
     n = len(targets)
-    tn = np.linalg.norm(targets, axis=1)
+    T0 = time_units(bases, bs.steps)
 
     def evaluate(u):  # u (n, 3) in [0, 1]^3
-        th = LO + (HI - LO) * np.clip(u, 0, 1)
+        th = stabilize(LO + (HI - LO) * np.clip(u, 0, 1), T0)
         F = simulate(bs, [theta_to_spec(th[i], bases[i]) for i in range(n)])
-        return np.linalg.norm(F - targets, axis=1) / tn
+        return rel_err(F, targets)
 
     t0 = time.perf_counter()
+    hist = []
+
+    def record(f):
+        hist.append(np.min(f, axis=1) if f.ndim == 2 else f)
+
     simplex = np.array([[0.5, 0.5, 0.5], [0.8, 0.5, 0.5], [0.5, 0.8, 0.5], [0.5, 0.5, 0.8]])
     S = np.repeat(simplex[None], n, 0)  # (n, 4, 3)
-    f = np.stack([evaluate(S[:, j]) for j in range(4)], 1)
+    cols = []
+    for j in range(4):
+        cols.append(evaluate(S[:, j]))
+        record(np.stack(cols, 1))
+    f = np.stack(cols, 1)
     evals = 4
     while evals + 2 <= max_evals:
         o = np.argsort(f, 1)
@@ -136,19 +213,25 @@ def fit_nelder_mead(targets, bases, bs, max_evals=60, log=None):
         c = S[:, :3].mean(1)
         xr = c + (c - S[:, 3])
         fr = evaluate(xr)
+        record(np.minimum(f.min(1), fr))
         expand = fr < f[:, 0]
         # one more evaluation per target: the expansion point where it may pay, else the contraction
         x2 = np.where(expand[:, None], c + 2 * (c - S[:, 3]), c + 0.5 * (S[:, 3] - c))
         f2 = evaluate(x2)
+        record(np.minimum(np.minimum(f.min(1), fr), f2))
         evals += 2
         shrink = np.zeros(n, bool)
         for i in range(n):
             if expand[i]:
                 S[i, 3], f[i, 3] = (x2[i], f2[i]) if f2[i] < fr[i] else (xr[i], fr[i])
-            elif fr[i] < f[i, 2]:
-                S[i, 3], f[i, 3] = xr[i], fr[i]
-            elif f2[i] < f[i, 3]:
+                continue
+            # both points were evaluated: keep the better of the acceptable reflection
+            # (better than the second worst) and the acceptable contraction (better than the worst)
+            ok_r, ok_c = fr[i] < f[i, 2], f2[i] < f[i, 3]
+            if ok_c and (not ok_r or f2[i] < fr[i]):
                 S[i, 3], f[i, 3] = x2[i], f2[i]
+            elif ok_r:
+                S[i, 3], f[i, 3] = xr[i], fr[i]
             else:
                 shrink[i] = True
         if shrink.any() and evals + 3 <= max_evals:  # shrink towards the best vertex, re-evaluated
@@ -156,9 +239,11 @@ def fit_nelder_mead(targets, bases, bs, max_evals=60, log=None):
             for j in range(1, 4):
                 fj = evaluate(S[:, j])
                 f[shrink, j] = fj[shrink]
+                record(f.min(1))
             evals += 3
         if log:
             log(f"nelder-mead evals {evals}: best median err {np.median(f.min(1)):.4f}")
     j = np.argmin(f, 1)
-    theta = LO + (HI - LO) * np.clip(S[np.arange(n), j], 0, 1)
-    return {"theta": theta, "sim_err": f[np.arange(n), j], "total_s": time.perf_counter() - t0, "evaluations": evals}
+    theta = stabilize(LO + (HI - LO) * np.clip(S[np.arange(n), j], 0, 1), T0)
+    return {"theta": theta, "sim_err": f[np.arange(n), j], "total_s": time.perf_counter() - t0, "evaluations": evals,
+            "history": np.array(hist)}

@@ -1,0 +1,140 @@
+"""Fitting benchmark: surrogate-assisted fits against simulation-only Nelder-Mead fits.
+
+Targets: ``n`` new cells from the batched workload's generator (seeds ``seed0 + i``: radius,
+lattice spacing, mesh seed, tip radius, speed and the hidden E, nu, viscosity), simulated for 10k
+steps, plus Gaussian noise of standard deviation 1 % of each curve's peak force (seed 5).  The
+fitters see only the noisy curve and the cell's geometry and protocol.
+
+Reported (JSON on stdout):
+- ``surrogate``: all targets fitted together by ``fit.fit_surrogate``: fits/s, the batch wall time
+  (the latency of every fit in the batch), the confirmed relative L2 error against the noisy
+  target (median, p90, p99, fraction <= 2 %), the error of the noise-free curve of the true
+  parameters (the floor), and the recovered-parameter errors;
+- ``nelder_mead``: the simulation-only baseline on the first ``n_nm`` targets with ``nm_evals``
+  simulations per target: fits/s, errors, and the evaluations each target needed to reach 2 %;
+- ``single``: p50/p99 latency of one-target fits (a batch of one cell) for both methods;
+- ``speedup``: time per fit of Nelder-Mead over time per fit of the surrogate path.
+
+``python -m nofluidx3d_amd.surrogate.fit_bench --weights DIR/surrogate.npz`` (on a GPU).
+"""
+
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import json
+import sys
+import time
+
+import numpy as np
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def hidden_blind(cells):
+    """The fitters' view of the cells: geometry and protocol, placeholder material."""
+    return [dataclasses.replace(c, E_si=1000.0, nu=0.4, viscosity_si=10.0) for c in cells]
+
+
+def make_targets(n, seed0=7000, steps=10000, noise=0.01, noise_seed=5, tile_k=256):
+    from nofluidx3d_amd.batch import BatchSimulation, make_meshes, sample_batch
+    from nofluidx3d_amd.surrogate import fit
+
+    t0 = time.time()
+    cells, _ = sample_batch(n, seed0=seed0, steps=steps)
+    meshes = make_meshes(cells)
+    log(f"targets: {n} meshes in {time.time() - t0:.1f}s")
+    bs = BatchSimulation(hidden_blind(cells), meshes, steps=steps, ncurve=256, tile_k=tile_k)
+    log(f"targets: batch built ({bs.T_real} tetra) at {time.time() - t0:.1f}s")
+    clean = fit.simulate(bs, cells)
+    rng = np.random.Generator(np.random.PCG64(noise_seed))
+    noisy = clean + noise * np.abs(clean).max(axis=1, keepdims=True) * rng.normal(size=clean.shape)
+    log(f"targets: simulated at {time.time() - t0:.1f}s")
+    return cells, meshes, bs, clean, noisy
+
+
+def theta_true(cells):
+    return np.array([[np.log10(c.E_si), c.nu, np.log10(c.viscosity_si)] for c in cells])
+
+
+def param_errors(theta, truth):
+    d = np.abs(theta - truth)
+    return {"log10_E_median": float(np.median(d[:, 0])), "nu_median": float(np.median(d[:, 1])),
+            "log10_viscosity_median": float(np.median(d[:, 2])),
+            "log10_viscosity_over_E_median": float(np.median(np.abs((theta[:, 2] - theta[:, 0]) -
+                                                                     (truth[:, 2] - truth[:, 0]))))}
+
+
+def err_stats(e):
+    return {"median": float(np.median(e)), "p90": float(np.percentile(e, 90)), "p99": float(np.percentile(e, 99)),
+            "frac_le_2pct": float(np.mean(e <= 0.02))}
+
+
+def main(argv=None):
+    from nofluidx3d_amd.batch import BatchSimulation
+    from nofluidx3d_amd.surrogate import fit, train
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--weights", required=True)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--n-nm", type=int, default=512)
+    ap.add_argument("--nm-evals", type=int, default=60)
+    ap.add_argument("--ntop", type=int, default=2)
+    ap.add_argument("--single", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=10000)
+    args = ap.parse_args(argv)
+
+    net, norm = train.load(args.weights)
+    cells, meshes, bs, clean, noisy = make_targets(args.n, steps=args.steps)
+    blind = hidden_blind(cells)
+    truth = theta_true(cells)
+    floor = fit.rel_err(clean, noisy)
+    out = {"targets": args.n, "noise": "Gaussian, sd 1% of each curve's peak force", "steps": args.steps,
+           "tetra": int(bs.T_real), "noise_floor_rel_l2": err_stats(floor)}
+
+    r = fit.fit_surrogate(net, norm, noisy, blind, bs, ntop=args.ntop, log=log)
+    out["surrogate"] = {
+        "fits_per_s": args.n / r["total_s"], "batch_wall_s": r["total_s"], "surrogate_s": r["surrogate_s"],
+        "inference_s": r["inference_s"], "refine_s": r["refine_s"], "confirm_s": r["confirm_s"],
+        "simulations_per_target": r["simulations_per_target"], "candidates_per_target": r["candidates_per_target"],
+        "confirmed_rel_l2": err_stats(r["sim_err"]), "param_errors": param_errors(r["theta"], truth)}
+    log("surrogate:", json.dumps(out["surrogate"]))
+    del bs
+
+    m = args.n_nm
+    bsn = BatchSimulation(blind[:m], meshes[:m], steps=args.steps, ncurve=256, tile_k=256)
+    nm = fit.fit_nelder_mead(noisy[:m], blind[:m], bsn, max_evals=args.nm_evals, log=log)
+    reach = np.argmax(nm["history"] <= 0.02, axis=0).astype(float) + 1
+    reach[~(nm["history"] <= 0.02).any(axis=0)] = np.nan
+    out["nelder_mead"] = {
+        "targets": m, "fits_per_s": m / nm["total_s"], "batch_wall_s": nm["total_s"], "evaluations": nm["evaluations"],
+        "rel_l2": err_stats(nm["sim_err"]), "param_errors": param_errors(nm["theta"], truth[:m]),
+        "frac_reaching_2pct": float(np.mean(np.isfinite(reach))),
+        "evals_to_2pct_median_of_reaching": float(np.nanmedian(reach)) if np.isfinite(reach).any() else None,
+        "median_err_after_evals": {str(k): float(np.median(nm["history"][k - 1])) for k in (4, 10, 20, 40)
+                                   if k <= len(nm["history"])}}
+    log("nelder-mead:", json.dumps(out["nelder_mead"]))
+    del bsn
+    out["speedup_time_per_fit"] = (nm["total_s"] / m) / (r["total_s"] / args.n)
+
+    lat_s, lat_n = [], []
+    for i in range(args.single):
+        b1 = BatchSimulation(blind[i:i + 1], meshes[i:i + 1], steps=args.steps, ncurve=256, tile_k=256)
+        t0 = time.perf_counter()
+        fit.fit_surrogate(net, norm, noisy[i:i + 1], blind[i:i + 1], b1, ntop=args.ntop)
+        lat_s.append(time.perf_counter() - t0)
+        if i < max(2, args.single // 4):
+            t0 = time.perf_counter()
+            fit.fit_nelder_mead(noisy[i:i + 1], blind[i:i + 1], b1, max_evals=args.nm_evals)
+            lat_n.append(time.perf_counter() - t0)
+        del b1
+    out["single"] = {"surrogate_p50_s": float(np.percentile(lat_s, 50)), "surrogate_p99_s": float(np.percentile(lat_s, 99)),
+                     "nelder_mead_p50_s": float(np.percentile(lat_n, 50)), "nelder_mead_p99_s": float(np.percentile(lat_n, 99)),
+                     "samples": [len(lat_s), len(lat_n)]}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

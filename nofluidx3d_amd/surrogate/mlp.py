@@ -8,7 +8,7 @@ Layout (all device tensors):
   its transpose HT[l] [out x B]; gradients dZ[l] BF16 [B x out] and dZT[l] [out x B].
 
 One training step (``train_step``): forward (FWD_GELU epilogues, STORE for the output layer),
-MSE loss and its gradient, then for l = L..1: backward-data GEMM for layer l-1 with the current
+MSE loss (optionally row-weighted) and its gradient, then for l = L..1: backward-data GEMM for layer l-1 with the current
 BF16 weights (BWD_GELU epilogue), weight-gradient GEMM with the fused Adam update (ADAM epilogue),
 bias gradient + Adam.  Weight initialisation follows torch.nn.Linear (U(-1/sqrt(fan_in),
 1/sqrt(fan_in)) for weights and biases) from a seeded generator.
@@ -147,9 +147,10 @@ class MLP:
         _lib.call("nfx_stream_sync", openCL.handle())
         torch.cuda.synchronize(self.dev)
 
-    def backward_dev(self, B, target, apply_adam=True):
-        """MSE gradient of the last forward pass; Adam updates (apply_adam) or raw gradients
-        (returned as lists of FP32 tensors dW, db)."""
+    def backward_dev(self, B, target, apply_adam=True, row_weight=None):
+        """MSE gradient of the last forward pass (rows weighted by the FP32 device tensor
+        ``row_weight`` when given); Adam updates (apply_adam) or raw gradients (returned as lists
+        of FP32 tensors dW, db)."""
         import torch
 
         from nofluidx3d_amd import openCL
@@ -157,7 +158,8 @@ class MLP:
         bufs = self.buffers(B, True)
         L = self.nl
         self.loss_dev.zero_()
-        _lib.call("nfx_mlp_mse", openCL.handle(), bufs["Y"].data_ptr(), target.data_ptr(), B, self.sizes[-1],
+        _lib.call("nfx_mlp_mse", openCL.handle(), bufs["Y"].data_ptr(), target.data_ptr(),
+                  None if row_weight is None else row_weight.data_ptr(), B, self.sizes[-1],
                   bufs["dZ"][L - 1].data_ptr(), bufs["dZT"][L - 1].data_ptr(), self.loss_dev.data_ptr())
         if apply_adam:
             self.step += 1
@@ -184,11 +186,11 @@ class MLP:
                 grads_b[l] = bufs["dZT"][l].float().sum(1)
         return grads_w, grads_b
 
-    def train_step(self, B, target):
+    def train_step(self, B, target, row_weight=None):
         """One Adam step on the batch already set with ``set_input(X, train=True)``; returns the
-        device loss tensor (FP64, mean squared error of the forward pass)."""
+        device loss tensor (FP64, row-weighted mean squared error of the forward pass)."""
         self.forward_dev(B, train=True)
-        self.backward_dev(B, target, apply_adam=True)
+        self.backward_dev(B, target, apply_adam=True, row_weight=row_weight)
         return self.loss_dev
 
     # -------------------------------------------------------------------------------------------

@@ -1,13 +1,23 @@
 """Training data from the batched simulator and training of the curve surrogate.
 
-Data: a pool of cell meshes (the batched workload's cells, seeds 1000+i) is simulated chunk after
+Data.  A pool of cell meshes (the batched workload's cells, seeds 1000+i) is simulated chunk after
 chunk with new physical parameters drawn from the batched workload's distribution (E log-uniform
 0.1-10 kPa, viscosity log-uniform 1-1000 Pa s, nu U(0.3, 0.49), tip radius U(1, 5) um, speed
 log-uniform 0.5-50 um/s; per-curve seeds ``seed0 + i``), 10k steps each, one 256-point curve per
-cell, normalised by E R^2.  Held-out curves use disjoint seeds.
+cell, normalised by E R^2.  Two held-out sets: new physics on the pool's meshes (disjoint seeds),
+and new cells (new meshes, seeds 5000+i) with new physics.
 
-Model: MLP 6 -> 512 x 4 (GELU) -> 256, inputs standardised (log10 E, nu, log10 eta, tip radius,
-log10 speed, cell radius), targets scaled by one global constant; Adam (lr 1e-3, batch 4096).
+Model.  MLP 6 -> 512 x 4 (GELU) -> 256 on ``batch.surrogate_inputs`` (standardised), targets
+scaled by one global constant; Adam (lr 1e-3, batch 4096).
+- Loss: the squared relative L2 error of each curve, i.e. the MSE with row weights
+  1 / mean(target_row^2) (floored at 1e-3 of the median), so that small-force curves count as much
+  as large ones.
+- Augmentation by an exact symmetry of the simulator: scaling E and the viscosity by one factor
+  leaves every simulation-unit quantity of a cell unchanged (p0 scales with E, the flow parameter
+  3 eta / p0 and the time scale do not change), so the normalised curve is invariant.  Each training
+  row's log10 E and log10 viscosity get one common random offset within the sampled ranges.
+- Early stopping: 5 % of the training curves are a validation split; the weights with the lowest
+  median validation error are kept.
 Error metric: relative L2 error of each held-out curve, ||pred - sim|| / ||sim||.
 
 ``python -m nofluidx3d_amd.surrogate.train --out DIR`` (on a GPU) writes ``DIR/surrogate.npz``
@@ -28,6 +38,8 @@ import numpy as np
 from nofluidx3d_amd.batch import CellSpec
 
 SIZES = (6, 512, 512, 512, 512, 256)
+LOG_E = (2.0, 4.0)  # sampled range of log10 E [Pa]
+LOG_ETA = (0.0, 3.0)  # sampled range of log10 viscosity [Pa s]
 
 
 def sample_physics(pool, n, seed0, steps, max_relax=0.1):
@@ -37,8 +49,8 @@ def sample_physics(pool, n, seed0, steps, max_relax=0.1):
         base = pool[i % len(pool)]
         rng = np.random.Generator(np.random.PCG64(seed0 + i))
         while True:
-            E = 10 ** rng.uniform(2.0, 4.0)
-            eta = 10 ** rng.uniform(0.0, 3.0)
+            E = 10 ** rng.uniform(*LOG_E)
+            eta = 10 ** rng.uniform(*LOG_ETA)
             nu = rng.uniform(0.3, 0.49)
             v = 10 ** rng.uniform(math.log10(0.5), math.log10(50.0))
             tip = rng.uniform(1.0, 5.0)
@@ -58,7 +70,7 @@ class Normaliser:
 
 
 def generate(bs, pool, n_chunks, seed0, log=print):
-    """Run ``n_chunks`` chunks of len(pool) cells; returns (features, normalised curves)."""
+    """Run ``n_chunks`` chunks of len(pool) cells; returns (inputs, normalised curves)."""
     X, Y = [], []
     for c in range(n_chunks):
         t0 = time.time()
@@ -73,40 +85,79 @@ def generate(bs, pool, n_chunks, seed0, log=print):
     return X[ok], Y[ok]
 
 
-def train(X, Y, iters=4000, batch=4096, seed=0, log=print, Xv=None, Yv=None, every=500):
+def scale_shift(X, rng):
+    """The E-viscosity symmetry: one common random offset on log10 E and log10 viscosity, uniform
+    over the offsets that keep both inside their sampled ranges."""
+    lo = np.maximum(LOG_E[0] - X[:, 0], LOG_ETA[0] - X[:, 2])
+    hi = np.minimum(LOG_E[1] - X[:, 0], LOG_ETA[1] - X[:, 2])
+    a = lo + (hi - lo) * rng.random(len(X))
+    out = np.array(X, np.float64, copy=True)
+    out[:, 0] += a
+    out[:, 2] += a
+    return out
+
+
+def row_weights(Y):
+    """1 / mean(y^2) per curve (floored at 1e-3 of the median), normalised to mean 1."""
+    ms = np.mean(np.asarray(Y, np.float64) ** 2, axis=1)
+    w = 1.0 / np.maximum(ms, 1e-3 * np.median(ms))
+    return (w / w.mean()).astype(np.float32)
+
+
+def train(X, Y, iters=4000, batch=4096, seed=0, log=print, Xv=None, Yv=None, every=250, augment=True,
+          relative=True, val_frac=0.05):
     import torch
 
     from nofluidx3d_amd.surrogate.mlp import MLP
 
+    rng = np.random.Generator(np.random.PCG64(seed))
+    perm = rng.permutation(len(X))
+    nval = max(1, int(val_frac * len(X))) if len(X) > 20 else 0
+    val, tr = perm[:nval], perm[nval:]
+    Xtr, Ytr = X[tr], Y[tr]
     x_mean, x_std = X.mean(0), X.std(0) + 1e-12
     y_scale = float(Y.std())
     norm = Normaliser(x_mean, x_std, y_scale)
     net = MLP(SIZES, seed=seed)
-    Xn = norm.x(X).astype(np.float32)
-    Yn = (Y / y_scale).astype(np.float32)
-    Yd = torch.from_numpy(Yn).to(net.dev)
-    rng = np.random.Generator(np.random.PCG64(seed))
-    B = min(batch, -(-len(X) // 128) * 128)
+    Yd = torch.from_numpy((Ytr / y_scale).astype(np.float32)).to(net.dev)
+    Wd = torch.from_numpy(row_weights(Ytr)).to(net.dev) if relative else None
+    B = min(batch, -(-len(Xtr) // 128) * 128)
     hist = []
+    best = (np.inf, None, 0)
     t0 = time.time()
     for it in range(iters):
-        idx = rng.integers(0, len(X), B)
-        net.set_input(Xn[idx], train=True)
-        tgt = Yd[torch.from_numpy(idx).to(net.dev)]
-        net.train_step(B, tgt)
+        idx = rng.integers(0, len(Xtr), B)
+        xb = scale_shift(Xtr[idx], rng) if augment else Xtr[idx]
+        net.set_input(norm.x(xb), train=True)
+        di = torch.from_numpy(idx).to(net.dev)
+        net.train_step(B, Yd[di], None if Wd is None else Wd[di])
         if (it + 1) % every == 0 or it == iters - 1:
             net.sync()
-            rec = {"iter": it + 1, "train_mse": net.loss_dev.item(), "seconds": time.time() - t0}
+            rec = {"iter": it + 1, "train_loss": net.loss_dev.item(), "seconds": time.time() - t0}
+            if nval:
+                ev = float(np.median(curve_errors(net, norm, X[val], Y[val])))
+                rec["val_rel_l2_median"] = ev
+                if ev < best[0]:
+                    best = (ev, net.state(), it + 1)
             if Xv is not None:
                 rec["heldout_rel_l2_median"] = float(np.median(curve_errors(net, norm, Xv, Yv)))
             hist.append(rec)
             log(rec, flush=True)
+    if best[1] is not None:
+        net.load_state(best[1])
+        hist.append({"kept_iter": best[2], "val_rel_l2_median": best[0]})
     return net, norm, hist
 
 
 def curve_errors(net, norm, X, Y):
     pred = net.predict(norm.x(X)) * norm.y_scale
     return np.linalg.norm(pred - Y, axis=1) / np.linalg.norm(Y, axis=1)
+
+
+def error_summary(err):
+    return {"n": int(len(err)), "mean": float(err.mean()), "median": float(np.median(err)),
+            "p90": float(np.percentile(err, 90)), "p95": float(np.percentile(err, 95)), "max": float(err.max()),
+            "frac_le_2pct": float((err <= 0.02).mean()), "frac_le_5pct": float((err <= 0.05).mean())}
 
 
 def save(path, net, norm):
@@ -128,39 +179,43 @@ def main(argv=None):
 
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
-    ap.add_argument("--chunks", type=int, default=40)
-    ap.add_argument("--heldout-chunks", type=int, default=2)
+    ap.add_argument("--chunks", type=int, default=130)
+    ap.add_argument("--heldout-chunks", type=int, default=6)
+    ap.add_argument("--newmesh-chunks", type=int, default=4)
     ap.add_argument("--pool", type=int, default=512)
     ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--iters", type=int, default=6000)
+    ap.add_argument("--iters", type=int, default=12000)
     ap.add_argument("--tile-k", type=int, default=256)
     args = ap.parse_args(argv)
     os.makedirs(args.out, exist_ok=True)
     log = lambda *a, **k: print(*a, **k, file=sys.stderr)
     t0 = time.time()
     pool, _ = sample_batch(args.pool, seed0=1000, steps=args.steps)
-    meshes = make_meshes(pool)
-    bs = BatchSimulation(pool, meshes, steps=args.steps, ncurve=256, tile_k=args.tile_k)
+    bs = BatchSimulation(pool, make_meshes(pool), steps=args.steps, ncurve=256, tile_k=args.tile_k)
     Xt, Yt = generate(bs, pool, args.chunks, seed0=100000, log=log)
     Xv, Yv = generate(bs, pool, args.heldout_chunks, seed0=900000, log=log)
+    del bs
+    pool2, _ = sample_batch(args.pool, seed0=5000, steps=args.steps)
+    bs2 = BatchSimulation(pool2, make_meshes(pool2), steps=args.steps, ncurve=256, tile_k=args.tile_k)
+    Xn, Yn = generate(bs2, pool2, args.newmesh_chunks, seed0=950000, log=log)
+    del bs2
     tgen = time.time() - t0
-    np.savez(os.path.join(args.out, "heldout.npz"), X=Xv, Y=Yv)
+    np.savez(os.path.join(args.out, "heldout.npz"), X=Xv, Y=Yv, Xn=Xn, Yn=Yn)
+    t1 = time.time()
     net, norm, hist = train(Xt, Yt, iters=args.iters, log=log, Xv=Xv, Yv=Yv)
-    err = curve_errors(net, norm, Xv, Yv)
+    ttrain = time.time() - t1
     save(os.path.join(args.out, "surrogate.npz"), net, norm)
     rep = {
         "train_curves": int(len(Xt)),
-        "heldout_curves": int(len(Xv)),
         "data_generation_s": tgen,
-        "tetra_per_chunk": int(bs.T_real),
+        "training_s": ttrain,
         "iters": args.iters,
         "history": hist,
-        "heldout_rel_l2": {"mean": float(err.mean()), "median": float(np.median(err)),
-                           "p95": float(np.percentile(err, 95)), "max": float(err.max()),
-                           "frac_le_2pct": float((err <= 0.02).mean())},
+        "heldout_physics_rel_l2": error_summary(curve_errors(net, norm, Xv, Yv)),
+        "heldout_cells_rel_l2": error_summary(curve_errors(net, norm, Xn, Yn)),
     }
     json.dump(rep, open(os.path.join(args.out, "report.json"), "w"), indent=1)
-    print(json.dumps(rep["heldout_rel_l2"]))
+    print(json.dumps({k: rep[k] for k in ("heldout_physics_rel_l2", "heldout_cells_rel_l2")}))
 
 
 if __name__ == "__main__":

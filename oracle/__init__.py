@@ -48,6 +48,23 @@ class VPParams(ctypes.Structure):
     ]
 
 
+class LegacyParams(ctypes.Structure):
+    """parameters of the legacy stress laws (orc_legacy_force): law 0 uses E, nu; law 1 shear,
+    bulk; law 2 shear, bulk, shear2; law 3 shear, phi"""
+
+    _fields_ = [
+        ("E", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("shear", ctypes.c_double),
+        ("bulk", ctypes.c_double),
+        ("shear2", ctypes.c_double),
+        ("phi", ctypes.c_double),
+    ]
+
+
+LINEAR_ELASTIC, LINEAR_DEVIATORIC, NEO_HOOKEAN_2, PORO_NEO_HOOKEAN = 4, 5, 6, 7  # run models (law + 4)
+
+
 class RunParams(ctypes.Structure):
     _fields_ = [
         ("model", ctypes.c_int),
@@ -66,6 +83,7 @@ class RunParams(ctypes.Structure):
         ("ind_move_steps", ctypes.c_double),
         ("fix_top_bottom", ctypes.c_int),
         ("nthreads", ctypes.c_int),
+        ("legacy", LegacyParams),
     ]
 
 
@@ -85,6 +103,8 @@ def lib():
                                       i64, i64, _dp, _up, _dp, _dp, _dp, ctypes.c_int]
         L.orc_viscoplastic.argtypes = [ctypes.POINTER(VPParams), i64, i64, _dp, _up, _dp, _dp, _dp, _dp,
                                        _dp, _dp, _dp, ctypes.c_int]
+        L.orc_legacy_force.argtypes = [ctypes.c_int, ctypes.POINTER(LegacyParams), i64, i64, _dp, _up, _dp, _dp,
+                                       _dp, _dp, ctypes.c_int]
         L.orc_sphere.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double, ctypes.c_double]
         L.orc_substrate.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double]
         L.orc_plane.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double]
@@ -176,6 +196,14 @@ def viscoplastic(vp: VPParams, points, tetras, ev, vol, force, acc, Fp, vonMises
     T = tetras.size // 4
     lib().orc_viscoplastic(ctypes.byref(vp), N, T, _d(points), _u(tetras), _d(ev), _d(vol), _d(force), _d(acc),
                            _d(Fp), _dn(vonMises), _dn(pressure), nthreads)
+
+
+def legacy_force(law, P: LegacyParams, points, tetras, ev, force, vonMises=None, pressure=None, nthreads=1):
+    """law 0..3 (linear elastic, linear deviatoric, second-order Neo-Hookean, poro-Neo-Hookean)"""
+    N = points.size // 3
+    T = tetras.size // 4
+    lib().orc_legacy_force(law, ctypes.byref(P), N, T, _d(points), _u(tetras), _d(ev), _d(force), _dn(vonMises),
+                           _dn(pressure), nthreads)
 
 
 def sphere(k, force, points, radius, pos):

@@ -3,7 +3,7 @@ libnfx surrogate kernels.
 
 Architecture and conventions match ``nofluidx3d_amd.surrogate.mlp``: y = W x + b per layer
 (W [out x in]), exact GELU 0.5 z (1 + erf(z / sqrt 2)) on hidden layers, identity output, loss
-mean((Y - T)^2) over all elements, Adam with bias correction.  Pinned in tests/test_oracle_mlp.py
+sum_ij w_i (Y - T)^2 / (number of elements) with optional row weights w, Adam with bias correction.  Pinned in tests/test_oracle_mlp.py
 against torch autograd in float64 and against central differences.
 """
 
@@ -32,11 +32,13 @@ def forward(Ws, bs, X):
     return h, cache
 
 
-def loss_and_grads(Ws, bs, X, T):
+def loss_and_grads(Ws, bs, X, T, w=None):
+    """loss = sum_ij w_i (Y - T)^2 / Y.size (w = 1 when None) and its parameter gradients"""
     Y, cache = forward(Ws, bs, X)
     n = Y.size
-    loss = np.mean((Y - T) ** 2)
-    dz = 2.0 * (Y - T) / n
+    w = np.ones(len(Y)) if w is None else np.asarray(w, np.float64)
+    loss = np.sum(w[:, None] * (Y - T) ** 2) / n
+    dz = 2.0 * w[:, None] * (Y - T) / n
     gW, gb = [None] * len(Ws), [None] * len(Ws)
     for l in range(len(Ws) - 1, -1, -1):
         h, _ = cache[l]

@@ -11,6 +11,8 @@
  *   orc_viscoplastic     kernels/InteractionFiniteStrainViscoplastic.cl:1-220, restated with
  *                        node 0 as the edge base (the legacy kernel used node 3; TetraCell.py:33-38
  *                        uses node 0), plus an optional yield offset (0 = reference law)
+ *   orc_legacy_force     kernels/InteractionLinearElastic.cl, InteractionLinearElasticDeviatoric.cl,
+ *                        InteractionSecondOrderNeoHookean.cl, InteractionPoroNeoHookean.cl (node-0 base)
  *   orc_sphere           kernels/Interactions/Sphere.cl:1-17
  *   orc_substrate        kernels/Interactions/Substrate.cl:1-7
  *   orc_plane            kernels/Interactions/Plane.cl:1-7
@@ -385,6 +387,133 @@ void orc_viscoplastic(const orc_vp_params* P, int64_t N, int64_t T, const double
     for (int64_t t = 0; t < T; t++) vp_one(t, P, N, T, points, tetras, ev, vol, force, acc, Fp, vonMises, pressure, 1);
 }
 
+/* Legacy stress laws (SimulationLegacy.py:696-840, Interactions.py:101-247 and 352-408), each a
+ * Cauchy stress from the displacement gradient of the node-0-based edge vectors, applied with the
+ * face-area nodal forces of the same kernels, restated with node 0 as the edge base:
+ *   law 0  InteractionLinearElastic.cl:30-83           sigma = 2 mu eps + lambda tr(eps) I
+ *   law 1  InteractionLinearElasticDeviatoric.cl:55-80 sigma = 2 mu dev(eps) + K tr(eps) I
+ *   law 2  InteractionSecondOrderNeoHookean.cl:57-96   sigma = J^-5/3 (mu + mu2 (trB J^-2/3 - 3)) dev(B) + K (J - 1) I
+ *   law 3  InteractionPoroNeoHookean.cl:57-82          sigma = a B + b I (Danielsson, Parks, Boyce 2002)
+ * with eps = (grad u + grad u^T)/2, B = F F^T, J = sqrt(det B). */
+typedef struct {
+    double E, nu;     /* law 0 */
+    double shear;     /* laws 1-3 */
+    double bulk;      /* laws 1-2 */
+    double shear2;    /* law 2: second-order coefficient */
+    double phi;       /* law 3: porous volume fraction */
+} orc_legacy_params;
+
+static void legacy_one(int64_t t, int law, const orc_legacy_params* P, int64_t N, int64_t T, const double* points,
+                       const uint32_t* tetras, const double* ev, double* force, double* vonMises, double* pressure,
+                       int atomic) {
+    const uint32_t id1 = tetras[T + t], id2 = tetras[2 * T + t], id3 = tetras[3 * T + t], id4 = tetras[t];
+    const d3 p1 = v3(points[id1], points[N + id1], points[2 * N + id1]);
+    const d3 p2 = v3(points[id2], points[N + id2], points[2 * N + id2]);
+    const d3 p3 = v3(points[id3], points[N + id3], points[2 * N + id3]);
+    const d3 p4 = v3(points[id4], points[N + id4], points[2 * N + id4]);
+    const d3 r1 = vsub(p1, p4), r2 = vsub(p2, p4), r3 = vsub(p3, p4);
+    const d3 R1 = v3(ev[t], ev[T + t], ev[2 * T + t]);
+    const d3 R2 = v3(ev[3 * T + t], ev[4 * T + t], ev[5 * T + t]);
+    const d3 R3 = v3(ev[6 * T + t], ev[7 * T + t], ev[8 * T + t]);
+    const d3 u1 = vsub(r1, R1), u2 = vsub(r2, R2), u3 = vsub(r3, R3);
+    const double divisor = vdot(vcross(R1, R2), R3);
+    const d3 invJ1 = vdivs(vcross(R2, R3), divisor);
+    const d3 invJ2 = vdivs(vcross(R3, R1), divisor);
+    const d3 invJ3 = vdivs(vcross(R1, R2), divisor);
+    /* rows of grad u: du_i/dR */
+    const d3 du1 = vadd(vadd(vscale(invJ1, u1.x), vscale(invJ2, u2.x)), vscale(invJ3, u3.x));
+    const d3 du2 = vadd(vadd(vscale(invJ1, u1.y), vscale(invJ2, u2.y)), vscale(invJ3, u3.y));
+    const d3 du3 = vadd(vadd(vscale(invJ1, u1.z), vscale(invJ2, u2.z)), vscale(invJ3, u3.z));
+    const d3 E1 = v3(1, 0, 0), E2 = v3(0, 1, 0), E3 = v3(0, 0, 1);
+    d3 s1, s2, s3;
+    if (law == 0 || law == 1) {
+        const d3 e1 = vscale(vadd(du1, v3(du1.x, du2.x, du3.x)), 0.5);
+        const d3 e2 = vscale(vadd(du2, v3(du1.y, du2.y, du3.y)), 0.5);
+        const d3 e3 = vscale(vadd(du3, v3(du1.z, du2.z, du3.z)), 0.5);
+        const double trE = e1.x + e2.y + e3.z;
+        if (law == 0) {
+            const double a = 2 * P->E / (2 + 2 * P->nu);
+            const double b = P->E * P->nu / ((1 + P->nu) * (1 - 2 * P->nu));
+            s1 = vadd(vscale(e1, a), vscale(E1, b * trE));
+            s2 = vadd(vscale(e2, a), vscale(E2, b * trE));
+            s3 = vadd(vscale(e3, a), vscale(E3, b * trE));
+        } else {
+            const d3 d1 = vsub(e1, vscale(E1, trE / 3.)), d2 = vsub(e2, vscale(E2, trE / 3.)),
+                     d3_ = vsub(e3, vscale(E3, trE / 3.));
+            s1 = vadd(vscale(d1, 2 * P->shear), vscale(E1, P->bulk * trE));
+            s2 = vadd(vscale(d2, 2 * P->shear), vscale(E2, P->bulk * trE));
+            s3 = vadd(vscale(d3_, 2 * P->shear), vscale(E3, P->bulk * trE));
+        }
+    } else {
+        const d3 F1 = vadd(du1, E1), F2 = vadd(du2, E2), F3 = vadd(du3, E3);
+        const d3 B1 = v3(vdot(F1, F1), vdot(F1, F2), vdot(F1, F3));
+        const d3 B2 = v3(vdot(F2, F1), vdot(F2, F2), vdot(F2, F3));
+        const d3 B3 = v3(vdot(F3, F1), vdot(F3, F2), vdot(F3, F3));
+        const double J = sqrt(vdot(vcross(B1, B2), B3));
+        const double trB = B1.x + B2.y + B3.z;
+        if (law == 2) {
+            const double a = 1.0 / pow(J, 5.0 / 3.0) * (P->shear + P->shear2 * (trB / pow(J, 2.0 / 3.0) - 3.0));
+            const double b = P->bulk * (J - 1.0);
+            s1 = vadd(vscale(vsub(B1, vscale(E1, trB / 3.0)), a), vscale(E1, b));
+            s2 = vadd(vscale(vsub(B2, vscale(E2, trB / 3.0)), a), vscale(E2, b));
+            s3 = vadd(vscale(vsub(B3, vscale(E3, trB / 3.0)), a), vscale(E3, b));
+        } else {
+            const double phi = P->phi, mu = P->shear;
+            const double eta = 1 + (J - 1) / phi;
+            const double a = mu / 2.0 * (4.0 / J - 2.0 / (J * J) -
+                                         2.0 * (phi + 2.0 * (J - 1.0)) / (pow(J, 5.0 / 3.0) * pow(eta, 1.0 / 3.0)));
+            const double b = mu / 2.0 * trB *
+                             (1.0 / (J * J) - 1.0 / (3.0 * pow(J, 2.0 / 3.0) * pow(eta, 1.0 / 3.0)) *
+                                                  ((4.0 - phi) * eta + (1.0 - phi)) /
+                                                  (phi * eta * eta + (1.0 - phi) * eta));
+            s1 = vadd(vscale(B1, a), vscale(E1, b));
+            s2 = vadd(vscale(B2, a), vscale(E2, b));
+            s3 = vadd(vscale(B3, a), vscale(E3, b));
+        }
+    }
+    const double trS = s1.x + s2.y + s3.z;
+    const d3 ds1 = vsub(s1, vscale(E1, trS / 3.0)), ds2 = vsub(s2, vscale(E2, trS / 3.0)),
+             ds3 = vsub(s3, vscale(E3, trS / 3.0));
+    if (vonMises) vonMises[t] = sqrt(3. / 2. * (vdot(ds1, ds1) + vdot(ds2, ds2) + vdot(ds3, ds3)));
+    if (pressure) pressure[t] = trS / 3.0;
+    /* outward unit normals, face areas and the 1/3 face-force split of the same kernels */
+    d3 n1 = vcross(r2, r3), n2 = vcross(r3, r1), n3 = vcross(r1, r2), n4 = vcross(vsub(r3, r1), vsub(r3, r2));
+    n1 = vdivs(n1, vlen(n1));
+    n2 = vdivs(n2, vlen(n2));
+    n3 = vdivs(n3, vlen(n3));
+    n4 = vdivs(n4, vlen(n4));
+    if (vdot(n1, vscale(r1, -1.0)) < 0) n1 = vscale(n1, -1.0);
+    if (vdot(n2, vscale(r2, -1.0)) < 0) n2 = vscale(n2, -1.0);
+    if (vdot(n3, vscale(r3, -1.0)) < 0) n3 = vscale(n3, -1.0);
+    if (vdot(n4, r1) < 0) n4 = vscale(n4, -1.0);
+    const double A1 = 0.5 * vlen(vcross(r2, r3)), A2 = 0.5 * vlen(vcross(r3, r1)), A3 = 0.5 * vlen(vcross(r1, r2));
+    const double A4 = 0.5 * vlen(vcross(vsub(r3, r1), vsub(r3, r2)));
+    const d3 fA1 = vscale(v3(vdot(s1, n1), vdot(s2, n1), vdot(s3, n1)), A1);
+    const d3 fA2 = vscale(v3(vdot(s1, n2), vdot(s2, n2), vdot(s3, n2)), A2);
+    const d3 fA3 = vscale(v3(vdot(s1, n3), vdot(s2, n3), vdot(s3, n3)), A3);
+    const d3 fA4 = vscale(v3(vdot(s1, n4), vdot(s2, n4), vdot(s3, n4)), A4);
+    const d3 f1 = vscale(vadd(vadd(fA2, fA3), fA4), -1.0 / 3.0);
+    const d3 f2 = vscale(vadd(vadd(fA3, fA4), fA1), -1.0 / 3.0);
+    const d3 f3 = vscale(vadd(vadd(fA4, fA1), fA2), -1.0 / 3.0);
+    const d3 f4 = vscale(vadd(vadd(fA1, fA2), fA3), -1.0 / 3.0);
+    scatter(force, id1, f1.x, atomic); scatter(force, N + id1, f1.y, atomic); scatter(force, 2 * N + id1, f1.z, atomic);
+    scatter(force, id2, f2.x, atomic); scatter(force, N + id2, f2.y, atomic); scatter(force, 2 * N + id2, f2.z, atomic);
+    scatter(force, id3, f3.x, atomic); scatter(force, N + id3, f3.y, atomic); scatter(force, 2 * N + id3, f3.z, atomic);
+    scatter(force, id4, f4.x, atomic); scatter(force, N + id4, f4.y, atomic); scatter(force, 2 * N + id4, f4.z, atomic);
+}
+
+void orc_legacy_force(int law, const orc_legacy_params* P, int64_t N, int64_t T, const double* points,
+                      const uint32_t* tetras, const double* ev, double* force, double* vonMises, double* pressure,
+                      int nthreads) {
+    nthreads = set_threads(nthreads);
+    if (nthreads == 1) {
+        for (int64_t t = 0; t < T; t++) legacy_one(t, law, P, N, T, points, tetras, ev, force, vonMises, pressure, 0);
+        return;
+    }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t t = 0; t < T; t++) legacy_one(t, law, P, N, T, points, tetras, ev, force, vonMises, pressure, 1);
+}
+
 /* Sphere.cl:1-17.  OpenCL normalize() of a zero vector returns the zero vector. */
 void orc_sphere(int64_t N, double k, double* force, const double* points, double radius, double spherePos) {
     for (int64_t p = 0; p < N; p++) {
@@ -475,6 +604,7 @@ typedef struct {
     double ind_k, ind_radius, ind_start, ind_speed, ind_move_steps;
     int fix_top_bottom;
     int nthreads;
+    orc_legacy_params legacy; /* models 4-7: legacy laws 0-3 */
 } orc_run_params;
 
 static double indenter_pos(const orc_run_params* R, int64_t time) {
@@ -490,6 +620,8 @@ void orc_run(const orc_run_params* R, int64_t N, int64_t T, double* points, cons
         const int64_t time = step0 + s;
         if (R->model == 3)
             orc_viscoplastic(&R->vp, N, T, points, tetras, ev, vol, force, acc, Fp, NULL, NULL, R->nthreads);
+        else if (R->model >= 4)
+            orc_legacy_force(R->model - 4, &R->legacy, N, T, points, tetras, ev, force, NULL, NULL, R->nthreads);
         else
             orc_tetra_force(R->model, R->E, R->nu, R->ratio, N, T, points, tetras, ev, vol, force, R->nthreads);
         if (R->substrate) orc_substrate(N, R->substrate_k, force, points, R->wall);

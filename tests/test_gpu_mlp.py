@@ -90,23 +90,26 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def test_forward_and_gradients_match_oracle(ctx):
+@pytest.mark.parametrize("weighted", [False, True])
+def test_forward_and_gradients_match_oracle(ctx, weighted):
     import torch
 
     net = small_net()
     B = 256
     X = RNG.normal(size=(B, 6))
     T = RNG.normal(scale=0.3, size=(B, 128))
+    w = RNG.uniform(0.1, 4.0, size=B).astype(np.float32) if weighted else None
     net.set_input(X, train=True)
     Y = net.forward_dev(B, train=True)
     Td = torch.tensor(T, dtype=torch.float32, device="cuda:0")
-    gW, gb = net.backward_dev(B, Td, apply_adam=False)
+    wd = None if w is None else torch.tensor(w, device="cuda:0")
+    gW, gb = net.backward_dev(B, Td, apply_adam=False, row_weight=wd)
     net.sync()
     Ws, bs = oracle_params(net)
     Xr = bf_round(X)
     Yo, _ = OM.forward(Ws, bs, Xr)
     assert rel(Y.cpu().numpy(), Yo) < 3e-2
-    loss, gWo, gbo = OM.loss_and_grads(Ws, bs, Xr, T)
+    loss, gWo, gbo = OM.loss_and_grads(Ws, bs, Xr, T, None if w is None else w.astype(np.float64))
     assert abs(net.loss_dev.item() - loss) < 3e-2 * loss
     for l in range(3):
         assert rel(gW[l].cpu().numpy()[:, :Ws[l].shape[1]], gWo[l]) < 3e-2, l

@@ -245,6 +245,10 @@ def test_vp_shape_state_conversions(ctx):
 # ------------------------------------------------------------------------------------------------
 
 
+LEGACY_SETTERS = {"LE": "setInteractionLinearElastic", "LD": "setInteractionLinearPoroElastic",
+                  "NH2": "setInteractionSecondOrderNeoHookean", "PNH": "setInteractionPoroNeoHookean"}
+
+
 def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, mesh=(4.0, 0.8, 1), material="NH",
               indenter="sphere", steps=400):
     from nofluidx3d_amd import configs
@@ -273,6 +277,9 @@ def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, 
         sim.setInteractionPlane()
     if material == "VP":
         sim.setInteractionFiniteStrainViscoplastic()
+    elif material in LEGACY_SETTERS:
+        cfg["CELL"].update({"PoroElasticFactor": 0.3, "SecondOrderNeoHookeanFactor": 0.5, "PorousVolumeFraction": 0.4})
+        getattr(sim, LEGACY_SETTERS[material])()
     else:
         sim.setInteractionTetra()
     sim.setInteractionVelocityVerlet()
@@ -280,12 +287,17 @@ def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, 
 
 
 def oracle_for(sim, nthreads=1):
-    from nofluidx3d_amd.interactions import FiniteStrainViscoplastic, Sphere, Substrate, Tetra
+    from nofluidx3d_amd.interactions import FiniteStrainViscoplastic, LegacyElastic, Sphere, Substrate, Tetra
 
     prm = O.RunParams()
     prm.nthreads = nthreads
     for inter in sim.interactions:
-        if isinstance(inter, Tetra):
+        if isinstance(inter, LegacyElastic):
+            p = inter.params
+            prm.model = inter.model
+            prm.legacy = O.LegacyParams(E=p.E, nu=p.nu, shear=p.shear, bulk=p.bulk, shear2=p.shear2,
+                                        phi=p.volume_fraction)
+        elif isinstance(inter, Tetra):
             v = inter.knl.values
             prm.model = O.MR if inter.model_code() == 2 else inter.model_code()
             prm.E, prm.nu, prm.ratio = (float(v[k]) for k in ("def_tetraYoungsModulus", "def_tetraPoissonRatio",
@@ -381,6 +393,76 @@ def test_viscoplastic_switching_between_fused_and_per_plugin_steps(ctx):
     gf = host(hist)
     keep = np.r_[0:300, 330:600]
     assert_parity(sim, sim.positions(), gf[keep], ref.positions(), of[keep])
+
+
+@pytest.mark.parametrize("law", [4, 5, 6, 7])
+def test_legacy_law_kernel_parity(ctx, law):
+    """nfx_legacy_force (atomic scatter and per-corner records + CSR gather) against the oracle's
+    restatement of the legacy kernel: nodal forces, von Mises stress and pressure."""
+    import torch
+
+    from nofluidx3d_amd import _lib
+
+    P, T = small_mesh()
+    N, nT = len(P), len(T)
+    dminv, v0, ev, vol = gpu_precompute(ctx, P, T)
+    x = deformed(P, amp=0.12)
+    x32 = O.soa(x.astype(np.float32).astype(np.float64))
+    E, nu = 0.01, 0.42
+    mu, K = E / (2 * (1 + nu)), E / (3 * (1 - 2 * nu))
+    prm = dict(E=E, nu=nu, shear=mu, bulk=K, shear2=0.5 * mu, phi=0.4)
+    f_r, vm_r, pr_r = np.zeros(3 * N), np.zeros(nT), np.zeros(nT)
+    O.legacy_force(law - 4, O.LegacyParams(**prm), x32, O.soa(T, np.uint32), O.soa(ev), f_r, vm_r, pr_r)
+    mat = _lib.Elastic(law, prm["E"], prm["nu"], 1.0, prm["shear"], prm["bulk"], prm["shear2"], prm["phi"])
+    xd, td = dev(x32, np.float32), dev(O.soa(T, np.uint32))
+    f_d = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
+    vm_d = torch.zeros(nT, dtype=torch.float32, device="cuda:0")
+    pr_d = torch.zeros(nT, dtype=torch.float32, device="cuda:0")
+    call("nfx_legacy_force", ctx, mat, xd.data_ptr(), td.data_ptr(), dminv.data_ptr(), N, nT, vm_d.data_ptr(),
+         pr_d.data_ptr(), f_d.data_ptr(), None)
+    assert np.max(np.abs(host(f_d) - f_r)) < 3e-5 * np.max(np.abs(f_r))
+    assert np.max(np.abs(host(vm_d) - vm_r)) < 3e-5 * np.max(vm_r)
+    assert np.max(np.abs(host(pr_d) - pr_r)) < 3e-5 * np.max(np.abs(pr_r))
+    off, slots = csr(T, N)
+    corners = torch.zeros(12 * nT, dtype=torch.float32, device="cuda:0")
+    g_d = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
+    call("nfx_legacy_force", ctx, mat, xd.data_ptr(), td.data_ptr(), dminv.data_ptr(), N, nT, None, None, None,
+         corners.data_ptr())
+    call("nfx_gather_corners", ctx, off.data_ptr(), slots.data_ptr(), corners.data_ptr(), N, g_d.data_ptr())
+    assert np.max(np.abs(host(g_d) - f_r)) < 3e-5 * np.max(np.abs(f_r))
+
+
+@pytest.mark.parametrize("material", ["LE", "LD", "NH2", "PNH"])
+@pytest.mark.parametrize("scatter", ["atomic", "tiled"])
+def test_legacy_law_fused_run_matches_oracle(ctx, material, scatter):
+    sim = build_sim(material=material, scatter=scatter)
+    gx, gf = run_gpu(sim, 500)
+    ref = oracle_for(sim)
+    of = ref.run(500)
+    assert_parity(sim, gx, gf, ref.positions(), of)
+
+
+def test_fused_run_writes_stress_of_the_last_step(ctx):
+    """With write_stress on, the last fused step of a call writes vonMises and pressure: equal to
+    the oracle's law evaluated at the state that step started from."""
+    from nofluidx3d_amd.interactions import LegacyElastic
+
+    sim = build_sim(material="NH2", scatter="tiled")
+    inter = next(i for i in sim.interactions if isinstance(i, LegacyElastic))
+    inter.write_stress = True
+    sim._plan = None
+    run_gpu(sim, 300)
+    ref = oracle_for(sim)
+    ref.run(299)
+    p = inter.params
+    f = np.zeros_like(ref.points)
+    vm, pr = np.zeros(ref.T), np.zeros(ref.T)
+    O.legacy_force(2, O.LegacyParams(E=p.E, nu=p.nu, shear=p.shear, bulk=p.bulk, shear2=p.shear2,
+                                     phi=p.volume_fraction), ref.points, ref.tetras, ref.ev, f, vm, pr)
+    gvm, gpr = host(sim.vonMises.buf), host(sim.pressure.buf)
+    assert np.max(vm) > 0
+    assert np.max(np.abs(gvm - vm)) < 1e-3 * np.max(vm)
+    assert np.max(np.abs(gpr - pr)) < 1e-3 * np.max(np.abs(pr))
 
 
 @pytest.mark.parametrize("scatter", ["csr", "tiled"])

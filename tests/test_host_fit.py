@@ -1,0 +1,116 @@
+"""Host logic of the surrogate-assisted fitter and the Nelder-Mead baseline, on the CPU, with an
+analytic stand-in for the batched simulator and for the network (both obey the simulator's exact
+E-viscosity symmetry: the normalised curve depends on viscosity / E, not on E)."""
+
+import numpy as np
+
+from nofluidx3d_amd.batch import CellSpec
+from nofluidx3d_amd.surrogate import fit
+from nofluidx3d_amd.surrogate.train import Normaliser, scale_shift
+
+NC = 256
+S = np.linspace(0.0, 1.0, NC)
+
+
+def shape(nu, log_ratio, tip, speed, R):
+    """normalised curve of an analytic 'cell' (smooth in every input)"""
+    k = 0.02 * (1.0 + nu) * np.sqrt(tip / (R + tip))
+    relax = 1.0 / (1.0 + 10 ** (log_ratio + 2.5) * speed / R)
+    return k[:, None] * S[None, :] ** 1.5 * (1.0 - 0.6 * relax[:, None] * S[None, :])
+
+
+class FakeBatch:
+    steps = 10000
+
+    def reset(self, specs):
+        self.specs = specs
+
+    def advance(self, n):
+        pass
+
+    def curves_normalized(self):
+        sp = self.specs
+        return shape(np.array([s.nu for s in sp]), np.log10([s.viscosity_si / s.E_si for s in sp]),
+                     np.array([s.tip_radius_um for s in sp]), np.array([s.speed_um_s for s in sp]),
+                     np.array([s.radius_um for s in sp]))
+
+
+class FakeNet:
+    def __init__(self, norm):
+        self.norm = norm
+
+    def predict(self, Xn):
+        X = np.asarray(Xn) * self.norm.x_std + self.norm.x_mean
+        return shape(X[:, 1], X[:, 2] - X[:, 0], 10 ** X[:, 3], 10 ** X[:, 4], 10 ** X[:, 5]) / self.norm.y_scale
+
+
+def cells(n, seed=3):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        out.append(CellSpec(rng.uniform(4, 10), 0.5, i, 10 ** rng.uniform(2.3, 3.7), rng.uniform(0.32, 0.47),
+                            10 ** rng.uniform(0.8, 2.8), rng.uniform(1, 5), 10 ** rng.uniform(0, 1.5)))
+    return out
+
+
+def blind(cs):
+    import dataclasses
+
+    return [dataclasses.replace(c, E_si=1000.0, nu=0.4, viscosity_si=10.0) for c in cs]
+
+
+def test_symmetry_helpers_keep_the_box_and_the_ratio():
+    rng = np.random.default_rng(0)
+    th = np.stack([rng.uniform(1, 5, 500), rng.uniform(0.2, 0.6, 500), rng.uniform(-1, 4, 500)], 1)
+    c = fit.clip_theta(th)
+    assert np.all(c >= fit.LO) and np.all(c <= fit.HI)
+    inside = np.all((th >= fit.LO) & (th <= fit.HI), axis=1)
+    assert np.array_equal(c[inside], th[inside])
+    X = np.stack([rng.uniform(2, 4, 300), rng.uniform(0.3, 0.49, 300), rng.uniform(0, 3, 300)] +
+                 [rng.uniform(0, 1, 300)] * 3, 1)
+    Xs = scale_shift(X, rng)
+    assert np.allclose(Xs[:, 2] - Xs[:, 0], X[:, 2] - X[:, 0])
+    assert np.all(Xs[:, 0] >= 2 - 1e-12) and np.all(Xs[:, 0] <= 4 + 1e-12)
+    assert np.all(Xs[:, 2] >= -1e-12) and np.all(Xs[:, 2] <= 3 + 1e-12)
+    assert np.array_equal(Xs[:, 1:2], X[:, 1:2]) and np.array_equal(Xs[:, 3:], X[:, 3:])
+
+
+def test_stabilize_enforces_the_relaxation_limit():
+    cs = cells(20)
+    T0 = fit.time_units(cs, 10000)
+    th = np.array([[3.5, 0.35, 0.0]] * 20)
+    st = fit.stabilize(th, T0)
+    for c, t in zip(cs, st):
+        spec = fit.theta_to_spec(t, c)
+        assert spec.relaxation_per_step(10000) <= 0.1 * (1 + 1e-9)
+    loose = np.array([[2.0, 0.35, 3.0]] * 20)
+    assert np.array_equal(fit.stabilize(loose, T0), loose)
+
+
+def test_surrogate_fit_recovers_parameters_with_an_exact_network():
+    cs = cells(24)
+    bs = FakeBatch()
+    targets = fit.simulate(bs, cs)
+    X = np.stack([np.log10([c.E_si for c in cs]), [c.nu for c in cs], np.log10([c.viscosity_si for c in cs]),
+                  np.log10([c.tip_radius_um for c in cs]), np.log10([c.speed_um_s for c in cs]),
+                  np.log10([c.radius_um for c in cs])], 1)
+    norm = Normaliser(X.mean(0), X.std(0) + 1e-12, 0.01)
+    r = fit.fit_surrogate(FakeNet(norm), norm, targets, blind(cs), bs, rounds=8, ncand=64, ntop=2)
+    assert np.median(r["sim_err"]) < 2e-3, np.median(r["sim_err"])
+    assert r["simulations_per_target"] == 3
+    truth = np.array([[np.log10(c.E_si), c.nu, np.log10(c.viscosity_si)] for c in cs])
+    # the amplitude fixes E exactly once the shape (nu, viscosity / E) is right
+    assert np.median(np.abs(r["theta"][:, 0] - truth[:, 0])) < 0.02
+    assert np.median(np.abs((r["theta"][:, 2] - r["theta"][:, 0]) - (truth[:, 2] - truth[:, 0]))) < 0.05
+
+
+def test_nelder_mead_history_is_monotone_and_improves():
+    cs = cells(16, seed=9)
+    bs = FakeBatch()
+    targets = fit.simulate(bs, cs)
+    r = fit.fit_nelder_mead(targets, blind(cs), bs, max_evals=40)
+    h = r["history"]
+    assert h.shape == (r["evaluations"], 16)
+    assert np.all(np.diff(h, axis=0) <= 1e-15)
+    assert np.median(h[-1]) < 0.5 * np.median(h[3])
+    assert np.allclose(np.min(h, axis=0), r["sim_err"])

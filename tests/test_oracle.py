@@ -366,6 +366,99 @@ def test_vp_shape_form_update_is_the_reference_update():
 
 
 # --------------------------------------------------------------------------------------------
+# legacy stress laws (InteractionLinearElastic / LinearElasticDeviatoric / SecondOrderNeoHookean /
+# PoroNeoHookean .cl, restated with the node-0 base)
+# --------------------------------------------------------------------------------------------
+
+
+def legacy_forces(law, P, X, x, tets=None):
+    _, t, ev, _ = one_tet(X, tets)
+    pts = O.soa(x)
+    f = np.zeros_like(pts)
+    vm, pr = np.zeros(1), np.zeros(1)
+    O.legacy_force(law, P, pts, t, ev, f, vm, pr)
+    return f.reshape(3, -1).T, vm[0], pr[0]
+
+
+def hooke(F, mu, lam):
+    eps = 0.5 * (F + F.T) - np.eye(3)
+    return 2 * mu * eps + lam * np.trace(eps) * np.eye(3)
+
+
+def test_legacy_linear_laws_closed_form_and_cross_identity():
+    """The face forces carry the law's Cauchy stress exactly (virial), Hooke's law for the
+    linear law, and the deviatoric law with K = lambda + 2 mu / 3 is the linear law."""
+    E, nu = 0.01, 0.4
+    mu, lam = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    K = lam + 2 * mu / 3
+    for _ in range(4):
+        F = random_F(0.2)
+        x = deform(F, REF_TET, RNG.normal(size=3))
+        f0, vm0, pr0 = legacy_forces(0, O.LegacyParams(E=E, nu=nu), REF_TET, x)
+        f1, vm1, pr1 = legacy_forces(1, O.LegacyParams(shear=mu, bulk=K), REF_TET, x)
+        sig = hooke(F, mu, lam)
+        assert np.allclose(virial_stress(REF_TET, x, f0), sig, rtol=1e-10, atol=1e-15)
+        assert np.allclose(f1, f0, rtol=1e-10, atol=1e-16)
+        dev = sig - np.trace(sig) / 3 * np.eye(3)
+        assert vm0 == pytest.approx(np.sqrt(1.5 * np.sum(dev * dev)), rel=1e-10)
+        assert pr0 == pytest.approx(np.trace(sig) / 3, rel=1e-10)
+        assert np.allclose(f0.sum(0), 0, atol=1e-15)
+    # small strain: the forces are the gradient of the small-strain energy up to O(strain^2)
+    x = deform(np.eye(3) + 1e-4 * RNG.uniform(-1, 1, (3, 3)), REF_TET)
+    f0 = legacy_forces(0, O.LegacyParams(E=E, nu=nu), REF_TET, x)[0]
+
+    def w(F):
+        eps = 0.5 * (F + F.T) - np.eye(3)
+        return mu * np.sum(eps * eps) + 0.5 * lam * np.trace(eps) ** 2
+
+    assert np.allclose(f0, energy_forces(w, REF_TET, x, h=1e-7), rtol=2e-3, atol=1e-3 * np.abs(f0).max())
+
+
+def test_legacy_second_order_neo_hookean_energy_consistency():
+    mu, mu2, K = 0.004, 0.003, 0.02
+    P = O.LegacyParams(shear=mu, bulk=K, shear2=mu2)
+
+    def psi(F):  # mu/2 (I1b - 3) + mu2/4 (I1b - 3)^2 + K/2 (J - 1)^2
+        J = np.linalg.det(F)
+        I1b = J ** (-2 / 3) * np.trace(F.T @ F)
+        return 0.5 * mu * (I1b - 3) + 0.25 * mu2 * (I1b - 3) ** 2 + 0.5 * K * (J - 1) ** 2
+
+    for _ in range(4):
+        x = deform(random_F(0.25), REF_TET) + 0.03 * RNG.normal(size=(4, 3))
+        f = legacy_forces(2, P, REF_TET, x)[0]
+        assert np.allclose(f, energy_forces(psi, REF_TET, x), rtol=1e-6, atol=1e-9 * np.abs(f).max())
+    f, vm, pr = legacy_forces(2, P, REF_TET, REF_TET.copy())
+    assert np.max(np.abs(f)) < 1e-15 and vm == 0 and pr == 0
+
+
+def test_legacy_poro_neo_hookean_energy_consistency():
+    """sigma = a B + b I with J sigma = 2 (dPsi/dI1) B + J (dPsi/dJ) I for Psi = I1 g(J),
+    g = a J / 2: the b of the law is I1 g'(J) exactly when the law is hyperelastic."""
+    mu, phi = 0.004, 0.3
+
+    def g(J):
+        eta = 1 + (J - 1) / phi
+        return 0.25 * mu * (4 - 2 / J - 2 * (phi + 2 * (J - 1)) / (J ** (2 / 3) * eta ** (1 / 3)))
+
+    def psi(F):
+        return np.trace(F.T @ F) * g(np.linalg.det(F))
+
+    P = O.LegacyParams(shear=mu, phi=phi)
+    for _ in range(4):
+        x = deform(random_F(0.2), REF_TET) + 0.02 * RNG.normal(size=(4, 3))
+        f = legacy_forces(3, P, REF_TET, x)[0]
+        assert np.allclose(f, energy_forces(psi, REF_TET, x), rtol=1e-6, atol=1e-9 * np.abs(f).max())
+    f, vm, pr = legacy_forces(3, P, REF_TET, REF_TET.copy())
+    assert np.max(np.abs(f)) < 1e-15
+    # objectivity: a rigid rotation of the deformed state rotates the forces
+    Q = scipy.linalg.expm(np.array([[0, -0.3, 0.2], [0.3, 0, -0.1], [-0.2, 0.1, 0]]))
+    x = deform(random_F(0.2), REF_TET)
+    fa = legacy_forces(3, P, REF_TET, x)[0]
+    fb = legacy_forces(3, P, REF_TET, x @ Q.T)[0]
+    assert np.allclose(fb, fa @ Q.T, rtol=1e-10, atol=1e-16)
+
+
+# --------------------------------------------------------------------------------------------
 # contacts, integrator, tip force
 # --------------------------------------------------------------------------------------------
 

@@ -34,6 +34,28 @@ def test_gradients_match_torch_autograd():
         assert np.allclose(gb[l], tb[l].grad.numpy(), rtol=1e-10, atol=1e-14)
 
 
+def test_row_weighted_loss_matches_torch_autograd():
+    Ws, bs = net()
+    X, T = RNG.normal(size=(13, 6)), RNG.normal(size=(13, 10))
+    w = RNG.uniform(0.2, 3.0, size=13)
+    loss, gW, gb = M.loss_and_grads(Ws, bs, X, T, w)
+    tW = [torch.tensor(x, requires_grad=True) for x in Ws]
+    tb = [torch.tensor(b, requires_grad=True) for b in bs]
+    h = torch.tensor(X)
+    for l in range(len(tW)):
+        h = h @ tW[l].T + tb[l]
+        if l < len(tW) - 1:
+            h = torch.nn.functional.gelu(h)
+    tl = (torch.tensor(w)[:, None] * (h - torch.tensor(T)) ** 2).mean()
+    tl.backward()
+    assert abs(loss - tl.item()) < 1e-13
+    for l in range(len(Ws)):
+        assert np.allclose(gW[l], tW[l].grad.numpy(), rtol=1e-10, atol=1e-14)
+        assert np.allclose(gb[l], tb[l].grad.numpy(), rtol=1e-10, atol=1e-14)
+    # uniform weights reduce to the plain mean
+    assert np.isclose(M.loss_and_grads(Ws, bs, X, T, np.full(13, 2.0))[0], 2 * M.loss_and_grads(Ws, bs, X, T)[0])
+
+
 def test_gradients_match_central_differences():
     Ws, bs = net((6, 8, 5))
     X, T = RNG.normal(size=(9, 6)), RNG.normal(size=(9, 5))
