@@ -208,6 +208,36 @@ int nfx_velocity_verlet(nfx_ctx* ctx, float* points, float* vel, float* force, f
 int nfx_tip_force(nfx_ctx* ctx, const float* points, int64_t N, int32_t indenter, double k, double radius,
                   double pos, double* out2);
 
+/* ---- legacy contacts and integrators (the kernels SimulationLegacy.py:139-330, 550-617 and
+ *      884-940 use), restated as written; force/points/vel/fold are 3N SoA FP32 ------------------ */
+/* Interactions/RoundTip.cl: sphere below its centre, cylinder above; force 0.1 max(|r| - R, 0) r/|r| */
+int nfx_round_tip_force(nfx_ctx* ctx, float* force, const float* points, int64_t N, double radius, double sphere_pos);
+/* InteractionAdhesivePlane.cl: Lennard-Jones wall, sigma = 2^-1/6 distance, on points with on_surface == 1 */
+int nfx_adhesive_plane_force(nfx_ctx* ctx, float* force, const float* points, int64_t N, double wall_pos,
+                             double adhesion, double distance, const int32_t* on_surface);
+/* PointZeroForce.cl: force -= k (x of node 0, 0, z of node 0) on every node */
+int nfx_point_zero_force(nfx_ctx* ctx, float* force, const float* points, int64_t N, double k);
+/* CalcCOM.cl: com[0] = mean x, com[1] = mean z (FP64, device, deterministic order) */
+int nfx_center_of_mass(nfx_ctx* ctx, const float* points, int64_t N, double* com);
+/* COMForce.cl: force -= k (com[0], 0, com[1]) on every node */
+int nfx_com_force(nfx_ctx* ctx, float* force, int64_t N, double k, const double* com);
+/* OverDamped.cl: x += f (node 3 keeps x and z when point3fix), f = 0 */
+int nfx_over_damped(nfx_ctx* ctx, float* points, float* force, int64_t N, int32_t point3fix);
+/* velocityVerlet2.cl: per-node masses (N), damping; fold holds the previous acceleration */
+int nfx_velocity_verlet2(nfx_ctx* ctx, float* points, float* vel, float* force, float* fold, const float* masses,
+                         int64_t N, double damping);
+/* InteractionSphereIntegral.cl (tip_shape 0: sphere at (0, tip_y, 0)) and InteractionTipIntegral.cl
+ * (tip_shape 1: sphere of radius R with a cone of half angle half_angle, apex region at tip_x/y/z):
+ * body force exp(-k dis) n by Bower's 4-point rule per tetra (scaled by |det r| / 4 as written);
+ * float-atomic scatter into force, or per-corner records (corners[12*t + 3*a + c]) when corners != NULL */
+int nfx_volume_integral_force(nfx_ctx* ctx, int32_t tip_shape, const float* points, const uint32_t* tetras, int64_t N,
+                              int64_t T, double radius, double half_angle, double tip_x, double tip_y, double tip_z,
+                              double k, float* force, float* corners);
+/* InteractionAdhesivePlaneSurfaceIntegral.cl: Lennard-Jones area density over the surface triangles
+ * (3 x NT SoA) by the 3-point edge-midpoint rule, y component, float-atomic scatter */
+int nfx_adhesive_plane_surface_force(nfx_ctx* ctx, float* force, const float* points, const uint32_t* triangles,
+                                     int64_t N, int64_t NT, double wall_pos, double adhesion, double distance);
+
 /* ---- fused stepping: Simulation.timeStep (Simulation.py:232-237) for steps
  *      [step0, step0 + nsteps): force pass, then one node pass that applies the substrate and
  *      indenter terms at the device-computed indenter position (Simulation.py:117-132, 173-183)

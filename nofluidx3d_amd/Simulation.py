@@ -33,6 +33,9 @@ from nofluidx3d_amd import _lib, openCL
 from nofluidx3d_amd.interactions import (FiniteStrainViscoplastic, LegacyElastic, LinearElastic,
                                         LinearElasticDeviatoric, Plane, PoroNeoHookean, SecondOrderNeoHookean,
                                         Sphere, Substrate, Tetra, VelocityVerlet)
+from nofluidx3d_amd.interactions.LegacyContacts import (AdhesivePlane, AdhesivePlaneSurfaceIntegral, CalcCOM,
+                                                        COMForce, OverDamped, PointZeroForce, RoundTip,
+                                                        SphereIntegral, TipIntegral, VelocityVerlet2)
 from nofluidx3d_amd.openCL import getCommandQueue, initializeOpenCLObjects
 from nofluidx3d_amd.TetraCell import TetraCell
 from nofluidx3d_amd.util.Bridge import Bridge
@@ -249,6 +252,115 @@ class Simulation:
         c = self.parameters["CELL"]
         shear = self._legacy_youngs() / (2 * (1 + 0.5))
         self.register(PoroNeoHookean(self.cell, shear, c["PorousVolumeFraction"]))
+
+    # legacy contacts and integrators (SimulationLegacy.py:139-330, 550-617, 884-940)
+    def _tip_start(self, radius, isSpherical):
+        if isSpherical:
+            return self.parameters["CELL"]["RadiusSIM"] + radius + self.parameters["InitialDistance"]
+        return self.cell.points[3][1] + radius + self.parameters["InitialDistance"]
+
+    def _legacy_tip(self, cls, record, isSpherical, **extra):
+        radius = self.parameters["SphereRadiusSI"] / self.L0
+        fn = self._indenter_law(self._tip_start(radius, isSpherical))
+        forceConst = self.parameters["PotentialForceConst"]
+        inter = cls(radius=radius, sphereFunc=fn, forceConst=forceConst, **extra)
+        self.register(inter)
+        if record:
+            scale = self.p0 * self.L0**2 * self.parameters.get("ReynoldScaling", 1.0) * \
+                self.parameters["CELL"].get("YoungsScaling", 1.0)
+
+            def indentationSI():
+                return (fn(0) - fn(self.time)) * self.L0
+
+            def forceSI():
+                if cls is RoundTip:  # SimulationLegacy.py:185-200: exp law summed over the nodes
+                    x = self.positions()
+                    c = np.array([0, fn(self.time), 0])
+                    L = np.linalg.norm(x - c, axis=-1)
+                    n = (x - c) / L[:, None]
+                    return -np.sum(np.exp(-forceConst * (L - radius)) * n[:, 1]) * scale
+                # SimulationLegacy.py:248-271: the same integral into a separate force buffer
+                f = self._record_force(inter)
+                return -np.sum(f[:, 1]) * scale
+
+            self.recordedQuantities.append(indentationSI)
+            self.recordedQuantities.append(forceSI)
+        return inter
+
+    def _record_force(self, inter):
+        import torch
+
+        saved = self.force.buf.clone()
+        self.force.buf.zero_()
+        torch.cuda.synchronize()
+        inter.beforeTimeStep(self.time)
+        inter.enqueue()
+        getCommandQueue().finish()
+        f = self.force.buf.cpu().numpy().astype(np.float64).reshape(3, -1).T.copy()
+        self.force.buf.copy_(saved)
+        torch.cuda.synchronize()
+        return f
+
+    def setInteractionRoundTip(self, record=True, isSpherical=True):
+        return self._legacy_tip(RoundTip, record, isSpherical)
+
+    def setInteractionSphereIntegral(self, record=True, isSpherical=True):
+        return self._legacy_tip(SphereIntegral, record, isSpherical)
+
+    def setInteractionTipIntegral(self, record=True, isSpherical=True):
+        halfAngle = self.parameters["TipHalfAngleDeg"] / 360.0 * 2 * np.pi
+        return self._legacy_tip(TipIntegral, record, isSpherical, halfAngle=halfAngle,
+                                spherePosX=self.cell.points[3][0], spherePosZ=self.cell.points[3][2])
+
+    def _adhesion_wall(self, isSpherical):
+        if isSpherical:
+            return -self.parameters["CELL"]["RadiusSIM"] - self.parameters["AdhesivePlaneDistance"]
+        return self.cell.points[4][1] - self.parameters["AdhesivePlaneDistance"]
+
+    def _adhesion_const(self):
+        """AdhesionConstant in simulation units if given, else AdhesionConstantSI over the energy
+        unit p0 L0^3 of this unit system (the legacy code divides by its own energy unit E0)."""
+        if "AdhesionConstant" in self.parameters:
+            return self.parameters["AdhesionConstant"]
+        return self.parameters["AdhesionConstantSI"] / (self.p0 * self.L0**3)
+
+    def _surface_triangles(self):
+        cd = self.cell.mesh.cells_dict
+        if 5 in cd:
+            return cd[5]
+        from nofluidx3d_amd.interactions.LegacyContacts import surface_triangles
+
+        return surface_triangles(self.cell.tetras)
+
+    def setInteractionAdhesivePlane(self, isSpherical=True):
+        wall = self._adhesion_wall(isSpherical)
+        on = np.isin(np.arange(self.numPoints), np.unique(self._surface_triangles())).astype(np.int32)
+        self.register(AdhesivePlane(lambda t: wall, self._adhesion_const(), self.parameters["AdhesivePlaneDistance"],
+                                    on))
+
+    def setInteractionAdhesivePlaneSurfaceIntegral(self, isSpherical=True):
+        wall = self._adhesion_wall(isSpherical)
+        self.register(AdhesivePlaneSurfaceIntegral(lambda t: wall, self._adhesion_const(),
+                                                   self.parameters["AdhesivePlaneDistance"],
+                                                   self._surface_triangles()))
+
+    def setInteractionCOMForce(self):
+        calc = CalcCOM()
+        self.register(calc)
+        self.register(COMForce(self.parameters["COMSpringConst"], calc))
+        self.com = calc
+
+    def setInteractionPointZeroForce(self):
+        self.register(PointZeroForce(self.parameters["COMSpringConst"]))
+
+    # integrators: register last
+    def setInteractionVelocityVerlet2(self, damping=0.1):
+        from nofluidx3d_amd.interactions.LegacyContacts import lumped_masses
+
+        self.register(VelocityVerlet2(lumped_masses(self.cell.tetras, self.cell.volumes, self.numPoints), damping))
+
+    def setInteractionOverDamped(self, point3fix):
+        self.register(OverDamped(point3fix))
 
     # Velocity Verlet Interaction, always add time integration interactions as last element
     def setInteractionVelocityVerlet(self, fixTopBottom=False):

@@ -153,6 +153,16 @@ SIGNATURES = {
                                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_vp_shape_from_fp": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_round_tip_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
+    "nfx_adhesive_plane_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64, c_f64, c_vp]),
+    "nfx_point_zero_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64]),
+    "nfx_center_of_mass": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "nfx_com_force": (ctypes.c_int, [c_vp, c_vp, c_i64, c_f64, c_vp]),
+    "nfx_over_damped": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i32]),
+    "nfx_velocity_verlet2": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_f64]),
+    "nfx_volume_integral_force": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, c_i64, c_i64, c_f64, c_f64, c_f64, c_f64,
+                                                  c_f64, c_f64, c_vp, c_vp]),
+    "nfx_adhesive_plane_surface_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_f64, c_f64, c_f64]),
     "nfx_legacy_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Elastic), c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp,
                                          c_vp, c_vp]),
     "nfx_vp_fp_from_shape": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),

@@ -106,6 +106,16 @@ def lib():
         L.orc_legacy_force.argtypes = [ctypes.c_int, ctypes.POINTER(LegacyParams), i64, i64, _dp, _up, _dp, _dp,
                                        _dp, _dp, ctypes.c_int]
         L.orc_sphere.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        f64, _ip = ctypes.c_double, ctypes.POINTER(ctypes.c_int32)
+        L.orc_round_tip.argtypes = [i64, _dp, _dp, f64, f64]
+        L.orc_adhesive_plane.argtypes = [i64, _dp, _dp, f64, f64, f64, _ip]
+        L.orc_point_zero_force.argtypes = [i64, _dp, _dp, f64]
+        L.orc_center_of_mass.argtypes = [i64, _dp, _dp]
+        L.orc_com_force.argtypes = [i64, _dp, f64, _dp]
+        L.orc_over_damped.argtypes = [i64, ctypes.c_int, _dp, _dp]
+        L.orc_velocity_verlet2.argtypes = [i64, _dp, _dp, _dp, _dp, _dp, f64]
+        L.orc_volume_quadrature.argtypes = [ctypes.c_int, i64, i64, _dp, _dp, _up, f64, f64, f64, f64, f64, f64]
+        L.orc_adhesive_plane_surface.argtypes = [i64, i64, _dp, _dp, _up, f64, f64, f64]
         L.orc_substrate.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double]
         L.orc_plane.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double]
         L.orc_velocity_verlet.argtypes = [i64, ctypes.c_int, _dp, _dp, _dp, _dp]
@@ -204,6 +214,51 @@ def legacy_force(law, P: LegacyParams, points, tetras, ev, force, vonMises=None,
     T = tetras.size // 4
     lib().orc_legacy_force(law, ctypes.byref(P), N, T, _d(points), _u(tetras), _d(ev), _d(force), _dn(vonMises),
                            _dn(pressure), nthreads)
+
+
+def _i32(a):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def round_tip(force, points, radius, pos):
+    lib().orc_round_tip(points.size // 3, _d(force), _d(points), radius, pos)
+
+
+def adhesive_plane(force, points, wall, adhesion, distance, on_surface):
+    lib().orc_adhesive_plane(points.size // 3, _d(force), _d(points), wall, adhesion, distance, _i32(on_surface))
+
+
+def point_zero_force(force, points, k):
+    lib().orc_point_zero_force(points.size // 3, _d(force), _d(points), k)
+
+
+def center_of_mass(points):
+    com = np.zeros(2)
+    lib().orc_center_of_mass(points.size // 3, _d(points), _d(com))
+    return com
+
+
+def com_force(force, k, com):
+    lib().orc_com_force(force.size // 3, _d(force), k, _d(np.ascontiguousarray(com, np.float64)))
+
+
+def over_damped(points, force, point3fix):
+    lib().orc_over_damped(points.size // 3, int(point3fix), _d(points), _d(force))
+
+
+def velocity_verlet2(points, vel, force, fold, masses, damping):
+    lib().orc_velocity_verlet2(points.size // 3, _d(points), _d(vel), _d(force), _d(fold), _d(masses), damping)
+
+
+def volume_quadrature(tip_shape, force, points, tetras, radius, half_angle, tip_pos, k):
+    lib().orc_volume_quadrature(int(tip_shape), points.size // 3, tetras.size // 4, _d(force), _d(points),
+                                _u(tetras), radius, half_angle, tip_pos[0], tip_pos[1], tip_pos[2], k)
+
+
+def adhesive_plane_surface(force, points, triangles, wall, adhesion, distance):
+    lib().orc_adhesive_plane_surface(points.size // 3, triangles.size // 3, _d(force), _d(points), _u(triangles),
+                                     wall, adhesion, distance)
 
 
 def sphere(k, force, points, radius, pos):

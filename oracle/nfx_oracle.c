@@ -13,6 +13,12 @@
  *                        uses node 0), plus an optional yield offset (0 = reference law)
  *   orc_legacy_force     kernels/InteractionLinearElastic.cl, InteractionLinearElasticDeviatoric.cl,
  *                        InteractionSecondOrderNeoHookean.cl, InteractionPoroNeoHookean.cl (node-0 base)
+ *   orc_round_tip, orc_adhesive_plane, orc_point_zero_force, orc_center_of_mass, orc_com_force,
+ *   orc_over_damped, orc_velocity_verlet2, orc_volume_quadrature, orc_adhesive_plane_surface
+ *                        the legacy contacts and integrators (RoundTip.cl, InteractionAdhesivePlane.cl,
+ *                        PointZeroForce.cl, CalcCOM.cl, COMForce.cl, OverDamped.cl, velocityVerlet2.cl,
+ *                        InteractionSphereIntegral.cl, InteractionTipIntegral.cl,
+ *                        InteractionAdhesivePlaneSurfaceIntegral.cl)
  *   orc_sphere           kernels/Interactions/Sphere.cl:1-17
  *   orc_substrate        kernels/Interactions/Substrate.cl:1-7
  *   orc_plane            kernels/Interactions/Plane.cl:1-7
@@ -512,6 +518,168 @@ void orc_legacy_force(int law, const orc_legacy_params* P, int64_t N, int64_t T,
     }
 #pragma omp parallel for num_threads(nthreads) schedule(static)
     for (int64_t t = 0; t < T; t++) legacy_one(t, law, P, N, T, points, tetras, ev, force, vonMises, pressure, 1);
+}
+
+/* ---- legacy contacts and integrators (the kernels under kernels/ used by SimulationLegacy.py:139-330, 550-617,
+ *      884-940), restated as written ------------------------------------------------------------ */
+
+/* Interactions/RoundTip.cl:1-21: sphere below its centre, a cylinder above it; prefactor 0.1 (the
+ * kernel ignores its forceConst argument); force k max(|r| - R, 0) r/|r| as written */
+void orc_round_tip(int64_t N, double* force, const double* points, double radius, double spherePos) {
+    for (int64_t p = 0; p < N; p++) {
+        const d3 pos = v3(points[p], points[N + p], points[2 * N + p]);
+        d3 r = vsub(pos, v3(0, spherePos, 0));
+        if (pos.y > spherePos) r.y = 0;
+        const double len = vlen(r);
+        const d3 n = len > 0 ? vscale(r, 1.0 / len) : v3(0, 0, 0);
+        const double dis = fmax(len - radius, 0.);
+        const d3 fc = vscale(n, 0.1 * dis);
+        force[p] += fc.x;
+        force[N + p] += fc.y;
+        force[2 * N + p] += fc.z;
+    }
+}
+
+/* InteractionAdhesivePlane.cl:1-14: Lennard-Jones wall on surface points, sigma = 2^(-1/6) rm */
+void orc_adhesive_plane(int64_t N, double* force, const double* points, double wallPos, double adhesion,
+                        double distance, const int32_t* onSurface) {
+    const double sigma = pow(0.5, 1.0 / 6.0) * distance;
+    for (int64_t p = 0; p < N; p++) {
+        if (onSurface[p] != 1) continue;
+        const double d = points[N + p] - wallPos;
+        force[N + p] += 24.0 * adhesion / d * (2 * pow(sigma / d, 12) - pow(sigma / d, 6));
+    }
+}
+
+/* PointZeroForce.cl:1-10: every point is pulled by -k times node 0's x and z */
+void orc_point_zero_force(int64_t N, double* force, const double* points, double k) {
+    for (int64_t p = 0; p < N; p++) {
+        force[p] += -k * points[0];
+        force[2 * N + p] += -k * points[2 * N];
+    }
+}
+
+/* CalcCOM.cl:1-8 (com[0] = mean x, com[1] = mean z) and COMForce.cl:1-7 */
+void orc_center_of_mass(int64_t N, const double* points, double* com) {
+    for (int id = 0; id < 2; id++) {
+        double sum = 0.0;
+        for (int64_t i = 0; i < N; i++) sum += points[2 * id * N + i];
+        com[id] = sum / (double)N;
+    }
+}
+void orc_com_force(int64_t N, double* force, double k, const double* com) {
+    for (int64_t p = 0; p < N; p++) {
+        force[p] += -k * com[0];
+        force[2 * N + p] += -k * com[1];
+    }
+}
+
+/* OverDamped.cl:1-21: x += f (point 3 keeps x and z when point3fix), f = 0 */
+void orc_over_damped(int64_t N, int point3fix, double* points, double* force) {
+    for (int64_t p = 0; p < N; p++) {
+        d3 f = v3(force[p], force[N + p], force[2 * N + p]);
+        if (point3fix && p == 3) f.x = f.z = 0.0;
+        points[p] += f.x;
+        points[N + p] += f.y;
+        points[2 * N + p] += f.z;
+        force[p] = force[N + p] = force[2 * N + p] = 0.0;
+    }
+}
+
+/* velocityVerlet2.cl:1-26: per-point masses, damping xi; fOld holds the previous acceleration */
+void orc_velocity_verlet2(int64_t N, double* points, double* vel, double* force, double* fold, const double* masses,
+                          double damping) {
+    for (int64_t p = 0; p < N; p++) {
+        d3 x = v3(points[p], points[N + p], points[2 * N + p]);
+        d3 v = v3(vel[p], vel[N + p], vel[2 * N + p]);
+        const d3 f = v3(force[p], force[N + p], force[2 * N + p]);
+        const d3 fo = v3(fold[p], fold[N + p], fold[2 * N + p]);
+        const double m = masses[p];
+        v = vscale(vadd(v, vscale(vadd(fo, vdivs(f, m)), 0.5)), 1.0 / (1 + damping / 2.0));
+        const d3 acc = vsub(vdivs(f, m), vscale(v, damping));
+        x = vadd(vadd(x, v), vscale(acc, 0.5));
+        points[p] = x.x; points[N + p] = x.y; points[2 * N + p] = x.z;
+        vel[p] = v.x; vel[N + p] = v.y; vel[2 * N + p] = v.z;
+        fold[p] = acc.x; fold[N + p] = acc.y; fold[2 * N + p] = acc.z;
+        force[p] = force[N + p] = force[2 * N + p] = 0.0;
+    }
+}
+
+/* InteractionSphereIntegral.cl:1-85 and InteractionTipIntegral.cl:1-114: the body-force density
+ * exp(-k dis) n integrated over each tetra with Bower's 4-point rule; node order of the legacy
+ * kernels (p1..p3 = corners 0..2 of the tetra array, p4 = corner 3 as there). */
+static d3 force_sphere(d3 pos, double radius, double spherePos, double k) {
+    const d3 r = vsub(pos, v3(0, spherePos, 0));
+    const d3 n = vdivs(r, vlen(r));
+    return vscale(n, exp(-k * (vlen(r) - radius)));
+}
+static d3 force_tip(d3 pos, double radius, double halfAngle, d3 tip, double k) {
+    const d3 rho = v3(pos.x - tip.x, 0, pos.z - tip.z);
+    if (pos.y < (tip.y - sin(halfAngle) * vlen(rho))) {
+        const d3 r = vsub(pos, tip);
+        const d3 n = vdivs(r, vlen(r));
+        return vscale(n, exp(-k * (vlen(r) - radius)));
+    }
+    const d3 eRho = vdivs(rho, vlen(rho));
+    const d3 n = vadd(vscale(v3(0, 1, 0), -sin(halfAngle)), vscale(eRho, cos(halfAngle)));
+    const double dis = vdot(pos, n) - vdot(tip, n) - radius;
+    return vscale(n, exp(-k * dis));
+}
+void orc_volume_quadrature(int tip_shape, int64_t N, int64_t T, double* force, const double* points,
+                           const uint32_t* tetras, double radius, double halfAngle, double tipX, double tipY,
+                           double tipZ, double k) {
+    const double alpha = 0.58541020, beta = 0.13819669;
+    const double xi[4][3] = {{alpha, beta, beta}, {beta, alpha, beta}, {beta, beta, alpha}, {beta, beta, beta}};
+    for (int64_t t = 0; t < T; t++) {
+        uint32_t id[4];
+        d3 p[4];
+        for (int a = 0; a < 4; a++) {
+            id[a] = tetras[a * T + t];
+            p[a] = v3(points[id[a]], points[N + id[a]], points[2 * N + id[a]]);
+        }
+        const d3 r1 = vsub(p[0], p[3]), r2 = vsub(p[1], p[3]), r3 = vsub(p[2], p[3]);
+        const double volume = fabs(vdot(r1, vcross(r2, r3)));
+        d3 F[4] = {v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0), v3(0, 0, 0)};
+        for (int q = 0; q < 4; q++) {
+            const double Nq[4] = {xi[q][0], xi[q][1], xi[q][2], 1.0 - xi[q][0] - xi[q][1] - xi[q][2]};
+            const d3 x = vadd(vadd(vadd(vscale(p[0], Nq[0]), vscale(p[1], Nq[1])), vscale(p[2], Nq[2])),
+                              vscale(p[3], Nq[3]));
+            const d3 fq = tip_shape ? force_tip(x, radius, halfAngle, v3(tipX, tipY, tipZ), k)
+                                    : force_sphere(x, radius, tipY, k);
+            for (int a = 0; a < 4; a++) F[a] = vadd(F[a], vscale(fq, Nq[a]));
+        }
+        for (int a = 0; a < 4; a++) {
+            force[id[a]] += 0.25 * volume * F[a].x;
+            force[N + id[a]] += 0.25 * volume * F[a].y;
+            force[2 * N + id[a]] += 0.25 * volume * F[a].z;
+        }
+    }
+}
+
+/* InteractionAdhesivePlaneSurfaceIntegral.cl:1-56: the Lennard-Jones area density integrated over
+ * each surface triangle with Bower's 3-point (edge mid-point) rule, y component */
+void orc_adhesive_plane_surface(int64_t N, int64_t NT, double* force, const double* points, const uint32_t* tri,
+                                double wallPos, double adhesion, double distance) {
+    const double sigma = pow(0.5, 1.0 / 6.0) * distance;
+    const double xi[3][2] = {{0.5, 0.0}, {0.0, 0.5}, {0.5, 0.5}};
+    for (int64_t t = 0; t < NT; t++) {
+        uint32_t id[3];
+        d3 p[3];
+        for (int a = 0; a < 3; a++) {
+            id[a] = tri[a * NT + t];
+            p[a] = v3(points[id[a]], points[N + id[a]], points[2 * N + id[a]]);
+        }
+        const double area = 0.5 * vlen(vcross(vsub(p[0], p[2]), vsub(p[1], p[2])));
+        double F[3] = {0, 0, 0};
+        for (int q = 0; q < 3; q++) {
+            const double Nq[3] = {xi[q][0], xi[q][1], 1.0 - xi[q][0] - xi[q][1]};
+            const double y = Nq[0] * p[0].y + Nq[1] * p[1].y + Nq[2] * p[2].y;
+            const double d = y - wallPos;
+            const double fq = 24.0 * adhesion / d * (2 * pow(sigma / d, 12) - pow(sigma / d, 6));
+            for (int a = 0; a < 3; a++) F[a] += fq * Nq[a];
+        }
+        for (int a = 0; a < 3; a++) force[N + id[a]] += area / 3.0 * F[a];
+    }
 }
 
 /* Sphere.cl:1-17.  OpenCL normalize() of a zero vector returns the zero vector. */

@@ -435,10 +435,13 @@ def test_legacy_law_kernel_parity(ctx, law):
 @pytest.mark.parametrize("material", ["LE", "LD", "NH2", "PNH"])
 @pytest.mark.parametrize("scatter", ["atomic", "tiled"])
 def test_legacy_law_fused_run_matches_oracle(ctx, material, scatter):
-    sim = build_sim(material=material, scatter=scatter)
-    gx, gf = run_gpu(sim, 500)
+    """200 steps with the tip starting 0.2 above the cell and the wall 0.3 below it: the linear
+    laws are not objective and, as in the reference, fail under the deep, fast indentation of the
+    other fused tests (the oracle itself diverges there after ~250 steps)."""
+    sim = build_sim(material=material, scatter=scatter, cfg_overrides={"InitialDistance": 0.2, "SubstrateDistance": 0.3})
+    gx, gf = run_gpu(sim, 200)
     ref = oracle_for(sim)
-    of = ref.run(500)
+    of = ref.run(200)
     assert_parity(sim, gx, gf, ref.positions(), of)
 
 

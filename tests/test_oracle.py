@@ -459,6 +459,158 @@ def test_legacy_poro_neo_hookean_energy_consistency():
 
 
 # --------------------------------------------------------------------------------------------
+# legacy contacts and integrators (RoundTip, AdhesivePlane, PointZeroForce, CalcCOM/COMForce,
+# OverDamped, velocityVerlet2, SphereIntegral/TipIntegral, AdhesivePlaneSurfaceIntegral)
+# --------------------------------------------------------------------------------------------
+
+
+def test_round_tip_closed_form():
+    R, ypos = 2.0, 5.0
+    pts = np.array([[0.0, 1.0, 0.0], [3.0, 5.5, 0.0], [0.5, 4.0, 0.2], [0.0, 9.0, 4.0]])
+    f = np.zeros(12)
+    O.round_tip(f, O.soa(pts), R, ypos)
+    f = f.reshape(3, -1).T
+    # below the centre: sphere; |r| = 4 -> 0.1 (4 - 2) along -y
+    assert np.allclose(f[0], [0, -0.2, 0])
+    # above the centre: cylinder (r_y = 0), |r| = 3 -> 0.1 (3 - 2) along +x
+    assert np.allclose(f[1], [0.1, 0, 0])
+    # inside the sphere: max(|r| - R, 0) = 0
+    assert np.allclose(f[2], 0)
+    assert np.allclose(f[3], [0, 0, 0.1 * (4 - 2)])
+
+
+def test_adhesive_plane_is_the_lennard_jones_wall():
+    eps, rm, wall = 0.3, 0.8, -1.0
+    sigma = 2 ** (-1 / 6) * rm
+    d = np.array([rm, 0.7, 1.1, 2.0])
+    pts = np.stack([np.zeros(4), wall + d, np.zeros(4)], 1)
+    on = np.array([1, 1, 1, 0], np.int32)
+    f = np.zeros(12)
+    O.adhesive_plane(f, O.soa(pts), wall, eps, rm, on)
+    fy = f.reshape(3, -1)[1]
+    U = lambda x: 4 * eps * ((sigma / x) ** 12 - (sigma / x) ** 6)
+    h = 1e-6
+    assert abs(fy[0]) < 1e-12  # the minimum of the potential sits at rm
+    for i in (1, 2):
+        assert fy[i] == pytest.approx(-(U(d[i] + h) - U(d[i] - h)) / (2 * h), rel=1e-6)
+    assert fy[3] == 0  # not a surface point
+
+
+def test_point_zero_and_center_of_mass_forces():
+    pts = RNG.normal(size=(7, 3))
+    P = O.soa(pts)
+    f = np.zeros(21)
+    O.point_zero_force(f, P, 0.5)
+    f = f.reshape(3, -1)
+    assert np.allclose(f[0], -0.5 * pts[0, 0]) and np.allclose(f[1], 0) and np.allclose(f[2], -0.5 * pts[0, 2])
+    com = O.center_of_mass(P)
+    assert np.allclose(com, [pts[:, 0].mean(), pts[:, 2].mean()])
+    g = np.zeros(21)
+    O.com_force(g, 2.0, com)
+    g = g.reshape(3, -1)
+    assert np.allclose(g[0], -2 * com[0]) and np.allclose(g[1], 0) and np.allclose(g[2], -2 * com[1])
+
+
+def test_over_damped_and_velocity_verlet2_recurrences():
+    N = 5
+    x = RNG.normal(size=3 * N)
+    f = RNG.normal(size=3 * N)
+    x0, f0 = x.copy(), f.copy()
+    O.over_damped(x, f, point3fix=True)
+    d = f0.copy()
+    d[3] = d[2 * N + 3] = 0  # node 3 keeps x and z
+    assert np.allclose(x, x0 + d) and np.all(f == 0)
+    # velocityVerlet2, no damping, constant acceleration a from rest: x_n = x_0 + a n (n + 1) / 2
+    m = np.full(N, 2.0)
+    a = RNG.normal(size=3 * N)
+    x, v, fo = np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N)
+    for n in range(1, 8):
+        F = a * np.repeat(m[None], 3, 0).ravel()
+        O.velocity_verlet2(x, v, F, fo, m, 0.0)
+        assert np.allclose(x, a * n * (n + 1) / 2)
+    # with damping xi the velocity tends to a / xi
+    x, v, fo = np.zeros(3 * N), np.zeros(3 * N), np.zeros(3 * N)
+    for _ in range(3000):
+        O.velocity_verlet2(x, v, a * 2.0, fo, m, 0.05)
+    assert np.allclose(v, a / 0.05, rtol=1e-6)
+
+
+def _tet_samples(P, n, rng):
+    w = rng.dirichlet(np.ones(4), size=n)
+    return w @ P
+
+
+def test_volume_integral_contacts():
+    """Bower's 4-point rule, scaled by |det r| / 4 as the kernels write it: the nodal forces sum
+    to 6 times the volume integral of the force density, their first moment to 6 times its first
+    moment; the density itself at closed-form points (sphere below the centre; the tip's cone)."""
+    rng = np.random.default_rng(3)
+    P = np.array([[0.0, 0.0, 0.0], [1.0, 0.1, 0.0], [0.2, 1.0, 0.1], [0.1, 0.2, 0.9]]) + [0.3, -3.0, 0.2]
+    R, ypos, k = 1.0, 0.5, 0.7
+
+    def dens(x):
+        r = x - [0, ypos, 0]
+        L = np.linalg.norm(r, axis=1)
+        return np.exp(-k * (L - R))[:, None] * r / L[:, None]
+
+    f = np.zeros(12)
+    O.volume_quadrature(0, f, O.soa(P), O.soa(np.array([[0, 1, 2, 3]]), np.uint32), R, 0.0, (0.0, ypos, 0.0), k)
+    f = f.reshape(3, -1).T
+    V = abs(np.linalg.det((P[:3] - P[3]).T)) / 6
+    xs = _tet_samples(P, 400000, rng)
+    dx = dens(xs)
+    assert np.allclose(f.sum(0), 6 * V * dx.mean(0), rtol=3e-3, atol=1e-4 * np.abs(f).max())
+    mom = sum(np.outer(f[a], P[a]) for a in range(4))
+    mom_ref = 6 * V * (dx[:, :, None] * xs[:, None, :]).mean(0)
+    # first moment: sum_a F_a x_a = (6 V) E[f(x) x] for the P1 interpolation weights (to quadrature accuracy)
+    assert np.allclose(mom, mom_ref, rtol=2e-2, atol=2e-2 * np.abs(mom_ref).max())
+    # a tiny tetra: the density at its centre, closed forms
+    small = 1e-3 * np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
+    V6 = abs(np.linalg.det((small[:3] - small[3]).T))
+    alpha = 0.3
+    tip = np.array([0.0, ypos, 0.0])
+    for tip_shape, c in [(0, np.array([0.0, -2.0, 0.0])), (1, np.array([0.0, -3.0, 0.0])),
+                         (1, np.array([4.0, 2.0, 0.0]))]:
+        Q = small + c
+        cc = Q.mean(0)
+        if tip_shape == 1 and cc[1] >= ypos - np.sin(alpha) * np.hypot(cc[0], cc[2]):
+            # beside the tip, above the cone line: n = cos a e_rho - sin a e_y, dis = x.n - tip.n - R
+            e = np.array([cc[0], 0.0, cc[2]]) / np.hypot(cc[0], cc[2])
+            n = np.cos(alpha) * e + np.array([0.0, -np.sin(alpha), 0.0])
+            expect = np.exp(-k * (cc @ n - tip @ n - R)) * n
+        else:  # sphere: below the centre of the sphere (or of the tip's rounded end)
+            r = cc - tip
+            expect = np.exp(-k * (np.linalg.norm(r) - R)) * r / np.linalg.norm(r)
+        g = np.zeros(12)
+        O.volume_quadrature(tip_shape, g, O.soa(Q), O.soa(np.array([[0, 1, 2, 3]]), np.uint32), R, alpha,
+                            tuple(tip), k)
+        total = g.reshape(3, -1).T.sum(0) / (1.5 * V6 / 6 * 4)  # 0.25 |det| sum_q f_q = 1.5 V6/6 * 4 * mean f
+        assert np.allclose(total, expect, rtol=2e-3, atol=1e-6), (tip_shape, total, expect)
+
+
+def test_adhesive_plane_surface_integral():
+    eps, rm, wall = 0.2, 0.8, -1.0
+    sigma = 2 ** (-1 / 6) * rm
+    lj = lambda d: 24 * eps / d * (2 * (sigma / d) ** 12 - (sigma / d) ** 6)
+    # a triangle parallel to the wall: constant density, area / 3 to each node
+    P = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]]) + [0, wall + 0.9, 0]
+    f = np.zeros(9)
+    O.adhesive_plane_surface(f, O.soa(P), O.soa(np.array([[0, 1, 2]]), np.uint32), wall, eps, rm)
+    fy = f.reshape(3, -1)[1]
+    assert np.allclose(fy, 0.5 / 3 * lj(0.9), rtol=1e-12)
+    # tilted: the sum is the area integral of the density (the 3-point rule is exact to degree 2;
+    # the density varies slowly here)
+    Q = np.array([[0.0, 2.0, 0.0], [1.0, 2.3, 0.0], [0.0, 2.1, 1.0]])
+    g = np.zeros(9)
+    O.adhesive_plane_surface(g, O.soa(Q), O.soa(np.array([[0, 1, 2]]), np.uint32), wall, eps, rm)
+    rng = np.random.default_rng(4)
+    w = rng.dirichlet(np.ones(3), size=400000)
+    ys = w @ Q[:, 1]
+    area = 0.5 * np.linalg.norm(np.cross(Q[0] - Q[2], Q[1] - Q[2]))
+    assert g.reshape(3, -1)[1].sum() == pytest.approx(area * lj(ys - wall).mean(), rel=2e-3)
+
+
+# --------------------------------------------------------------------------------------------
 # contacts, integrator, tip force
 # --------------------------------------------------------------------------------------------
 
