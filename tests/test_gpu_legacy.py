@@ -87,7 +87,8 @@ def test_node_contacts(ctx, mesh):
     fr = np.zeros(3 * N)
     O.adhesive_plane(fr, x, wall, 0.01, 1.0, on)
     fd.zero_()
-    call("nfx_adhesive_plane_force", ctx, fd.data_ptr(), xd.data_ptr(), N, wall, 0.01, 1.0, dev(on).data_ptr())
+    ond = dev(on)  # keep every device buffer referenced until the kernels have run
+    call("nfx_adhesive_plane_force", ctx, fd.data_ptr(), xd.data_ptr(), N, wall, 0.01, 1.0, ond.data_ptr())
     rel_close(host(fd), fr, 1e-4)  # (sigma/d)^12 in FP32
     # point zero and centre of mass
     fr = np.zeros(3 * N)
@@ -166,7 +167,8 @@ def test_volume_integrals(ctx, mesh, tip_shape):
     gd = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
     call("nfx_volume_integral_force", ctx, tip_shape, xd.data_ptr(), td.data_ptr(), N, nT, R, alpha, *tip, k, None,
          corners.data_ptr())
-    call("nfx_gather_corners", ctx, dev(off).data_ptr(), dev(slots).data_ptr(), corners.data_ptr(), N, gd.data_ptr())
+    offd, slotsd = dev(off), dev(slots)
+    call("nfx_gather_corners", ctx, offd.data_ptr(), slotsd.data_ptr(), corners.data_ptr(), N, gd.data_ptr())
     rel_close(host(gd), fr)
 
 
@@ -182,7 +184,8 @@ def test_surface_adhesion(ctx, mesh):
     fr = np.zeros(3 * N)
     O.adhesive_plane_surface(fr, x, O.soa(tri, np.uint32), wall, 0.02, 1.0)
     fd = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
-    call("nfx_adhesive_plane_surface_force", ctx, fd.data_ptr(), dev(x, np.float32).data_ptr(),
-         dev(O.soa(tri, np.uint32)).data_ptr(), N, len(tri), wall, 0.02, 1.0)
+    xd, trid = dev(x, np.float32), dev(O.soa(tri, np.uint32))
+    call("nfx_adhesive_plane_surface_force", ctx, fd.data_ptr(), xd.data_ptr(), trid.data_ptr(), N, len(tri), wall,
+         0.02, 1.0)
     assert np.abs(fr).max() > 0
     rel_close(host(fd), fr, 1e-4)

@@ -158,7 +158,8 @@ def test_node_kernels_parity(ctx):
         fr = f0.copy()
         fn(args[0], fr, xs, *args[1:])
         fd = dev(f0, np.float32)
-        call(name, ctx, fd.data_ptr(), dev(xs, np.float32).data_ptr(), N, *args)
+        xsd = dev(xs, np.float32)  # referenced until the kernel has run
+        call(name, ctx, fd.data_ptr(), xsd.data_ptr(), N, *args)
         assert np.max(np.abs(host(fd) - fr)) < 1e-6, name
     # velocity Verlet, with fixTopBottom
     v = RNG.normal(scale=1e-2, size=3 * N).astype(np.float32).astype(np.float64)
@@ -174,7 +175,8 @@ def test_node_kernels_parity(ctx):
     # FP64 tip-force reduction vs oracle sum on the same FP32 positions
     out = torch.zeros(2, dtype=torch.float64, device="cuda:0")
     for ind, R, pos in ((1, 3.0, 2.0), (2, 0.0, 1.0)):
-        call("nfx_tip_force", ctx, dev(xs, np.float32).data_ptr(), N, ind, 0.1, R, pos, out.data_ptr())
+        xsd = dev(xs, np.float32)
+        call("nfx_tip_force", ctx, xsd.data_ptr(), N, ind, 0.1, R, pos, out.data_ptr())
         f, m = host(out)
         fr, mr = O.tip_force(xs, ind, 0.1, R, pos)
         assert f == pytest.approx(fr, rel=1e-5) and m == pytest.approx(mr, rel=1e-5)
