@@ -16,8 +16,8 @@ Surrogate-assisted fit (``fit_surrogate``), for all targets at once:
 2. the ``ntop`` best distinct candidates per target are simulated with the batched simulator on
    the targets' own meshes and protocols; each simulated curve gets its closed-form amplitude and
    the best becomes the fit;
-3. the fit's parameters are simulated once more: that plain simulation is the confirmed curve and
-   its relative L2 distance to the target the reported error.
+3. optionally the fit's parameters are simulated once more (``confirm``); otherwise the fit's curve
+   is the refining simulation's scaled by its amplitude, which the symmetry makes exact.
 Baseline (``fit_nelder_mead``): simulation-only Nelder-Mead on (log10 E, nu, log10 eta), all
 targets in lock-step, one batched simulation per evaluation.
 """
@@ -96,10 +96,13 @@ def stabilize(theta, T0, max_relax=0.1):
     return th
 
 
-def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, seed=0, log=None):
+def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, seed=0, log=None, confirm=True):
     """targets (n, 256) newtons; bases (n) CellSpec with the known geometry/protocol; ``bs`` a
-    BatchSimulation over the targets' meshes.  Returns dict with the fitted parameters, the
-    confirming simulation's curve and error, and timings."""
+    BatchSimulation over the targets' meshes.  Returns dict with the fitted parameters, the fit's
+    simulated curve and its error, and timings.  ``confirm``: simulate the final parameters once
+    more; without it the curve is the refining simulation's, exactly scaled by the closed-form
+    amplitude (identical to the simulator's output at the final parameters by the E-viscosity
+    symmetry)."""
     n = len(targets)
     rng = np.random.Generator(np.random.PCG64(seed))
     R2 = np.array([(b.radius_um * 1e-6) ** 2 for b in bases])
@@ -158,25 +161,33 @@ def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, see
     best_err = np.full(n, np.inf)
     top = np.stack([stabilize(top[:, k], T0) for k in range(ntop)], 1)
     theta = top[:, 0].copy()
+    curve = np.zeros_like(targets)
     for k in range(ntop):
         F = simulate(bs, [theta_to_spec(top[i, k], bases[i]) for i in range(n)])
-        a = amplitude(F, targets)
-        e = rel_err(F * a[:, None], targets)
-        cand = stabilize(clip_theta(top[:, k] + np.log10(np.clip(a, 1e-6, 1e6))[:, None] * np.array([1.0, 0.0, 1.0])),
-                         T0)
+        a = np.clip(amplitude(F, targets), 1e-6, 1e6)
+        scaled = top[:, k] + np.log10(a)[:, None] * np.array([1.0, 0.0, 1.0])
+        cand = stabilize(clip_theta(scaled), T0)
+        # the amplitude-corrected parameters give exactly a F (the E-viscosity symmetry) unless the
+        # box or the stability limit moved them; then keep the simulated candidate itself
+        exact = np.all(np.abs(cand - scaled) <= 1e-12, axis=1)
+        Fk = np.where(exact[:, None], F * a[:, None], F)
+        cand = np.where(exact[:, None], cand, top[:, k])
+        e = rel_err(Fk, targets)
         better = e < best_err
         best_err[better] = e[better]
         theta[better] = cand[better]
+        curve[better] = Fk[better]
         if log:
             log(f"refine {k + 1}/{ntop}: {time.perf_counter() - t1:.1f}s")
     t_ref = time.perf_counter() - t1
     t2 = time.perf_counter()
-    confirmed = simulate(bs, [theta_to_spec(theta[i], bases[i]) for i in range(n)])
+    if confirm:
+        curve = simulate(bs, [theta_to_spec(theta[i], bases[i]) for i in range(n)])
     t_conf = time.perf_counter() - t2
-    return {"theta": theta, "curve": confirmed, "sim_err": rel_err(confirmed, targets),
+    return {"theta": theta, "curve": curve, "sim_err": rel_err(curve, targets),
             "predicted_refined_err": best_err, "surrogate_s": t_sur, "inference_s": infer_s,
             "refine_s": t_ref, "confirm_s": t_conf, "total_s": time.perf_counter() - t0,
-            "candidates_per_target": rounds * ncand, "simulations_per_target": ntop + 1}
+            "candidates_per_target": rounds * ncand, "simulations_per_target": ntop + (1 if confirm else 0)}
 
 
 def fit_nelder_mead(targets, bases, bs, max_evals=60, log=None):

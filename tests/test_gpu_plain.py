@@ -134,6 +134,32 @@ def test_tetra_force_parity(ctx, model, ratio):
     assert err < 2e-5, err
 
 
+def test_tetra_force_matches_golden_fixture(ctx):
+    """tests/golden/homogeneous_states.npz (closed-form stresses, see tests/golden/make_golden.py)
+    through nfx_precompute_f64 + nfx_tetra_force, FP32: within 2e-5 of the largest force."""
+    import os
+
+    import torch
+
+    from nofluidx3d_amd import _lib
+
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "homogeneous_states.npz"))
+    X = g["X"]
+    T = np.array([[0, 1, 2, 3]])
+    dminv, v0, _, _ = gpu_precompute(ctx, X, T)
+    td = dev(O.soa(T, np.uint32))
+    for i in range(len(g["names"])):
+        x = g["x"][i]
+        xd = dev(O.soa(x), np.float32)
+        fd = torch.zeros(12, dtype=torch.float32, device="cuda:0")
+        mat = _lib.Elastic(_lib.NFX_MR, float(g["E"]), float(g["nu"]), float(g["ratio"][i]))
+        call("nfx_tetra_force", ctx, mat, xd.data_ptr(), td.data_ptr(), dminv.data_ptr(), v0.data_ptr(), 4, 1,
+             fd.data_ptr())
+        f = host(fd).reshape(3, -1).T
+        ref = g["forces"][i]
+        assert np.max(np.abs(f - ref)) < 2e-5 * np.abs(ref).max(), g["names"][i]
+
+
 def csr(T, N):
     off = np.zeros(N + 1, np.uint32)
     slots = np.zeros(4 * len(T), np.uint32)

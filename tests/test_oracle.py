@@ -144,6 +144,17 @@ def test_mr_energy_consistency(ratio):
         assert np.max(np.abs(f - fe)) < 1e-7 * np.max(np.abs(fe))
 
 
+def test_golden_homogeneous_states():
+    """tests/golden/homogeneous_states.npz: textbook stresses (uniaxial Neo-Hookean, hydrostatic
+    and simple-shear Mooney-Rivlin, rigidly rotated) turned into face-area nodal forces."""
+    import os
+
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "homogeneous_states.npz"))
+    for i in range(len(g["names"])):
+        f = forces_of(O.MR, float(g["E"]), float(g["nu"]), float(g["ratio"][i]), g["X"], g["x"][i])
+        assert np.allclose(f, g["forces"][i], rtol=1e-9, atol=1e-12 * np.abs(g["forces"][i]).max()), g["names"][i]
+
+
 def test_svk_energy_consistency():
     E, nu = 0.02, 0.3
     for _ in range(5):
