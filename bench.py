@@ -293,6 +293,9 @@ def main():
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_large_cell.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (HBM-counter calibration under rocprofv3)")
+    ap.add_argument("--only", default=None, choices=[None, "batched", "surrogate"],
+                    help="run just one secondary measurement and print its block (profiling runs)")
+    ap.add_argument("--batched-steps", type=int, default=10000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only "
@@ -304,6 +307,13 @@ def main():
     from nofluidx3d_amd import configs
 
     assert torch.cuda.is_available(), "no HIP device"
+    if args.only == "batched":
+        print(json.dumps(bench_batched(steps=args.batched_steps)), flush=True)
+        return
+    if args.only == "surrogate":
+        print(json.dumps(bench_surrogate(os.path.join(ROOT, "nofluidx3d_amd", "surrogate", "weights",
+                                                      "surrogate.npz"))), flush=True)
+        return
     if args.workload == "large":
         cfg = configs.large_cell()
         material = args.material or "viscoplastic"
@@ -403,7 +413,7 @@ def main():
         out["stream_copy"] = sc
         out["roofline"]["frac_of_stream_copy"] = achieved / sc["gbs"]
     if not args.no_batched and not args.no_extra:
-        out["batched"] = bench_batched()
+        out["batched"] = bench_batched(steps=args.batched_steps)
         if "plain_small" in out:
             out["batched"]["x_plain_small"] = out["batched"]["value"] / out["plain_small"]["value"]
     weights = os.path.join(ROOT, "nofluidx3d_amd", "surrogate", "weights", "surrogate.npz")

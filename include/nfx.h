@@ -175,6 +175,20 @@ int nfx_tetra_corner_forces(nfx_ctx* ctx, const nfx_elastic* mat, const float* p
 int nfx_legacy_force(nfx_ctx* ctx, const nfx_elastic* mat, const float* points, const uint32_t* tetras,
                      const float* dminv, int64_t N, int64_t T, float* von_mises, float* pressure, float* force,
                      float* corners);
+/* InteractionLinearViscoelastic.cl (wrapper Interactions.py:518-600), node-0 edge base: small-strain
+ * generalized Maxwell law sigma = 2 mu w_inf dev(eps) + kappa tr(eps) I + sum_j h_j,
+ * h_j <- exp(-1/tau_j) h_j + 2 mu w_j (dev(eps) - dev(eps)_old), w_inf = 1 - sum_j w_j; tau_j in steps.
+ * State: dev_old 9T SoA, transient 9*nterms*T (term j, entry c of tetra t at (9j + c)T + t), zero at
+ * the start.  Atomic scatter when corners is NULL; von_mises/pressure nullable. */
+typedef struct {
+    double E, nu;
+    int32_t nterms; /* 0..8 */
+    double weights[8];
+    double taus[8];
+} nfx_viscoelastic;
+int nfx_linear_viscoelastic_force(nfx_ctx* ctx, const nfx_viscoelastic* p, const float* points, const uint32_t* tetras,
+                                  const float* dminv, int64_t N, int64_t T, float* dev_old, float* transient,
+                                  float* von_mises, float* pressure, float* force, float* corners);
 /* InteractionFiniteStrainViscoplastic.cl:1-220 (node-0 edge base); atomic scatter when corners
  * is NULL, per-corner output otherwise */
 int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float* points, const uint32_t* tetras,

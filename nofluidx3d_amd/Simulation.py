@@ -31,8 +31,8 @@ import numpy as np
 
 from nofluidx3d_amd import _lib, openCL
 from nofluidx3d_amd.interactions import (FiniteStrainViscoplastic, LegacyElastic, LinearElastic,
-                                        LinearElasticDeviatoric, Plane, PoroNeoHookean, SecondOrderNeoHookean,
-                                        Sphere, Substrate, Tetra, VelocityVerlet)
+                                        LinearElasticDeviatoric, LinearViscoelastic, Plane, PoroNeoHookean,
+                                        SecondOrderNeoHookean, Sphere, Substrate, Tetra, VelocityVerlet)
 from nofluidx3d_amd.interactions.LegacyContacts import (AdhesivePlane, AdhesivePlaneSurfaceIntegral, CalcCOM,
                                                         COMForce, OverDamped, PointZeroForce, RoundTip,
                                                         SphereIntegral, TipIntegral, VelocityVerlet2)
@@ -247,6 +247,20 @@ class Simulation:
         shear = E / (2 * (1 + nu))
         self.register(SecondOrderNeoHookean(self.cell, shear, shear * c["SecondOrderNeoHookeanFactor"],
                                             E / (3 * (1 - 2 * nu))))
+
+    def setInteractionLinearViscoelastic(self):
+        """Prony series from CELL["ViscoelasticWeights"] and the relaxation times in seconds
+        (CELL["ViscoelasticRelaxationTimesSI"], converted to steps with T0) or in steps
+        (CELL["ViscoelasticRelaxationTimes"])."""
+        c = self.parameters["CELL"]
+        if "ViscoelasticRelaxationTimesSI" in c:
+            taus = [t / self.T0 for t in c["ViscoelasticRelaxationTimesSI"]]
+        else:
+            taus = list(c["ViscoelasticRelaxationTimes"])
+        inter = LinearViscoelastic(self.cell, self._legacy_youngs(), c["PoissonRatio"], list(c["ViscoelasticWeights"]),
+                                   taus)
+        self.register(inter)
+        return inter
 
     def setInteractionPoroNeoHookean(self):
         c = self.parameters["CELL"]
@@ -546,7 +560,8 @@ class Simulation:
         self._print("\n\n")
         self._print(f"VTKInterval: {self.VTKInterval}")
         for inter in self.interactions:
-            if isinstance(inter, (FiniteStrainViscoplastic, LegacyElastic)) and (writeStress or writePressure):
+            if isinstance(inter, (FiniteStrainViscoplastic, LegacyElastic, LinearViscoelastic)) and \
+                    (writeStress or writePressure):
                 inter.write_stress = True
                 inter.setArgs()
                 self._plan = None

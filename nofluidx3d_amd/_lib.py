@@ -29,6 +29,10 @@ class Elastic(ctypes.Structure):
                 ("shear2", c_f64), ("volume_fraction", c_f64)]
 
 
+class Viscoelastic(ctypes.Structure):
+    _fields_ = [("E", c_f64), ("nu", c_f64), ("nterms", c_i32), ("weights", c_f64 * 8), ("taus", c_f64 * 8)]
+
+
 class Viscoplastic(ctypes.Structure):
     _fields_ = [
         ("mu1", c_f64),
@@ -154,6 +158,8 @@ SIGNATURES = {
     "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_vp_shape_from_fp": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_round_tip_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
+    "nfx_linear_viscoelastic_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Viscoelastic), c_vp, c_vp, c_vp, c_i64,
+                                                      c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "nfx_adhesive_plane_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64, c_f64, c_vp]),
     "nfx_point_zero_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64]),
     "nfx_center_of_mass": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),

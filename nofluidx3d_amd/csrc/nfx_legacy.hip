@@ -8,6 +8,7 @@
 //   k_vv2              velocityVerlet2.cl                mass-weighted damped Verlet
 //   k_volume_quad      InteractionSphereIntegral.cl, InteractionTipIntegral.cl
 //                      exp(-k dis) n body force, Bower's 4-point rule per tetra
+//   k_lve              InteractionLinearViscoelastic.cl   small-strain Prony-series law (node-0 base)
 //   k_surface_adhesion InteractionAdhesivePlaneSurfaceIntegral.cl
 //                      Lennard-Jones area density, 3-point edge-midpoint rule per surface triangle
 // Scatters into the force array use float atomics, as the reference's atomicAdd; the volume
@@ -238,6 +239,81 @@ __global__ __launch_bounds__(kB) void k_surface_adhesion(float* f, const float* 
     for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&f[N + id[a]], area * (1.0f / 3.0f) * F[a]);
 }
 
+// InteractionLinearViscoelastic.cl: small-strain generalized Maxwell (Prony) law, node-0 base
+struct LveConst {
+    float two_mu_inf, kappa, two_mu;
+    int nterms;
+    float w[8], decay[8];
+};
+
+template <bool CORNERS>
+__global__ __launch_bounds__(kB) void k_lve(float* f, float* corners, const float* __restrict__ x,
+                                            const uint32_t* __restrict__ tets, const float* __restrict__ dminv,
+                                            int64_t N, int64_t T, LveConst c, float* dev_old, float* transient,
+                                            float* vm, float* pr) {
+    const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
+    if (t >= T) return;
+    const uint32_t n0 = tets[t], n1 = tets[T + t], n2 = tets[2 * T + t], n3 = tets[3 * T + t];
+    float D[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) D[k] = dminv[k * T + t];
+    const V3 p0 = load3(x, N, n0), r1 = load3(x, N, n1) - p0, r2 = load3(x, N, n2) - p0, r3 = load3(x, N, n3) - p0;
+    const M9 F = deformation_gradient(r1, r2, r3, D);
+    float e[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) e[i][k] = 0.5f * (F.m[i][k] + F.m[k][i]) - (i == k ? 1.0f : 0.0f);
+    const float tr = e[0][0] + e[1][1] + e[2][2];
+    float sig[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int q = 3 * i + k;
+            const float dv = e[i][k] - (i == k ? tr * (1.0f / 3.0f) : 0.0f);
+            const float rate = dv - dev_old[q * T + t];
+            dev_old[q * T + t] = dv;
+            float s = c.two_mu_inf * dv + (i == k ? c.kappa * tr : 0.0f);
+            for (int j = 0; j < c.nterms; ++j) {
+                float* h = &transient[(9 * (int64_t)j + q) * T + t];
+                const float hn = c.decay[j] * *h + c.two_mu * c.w[j] * rate;
+                *h = hn;
+                s += hn;
+            }
+            sig[i][k] = s;
+        }
+    // the legacy kernel keeps sigma as computed (symmetric up to rounding); its face forces use the rows
+    const S6 S{sig[0][0], sig[1][1], sig[2][2], 0.5f * (sig[0][1] + sig[1][0]), 0.5f * (sig[1][2] + sig[2][1]),
+               0.5f * (sig[0][2] + sig[2][0])};
+    if (vm) {
+        const float t3 = (S.xx + S.yy + S.zz) * (1.0f / 3.0f);
+        const float dx = S.xx - t3, dy = S.yy - t3, dz = S.zz - t3;
+        vm[t] = sqrtf(1.5f * (dx * dx + dy * dy + dz * dz + 2.0f * (S.xy * S.xy + S.yz * S.yz + S.xz * S.xz)));
+        pr[t] = t3;
+    }
+    const float detr = dot(r1, cross(r2, r3));
+    const Corners fc = cofactor_forces(S, r1, r2, r3, detr < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
+    const uint32_t id[4] = {n0, n1, n2, n3};
+    const V3 fa[4] = {fc.f0, fc.f1, fc.f2, fc.f3};
+    if constexpr (CORNERS) {
+        float* cc = corners + 12 * t;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            cc[3 * a] = fa[a].x;
+            cc[3 * a + 1] = fa[a].y;
+            cc[3 * a + 2] = fa[a].z;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            unsafeAtomicAdd(&f[id[a]], fa[a].x);
+            unsafeAtomicAdd(&f[N + id[a]], fa[a].y);
+            unsafeAtomicAdd(&f[2 * N + id[a]], fa[a].z);
+        }
+    }
+}
+
 static int check_n(int64_t n, const char* what) {
     NFX_REQUIRE(n > 0 && n < (int64_t(1) << 31), "%s count %lld out of range", what, (long long)n);
     return NFX_OK;
@@ -325,6 +401,38 @@ int nfx_volume_integral_force(nfx_ctx* c, int32_t tip_shape, const float* points
     else if (tip_shape) k_volume_quad<true, false><<<nb, kB, 0, c->stream>>>(force, corners, points, tetras, N, T, q);
     else if (corners) k_volume_quad<false, true><<<nb, kB, 0, c->stream>>>(force, corners, points, tetras, N, T, q);
     else k_volume_quad<false, false><<<nb, kB, 0, c->stream>>>(force, corners, points, tetras, N, T, q);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_linear_viscoelastic_force(nfx_ctx* c, const nfx_viscoelastic* p, const float* points, const uint32_t* tetras,
+                                  const float* dminv, int64_t N, int64_t T, float* dev_old, float* transient,
+                                  float* von_mises, float* pressure, float* force, float* corners) {
+    NFX_REQUIRE(c && p && points && tetras && dminv && dev_old && (force || corners), "NULL argument");
+    NFX_REQUIRE(p->nterms >= 0 && p->nterms <= 8, "viscoelastic terms %d outside 0..8", p->nterms);
+    NFX_REQUIRE(p->nterms == 0 || transient, "NULL transient stresses");
+    NFX_REQUIRE(!von_mises == !pressure, "von_mises and pressure go together");
+    if (int rc = check_n(N, "node")) return rc;
+    if (int rc = check_n(T, "tetra")) return rc;
+    LveConst k{};
+    const double mu = p->E / (2 * (1 + p->nu));
+    double winf = 1.0;
+    for (int j = 0; j < p->nterms; ++j) {
+        NFX_REQUIRE(p->taus[j] > 0, "relaxation time %d must be positive", j);
+        winf -= p->weights[j];
+        k.w[j] = (float)p->weights[j];
+        k.decay[j] = (float)std::exp(-1.0 / p->taus[j]);
+    }
+    k.two_mu_inf = (float)(2 * mu * winf);
+    k.two_mu = (float)(2 * mu);
+    k.kappa = (float)(p->E / (3 * (1 - 2 * p->nu)));
+    k.nterms = p->nterms;
+    if (corners)
+        k_lve<true><<<nb_of(T), kB, 0, c->stream>>>(force, corners, points, tetras, dminv, N, T, k, dev_old, transient,
+                                                   von_mises, pressure);
+    else
+        k_lve<false><<<nb_of(T), kB, 0, c->stream>>>(force, corners, points, tetras, dminv, N, T, k, dev_old, transient,
+                                                    von_mises, pressure);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }

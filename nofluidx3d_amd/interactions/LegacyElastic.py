@@ -94,3 +94,52 @@ class PoroNeoHookean(LegacyElastic):
 
     def __init__(self, referenceCell, shearModulus, volumeFraction):
         super().__init__(referenceCell, shear=shearModulus, volume_fraction=volumeFraction)
+
+
+class LinearViscoelastic(Interaction):
+    """Small-strain generalized Maxwell law (reference ``Interactions.py:518-600``, kernel
+    ``InteractionLinearViscoelastic.cl``): sigma = 2 mu w_inf dev(eps) + kappa tr(eps) I + sum_j h_j,
+    h_j relaxing with exp(-1/tau_j) per step (tau_j in steps), w_inf = 1 - sum_j w_j.  State: the
+    previous deviatoric strain and the transient stresses, zero at the start (the reference's
+    wrapper allocates them READ_ONLY and names an undefined buffer, Interactions.py:566-596; here
+    they are ordinary state)."""
+
+    def __init__(self, referenceCell, youngsModulus, poissonRatio, weights, relaxationTimes):
+        if len(weights) != len(relaxationTimes):
+            raise ValueError("weights and relaxationTimes must be of equal number")
+        if len(weights) > 8:
+            raise ValueError("at most 8 viscoelastic terms")
+        self.referenceCell = referenceCell
+        p = _lib.Viscoelastic()
+        p.E, p.nu, p.nterms = float(youngsModulus), float(poissonRatio), len(weights)
+        for j, (w, t) in enumerate(zip(weights, relaxationTimes)):
+            p.weights[j], p.taus[j] = float(w), float(t)
+        self.params = p
+        self.write_stress = False
+
+    def createBuffers(self):
+        import torch
+
+        self.dminvB, self.volumesB = precompute_reference(self.referenceCell, self.simulation)
+        self.referenceCell = None
+        T, dev = self.simulation.numTetra, self.dminvB.device
+        self.oldDeviatoricStrainB = torch.zeros(9 * T, dtype=torch.float32, device=dev)
+        self.transientStressB = torch.zeros(max(9 * self.params.nterms * T, 1), dtype=torch.float32, device=dev)
+
+    def build(self):
+        self.scope = self.simulation.numTetra
+
+    def enqueue(self):
+        sim = self.simulation
+        h = openCL.handle()
+        vm = sim.vonMises.ptr if self.write_stress else None
+        pr = sim.pressure.ptr if self.write_stress else None
+        args = (h, self.params, sim.points.ptr, sim.tetras.ptr, self.dminvB.data_ptr(), sim.numPoints, sim.numTetra,
+                self.oldDeviatoricStrainB.data_ptr(), self.transientStressB.data_ptr(), vm, pr)
+        if getattr(sim, "scatter", "atomic") in ("csr", "tiled"):
+            off, slots, corners = sim.node_csr()
+            _lib.call("nfx_linear_viscoelastic_force", *args, None, corners.data_ptr())
+            _lib.call("nfx_gather_corners", h, off.data_ptr(), slots.data_ptr(), corners.data_ptr(), sim.numPoints,
+                      sim.force.ptr)
+        else:
+            _lib.call("nfx_linear_viscoelastic_force", *args, sim.force.ptr, None)

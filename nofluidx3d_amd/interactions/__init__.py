@@ -3,7 +3,7 @@ finite-strain viscoplastic material and the legacy stress laws."""
 
 from nofluidx3d_amd.interactions.FiniteStrainViscoplastic import FiniteStrainViscoplastic
 from nofluidx3d_amd.interactions.LegacyElastic import (LegacyElastic, LinearElastic, LinearElasticDeviatoric,
-                                                       PoroNeoHookean, SecondOrderNeoHookean)
+                                                       LinearViscoelastic, PoroNeoHookean, SecondOrderNeoHookean)
 from nofluidx3d_amd.interactions.Plane import Plane
 from nofluidx3d_amd.interactions.Sphere import Sphere
 from nofluidx3d_amd.interactions.Substrate import Substrate
@@ -15,6 +15,7 @@ __all__ = [
     "LegacyElastic",
     "LinearElastic",
     "LinearElasticDeviatoric",
+    "LinearViscoelastic",
     "PoroNeoHookean",
     "SecondOrderNeoHookean",
     "Plane",

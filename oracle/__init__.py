@@ -65,6 +65,21 @@ class LegacyParams(ctypes.Structure):
 LINEAR_ELASTIC, LINEAR_DEVIATORIC, NEO_HOOKEAN_2, PORO_NEO_HOOKEAN = 4, 5, 6, 7  # run models (law + 4)
 
 
+class LVEParams(ctypes.Structure):
+    """linear viscoelastic (Prony) law: E, nu, up to 8 terms of weights and relaxation times [steps]"""
+
+    _fields_ = [("E", ctypes.c_double), ("nu", ctypes.c_double), ("nterms", ctypes.c_int32),
+                ("weights", ctypes.c_double * 8), ("taus", ctypes.c_double * 8)]
+
+
+def lve_params(E, nu, weights, taus):
+    p = LVEParams()
+    p.E, p.nu, p.nterms = E, nu, len(weights)
+    for j, (w, t) in enumerate(zip(weights, taus)):
+        p.weights[j], p.taus[j] = w, t
+    return p
+
+
 class RunParams(ctypes.Structure):
     _fields_ = [
         ("model", ctypes.c_int),
@@ -106,6 +121,8 @@ def lib():
         L.orc_legacy_force.argtypes = [ctypes.c_int, ctypes.POINTER(LegacyParams), i64, i64, _dp, _up, _dp, _dp,
                                        _dp, _dp, ctypes.c_int]
         L.orc_sphere.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double, ctypes.c_double]
+        L.orc_linear_viscoelastic.argtypes = [ctypes.POINTER(LVEParams), i64, i64, _dp, _up, _dp, _dp, _dp, _dp, _dp,
+                                              _dp]
         f64, _ip = ctypes.c_double, ctypes.POINTER(ctypes.c_int32)
         L.orc_round_tip.argtypes = [i64, _dp, _dp, f64, f64]
         L.orc_adhesive_plane.argtypes = [i64, _dp, _dp, f64, f64, f64, _ip]
@@ -259,6 +276,11 @@ def volume_quadrature(tip_shape, force, points, tetras, radius, half_angle, tip_
 def adhesive_plane_surface(force, points, triangles, wall, adhesion, distance):
     lib().orc_adhesive_plane_surface(points.size // 3, triangles.size // 3, _d(force), _d(points), _u(triangles),
                                      wall, adhesion, distance)
+
+
+def linear_viscoelastic(P: LVEParams, points, tetras, ev, force, devold, transient, vonMises=None, pressure=None):
+    lib().orc_linear_viscoelastic(ctypes.byref(P), points.size // 3, tetras.size // 4, _d(points), _u(tetras), _d(ev),
+                                  _d(force), _d(devold), _d(transient), _dn(vonMises), _dn(pressure))
 
 
 def sphere(k, force, points, radius, pos):

@@ -11,6 +11,7 @@
  *   orc_viscoplastic     kernels/InteractionFiniteStrainViscoplastic.cl:1-220, restated with
  *                        node 0 as the edge base (the legacy kernel used node 3; TetraCell.py:33-38
  *                        uses node 0), plus an optional yield offset (0 = reference law)
+ *   orc_linear_viscoelastic  kernels/InteractionLinearViscoelastic.cl (node-0 base)
  *   orc_legacy_force     kernels/InteractionLinearElastic.cl, InteractionLinearElasticDeviatoric.cl,
  *                        InteractionSecondOrderNeoHookean.cl, InteractionPoroNeoHookean.cl (node-0 base)
  *   orc_round_tip, orc_adhesive_plane, orc_point_zero_force, orc_center_of_mass, orc_com_force,
@@ -506,6 +507,102 @@ static void legacy_one(int64_t t, int law, const orc_legacy_params* P, int64_t N
     scatter(force, id2, f2.x, atomic); scatter(force, N + id2, f2.y, atomic); scatter(force, 2 * N + id2, f2.z, atomic);
     scatter(force, id3, f3.x, atomic); scatter(force, N + id3, f3.y, atomic); scatter(force, 2 * N + id3, f3.z, atomic);
     scatter(force, id4, f4.x, atomic); scatter(force, N + id4, f4.y, atomic); scatter(force, 2 * N + id4, f4.z, atomic);
+}
+
+/* InteractionLinearViscoelastic.cl:1-224 (wrapper Interactions.py:518-600): small-strain
+ * generalized Maxwell (Prony) law, node-0 base.  sigma = 2 mu w_inf dev(eps) + kappa tr(eps) I
+ * + sum_j h_j with h_j <- exp(-1/tau_j) h_j + 2 mu w_j (dev(eps) - dev(eps)_old), w_inf = 1 - sum w_j;
+ * state per tetra: dev(eps)_old (9 SoA) and h_j (9 SoA per term, term j at 9 j T). */
+typedef struct {
+    double E, nu;
+    int32_t nterms;
+    double weights[8], taus[8];
+} orc_lve_params;
+
+static void lve_one(int64_t t, const orc_lve_params* P, int64_t N, int64_t T, const double* points,
+                    const uint32_t* tetras, const double* ev, double* force, double* devold, double* transient,
+                    double* vonMises, double* pressure, int atomic) {
+    const uint32_t id1 = tetras[T + t], id2 = tetras[2 * T + t], id3 = tetras[3 * T + t], id4 = tetras[t];
+    const d3 p1 = v3(points[id1], points[N + id1], points[2 * N + id1]);
+    const d3 p2 = v3(points[id2], points[N + id2], points[2 * N + id2]);
+    const d3 p3 = v3(points[id3], points[N + id3], points[2 * N + id3]);
+    const d3 p4 = v3(points[id4], points[N + id4], points[2 * N + id4]);
+    const d3 r1 = vsub(p1, p4), r2 = vsub(p2, p4), r3 = vsub(p3, p4);
+    const d3 R1 = v3(ev[t], ev[T + t], ev[2 * T + t]);
+    const d3 R2 = v3(ev[3 * T + t], ev[4 * T + t], ev[5 * T + t]);
+    const d3 R3 = v3(ev[6 * T + t], ev[7 * T + t], ev[8 * T + t]);
+    const d3 u1 = vsub(r1, R1), u2 = vsub(r2, R2), u3 = vsub(r3, R3);
+    const double divisor = vdot(vcross(R1, R2), R3);
+    const d3 invJ1 = vdivs(vcross(R2, R3), divisor);
+    const d3 invJ2 = vdivs(vcross(R3, R1), divisor);
+    const d3 invJ3 = vdivs(vcross(R1, R2), divisor);
+    const d3 du1 = vadd(vadd(vscale(invJ1, u1.x), vscale(invJ2, u2.x)), vscale(invJ3, u3.x));
+    const d3 du2 = vadd(vadd(vscale(invJ1, u1.y), vscale(invJ2, u2.y)), vscale(invJ3, u3.y));
+    const d3 du3 = vadd(vadd(vscale(invJ1, u1.z), vscale(invJ2, u2.z)), vscale(invJ3, u3.z));
+    const d3 E1 = v3(1, 0, 0), E2 = v3(0, 1, 0), E3 = v3(0, 0, 1);
+    const d3 e1 = vscale(vadd(du1, v3(du1.x, du2.x, du3.x)), 0.5);
+    const d3 e2 = vscale(vadd(du2, v3(du1.y, du2.y, du3.y)), 0.5);
+    const d3 e3 = vscale(vadd(du3, v3(du1.z, du2.z, du3.z)), 0.5);
+    const double trE = e1.x + e2.y + e3.z;
+    const d3 de[3] = {vsub(e1, vscale(E1, trE / 3.)), vsub(e2, vscale(E2, trE / 3.)), vsub(e3, vscale(E3, trE / 3.))};
+    const double mu = P->E / (2 * (1 + P->nu)), kappa = P->E / (3 * (1 - 2 * P->nu));
+    double wel = 1.0;
+    for (int j = 0; j < P->nterms; j++) wel -= P->weights[j];
+    double sig[3][3], dot_[3][3];
+    for (int i = 0; i < 3; i++) {
+        const double dv[3] = {de[i].x, de[i].y, de[i].z};
+        for (int k = 0; k < 3; k++) {
+            const int c = 3 * i + k;
+            dot_[i][k] = dv[k] - devold[c * T + t];
+            devold[c * T + t] = dv[k];
+            sig[i][k] = 2 * mu * wel * dv[k] + (i == k ? kappa * trE : 0.0);
+        }
+    }
+    for (int j = 0; j < P->nterms; j++) {
+        const double decay = exp(-1. / P->taus[j]);
+        for (int c = 0; c < 9; c++) {
+            double* h = &transient[(9 * (int64_t)j + c) * T + t];
+            *h = decay * *h + 2 * mu * P->weights[j] * dot_[c / 3][c % 3];
+            sig[c / 3][c % 3] += *h;
+        }
+    }
+    const d3 s1 = v3(sig[0][0], sig[0][1], sig[0][2]), s2 = v3(sig[1][0], sig[1][1], sig[1][2]),
+             s3 = v3(sig[2][0], sig[2][1], sig[2][2]);
+    const double trS = s1.x + s2.y + s3.z;
+    const d3 ds1 = vsub(s1, vscale(E1, trS / 3.0)), ds2 = vsub(s2, vscale(E2, trS / 3.0)),
+             ds3 = vsub(s3, vscale(E3, trS / 3.0));
+    if (vonMises) vonMises[t] = sqrt(3. / 2. * (vdot(ds1, ds1) + vdot(ds2, ds2) + vdot(ds3, ds3)));
+    if (pressure) pressure[t] = trS / 3.0;
+    d3 n1 = vcross(r2, r3), n2 = vcross(r3, r1), n3 = vcross(r1, r2), n4 = vcross(vsub(r3, r1), vsub(r3, r2));
+    n1 = vdivs(n1, vlen(n1));
+    n2 = vdivs(n2, vlen(n2));
+    n3 = vdivs(n3, vlen(n3));
+    n4 = vdivs(n4, vlen(n4));
+    if (vdot(n1, vscale(r1, -1.0)) < 0) n1 = vscale(n1, -1.0);
+    if (vdot(n2, vscale(r2, -1.0)) < 0) n2 = vscale(n2, -1.0);
+    if (vdot(n3, vscale(r3, -1.0)) < 0) n3 = vscale(n3, -1.0);
+    if (vdot(n4, r1) < 0) n4 = vscale(n4, -1.0);
+    const double A1 = 0.5 * vlen(vcross(r2, r3)), A2 = 0.5 * vlen(vcross(r3, r1)), A3 = 0.5 * vlen(vcross(r1, r2));
+    const double A4 = 0.5 * vlen(vcross(vsub(r3, r1), vsub(r3, r2)));
+    const d3 fA1 = vscale(v3(vdot(s1, n1), vdot(s2, n1), vdot(s3, n1)), A1);
+    const d3 fA2 = vscale(v3(vdot(s1, n2), vdot(s2, n2), vdot(s3, n2)), A2);
+    const d3 fA3 = vscale(v3(vdot(s1, n3), vdot(s2, n3), vdot(s3, n3)), A3);
+    const d3 fA4 = vscale(v3(vdot(s1, n4), vdot(s2, n4), vdot(s3, n4)), A4);
+    const d3 f1 = vscale(vadd(vadd(fA2, fA3), fA4), -1.0 / 3.0);
+    const d3 f2 = vscale(vadd(vadd(fA3, fA4), fA1), -1.0 / 3.0);
+    const d3 f3 = vscale(vadd(vadd(fA4, fA1), fA2), -1.0 / 3.0);
+    const d3 f4 = vscale(vadd(vadd(fA1, fA2), fA3), -1.0 / 3.0);
+    scatter(force, id1, f1.x, atomic); scatter(force, N + id1, f1.y, atomic); scatter(force, 2 * N + id1, f1.z, atomic);
+    scatter(force, id2, f2.x, atomic); scatter(force, N + id2, f2.y, atomic); scatter(force, 2 * N + id2, f2.z, atomic);
+    scatter(force, id3, f3.x, atomic); scatter(force, N + id3, f3.y, atomic); scatter(force, 2 * N + id3, f3.z, atomic);
+    scatter(force, id4, f4.x, atomic); scatter(force, N + id4, f4.y, atomic); scatter(force, 2 * N + id4, f4.z, atomic);
+}
+
+void orc_linear_viscoelastic(const orc_lve_params* P, int64_t N, int64_t T, const double* points,
+                             const uint32_t* tetras, const double* ev, double* force, double* devold,
+                             double* transient, double* vonMises, double* pressure) {
+    for (int64_t t = 0; t < T; t++)
+        lve_one(t, P, N, T, points, tetras, ev, force, devold, transient, vonMises, pressure, 0);
 }
 
 void orc_legacy_force(int law, const orc_legacy_params* P, int64_t N, int64_t T, const double* points,

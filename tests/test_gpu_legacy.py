@@ -189,3 +189,45 @@ def test_surface_adhesion(ctx, mesh):
          0.02, 1.0)
     assert np.abs(fr).max() > 0
     rel_close(host(fd), fr, 1e-4)
+
+
+def test_linear_viscoelastic_steps(ctx, mesh):
+    """nfx_linear_viscoelastic_force over 12 steps of a changing deformation (state carried from
+    step to step) against the oracle: forces, von Mises stress, pressure and both state arrays."""
+    import torch
+
+    from nofluidx3d_amd import _lib
+
+    P, T, x0 = mesh
+    N, nT = len(P), len(T)
+    ev, vol = O.edge_vectors_volumes(P, T)
+    evd, vold = dev(O.soa(ev)), dev(np.ascontiguousarray(vol))
+    dminv = torch.empty(9 * nT, dtype=torch.float32, device="cuda:0")
+    v0 = torch.empty(nT, dtype=torch.float32, device="cuda:0")
+    call("nfx_precompute_f64", ctx, evd.data_ptr(), vold.data_ptr(), nT, dminv.data_ptr(), v0.data_ptr())
+    w, tau = [0.3, 0.25, 0.1], [3.0, 15.0, 60.0]
+    Po = O.lve_params(0.01, 0.42, w, tau)
+    Pg = _lib.Viscoelastic()
+    Pg.E, Pg.nu, Pg.nterms = 0.01, 0.42, 3
+    for j in range(3):
+        Pg.weights[j], Pg.taus[j] = w[j], tau[j]
+    do_r, tr_r = np.zeros(9 * nT), np.zeros(27 * nT)
+    do_d = torch.zeros(9 * nT, dtype=torch.float32, device="cuda:0")
+    tr_d = torch.zeros(27 * nT, dtype=torch.float32, device="cuda:0")
+    vm_d = torch.zeros(nT, dtype=torch.float32, device="cuda:0")
+    pr_d = torch.zeros(nT, dtype=torch.float32, device="cuda:0")
+    td = dev(O.soa(T, np.uint32))
+    for s in range(12):
+        Fh = np.eye(3) + 0.01 * (s + 1) * np.array([[0.2, 0.5, 0.0], [0.1, -0.3, 0.2], [0.0, 0.4, 0.1]])
+        x = O.soa((P @ Fh.T).astype(np.float32).astype(np.float64))
+        fr, vm_r, pr_r = np.zeros(3 * N), np.zeros(nT), np.zeros(nT)
+        O.linear_viscoelastic(Po, x, O.soa(T, np.uint32), O.soa(ev), fr, do_r, tr_r, vm_r, pr_r)
+        xd = dev(x, np.float32)
+        fd = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
+        call("nfx_linear_viscoelastic_force", ctx, Pg, xd.data_ptr(), td.data_ptr(), dminv.data_ptr(), N, nT,
+             do_d.data_ptr(), tr_d.data_ptr(), vm_d.data_ptr(), pr_d.data_ptr(), fd.data_ptr(), None)
+        rel_close(host(fd), fr, 1e-4)
+        rel_close(host(vm_d), vm_r, 1e-4)
+        rel_close(host(pr_d), pr_r, 1e-4)
+    rel_close(host(do_d), do_r, 1e-4)
+    rel_close(host(tr_d), tr_r, 1e-4)

@@ -469,6 +469,34 @@ def test_legacy_poro_neo_hookean_energy_consistency():
     assert np.allclose(fb, fa @ Q.T, rtol=1e-10, atol=1e-16)
 
 
+def test_linear_viscoelastic_prony_relaxation():
+    """InteractionLinearViscoelastic.cl: a shear strain held from step 1 relaxes the deviatoric
+    stress as the Prony series 2 mu (w_inf + sum_j w_j exp(-(k-1)/tau_j)) eps (the instantaneous
+    modulus at step 1); without terms the law is the linear-elastic law."""
+    E, nu = 0.01, 0.4
+    mu = E / (2 * (1 + nu))
+    w, tau = [0.3, 0.2], [5.0, 40.0]
+    P = O.lve_params(E, nu, w, tau)
+    g = 0.02
+    F = np.array([[1, g, 0], [0, 1, 0], [0, 0, 1.0]])
+    x = deform(F, REF_TET)
+    _, t, ev, _ = one_tet(REF_TET)
+    devold, trans = np.zeros(9), np.zeros(18)
+    eps12 = 0.5 * g
+    for k in range(1, 30):
+        f = np.zeros(12)
+        O.linear_viscoelastic(P, O.soa(x), t, ev, f, devold, trans)
+        sig = virial_stress(REF_TET, x, f.reshape(3, -1).T)
+        G = mu * (1 - sum(w) + sum(wj * np.exp(-(k - 1) / tj) for wj, tj in zip(w, tau)))
+        assert sig[0, 1] == pytest.approx(2 * G * eps12, rel=1e-9), k
+    # no terms: the linear-elastic law
+    x = deform(random_F(0.1), REF_TET)
+    f0 = legacy_forces(0, O.LegacyParams(E=E, nu=nu), REF_TET, x)[0]
+    f = np.zeros(12)
+    O.linear_viscoelastic(O.lve_params(E, nu, [], []), O.soa(x), t, ev, f, np.zeros(9), np.zeros(0) + np.zeros(1))
+    assert np.allclose(f.reshape(3, -1).T, f0, rtol=1e-10, atol=1e-16)
+
+
 # --------------------------------------------------------------------------------------------
 # legacy contacts and integrators (RoundTip, AdhesivePlane, PointZeroForce, CalcCOM/COMForce,
 # OverDamped, velocityVerlet2, SphereIntegral/TipIntegral, AdhesivePlaneSurfaceIntegral)
