@@ -12,7 +12,8 @@ shape parameters (nu, log10 eta/E) only.
 Surrogate-assisted fit (``fit_surrogate``), for all targets at once:
 1. ``rounds`` cross-entropy rounds of ``ncand`` candidates (nu, log10 eta/E) per target, each
    round one batched inference over targets x ncand rows; each candidate's E comes from the
-   closed-form amplitude against the target (clipped to the sampled E and viscosity ranges);
+   closed-form amplitude against the target (the network is evaluated at the candidate's symmetric
+   image inside the sampled E and viscosity ranges);
 2. the ``ntop`` best distinct candidates per target are simulated with the batched simulator on
    the targets' own meshes and protocols; each simulated curve gets its closed-form amplitude and
    the best becomes the fit;
@@ -123,23 +124,24 @@ def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, see
             es = np.stack([elite[..., 1], elite[..., 2] - elite[..., 0]], -1)
             mu, sd = es.mean(1), es.std(1) + 0.02 * (shi - slo)
             sh = np.clip(mu[:, None, :] + sd[:, None, :] * rng.normal(size=(n, ncand, 2)), slo, shi)
-        # E from the amplitude of the predicted shape at a reference E (the network is E-invariant
-        # up to its error, so one pass at the reference E and one at the amplitude-corrected E)
+        # E from the closed-form amplitude of the predicted shape.  The network sees the symmetric
+        # image of each candidate inside the sampled box (same nu and viscosity / E); the candidate
+        # itself keeps the amplitude's E, which may leave the box (the simulator takes any E).
         th = np.zeros((n, ncand, 3))
         th[..., 1] = sh[..., 0]
         th[..., 0] = 3.0
         th[..., 2] = th[..., 0] + sh[..., 1]
-        th = stabilize(clip_theta(th).reshape(-1, 3), T0r).reshape(n, ncand, 3)
+        th = stabilize(th.reshape(-1, 3), T0r).reshape(n, ncand, 3)
         for _ in range(2):
-            rows = th.reshape(-1, 3)
+            img = clip_theta(th)
             ti = time.perf_counter()
-            pred = net.predict(norm.x(features(rows, rep))) * norm.y_scale
+            pred = net.predict(norm.x(features(img.reshape(-1, 3), rep))) * norm.y_scale
             infer_s += time.perf_counter() - ti
-            F = pred.reshape(n, ncand, -1) * (10 ** th[:, :, :1]) * R2[:, None, None]
+            F = pred.reshape(n, ncand, -1) * (10 ** img[:, :, :1]) * R2[:, None, None]
             a = np.clip(amplitude(F, targets[:, None, :]), 1e-6, 1e6)
+            th = img.copy()
             th[..., 0] += np.log10(a)
             th[..., 2] += np.log10(a)
-            th = stabilize(clip_theta(th).reshape(-1, 3), T0r).reshape(n, ncand, 3)
         err = rel_err(F * a[..., None], targets[:, None, :])
         all_theta = np.concatenate([all_theta, th], 1)
         all_err = np.concatenate([all_err, err], 1)
@@ -166,9 +168,10 @@ def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, see
         F = simulate(bs, [theta_to_spec(top[i, k], bases[i]) for i in range(n)])
         a = np.clip(amplitude(F, targets), 1e-6, 1e6)
         scaled = top[:, k] + np.log10(a)[:, None] * np.array([1.0, 0.0, 1.0])
-        cand = stabilize(clip_theta(scaled), T0)
-        # the amplitude-corrected parameters give exactly a F (the E-viscosity symmetry) unless the
-        # box or the stability limit moved them; then keep the simulated candidate itself
+        cand = stabilize(scaled, T0)
+        # the amplitude-corrected parameters give exactly a F (the E-viscosity symmetry; the
+        # stability limit is invariant under it); a candidate that the limit did move keeps its
+        # simulated curve unscaled
         exact = np.all(np.abs(cand - scaled) <= 1e-12, axis=1)
         Fk = np.where(exact[:, None], F * a[:, None], F)
         cand = np.where(exact[:, None], cand, top[:, k])
