@@ -239,7 +239,7 @@ def bench_surrogate(weights, n_targets=512, log=log):
            "train_step_ms_batch4096": t_train * 1e3, "train_tflops": 3 * flops_row * B / t_train / 1e12,
            "mfma_peak_tflops_bf16_dense": 2500.0}
     cells, meshes, bs, clean, noisy = fit_bench.make_targets(n_targets)
-    r = fit.fit_surrogate(net, norm, noisy, fit_bench.hidden_blind(cells), bs, ntop=2)
+    r = fit.fit_surrogate(net, norm, noisy, fit_bench.hidden_blind(cells), bs, ntop=1, corrections=1)
     out["fit"] = {"targets": n_targets, "fits_per_s": n_targets / r["total_s"], "batch_wall_s": r["total_s"],
                   "simulations_per_target": r["simulations_per_target"],
                   "confirmed_rel_l2": fit_bench.err_stats(r["sim_err"]),

@@ -17,7 +17,10 @@ Surrogate-assisted fit (``fit_surrogate``), for all targets at once:
 2. the ``ntop`` best distinct candidates per target are simulated with the batched simulator on
    the targets' own meshes and protocols; each simulated curve gets its closed-form amplitude and
    the best becomes the fit;
-3. optionally the fit's parameters are simulated once more (``confirm``); otherwise the fit's curve
+3. ``corrections`` corrected rounds: the residual of the best simulation against the network at
+   the same parameters is added to every prediction, the search runs again around the best point
+   and its best candidate is simulated;
+4. optionally the fit's parameters are simulated once more (``confirm``); otherwise the fit's curve
    is the refining simulation's scaled by its amplitude, which the symmetry makes exact.
 Baseline (``fit_nelder_mead``): simulation-only Nelder-Mead on (log10 E, nu, log10 eta), all
 targets in lock-step, one batched simulation per evaluation.
@@ -97,32 +100,29 @@ def stabilize(theta, T0, max_relax=0.1):
     return th
 
 
-def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, seed=0, log=None, confirm=True):
-    """targets (n, 256) newtons; bases (n) CellSpec with the known geometry/protocol; ``bs`` a
-    BatchSimulation over the targets' meshes.  Returns dict with the fitted parameters, the fit's
-    simulated curve and its error, and timings.  ``confirm``: simulate the final parameters once
-    more; without it the curve is the refining simulation's, exactly scaled by the closed-form
-    amplitude (identical to the simulator's output at the final parameters by the E-viscosity
-    symmetry)."""
+def _search(net, norm, targets, bases, R2, T0r, rounds, ncand, rng, corr=None, start=None):
+    """Cross-entropy search over the shape parameters (nu, log10 eta/E) of every target at once.
+    ``corr`` (n, ncurve): additive correction of the normalised prediction per target; ``start``
+    (mu, sd), each (n, 2): the first round's sampling distribution (else uniform over the box).
+    Returns every candidate (n, rounds*ncand, 3) with its amplitude-fitted E, their errors against
+    the targets and the inference time."""
     n = len(targets)
-    rng = np.random.Generator(np.random.PCG64(seed))
-    R2 = np.array([(b.radius_um * 1e-6) ** 2 for b in bases])
     slo, shi = np.array([LO[1], LO[2] - HI[0]]), np.array([HI[1], HI[2] - LO[0]])  # nu, log10 eta/E
     rep = [b for b in bases for _ in range(ncand)]
-    T0 = time_units(bases, bs.steps)
-    T0r = np.repeat(T0, ncand)
-    t0 = time.perf_counter()
     all_theta = np.zeros((n, 0, 3))
     all_err = np.zeros((n, 0))
     infer_s = 0.0
     for r in range(rounds):
-        if r == 0:
+        if r == 0 and start is None:
             sh = slo + (shi - slo) * rng.random((n, ncand, 2))
         else:
-            order = np.argsort(all_err, axis=1)[:, :8]
-            elite = np.take_along_axis(all_theta, order[:, :, None], 1)
-            es = np.stack([elite[..., 1], elite[..., 2] - elite[..., 0]], -1)
-            mu, sd = es.mean(1), es.std(1) + 0.02 * (shi - slo)
+            if r == 0:
+                mu, sd = start
+            else:
+                order = np.argsort(all_err, axis=1)[:, :8]
+                elite = np.take_along_axis(all_theta, order[:, :, None], 1)
+                es = np.stack([elite[..., 1], elite[..., 2] - elite[..., 0]], -1)
+                mu, sd = es.mean(1), es.std(1) + 0.02 * (shi - slo)
             sh = np.clip(mu[:, None, :] + sd[:, None, :] * rng.normal(size=(n, ncand, 2)), slo, shi)
         # E from the closed-form amplitude of the predicted shape.  The network sees the symmetric
         # image of each candidate inside the sampled box (same nu and viscosity / E); the candidate
@@ -137,7 +137,10 @@ def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, see
             ti = time.perf_counter()
             pred = net.predict(norm.x(features(img.reshape(-1, 3), rep))) * norm.y_scale
             infer_s += time.perf_counter() - ti
-            F = pred.reshape(n, ncand, -1) * (10 ** img[:, :, :1]) * R2[:, None, None]
+            pred = pred.reshape(n, ncand, -1)
+            if corr is not None:
+                pred = pred + corr[:, None, :]
+            F = pred * (10 ** img[:, :, :1]) * R2[:, None, None]
             a = np.clip(amplitude(F, targets[:, None, :]), 1e-6, 1e6)
             th = img.copy()
             th[..., 0] += np.log10(a)
@@ -145,44 +148,101 @@ def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, see
         err = rel_err(F * a[..., None], targets[:, None, :])
         all_theta = np.concatenate([all_theta, th], 1)
         all_err = np.concatenate([all_err, err], 1)
-    t_sur = time.perf_counter() - t0
+    return all_theta, all_err, infer_s
+
+
+def _distinct_best(all_theta, all_err, k):
+    """the k best candidates per target that differ by more than 1e-3 of the box in some
+    coordinate (the last one repeated where fewer exist)"""
+    n = len(all_theta)
     order = np.argsort(all_err, axis=1)
-    top = np.zeros((n, ntop, 3))
+    top = np.zeros((n, k, 3))
     for i in range(n):
         picked = []
         for j in order[i]:
             cand = all_theta[i, j]
             if all(np.max(np.abs(cand - p) / (HI - LO)) > 1e-3 for p in picked):
                 picked.append(cand)
-            if len(picked) == ntop:
+            if len(picked) == k:
                 break
-        while len(picked) < ntop:
+        while len(picked) < k:
             picked.append(picked[-1])
         top[i] = np.array(picked)
+    return top
+
+
+def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, corrections=0, corr_rounds=4, seed=0,
+                  log=None, confirm=True):
+    """targets (n, 256) newtons; bases (n) CellSpec with the known geometry/protocol; ``bs`` a
+    BatchSimulation over the targets' meshes.  Returns dict with the fitted parameters, the fit's
+    simulated curve and its error, and timings.
+
+    ``corrections``: after the ``ntop`` refining simulations, that many corrected rounds.  Each
+    adds the residual of the best simulation so far against the network at the same parameters
+    (normalised curves; it carries what the network cannot see, chiefly the target's own mesh) to
+    every prediction, searches again (``corr_rounds`` cross-entropy rounds around the best point)
+    and simulates the new best candidate: a zeroth-order additive correction of the surrogate by
+    the simulator, as in multi-fidelity model management.
+
+    ``confirm``: simulate the final parameters once more; without it the curve is the refining
+    simulation's, exactly scaled by the closed-form amplitude (identical to the simulator's output
+    at the final parameters by the E-viscosity symmetry)."""
+    n = len(targets)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    R2 = np.array([(b.radius_um * 1e-6) ** 2 for b in bases])
+    T0 = time_units(bases, bs.steps)
+    T0r = np.repeat(T0, ncand)
+    t0 = time.perf_counter()
+    all_theta, all_err, infer_s = _search(net, norm, targets, bases, R2, T0r, rounds, ncand, rng)
+    t_sur = time.perf_counter() - t0
+    top = _distinct_best(all_theta, all_err, ntop)
     t1 = time.perf_counter()
     best_err = np.full(n, np.inf)
-    top = np.stack([stabilize(top[:, k], T0) for k in range(ntop)], 1)
     theta = top[:, 0].copy()
     curve = np.zeros_like(targets)
-    for k in range(ntop):
-        F = simulate(bs, [theta_to_spec(top[i, k], bases[i]) for i in range(n)])
+    best_norm = np.zeros_like(targets)  # normalised simulated curve at the best parameters
+
+    def refine(cands, label):
+        nonlocal theta, curve, best_err, best_norm
+        cands = stabilize(cands, T0)
+        F = simulate(bs, [theta_to_spec(cands[i], bases[i]) for i in range(n)])
+        cn = bs.curves_normalized()
         a = np.clip(amplitude(F, targets), 1e-6, 1e6)
-        scaled = top[:, k] + np.log10(a)[:, None] * np.array([1.0, 0.0, 1.0])
+        scaled = cands + np.log10(a)[:, None] * np.array([1.0, 0.0, 1.0])
         cand = stabilize(scaled, T0)
         # the amplitude-corrected parameters give exactly a F (the E-viscosity symmetry; the
         # stability limit is invariant under it); a candidate that the limit did move keeps its
         # simulated curve unscaled
         exact = np.all(np.abs(cand - scaled) <= 1e-12, axis=1)
         Fk = np.where(exact[:, None], F * a[:, None], F)
-        cand = np.where(exact[:, None], cand, top[:, k])
+        cand = np.where(exact[:, None], cand, cands)
         e = rel_err(Fk, targets)
         better = e < best_err
         best_err[better] = e[better]
         theta[better] = cand[better]
         curve[better] = Fk[better]
+        best_norm[better] = cn[better]
         if log:
-            log(f"refine {k + 1}/{ntop}: {time.perf_counter() - t1:.1f}s")
-    t_ref = time.perf_counter() - t1
+            log(f"refine {label}: {time.perf_counter() - t1:.1f}s, median error {np.median(best_err):.4f}")
+
+    for k in range(ntop):
+        refine(top[:, k], f"{k + 1}/{ntop}")
+    for c in range(corrections):
+        rep = list(bases)
+        img = clip_theta(theta)
+        ti = time.perf_counter()
+        at = net.predict(norm.x(features(img, rep))) * norm.y_scale
+        infer_s += time.perf_counter() - ti
+        corr = best_norm - at
+        mu = np.stack([theta[:, 1], theta[:, 2] - theta[:, 0]], 1)
+        sd = np.repeat(np.array([[0.02, 0.15]]), n, 0)
+        ts = time.perf_counter()
+        th_c, err_c, inf_c = _search(net, norm, targets, bases, R2, T0r, corr_rounds, ncand, rng, corr=corr,
+                                     start=(mu, sd))
+        infer_s += inf_c
+        t_sur += time.perf_counter() - ts
+        refine(_distinct_best(th_c, err_c, 1)[:, 0], f"correction {c + 1}/{corrections}")
+    t_ref = time.perf_counter() - t1 - (t_sur - (t1 - t0))
     t2 = time.perf_counter()
     if confirm:
         curve = simulate(bs, [theta_to_spec(theta[i], bases[i]) for i in range(n)])
@@ -190,7 +250,8 @@ def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, see
     return {"theta": theta, "curve": curve, "sim_err": rel_err(curve, targets),
             "predicted_refined_err": best_err, "surrogate_s": t_sur, "inference_s": infer_s,
             "refine_s": t_ref, "confirm_s": t_conf, "total_s": time.perf_counter() - t0,
-            "candidates_per_target": rounds * ncand, "simulations_per_target": ntop + (1 if confirm else 0)}
+            "candidates_per_target": (rounds + corrections * corr_rounds) * ncand,
+            "simulations_per_target": ntop + corrections + (1 if confirm else 0)}
 
 
 def fit_nelder_mead(targets, bases, bs, max_evals=60, log=None):

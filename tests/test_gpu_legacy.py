@@ -218,7 +218,8 @@ def test_linear_viscoelastic_steps(ctx, mesh):
     pr_d = torch.zeros(nT, dtype=torch.float32, device="cuda:0")
     td = dev(O.soa(T, np.uint32))
     for s in range(12):
-        Fh = np.eye(3) + 0.01 * (s + 1) * np.array([[0.2, 0.5, 0.0], [0.1, -0.3, 0.2], [0.0, 0.4, 0.1]])
+        # a volumetric part (trace 0.3) so that the pressure is not rounding noise around zero
+        Fh = np.eye(3) + 0.01 * (s + 1) * np.array([[0.2, 0.5, 0.0], [0.1, -0.1, 0.2], [0.0, 0.4, 0.2]])
         x = O.soa((P @ Fh.T).astype(np.float32).astype(np.float64))
         fr, vm_r, pr_r = np.zeros(3 * N), np.zeros(nT), np.zeros(nT)
         O.linear_viscoelastic(Po, x, O.soa(T, np.uint32), O.soa(ev), fr, do_r, tr_r, vm_r, pr_r)
@@ -228,6 +229,7 @@ def test_linear_viscoelastic_steps(ctx, mesh):
              do_d.data_ptr(), tr_d.data_ptr(), vm_d.data_ptr(), pr_d.data_ptr(), fd.data_ptr(), None)
         rel_close(host(fd), fr, 1e-4)
         rel_close(host(vm_d), vm_r, 1e-4)
-        rel_close(host(pr_d), pr_r, 1e-4)
+        # pressure = kappa tr(eps): relative to the stress scale (tr(eps) cancels to a few ulps of F)
+        rel_close(host(pr_d), pr_r, 1e-4 * max(np.abs(vm_r).max() / np.abs(pr_r).max(), 1.0))
     rel_close(host(do_d), do_r, 1e-4)
     rel_close(host(tr_d), tr_r, 1e-4)

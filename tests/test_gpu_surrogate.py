@@ -58,15 +58,15 @@ def test_e_viscosity_scaling_is_exact(pipeline):
 
 
 def test_surrogate_fit_beats_simulation_only_at_equal_budget(pipeline):
-    """Surrogate screen + 2 refining simulations + 1 confirming simulation per target against
-    Nelder-Mead with 6 simulations per target (the initial simplex and one step)."""
+    """Surrogate screen + 1 refining simulation + 1 corrected round + 1 confirming simulation per
+    target against Nelder-Mead with 6 simulations per target (the initial simplex and one step)."""
     pool, bs, net, norm, hist, _ = pipeline
     from nofluidx3d_amd.surrogate import fit, train
 
     truth = train.sample_physics(pool, len(pool), 777, bs.steps)
     targets = fit.simulate(bs, truth)
     blind = [dataclasses.replace(c, E_si=1000.0, nu=0.4, viscosity_si=10.0) for c in truth]
-    sur = fit.fit_surrogate(net, norm, targets, blind, bs, rounds=6, ntop=2)
+    sur = fit.fit_surrogate(net, norm, targets, blind, bs, rounds=6, ntop=1, corrections=1)
     nm = fit.fit_nelder_mead(targets, blind, bs, max_evals=6)
     assert sur["simulations_per_target"] == 3
     assert np.median(sur["sim_err"]) < np.median(nm["sim_err"])
