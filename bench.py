@@ -10,7 +10,8 @@ Printed JSON line (one):
 - value = tetra-updates/s = T * steps / wall seconds of the timed region;
 - hbm_gbs = algorithmic bytes per step / ms_per_step (96 T + 108 N for the viscoplastic step:
   connectivity 16 B + shape state S = Dm^-1 Fp^-1 read and written 72 B + epsbar read and written
-  8 B per tetra; 56 T + 108 N for the elastic one: connectivity 16 + Dm^-1 36 + V0 4);
+  8 B per tetra; 56 T + 108 N for the elastic one: connectivity 16 + Dm^-1 36 + V0 4; the batch's
+  linear Maxwell law keeps no epsbar: 88 T + 108 N);
 - roofline: the dominant kernel (the force pass) -- algorithmic bytes per launch (96 T + 24 N)
   over its HIP-event average duration, against 8 TB/s; ``traffic`` = measured HBM bytes per
   launch from rocprofv3 FETCH_SIZE + WRITE_SIZE when ``--traffic-json`` names a file holding them;
@@ -163,8 +164,8 @@ def bench_batched(steps=10000, ncells=512, log=log):
     t0 = time.time()
     meshes = make_meshes(specs)
     tmesh = time.time() - t0
-    K, tuned = tune_batch(specs, meshes, steps, use_cache=False, log=log)
-    b = BatchSimulation(specs, meshes, steps=steps, ncurve=256, tile_k=K)
+    best, tuned = tune_batch(specs, meshes, steps, use_cache=False, log=log)
+    b = BatchSimulation(specs, meshes, steps=steps, ncurve=256, tile_k=best["tile_k"], stream_nt=best["stream_nt"])
     b.advance(100)  # includes the graph capture
     b.sync()
     t0 = time.perf_counter()
@@ -176,7 +177,9 @@ def bench_batched(steps=10000, ncells=512, log=log):
     curves = b.curves_normalized()
     ok = np.isfinite(curves).all(axis=1)
     tet = np.array([len(m[1]) for m in meshes])
-    step_bytes = VP_TET_BYTES * b.T_real + 108 * b.N
+    # the batch's linear Maxwell law does not read the accumulated plastic strain: it is not kept
+    tet_bytes = VP_TET_BYTES if b.eps is not None else VP_TET_BYTES - 8
+    step_bytes = tet_bytes * b.T_real + 108 * b.N
     return {
         "workload": f"{ncells} cells (seeds 1000+i), {b.T_real} tetra, {b.N} nodes, NH + Maxwell viscoplastic, "
                     f"{steps} steps each, 256-point curve per cell",
@@ -185,8 +188,10 @@ def bench_batched(steps=10000, ncells=512, log=log):
         "nodes": int(b.N),
         "tetra_per_cell": {"min": int(tet.min()), "median": int(np.median(tet)), "max": int(tet.max())},
         "sampling_redraws": stats,
-        "tile_k": K,
+        "tile_k": best["tile_k"],
+        "stream_nt": best["stream_nt"],
         "autotune_ms_per_step": tuned,
+        "algorithmic_bytes_per_step": int(step_bytes),
         "ms_per_step": ms,
         "value": b.T_real * 1e3 / ms,
         "unit": "tetra-updates/s",
@@ -345,7 +350,7 @@ def main():
     traffic = None
     if args.traffic_json and os.path.exists(args.traffic_json):
         tj = json.load(open(args.traffic_json))
-        key = f"{material}_{args.scatter}_{sim.tile_k}"
+        key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if sim.stream_nt else ''}"
         if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}):
             traffic = tj["force_pass_bytes"][key]
     out = {
@@ -370,6 +375,7 @@ def main():
             "tetra": nT,
             "scatter": args.scatter,
             "tile_k": sim.tile_k,
+            "stream_nt": sim.stream_nt,
             "autotune_ms_per_step": tuned,
             "parallelism": "single GPU",
         },

@@ -92,7 +92,8 @@ typedef struct {
     double E, nu, ratio;
     nfx_viscoplastic vp;
     float* Fp;        /* 9T, model NFX_VISCOPLASTIC */
-    float* epsbar;    /* T */
+    float* epsbar;    /* T; may be NULL on the tiled path when the law is linear (flow exponent 1,
+                         hardening exponent 0: the flow does not read it), and then is not kept */
     float* von_mises; /* T, nullable: when set (with pressure), nfx_run_steps writes the per-tetra von
                          Mises stress and pressure of the last step of the call (viscoplastic and
                          legacy laws) */
@@ -102,11 +103,14 @@ typedef struct {
     const uint32_t* csr_slots;   /* 4T, slot = 4*t + corner, ascending per node */
     float* corners;              /* 12T scratch: force of corner a of tetra t at 12*t + 3*a + c */
     int32_t tile_k;              /* NFX_SCATTER_TILED: tetra per tile (256 or 512) */
-    int32_t tile_max_nodes;      /* MX: tile-node capacity per tile (largest distinct-node count) */
+    int32_t tile_max_nodes;      /* MX: tile-node capacity per tile (largest distinct-node count,
+                                    at most 256) */
     const uint32_t* tile_count;  /* ntiles: distinct nodes of tile c (<= MX) */
     const uint32_t* tile_nodes;  /* ntiles*MX: node ids, ascending within a tile, padded */
     const uint16_t* tile_lend;   /* ntiles*MX: end of each tile-node's run in the tile's slot list */
-    const uint16_t* tile_slots;  /* 4*tile_k*ntiles: local slots 4*(t-t0)+a grouped by tile-node */
+    const uint16_t* tile_corners; /* 4*tile_k*ntiles: for corner a of tetra t (entry 4*(t-t0)+a),
+                                    (rank << 8) | tile-node: the tile-local node (< 256) and the
+                                    corner's place among that node's corners in slot order */
     const uint32_t* tile_dst;    /* ntiles*MX: position of each tile-node's partial in node order */
     const uint32_t* node_off;    /* N+1: partials of node n at [node_off[n], node_off[n+1]) */
     float* partials;             /* 3*M scratch, node order (tile order within a node) */
@@ -118,6 +122,10 @@ typedef struct {
     float* vp_shape; /* 9T, model NFX_VISCOPLASTIC with NFX_SCATTER_TILED: the state in shape form
                         S = Dm^-1 Fp^-1 (replaces Fp and dminv on that path; nfx_vp_shape_from_fp) */
     double shear, bulk, shear2, volume_fraction; /* legacy laws (as in nfx_elastic) */
+    int32_t stream_nt; /* NFX_SCATTER_TILED with NFX_MR or NFX_VISCOPLASTIC: 1 = the force pass reads
+                          and writes the per-tetra state and tile tables with the nontemporal cache
+                          policy (faster when they exceed the memory-side cache), 0 = default policy;
+                          an autotuner choice, results are identical */
 } nfx_plan;
 
 /* Per-cell parameters of a batch, in that cell's simulation units.  The batched material is the
@@ -132,7 +140,11 @@ typedef struct {
 } nfx_cell;
 
 /* A batch is the concatenation of many cells' meshes, with tiles aligned to cells (a tile never
- * spans two cells, so each cell sums its forces in the same order as when it runs alone). */
+ * spans two cells, so each cell sums its forces in the same order as when it runs alone).
+ * Batched steps keep the damped velocity-Verlet state as (x, u) with u = a v + f/2 of the last
+ * step (a = 1 - xi/2) in plan.vel: the same recurrence as VelocityVerlet.cl (x += u,
+ * u' = (a/b)(u + f'/2) + f'/2, b = 1 + xi/2) without the previous force; plan.fold is unused.
+ * Zero v and f give u = 0. */
 typedef struct {
     int32_t ncells;
     const uint16_t* tile_ntet;     /* ntiles: tetra in each tile */
@@ -143,6 +155,11 @@ typedef struct {
     double* curves;                /* ncells*ncurve FP64 indenter-force samples */
     int32_t ncurve;
     int64_t curve_steps; /* sample k is taken at the first step s with floor(s*ncurve/curve_steps) == k */
+    /* With every cell at least 256 nodes (so that a node block of 256 spans at most two cells) and
+     * this scratch, the node pass takes the curve samples itself, with no launch of its own: */
+    int32_t min_cell_nodes;  /* smallest node count of a cell */
+    double* curve_partials;  /* 2*ceil(N/256) FP64, nullable */
+    uint32_t* curve_counter; /* 1, zero before the first step (the node pass leaves it zero), nullable */
 } nfx_batch;
 
 /* ---- device and queue: replaces openCL.py:17-50 -------------------------------------------- */
@@ -204,10 +221,11 @@ int nfx_gather_corners(nfx_ctx* ctx, const uint32_t* csr_offsets, const uint32_t
                        int64_t N, float* force);
 /* host-side tiling for NFX_SCATTER_TILED (host pointers).  Call once with tile_nodes == NULL to get
  * *max_nodes (MX), *m_total and tile_count, then again with arrays of ntiles*MX entries
- * (tile_nodes, tile_lend, tile_dst), 4*tile_k*ntiles (tile_slots) and N+1 (node_off). */
+ * (tile_nodes, tile_lend, tile_dst), 4*tile_k*ntiles (tile_corners) and N+1 (node_off).  The second
+ * call fails when MX > 256 or a node has more than 256 corners in one tile (use tile_k 256). */
 int nfx_build_tiles_host(const uint32_t* tetras_host, int64_t N, int64_t T, int32_t tile_k, int32_t* max_nodes,
                          int64_t* m_total, uint32_t* tile_count, uint32_t* tile_nodes, uint16_t* tile_lend,
-                         uint16_t* tile_slots, uint32_t* tile_dst, uint32_t* node_off);
+                         uint16_t* tile_corners, uint32_t* tile_dst, uint32_t* node_off);
 /* host-side CSR build (host pointers): offsets N+1, slots 4T ascending per node */
 int nfx_build_node_csr_host(const uint32_t* tetras_host, int64_t N, int64_t T, uint32_t* offsets_host,
                             uint32_t* slots_host);

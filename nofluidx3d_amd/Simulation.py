@@ -18,6 +18,8 @@ Extensions (keyword arguments):
   "atomic" float atomics in arbitrary order, as the reference's atomicAdd.
   The physics is the same; only the floating-point summation order differs.
 ``tile_k`` tetra per tile for "tiled" (256 or 512);
+``stream_nt`` cache policy of the tiled force pass's per-tetra streams (1 nontemporal, 0 default,
+  None: nontemporal once they exceed the 256 MB memory-side cache; ``autotune`` times both);
 ``fused`` and ``use_graph`` switch the fused step and the graph replay off.
 """
 
@@ -30,6 +32,7 @@ import time
 import numpy as np
 
 from nofluidx3d_amd import _lib, openCL
+from nofluidx3d_amd.autotune import default_stream_nt
 from nofluidx3d_amd.interactions import (FiniteStrainViscoplastic, LegacyElastic, LinearElastic,
                                         LinearElasticDeviatoric, LinearViscoelastic, Plane, PoroNeoHookean,
                                         SecondOrderNeoHookean, Sphere, Substrate, Tetra, VelocityVerlet)
@@ -48,7 +51,8 @@ def _law(fn, start, speed, move_steps):
 
 
 class Simulation:
-    def __init__(self, con, scatter="tiled", fused=True, use_graph=True, mesh=None, verbose=True, tile_k=512):
+    def __init__(self, con, scatter="tiled", fused=True, use_graph=True, mesh=None, verbose=True, tile_k=512,
+                 stream_nt=None):
         if isinstance(con, dict):
             self.parameters = dict(con)
         else:
@@ -58,6 +62,7 @@ class Simulation:
             raise ValueError(f"scatter must be 'tiled', 'csr' or 'atomic', not {scatter!r}")
         self.scatter = scatter
         self.tile_k = tile_k
+        self.stream_nt = stream_nt
         self.fused = fused
         self.use_graph = use_graph
         self.verbose = verbose
@@ -421,14 +426,17 @@ class Simulation:
             _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), self.numPoints, self.numTetra, K,
                       ctypes.byref(mx), ctypes.byref(m), count.ctypes.data_as(vp), None, None, None, None, None)
             MX, M = mx.value, m.value
+            if MX > min(K, _lib.TILE_MAX_NODES):  # not tileable at this size: the plan falls back to CSR
+                self._tiles = {"tile_k": K, "max_nodes": MX, "m_total": M}
+                return self._tiles
             tile_nodes = np.zeros(ntiles * MX, np.uint32)
             tile_lend = np.zeros(ntiles * MX, np.uint16)
-            tile_slots = np.zeros(4 * K * ntiles, np.uint16)
+            tile_corners = np.zeros(4 * K * ntiles, np.uint16)
             tile_dst = np.zeros(ntiles * MX, np.uint32)
             node_off = np.zeros(self.numPoints + 1, np.uint32)
             _lib.call("nfx_build_tiles_host", tets.ctypes.data_as(vp), self.numPoints, self.numTetra, K,
                       ctypes.byref(mx), ctypes.byref(m), count.ctypes.data_as(vp), tile_nodes.ctypes.data_as(vp),
-                      tile_lend.ctypes.data_as(vp), tile_slots.ctypes.data_as(vp), tile_dst.ctypes.data_as(vp),
+                      tile_lend.ctypes.data_as(vp), tile_corners.ctypes.data_as(vp), tile_dst.ctypes.data_as(vp),
                       node_off.ctypes.data_as(vp))
             dev = self.points.buf.device
             to = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a.view(np.int16)).to(dev)
@@ -439,7 +447,7 @@ class Simulation:
                 "tile_count": to(count),
                 "tile_nodes": to(tile_nodes),
                 "tile_lend": to(tile_lend),
-                "tile_slots": to(tile_slots),
+                "tile_corners": to(tile_corners),
                 "tile_dst": to(tile_dst),
                 "node_off": to(node_off),
                 "partials": torch.zeros(3 * M, dtype=torch.float32, device=dev),
@@ -484,15 +492,18 @@ class Simulation:
             off, slots, corners = self.node_csr()
             p.scatter = _lib.SCATTER_CSR
             p.csr_offsets, p.csr_slots, p.corners = off.data_ptr(), slots.data_ptr(), corners.data_ptr()
-        elif self.scatter == "tiled" and self.node_tiles()["max_nodes"] > self.node_tiles()["tile_k"]:
-            # a tile touching more distinct nodes than it has threads: use the CSR gather instead
+        elif self.scatter == "tiled" and self.node_tiles()["max_nodes"] > min(self.node_tiles()["tile_k"],
+                                                                               _lib.TILE_MAX_NODES):
+            # a tile touching more distinct nodes than it has threads (or than its 8-bit tile-node
+            # index allows): use the CSR gather instead
             self.scatter = "csr"
             return self._build_plan()
         elif self.scatter == "tiled":
             t = self.node_tiles()
             p.scatter = _lib.SCATTER_TILED
             p.tile_k, p.tile_max_nodes = t["tile_k"], t["max_nodes"]
-            for k in ("tile_count", "tile_nodes", "tile_lend", "tile_slots", "tile_dst", "node_off", "partials"):
+            p.stream_nt = default_stream_nt(self.numTetra) if self.stream_nt is None else int(self.stream_nt)
+            for k in ("tile_count", "tile_nodes", "tile_lend", "tile_corners", "tile_dst", "node_off", "partials"):
                 setattr(p, k, t[k].data_ptr())
         else:
             p.scatter = _lib.SCATTER_ATOMIC

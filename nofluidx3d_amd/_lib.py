@@ -76,7 +76,7 @@ class Plan(ctypes.Structure):
         ("tile_count", c_vp),
         ("tile_nodes", c_vp),
         ("tile_lend", c_vp),
-        ("tile_slots", c_vp),
+        ("tile_corners", c_vp),
         ("tile_dst", c_vp),
         ("node_off", c_vp),
         ("partials", c_vp),
@@ -95,6 +95,7 @@ class Plan(ctypes.Structure):
         ("bulk", c_f64),
         ("shear2", c_f64),
         ("volume_fraction", c_f64),
+        ("stream_nt", c_i32),
     ]
 
 
@@ -109,8 +110,13 @@ class Batch(ctypes.Structure):
         ("curves", c_vp),
         ("ncurve", c_i32),
         ("curve_steps", c_i64),
+        ("min_cell_nodes", c_i32),
+        ("curve_partials", c_vp),
+        ("curve_counter", c_vp),
     ]
 
+
+TILE_MAX_NODES = 256  # tile-node index in 8 bits (nfx_plan.tile_corners)
 
 NFX_MLP_STORE, NFX_MLP_FWD_GELU, NFX_MLP_BWD_GELU, NFX_MLP_ADAM = 0, 1, 2, 3
 
@@ -213,11 +219,17 @@ def _bind(L, sigs):
 
 
 def lib():
-    """Load libnfx.so (raises if it has not been built)."""
+    """Load libnfx.so (raises if it has not been built).
+
+    torch is imported first: its HIP runtime (SONAME libamdhip64.so.7) is then already in the
+    process when libnfx.so asks for libamdhip64.so.7, so both share one runtime.  Loaded the other
+    way round, torch would bring a second HIP runtime of its own that finds no device."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise NfxError(f"{LIB_PATH} is missing: build it with `python -m nofluidx3d_amd.build`")
+        import torch  # noqa: F401  (loads torch's HIP runtime; initialises no device)
+
         L = ctypes.CDLL(LIB_PATH)
         _bind(L, SIGNATURES)
         _lib = L

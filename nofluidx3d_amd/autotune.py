@@ -7,6 +7,9 @@ physics; tile size only changes the floating-point summation order, by rounding.
 
 Tunables:
 - ``tile_k``: tetra per tile = threads per workgroup of the force pass (256 or 512);
+- ``stream_nt``: cache policy of the force pass's per-tetra streams (state, tile tables): 1 =
+  nontemporal, 0 = default.  A state that fits MI355X's 256 MB memory-side cache is served from it
+  from step to step under the default policy; a larger one streams faster past it;
 - ``graph_steps``: steps per captured hipGraph replay block (the library captures at most 100).
 """
 
@@ -17,6 +20,18 @@ import os
 import time
 
 TILE_CANDIDATES = (256, 512)
+STREAM_CANDIDATES = (0, 1)
+MEMORY_SIDE_CACHE_BYTES = 256 << 20
+STATE_BYTES_PER_TETRA = 48  # force pass streams: 9-float state, plastic strain or volume, 4 slots
+
+
+def default_stream_nt(tetra):
+    """the cache policy to use untuned: nontemporal once the streamed state exceeds the cache"""
+    return int(STATE_BYTES_PER_TETRA * int(tetra) > MEMORY_SIDE_CACHE_BYTES)
+
+
+def _label(K, nt):
+    return f"{K}{'nt' if nt else ''}"
 
 
 def _cache_path():
@@ -45,17 +60,18 @@ def _store(key, value):
 
 
 def tune_batch(specs, meshes, steps, ncurve=256, nrep=10, use_cache=True, log=None):
-    """Pick ``tile_k`` for a batch by timing one force+node+curve pass per candidate.
-    Returns (best_tile_k, {tile_k: ms_per_step})."""
+    """Pick ``tile_k`` and ``stream_nt`` for a batch by timing force+node+curve passes per
+    candidate.  Returns ({"tile_k": K, "stream_nt": nt}, {label: ms_per_step}), labels "256",
+    "256nt", ..."""
     from nofluidx3d_amd.batch import BatchSimulation
 
     tetra = sum(len(m[1]) for m in meshes)
-    key = f"batch:{len(specs)}:{tetra}:{specs[0].viscosity_si is not None}"
+    key = f"batch2:{len(specs)}:{tetra}:{specs[0].viscosity_si is not None}"
     if use_cache:
         hit = _load().get(key)
         if hit:
-            return hit["tile_k"], {int(k): v for k, v in hit["ms"].items()}
-    times = {}
+            return hit["best"], hit["ms"]
+    times, choice = {}, {}
     for K in TILE_CANDIDATES:
         try:
             b = BatchSimulation(specs, meshes, steps=steps, ncurve=ncurve, tile_k=K)
@@ -63,19 +79,23 @@ def tune_batch(specs, meshes, steps, ncurve=256, nrep=10, use_cache=True, log=No
             if log:
                 log(f"tile_k {K}: {e}")
             continue
-        b.time_passes(2)
-        f, n, c = b.time_passes(nrep)
-        times[K] = f + n + c
-        if log:
-            log(f"tile_k {K}: force {f:.4f} ms, node {n:.4f} ms, curve {c:.4f} ms")
+        for nt in STREAM_CANDIDATES:
+            b.plan.stream_nt = nt
+            b.time_passes(2)
+            f, n, c = b.time_passes(nrep)
+            lab = _label(K, nt)
+            times[lab], choice[lab] = f + n + c, {"tile_k": K, "stream_nt": nt}
+            if log:
+                log(f"{lab}: force {f:.4f} ms, node {n:.4f} ms, curve {c:.4f} ms")
         del b
-    best = min(times, key=times.get)
-    _store(key, {"tile_k": best, "ms": times, "when": time.time()})
+    best = choice[min(times, key=times.get)]
+    _store(key, {"best": best, "ms": times, "when": time.time()})
     return best, times
 
 
 def tune_simulation(sim, nrep=20, log=None):
-    """Pick ``tile_k`` for a single-cell Simulation with the tiled scatter (mutates ``sim.tile_k``)."""
+    """Pick ``tile_k`` and ``stream_nt`` for a single-cell Simulation with the tiled scatter (sets
+    ``sim.tile_k`` and ``sim.stream_nt``).  Returns ((tile_k, stream_nt), {label: ms_per_step})."""
     import ctypes
 
     import torch
@@ -83,34 +103,37 @@ def tune_simulation(sim, nrep=20, log=None):
     from nofluidx3d_amd import _lib, openCL
 
     if sim.scatter != "tiled":
-        return sim.tile_k, {}
+        return (sim.tile_k, sim.stream_nt), {}
     saved_time = sim.time
     sim.sync_forms()
     openCL.getCommandQueue().finish()
     snapshot = [t.clone() for t in sim.state_tensors()]  # timing runs real steps: restore afterwards
-    times = {}
+    times, choice = {}, {}
     for K in TILE_CANDIDATES:
         sim.tile_k = K
         sim._tiles = None
-        sim._plan = None
-        p = sim.plan()
-        if p is None or sim.scatter != "tiled":
-            sim.scatter = "tiled"
-            continue
-        mf, mn = ctypes.c_double(), ctypes.c_double()
-        _lib.call("nfx_time_passes", openCL.handle(), p, sim.time, nrep, ctypes.byref(mf), ctypes.byref(mn))
-        sim.time += nrep
-        times[K] = mf.value + mn.value
-        if log:
-            log(f"tile_k {K}: force {mf.value:.4f} ms, node {mn.value:.4f} ms")
+        for nt in STREAM_CANDIDATES:
+            sim.stream_nt = nt
+            sim._plan = None
+            p = sim.plan()
+            if p is None or sim.scatter != "tiled":
+                sim.scatter = "tiled"
+                break
+            mf, mn = ctypes.c_double(), ctypes.c_double()
+            _lib.call("nfx_time_passes", openCL.handle(), p, sim.time, nrep, ctypes.byref(mf), ctypes.byref(mn))
+            sim.time += nrep
+            lab = _label(K, nt)
+            times[lab], choice[lab] = mf.value + mn.value, (K, nt)
+            if log:
+                log(f"{lab}: force {mf.value:.4f} ms, node {mn.value:.4f} ms")
     openCL.getCommandQueue().finish()
     for t, s0 in zip(sim.state_tensors(), snapshot):
         t.copy_(s0)
     torch.cuda.synchronize()
     sim.forms_restored()
     sim.time = saved_time
-    best = min(times, key=times.get)
-    sim.tile_k = best
+    best = choice[min(times, key=times.get)]
+    sim.tile_k, sim.stream_nt = best
     sim._tiles = None
     sim._plan = None
     return best, times

@@ -4,8 +4,10 @@ together on one GPU.
 The reference simulates one cell per process (``Simulation.py``); this module runs a population
 of cells as one concatenated mesh.  Tiles of the tiled scatter are aligned to cells, so every cell
 sums its nodal forces in exactly the order it would use alone, and a cell's batched trajectory
-equals its single-cell run.  Per step: one curve-sampling pass (per-cell FP64 indenter force on
-sampling steps), one force pass over all tiles, one node pass over all nodes; steps replay from a
+equals its single-cell run up to rounding (the batch carries the damped velocity-Verlet state as
+(x, u = a v + f/2), the same recurrence without the previous force).  Per step: one force pass over
+all tiles and one node pass over all nodes, which on sampling steps also sums each cell's FP64
+indenter force (a separate curve pass when a cell has fewer than 256 nodes); steps replay from a
 captured hipGraph (``nfx_batch_run_steps``).
 
 Cell model (per cell, in that cell's simulation units, ``Simulation.py:42-56``): Neo-Hookean
@@ -29,6 +31,7 @@ import os
 import numpy as np
 
 from nofluidx3d_amd import _lib
+from nofluidx3d_amd.autotune import default_stream_nt
 
 E_SIM = 1e-2
 
@@ -216,14 +219,14 @@ def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
                   cnt.ctypes.data_as(vp), None, None, None, None, None)
         per.append((tt, n, nT, ntiles, cnt, m.value))
         MX = max(MX, mx.value)
-    if MX > K:
-        raise _lib.NfxError(f"a tile touches {MX} nodes > tile_k {K}; use a larger tile_k")
+    if MX > min(K, _lib.TILE_MAX_NODES):
+        raise _lib.NfxError(f"a tile touches {MX} nodes > min(tile_k {K}, {_lib.TILE_MAX_NODES}); use another tile_k")
     nt_tot = sum(p[3] for p in per)
     h = {
         "tile_count": np.zeros(nt_tot, np.uint32),
         "tile_nodes": np.zeros(nt_tot * MX, np.uint32),
         "tile_lend": np.zeros(nt_tot * MX, np.uint16),
-        "tile_slots": np.zeros(4 * K * nt_tot, np.uint16),
+        "tile_corners": np.zeros(4 * K * nt_tot, np.uint16),
         "tile_dst": np.zeros(nt_tot * MX, np.uint32),
         "tile_ntet": np.zeros(nt_tot, np.uint16),
         "tile_cell": np.zeros(nt_tot, np.uint32),
@@ -244,7 +247,7 @@ def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
         h["tile_count"][sl] = cnt
         h["tile_nodes"][tpos * MX:(tpos + ntiles) * MX] = tn + node_off[i]
         h["tile_lend"][tpos * MX:(tpos + ntiles) * MX] = tl
-        h["tile_slots"][4 * K * tpos:4 * K * (tpos + ntiles)] = ts
+        h["tile_corners"][4 * K * tpos:4 * K * (tpos + ntiles)] = ts
         h["tile_dst"][tpos * MX:(tpos + ntiles) * MX] = td + ppos
         h["tile_ntet"][sl] = np.minimum(K, nT - K * np.arange(ntiles))
         h["tile_cell"][sl] = i
@@ -265,7 +268,8 @@ def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
 class BatchSimulation:
     """All cells of ``specs`` on the current device.  ``meshes``: list of (points_um, tets)."""
 
-    def __init__(self, specs, meshes, steps=10000, ncurve=256, tile_k=256, use_graph=True, device_id=0):
+    def __init__(self, specs, meshes, steps=10000, ncurve=256, tile_k=256, use_graph=True, device_id=0,
+                 plastic_strain=False, stream_nt=None):
         import torch
 
         from nofluidx3d_amd import openCL
@@ -311,17 +315,21 @@ class BatchSimulation:
         if self.model == _lib.NFX_VISCOPLASTIC:
             # state in shape form S = Dm^-1 Fp^-1 (the tiled force pass's form); S = Dm^-1 at Fp = I
             self.shape = self.dminv.clone()
-            self.eps = torch.zeros(self.T, dtype=torch.float32, device=dev)
+            # the linear Maxwell law does not read the accumulated plastic strain: kept on request
+            self.eps = torch.zeros(self.T, dtype=torch.float32, device=dev) if plastic_strain else None
         else:
             self.shape = self.eps = None
         self.t_count, self.t_nodes, self.t_lend = u32(h["tile_count"]), u32(h["tile_nodes"]), u16(h["tile_lend"])
-        self.t_slots, self.t_dst = u16(h["tile_slots"]), u32(h["tile_dst"])
+        self.t_corners, self.t_dst = u16(h["tile_corners"]), u32(h["tile_dst"])
         self.t_ntet, self.t_cell = u16(h["tile_ntet"]), u32(h["tile_cell"])
         self.node_poff, self.node_cell = u32(h["node_poff"]), u32(h["node_cell"])
         self.cell_node_off = u32(self.node_off)
         self.partials = torch.zeros(3 * h["m_total"], dtype=torch.float32, device=dev)
         self.cells = torch.from_numpy(self.cells_host.view(np.uint8).copy()).to(dev)
         self.curves = torch.zeros(nc * ncurve, dtype=torch.float64, device=dev)
+        # scratch of the curve sampling inside the node pass (blocks of 256 nodes)
+        self.curve_partials = torch.zeros(2 * ((self.N + 255) // 256), dtype=torch.float64, device=dev)
+        self.curve_counter = torch.zeros(1, dtype=torch.int32, device=dev)
         torch.cuda.synchronize(dev)
 
         p = _lib.Plan()
@@ -331,11 +339,14 @@ class BatchSimulation:
         p.force = 0
         p.model = self.model
         if self.shape is not None:
-            p.vp_shape, p.epsbar = self.shape.data_ptr(), self.eps.data_ptr()
+            p.vp_shape = self.shape.data_ptr()
+            p.epsbar = self.eps.data_ptr() if self.eps is not None else 0
+            p.vp.flow_exponent, p.vp.hardening_exponent = 1.0, 0.0  # every cell: linear Maxwell flow
+        p.stream_nt = default_stream_nt(self.T) if stream_nt is None else int(stream_nt)
         p.scatter = _lib.SCATTER_TILED
         p.tile_k, p.tile_max_nodes = K, MX
         p.tile_count, p.tile_nodes, p.tile_lend = (t.data_ptr() for t in (self.t_count, self.t_nodes, self.t_lend))
-        p.tile_slots, p.tile_dst = self.t_slots.data_ptr(), self.t_dst.data_ptr()
+        p.tile_corners, p.tile_dst = self.t_corners.data_ptr(), self.t_dst.data_ptr()
         p.node_off, p.partials = self.node_poff.data_ptr(), self.partials.data_ptr()
         p.substrate, p.indenter = 1, _lib.INDENTER_SPHERE
         self.plan = p
@@ -345,6 +356,8 @@ class BatchSimulation:
         b.node_cell, b.cell_node_off, b.cells = self.node_cell.data_ptr(), self.cell_node_off.data_ptr(), \
             self.cells.data_ptr()
         b.curves, b.ncurve, b.curve_steps = self.curves.data_ptr(), ncurve, steps
+        b.min_cell_nodes = int(np.min(np.diff(np.asarray(self.node_off, dtype=np.int64))))
+        b.curve_partials, b.curve_counter = self.curve_partials.data_ptr(), self.curve_counter.data_ptr()
         self.batch = b
 
     def reset(self, specs):

@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace nfx {
 
 // Hardware transcendental/division approximations (v_rcp/v_sqrt/v_exp/v_log, about 1 ulp, no
@@ -23,6 +25,56 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
+
+// Streaming accesses: data that a pass touches once (per-tetra state, tile tables) may take the
+// nontemporal cache policy, so that they do not evict the node positions that neighbouring tiles
+// gather again.  That pays when the state exceeds the 256 MB memory-side cache; a state that fits
+// is better served by the default policy, which keeps it there from step to step.
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+template <class T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+// Element i of an array whose base address every lane of the wave holds (a workgroup's tile or
+// state plane).  The base is made explicitly wave-uniform and the lane offset is a 32-bit byte
+// offset, so the access takes the scalar-base + vector-offset form with no per-lane 64-bit address
+// arithmetic (the compiler otherwise derives each of a tetra's nine state planes from one 64-bit
+// per-lane address).  Only for bases that are the same on every lane.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> uat(T* base, uint32_t i) {
+    const uint64_t a = reinterpret_cast<uint64_t>(base);
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+    using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    const gptr<B> b = (gptr<B>)((hi << 32) | lo);
+    return (gptr<T>)(b + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <bool NT, class T>
+__device__ __forceinline__ T ld_g(gptr<const T> p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st_g(gptr<T> p, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+// the same, with the policy a compile-time choice of the caller
+template <bool NT, class T>
+__device__ __forceinline__ T ld_s(const T* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, class T>
+__device__ __forceinline__ void st_s(T* p, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 struct V3 {
     float x, y, z;
@@ -473,10 +525,18 @@ __device__ __forceinline__ float plane_term(float y, float k, float wall) { retu
 struct VV {
     static constexpr float a = 1.0f - 0.5f * 0.005f;
     static constexpr float inv_b = 1.0f / (1.0f + 0.5f * 0.005f);
+    static constexpr float a_over_b = a * inv_b;
 };
 __device__ __forceinline__ void vv_update(V3& x, V3& v, V3 f, V3 fo) {
     v = (v * VV::a + (f + fo) * 0.5f) * VV::inv_b;
     x = x + v * VV::a + f * 0.5f;
+}
+// The same recurrence on (x, u), u = a v + f/2 of the previous step: v' = (u + f'/2)/b, so
+// x' = x + a v' + f'/2 = x + u' with u' = (a/b)(u + f'/2) + f'/2.  No previous force is kept.
+__device__ __forceinline__ void vv_update_u(V3& x, V3& u, V3 f) {
+    const V3 h = f * 0.5f;
+    u = (u + h) * VV::a_over_b + h;
+    x = x + u;
 }
 
 // Simulation.py:173-183 / 117-132, in double

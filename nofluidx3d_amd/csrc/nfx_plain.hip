@@ -236,15 +236,16 @@ __global__ __launch_bounds__(kBlock) void k_vp_convert(const float* __restrict__
 
 // ---- force pass, tiled deterministic scatter ---------------------------------------------------
 // One workgroup per tile of K consecutive (Morton-ordered) tetra, one thread per tetra.  The
-// tile's distinct nodes are listed on the host (tile_nodes, ascending, MX entries per tile) with,
-// for each, the run of the tile's corner slots that touch it (tile_slots grouped by node; tile_lend
-// = run ends) and the position of its partial sum in node order (tile_dst).  Phases:
+// tile's distinct nodes are listed on the host (tile_nodes, ascending, MX <= 256 entries per tile)
+// with, for each, the end of its run in the tile's corner list sorted by (node, slot) (tile_lend)
+// and the position of its partial sum in node order (tile_dst).  Each corner slot 4*(t-t0)+a
+// carries its tile-local node and its rank in that node's run (tile_corners), so its place in the
+// sorted list is run start + rank.  Phases:
 //   0  issue the tetra's streaming loads (Dm^-1 and V0, or the viscoplastic shape state and
-//      epsbar) and the tile metadata
-//   1  slot list and node positions -> LDS
-//   2  invert the slot runs into per-corner local node ids (conn)
-//   3  material law on LDS positions, corner forces -> LDS
-//   4  one thread per tile-node: sum its slot run in list order -> partials[tile_dst]
+//      epsbar), its corner records and the tile metadata
+//   1  node positions and run starts -> LDS
+//   2  material law on LDS positions; each corner force -> LDS at its place in the sorted list
+//   3  one thread per tile-node: sum its contiguous run -> partials[tile_dst]
 // The node pass sums each node's contiguous partials in tile order: no atomics, fixed order.
 struct TileArgs {
     int64_t N, T;
@@ -254,7 +255,7 @@ struct TileArgs {
     const uint32_t* __restrict__ tile_count;
     const uint32_t* __restrict__ tile_nodes;
     const uint16_t* __restrict__ tile_lend;
-    const uint16_t* __restrict__ tile_slots;
+    const uint16_t* __restrict__ tile_corners;
     const uint32_t* __restrict__ tile_dst;
     float* __restrict__ partials;
     int32_t max_nodes;
@@ -264,22 +265,26 @@ struct TileArgs {
     const nfx_cell* __restrict__ cells;
 };
 
+// LDS of a tile: corner forces 12K floats, then per tile-node its position (3 floats) and run
+// start (u16)
 template <int K>
 __host__ __device__ constexpr size_t tile_lds_bytes_fixed() {
-    return 2 * 4 * K + 2 * 4 * K + 4 * 12 * K;
+    return 4 * 12 * K;
 }
+__host__ __device__ constexpr size_t tile_lds_bytes_per_node() { return 12 + 2; }
 
-template <int MODEL, int K, bool STRESS>
+// NT: cache policy of the streams; EPS: the viscoplastic law keeps its accumulated plastic strain
+// (the linear law does not read it)
+template <int MODEL, int K, bool STRESS, bool NT = false, bool EPS = true>
 __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
     // one thread per tetra of the tile (blockDim.x == K)
     extern __shared__ __align__(16) unsigned char smem[];
-    uint16_t* s_slots = reinterpret_cast<uint16_t*>(smem);
-    uint16_t* s_conn = s_slots + 4 * K;
-    float* s_cf = reinterpret_cast<float*>(s_conn + 4 * K);
-    float* s_x = s_cf + 12 * K;
+    float* s_cf = reinterpret_cast<float*>(smem);  // [component][4K], sorted-corner order
     const int MX = a.max_nodes;
+    float* s_x = s_cf + 12 * K;                    // [component][MX]
+    uint16_t* s_lb = reinterpret_cast<uint16_t*>(s_x + 3 * MX);
 
     const int64_t c = xcd_block(blockIdx.x, gridDim.x);
     const int64_t T = a.T, N = a.N;
@@ -309,12 +314,12 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     if (active) {
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) D[k] = (Sst + t0 + k * T)[u];
-            e0 = (eps + t0)[u];
+            for (int k = 0; k < 9; ++k) D[k] = ld_g<NT, float>(uat((const float*)(Sst + t0 + k * T), u));
+            if constexpr (EPS) e0 = ld_g<NT, float>(uat((const float*)(eps + t0), u));
         } else {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) D[k] = (a.dminv + t0 + k * T)[u];
-            if constexpr (MODEL < NFX_VISCOPLASTIC) v0 = (a.v0 + t0)[u];
+            for (int k = 0; k < 9; ++k) D[k] = ld_g<NT, float>(uat(a.dminv + t0 + k * T, u));
+            if constexpr (MODEL < NFX_VISCOPLASTIC) v0 = ld_g<NT, float>(uat(a.v0 + t0, u));
         }
     }
 
@@ -325,30 +330,24 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     uint32_t nid = 0, dst = 0;
     int lb = 0, le = 0;
     if (owner) {
-        nid = a.tile_nodes[j0 + tl];
-        lb = tl ? a.tile_lend[j0 + tl - 1] : 0;
-        le = a.tile_lend[j0 + tl];
-        dst = a.tile_dst[j0 + tl];
+        nid = ld_g<NT, uint32_t>(uat(a.tile_nodes + j0, u));
+        lb = tl ? ld_g<NT, uint16_t>(uat(a.tile_lend + j0, u - 1)) : 0;
+        le = ld_g<NT, uint16_t>(uat(a.tile_lend + j0, u));
+        dst = ld_g<NT, uint32_t>(uat(a.tile_dst + j0, u));
     }
-    if (active) {
-        // the tile's 4 slots of this tetra position, as one 8-byte load
-        reinterpret_cast<uint2*>(s_slots)[tl] = reinterpret_cast<const uint2*>(a.tile_slots + 4 * K * c)[tl];
-    }
+    // this tetra's 4 corner records (tile-node | rank << 8), as one 8-byte load
+    uint64_t cr = 0;
+    if (active) cr = ld_g<NT, uint64_t>(uat(reinterpret_cast<const uint64_t*>(a.tile_corners + 4 * K * c), u));
     if (owner) {
         s_x[tl] = a.x[nid];
         s_x[MX + tl] = a.x[N + nid];
         s_x[2 * MX + tl] = a.x[2 * N + nid];
-    }
-    __syncthreads();
-    if (tl < M) {
-        for (int q = lb; q < le; ++q) {
-            const int sl = s_slots[q];
-            s_conn[(sl & 3) * K + (sl >> 2)] = (uint16_t)tl;
-        }
+        s_lb[tl] = (uint16_t)lb;
     }
     __syncthreads();
     if (active) {
-        const int l0 = s_conn[tl], l1 = s_conn[K + tl], l2 = s_conn[2 * K + tl], l3 = s_conn[3 * K + tl];
+        const int l0 = (int)(cr & 0xFF), l1 = (int)((cr >> 16) & 0xFF), l2 = (int)((cr >> 32) & 0xFF),
+                  l3 = (int)((cr >> 48) & 0xFF);
         const V3 p0 = mk(s_x[l0], s_x[MX + l0], s_x[2 * MX + l0]);
         const V3 p1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
         const V3 p2 = mk(s_x[l2], s_x[MX + l2], s_x[2 * MX + l2]);
@@ -357,19 +356,19 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
             const VPOut o = vp_tetra_shape(p0, p1, p2, p3, D, e0, P);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) (Sst + t0 + k * T)[u] = o.Fp[k];
-            (eps + t0)[u] = o.epsbar;
+            for (int k = 0; k < 9; ++k) st_g<NT, float>(uat(Sst + t0 + k * T, u), o.Fp[k]);
+            if constexpr (EPS) st_g<NT, float>(uat(eps + t0, u), o.epsbar);
             if constexpr (STRESS) {
-                (vm + t0)[u] = o.von_mises;
-                (pr + t0)[u] = o.pressure;
+                st_s<NT>(vm + t0 + u, o.von_mises);
+                st_s<NT>(pr + t0 + u, o.pressure);
             }
             f = o.f;
         } else if constexpr (MODEL >= NFX_LINEAR_ELASTIC) {
             float svm, spr;
             f = legacy_tetra<MODEL>(p0, p1, p2, p3, D, e.L, svm, spr);
             if constexpr (STRESS) {
-                (vm + t0)[u] = svm;
-                (pr + t0)[u] = spr;
+                st_s<NT>(vm + t0 + u, svm);
+                st_s<NT>(pr + t0 + u, spr);
             }
         } else if constexpr (MODEL == NFX_MR) {
             f = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
@@ -378,20 +377,21 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
         } else {
             f = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
         }
-        s_cf[0 * K + tl] = f.f0.x; s_cf[1 * K + tl] = f.f0.y; s_cf[2 * K + tl] = f.f0.z;
-        s_cf[3 * K + tl] = f.f1.x; s_cf[4 * K + tl] = f.f1.y; s_cf[5 * K + tl] = f.f1.z;
-        s_cf[6 * K + tl] = f.f2.x; s_cf[7 * K + tl] = f.f2.y; s_cf[8 * K + tl] = f.f2.z;
-        s_cf[9 * K + tl] = f.f3.x; s_cf[10 * K + tl] = f.f3.y; s_cf[11 * K + tl] = f.f3.z;
+        // each corner's place in the tile's (node, slot)-sorted corner list: run start + rank
+        const int q0 = s_lb[l0] + (int)((cr >> 8) & 0xFF), q1 = s_lb[l1] + (int)((cr >> 24) & 0xFF);
+        const int q2 = s_lb[l2] + (int)((cr >> 40) & 0xFF), q3 = s_lb[l3] + (int)((cr >> 56) & 0xFF);
+        s_cf[q0] = f.f0.x; s_cf[4 * K + q0] = f.f0.y; s_cf[8 * K + q0] = f.f0.z;
+        s_cf[q1] = f.f1.x; s_cf[4 * K + q1] = f.f1.y; s_cf[8 * K + q1] = f.f1.z;
+        s_cf[q2] = f.f2.x; s_cf[4 * K + q2] = f.f2.y; s_cf[8 * K + q2] = f.f2.z;
+        s_cf[q3] = f.f3.x; s_cf[4 * K + q3] = f.f3.y; s_cf[8 * K + q3] = f.f3.z;
     }
     __syncthreads();
     if (tl < M) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (int q = lb; q < le; ++q) {
-            const int sl = s_slots[q];
-            const int base = 3 * (sl & 3) * K + (sl >> 2);
-            fx += s_cf[base];
-            fy += s_cf[base + K];
-            fz += s_cf[base + 2 * K];
+        for (int q = lb; q < le; ++q) {  // the run in slot order
+            fx += s_cf[q];
+            fy += s_cf[4 * K + q];
+            fz += s_cf[8 * K + q];
         }
         float* o = a.partials + 3 * (int64_t)dst;
         o[0] = fx;
@@ -638,10 +638,14 @@ __global__ __launch_bounds__(kBlock) void k_tip(const float* __restrict__ x, int
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
-// STREAM-style copy (16-byte vector loads/stores, grid-stride): the measured HBM ceiling
-__global__ __launch_bounds__(kBlock) void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b,
+// STREAM-style copy, the measured HBM ceiling: one 16-byte element per thread, nontemporal loads
+// and stores (the fastest of the copy forms measured on MI355X: grid-stride loops and unrolled
+// forms were slower)
+typedef float nfx_f4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const nfx_f4* __restrict__ a, nfx_f4* __restrict__ b,
                                                         int64_t n4) {
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += (int64_t)gridDim.x * kBlock) b[i] = a[i];
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < n4) st_nt(b + i, ld_nt(a + i));
 }
 
 // ---- host helpers -----------------------------------------------------------------------------
@@ -687,7 +691,9 @@ static int validate_plan(const nfx_plan* p) {
     NFX_REQUIRE(p->tetras && p->dminv && p->points && p->vel && p->fold, "plan: NULL mesh/state pointer");
     NFX_REQUIRE(p->model >= NFX_SVKK && p->model <= NFX_PORO_NEO_HOOKEAN, "plan: unknown model %d", p->model);
     NFX_REQUIRE(p->model > NFX_MR || p->v0, "plan: NULL v0");
-    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->epsbar, "plan: viscoplastic state missing");
+    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->epsbar ||
+                    (p->scatter == NFX_SCATTER_TILED && p->vp.flow_exponent == 1.0 && p->vp.hardening_exponent == 0.0),
+                "plan: viscoplastic state epsbar missing (it may be NULL only for the linear law on the tiled path)");
     NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->scatter == NFX_SCATTER_TILED || p->Fp,
                 "plan: viscoplastic state Fp missing");
     NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->scatter != NFX_SCATTER_TILED || p->vp_shape,
@@ -696,13 +702,13 @@ static int validate_plan(const nfx_plan* p) {
                 p->scatter);
     if (p->scatter == NFX_SCATTER_TILED) {
         NFX_REQUIRE(p->tile_k == 256 || p->tile_k == 512, "plan: tile_k must be 256 or 512");
-        NFX_REQUIRE(p->tile_count && p->tile_nodes && p->tile_lend && p->tile_slots && p->tile_dst && p->node_off &&
+        NFX_REQUIRE(p->tile_count && p->tile_nodes && p->tile_lend && p->tile_corners && p->tile_dst && p->node_off &&
                         p->partials,
                     "plan: tiled scatter needs the tile arrays");
-        NFX_REQUIRE(p->tile_max_nodes > 0 && p->tile_max_nodes <= p->tile_k,
-                    "plan: tile_max_nodes %d exceeds tile_k %d (use a larger tile or another scatter mode)",
+        NFX_REQUIRE(p->tile_max_nodes > 0 && p->tile_max_nodes <= p->tile_k && p->tile_max_nodes <= 256,
+                    "plan: tile_max_nodes %d exceeds tile_k %d or 256 (use another tile size or scatter mode)",
                     p->tile_max_nodes, p->tile_k);
-        const size_t lds = (size_t)p->tile_k * 64 + 12 * (size_t)p->tile_max_nodes;
+        const size_t lds = (size_t)p->tile_k * 48 + tile_lds_bytes_per_node() * (size_t)p->tile_max_nodes;
         NFX_REQUIRE(lds <= 65536, "plan: tile needs %zu B of LDS (max 65536); use a smaller tile_k", lds);
     }
     NFX_REQUIRE(p->scatter != NFX_SCATTER_ATOMIC || p->force, "plan: atomic scatter needs force");
@@ -721,11 +727,22 @@ static void launch_tile_legacy(unsigned int nb, size_t lds, hipStream_t s, const
         k_tile_force<MODEL, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
 }
 
+template <int K, bool STRESS, bool NT>
+static void launch_tile_vp(unsigned int nb, size_t lds, hipStream_t s, const TileArgs& a, const ElasticDev& e,
+                           const VPConst& c, const nfx_plan& p) {
+    float* vm = STRESS ? p.von_mises : nullptr;
+    float* pr = STRESS ? p.pressure : nullptr;
+    if (p.epsbar)
+        k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar, vm, pr);
+    else
+        k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, nullptr, vm, pr);
+}
+
 template <int K>
 static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     // ctx->batch is set only while a batched step is being enqueued
     TileArgs a{p.N,         p.T,          p.dminv,     p.v0,     p.points,         p.tile_count,
-               p.tile_nodes, p.tile_lend,  p.tile_slots, p.tile_dst, p.partials,     p.tile_max_nodes,
+               p.tile_nodes, p.tile_lend,  p.tile_corners, p.tile_dst, p.partials,   p.tile_max_nodes,
                nullptr,     nullptr,      nullptr};
     if (ctx->batch) {
         a.tile_ntet = ctx->batch->tile_ntet;
@@ -733,14 +750,18 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
         a.cells = ctx->batch->cells;
     }
     const unsigned int nb = (unsigned int)((p.T + K - 1) / K);
-    const size_t lds = tile_lds_bytes_fixed<K>() + 12 * (size_t)p.tile_max_nodes;
+    const size_t lds = tile_lds_bytes_fixed<K>() + tile_lds_bytes_per_node() * (size_t)p.tile_max_nodes;
     const ElasticDev e = elastic_consts(p);
     const VPConst c = vp_consts(p.vp);
     hipStream_t s = ctx->stream;
     float* vm = st ? p.von_mises : nullptr;
     float* pr = st ? p.pressure : nullptr;
+    const bool nt = p.stream_nt != 0;
     switch (p.model) {
-        case NFX_MR: k_tile_force<NFX_MR, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_MR:
+            if (nt) k_tile_force<NFX_MR, K, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
+            else k_tile_force<NFX_MR, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
+            break;
         case NFX_SVK: k_tile_force<NFX_SVK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
         case NFX_SVKK: k_tile_force<NFX_SVKK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
         case NFX_LINEAR_ELASTIC: launch_tile_legacy<NFX_LINEAR_ELASTIC, K>(nb, lds, s, a, e, c, vm, pr); break;
@@ -748,12 +769,10 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
         case NFX_NEO_HOOKEAN_2: launch_tile_legacy<NFX_NEO_HOOKEAN_2, K>(nb, lds, s, a, e, c, vm, pr); break;
         case NFX_PORO_NEO_HOOKEAN: launch_tile_legacy<NFX_PORO_NEO_HOOKEAN, K>(nb, lds, s, a, e, c, vm, pr); break;
         default:
-            if (st)
-                k_tile_force<NFX_VISCOPLASTIC, K, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar, p.von_mises,
-                                                                           p.pressure);
-            else
-                k_tile_force<NFX_VISCOPLASTIC, K, false><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar, nullptr,
-                                                                            nullptr);
+            if (st && nt) launch_tile_vp<K, true, true>(nb, lds, s, a, e, c, p);
+            else if (st) launch_tile_vp<K, true, false>(nb, lds, s, a, e, c, p);
+            else if (nt) launch_tile_vp<K, false, true>(nb, lds, s, a, e, c, p);
+            else launch_tile_vp<K, false, false>(nb, lds, s, a, e, c, p);
     }
 }
 
@@ -853,45 +872,115 @@ static int launch_node_pass(nfx_ctx* ctx, const nfx_plan& p, int64_t step, const
 struct BatchNodeArgs {
     int64_t N;
     float* x;
-    float* v;
-    float* fold;
+    float* u;  // velocity-Verlet state u = a v + f/2
     const uint32_t* __restrict__ node_off;
     const float* __restrict__ partials;
     const uint32_t* __restrict__ node_cell;
     const nfx_cell* __restrict__ cells;
+    // curve sampling inside the node pass (CURVE): per-block partial sums, finished by the last block
+    const uint32_t* __restrict__ cell_node_off;
+    int32_t ncells, ncurve;
+    int64_t curve_steps;
+    double* curves;
+    double* curve_partials;
+    unsigned int* curve_counter;
 };
 
+// On a sampling step, the block's indenter forces summed per cell (a block of kBlock nodes spans
+// at most two cells: slot 0 is the cell of its first node, slot 1 the next one); the last block
+// to finish sums every cell's block partials in ascending block order into the curve sample.
+__device__ void batch_curve_block(const BatchNodeArgs& a, double s, bool second, int64_t k) {
+    __shared__ double w0[kBlock / 64], w1[kBlock / 64];
+    __shared__ bool last;
+    double s0 = second ? 0.0 : s, s1 = second ? s : 0.0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        w0[wid] = s0;
+        w1[wid] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int q = 0; q < kBlock / 64; ++q) {
+            t0 += w0[q];
+            t1 += w1[q];
+        }
+        a.curve_partials[2 * blockIdx.x] = t0;
+        a.curve_partials[2 * blockIdx.x + 1] = t1;
+        __threadfence();
+        last = atomicAdd(a.curve_counter, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int c = threadIdx.x; c < a.ncells; c += kBlock) {
+        const int64_t b0 = a.cell_node_off[c] / kBlock, b1 = (a.cell_node_off[c + 1] - 1) / kBlock;
+        double t = 0.0;
+        for (int64_t b = b0; b <= b1; ++b) {
+            const int slot = a.node_cell[b * kBlock] == (uint32_t)c ? 0 : 1;
+            t += __hip_atomic_load(&a.curve_partials[2 * b + slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        a.curves[(int64_t)c * a.ncurve + k] = t;
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(a.curve_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Node pass of a batch: each node applies its own cell's substrate and indenter (position from
-// the step index with the cell's law), then the damped velocity-Verlet update.
-template <bool SUB, int IND>
+// the step index with the cell's law), then the damped velocity-Verlet update in the (x, u) form.
+// CURVE: on a sampling step the indenter forces at the positions before the update (the step's
+// start) are also summed per cell into the curves.
+template <bool SUB, int IND, bool CURVE>
 __global__ __launch_bounds__(kBlock) void k_batch_node(BatchNodeArgs a, int64_t step, const int64_t* step_dev,
                                                        int32_t step_off) {
     const int64_t time = step_dev ? (*step_dev + step_off) : step;
     const int64_t N = a.N;
     const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (n >= N) return;
-    V3 f = mk(0.f, 0.f, 0.f);
-    const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
-    for (uint32_t j = b; j < e; ++j) {
-        const float* p = a.partials + 3 * (int64_t)j;
-        f = f + mk(p[0], p[1], p[2]);
+    bool sample = false;
+    int64_t k = 0;
+    if constexpr (CURVE) {
+        if (time >= 0 && time < a.curve_steps) {
+            k = time * a.ncurve / a.curve_steps;
+            sample = time == 0 || (time - 1) * a.ncurve / a.curve_steps != k;
+        }
     }
-    const nfx_cell* cp = a.cells + a.node_cell[n];
-    V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
-    if constexpr (SUB) f.y += substrate_term(x.y, cp->substrate_k, cp->wall);
-    if constexpr (IND == NFX_INDENTER_SPHERE) {
-        const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
-        f = f + sphere_term(x, cp->tip_k, cp->tip_radius, pos);
-    } else if constexpr (IND == NFX_INDENTER_PLANE) {
-        const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
-        f.y += plane_term(x.y, cp->tip_k, pos);
+    double tip = 0.0;
+    bool second = false;
+    if (n < N) {
+        V3 f = mk(0.f, 0.f, 0.f);
+        const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
+        for (uint32_t j = b; j < e; ++j) {
+            const float* p = a.partials + 3 * (int64_t)j;
+            f = f + mk(p[0], p[1], p[2]);
+        }
+        const uint32_t cell = a.node_cell[n];
+        const nfx_cell* cp = a.cells + cell;
+        V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
+        if constexpr (SUB) f.y += substrate_term(x.y, cp->substrate_k, cp->wall);
+        if constexpr (IND == NFX_INDENTER_SPHERE) {
+            const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
+            const V3 fs = sphere_term(x, cp->tip_k, cp->tip_radius, pos);
+            f = f + fs;
+            tip = -(double)fs.y;
+        } else if constexpr (IND == NFX_INDENTER_PLANE) {
+            const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
+            const float fp = plane_term(x.y, cp->tip_k, pos);
+            f.y += fp;
+            tip = -(double)fp;
+        }
+        if constexpr (CURVE) second = cell != a.node_cell[(int64_t)blockIdx.x * kBlock];
+        V3 u = mk(a.u[n], a.u[N + n], a.u[2 * N + n]);
+        vv_update_u(x, u, f);
+        a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
+        a.u[n] = u.x; a.u[N + n] = u.y; a.u[2 * N + n] = u.z;
     }
-    V3 v = mk(a.v[n], a.v[N + n], a.v[2 * N + n]);
-    const V3 fo = mk(a.fold[n], a.fold[N + n], a.fold[2 * N + n]);
-    vv_update(x, v, f, fo);
-    a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
-    a.v[n] = v.x; a.v[N + n] = v.y; a.v[2 * N + n] = v.z;
-    a.fold[n] = f.x; a.fold[N + n] = f.y; a.fold[2 * N + n] = f.z;
+    if constexpr (CURVE) {
+        if (sample) batch_curve_block(a, tip, second, k);  // uniform over the grid
+    }
 }
 
 // One workgroup per cell: on a sampling step, the cell's indenter force (Simulation.py:198-212 in
@@ -943,17 +1032,31 @@ static int validate_batch(const nfx_plan* p, const nfx_batch* b) {
 }
 
 template <bool SUB>
-static void launch_batch_node_sub(nfx_ctx* ctx, const nfx_plan& p, const BatchNodeArgs& a, int64_t step,
+static void launch_batch_node_sub(nfx_ctx* ctx, const nfx_plan& p, const BatchNodeArgs& a, bool curve, int64_t step,
                                   const int64_t* sd, int32_t off) {
     const unsigned int nb = nblocks(p.N);
-    if (p.indenter == NFX_INDENTER_SPHERE) k_batch_node<SUB, 1><<<nb, kBlock, 0, ctx->stream>>>(a, step, sd, off);
-    else if (p.indenter == NFX_INDENTER_PLANE) k_batch_node<SUB, 2><<<nb, kBlock, 0, ctx->stream>>>(a, step, sd, off);
-    else k_batch_node<SUB, 0><<<nb, kBlock, 0, ctx->stream>>>(a, step, sd, off);
+    hipStream_t s = ctx->stream;
+    if (p.indenter == NFX_INDENTER_SPHERE) {
+        if (curve) k_batch_node<SUB, 1, true><<<nb, kBlock, 0, s>>>(a, step, sd, off);
+        else k_batch_node<SUB, 1, false><<<nb, kBlock, 0, s>>>(a, step, sd, off);
+    } else if (p.indenter == NFX_INDENTER_PLANE) {
+        if (curve) k_batch_node<SUB, 2, true><<<nb, kBlock, 0, s>>>(a, step, sd, off);
+        else k_batch_node<SUB, 2, false><<<nb, kBlock, 0, s>>>(a, step, sd, off);
+    } else {
+        k_batch_node<SUB, 0, false><<<nb, kBlock, 0, s>>>(a, step, sd, off);
+    }
+}
+
+// the node pass samples the curves itself (no launch of the curve kernel)
+static bool curves_in_node_pass(const nfx_plan& p, const nfx_batch& b) {
+    return b.curves && p.indenter != NFX_INDENTER_NONE && b.min_cell_nodes >= kBlock && b.curve_partials &&
+           b.curve_counter;
 }
 
 static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b, int64_t step, const int64_t* sd,
                              int32_t off, int which = 7) {
-    if ((which & 1) && b.curves) {
+    const bool fused_curves = curves_in_node_pass(p, b);
+    if ((which & 1) && b.curves && !fused_curves) {
         if (p.indenter == NFX_INDENTER_SPHERE)
             k_batch_curves<1><<<b.ncells, kBlock, 0, ctx->stream>>>(p.points, p.N, b.cell_node_off, b.cells, b.curves,
                                                                     b.ncurve, b.curve_steps, step, sd, off);
@@ -969,9 +1072,11 @@ static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b
         if (rc) return rc;
     }
     if (which & 4) {
-        BatchNodeArgs a{p.N, p.points, p.vel, p.fold, p.node_off, p.partials, b.node_cell, b.cells};
-        if (p.substrate) launch_batch_node_sub<true>(ctx, p, a, step, sd, off);
-        else launch_batch_node_sub<false>(ctx, p, a, step, sd, off);
+        BatchNodeArgs a{p.N,        p.points,        p.vel,     p.node_off,          p.partials,
+                        b.node_cell, b.cells,        b.cell_node_off, b.ncells,      b.ncurve,
+                        b.curve_steps, b.curves,     b.curve_partials, b.curve_counter};
+        if (p.substrate) launch_batch_node_sub<true>(ctx, p, a, fused_curves, step, sd, off);
+        else launch_batch_node_sub<false>(ctx, p, a, fused_curves, step, sd, off);
         NFX_CHECK_LAUNCH();
     }
     return NFX_OK;
@@ -1278,14 +1383,16 @@ int nfx_build_node_csr_host(const uint32_t* tet, int64_t N, int64_t T, uint32_t*
 }
 
 int nfx_build_tiles_host(const uint32_t* tet, int64_t N, int64_t T, int32_t K, int32_t* max_nodes, int64_t* m_total,
-                         uint32_t* tile_count, uint32_t* tile_nodes, uint16_t* tile_lend, uint16_t* tile_slots,
+                         uint32_t* tile_count, uint32_t* tile_nodes, uint16_t* tile_lend, uint16_t* tile_corners,
                          uint32_t* tile_dst, uint32_t* node_off) {
     NFX_REQUIRE(tet && max_nodes && m_total && tile_count, "NULL argument");
     NFX_REQUIRE(K == 256 || K == 512, "tile_k must be 256 or 512");
     int rc = check_sizes(N, T);
     if (rc) return rc;
     const bool fill = tile_nodes != nullptr;
-    NFX_REQUIRE(!fill || (tile_lend && tile_slots && tile_dst && node_off), "NULL output array");
+    NFX_REQUIRE(!fill || (tile_lend && tile_corners && tile_dst && node_off), "NULL output array");
+    NFX_REQUIRE(!fill || *max_nodes <= 256, "max_nodes %d > 256: a tile-node index must fit 8 bits (use tile_k 256)",
+                *max_nodes);
     const int64_t ntiles = (T + K - 1) / K;
     std::vector<uint64_t> key((size_t)4 * K);
     const int32_t MX = fill ? *max_nodes : 0;
@@ -1303,7 +1410,9 @@ int nfx_build_tiles_host(const uint32_t* tet, int64_t N, int64_t T, int32_t K, i
                 key[(size_t)(4 * tl + a)] = ((uint64_t)n << 12) | (uint64_t)(4 * tl + a);
             }
         std::sort(key.begin(), key.begin() + 4 * nt);
-        int M = 0;
+        int M = 0, run0 = 0;
+        if (fill)
+            for (int q = 4 * nt; q < 4 * K; ++q) tile_corners[(size_t)4 * K * c + q] = 0;  // padding slots
         for (int q = 0; q < 4 * nt; ++q) {
             const uint32_t n = (uint32_t)(key[(size_t)q] >> 12);
             const bool first = q == 0 || (uint32_t)(key[(size_t)q - 1] >> 12) != n;
@@ -1314,14 +1423,17 @@ int nfx_build_tiles_host(const uint32_t* tet, int64_t N, int64_t T, int32_t K, i
                     cnt[n + 1]++;
                 }
                 ++M;
+                run0 = q;
             }
             if (fill) {
-                tile_slots[(size_t)4 * K * c + q] = (uint16_t)(key[(size_t)q] & 0xFFF);
+                const int rank = q - run0;
+                NFX_REQUIRE(rank < 256, "tile %lld: node %u has more than 256 corners (use tile_k 256)", (long long)c,
+                            n);
+                tile_corners[(size_t)4 * K * c + (key[(size_t)q] & 0xFFF)] = (uint16_t)((rank << 8) | (M - 1));
                 tile_lend[c * MX + M - 1] = (uint16_t)(q + 1);
             }
         }
         if (fill) {
-            for (int q = 4 * nt; q < 4 * K; ++q) tile_slots[(size_t)4 * K * c + q] = 0;
             for (int i = M; i < MX; ++i) {  // padding: a valid node id, an empty run, never read
                 tile_nodes[c * MX + i] = tile_nodes[c * MX];
                 tile_lend[c * MX + i] = (uint16_t)(4 * nt);
@@ -1535,11 +1647,12 @@ int nfx_batch_time_passes(nfx_ctx* c, const nfx_plan* p, const nfx_batch* b, int
 int nfx_stream_copy(nfx_ctx* c, const void* src, void* dst, int64_t bytes, int32_t nrep, double* ms) {
     NFX_REQUIRE(c && src && dst && ms && nrep > 0 && bytes > 0 && bytes % 16 == 0, "bad argument");
     const int64_t n4 = bytes / 16;
-    const unsigned int nb = (unsigned int)std::min<int64_t>((n4 + kBlock - 1) / kBlock, (int64_t)c->num_cu * 16);
-    k_stream_copy<<<nb, kBlock, 0, c->stream>>>((const float4*)src, (float4*)dst, n4);
+    NFX_REQUIRE((n4 + kBlock - 1) / kBlock < (int64_t(1) << 31), "copy too large");
+    const unsigned int nb = (unsigned int)((n4 + kBlock - 1) / kBlock);
+    k_stream_copy<<<nb, kBlock, 0, c->stream>>>((const nfx_f4*)src, (nfx_f4*)dst, n4);
     NFX_CHECK_LAUNCH();
     NFX_HIP(hipEventRecord(c->ev0, c->stream));
-    for (int i = 0; i < nrep; ++i) k_stream_copy<<<nb, kBlock, 0, c->stream>>>((const float4*)src, (float4*)dst, n4);
+    for (int i = 0; i < nrep; ++i) k_stream_copy<<<nb, kBlock, 0, c->stream>>>((const nfx_f4*)src, (nfx_f4*)dst, n4);
     NFX_CHECK_LAUNCH();
     NFX_HIP(hipEventRecord(c->ev1, c->stream));
     NFX_HIP(hipEventSynchronize(c->ev1));

@@ -1,11 +1,14 @@
 """GPU tests of the batched stepper (cells of different meshes and parameters in one launch per pass).
 
-- every cell of a batch matches its own single-cell Simulation run (same kernels, cell-aligned
-  tiles): positions within 1e-5 * RadiusSIM, curve within 1e-5 * max|F| (rounding-level, from the
-  cell-constant conversions and the FP64 curve reduction order);
+- every cell of a batch matches its own single-cell Simulation run (same force kernels, cell-aligned
+  tiles): positions within 1e-5 * RadiusSIM, curve within 1e-4 * max|F| (the batch's (x, u)
+  velocity-Verlet form rounds differently from the single cell's (x, v, f_old) form; the tip force
+  is a penetration depth times k, which magnifies that rounding);
 - one cell against the FP64 oracle: 1e-4 (the plain-path tolerance);
 - bitwise: repeated runs, graph vs eager;
-- tile size (an autotuned choice) changes only the rounding: 1e-4.
+- tile size (an autotuned choice) changes only the rounding: 1e-4;
+- cache policy and the unread plastic strain: bitwise; curve sampling in the node pass against the
+  separate curve kernel: 1e-12 (FP64 summation order).
 """
 
 import numpy as np
@@ -48,12 +51,12 @@ def specs_for(viscous):
     return out
 
 
-def make_batch(viscous, tile_k=256, use_graph=True):
+def make_batch(viscous, tile_k=256, use_graph=True, **kw):
     from nofluidx3d_amd.batch import BatchSimulation, make_meshes
 
     specs = specs_for(viscous)
     meshes = make_meshes(specs, workers=1, cache=None)
-    return BatchSimulation(specs, meshes, steps=STEPS, ncurve=NCURVE, tile_k=tile_k, use_graph=use_graph), meshes
+    return BatchSimulation(specs, meshes, steps=STEPS, ncurve=NCURVE, tile_k=tile_k, use_graph=use_graph, **kw), meshes
 
 
 def sample_steps():
@@ -96,7 +99,7 @@ def test_batch_cells_match_single_cell_runs(ctx, viscous):
         assert np.max(np.abs(b.positions(i) - x)) < 1e-5 * R, i
         ref = hist[ks]
         assert np.max(np.abs(ref)) > 0, i
-        assert np.max(np.abs(curves[i] - ref)) < 1e-5 * np.max(np.abs(ref)), i
+        assert np.max(np.abs(curves[i] - ref)) < 1e-4 * np.max(np.abs(ref)), i
 
 
 def test_batch_cell_matches_oracle(ctx):
@@ -148,3 +151,34 @@ def test_tile_size_changes_only_rounding(ctx):
     assert np.max(np.abs(ca - cb)) < 1e-4 * np.max(np.abs(ca))
     for i in range(a.ncells):
         assert np.max(np.abs(a.positions(i) - b.positions(i))) < 1e-4 * a.radius_sim[i]
+
+
+def test_cache_policy_and_plastic_strain_change_nothing(ctx):
+    """The nontemporal cache policy of the force pass and keeping the (unread) plastic strain of the
+    linear Maxwell law leave every bit of the run unchanged."""
+    a, _ = make_batch(True, stream_nt=0, plastic_strain=True)
+    a.advance(STEPS)
+    b, _ = make_batch(True, stream_nt=1, plastic_strain=False)
+    b.advance(STEPS)
+    assert a.eps is not None and b.eps is None
+    assert float(a.eps.abs().max()) > 0
+    assert np.array_equal(a.curves_sim(), b.curves_sim())
+    for i in range(a.ncells):
+        assert np.array_equal(a.positions(i), b.positions(i))
+
+
+def test_curves_sampled_in_the_node_pass_equal_the_curve_kernel(ctx):
+    """The node pass's own curve sampling (per-block partial sums per cell, finished by the last
+    block) against the separate one-workgroup-per-cell curve kernel: same FP64 sums up to their
+    order; positions bitwise."""
+    a, _ = make_batch(True)
+    assert a.batch.min_cell_nodes >= 256  # the node pass samples (a block spans at most two cells)
+    a.advance(STEPS)
+    b, _ = make_batch(True)
+    b.batch.min_cell_nodes = 0  # forces the separate curve kernel
+    b.advance(STEPS)
+    ca, cb = a.curves_sim(), b.curves_sim()
+    assert np.max(np.abs(cb)) > 0
+    assert np.max(np.abs(ca - cb)) <= 1e-12 * np.max(np.abs(cb))
+    assert np.array_equal(a.x.cpu().numpy(), b.x.cpu().numpy())
+    assert int(a.curve_counter.cpu()[0]) == 0

@@ -561,3 +561,34 @@ def test_small_cell_full_run_matches_oracle(ctx):
     of = ref.run(n)
     pe, fe = assert_parity(sim, gx, gf[::100], ref.positions(), of[::100])
     print(f"small cell 20k steps: pos_err {pe:.2e} force_err {fe:.2e} max F {np.max(np.abs(of)):.3e}")
+
+
+@pytest.mark.parametrize("material", ["NH", "VP"])
+def test_force_pass_cache_policy_changes_no_bit(ctx, material):
+    """The tiled force pass with the nontemporal cache policy gives the same bits as the default."""
+    runs = []
+    for nt in (0, 1):
+        sim = build_sim(material=material, scatter="tiled")
+        sim.stream_nt = nt
+        runs.append(run_gpu(sim, 300))
+        assert sim.plan().stream_nt == nt
+    (xa, fa), (xb, fb) = runs
+    assert np.array_equal(xa, xb) and np.array_equal(fa, fb)
+
+
+def test_device_selection_before_any_torch_use(ctx):
+    """A fresh process whose first GPU call is initializeOpenCLObjects (no torch use before it):
+    libnfx must share torch's HIP runtime, or torch finds no device afterwards."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("from nofluidx3d_amd import openCL\n"
+            "openCL.initializeOpenCLObjects(0)\n"
+            "import torch\n"
+            "x = torch.ones(4, device='cuda:0')\n"
+            "print('ok', float(x.sum()), openCL.getDevice().name)\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.startswith("ok 4.0"), r.stdout

@@ -237,17 +237,21 @@ def test_tile_builder_invariants(K):
         ends = tl[c * MX:c * MX + count[c]].astype(np.int64)
         starts = np.concatenate([[0], ends[:-1]])
         assert ends[-1] == 4 * nt
-        seen = []
+        rec = ts[4 * K * c:4 * K * c + 4 * nt].astype(np.int64)
+        local, rank = rec & 0xFF, rec >> 8
+        # every corner names its node, and its places (run start + rank) enumerate the sorted list
+        assert np.all(nodes[local] == sub.reshape(-1))
+        place = starts[local] + rank
+        assert sorted(place) == list(range(4 * nt))
         for i, (b, e) in enumerate(zip(starts, ends)):
-            run = ts[4 * K * c + b:4 * K * c + e].astype(np.int64)
-            assert np.all(np.diff(run) > 0)  # ascending slots within the run
-            assert np.all(sub[run >> 2, run & 3] == nodes[i])
-            seen.extend(run)
+            slots = np.nonzero(local == i)[0]  # this node's corner slots, ascending
+            assert len(slots) == e - b
+            assert np.array_equal(place[slots], np.arange(b, e))  # slot order within the run
             d = td[c * MX + i]
             assert no[nodes[i]] <= d < no[nodes[i] + 1]
             assert owner[d] == -1
             owner[d] = c
-        assert sorted(seen) == list(range(4 * nt))
+        assert np.all(ts[4 * K * c + 4 * nt:4 * K * (c + 1)] == 0)  # padding slots
         assert np.all(tn[c * MX + count[c]:(c + 1) * MX] < N)  # padding stays a valid node id
     assert np.all(owner >= 0) and no[-1] == M
     # within a node, partial positions follow tile order
