@@ -155,11 +155,6 @@ typedef struct {
     double* curves;                /* ncells*ncurve FP64 indenter-force samples */
     int32_t ncurve;
     int64_t curve_steps; /* sample k is taken at the first step s with floor(s*ncurve/curve_steps) == k */
-    /* With every cell at least 256 nodes (so that a node block of 256 spans at most two cells) and
-     * this scratch, the node pass takes the curve samples itself, with no launch of its own: */
-    int32_t min_cell_nodes;  /* smallest node count of a cell */
-    double* curve_partials;  /* 2*ceil(N/256) FP64, nullable */
-    uint32_t* curve_counter; /* 1, zero before the first step (the node pass leaves it zero), nullable */
 } nfx_batch;
 
 /* ---- device and queue: replaces openCL.py:17-50 -------------------------------------------- */

@@ -110,9 +110,6 @@ class Batch(ctypes.Structure):
         ("curves", c_vp),
         ("ncurve", c_i32),
         ("curve_steps", c_i64),
-        ("min_cell_nodes", c_i32),
-        ("curve_partials", c_vp),
-        ("curve_counter", c_vp),
     ]
 
 

@@ -6,9 +6,9 @@ of cells as one concatenated mesh.  Tiles of the tiled scatter are aligned to ce
 sums its nodal forces in exactly the order it would use alone, and a cell's batched trajectory
 equals its single-cell run up to rounding (the batch carries the damped velocity-Verlet state as
 (x, u = a v + f/2), the same recurrence without the previous force).  Per step: one force pass over
-all tiles and one node pass over all nodes, which on sampling steps also sums each cell's FP64
-indenter force (a separate curve pass when a cell has fewer than 256 nodes); steps replay from a
-captured hipGraph (``nfx_batch_run_steps``).
+all tiles, one node pass over all nodes and a curve pass (one workgroup per cell: on sampling
+steps the cell's FP64 indenter force, otherwise an early exit); steps replay from a captured
+hipGraph (``nfx_batch_run_steps``).
 
 Cell model (per cell, in that cell's simulation units, ``Simulation.py:42-56``): Neo-Hookean
 elasticity with E_sim = 0.01 and the cell's Poisson ratio, optionally the Maxwell-type
@@ -327,9 +327,6 @@ class BatchSimulation:
         self.partials = torch.zeros(3 * h["m_total"], dtype=torch.float32, device=dev)
         self.cells = torch.from_numpy(self.cells_host.view(np.uint8).copy()).to(dev)
         self.curves = torch.zeros(nc * ncurve, dtype=torch.float64, device=dev)
-        # scratch of the curve sampling inside the node pass (blocks of 256 nodes)
-        self.curve_partials = torch.zeros(2 * ((self.N + 255) // 256), dtype=torch.float64, device=dev)
-        self.curve_counter = torch.zeros(1, dtype=torch.int32, device=dev)
         torch.cuda.synchronize(dev)
 
         p = _lib.Plan()
@@ -356,8 +353,6 @@ class BatchSimulation:
         b.node_cell, b.cell_node_off, b.cells = self.node_cell.data_ptr(), self.cell_node_off.data_ptr(), \
             self.cells.data_ptr()
         b.curves, b.ncurve, b.curve_steps = self.curves.data_ptr(), ncurve, steps
-        b.min_cell_nodes = int(np.min(np.diff(np.asarray(self.node_off, dtype=np.int64))))
-        b.curve_partials, b.curve_counter = self.curve_partials.data_ptr(), self.curve_counter.data_ptr()
         self.batch = b
 
     def reset(self, specs):

@@ -877,110 +877,37 @@ struct BatchNodeArgs {
     const float* __restrict__ partials;
     const uint32_t* __restrict__ node_cell;
     const nfx_cell* __restrict__ cells;
-    // curve sampling inside the node pass (CURVE): per-block partial sums, finished by the last block
-    const uint32_t* __restrict__ cell_node_off;
-    int32_t ncells, ncurve;
-    int64_t curve_steps;
-    double* curves;
-    double* curve_partials;
-    unsigned int* curve_counter;
 };
-
-// On a sampling step, the block's indenter forces summed per cell (a block of kBlock nodes spans
-// at most two cells: slot 0 is the cell of its first node, slot 1 the next one); the last block
-// to finish sums every cell's block partials in ascending block order into the curve sample.
-__device__ void batch_curve_block(const BatchNodeArgs& a, double s, bool second, int64_t k) {
-    __shared__ double w0[kBlock / 64], w1[kBlock / 64];
-    __shared__ bool last;
-    double s0 = second ? 0.0 : s, s1 = second ? s : 0.0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off);
-        s1 += __shfl_xor(s1, off);
-    }
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) {
-        w0[wid] = s0;
-        w1[wid] = s1;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t0 = 0.0, t1 = 0.0;
-        for (int q = 0; q < kBlock / 64; ++q) {
-            t0 += w0[q];
-            t1 += w1[q];
-        }
-        a.curve_partials[2 * blockIdx.x] = t0;
-        a.curve_partials[2 * blockIdx.x + 1] = t1;
-        __threadfence();
-        last = atomicAdd(a.curve_counter, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    for (int c = threadIdx.x; c < a.ncells; c += kBlock) {
-        const int64_t b0 = a.cell_node_off[c] / kBlock, b1 = (a.cell_node_off[c + 1] - 1) / kBlock;
-        double t = 0.0;
-        for (int64_t b = b0; b <= b1; ++b) {
-            const int slot = a.node_cell[b * kBlock] == (uint32_t)c ? 0 : 1;
-            t += __hip_atomic_load(&a.curve_partials[2 * b + slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        a.curves[(int64_t)c * a.ncurve + k] = t;
-    }
-    if (threadIdx.x == 0) __hip_atomic_store(a.curve_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Node pass of a batch: each node applies its own cell's substrate and indenter (position from
 // the step index with the cell's law), then the damped velocity-Verlet update in the (x, u) form.
-// CURVE: on a sampling step the indenter forces at the positions before the update (the step's
-// start) are also summed per cell into the curves.
-template <bool SUB, int IND, bool CURVE>
+template <bool SUB, int IND>
 __global__ __launch_bounds__(kBlock) void k_batch_node(BatchNodeArgs a, int64_t step, const int64_t* step_dev,
                                                        int32_t step_off) {
     const int64_t time = step_dev ? (*step_dev + step_off) : step;
     const int64_t N = a.N;
     const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    bool sample = false;
-    int64_t k = 0;
-    if constexpr (CURVE) {
-        if (time >= 0 && time < a.curve_steps) {
-            k = time * a.ncurve / a.curve_steps;
-            sample = time == 0 || (time - 1) * a.ncurve / a.curve_steps != k;
-        }
+    if (n >= N) return;
+    V3 f = mk(0.f, 0.f, 0.f);
+    const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
+    for (uint32_t j = b; j < e; ++j) {
+        const float* p = a.partials + 3 * (int64_t)j;
+        f = f + mk(p[0], p[1], p[2]);
     }
-    double tip = 0.0;
-    bool second = false;
-    if (n < N) {
-        V3 f = mk(0.f, 0.f, 0.f);
-        const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
-        for (uint32_t j = b; j < e; ++j) {
-            const float* p = a.partials + 3 * (int64_t)j;
-            f = f + mk(p[0], p[1], p[2]);
-        }
-        const uint32_t cell = a.node_cell[n];
-        const nfx_cell* cp = a.cells + cell;
-        V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
-        if constexpr (SUB) f.y += substrate_term(x.y, cp->substrate_k, cp->wall);
-        if constexpr (IND == NFX_INDENTER_SPHERE) {
-            const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
-            const V3 fs = sphere_term(x, cp->tip_k, cp->tip_radius, pos);
-            f = f + fs;
-            tip = -(double)fs.y;
-        } else if constexpr (IND == NFX_INDENTER_PLANE) {
-            const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
-            const float fp = plane_term(x.y, cp->tip_k, pos);
-            f.y += fp;
-            tip = -(double)fp;
-        }
-        if constexpr (CURVE) second = cell != a.node_cell[(int64_t)blockIdx.x * kBlock];
-        V3 u = mk(a.u[n], a.u[N + n], a.u[2 * N + n]);
-        vv_update_u(x, u, f);
-        a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
-        a.u[n] = u.x; a.u[N + n] = u.y; a.u[2 * N + n] = u.z;
+    const nfx_cell* cp = a.cells + a.node_cell[n];
+    V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
+    if constexpr (SUB) f.y += substrate_term(x.y, cp->substrate_k, cp->wall);
+    if constexpr (IND == NFX_INDENTER_SPHERE) {
+        const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
+        f = f + sphere_term(x, cp->tip_k, cp->tip_radius, pos);
+    } else if constexpr (IND == NFX_INDENTER_PLANE) {
+        const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
+        f.y += plane_term(x.y, cp->tip_k, pos);
     }
-    if constexpr (CURVE) {
-        if (sample) batch_curve_block(a, tip, second, k);  // uniform over the grid
-    }
+    V3 u = mk(a.u[n], a.u[N + n], a.u[2 * N + n]);
+    vv_update_u(x, u, f);
+    a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
+    a.u[n] = u.x; a.u[N + n] = u.y; a.u[2 * N + n] = u.z;
 }
 
 // One workgroup per cell: on a sampling step, the cell's indenter force (Simulation.py:198-212 in
@@ -1032,31 +959,18 @@ static int validate_batch(const nfx_plan* p, const nfx_batch* b) {
 }
 
 template <bool SUB>
-static void launch_batch_node_sub(nfx_ctx* ctx, const nfx_plan& p, const BatchNodeArgs& a, bool curve, int64_t step,
+static void launch_batch_node_sub(nfx_ctx* ctx, const nfx_plan& p, const BatchNodeArgs& a, int64_t step,
                                   const int64_t* sd, int32_t off) {
     const unsigned int nb = nblocks(p.N);
     hipStream_t s = ctx->stream;
-    if (p.indenter == NFX_INDENTER_SPHERE) {
-        if (curve) k_batch_node<SUB, 1, true><<<nb, kBlock, 0, s>>>(a, step, sd, off);
-        else k_batch_node<SUB, 1, false><<<nb, kBlock, 0, s>>>(a, step, sd, off);
-    } else if (p.indenter == NFX_INDENTER_PLANE) {
-        if (curve) k_batch_node<SUB, 2, true><<<nb, kBlock, 0, s>>>(a, step, sd, off);
-        else k_batch_node<SUB, 2, false><<<nb, kBlock, 0, s>>>(a, step, sd, off);
-    } else {
-        k_batch_node<SUB, 0, false><<<nb, kBlock, 0, s>>>(a, step, sd, off);
-    }
-}
-
-// the node pass samples the curves itself (no launch of the curve kernel)
-static bool curves_in_node_pass(const nfx_plan& p, const nfx_batch& b) {
-    return b.curves && p.indenter != NFX_INDENTER_NONE && b.min_cell_nodes >= kBlock && b.curve_partials &&
-           b.curve_counter;
+    if (p.indenter == NFX_INDENTER_SPHERE) k_batch_node<SUB, 1><<<nb, kBlock, 0, s>>>(a, step, sd, off);
+    else if (p.indenter == NFX_INDENTER_PLANE) k_batch_node<SUB, 2><<<nb, kBlock, 0, s>>>(a, step, sd, off);
+    else k_batch_node<SUB, 0><<<nb, kBlock, 0, s>>>(a, step, sd, off);
 }
 
 static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b, int64_t step, const int64_t* sd,
                              int32_t off, int which = 7) {
-    const bool fused_curves = curves_in_node_pass(p, b);
-    if ((which & 1) && b.curves && !fused_curves) {
+    if ((which & 1) && b.curves) {
         if (p.indenter == NFX_INDENTER_SPHERE)
             k_batch_curves<1><<<b.ncells, kBlock, 0, ctx->stream>>>(p.points, p.N, b.cell_node_off, b.cells, b.curves,
                                                                     b.ncurve, b.curve_steps, step, sd, off);
@@ -1072,11 +986,9 @@ static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b
         if (rc) return rc;
     }
     if (which & 4) {
-        BatchNodeArgs a{p.N,        p.points,        p.vel,     p.node_off,          p.partials,
-                        b.node_cell, b.cells,        b.cell_node_off, b.ncells,      b.ncurve,
-                        b.curve_steps, b.curves,     b.curve_partials, b.curve_counter};
-        if (p.substrate) launch_batch_node_sub<true>(ctx, p, a, fused_curves, step, sd, off);
-        else launch_batch_node_sub<false>(ctx, p, a, fused_curves, step, sd, off);
+        BatchNodeArgs a{p.N, p.points, p.vel, p.node_off, p.partials, b.node_cell, b.cells};
+        if (p.substrate) launch_batch_node_sub<true>(ctx, p, a, step, sd, off);
+        else launch_batch_node_sub<false>(ctx, p, a, step, sd, off);
         NFX_CHECK_LAUNCH();
     }
     return NFX_OK;

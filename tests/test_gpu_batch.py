@@ -7,8 +7,7 @@
 - one cell against the FP64 oracle: 1e-4 (the plain-path tolerance);
 - bitwise: repeated runs, graph vs eager;
 - tile size (an autotuned choice) changes only the rounding: 1e-4;
-- cache policy and the unread plastic strain: bitwise; curve sampling in the node pass against the
-  separate curve kernel: 1e-12 (FP64 summation order).
+- cache policy and the unread plastic strain: bitwise.
 """
 
 import numpy as np
@@ -166,19 +165,3 @@ def test_cache_policy_and_plastic_strain_change_nothing(ctx):
     for i in range(a.ncells):
         assert np.array_equal(a.positions(i), b.positions(i))
 
-
-def test_curves_sampled_in_the_node_pass_equal_the_curve_kernel(ctx):
-    """The node pass's own curve sampling (per-block partial sums per cell, finished by the last
-    block) against the separate one-workgroup-per-cell curve kernel: same FP64 sums up to their
-    order; positions bitwise."""
-    a, _ = make_batch(True)
-    assert a.batch.min_cell_nodes >= 256  # the node pass samples (a block spans at most two cells)
-    a.advance(STEPS)
-    b, _ = make_batch(True)
-    b.batch.min_cell_nodes = 0  # forces the separate curve kernel
-    b.advance(STEPS)
-    ca, cb = a.curves_sim(), b.curves_sim()
-    assert np.max(np.abs(cb)) > 0
-    assert np.max(np.abs(ca - cb)) <= 1e-12 * np.max(np.abs(cb))
-    assert np.array_equal(a.x.cpu().numpy(), b.x.cpu().numpy())
-    assert int(a.curve_counter.cpu()[0]) == 0
