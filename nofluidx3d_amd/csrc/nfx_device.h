@@ -166,6 +166,13 @@ __device__ __forceinline__ M9 mul9(const M9& A, const M9& B) {
 // exp2(-log2(J)/3) so that J < 0 yields NaN and J = 0 yields Inf, as pow() does in the reference.
 __device__ __forceinline__ S6 mr_stress(const S6& B, float J, float mu1, float mu2, float kappa) {
     const float trB = B.xx + B.yy + B.zz;
+    if (mu2 == 0.0f) {  // Neo-Hookean (uniform branch): no B^2 terms
+        const float t = fexp2(flog2(J) * (-1.0f / 3.0f));
+        const float t2 = t * t;
+        const float a = mu1 * t2 * t2 * t;
+        const float iso = kappa * (J - 1.0f) - trB * (1.0f / 3.0f) * a;
+        return S6{a * B.xx + iso, a * B.yy + iso, a * B.zz + iso, a * B.xy, a * B.yz, a * B.xz};
+    }
     // B^2 (symmetric)
     S6 B2;
     B2.xx = B.xx * B.xx + B.xy * B.xy + B.xz * B.xz;
@@ -198,10 +205,11 @@ struct Corners {
 
 // f_k = -(sgn/6) sigma c_k
 __device__ __forceinline__ Corners cofactor_forces(const S6& sig, V3 r1, V3 r2, V3 r3, float scale) {
+    const S6 s{sig.xx * scale, sig.yy * scale, sig.zz * scale, sig.xy * scale, sig.yz * scale, sig.xz * scale};
     Corners c;
-    c.f1 = smul(sig, cross(r2, r3)) * scale;
-    c.f2 = smul(sig, cross(r3, r1)) * scale;
-    c.f3 = smul(sig, cross(r1, r2)) * scale;
+    c.f1 = smul(s, cross(r2, r3));
+    c.f2 = smul(s, cross(r3, r1));
+    c.f3 = smul(s, cross(r1, r2));
     c.f0 = (c.f1 + c.f2 + c.f3) * -1.0f;
     return c;
 }
@@ -420,16 +428,29 @@ __device__ __forceinline__ S6 vp_stress_flow(const M9& Fe, V3 r1, V3 r2, V3 r3, 
     const float svm = fsqrt(1.5f * dd);
     o.von_mises = svm;
     o.pressure = tr3;
-    const float over = P.sy > 0.0f ? fmaxf(svm - P.sy, 0.0f) : svm;
-    float rate;
-    if (P.linear) {  // m = 1, n = 0: sigma0 = Y, rate linear in the overstress (uniform branch)
-        rate = P.rate * over * frcp(P.Y);
+    float rate, g;  // Dp = g dev(sigma)
+    if (P.linear && !(P.sy > 0.0f)) {
+        // m = 1, n = 0, no yield offset (uniform branch): rate = rate0 svm / Y, so
+        // g = 1.5 rate / svm = 1.5 rate0 / Y needs no von Mises stress (svm = 0 means dev = 0)
+        const float k = P.rate * frcp(P.Y);
+        rate = k * svm;
+        g = 1.5f * k;
+        o.epsbar = epsbar + rate;
     } else {
-        const float sigma0 = P.Y * powf(1.0f + epsbar / P.eps0, P.n);
-        rate = P.rate * powf(over / sigma0, P.m);
+        const float over = P.sy > 0.0f ? fmaxf(svm - P.sy, 0.0f) : svm;
+        if (P.linear) {  // m = 1, n = 0: sigma0 = Y, rate linear in the overstress
+            rate = P.rate * over * frcp(P.Y);
+        } else {
+        // sigma0 = Y (1 + epsbar/eps0)^n and (over/sigma0)^m as 2^(p log2 x) on the hardware
+        // exp2/log2 (about 1e-6 relative; a libm powf here is hundreds of instructions, part of
+        // which the compiler hoists into the linear path as well); 0^m = 0 for m > 0, 1 for m = 0
+        const float sigma0 = P.Y * fexp2(P.n * flog2(1.0f + epsbar * frcp(P.eps0)));
+        const float ratio = over * frcp(sigma0);
+        rate = P.rate * (ratio > 0.0f ? fexp2(P.m * flog2(ratio)) : (P.m == 0.0f ? 1.0f : 0.0f));
+        }
+        o.epsbar = epsbar + rate;
+        g = (svm != 0.0f) ? rate * 1.5f * frcp(svm) : 0.0f;
     }
-    o.epsbar = epsbar + rate;
-    const float g = (svm != 0.0f) ? rate * 1.5f * frcp(svm) : 0.0f;
     S6 Dp;
     Dp.xx = g * dv.xx;
     Dp.yy = g * dv.yy;

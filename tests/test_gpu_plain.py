@@ -208,7 +208,11 @@ def test_node_kernels_parity(ctx):
         assert f == pytest.approx(fr, rel=1e-5) and m == pytest.approx(mr, rel=1e-5)
 
 
-def test_viscoplastic_kernel_parity(ctx):
+@pytest.mark.parametrize("law", [(1.0, 0.0, 1.0, 0.0005), (1.0, 0.0, 1.0, 0.0), (2.5, 0.3, 0.01, 0.0),
+                                 (0.7, 1.2, 0.05, 0.0003)])
+def test_viscoplastic_kernel_parity(ctx, law):
+    """Flow exponent m, hardening exponent n, hardening threshold eps0 and yield offset: the linear
+    law with and without the offset (its own code path) and two power laws."""
     import torch
 
     from nofluidx3d_amd import _lib
@@ -220,8 +224,9 @@ def test_viscoplastic_kernel_parity(ctx):
     x32 = O.soa(x.astype(np.float32).astype(np.float64))
     E, nu, ratio = 0.01, 0.46, 0.75
     mu = E / (2 * (1 + nu))
-    prm = dict(mu1=ratio * mu, mu2=(1 - ratio) * mu, kappa=E / (3 * (1 - 2 * nu)), m=1.0, Y=0.002, rate=5e-3,
-               eps0=1.0, n=0.0, sy=0.0005)
+    m, n, eps0, sy = law
+    prm = dict(mu1=ratio * mu, mu2=(1 - ratio) * mu, kappa=E / (3 * (1 - 2 * nu)), m=m, Y=0.002, rate=5e-3,
+               eps0=eps0, n=n, sy=sy)
     vp_o = O.VPParams(*prm.values())
     vp_g = _lib.Viscoplastic(*prm.values())
     Fp_r = O.soa(np.tile(np.eye(3).reshape(1, 9), (nT, 1)))
