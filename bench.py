@@ -11,7 +11,7 @@ Printed JSON line (one):
 - hbm_gbs = algorithmic bytes per step / ms_per_step (96 T + 108 N for the viscoplastic step:
   connectivity 16 B + shape state S = Dm^-1 Fp^-1 read and written 72 B + epsbar read and written
   8 B per tetra; 56 T + 108 N for the elastic one: connectivity 16 + Dm^-1 36 + V0 4; the batch's
-  linear Maxwell law keeps no epsbar: 88 T + 108 N);
+  linear Maxwell law keeps no epsbar and its state is the symmetric G = S S^T: 64 T + 108 N);
 - roofline: the dominant kernel (the force pass) -- algorithmic bytes per launch (96 T + 24 N)
   over its HIP-event average duration, against 8 TB/s; ``traffic`` = measured HBM bytes per
   launch from rocprofv3 FETCH_SIZE + WRITE_SIZE when ``--traffic-json`` names a file holding them;
@@ -177,8 +177,9 @@ def bench_batched(steps=10000, ncells=512, log=log):
     curves = b.curves_normalized()
     ok = np.isfinite(curves).all(axis=1)
     tet = np.array([len(m[1]) for m in meshes])
-    # the batch's linear Maxwell law does not read the accumulated plastic strain: it is not kept
-    tet_bytes = VP_TET_BYTES if b.eps is not None else VP_TET_BYTES - 8
+    # the batch's linear Maxwell law does not read the accumulated plastic strain (not kept), and its
+    # state is the symmetric G = S S^T (6 floats) instead of S (9)
+    tet_bytes = 16 + 2 * (24 if b.sym_state else 36) + (8 if b.eps is not None else 0)
     step_bytes = tet_bytes * b.T_real + 108 * b.N
     return {
         "workload": f"{ncells} cells (seeds 1000+i), {b.T_real} tetra, {b.N} nodes, NH + Maxwell viscoplastic, "

@@ -122,6 +122,10 @@ typedef struct {
     float* vp_shape; /* 9T, model NFX_VISCOPLASTIC with NFX_SCATTER_TILED: the state in shape form
                         S = Dm^-1 Fp^-1 (replaces Fp and dminv on that path; nfx_vp_shape_from_fp) */
     double shear, bulk, shear2, volume_fraction; /* legacy laws (as in nfx_elastic) */
+    int32_t vp_sym;    /* NFX_VISCOPLASTIC with NFX_SCATTER_TILED: 1 = vp_shape holds the symmetric state
+                          G = S S^T (6T: xx, yy, zz, xy, yz, xz).  The stress, flow and forces depend
+                          on G alone, so the run is the same, with a third less state; Fp is not
+                          recoverable from it (nfx_vp_sym_from_shape) */
     int32_t stream_nt; /* NFX_SCATTER_TILED with NFX_MR or NFX_VISCOPLASTIC: 1 = the force pass reads
                           and writes the per-tetra state and tile tables with the nontemporal cache
                           policy (faster when they exceed the memory-side cache), 0 = default policy;
@@ -211,6 +215,8 @@ int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float
  * (plasticDeformationTensor, Interactions.py:279-281), 9T SoA FP32 each: */
 int nfx_vp_shape_from_fp(nfx_ctx* ctx, const float* dminv, const float* Fp, int64_t T, float* shape);
 int nfx_vp_fp_from_shape(nfx_ctx* ctx, const float* dminv, const float* shape, int64_t T, float* Fp);
+/* the symmetric state G = S S^T (6T) from the shape state S (9T; S = Dm^-1 at Fp = I) */
+int nfx_vp_sym_from_shape(nfx_ctx* ctx, const float* shape, int64_t T, float* sym);
 /* deterministic gather: force[c*N+n] += sum over slots of node n, in slot order */
 int nfx_gather_corners(nfx_ctx* ctx, const uint32_t* csr_offsets, const uint32_t* csr_slots, const float* corners,
                        int64_t N, float* force);

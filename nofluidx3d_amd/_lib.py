@@ -95,6 +95,7 @@ class Plan(ctypes.Structure):
         ("bulk", c_f64),
         ("shear2", c_f64),
         ("volume_fraction", c_f64),
+        ("vp_sym", c_i32),
         ("stream_nt", c_i32),
     ]
 
@@ -175,6 +176,7 @@ SIGNATURES = {
     "nfx_legacy_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Elastic), c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp,
                                          c_vp, c_vp]),
     "nfx_vp_fp_from_shape": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_vp_sym_from_shape": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
     "nfx_build_node_csr_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp]),
     "nfx_build_tiles_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

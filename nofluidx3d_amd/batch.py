@@ -269,7 +269,7 @@ class BatchSimulation:
     """All cells of ``specs`` on the current device.  ``meshes``: list of (points_um, tets)."""
 
     def __init__(self, specs, meshes, steps=10000, ncurve=256, tile_k=256, use_graph=True, device_id=0,
-                 plastic_strain=False, stream_nt=None):
+                 plastic_strain=False, stream_nt=None, sym_state=True):
         import torch
 
         from nofluidx3d_amd import openCL
@@ -312,9 +312,17 @@ class BatchSimulation:
                   self.v0.data_ptr())
         _lib.call("nfx_stream_sync", hh)
         del ev, vol
+        self.sym_state = bool(sym_state) and self.model == _lib.NFX_VISCOPLASTIC
         if self.model == _lib.NFX_VISCOPLASTIC:
-            # state in shape form S = Dm^-1 Fp^-1 (the tiled force pass's form); S = Dm^-1 at Fp = I
-            self.shape = self.dminv.clone()
+            # state in shape form S = Dm^-1 Fp^-1 (S = Dm^-1 at Fp = I), or, by default, its symmetric
+            # part G = S S^T: the stress, flow and forces depend on G alone, and the batch never
+            # reports Fp, so a third of the state traffic is saved
+            if self.sym_state:
+                self.shape = torch.empty(6 * self.T, dtype=torch.float32, device=dev)
+                _lib.call("nfx_vp_sym_from_shape", hh, self.dminv.data_ptr(), self.T, self.shape.data_ptr())
+                _lib.call("nfx_stream_sync", hh)
+            else:
+                self.shape = self.dminv.clone()
             # the linear Maxwell law does not read the accumulated plastic strain: kept on request
             self.eps = torch.zeros(self.T, dtype=torch.float32, device=dev) if plastic_strain else None
         else:
@@ -339,6 +347,7 @@ class BatchSimulation:
             p.vp_shape = self.shape.data_ptr()
             p.epsbar = self.eps.data_ptr() if self.eps is not None else 0
             p.vp.flow_exponent, p.vp.hardening_exponent = 1.0, 0.0  # every cell: linear Maxwell flow
+            p.vp_sym = 1 if self.sym_state else 0
         p.stream_nt = default_stream_nt(self.T) if stream_nt is None else int(stream_nt)
         p.scatter = _lib.SCATTER_TILED
         p.tile_k, p.tile_max_nodes = K, MX
@@ -385,7 +394,10 @@ class BatchSimulation:
         _lib.call("nfx_precompute_f64", hh, ev.data_ptr(), vol.data_ptr(), self.T, self.dminv.data_ptr(),
                   self.v0.data_ptr())
         _lib.call("nfx_stream_sync", hh)
-        if self.shape is not None:
+        if self.sym_state:
+            _lib.call("nfx_vp_sym_from_shape", hh, self.dminv.data_ptr(), self.T, self.shape.data_ptr())
+            _lib.call("nfx_stream_sync", hh)
+        elif self.shape is not None:
             self.shape.copy_(self.dminv)
             torch.cuda.synchronize(dev)
         del ev, vol

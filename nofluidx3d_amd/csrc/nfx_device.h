@@ -410,9 +410,8 @@ struct VPConst {
 // J = sqrt(det Be) (:82-107), the corner forces, von Mises stress and pressure (:144-151), the
 // flow rate (:153, optionally with the yield offset) and epsbar += rate (:156).  Returns the
 // plastic stretching Dp = 1.5 rate dev(sigma) / svm (:160-169) as a symmetric tensor.
-__device__ __forceinline__ S6 vp_stress_flow(const M9& Fe, V3 r1, V3 r2, V3 r3, float epsbar, const VPConst& P,
-                                             VPOut& o) {
-    const S6 B = aat(Fe);
+__device__ __forceinline__ S6 vp_stress_flow_be(const S6& B, V3 r1, V3 r2, V3 r3, float epsbar, const VPConst& P,
+                                                VPOut& o) {
     const float detB = B.xx * (B.yy * B.zz - B.yz * B.yz) - B.xy * (B.xy * B.zz - B.yz * B.xz) +
                        B.xz * (B.xy * B.yz - B.yy * B.xz);
     const float J = fsqrt(detB);
@@ -459,6 +458,10 @@ __device__ __forceinline__ S6 vp_stress_flow(const M9& Fe, V3 r1, V3 r2, V3 r3, 
     Dp.yz = g * dv.yz;
     Dp.xz = g * dv.xz;
     return Dp;
+}
+__device__ __forceinline__ S6 vp_stress_flow(const M9& Fe, V3 r1, V3 r2, V3 r3, float epsbar, const VPConst& P,
+                                             VPOut& o) {
+    return vp_stress_flow_be(aat(Fe), r1, r2, r3, epsbar, P, o);
 }
 
 // State Fp (the reference's plasticDeformationTensor): Fe = F Fp^-1, Fp += Fe^-1 Dp F (:171-206).
@@ -522,6 +525,52 @@ __device__ __forceinline__ VPOut vp_tetra_shape(V3 p0, V3 p1, V3 p2, V3 p3, cons
         o.Fp[1 * 3 + j] = S[1 * 3 + j] + dot(c2, w) * idet;
         o.Fp[2 * 3 + j] = S[2 * 3 + j] + dot(c3, w) * idet;
     }
+    return o;
+}
+
+// The same law on the symmetric state G = S S^T (6 floats: xx, yy, zz, xy, yz, xz).  The elastic
+// left Cauchy-Green tensor is Be = Fe Fe^T = r G r^T, so the isotropic stress, the flow and the
+// forces depend on G alone (the plastic rotation that G drops enters none of them), and
+// S' = r^-1 (I + Dp)^-1 r S gives exactly
+//     G' = G + r^-1 (-N Be - Be N + N Be N) r^-T,   N = (I + Dp)^-1 Dp,
+// an increment that is zero when nothing flows.  Fp is not recoverable from G.
+__device__ __forceinline__ VPOut vp_tetra_sym(V3 p0, V3 p1, V3 p2, V3 p3, const float* G6, float epsbar,
+                                              const VPConst& P) {
+    VPOut o;
+    const V3 r1 = p1 - p0, r2 = p2 - p0, r3 = p3 - p0;
+    const S6 G{G6[0], G6[1], G6[2], G6[3], G6[4], G6[5]};
+    // Be = r G r^T with the rows of r (component i of the three edges)
+    const V3 qx = mk(r1.x, r2.x, r3.x), qy = mk(r1.y, r2.y, r3.y), qz = mk(r1.z, r2.z, r3.z);
+    const V3 tx = smul(G, qx), ty = smul(G, qy), tz = smul(G, qz);
+    const S6 Be{dot(tx, qx), dot(ty, qy), dot(tz, qz), dot(tx, qy), dot(ty, qz), dot(tx, qz)};
+    const S6 Dp = vp_stress_flow_be(Be, r1, r2, r3, epsbar, P, o);
+    // Ai = (I + Dp)^-1 (symmetric, by cofactors) and N = Ai Dp (symmetric: they commute)
+    const float axx = 1.0f + Dp.xx, ayy = 1.0f + Dp.yy, azz = 1.0f + Dp.zz;
+    const float cxx = ayy * azz - Dp.yz * Dp.yz, cyy = axx * azz - Dp.xz * Dp.xz, czz = axx * ayy - Dp.xy * Dp.xy;
+    const float cxy = Dp.xz * Dp.yz - Dp.xy * azz, cyz = Dp.xy * Dp.xz - axx * Dp.yz, cxz = Dp.xy * Dp.yz - ayy * Dp.xz;
+    const float ia = frcp(axx * cxx + Dp.xy * cxy + Dp.xz * cxz);
+    const S6 Ai{cxx * ia, cyy * ia, czz * ia, cxy * ia, cyz * ia, cxz * ia};
+    const S6 N{Ai.xx * Dp.xx + Ai.xy * Dp.xy + Ai.xz * Dp.xz, Ai.xy * Dp.xy + Ai.yy * Dp.yy + Ai.yz * Dp.yz,
+               Ai.xz * Dp.xz + Ai.yz * Dp.yz + Ai.zz * Dp.zz, Ai.xx * Dp.xy + Ai.xy * Dp.yy + Ai.xz * Dp.yz,
+               Ai.xy * Dp.xz + Ai.yy * Dp.yz + Ai.yz * Dp.zz, Ai.xx * Dp.xz + Ai.xy * Dp.yz + Ai.xz * Dp.zz};
+    // P = N Be (columns), then X = N Be N - P - P^T (symmetric)
+    const V3 b0 = mk(Be.xx, Be.xy, Be.xz), b1 = mk(Be.xy, Be.yy, Be.yz), b2 = mk(Be.xz, Be.yz, Be.zz);
+    const V3 pc0 = smul(N, b0), pc1 = smul(N, b1), pc2 = smul(N, b2);  // P[:, j]
+    const V3 px = mk(pc0.x, pc1.x, pc2.x), py = mk(pc0.y, pc1.y, pc2.y), pz = mk(pc0.z, pc1.z, pc2.z);  // rows
+    const V3 n0 = mk(N.xx, N.xy, N.xz), n1 = mk(N.xy, N.yy, N.yz), n2 = mk(N.xz, N.yz, N.zz);
+    const S6 X{dot(px, n0) - 2.0f * px.x, dot(py, n1) - 2.0f * py.y, dot(pz, n2) - 2.0f * pz.z,
+               dot(px, n1) - px.y - py.x, dot(py, n2) - py.z - pz.y, dot(px, n2) - px.z - pz.x};
+    // r^-1 = [c1; c2; c3] / det r:  dG = C X C^T / det^2
+    const V3 c1 = cross(r2, r3), c2 = cross(r3, r1), c3 = cross(r1, r2);
+    const float idet = frcp(dot(r1, c1));
+    const float id2 = idet * idet;
+    const V3 u1 = smul(X, c1), u2 = smul(X, c2), u3 = smul(X, c3);
+    o.Fp[0] = G.xx + dot(c1, u1) * id2;
+    o.Fp[1] = G.yy + dot(c2, u2) * id2;
+    o.Fp[2] = G.zz + dot(c3, u3) * id2;
+    o.Fp[3] = G.xy + dot(c1, u2) * id2;
+    o.Fp[4] = G.yz + dot(c2, u3) * id2;
+    o.Fp[5] = G.xz + dot(c1, u3) * id2;
     return o;
 }
 

@@ -234,6 +234,22 @@ __global__ __launch_bounds__(kBlock) void k_vp_convert(const float* __restrict__
         for (int j = 0; j < 3; ++j) out[(3 * i + j) * T + t] = Y.m[i][j];
 }
 
+// G = S S^T per tetra (the symmetric viscoplastic state; planes xx, yy, zz, xy, yz, xz)
+__global__ __launch_bounds__(kBlock) void k_vp_sym(const float* __restrict__ S, int64_t T, float* __restrict__ G) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    float m[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) m[k] = S[k * T + t];
+    const V3 a = mk(m[0], m[1], m[2]), b = mk(m[3], m[4], m[5]), c = mk(m[6], m[7], m[8]);  // rows of S
+    G[t] = dot(a, a);
+    G[T + t] = dot(b, b);
+    G[2 * T + t] = dot(c, c);
+    G[3 * T + t] = dot(a, b);
+    G[4 * T + t] = dot(b, c);
+    G[5 * T + t] = dot(a, c);
+}
+
 // ---- force pass, tiled deterministic scatter ---------------------------------------------------
 // One workgroup per tile of K consecutive (Morton-ordered) tetra, one thread per tetra.  The
 // tile's distinct nodes are listed on the host (tile_nodes, ascending, MX <= 256 entries per tile)
@@ -274,8 +290,9 @@ __host__ __device__ constexpr size_t tile_lds_bytes_fixed() {
 __host__ __device__ constexpr size_t tile_lds_bytes_per_node() { return 12 + 2; }
 
 // NT: cache policy of the streams; EPS: the viscoplastic law keeps its accumulated plastic strain
-// (the linear law does not read it)
-template <int MODEL, int K, bool STRESS, bool NT = false, bool EPS = true>
+// (the linear law does not read it); SYM: its state is the symmetric G = S S^T (6 planes) instead
+// of the shape state S (9 planes)
+template <int MODEL, int K, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false>
 __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
@@ -314,7 +331,7 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     if (active) {
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
 #pragma unroll
-            for (int k = 0; k < 9; ++k) D[k] = ld_g<NT, float>(uat((const float*)(Sst + t0 + k * T), u));
+            for (int k = 0; k < (SYM ? 6 : 9); ++k) D[k] = ld_g<NT, float>(uat((const float*)(Sst + t0 + k * T), u));
             if constexpr (EPS) e0 = ld_g<NT, float>(uat((const float*)(eps + t0), u));
         } else {
 #pragma unroll
@@ -354,9 +371,11 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
         const V3 p3 = mk(s_x[l3], s_x[MX + l3], s_x[2 * MX + l3]);
         Corners f;
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
-            const VPOut o = vp_tetra_shape(p0, p1, p2, p3, D, e0, P);
+            VPOut o;
+            if constexpr (SYM) o = vp_tetra_sym(p0, p1, p2, p3, D, e0, P);
+            else o = vp_tetra_shape(p0, p1, p2, p3, D, e0, P);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) st_g<NT, float>(uat(Sst + t0 + k * T, u), o.Fp[k]);
+            for (int k = 0; k < (SYM ? 6 : 9); ++k) st_g<NT, float>(uat(Sst + t0 + k * T, u), o.Fp[k]);
             if constexpr (EPS) st_g<NT, float>(uat(eps + t0, u), o.epsbar);
             if constexpr (STRESS) {
                 st_s<NT>(vm + t0 + u, o.von_mises);
@@ -698,6 +717,8 @@ static int validate_plan(const nfx_plan* p) {
                 "plan: viscoplastic state Fp missing");
     NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->scatter != NFX_SCATTER_TILED || p->vp_shape,
                 "plan: the tiled viscoplastic force needs the shape state vp_shape");
+    NFX_REQUIRE(!p->vp_sym || (p->model == NFX_VISCOPLASTIC && p->scatter == NFX_SCATTER_TILED),
+                "plan: the symmetric state (vp_sym) is for the tiled viscoplastic force only");
     NFX_REQUIRE(p->scatter >= NFX_SCATTER_ATOMIC && p->scatter <= NFX_SCATTER_TILED, "plan: unknown scatter %d",
                 p->scatter);
     if (p->scatter == NFX_SCATTER_TILED) {
@@ -732,10 +753,18 @@ static void launch_tile_vp(unsigned int nb, size_t lds, hipStream_t s, const Til
                            const VPConst& c, const nfx_plan& p) {
     float* vm = STRESS ? p.von_mises : nullptr;
     float* pr = STRESS ? p.pressure : nullptr;
-    if (p.epsbar)
+    if (p.vp_sym) {
+        if (p.epsbar)
+            k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar,
+                                                                                        vm, pr);
+        else
+            k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, nullptr,
+                                                                                         vm, pr);
+    } else if (p.epsbar) {
         k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar, vm, pr);
-    else
+    } else {
         k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, nullptr, vm, pr);
+    }
 }
 
 template <int K>
@@ -1246,6 +1275,14 @@ int nfx_vp_shape_from_fp(nfx_ctx* c, const float* dminv, const float* Fp, int64_
     NFX_REQUIRE(c && dminv && Fp && shape, "NULL argument");
     NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
     k_vp_convert<true><<<nblocks(T), kBlock, 0, c->stream>>>(dminv, Fp, T, shape);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_vp_sym_from_shape(nfx_ctx* c, const float* shape, int64_t T, float* sym) {
+    NFX_REQUIRE(c && shape && sym, "NULL argument");
+    NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
+    k_vp_sym<<<nblocks(T), kBlock, 0, c->stream>>>(shape, T, sym);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }

@@ -165,3 +165,18 @@ def test_cache_policy_and_plastic_strain_change_nothing(ctx):
     for i in range(a.ncells):
         assert np.array_equal(a.positions(i), b.positions(i))
 
+
+
+def test_symmetric_state_matches_shape_state(ctx):
+    """The batch's symmetric viscoplastic state G = S S^T against the shape state S: the same
+    dynamics (the stress and flow depend on G alone), rounding apart."""
+    a, _ = make_batch(True, sym_state=True)
+    a.advance(STEPS)
+    b, _ = make_batch(True, sym_state=False)
+    b.advance(STEPS)
+    assert a.shape.numel() == 6 * a.T and b.shape.numel() == 9 * b.T
+    ca, cb = a.curves_sim(), b.curves_sim()
+    assert np.max(np.abs(cb)) > 0
+    assert np.max(np.abs(ca - cb)) < 1e-4 * np.max(np.abs(cb))
+    for i in range(a.ncells):
+        assert np.max(np.abs(a.positions(i) - b.positions(i))) < 1e-5 * a.radius_sim[i]

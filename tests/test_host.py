@@ -316,3 +316,32 @@ def test_batch_host_arrays_indexing_is_in_bounds():
         used = np.concatenate([h["tile_dst"][c * MX:c * MX + h["tile_count"][c]] for c in range(ntiles)])
         assert np.array_equal(np.sort(used), np.arange(M))
         assert len(h["node_cell"]) == N and h["node_cell"].max() == len(specs) - 1
+
+
+def test_symmetric_viscoplastic_state_update_identity():
+    """The batch's viscoplastic state G = S S^T (S = Dm^-1 Fp^-1) evolves on its own: the update
+    G' = G + r^-1 (-N Be - Be N + N Be N) r^-T with Be = r G r^T and N = (I + Dp)^-1 Dp equals
+    S' S'^T of the shape-state update S' = S - r^-1 (I + Dp)^-1 Dp (r S), and Be = Fe Fe^T, for
+    random edges, states and trace-free symmetric Dp (FP64)."""
+    rng = np.random.default_rng(11)
+    for _ in range(50):
+        r = rng.normal(size=(3, 3)) * 0.2 + np.eye(3)  # well-conditioned edge matrices
+        S = np.linalg.inv(r) @ (np.eye(3) + 0.2 * rng.normal(size=(3, 3)))
+        A = rng.normal(size=(3, 3))
+        D = (A + A.T) / 2
+        D -= np.trace(D) / 3 * np.eye(3)
+        D *= rng.uniform(1e-4, 0.2) / np.abs(D).max()
+        Fe = r @ S
+        Sp = S - np.linalg.inv(r) @ np.linalg.inv(np.eye(3) + D) @ D @ Fe
+        G = S @ S.T
+        Be = r @ G @ r.T
+        N = np.linalg.inv(np.eye(3) + D) @ D
+        ri = np.linalg.inv(r)
+        Gp = G + ri @ (-N @ Be - Be @ N + N @ Be @ N) @ ri.T
+        assert np.allclose(Be, Fe @ Fe.T, rtol=0, atol=1e-11 * np.abs(Be).max())
+        assert np.allclose(Gp, Sp @ Sp.T, rtol=0, atol=1e-11 * np.abs(Gp).max())
+        # a dropped term fails: without N Be N the update is off by O(|Dp|^2)
+        Gx = G + ri @ (-N @ Be - Be @ N) @ ri.T
+        assert not np.allclose(Gx, Sp @ Sp.T, rtol=0, atol=1e-11 * np.abs(Gp).max())
+    # no flow: the increment vanishes exactly
+    assert np.array_equal(np.linalg.inv(np.eye(3) + np.zeros((3, 3))) @ np.zeros((3, 3)), np.zeros((3, 3)))
