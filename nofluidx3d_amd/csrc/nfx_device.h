@@ -533,7 +533,9 @@ __device__ __forceinline__ VPOut vp_tetra_shape(V3 p0, V3 p1, V3 p2, V3 p3, cons
 // forces depend on G alone (the plastic rotation that G drops enters none of them), and
 // S' = r^-1 (I + Dp)^-1 r S gives exactly
 //     G' = G + r^-1 (-N Be - Be N + N Be N) r^-T,   N = (I + Dp)^-1 Dp,
-// an increment that is zero when nothing flows.  Fp is not recoverable from G.
+// an increment that is zero when nothing flows.  Dp is proportional to the deviatoric stress of an
+// isotropic law, a polynomial in Be, so N commutes with Be and the bracket is (N^2 - 2N) Be.
+// Fp is not recoverable from G.
 __device__ __forceinline__ VPOut vp_tetra_sym(V3 p0, V3 p1, V3 p2, V3 p3, const float* G6, float epsbar,
                                               const VPConst& P) {
     VPOut o;
@@ -553,13 +555,11 @@ __device__ __forceinline__ VPOut vp_tetra_sym(V3 p0, V3 p1, V3 p2, V3 p3, const 
     const S6 N{Ai.xx * Dp.xx + Ai.xy * Dp.xy + Ai.xz * Dp.xz, Ai.xy * Dp.xy + Ai.yy * Dp.yy + Ai.yz * Dp.yz,
                Ai.xz * Dp.xz + Ai.yz * Dp.yz + Ai.zz * Dp.zz, Ai.xx * Dp.xy + Ai.xy * Dp.yy + Ai.xz * Dp.yz,
                Ai.xy * Dp.xz + Ai.yy * Dp.yz + Ai.yz * Dp.zz, Ai.xx * Dp.xz + Ai.xy * Dp.yz + Ai.xz * Dp.zz};
-    // P = N Be (columns), then X = N Be N - P - P^T (symmetric)
-    const V3 b0 = mk(Be.xx, Be.xy, Be.xz), b1 = mk(Be.xy, Be.yy, Be.yz), b2 = mk(Be.xz, Be.yz, Be.zz);
-    const V3 pc0 = smul(N, b0), pc1 = smul(N, b1), pc2 = smul(N, b2);  // P[:, j]
-    const V3 px = mk(pc0.x, pc1.x, pc2.x), py = mk(pc0.y, pc1.y, pc2.y), pz = mk(pc0.z, pc1.z, pc2.z);  // rows
+    // X = (N^2 - 2N) Be (symmetric: the factors commute)
     const V3 n0 = mk(N.xx, N.xy, N.xz), n1 = mk(N.xy, N.yy, N.yz), n2 = mk(N.xz, N.yz, N.zz);
-    const S6 X{dot(px, n0) - 2.0f * px.x, dot(py, n1) - 2.0f * py.y, dot(pz, n2) - 2.0f * pz.z,
-               dot(px, n1) - px.y - py.x, dot(py, n2) - py.z - pz.y, dot(px, n2) - px.z - pz.x};
+    const V3 m0 = smul(N, n0) - n0 * 2.0f, m1 = smul(N, n1) - n1 * 2.0f, m2 = smul(N, n2) - n2 * 2.0f;
+    const V3 b0 = mk(Be.xx, Be.xy, Be.xz), b1 = mk(Be.xy, Be.yy, Be.yz), b2 = mk(Be.xz, Be.yz, Be.zz);
+    const S6 X{dot(m0, b0), dot(m1, b1), dot(m2, b2), dot(m0, b1), dot(m1, b2), dot(m0, b2)};
     // r^-1 = [c1; c2; c3] / det r:  dG = C X C^T / det^2
     const V3 c1 = cross(r2, r3), c2 = cross(r3, r1), c3 = cross(r1, r2);
     const float idet = frcp(dot(r1, c1));

@@ -479,6 +479,24 @@ __device__ void reduce_finish(double s, double m, double* partials, unsigned int
 }
 
 // ---- node pass ------------------------------------------------------------------------------
+// Sum of a node's tile partials [b, e) in order, four records' loads in flight at a time (zeros past
+// the end, which leave the sum unchanged): a node has 1-8 partials, and a loop of dependent single
+// loads would wait a full memory latency per partial.
+__device__ __forceinline__ V3 sum_partials(const float* __restrict__ partials, uint32_t b, uint32_t e) {
+    V3 f = mk(0.f, 0.f, 0.f);
+    for (uint32_t j = b; j < e; j += 4) {
+        V3 q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float* p = partials + 3 * (int64_t)(j + i);
+            q[i] = (j + i < e) ? mk(p[0], p[1], p[2]) : mk(0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) f = f + q[i];
+    }
+    return f;
+}
+
 struct NodeArgs {
     int64_t N;
     float* x;
@@ -534,12 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
         if constexpr (GATHER == 1) {
             f = gather_node<G>(a, n, sub);
         } else if constexpr (GATHER == 2) {
-            f = mk(0.f, 0.f, 0.f);
-            const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
-            for (uint32_t j = b; j < e; ++j) {
-                const float* p = a.partials + 3 * (int64_t)j;
-                f = f + mk(p[0], p[1], p[2]);
-            }
+            f = sum_partials(a.partials, a.node_off[n], a.node_off[n + 1]);
         } else {
             f = mk(a.force[n], a.force[N + n], a.force[2 * N + n]);
         }
@@ -917,12 +930,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_node(BatchNodeArgs a, int64_t 
     const int64_t N = a.N;
     const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (n >= N) return;
-    V3 f = mk(0.f, 0.f, 0.f);
-    const uint32_t b = a.node_off[n], e = a.node_off[n + 1];
-    for (uint32_t j = b; j < e; ++j) {
-        const float* p = a.partials + 3 * (int64_t)j;
-        f = f + mk(p[0], p[1], p[2]);
-    }
+    V3 f = sum_partials(a.partials, a.node_off[n], a.node_off[n + 1]);
     const nfx_cell* cp = a.cells + a.node_cell[n];
     V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
     if constexpr (SUB) f.y += substrate_term(x.y, cp->substrate_k, cp->wall);
