@@ -159,6 +159,13 @@ typedef struct {
     double* curves;                /* ncells*ncurve FP64 indenter-force samples */
     int32_t ncurve;
     int64_t curve_steps; /* sample k is taken at the first step s with floor(s*ncurve/curve_steps) == k */
+    /* nullable, all four or none: the tile-node ids and partial positions as 16-bit offsets from
+     * per-tile bases (a tile never leaves its cell; every offset below 65536), which the force pass
+     * then reads in place of plan.tile_nodes and plan.tile_dst */
+    const uint16_t* tile_nodes16;   /* ntiles*MX: tile_nodes - tile_node_base[tile] */
+    const uint16_t* tile_dst16;     /* ntiles*MX: tile_dst - tile_dst_base[tile] (padding 0) */
+    const uint32_t* tile_node_base; /* ntiles */
+    const uint32_t* tile_dst_base;  /* ntiles */
 } nfx_batch;
 
 /* ---- device and queue: replaces openCL.py:17-50 -------------------------------------------- */

@@ -111,6 +111,10 @@ class Batch(ctypes.Structure):
         ("curves", c_vp),
         ("ncurve", c_i32),
         ("curve_steps", c_i64),
+        ("tile_nodes16", c_vp),
+        ("tile_dst16", c_vp),
+        ("tile_node_base", c_vp),
+        ("tile_dst_base", c_vp),
     ]
 
 

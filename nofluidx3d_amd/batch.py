@@ -157,6 +157,23 @@ def sim_points(P_um, radius_sim):
     return p * (radius_sim / abs(p[1, 0]))
 
 
+def narrow_tile_arrays(tile_nodes, tile_dst, tile_count, MX):
+    """The tile-node ids and partial positions as 16-bit offsets from per-tile bases (the first
+    tile-node's: ids ascend within a tile, and so do the node-ordered partial positions).  A tile
+    never leaves its cell, so the offsets fit when every cell has fewer than 65536 nodes; returns
+    {} (the force pass keeps the 32-bit arrays) when one does not."""
+    nodes = tile_nodes.reshape(-1, MX).astype(np.int64)
+    dst = tile_dst.reshape(-1, MX).astype(np.int64)
+    real = np.arange(MX)[None, :] < tile_count[:, None]
+    nb, db = nodes[:, 0], dst[:, 0]
+    on = np.where(real, nodes - nb[:, None], 0)
+    od = np.where(real, dst - db[:, None], 0)
+    if on.min() < 0 or od.min() < 0 or on.max() > 0xFFFF or od.max() > 0xFFFF:
+        return {}
+    return {"tile_nodes16": on.astype(np.uint16).reshape(-1), "tile_dst16": od.astype(np.uint16).reshape(-1),
+            "tile_node_base": nb.astype(np.uint32), "tile_dst_base": db.astype(np.uint32)}
+
+
 def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
     """All host-side arrays of a batch (numpy), before upload.  Tiles are built per cell with a
     common capacity MX and concatenated with node/partial offsets (skipped with geometry_only:
@@ -254,6 +271,7 @@ def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
         h["node_poff"][node_off[i]:node_off[i] + n + 1] = no + ppos
         tpos += ntiles
         ppos += M
+    h.update(narrow_tile_arrays(h["tile_nodes"], h["tile_dst"], h["tile_count"], MX))
     h.update({
         "cells": cells, "units": units, "radius_sim": radius_sim, "max_nodes": MX, "m_total": ppos,
         "node_off": np.array(node_off, np.int64), "tet_pad_off": np.array(tet_pad_off, np.int64),
@@ -266,10 +284,16 @@ def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
 
 
 class BatchSimulation:
-    """All cells of ``specs`` on the current device.  ``meshes``: list of (points_um, tets)."""
+    """All cells of ``specs`` on the current device.  ``meshes``: list of (points_um, tets).
+
+    Options (none changes the physics; each cuts bytes per step):
+    ``plastic_strain`` keep the accumulated plastic strain (the linear Maxwell law does not read it);
+    ``stream_nt`` cache policy of the force pass's streams (None: nontemporal beyond the 256 MB
+    memory-side cache); ``sym_state`` viscoplastic state G = S S^T (6 floats) instead of S (9);
+    ``narrow`` 16-bit tile-node ids and partial positions (when every cell has < 65536 nodes)."""
 
     def __init__(self, specs, meshes, steps=10000, ncurve=256, tile_k=256, use_graph=True, device_id=0,
-                 plastic_strain=False, stream_nt=None, sym_state=True):
+                 plastic_strain=False, stream_nt=None, sym_state=True, narrow=True):
         import torch
 
         from nofluidx3d_amd import openCL
@@ -330,6 +354,9 @@ class BatchSimulation:
         self.t_count, self.t_nodes, self.t_lend = u32(h["tile_count"]), u32(h["tile_nodes"]), u16(h["tile_lend"])
         self.t_corners, self.t_dst = u16(h["tile_corners"]), u32(h["tile_dst"])
         self.t_ntet, self.t_cell = u16(h["tile_ntet"]), u32(h["tile_cell"])
+        # 16-bit tile-node metadata (absent when a cell has 65536 nodes or more)
+        self.t_narrow = (u16(h["tile_nodes16"]), u16(h["tile_dst16"]), u32(h["tile_node_base"]),
+                         u32(h["tile_dst_base"])) if narrow and "tile_nodes16" in h else None
         self.node_poff, self.node_cell = u32(h["node_poff"]), u32(h["node_cell"])
         self.cell_node_off = u32(self.node_off)
         self.partials = torch.zeros(3 * h["m_total"], dtype=torch.float32, device=dev)
@@ -362,6 +389,8 @@ class BatchSimulation:
         b.node_cell, b.cell_node_off, b.cells = self.node_cell.data_ptr(), self.cell_node_off.data_ptr(), \
             self.cells.data_ptr()
         b.curves, b.ncurve, b.curve_steps = self.curves.data_ptr(), ncurve, steps
+        if self.t_narrow is not None:
+            b.tile_nodes16, b.tile_dst16, b.tile_node_base, b.tile_dst_base = (t.data_ptr() for t in self.t_narrow)
         self.batch = b
 
     def reset(self, specs):

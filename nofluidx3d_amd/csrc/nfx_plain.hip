@@ -279,6 +279,11 @@ struct TileArgs {
     const uint16_t* __restrict__ tile_ntet;
     const uint32_t* __restrict__ tile_cell;
     const nfx_cell* __restrict__ cells;
+    // batches (NARROW): tile-node ids and partial positions as 16-bit offsets from per-tile bases
+    const uint16_t* __restrict__ tile_nodes16;
+    const uint16_t* __restrict__ tile_dst16;
+    const uint32_t* __restrict__ tile_node_base;
+    const uint32_t* __restrict__ tile_dst_base;
 };
 
 // LDS of a tile: corner forces 12K floats, then per tile-node its position (3 floats) and run
@@ -291,8 +296,8 @@ __host__ __device__ constexpr size_t tile_lds_bytes_per_node() { return 12 + 2; 
 
 // NT: cache policy of the streams; EPS: the viscoplastic law keeps its accumulated plastic strain
 // (the linear law does not read it); SYM: its state is the symmetric G = S S^T (6 planes) instead
-// of the shape state S (9 planes)
-template <int MODEL, int K, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false>
+// of the shape state S (9 planes); NARROW: 16-bit tile-node metadata (batches)
+template <int MODEL, int K, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false, bool NARROW = false>
 __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
@@ -347,10 +352,15 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     uint32_t nid = 0, dst = 0;
     int lb = 0, le = 0;
     if (owner) {
-        nid = ld_g<NT, uint32_t>(uat(a.tile_nodes + j0, u));
+        if constexpr (NARROW) {
+            nid = a.tile_node_base[c] + (uint32_t)ld_g<NT, uint16_t>(uat(a.tile_nodes16 + j0, u));
+            dst = a.tile_dst_base[c] + (uint32_t)ld_g<NT, uint16_t>(uat(a.tile_dst16 + j0, u));
+        } else {
+            nid = ld_g<NT, uint32_t>(uat(a.tile_nodes + j0, u));
+            dst = ld_g<NT, uint32_t>(uat(a.tile_dst + j0, u));
+        }
         lb = tl ? ld_g<NT, uint16_t>(uat(a.tile_lend + j0, u - 1)) : 0;
         le = ld_g<NT, uint16_t>(uat(a.tile_lend + j0, u));
-        dst = ld_g<NT, uint32_t>(uat(a.tile_dst + j0, u));
     }
     // this tetra's 4 corner records (tile-node | rank << 8), as one 8-byte load
     uint64_t cr = 0;
@@ -766,6 +776,17 @@ static void launch_tile_vp(unsigned int nb, size_t lds, hipStream_t s, const Til
                            const VPConst& c, const nfx_plan& p) {
     float* vm = STRESS ? p.von_mises : nullptr;
     float* pr = STRESS ? p.pressure : nullptr;
+    if constexpr (!STRESS) {  // the batch's form: symmetric state with 16-bit tile-node metadata
+        if (p.vp_sym && a.tile_nodes16) {
+            if (p.epsbar)
+                k_tile_force<NFX_VISCOPLASTIC, K, false, NT, true, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape,
+                                                                                                p.epsbar, vm, pr);
+            else
+                k_tile_force<NFX_VISCOPLASTIC, K, false, NT, false, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape,
+                                                                                                 nullptr, vm, pr);
+            return;
+        }
+    }
     if (p.vp_sym) {
         if (p.epsbar)
             k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar,
@@ -785,11 +806,18 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     // ctx->batch is set only while a batched step is being enqueued
     TileArgs a{p.N,         p.T,          p.dminv,     p.v0,     p.points,         p.tile_count,
                p.tile_nodes, p.tile_lend,  p.tile_corners, p.tile_dst, p.partials,   p.tile_max_nodes,
-               nullptr,     nullptr,      nullptr};
+               nullptr,     nullptr,      nullptr,    nullptr,   nullptr,          nullptr,
+               nullptr};
     if (ctx->batch) {
         a.tile_ntet = ctx->batch->tile_ntet;
         a.tile_cell = ctx->batch->tile_cell;
         a.cells = ctx->batch->cells;
+        if (ctx->batch->tile_nodes16) {
+            a.tile_nodes16 = ctx->batch->tile_nodes16;
+            a.tile_dst16 = ctx->batch->tile_dst16;
+            a.tile_node_base = ctx->batch->tile_node_base;
+            a.tile_dst_base = ctx->batch->tile_dst_base;
+        }
     }
     const unsigned int nb = (unsigned int)((p.T + K - 1) / K);
     const size_t lds = tile_lds_bytes_fixed<K>() + tile_lds_bytes_per_node() * (size_t)p.tile_max_nodes;
@@ -801,7 +829,14 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     const bool nt = p.stream_nt != 0;
     switch (p.model) {
         case NFX_MR:
-            if (nt) k_tile_force<NFX_MR, K, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
+            if (a.tile_nodes16 && nt)
+                k_tile_force<NFX_MR, K, false, true, true, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr,
+                                                                                         nullptr, nullptr);
+            else if (a.tile_nodes16)
+                k_tile_force<NFX_MR, K, false, false, true, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr,
+                                                                                          nullptr, nullptr);
+            else if (nt)
+                k_tile_force<NFX_MR, K, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
             else k_tile_force<NFX_MR, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
             break;
         case NFX_SVK: k_tile_force<NFX_SVK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
@@ -991,6 +1026,9 @@ static int validate_batch(const nfx_plan* p, const nfx_batch* b) {
                 "batch: NULL table");
     NFX_REQUIRE(!b->curves || (b->ncurve > 0 && b->curve_steps > 0 && p->indenter != NFX_INDENTER_NONE),
                 "batch: curves need ncurve, curve_steps and an indenter");
+    const int narrow = (b->tile_nodes16 != nullptr) + (b->tile_dst16 != nullptr) + (b->tile_node_base != nullptr) +
+                       (b->tile_dst_base != nullptr);
+    NFX_REQUIRE(narrow == 0 || narrow == 4, "batch: the 16-bit tile arrays come all four or none");
     NFX_REQUIRE(p->T % p->tile_k == 0, "batch: padded tetra count must be a multiple of tile_k");
     return NFX_OK;
 }

@@ -180,3 +180,15 @@ def test_symmetric_state_matches_shape_state(ctx):
     assert np.max(np.abs(ca - cb)) < 1e-4 * np.max(np.abs(cb))
     for i in range(a.ncells):
         assert np.max(np.abs(a.positions(i) - b.positions(i))) < 1e-5 * a.radius_sim[i]
+
+
+def test_narrow_tile_metadata_changes_no_bit(ctx):
+    """16-bit tile-node ids and partial positions against the 32-bit arrays: the same run, bitwise."""
+    a, _ = make_batch(True)
+    assert a.t_narrow is not None
+    a.advance(STEPS)
+    b, _ = make_batch(True, narrow=False)
+    assert b.t_narrow is None
+    b.advance(STEPS)
+    assert np.array_equal(a.curves_sim(), b.curves_sim())
+    assert np.array_equal(a.x.cpu().numpy(), b.x.cpu().numpy())

@@ -345,3 +345,29 @@ def test_symmetric_viscoplastic_state_update_identity():
         assert not np.allclose(Gx, Sp @ Sp.T, rtol=0, atol=1e-11 * np.abs(Gp).max())
     # no flow: the increment vanishes exactly
     assert np.array_equal(np.linalg.inv(np.eye(3) + np.zeros((3, 3))) @ np.zeros((3, 3)), np.zeros((3, 3)))
+
+
+def test_narrow_tile_arrays_round_trip():
+    """The batch's 16-bit tile-node metadata (offsets from each tile's first entry) gives back the
+    32-bit ids and partial positions of every real tile-node, and is refused when an offset would
+    not fit 16 bits."""
+    from nofluidx3d_amd.batch import narrow_tile_arrays
+
+    rng = np.random.default_rng(4)
+    ntiles, MX = 40, 24
+    count = rng.integers(5, MX + 1, ntiles).astype(np.uint32)
+    nodes = np.zeros((ntiles, MX), np.uint32)
+    dst = np.zeros((ntiles, MX), np.uint32)
+    for c in range(ntiles):
+        base = rng.integers(0, 10 ** 6)
+        nodes[c, :count[c]] = base + np.sort(rng.choice(60000, count[c], replace=False))
+        nodes[c, count[c]:] = nodes[c, 0]  # padding: a valid id
+        dst[c, :count[c]] = 3 * base + np.sort(rng.choice(60000, count[c], replace=False))
+    h = narrow_tile_arrays(nodes.reshape(-1), dst.reshape(-1), count, MX)
+    n16, d16 = h["tile_nodes16"].reshape(ntiles, MX), h["tile_dst16"].reshape(ntiles, MX)
+    for c in range(ntiles):
+        k = count[c]
+        assert np.array_equal(h["tile_node_base"][c] + n16[c, :k].astype(np.uint32), nodes[c, :k])
+        assert np.array_equal(h["tile_dst_base"][c] + d16[c, :k].astype(np.uint32), dst[c, :k])
+    nodes[3, count[3] - 1] = nodes[3, 0] + 70000  # an offset beyond 16 bits
+    assert narrow_tile_arrays(nodes.reshape(-1), dst.reshape(-1), count, MX) == {}
