@@ -222,6 +222,12 @@ int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float
  * (plasticDeformationTensor, Interactions.py:279-281), 9T SoA FP32 each: */
 int nfx_vp_shape_from_fp(nfx_ctx* ctx, const float* dminv, const float* Fp, int64_t T, float* shape);
 int nfx_vp_fp_from_shape(nfx_ctx* ctx, const float* dminv, const float* shape, int64_t T, float* Fp);
+/* A batch's geometry from its cells' recentred FP64 positions p_rel (3N SoA, micrometres) and per-cell
+ * scales (RadiusSIM / |p[1].x|): FP32 positions, Dm^-1 and V0, exactly as the host construction
+ * computes them (TetraCell.py:15-41); padding tetra get the identity shape (a reset on the device) */
+int nfx_batch_geometry(nfx_ctx* ctx, const double* p_rel, const double* cell_scale, const uint32_t* node_cell,
+                       const uint32_t* tetras, const uint16_t* tile_ntet, int32_t tile_k, int64_t N, int64_t T,
+                       float* points, float* dminv, float* v0);
 /* the symmetric state G = S S^T (6T) from the shape state S (9T; S = Dm^-1 at Fp = I) */
 int nfx_vp_sym_from_shape(nfx_ctx* ctx, const float* shape, int64_t T, float* sym);
 /* deterministic gather: force[c*N+n] += sum over slots of node n, in slot order */

@@ -181,6 +181,7 @@ SIGNATURES = {
                                          c_vp, c_vp]),
     "nfx_vp_fp_from_shape": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_vp_sym_from_shape": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "nfx_batch_geometry": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "nfx_build_node_csr_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp]),
     "nfx_build_tiles_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
                                             c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),

@@ -147,8 +147,16 @@ def make_meshes(specs, workers=None, cache="/tmp/nfx_cells"):
     workers = workers or min(16, os.cpu_count() or 1)
     if workers <= 1 or len(jobs) < 4:
         return [_mesh_job(j) for j in jobs]
-    with mp.get_context("spawn").Pool(workers) as pool:
-        return pool.map(_mesh_job, jobs, chunksize=4)
+    # close + join: the workers end on their own rather than by SIGTERM (a profiler preloaded into
+    # them handles that signal and can stall the teardown)
+    pool = mp.get_context("spawn").Pool(workers)
+    try:
+        out = pool.map(_mesh_job, jobs, chunksize=4)
+        pool.close()
+        pool.join()
+        return out
+    finally:
+        pool.terminate()
 
 
 def sim_points(P_um, radius_sim):
@@ -174,10 +182,37 @@ def narrow_tile_arrays(tile_nodes, tile_dst, tile_count, MX):
             "tile_node_base": nb.astype(np.uint32), "tile_dst_base": db.astype(np.uint32)}
 
 
-def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
+def fill_cell(c, spec, u, steps):
+    """One nfx_cell record: the material (E_sim = 0.01, Neo-Hookean, optional Maxwell flow), the
+    substrate and the tip law in the cell's simulation units ``u`` (the constants the single-cell
+    path computes)."""
+    Rs = u["RadiusSIM"]
+    mu = E_SIM / (2 * (1 + spec.nu))
+    c["mu1"], c["mu2"], c["kappa"] = mu, 0.0, E_SIM / (3 * (1 - 2 * spec.nu))
+    if spec.viscosity_si is not None:
+        c["vp_rate"] = u["T0"]
+        c["vp_yield"] = 3.0 * spec.viscosity_si * 1.0 / u["p0"]
+        c["vp_offset"] = spec.yield_si / u["p0"]
+    tipR = spec.tip_radius_um * 1e-6 / u["L0"]
+    c["wall"], c["substrate_k"] = -Rs, 0.1
+    c["tip_radius"], c["tip_k"] = tipR, 0.1
+    c["tip_start"] = Rs + tipR
+    c["tip_speed"] = spec.speed_um_s * 1e-6 / u["V0"]
+    c["tip_move_steps"] = float(steps)
+
+
+def cell_table(specs, steps):
+    """The nfx_cell records, units and RadiusSIM of ``specs`` (no geometry)."""
+    cells = np.zeros(len(specs), dtype=CELL_DTYPE)
+    units = [s.units(steps) for s in specs]
+    for i, (spec, u) in enumerate(zip(specs, units)):
+        fill_cell(cells[i], spec, u, steps)
+    return cells, units, np.array([u["RadiusSIM"] for u in units])
+
+
+def build_host_arrays(specs, meshes, steps, tile_k):
     """All host-side arrays of a batch (numpy), before upload.  Tiles are built per cell with a
-    common capacity MX and concatenated with node/partial offsets (skipped with geometry_only:
-    positions, reference shapes and the cell table only)."""
+    common capacity MX and concatenated with node/partial offsets."""
     K = int(tile_k)
     nc = len(specs)
     cells = np.zeros(nc, dtype=CELL_DTYPE)
@@ -205,24 +240,8 @@ def build_host_arrays(specs, meshes, steps, tile_k, geometry_only=False):
         tile_meta.append((T, len(P), nT))
         node_off.append(node_off[-1] + len(P))
         tet_pad_off.append(tet_pad_off[-1] + nTp)
-        # material (E_sim = 0.01, Neo-Hookean): the constants the single-cell path computes
-        mu = E_SIM / (2 * (1 + spec.nu))
-        c = cells[i]
-        c["mu1"], c["mu2"], c["kappa"] = mu, 0.0, E_SIM / (3 * (1 - 2 * spec.nu))
-        if spec.viscosity_si is not None:
-            c["vp_rate"] = u["T0"]
-            c["vp_yield"] = 3.0 * spec.viscosity_si * 1.0 / u["p0"]
-            c["vp_offset"] = spec.yield_si / u["p0"]
-        tipR = spec.tip_radius_um * 1e-6 / u["L0"]
-        c["wall"], c["substrate_k"] = -Rs, 0.1
-        c["tip_radius"], c["tip_k"] = tipR, 0.1
-        c["tip_start"] = Rs + tipR
-        c["tip_speed"] = spec.speed_um_s * 1e-6 / u["V0"]
-        c["tip_move_steps"] = float(steps)
+        fill_cell(cells[i], spec, u, steps)
     N = int(node_off[-1])
-    if geometry_only:
-        return {"cells": cells, "units": units, "radius_sim": radius_sim, "x": np.concatenate(pts),
-                "ev": np.concatenate(evs), "vol": np.concatenate(vols)}
 
     vp = ctypes.c_void_p
     per = []
@@ -362,6 +381,12 @@ class BatchSimulation:
         self.partials = torch.zeros(3 * h["m_total"], dtype=torch.float32, device=dev)
         self.cells = torch.from_numpy(self.cells_host.view(np.uint8).copy()).to(dev)
         self.curves = torch.zeros(nc * ncurve, dtype=torch.float64, device=dev)
+        # the initial state, kept on the device for reset(): positions and the viscoplastic state
+        # the recentred FP64 positions and each cell's |p[1].x| (sim_points), kept on the device so that
+        # reset() rebuilds the geometry there (RadiusSIM, hence the scale, changes with nu)
+        p_rel = np.concatenate([np.asarray(P, np.float64) - np.asarray(P, np.float64)[0] for P, _ in meshes])
+        self.p_rel = torch.from_numpy(np.ascontiguousarray(p_rel.T.reshape(-1))).to(dev)
+        self.p_abs1 = np.array([abs(float(P[1][0]) - float(P[0][0])) for P, _ in meshes])
         torch.cuda.synchronize(dev)
 
         p = _lib.Plan()
@@ -395,7 +420,9 @@ class BatchSimulation:
 
     def reset(self, specs):
         """Start over with new physical parameters on the same meshes: only E, nu, viscosity, yield,
-        tip radius, speed and indentation depth may change.  Topology and tile arrays are kept."""
+        tip radius, speed and indentation depth may change.  Topology, tiles and reference shapes
+        are kept; the geometry in simulation units (RadiusSIM depends on nu) is rebuilt on the device
+        from the recentred FP64 positions, exactly as construction built it."""
         import torch
 
         from nofluidx3d_amd import openCL
@@ -405,31 +432,29 @@ class BatchSimulation:
                 raise ValueError("reset keeps the meshes: radius, spacing and seed must not change")
         if (specs[0].viscosity_si is not None) != (self.model == _lib.NFX_VISCOPLASTIC):
             raise ValueError("reset keeps the material model")
+        cells, units, radius_sim = cell_table(specs, self.steps)
         self.specs = list(specs)
-        h = build_host_arrays(specs, self._meshes, self.steps, self.tile_k, geometry_only=True)
-        self.cells_host, self.units, self.radius_sim = h["cells"], h["units"], h["radius_sim"]
+        self.cells_host, self.units, self.radius_sim = cells, units, radius_sim
+        self.sync()  # the library's last steps have finished with these buffers
         dev = self.x.device
-        self.x.copy_(torch.from_numpy(np.ascontiguousarray(h["x"].T.reshape(-1), np.float32)))
+        scale = torch.from_numpy(radius_sim / self.p_abs1).to(dev)
+        torch.cuda.synchronize(dev)
+        hh = openCL.handle()
+        _lib.call("nfx_batch_geometry", hh, self.p_rel.data_ptr(), scale.data_ptr(), self.node_cell.data_ptr(),
+                  self.tetras.data_ptr(), self.t_ntet.data_ptr(), self.tile_k, self.N, self.T, self.x.data_ptr(),
+                  self.dminv.data_ptr(), self.v0.data_ptr())
+        if self.sym_state:
+            _lib.call("nfx_vp_sym_from_shape", hh, self.dminv.data_ptr(), self.T, self.shape.data_ptr())
+        _lib.call("nfx_stream_sync", hh)
+        if self.shape is not None and not self.sym_state:
+            self.shape.copy_(self.dminv)
         self.v.zero_()
         self.fold.zero_()
-        ev = torch.from_numpy(np.ascontiguousarray(h["ev"].T.reshape(-1))).to(dev)
-        vol = torch.from_numpy(np.ascontiguousarray(h["vol"])).to(dev)
         if self.eps is not None:
             self.eps.zero_()
         self.cells.copy_(torch.from_numpy(self.cells_host.view(np.uint8).copy()))
         self.curves.zero_()
         torch.cuda.synchronize(dev)
-        hh = openCL.handle()
-        _lib.call("nfx_precompute_f64", hh, ev.data_ptr(), vol.data_ptr(), self.T, self.dminv.data_ptr(),
-                  self.v0.data_ptr())
-        _lib.call("nfx_stream_sync", hh)
-        if self.sym_state:
-            _lib.call("nfx_vp_sym_from_shape", hh, self.dminv.data_ptr(), self.T, self.shape.data_ptr())
-            _lib.call("nfx_stream_sync", hh)
-        elif self.shape is not None:
-            self.shape.copy_(self.dminv)
-            torch.cuda.synchronize(dev)
-        del ev, vol
         self.time = 0
 
     def features(self):

@@ -116,15 +116,8 @@ __device__ __forceinline__ void store_corners(float* __restrict__ c, int64_t t, 
 // Inputs are the reference's Tetra buffers (Tetra.py:53-58): edge vectors R1..R3 as 9 SoA FP64
 // arrays and the signed volumes.  Dm = [R1 | R2 | R3] (tetra.cl:40-47) is inverted once in FP64
 // (the reference re-inverts it every step, tetra.cl:53); rows of Dm^-1 are (R2xR3, R3xR1, R1xR2)/det.
-__global__ __launch_bounds__(kBlock) void k_precompute_f64(const double* __restrict__ ev, const double* __restrict__ vol,
-                                                           int64_t T, float* __restrict__ dminv, float* __restrict__ v0) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= T) return;
-    double R[3][3];  // R[k] = column k+1 of Dm
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) R[k][c] = ev[(3 * k + c) * T + t];
+// Dm^-1 of tetra t from its edge vectors R[k] (column k+1 of Dm), in FP64, stored as FP32
+__device__ __forceinline__ void dminv_from_edges(const double (&R)[3][3], int64_t T, int64_t t, float* __restrict__ dminv) {
     double cr[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -139,7 +132,72 @@ __global__ __launch_bounds__(kBlock) void k_precompute_f64(const double* __restr
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) dminv[(3 * i + j) * T + t] = (float)(cr[i][j] / det);
+}
+
+__global__ __launch_bounds__(kBlock) void k_precompute_f64(const double* __restrict__ ev, const double* __restrict__ vol,
+                                                           int64_t T, float* __restrict__ dminv, float* __restrict__ v0) {
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    double R[3][3];  // R[k] = column k+1 of Dm
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) R[k][c] = ev[(3 * k + c) * T + t];
+    dminv_from_edges(R, T, t, dminv);
     v0[t] = (float)vol[t];
+}
+
+// A batch's geometry in simulation units from its cells' recentred positions p (FP64, micrometres)
+// and per-cell scales f (RadiusSIM / |p[1].x|, TetraCell.py:15-20): positions x = p f as FP32, and
+// per tetra the edges R_k = x_k - x_0, V0 = R1 . (R2 x R3) / 6 and Dm^-1, in the operation order of
+// the host construction (no contraction), so a reset rebuilds exactly what construction built.
+// Padding tetra (past tile_ntet in their tile) get the identity shape and V0 = 1/6.
+__global__ __launch_bounds__(kBlock) void k_batch_points(const double* __restrict__ p, const double* __restrict__ f,
+                                                         const uint32_t* __restrict__ node_cell, int64_t N,
+                                                         float* __restrict__ x) {
+#pragma clang fp contract(off)
+    const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (n >= N) return;
+    const double s = f[node_cell[n]];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c * N + n] = (float)(p[c * N + n] * s);
+}
+
+__global__ __launch_bounds__(kBlock) void k_batch_edges(const double* __restrict__ p, const double* __restrict__ f,
+                                                        const uint32_t* __restrict__ node_cell,
+                                                        const uint32_t* __restrict__ tets,
+                                                        const uint16_t* __restrict__ tile_ntet, int32_t K, int64_t N,
+                                                        int64_t T, float* __restrict__ dminv, float* __restrict__ v0) {
+#pragma clang fp contract(off)
+    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= T) return;
+    double R[3][3];
+    double vol;
+    if (t % K >= (int64_t)tile_ntet[t / K]) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) R[k][c] = (k == c) ? 1.0 : 0.0;
+        vol = 1.0 / 6.0;
+    } else {
+        const uint32_t n0 = tets[t];
+        const double s = f[node_cell[n0]];
+        double x0[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x0[c] = p[c * N + n0] * s;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t nk = tets[(int64_t)(k + 1) * T + t];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) R[k][c] = p[c * N + nk] * s - x0[c];
+        }
+        const double cx = R[1][1] * R[2][2] - R[1][2] * R[2][1];
+        const double cy = R[1][2] * R[2][0] - R[1][0] * R[2][2];
+        const double cz = R[1][0] * R[2][1] - R[1][1] * R[2][0];
+        vol = (R[0][0] * cx + R[0][1] * cy + R[0][2] * cz) / 6.0;
+    }
+    dminv_from_edges(R, T, t, dminv);
+    v0[t] = (float)vol;
 }
 
 // ---- force pass, elastic models -------------------------------------------------------------
@@ -1217,6 +1275,22 @@ int nfx_device_name(nfx_ctx* c, char* buf, int32_t len) {
     hipDeviceProp_t prop;
     NFX_HIP(hipGetDeviceProperties(&prop, c->device));
     snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return NFX_OK;
+}
+
+int nfx_batch_geometry(nfx_ctx* c, const double* p_rel, const double* cell_scale, const uint32_t* node_cell,
+                       const uint32_t* tetras, const uint16_t* tile_ntet, int32_t tile_k, int64_t N, int64_t T,
+                       float* points, float* dminv, float* v0) {
+    NFX_REQUIRE(c && p_rel && cell_scale && node_cell && tetras && tile_ntet && points && dminv && v0, "NULL argument");
+    NFX_REQUIRE(tile_k == 256 || tile_k == 512, "tile_k must be 256 or 512");
+    int rc = check_sizes(N, T);
+    if (rc) return rc;
+    NFX_REQUIRE(T % tile_k == 0, "padded tetra count must be a multiple of tile_k");
+    k_batch_points<<<nblocks(N), kBlock, 0, c->stream>>>(p_rel, cell_scale, node_cell, N, points);
+    NFX_CHECK_LAUNCH();
+    k_batch_edges<<<nblocks(T), kBlock, 0, c->stream>>>(p_rel, cell_scale, node_cell, tetras, tile_ntet, tile_k, N, T,
+                                                        dminv, v0);
+    NFX_CHECK_LAUNCH();
     return NFX_OK;
 }
 

@@ -192,3 +192,33 @@ def test_narrow_tile_metadata_changes_no_bit(ctx):
     b.advance(STEPS)
     assert np.array_equal(a.curves_sim(), b.curves_sim())
     assert np.array_equal(a.x.cpu().numpy(), b.x.cpu().numpy())
+
+
+def test_reset_reproduces_a_fresh_run(ctx):
+    """reset() rebuilds the initial state on the device: a rerun equals the first run bitwise, and
+    new physics equals a batch built with it."""
+    import dataclasses
+
+    a, meshes = make_batch(True)
+    first_radius = a.radius_sim.copy()
+    a.advance(STEPS)
+    first = a.curves_sim().copy()
+    xa = a.x.cpu().numpy().copy()
+    a.reset(a.specs)
+    a.advance(STEPS)
+    assert np.array_equal(a.curves_sim(), first) and np.array_equal(a.x.cpu().numpy(), xa)
+    # new E, viscosity, tip and nu (the relaxation per step stays put): nu moves RadiusSIM, so the
+    # geometry is rebuilt on the device, exactly as construction builds it
+    new = [dataclasses.replace(s, E_si=1.3 * s.E_si, viscosity_si=1.3 * s.viscosity_si, nu=0.97 * s.nu,
+                               tip_radius_um=1.1 * s.tip_radius_um) for s in a.specs]
+    from nofluidx3d_amd.batch import BatchSimulation
+
+    b = BatchSimulation(new, meshes, steps=STEPS, ncurve=NCURVE, tile_k=256)
+    a.reset(new)
+    assert not np.array_equal(a.radius_sim, first_radius)
+    assert np.array_equal(a.x.cpu().numpy(), b.x.cpu().numpy())
+    assert np.array_equal(a.dminv.cpu().numpy(), b.dminv.cpu().numpy())
+    assert np.array_equal(a.shape.cpu().numpy(), b.shape.cpu().numpy())
+    a.advance(STEPS)
+    b.advance(STEPS)
+    assert np.array_equal(a.curves_sim(), b.curves_sim())
