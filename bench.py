@@ -313,6 +313,13 @@ def main():
     from nofluidx3d_amd import configs
 
     assert torch.cuda.is_available(), "no HIP device"
+    if args.calibrate_copy:
+        a = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if args.only == "batched":
         print(json.dumps(bench_batched(steps=args.batched_steps)), flush=True)
         return
@@ -335,13 +342,6 @@ def main():
         from nofluidx3d_amd.autotune import tune_simulation
 
         _, tuned = tune_simulation(sim, log=log)
-    if args.calibrate_copy:
-        a = torch.ones(args.calibrate_copy // 4, dtype=torch.float32, device="cuda:0")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
     ms = time_steps(sim, args.warmup, args.steps) * 1e3
     ms_force, ms_node = pass_times(sim, min(args.steps, 100))
     vp = material == "viscoplastic"
