@@ -100,6 +100,26 @@ def stabilize(theta, T0, max_relax=0.1):
     return th
 
 
+def _scores(pred, targets, scale, corr=None):
+    """Closed-form amplitude a (clipped to [1e-6, 1e6]) and relative L2 error of a F against the
+    targets for candidate curves F = scale (pred + corr): pred (n, m, k) float32, targets (n, k),
+    scale (n, m), corr (n, k) or None.  Both follow from the dot products p.t and p.p (and t.t):
+    ||a F - t||^2 = a^2 F.F - 2 a F.t + t.t, so no (n, m, k) temporaries are formed."""
+    t = np.asarray(targets, np.float64)
+    p64 = pred.astype(np.float64)
+    pt = np.matmul(p64, t[:, :, None])[..., 0]
+    pp = np.einsum("ijk,ijk->ij", p64, p64)
+    if corr is not None:
+        c = np.asarray(corr, np.float64)
+        pt = pt + np.einsum("ik,ik->i", c, t)[:, None]
+        pp = pp + 2.0 * np.matmul(p64, c[:, :, None])[..., 0] + np.einsum("ik,ik->i", c, c)[:, None]
+    tt = np.einsum("ik,ik->i", t, t)[:, None]
+    ft, ff = scale * pt, scale * scale * pp
+    a = np.clip(ft / np.maximum(ff, 1e-300), 1e-6, 1e6)
+    err = np.sqrt(np.maximum(a * a * ff - 2.0 * a * ft + tt, 0.0) / tt)
+    return a, err
+
+
 def _search(net, norm, targets, bases, R2, T0r, rounds, ncand, rng, corr=None, start=None):
     """Cross-entropy search over the shape parameters (nu, log10 eta/E) of every target at once.
     ``corr`` (n, ncurve): additive correction of the normalised prediction per target; ``start``
@@ -135,17 +155,12 @@ def _search(net, norm, targets, bases, R2, T0r, rounds, ncand, rng, corr=None, s
         for _ in range(2):
             img = clip_theta(th)
             ti = time.perf_counter()
-            pred = net.predict(norm.x(features(img.reshape(-1, 3), rep))) * norm.y_scale
+            pred = net.predict(norm.x(features(img.reshape(-1, 3), rep))) * np.float32(norm.y_scale)
             infer_s += time.perf_counter() - ti
-            pred = pred.reshape(n, ncand, -1)
-            if corr is not None:
-                pred = pred + corr[:, None, :]
-            F = pred * (10 ** img[:, :, :1]) * R2[:, None, None]
-            a = np.clip(amplitude(F, targets[:, None, :]), 1e-6, 1e6)
+            a, err = _scores(pred.reshape(n, ncand, -1), targets, 10 ** img[..., 0] * R2[:, None], corr)
             th = img.copy()
             th[..., 0] += np.log10(a)
             th[..., 2] += np.log10(a)
-        err = rel_err(F * a[..., None], targets[:, None, :])
         all_theta = np.concatenate([all_theta, th], 1)
         all_err = np.concatenate([all_err, err], 1)
     return all_theta, all_err, infer_s

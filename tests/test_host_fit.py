@@ -59,6 +59,28 @@ def blind(cs):
     return [dataclasses.replace(c, E_si=1000.0, nu=0.4, viscosity_si=10.0) for c in cs]
 
 
+def test_candidate_scores_equal_the_direct_amplitude_and_error():
+    """fit._scores (dot-product form) against the direct computation (scaled curves, closed-form
+    amplitude, explicit residual norm), with and without the additive correction; candidate 0 of
+    each target is a near-fit (amplitude 3, residual 1 %) so the unclipped branch and small errors
+    are covered."""
+    rng = np.random.default_rng(11)
+    n, m = 5, 7
+    pred = rng.normal(size=(n, m, NC)).astype(np.float32)
+    scale = 10 ** rng.uniform(-12, -9, size=(n, m))
+    corr = 0.3 * rng.normal(size=(n, NC))
+    for c in (None, corr):
+        P = pred.astype(np.float64) + (0 if c is None else c[:, None, :])
+        t = 3.0 * scale[:, :1] * P[:, 0] * (1.0 + 0.01 * rng.normal(size=(n, NC)))
+        F = scale[..., None] * P
+        a_ref = np.clip(np.sum(F * t[:, None], -1) / np.sum(F * F, -1), 1e-6, 1e6)
+        e_ref = np.linalg.norm(a_ref[..., None] * F - t[:, None], axis=-1) / np.linalg.norm(t, axis=-1)[:, None]
+        a, e = fit._scores(pred, t, scale, c)
+        assert np.all(np.abs(a[:, 0] - 3.0) < 0.01) and np.all(e[:, 0] < 0.02)
+        assert np.allclose(a, a_ref, rtol=1e-12, atol=0)
+        assert np.allclose(e, e_ref, rtol=1e-9, atol=0)
+
+
 def test_symmetry_helpers_keep_the_box_and_the_ratio():
     rng = np.random.default_rng(0)
     th = np.stack([rng.uniform(1, 5, 500), rng.uniform(0.2, 0.6, 500), rng.uniform(-1, 4, 500)], 1)
