@@ -247,6 +247,11 @@ def check(rc: int, what: str = ""):
 
 
 def call(name: str, *args):
+    """Call ``name`` and raise NfxError on a non-zero return.  The argument count is checked
+    against the declared signature (ctypes would pass surplus arguments on silently)."""
+    want = len(SIGNATURES[name][1])
+    if len(args) != want:
+        raise TypeError(f"{name} takes {want} arguments ({len(args)} given)")
     check(getattr(lib(), name)(*args), name)
 
 

@@ -1211,22 +1211,11 @@ int nfx_device_count(int* count) {
     return NFX_OK;
 }
 
-int nfx_init(int device_id, void* stream, nfx_ctx** out) {
-    NFX_REQUIRE(out, "ctx out-pointer is NULL");
-    int count = 0;
-    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
-        set_error("no HIP device available");
-        return NFX_ERR_DEVICE;
-    }
-    if (device_id < 0 || device_id >= count) {
-        set_error("device %d not found (%d HIP devices)", device_id, count);
-        return NFX_ERR_DEVICE;
-    }
-    NFX_HIP(hipSetDevice(device_id));
-    nfx_ctx* c = new nfx_ctx();
-    c->device = device_id;
+namespace nfx {
+// device resources of a new context (on the current device); on failure the caller destroys it
+static int init_ctx(nfx_ctx* c, void* stream) {
     hipDeviceProp_t prop;
-    NFX_HIP(hipGetDeviceProperties(&prop, device_id));
+    NFX_HIP(hipGetDeviceProperties(&prop, c->device));
     c->num_cu = prop.multiProcessorCount;
     if (stream) {
         c->stream = (hipStream_t)stream;
@@ -1243,6 +1232,29 @@ int nfx_init(int device_id, void* stream, nfx_ctx** out) {
     NFX_HIP(hipEventCreate(&c->ev0));
     NFX_HIP(hipEventCreate(&c->ev1));
     NFX_HIP(hipDeviceSynchronize());
+    return NFX_OK;
+}
+}  // namespace nfx
+
+int nfx_init(int device_id, void* stream, nfx_ctx** out) {
+    NFX_REQUIRE(out, "ctx out-pointer is NULL");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        set_error("no HIP device available");
+        return NFX_ERR_DEVICE;
+    }
+    if (device_id < 0 || device_id >= count) {
+        set_error("device %d not found (%d HIP devices)", device_id, count);
+        return NFX_ERR_DEVICE;
+    }
+    NFX_HIP(hipSetDevice(device_id));
+    nfx_ctx* c = new nfx_ctx();
+    c->device = device_id;
+    const int rc = nfx::init_ctx(c, stream);
+    if (rc != NFX_OK) {
+        nfx_destroy(c);  // frees what was set up (the error message stays)
+        return rc;
+    }
     *out = c;
     return NFX_OK;
 }
@@ -1257,8 +1269,8 @@ int nfx_destroy(nfx_ctx* c) {
     (void)hipFree(c->counter);
     (void)hipFree(c->step_dev);
     (void)hipFree(c->hist_dev);
-    (void)hipEventDestroy(c->ev0);
-    (void)hipEventDestroy(c->ev1);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return NFX_OK;
