@@ -76,14 +76,18 @@ class CellSpec:
         return G * self.units(steps)["T0"] / self.viscosity_si
 
 
-def surrogate_inputs(E_si, nu, viscosity_si, tip_radius_um, speed_um_s, radius_um):
-    """The six surrogate inputs (n, 6): log10 E [Pa], nu, log10 viscosity [Pa s], log10 tip radius
-    [um], log10 speed [um/s], log10 cell radius [um].  In logs, the dimensionless groups of the
-    problem (viscosity/E, tip/cell radius, the Deborah number viscosity*speed/(E*radius)) are
-    linear combinations of the inputs."""
-    return np.stack([np.log10(np.asarray(E_si, np.float64)), np.asarray(nu, np.float64),
-                     np.log10(np.asarray(viscosity_si, np.float64)), np.log10(np.asarray(tip_radius_um, np.float64)),
-                     np.log10(np.asarray(speed_um_s, np.float64)), np.log10(np.asarray(radius_um, np.float64))], 1)
+def surrogate_inputs(E_si, nu, viscosity_si, tip_radius_um, speed_um_s, radius_um, spacing_um):
+    """The six surrogate inputs (n, 6): log10 (viscosity / E) [s], nu, log10 tip radius [um],
+    log10 speed [um/s], log10 cell radius [um], log10 lattice spacing [um].
+    - E and the viscosity enter only through their ratio: scaling both by one factor leaves the
+      run in simulation units unchanged, so the curve normalised by E R^2 is invariant.
+    - The lattice spacing sets the mesh resolution, which moves the curve by several per cent at
+      fixed physics.
+    In logs, the dimensionless groups of the problem (tip / cell radius, the Deborah number
+    viscosity * speed / (E * radius), radius / spacing) are linear combinations of the inputs."""
+    f = lambda v: np.asarray(v, np.float64)  # noqa: E731
+    return np.stack([np.log10(f(viscosity_si) / f(E_si)), f(nu), np.log10(f(tip_radius_um)), np.log10(f(speed_um_s)),
+                     np.log10(f(radius_um)), np.log10(f(spacing_um))], 1)
 
 
 def estimated_tetra(radius_um, spacing_um):
@@ -461,7 +465,8 @@ class BatchSimulation:
         """Surrogate inputs of every cell (``surrogate_inputs``)."""
         return surrogate_inputs([s.E_si for s in self.specs], [s.nu for s in self.specs],
                                 [s.viscosity_si or 1.0 for s in self.specs], [s.tip_radius_um for s in self.specs],
-                                [s.speed_um_s for s in self.specs], [s.radius_um for s in self.specs])
+                                [s.speed_um_s for s in self.specs], [s.radius_um for s in self.specs],
+                                [s.spacing_um for s in self.specs])
 
     def advance(self, nsteps):
         from nofluidx3d_amd import openCL

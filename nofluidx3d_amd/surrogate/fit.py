@@ -48,7 +48,8 @@ def theta_to_spec(theta, base):
 def features(theta, bases):
     """surrogate inputs for parameter rows theta (n, 3) of cells ``bases`` (n)"""
     return surrogate_inputs(10 ** theta[:, 0], theta[:, 1], 10 ** theta[:, 2], [b.tip_radius_um for b in bases],
-                            [b.speed_um_s for b in bases], [b.radius_um for b in bases])
+                            [b.speed_um_s for b in bases], [b.radius_um for b in bases],
+                            [b.spacing_um for b in bases])
 
 
 def curve_force(curve_norm, spec):

@@ -7,15 +7,16 @@ log-uniform 0.5-50 um/s; per-curve seeds ``seed0 + i``), 10k steps each, one 256
 cell, normalised by E R^2.  Two held-out sets: new physics on the pool's meshes (disjoint seeds),
 and new cells (new meshes, seeds 5000+i) with new physics.
 
-Model.  MLP 6 -> 512 x 4 (GELU) -> 256 on ``batch.surrogate_inputs`` (standardised), targets
-scaled by one global constant; Adam (lr 1e-3, batch 4096).
+Model.  MLP 6 -> 512 x 4 (GELU) -> 256 on ``batch.surrogate_inputs`` (standardised: log10
+viscosity / E, nu, log10 tip radius, log10 speed, log10 cell radius, log10 lattice spacing),
+targets scaled by one global constant; Adam (lr 1e-3, batch 4096).
 - Loss: the squared relative L2 error of each curve, i.e. the MSE with row weights
   1 / mean(target_row^2) (floored at 1e-3 of the median), so that small-force curves count as much
   as large ones.
-- Augmentation by an exact symmetry of the simulator: scaling E and the viscosity by one factor
-  leaves every simulation-unit quantity of a cell unchanged (p0 scales with E, the flow parameter
-  3 eta / p0 and the time scale do not change), so the normalised curve is invariant.  Each training
-  row's log10 E and log10 viscosity get one common random offset within the sampled ranges.
+- An exact symmetry of the simulator is built into the inputs: scaling E and the viscosity by one
+  factor leaves every simulation-unit quantity of a cell unchanged (p0 scales with E, the flow
+  parameter 3 eta / p0 and the time scale do not change), so the normalised curve depends on their
+  ratio only, which is one input.
 - Early stopping: 5 % of the training curves are a validation split; the weights with the lowest
   median validation error are kept.
 Error metric: relative L2 error of each held-out curve, ||pred - sim|| / ||sim||.
@@ -38,6 +39,9 @@ import numpy as np
 from nofluidx3d_amd.batch import CellSpec
 
 SIZES = (6, 512, 512, 512, 512, 256)
+# the network's inputs, in the order of batch.surrogate_inputs (stored with the weights)
+INPUTS = ("log10_viscosity_over_E", "nu", "log10_tip_radius_um", "log10_speed_um_s", "log10_radius_um",
+          "log10_spacing_um")
 LOG_E = (2.0, 4.0)  # sampled range of log10 E [Pa]
 LOG_ETA = (0.0, 3.0)  # sampled range of log10 viscosity [Pa s]
 
@@ -85,18 +89,6 @@ def generate(bs, pool, n_chunks, seed0, log=print):
     return X[ok], Y[ok]
 
 
-def scale_shift(X, rng):
-    """The E-viscosity symmetry: one common random offset on log10 E and log10 viscosity, uniform
-    over the offsets that keep both inside their sampled ranges."""
-    lo = np.maximum(LOG_E[0] - X[:, 0], LOG_ETA[0] - X[:, 2])
-    hi = np.minimum(LOG_E[1] - X[:, 0], LOG_ETA[1] - X[:, 2])
-    a = lo + (hi - lo) * rng.random(len(X))
-    out = np.array(X, np.float64, copy=True)
-    out[:, 0] += a
-    out[:, 2] += a
-    return out
-
-
 def row_weights(Y):
     """1 / mean(y^2) per curve (floored at 1e-3 of the median), normalised to mean 1."""
     ms = np.mean(np.asarray(Y, np.float64) ** 2, axis=1)
@@ -104,8 +96,8 @@ def row_weights(Y):
     return (w / w.mean()).astype(np.float32)
 
 
-def train(X, Y, iters=4000, batch=4096, seed=0, log=print, Xv=None, Yv=None, every=250, augment=True,
-          relative=True, val_frac=0.05):
+def train(X, Y, iters=4000, batch=4096, seed=0, log=print, Xv=None, Yv=None, every=250, relative=True,
+          val_frac=0.05):
     import torch
 
     from nofluidx3d_amd.surrogate.mlp import MLP
@@ -115,7 +107,10 @@ def train(X, Y, iters=4000, batch=4096, seed=0, log=print, Xv=None, Yv=None, eve
     nval = max(1, int(val_frac * len(X))) if len(X) > 20 else 0
     val, tr = perm[:nval], perm[nval:]
     Xtr, Ytr = X[tr], Y[tr]
-    x_mean, x_std = X.mean(0), X.std(0) + 1e-12
+    # FP64 statistics; an input that is constant over the data (e.g. one lattice spacing) keeps
+    # scale 1, so rounding noise around its mean is not blown up
+    x_mean, x_std = X.astype(np.float64).mean(0), X.astype(np.float64).std(0)
+    x_std = np.where(x_std > 1e-6, x_std, 1.0)
     y_scale = float(Y.std())
     norm = Normaliser(x_mean, x_std, y_scale)
     net = MLP(SIZES, seed=seed)
@@ -127,8 +122,7 @@ def train(X, Y, iters=4000, batch=4096, seed=0, log=print, Xv=None, Yv=None, eve
     t0 = time.time()
     for it in range(iters):
         idx = rng.integers(0, len(Xtr), B)
-        xb = scale_shift(Xtr[idx], rng) if augment else Xtr[idx]
-        net.set_input(norm.x(xb), train=True)
+        net.set_input(norm.x(Xtr[idx]), train=True)
         di = torch.from_numpy(idx).to(net.dev)
         net.train_step(B, Yd[di], None if Wd is None else Wd[di])
         if (it + 1) % every == 0 or it == iters - 1:
@@ -162,13 +156,19 @@ def error_summary(err):
 
 def save(path, net, norm):
     st = net.state()
-    np.savez(path, x_mean=norm.x_mean, x_std=norm.x_std, y_scale=norm.y_scale, sizes=np.array(net.sizes), **st)
+    np.savez(path, x_mean=norm.x_mean, x_std=norm.x_std, y_scale=norm.y_scale, sizes=np.array(net.sizes),
+             inputs=np.array(INPUTS), **st)
 
 
 def load(path):
+    """Network and normalisation saved by ``save``; weights trained on another input layout are
+    refused (their inputs would be misread)."""
     from nofluidx3d_amd.surrogate.mlp import MLP
 
     d = np.load(path)
+    got = tuple(str(v) for v in d["inputs"]) if "inputs" in d.files else ("(unnamed, before the ratio input)",)
+    if got != INPUTS:
+        raise ValueError(f"{path}: weights trained on inputs {got}, this build feeds {INPUTS}")
     net = MLP(tuple(int(v) for v in d["sizes"]), seed=0)
     net.load_state({k: d[k] for k in d.files if k[0] in "Wb" and k[1:].isdigit()})
     return net, Normaliser(d["x_mean"], d["x_std"], float(d["y_scale"]))

@@ -6,7 +6,7 @@ import numpy as np
 
 from nofluidx3d_amd.batch import CellSpec
 from nofluidx3d_amd.surrogate import fit
-from nofluidx3d_amd.surrogate.train import Normaliser, scale_shift
+from nofluidx3d_amd.surrogate.train import Normaliser
 
 NC = 256
 S = np.linspace(0.0, 1.0, NC)
@@ -41,7 +41,8 @@ class FakeNet:
 
     def predict(self, Xn):
         X = np.asarray(Xn) * self.norm.x_std + self.norm.x_mean
-        return shape(X[:, 1], X[:, 2] - X[:, 0], 10 ** X[:, 3], 10 ** X[:, 4], 10 ** X[:, 5]) / self.norm.y_scale
+        # surrogate_inputs layout: log10 viscosity/E, nu, log10 tip, log10 speed, log10 R, log10 spacing
+        return shape(X[:, 1], X[:, 0], 10 ** X[:, 2], 10 ** X[:, 3], 10 ** X[:, 4]) / self.norm.y_scale
 
 
 def cells(n, seed=3):
@@ -88,13 +89,13 @@ def test_symmetry_helpers_keep_the_box_and_the_ratio():
     assert np.all(c >= fit.LO) and np.all(c <= fit.HI)
     inside = np.all((th >= fit.LO) & (th <= fit.HI), axis=1)
     assert np.array_equal(c[inside], th[inside])
-    X = np.stack([rng.uniform(2, 4, 300), rng.uniform(0.3, 0.49, 300), rng.uniform(0, 3, 300)] +
-                 [rng.uniform(0, 1, 300)] * 3, 1)
-    Xs = scale_shift(X, rng)
-    assert np.allclose(Xs[:, 2] - Xs[:, 0], X[:, 2] - X[:, 0])
-    assert np.all(Xs[:, 0] >= 2 - 1e-12) and np.all(Xs[:, 0] <= 4 + 1e-12)
-    assert np.all(Xs[:, 2] >= -1e-12) and np.all(Xs[:, 2] <= 3 + 1e-12)
-    assert np.array_equal(Xs[:, 1:2], X[:, 1:2]) and np.array_equal(Xs[:, 3:], X[:, 3:])
+    # the network's inputs see E and the viscosity through their ratio only (the exact symmetry)
+    cs = cells(6)
+    th = np.array([[np.log10(c.E_si), c.nu, np.log10(c.viscosity_si)] for c in cs])
+    shifted = th + np.array([0.37, 0.0, 0.37])
+    assert np.allclose(fit.features(th, cs), fit.features(shifted, cs), rtol=0, atol=1e-12)
+    X = fit.features(th, cs)
+    assert np.allclose(X[:, 0], th[:, 2] - th[:, 0]) and np.allclose(X[:, 5], np.log10([c.spacing_um for c in cs]))
 
 
 def test_stabilize_enforces_the_relaxation_limit():
@@ -113,10 +114,9 @@ def test_surrogate_fit_recovers_parameters_with_an_exact_network():
     cs = cells(24)
     bs = FakeBatch()
     targets = fit.simulate(bs, cs)
-    X = np.stack([np.log10([c.E_si for c in cs]), [c.nu for c in cs], np.log10([c.viscosity_si for c in cs]),
-                  np.log10([c.tip_radius_um for c in cs]), np.log10([c.speed_um_s for c in cs]),
-                  np.log10([c.radius_um for c in cs])], 1)
-    norm = Normaliser(X.mean(0), X.std(0) + 1e-12, 0.01)
+    th = np.array([[np.log10(c.E_si), c.nu, np.log10(c.viscosity_si)] for c in cs])
+    X = fit.features(th, cs)
+    norm = Normaliser(X.mean(0), np.where(X.std(0) > 1e-6, X.std(0), 1.0), 0.01)
     r = fit.fit_surrogate(FakeNet(norm), norm, targets, blind(cs), bs, rounds=8, ncand=64, ntop=2)
     assert np.median(r["sim_err"]) < 2e-3, np.median(r["sim_err"])
     assert r["simulations_per_target"] == 3
@@ -144,8 +144,8 @@ class BiasedNet(FakeNet):
 
     def predict(self, Xn):
         X = np.asarray(Xn) * self.norm.x_std + self.norm.x_mean
-        base = shape(X[:, 1], X[:, 2] - X[:, 0], 10 ** X[:, 3], 10 ** X[:, 4], 10 ** X[:, 5])
-        return base * (1.0 + 0.08 * np.sin(6 * S[None, :] + 7.0 * X[:, 5:6])) / self.norm.y_scale
+        base = shape(X[:, 1], X[:, 0], 10 ** X[:, 2], 10 ** X[:, 3], 10 ** X[:, 4])
+        return base * (1.0 + 0.08 * np.sin(6 * S[None, :] + 7.0 * X[:, 4:5])) / self.norm.y_scale
 
 
 def test_corrected_round_beats_a_second_candidate_at_equal_simulations():
@@ -155,10 +155,9 @@ def test_corrected_round_beats_a_second_candidate_at_equal_simulations():
     cs = cells(32, seed=5)
     bs = FakeBatch()
     targets = fit.simulate(bs, cs)
-    X = np.stack([np.log10([c.E_si for c in cs]), [c.nu for c in cs], np.log10([c.viscosity_si for c in cs]),
-                  np.log10([c.tip_radius_um for c in cs]), np.log10([c.speed_um_s for c in cs]),
-                  np.log10([c.radius_um for c in cs])], 1)
-    norm = Normaliser(X.mean(0), X.std(0) + 1e-12, 0.01)
+    th = np.array([[np.log10(c.E_si), c.nu, np.log10(c.viscosity_si)] for c in cs])
+    X = fit.features(th, cs)
+    norm = Normaliser(X.mean(0), np.where(X.std(0) > 1e-6, X.std(0), 1.0), 0.01)
     two = fit.fit_surrogate(BiasedNet(norm), norm, targets, blind(cs), bs, ntop=2, confirm=False)
     cor = fit.fit_surrogate(BiasedNet(norm), norm, targets, blind(cs), bs, ntop=1, corrections=1, confirm=False)
     assert two["simulations_per_target"] == cor["simulations_per_target"] == 2
