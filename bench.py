@@ -249,6 +249,7 @@ def bench_surrogate(weights, n_targets=512, log=log):
     out["fit"] = {"targets": n_targets, "fits_per_s": n_targets / r["total_s"], "batch_wall_s": r["total_s"],
                   "simulations_per_target": r["simulations_per_target"],
                   "confirmed_rel_l2": fit_bench.err_stats(r["sim_err"]),
+                  "confirmed_rel_l2_vs_noise_free_curve": fit_bench.err_stats(fit.rel_err(r["curve"], clean)),
                   "noise_floor_rel_l2_median": float(np.median(fit.rel_err(clean, noisy))),
                   "param_errors": fit_bench.param_errors(r["theta"], fit_bench.theta_true(cells))}
     del bs
