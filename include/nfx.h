@@ -337,12 +337,16 @@ typedef struct {
     float bc2;         /* 1 / sqrt(1 - beta2^t): w -= lr_t m / (sqrt(v) bc2 + eps) */
     float beta1, beta2, eps;
     float grad_scale;  /* multiplies the gradient before the update */
+    float* workspace;  /* FP32 [split_k x M x N], split_k > 1 only */
+    int32_t split_k;   /* 0 or 1: one pass over K; S > 1 (S <= 64, K % (32 S) == 0): K in S slices
+                          accumulated into the workspace, then summed in slice order and the
+                          epilogue applied (for GEMMs with a small M x N and a long K) */
 } nfx_mlp_gemm_args;
 
 int nfx_mlp_gemm(nfx_ctx* ctx, const nfx_mlp_gemm_args* args);
 /* row-weighted mean squared error sum_ij w_i (Y - T)^2 / (M N) over M x N FP32 (accumulated into
  * *loss_sum, FP64, device) and its gradient 2 w_i (Y - T) / (M N) as BF16 in [M x N] (dZ) and
- * [N x M] (dZT).  row_weight (M FP32) is nullable: w_i = 1. */
+ * [N x M] (dZT).  row_weight (M FP32) is nullable: w_i = 1.  M and N are multiples of 32. */
 int nfx_mlp_mse(nfx_ctx* ctx, const float* Y, const float* target, const float* row_weight, int32_t M, int32_t N,
                 void* dZ, void* dZT, double* loss_sum);
 /* bias gradient = row sums of dZT [N x M] (BF16), then Adam on the FP32 bias b with moments m, v */

@@ -146,6 +146,8 @@ class MlpGemm(ctypes.Structure):
         ("beta2", ctypes.c_float),
         ("eps", ctypes.c_float),
         ("grad_scale", ctypes.c_float),
+        ("workspace", c_vp),
+        ("split_k", c_i32),
     ]
 
 

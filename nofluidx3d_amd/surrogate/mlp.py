@@ -9,8 +9,9 @@ Layout (all device tensors):
 
 One training step (``train_step``): forward (FWD_GELU epilogues, STORE for the output layer),
 MSE loss (optionally row-weighted) and its gradient, then for l = L..1: backward-data GEMM for layer l-1 with the current
-BF16 weights (BWD_GELU epilogue), weight-gradient GEMM with the fused Adam update (ADAM epilogue),
-bias gradient + Adam.  Weight initialisation follows torch.nn.Linear (U(-1/sqrt(fan_in),
+BF16 weights (BWD_GELU epilogue), weight-gradient GEMM with the fused Adam update (ADAM epilogue;
+split over K = batch into ``split_k`` slices, since its out x in output is too small to fill the
+chip), bias gradient + Adam.  Weight initialisation follows torch.nn.Linear (U(-1/sqrt(fan_in),
 1/sqrt(fan_in)) for weights and biases) from a seeded generator.
 """
 
@@ -28,6 +29,17 @@ LR, BETA1, BETA2, EPS = 1e-3, 0.9, 0.999, 1e-8
 
 def _pad32(n):
     return -(-n // 32) * 32
+
+
+def split_k(M, N, K, min_groups=512, max_split=64):
+    """K slices for a GEMM of M x N outputs over K: doubled until the grid has ``min_groups``
+    workgroups of 128 x 32 outputs (the weight-gradient GEMMs have few outputs and K = batch),
+    each slice a multiple of 32 long"""
+    groups = (M // 128) * (N // 32)
+    s = 1
+    while s < max_split and groups * s < min_groups and K % (64 * s) == 0:
+        s *= 2
+    return s
 
 
 class MLP:
@@ -80,6 +92,10 @@ class MLP:
                  "H": [torch.empty(B, n, dtype=bf, device=self.dev) for n in self.sizes[1:-1]],
                  "Y": torch.empty(B, self.sizes[-1], dtype=f32, device=self.dev)}
             if train:
+                # weight-gradient GEMMs (M = out, N = in_pad, K = B) in K slices, one shared workspace
+                d["split"] = [split_k(n_out, n_in, B) for n_out, n_in in zip(self.sizes[1:], self.in_pad)]
+                ws = max(s * n_out * n_in for s, n_out, n_in in zip(d["split"], self.sizes[1:], self.in_pad))
+                d["ws"] = torch.empty(ws, dtype=f32, device=self.dev)
                 d["Z"] = [torch.empty(B, n, dtype=f32, device=self.dev) for n in self.sizes[1:-1]]
                 d["HT"] = [torch.empty(n, B, dtype=bf, device=self.dev) for n in self.sizes[1:-1]]
                 d["dZ"] = [torch.empty(B, n, dtype=bf, device=self.dev) for n in self.sizes[1:]]
@@ -89,12 +105,12 @@ class MLP:
 
     @staticmethod
     def _gemm(epi, M, N, K, A, lda, B, ldb, bias=None, C=None, Z=None, H=None, HT=None, m=None, v=None, lr_t=0.0,
-              bc2=1.0, grad_scale=1.0):
+              bc2=1.0, grad_scale=1.0, split=1, ws=None):
         from nofluidx3d_amd import openCL
 
         p = lambda t: None if t is None else t.data_ptr()
         g = _lib.MlpGemm(epi, M, N, K, p(A), lda, p(B), ldb, p(bias), p(C), p(Z), p(H), p(HT), p(m), p(v),
-                         lr_t, bc2, BETA1, BETA2, EPS, grad_scale)
+                         lr_t, bc2, BETA1, BETA2, EPS, grad_scale, p(ws) if split > 1 else None, split)
         _lib.call("nfx_mlp_gemm", openCL.handle(), ctypes.byref(g))
 
     def set_input(self, X, train=False):
@@ -173,15 +189,18 @@ class MLP:
             if l > 0:  # gradient into layer l-1 with the current weights, before they change
                 self._gemm(_lib.NFX_MLP_BWD_GELU, B, K_in, N_out, bufs["dZ"][l], N_out, self.WbT[l], N_out,
                            Z=bufs["Z"][l - 1], H=bufs["dZ"][l - 1], HT=bufs["dZT"][l - 1])
+            S = bufs["split"][l]
             if apply_adam:
                 self._gemm(_lib.NFX_MLP_ADAM, N_out, K_in, B, bufs["dZT"][l], B, prev_HT, B, C=self.W[l],
-                           H=self.Wb[l], HT=self.WbT[l], m=self.mW[l], v=self.vW[l], lr_t=lr_t, bc2=bc2)
+                           H=self.Wb[l], HT=self.WbT[l], m=self.mW[l], v=self.vW[l], lr_t=lr_t, bc2=bc2, split=S,
+                           ws=bufs["ws"])
                 _lib.call("nfx_mlp_bias_adam", openCL.handle(), bufs["dZT"][l].data_ptr(), B, N_out,
                           self.b[l].data_ptr(), self.mb[l].data_ptr(), self.vb[l].data_ptr(), lr_t, bc2, BETA1,
                           BETA2, EPS, 1.0)
             else:
                 gw = torch.empty(N_out, K_in, dtype=torch.float32, device=self.dev)
-                self._gemm(_lib.NFX_MLP_STORE, N_out, K_in, B, bufs["dZT"][l], B, prev_HT, B, C=gw)
+                self._gemm(_lib.NFX_MLP_STORE, N_out, K_in, B, bufs["dZT"][l], B, prev_HT, B, C=gw, split=S,
+                           ws=bufs["ws"])
                 grads_w[l] = gw
                 grads_b[l] = bufs["dZT"][l].float().sum(1)
         return grads_w, grads_b

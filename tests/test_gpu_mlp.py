@@ -74,6 +74,45 @@ def test_gemm_nt_matches_fp64_product(ctx, M, N, K):
     assert torch.equal(dZ.t().contiguous(), dZT)
 
 
+@pytest.mark.parametrize("M,N,K,S", [(128, 32, 4096, 64), (512, 512, 4096, 8), (256, 64, 1024, 4)])
+def test_split_k_matches_one_pass(ctx, M, N, K, S):
+    """split-K (K slices accumulated into a workspace, summed in slice order, then the epilogue)
+    against the one-pass GEMM: STORE within FP32 rounding of the FP64 product; the fused Adam
+    update of the same master weights within 1e-6, its BF16 copies exact transposes"""
+    import math
+
+    import torch
+
+    from nofluidx3d_amd import _lib
+    from nofluidx3d_amd.surrogate.mlp import MLP
+
+    A, B = RNG.normal(size=(M, K)), RNG.normal(size=(N, K))
+    Ad, Bd = bf(A).cuda(), bf(B).cuda()
+    ws = torch.empty(S * M * N, dtype=torch.float32, device="cuda:0")
+    ref = bf_round(A) @ bf_round(B).T
+    outs = []
+    for split in (1, S):
+        C = torch.empty(M, N, dtype=torch.float32, device="cuda:0")
+        MLP._gemm(_lib.NFX_MLP_STORE, M, N, K, Ad, K, Bd, K, C=C, split=split, ws=ws)
+        outs.append(C.cpu().numpy())
+        assert np.max(np.abs(outs[-1] - ref)) < 1e-5 * np.max(np.abs(ref)) + 1e-4, split
+    assert not np.array_equal(outs[0], np.zeros_like(outs[0]))
+    W0 = RNG.normal(size=(M, N)).astype(np.float32)
+    got = []
+    for split in (1, S):
+        W = torch.tensor(W0, device="cuda:0")
+        m, v = torch.zeros_like(W), torch.zeros_like(W)
+        H = torch.empty(M, N, dtype=torch.bfloat16, device="cuda:0")
+        HT = torch.empty(N, M, dtype=torch.bfloat16, device="cuda:0")
+        MLP._gemm(_lib.NFX_MLP_ADAM, M, N, K, Ad, K, Bd, K, C=W, H=H, HT=HT, m=m, v=v, lr_t=1e-2,
+                  bc2=1.0 / math.sqrt(1.0 - 0.999), split=split, ws=ws)
+        torch.cuda.synchronize()
+        assert torch.equal(H.t().contiguous(), HT) and torch.equal(H, W.to(torch.bfloat16))
+        got.append(W.cpu().numpy())
+    assert np.max(np.abs(got[0] - W0)) > 5e-4  # the step moved the weights (about lr per entry)
+    assert np.max(np.abs(got[0] - got[1])) < 1e-6
+
+
 def small_net():
     from nofluidx3d_amd.surrogate.mlp import MLP
 

@@ -28,22 +28,34 @@ def test_library_exports_every_declared_symbol():
         assert s in _lib.SIGNATURES, f"no ctypes signature for {s}"
 
 
-def test_header_structs_match_ctypes():
-    """field order of nfx_plan in include/nfx.h equals the ctypes Plan."""
-    from nofluidx3d_amd import _lib
-
-    text = open(os.path.join(ROOT, "include", "nfx.h")).read()
-    body = text[text.index("typedef struct {\n    int64_t N, T;"):text.index("} nfx_plan;")]
+def header_struct_fields(text, name):
+    """field names of ``typedef struct { ... } name;`` in the header text"""
+    end = text.index(f"}} {name};")
+    body = text[text.rindex("typedef struct {", 0, end) + len("typedef struct {"):end]
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     names = []
     for decl in body.split(";"):
-        decl = decl.strip().replace("typedef struct {", "").strip()
-        if not decl:
+        parts = [w for w in decl.replace("*", " ").split() if w not in ("const", "struct")]
+        if not parts:
             continue
-        parts = [w for w in decl.replace("*", " ").split() if w != "const"]
         for n in " ".join(parts[1:]).split(","):
-            names.append(n.strip())
-    assert names == [f[0] for f in _lib.Plan._fields_]
+            names.append(re.sub(r"\[.*\]", "", n).strip())
+    return names
+
+
+def test_header_structs_match_ctypes():
+    """field order of every struct in include/nfx.h equals its ctypes mirror (and the nfx_cell
+    record's numpy dtype)"""
+    from nofluidx3d_amd import _lib
+    from nofluidx3d_amd.batch import CELL_DTYPE
+
+    text = open(os.path.join(ROOT, "include", "nfx.h")).read()
+    mirrors = {"nfx_elastic": _lib.Elastic, "nfx_viscoplastic": _lib.Viscoplastic, "nfx_plan": _lib.Plan,
+               "nfx_batch": _lib.Batch, "nfx_viscoelastic": _lib.Viscoelastic, "nfx_mlp_gemm_args": _lib.MlpGemm}
+    assert set(re.findall(r"^} (\w+);", text, flags=re.M)) == set(mirrors) | {"nfx_cell"}
+    for name, cls in mirrors.items():
+        assert header_struct_fields(text, name) == [f[0] for f in cls._fields_], name
+    assert header_struct_fields(text, "nfx_cell") == list(CELL_DTYPE.names)
 
 
 def test_node_csr_host_matches_numpy():
