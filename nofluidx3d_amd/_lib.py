@@ -210,13 +210,6 @@ _lib = None
 _extra_signatures = []
 
 
-def register_signatures(sigs: dict):
-    """Other modules (batched stepper, surrogate) add their entry points here."""
-    SIGNATURES.update(sigs)
-    if _lib is not None:
-        _bind(_lib, sigs)
-
-
 def _bind(L, sigs):
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)

@@ -110,10 +110,3 @@ def write_mesh(cfg, points=None, tets=None):
     os.makedirs(os.path.dirname(cfg["CELL"]["InitVTK"]), exist_ok=True)
     vtk.from_tetra(points * 1e-6, tets).save(cfg["CELL"]["InitVTK"])
     return points, tets
-
-
-def scaled_points(cfg, points_um):
-    """Positions in simulation units as TetraCell produces them (recentre on node 0, scale so that
-    |x(node 1)| = RadiusSIM)."""
-    p = np.asarray(points_um, np.float64) - points_um[0]
-    return p * (cfg["CELL"]["RadiusSIM"] / abs(p[1, 0]))

@@ -64,12 +64,7 @@ __device__ __forceinline__ void st_g(gptr<T> p, T v) {
     if constexpr (NT) __builtin_nontemporal_store(v, p);
     else *p = v;
 }
-// the same, with the policy a compile-time choice of the caller
-template <bool NT, class T>
-__device__ __forceinline__ T ld_s(const T* p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
+// a store through a plain pointer, with the policy a compile-time choice of the caller
 template <bool NT, class T>
 __device__ __forceinline__ void st_s(T* p, T v) {
     if constexpr (NT) __builtin_nontemporal_store(v, p);
@@ -124,12 +119,6 @@ __device__ __forceinline__ S6 aat(const M9& A) {
     s.yz = A.m[1][0] * A.m[2][0] + A.m[1][1] * A.m[2][1] + A.m[1][2] * A.m[2][2];
     s.xz = A.m[0][0] * A.m[2][0] + A.m[0][1] * A.m[2][1] + A.m[0][2] * A.m[2][2];
     return s;
-}
-
-__device__ __forceinline__ float det9(const M9& A) {
-    return A.m[0][0] * (A.m[1][1] * A.m[2][2] - A.m[1][2] * A.m[2][1]) -
-           A.m[0][1] * (A.m[1][0] * A.m[2][2] - A.m[1][2] * A.m[2][0]) +
-           A.m[0][2] * (A.m[1][0] * A.m[2][1] - A.m[1][1] * A.m[2][0]);
 }
 
 // inverse by cofactors / det

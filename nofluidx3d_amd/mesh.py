@@ -112,15 +112,3 @@ def generate_sphere_mesh(
         tk = _morton_keys(P[tets].mean(1))
         tets = tets[np.argsort(tk, kind="stable")]
     return np.ascontiguousarray(P), np.ascontiguousarray(tets)
-
-
-def mesh_stats(points: np.ndarray, tets: np.ndarray) -> dict:
-    q = tetra_quality(points, tets)
-    v = signed_volumes(points, tets)
-    return {
-        "nodes": int(len(points)),
-        "tetra": int(len(tets)),
-        "min_quality": float(q.min()),
-        "min_volume": float(v.min()),
-        "volume": float(v.sum()),
-    }

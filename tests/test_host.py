@@ -187,6 +187,22 @@ def test_tetracell_recenter_scale_and_reference_quantities(tmp_path):
     assert np.allclose(c.mesh.cell_data["volumeChange"], 1.1**3 - 1)
 
 
+def test_config_mesh_written_to_init_vtk_reads_back_through_tetracell(tmp_path):
+    """configs.write_mesh puts a config's seeded mesh (micrometres) at CELL.InitVTK in metres;
+    TetraCell reads it back recentred on node 0 and scaled to RadiusSIM."""
+    from nofluidx3d_amd import configs
+    from nofluidx3d_amd.TetraCell import TetraCell
+
+    cfg = configs.small_cell(outdir=str(tmp_path) + "/", steps=10, frames=1)
+    cfg["MESH"] = {"radius_um": 3.0, "spacing_um": 0.75, "seed": 4}
+    cfg["CELL"]["InitVTK"] = str(tmp_path / "mesh" / "cell.vtk")
+    P, T = configs.write_mesh(cfg)
+    c = TetraCell(cfg["CELL"]["InitVTK"], radius=cfg["CELL"]["RadiusSIM"])
+    assert np.array_equal(c.tetras, T)
+    scale = cfg["CELL"]["RadiusSIM"] / abs(P[1, 0] - P[0, 0])
+    assert np.allclose(c.points, (P - P[0]) * scale, rtol=0, atol=1e-9 * cfg["CELL"]["RadiusSIM"])
+
+
 def test_kernel_builder_defines_render_like_the_reference():
     from nofluidx3d_amd.KernelBuilder import KernelBuilder
 
