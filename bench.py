@@ -125,7 +125,7 @@ def cpu_baseline(P, T, cfg, material, target_s=12.0):
     else:
         prm.model, prm.E, prm.nu, prm.ratio = O.MR, E, nu, ratio
     prm.substrate, prm.substrate_k = 1, 0.1
-    prm.wall = -(cfg["CELL"]["RadiusSIM"] + cfg["SubstrateDistance"])
+    prm.wall = -(cfg["CELL"]["RadiusSIM"] + cfg.get("SubstrateDistance", cfg["InitialDistance"]))
     R = cfg["SphereRadiusSI"] / u["L0"]
     prm.indenter, prm.ind_k, prm.ind_radius = 1, 0.1, R
     prm.ind_start = cfg["CELL"]["RadiusSIM"] + R + cfg["InitialDistance"]

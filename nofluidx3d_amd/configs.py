@@ -12,10 +12,14 @@ scaling (a quasi-static reading): ``RhoSI`` is chosen so that T0 = (simulated ti
                 Mooney-Rivlin C10 150 Pa, C01 50 Pa, K 5 kPa (mu1 300, mu2 100, ratio 0.75,
                 E 1168.8 Pa, nu 0.4610) with the Maxwell-type viscoplastic mapping
                 (m = 1, n = 0, Y = 3 eta * 1/s, rate = T0 per step, yield offset 200 Pa);
-                tip R 5 um, 3 um at 5 um/s then a 2 s hold; T0 = 1e-5 s.
+                tip R 5 um, 3 um at 5 um/s then a 2 s hold; T0 = 1e-5 s; the wall touches the
+                cell's bottom (the reference's placement, -(RadiusSIM + InitialDistance)).
 The reference's vertical axis is y; "bottom z < -7.5 um" maps to y.  ``InitialDistance`` sets
-both the tip start and the wall in the reference (``Simulation.py:107,163``); the wall offset is
-given separately here (``SubstrateDistance``, read by ``setInteractionSubstrate``).
+both the tip start and the wall in the reference (``Simulation.py:107,163``); the small cell's wall
+offset is given separately (``SubstrateDistance``, read by ``setInteractionSubstrate``).  The
+large cell keeps the reference's wall: 0.5 um inside its bottom is 2.5 of its 0.2 um lattice
+spacings, and the sudden compression of that cap makes the explicit step of this nearly
+incompressible material diverge within ten steps (the FP64 oracle does the same).
 """
 
 from __future__ import annotations
@@ -75,7 +79,6 @@ def large_cell(outdir="/tmp/nfx_large/", radius_sim=50.0, T0=1e-5):
         "VTKFramesTotal": 260,
         "OutputDir": outdir,
         "InitialDistance": 0.0,
-        "SubstrateDistance": -0.5e-6 / L0,
         "MoveTimeSI": move,
         "VelocitySI": 5e-6,
         "SphereRadiusSI": 5e-6,
