@@ -156,16 +156,17 @@ def cpu_baseline(P, T, cfg, material, target_s=12.0):
 
 def bench_batched(steps=10000, ncells=512, log=log):
     """The batched workload: 512 seeded cells (seeds 1000+i), 10k steps each, one 256-point curve per
-    cell; tile size from the autotuner.  Returns the JSON block."""
+    cell; tile size, cache policy and cell grouping from the autotuner.  Returns the JSON block."""
     from nofluidx3d_amd.autotune import tune_batch
-    from nofluidx3d_amd.batch import BatchSimulation, make_meshes, sample_batch
+    from nofluidx3d_amd.batch import CellGroups, make_meshes, sample_batch
 
     specs, stats = sample_batch(ncells, seed0=1000, steps=steps)
     t0 = time.time()
     meshes = make_meshes(specs)
     tmesh = time.time() - t0
     best, tuned = tune_batch(specs, meshes, steps, use_cache=False, log=log)
-    b = BatchSimulation(specs, meshes, steps=steps, ncurve=256, tile_k=best["tile_k"], stream_nt=best["stream_nt"])
+    b = CellGroups(specs, meshes, max_tetra=best.get("group_tetra") or None, steps=steps, ncurve=256,
+                   tile_k=best["tile_k"], stream_nt=best["stream_nt"])
     b.advance(100)  # includes the graph capture
     b.sync()
     t0 = time.perf_counter()
@@ -190,7 +191,9 @@ def bench_batched(steps=10000, ncells=512, log=log):
         "tetra_per_cell": {"min": int(tet.min()), "median": int(np.median(tet)), "max": int(tet.max())},
         "sampling_redraws": stats,
         "tile_k": best["tile_k"],
-        "stream_nt": best["stream_nt"],
+        "stream_nt": b.stream_nt if len(b.groups) > 1 else b.stream_nt[0],
+        "cell_groups": len(b.groups),
+        "group_tetra": int(best.get("group_tetra") or 0),
         "autotune_ms_per_step": tuned,
         "algorithmic_bytes_per_step": int(step_bytes),
         "ms_per_step": ms,

@@ -1,7 +1,7 @@
 """Launch-shape autotuner for the tiled force pass (single cells and batches).
 
-Candidates are timed on the device with HIP events over a few steps (``nfx_time_passes`` /
-``nfx_batch_time_passes``); the fastest wins and is cached per problem signature in a JSON file
+Candidates are timed over a few steps (a single cell: the force and node passes with HIP events,
+``nfx_time_passes``; a batch: graph-replayed steps, launch costs included); the fastest wins and is cached per problem signature in a JSON file
 (``$NFX_TUNE_CACHE`` or ``~/.cache/nofluidx3d_amd/tune.json``).  Every candidate computes the same
 physics; tile size only changes the floating-point summation order, by rounding.
 
@@ -10,7 +10,10 @@ Tunables:
 - ``stream_nt``: cache policy of the force pass's per-tetra streams (state, tile tables): 1 =
   nontemporal, 0 = default.  A state that fits MI355X's 256 MB memory-side cache is served from it
   from step to step under the default policy; a larger one streams faster past it;
-- ``graph_steps``: steps per captured hipGraph replay block (the library captures at most 100).
+- ``graph_steps``: steps per captured hipGraph replay block (the library captures at most 100);
+- ``group_tetra`` (batches): run the cells as consecutive groups of at most this many tetra, each
+  group all its steps before the next (``batch.CellGroups``), so that a group's working set stays
+  in the memory-side cache from step to step; 0 = one batch.
 """
 
 from __future__ import annotations
@@ -23,11 +26,20 @@ TILE_CANDIDATES = (256, 512)
 STREAM_CANDIDATES = (0, 1)
 MEMORY_SIDE_CACHE_BYTES = 256 << 20
 STATE_BYTES_PER_TETRA = 48  # force pass streams: 9-float state, plastic strain or volume, 4 slots
+# a batch's per-step working set per tetra: symmetric state 24 B, corner records 8 B, tile-node ids,
+# partial slots and positions (0.42 tile-nodes per tetra) and the nodes' state (about 0.16 nodes per
+# tetra), an estimate the autotuner checks by timing
+RESIDENT_BYTES_PER_TETRA = 56
 
 
 def default_stream_nt(tetra):
     """the cache policy to use untuned: nontemporal once the streamed state exceeds the cache"""
     return int(STATE_BYTES_PER_TETRA * int(tetra) > MEMORY_SIDE_CACHE_BYTES)
+
+
+def group_tetra():
+    """tetra per cell group whose per-step working set fits the memory-side cache"""
+    return MEMORY_SIDE_CACHE_BYTES // RESIDENT_BYTES_PER_TETRA
 
 
 def _label(K, nt):
@@ -59,14 +71,32 @@ def _store(key, value):
         pass
 
 
-def tune_batch(specs, meshes, steps, ncurve=256, nrep=10, use_cache=True, log=None):
-    """Pick ``tile_k`` and ``stream_nt`` for a batch by timing force+node+curve passes per
-    candidate.  Returns ({"tile_k": K, "stream_nt": nt}, {label: ms_per_step}), labels "256",
-    "256nt", ..."""
-    from nofluidx3d_amd.batch import BatchSimulation
+def _time_steps(b, nsteps):
+    """ms per step of ``nsteps`` graph-replayed steps of a batch, or of cell groups summed over the
+    groups: each group warms up (100 steps, capturing its graph) right before its own timed steps,
+    as it runs all its steps in one call in use (the library caches one batch graph at a time)"""
+    total = 0.0
+    for part in getattr(b, "groups", [b]):
+        part.advance(100)
+        part.sync()
+        t0 = time.perf_counter()
+        part.advance(nsteps)
+        part.sync()
+        total += (time.perf_counter() - t0) / nsteps * 1e3
+    return total
+
+
+def tune_batch(specs, meshes, steps, ncurve=256, nsteps=300, use_cache=True, log=None):
+    """Pick ``tile_k``, ``stream_nt`` and ``group_tetra`` for a population of cells by timing
+    ``nsteps`` graph-replayed steps per candidate (the way the population then runs, launch costs
+    included): one batch with each tile size and cache policy, and, when the population exceeds one
+    cache-sized group, the cell groups of ``group_tetra()`` with each tile size (the default policy
+    per group).  Returns ({"tile_k": K, "stream_nt": nt or None, "group_tetra": G or 0},
+    {label: ms_per_step}), labels "256", "256nt", ..., "256g4" (4 groups)."""
+    from nofluidx3d_amd.batch import BatchSimulation, CellGroups
 
     tetra = sum(len(m[1]) for m in meshes)
-    key = f"batch2:{len(specs)}:{tetra}:{specs[0].viscosity_si is not None}"
+    key = f"batch4:{len(specs)}:{tetra}:{specs[0].viscosity_si is not None}"
     if use_cache:
         hit = _load().get(key)
         if hit:
@@ -81,13 +111,25 @@ def tune_batch(specs, meshes, steps, ncurve=256, nrep=10, use_cache=True, log=No
             continue
         for nt in STREAM_CANDIDATES:
             b.plan.stream_nt = nt
-            b.time_passes(2)
-            f, n, c = b.time_passes(nrep)
             lab = _label(K, nt)
-            times[lab], choice[lab] = f + n + c, {"tile_k": K, "stream_nt": nt}
+            times[lab], choice[lab] = _time_steps(b, nsteps), {"tile_k": K, "stream_nt": nt, "group_tetra": 0}
             if log:
-                log(f"{lab}: force {f:.4f} ms, node {n:.4f} ms, curve {c:.4f} ms")
+                log(f"{lab}: {times[lab]:.4f} ms/step")
         del b
+    G = group_tetra()
+    if tetra > G:
+        for K in TILE_CANDIDATES:
+            try:
+                cg = CellGroups(specs, meshes, max_tetra=G, steps=steps, ncurve=ncurve, tile_k=K)
+            except Exception as e:
+                if log:
+                    log(f"groups, tile_k {K}: {e}")
+                continue
+            lab = f"{K}g{len(cg.groups)}"
+            times[lab], choice[lab] = _time_steps(cg, nsteps), {"tile_k": K, "stream_nt": None, "group_tetra": G}
+            if log:
+                log(f"{lab}: {times[lab]:.4f} ms/step")
+            del cg
     best = choice[min(times, key=times.get)]
     _store(key, {"best": best, "ms": times, "when": time.time()})
     return best, times

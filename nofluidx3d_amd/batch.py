@@ -529,6 +529,100 @@ class BatchSimulation:
         return cfg
 
 
+
+def cell_groups(meshes, max_tetra):
+    """Consecutive cell ranges [a, b) whose tetra counts sum to at most ``max_tetra`` (a cell larger
+    than that forms a group of its own)."""
+    ranges, a, acc = [], 0, 0
+    for i, (_, T) in enumerate(meshes):
+        if i > a and acc + len(T) > max_tetra:
+            ranges.append((a, i))
+            a, acc = i, 0
+        acc += len(T)
+    ranges.append((a, len(meshes)))
+    return ranges
+
+
+class CellGroups:
+    """A population of cells advanced group by group: consecutive cells whose tetra counts sum to
+    at most ``max_tetra`` form one BatchSimulation, and each group runs its steps before the next
+    one starts.  Cells are independent and tiles never span two cells, so every cell's trajectory
+    is bitwise the one it has in a single batch.  A group whose per-step working set fits the
+    256 MB memory-side cache is served from it from step to step under the default cache policy
+    (``autotune.group_tetra``), which beats streaming the whole population past it each step.
+    ``max_tetra=None``: one group (a single batch).  Other keyword arguments go to every group's
+    BatchSimulation; ``stream_nt`` defaults per group (the default policy for a group that fits).
+
+    Offers the BatchSimulation interface the benchmark, the surrogate's data generation and the
+    fitter use: advance, sync, reset, curves_sim, curves_normalized, features, positions,
+    single_cell_config, time_passes, and the per-cell specs, units, radius_sim, cells_host."""
+
+    def __init__(self, specs, meshes, max_tetra=None, **kw):
+        self.ranges = cell_groups(meshes, max_tetra) if max_tetra else [(0, len(specs))]
+        self.groups = [BatchSimulation(specs[a:b], meshes[a:b], **kw) for a, b in self.ranges]
+        g0 = self.groups[0]
+        self.steps, self.ncurve, self.tile_k, self.sym_state = g0.steps, g0.ncurve, g0.tile_k, g0.sym_state
+        self.ncells = len(specs)
+        self.T_real = sum(g.T_real for g in self.groups)
+        self.N = sum(g.N for g in self.groups)
+        self.stream_nt = [int(g.plan.stream_nt) for g in self.groups]
+        self._collect()
+
+    def _collect(self):
+        self.specs = [s for g in self.groups for s in g.specs]
+        self.units = [u for g in self.groups for u in g.units]
+        self.radius_sim = np.concatenate([g.radius_sim for g in self.groups])
+        self.cells_host = np.concatenate([g.cells_host for g in self.groups])
+
+    def _locate(self, i):
+        for (a, b), g in zip(self.ranges, self.groups):
+            if a <= i < b:
+                return g, i - a
+        raise IndexError(i)
+
+    @property
+    def time(self):
+        return self.groups[0].time
+
+    @property
+    def eps(self):
+        return self.groups[0].eps
+
+    def advance(self, nsteps):
+        for g in self.groups:
+            g.advance(nsteps)
+
+    def sync(self):
+        for g in self.groups:
+            g.sync()
+
+    def reset(self, specs):
+        for (a, b), g in zip(self.ranges, self.groups):
+            g.reset(specs[a:b])
+        self._collect()
+
+    def time_passes(self, nrep=20):
+        """per-step ms of the force, node and curve passes, summed over the groups"""
+        t = np.array([g.time_passes(nrep) for g in self.groups])
+        return tuple(float(v) for v in t.sum(axis=0))
+
+    def curves_sim(self):
+        return np.concatenate([g.curves_sim() for g in self.groups])
+
+    def curves_normalized(self):
+        return np.concatenate([g.curves_normalized() for g in self.groups])
+
+    def features(self):
+        return np.concatenate([g.features() for g in self.groups])
+
+    def positions(self, i):
+        g, j = self._locate(i)
+        return g.positions(j)
+
+    def single_cell_config(self, i, outdir="/tmp/nfx_batch_cell/"):
+        g, j = self._locate(i)
+        return g.single_cell_config(j, outdir)
+
 CELL_DTYPE = np.dtype([
     ("mu1", np.float32), ("mu2", np.float32), ("kappa", np.float32),
     ("vp_rate", np.float32), ("vp_yield", np.float32), ("vp_offset", np.float32),

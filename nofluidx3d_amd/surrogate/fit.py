@@ -190,7 +190,7 @@ def _distinct_best(all_theta, all_err, k):
 def fit_surrogate(net, norm, targets, bases, bs, rounds=8, ncand=64, ntop=4, corrections=0, corr_rounds=4, seed=0,
                   log=None, confirm=True):
     """targets (n, 256) newtons; bases (n) CellSpec with the known geometry/protocol; ``bs`` a
-    BatchSimulation over the targets' meshes.  Returns dict with the fitted parameters, the fit's
+    BatchSimulation or CellGroups over the targets' meshes.  Returns dict with the fitted parameters, the fit's
     simulated curve and its error, and timings.
 
     ``corrections``: after the ``ntop`` refining simulations, that many corrected rounds.  Each

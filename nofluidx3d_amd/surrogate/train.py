@@ -175,7 +175,8 @@ def load(path):
 
 
 def main(argv=None):
-    from nofluidx3d_amd.batch import BatchSimulation, make_meshes, sample_batch
+    from nofluidx3d_amd.autotune import group_tetra
+    from nofluidx3d_amd.batch import CellGroups, make_meshes, sample_batch
 
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -191,12 +192,13 @@ def main(argv=None):
     log = lambda *a, **k: print(*a, **k, file=sys.stderr)
     t0 = time.time()
     pool, _ = sample_batch(args.pool, seed0=1000, steps=args.steps)
-    bs = BatchSimulation(pool, make_meshes(pool), steps=args.steps, ncurve=256, tile_k=args.tile_k)
+    bs = CellGroups(pool, make_meshes(pool), max_tetra=group_tetra(), steps=args.steps, ncurve=256, tile_k=args.tile_k)
     Xt, Yt = generate(bs, pool, args.chunks, seed0=100000, log=log)
     Xv, Yv = generate(bs, pool, args.heldout_chunks, seed0=900000, log=log)
     del bs
     pool2, _ = sample_batch(args.pool, seed0=5000, steps=args.steps)
-    bs2 = BatchSimulation(pool2, make_meshes(pool2), steps=args.steps, ncurve=256, tile_k=args.tile_k)
+    bs2 = CellGroups(pool2, make_meshes(pool2), max_tetra=group_tetra(), steps=args.steps, ncurve=256,
+                     tile_k=args.tile_k)
     Xn, Yn = generate(bs2, pool2, args.newmesh_chunks, seed0=950000, log=log)
     del bs2
     tgen = time.time() - t0

@@ -222,3 +222,32 @@ def test_reset_reproduces_a_fresh_run(ctx):
     a.advance(STEPS)
     b.advance(STEPS)
     assert np.array_equal(a.curves_sim(), b.curves_sim())
+
+
+def test_cell_groups_equal_one_batch_bitwise(ctx):
+    """The cells run as consecutive groups (each group all its steps before the next) give every
+    cell bitwise the positions and curve it has in one batch; reset keeps that."""
+    from nofluidx3d_amd.batch import BatchSimulation, CellGroups, cell_groups, make_meshes
+
+    specs = specs_for(True)
+    meshes = make_meshes(specs, workers=1, cache=None)
+    sizes = [len(m[1]) for m in meshes]
+    budget = sizes[0] + sizes[1]  # groups [0, 2), then by size
+    assert len(cell_groups(meshes, budget)) >= 3
+    one = BatchSimulation(specs, meshes, steps=STEPS, ncurve=NCURVE, tile_k=256, stream_nt=0)
+    grp = CellGroups(specs, meshes, max_tetra=budget, steps=STEPS, ncurve=NCURVE, tile_k=256, stream_nt=0)
+    assert grp.ranges == cell_groups(meshes, budget) and grp.T_real == one.T_real
+    for b in (one, grp):
+        b.advance(STEPS)
+    assert np.array_equal(grp.curves_sim(), one.curves_sim())
+    for i in range(len(specs)):
+        assert np.array_equal(grp.positions(i), one.positions(i)), i
+    assert np.array_equal(grp.radius_sim, one.radius_sim) and grp.cells_host.tobytes() == one.cells_host.tobytes()
+    import dataclasses
+
+    new = [dataclasses.replace(s, E_si=1.2 * s.E_si, viscosity_si=1.2 * s.viscosity_si) for s in specs]
+    for b in (one, grp):
+        b.reset(new)
+        b.advance(STEPS)
+    assert np.array_equal(grp.curves_sim(), one.curves_sim())
+
