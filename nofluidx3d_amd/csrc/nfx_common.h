@@ -65,6 +65,11 @@ struct nfx_ctx {
     bool graph_hist = false;
     // batched stepping: the batch being enqueued (NULL otherwise) and its graph cache
     const nfx_batch* batch = nullptr;
+    // ... and, while its force pass is enqueued with the step's curve sample riding in it, that step
+    bool curve_fused = false;
+    int64_t curve_step = 0;
+    const int64_t* curve_step_dev = nullptr;
+    int32_t curve_off = 0;
     hipGraphExec_t bgraph_exec = nullptr;
     hipGraph_t bgraph = nullptr;
     nfx_plan bgraph_plan{};

@@ -342,7 +342,47 @@ struct TileArgs {
     const uint16_t* __restrict__ tile_dst16;
     const uint32_t* __restrict__ tile_node_base;
     const uint32_t* __restrict__ tile_dst_base;
+    // batches: the step's curve samples ride in the same launch, one workgroup per cell appended
+    // after the tiles (curve_blocks = 0: none); they read the positions the tiles read
+    int32_t curve_blocks;
+    int32_t curve_ind;
+    const uint32_t* __restrict__ cell_node_off;
+    double* curves;
+    int32_t ncurve;
+    int64_t curve_steps, curve_step;
+    const int64_t* curve_step_dev;
+    int32_t curve_off;
 };
+
+// One cell's sample of the batched curves, by one workgroup of any size (<= 512): on the first
+// step of each of the ncurve sample intervals, the indenter force on the cell's nodes at the
+// current positions, summed in FP64 in a fixed order (strided per-thread sums, then lanes and
+// waves in order).  Every exit before the barrier is uniform over the workgroup.
+__device__ void batch_curve_sample(const float* __restrict__ x, int64_t N, const uint32_t* __restrict__ cell_node_off,
+                                   const nfx_cell* __restrict__ cells, double* curves, int32_t ncurve,
+                                   int64_t curve_steps, int64_t time, int cell, int ind) {
+    if (time < 0 || time >= curve_steps) return;
+    const int64_t k = time * ncurve / curve_steps;
+    if (time > 0 && (time - 1) * ncurve / curve_steps == k) return;  // not the first step of sample k
+    const nfx_cell cp = cells[cell];
+    const float pos = (float)indenter_pos(cp.tip_start, cp.tip_speed, cp.tip_move_steps, time);
+    double s = 0.0;
+    for (int64_t n = cell_node_off[cell] + threadIdx.x; n < cell_node_off[cell + 1]; n += blockDim.x) {
+        const V3 p = mk(x[n], x[N + n], x[2 * N + n]);
+        s -= ind == NFX_INDENTER_SPHERE ? (double)sphere_term(p, cp.tip_k, cp.tip_radius, pos).y
+                                        : (double)plane_term(p.y, cp.tip_k, pos);
+    }
+    __shared__ double ws[8];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += ws[w];
+        curves[(int64_t)cell * ncurve + k] = t;
+    }
+}
 
 // LDS of a tile: corner forces 12K floats, then per tile-node its position (3 floats) and run
 // start (u16)
@@ -366,7 +406,14 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     float* s_x = s_cf + 12 * K;                    // [component][MX]
     uint16_t* s_lb = reinterpret_cast<uint16_t*>(s_x + 3 * MX);
 
-    const int64_t c = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t ntile_blocks = gridDim.x - (uint32_t)a.curve_blocks;
+    if (blockIdx.x >= ntile_blocks) {  // the step's curve sample of cell blockIdx.x - ntile_blocks
+        const int64_t time = a.curve_step_dev ? (*a.curve_step_dev + a.curve_off) : a.curve_step;
+        batch_curve_sample(a.x, a.N, a.cell_node_off, a.cells, a.curves, a.ncurve, a.curve_steps, time,
+                           (int)(blockIdx.x - ntile_blocks), a.curve_ind);
+        return;
+    }
+    const int64_t c = xcd_block(blockIdx.x, ntile_blocks);
     const int64_t T = a.T, N = a.N;
     const int64_t t0 = c * K;
     int nt = (int)((T - t0) < K ? (T - t0) : K);
@@ -876,8 +923,19 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
             a.tile_node_base = ctx->batch->tile_node_base;
             a.tile_dst_base = ctx->batch->tile_dst_base;
         }
+        if (ctx->curve_fused) {
+            a.curve_blocks = ctx->batch->ncells;
+            a.curve_ind = p.indenter;
+            a.cell_node_off = ctx->batch->cell_node_off;
+            a.curves = ctx->batch->curves;
+            a.ncurve = ctx->batch->ncurve;
+            a.curve_steps = ctx->batch->curve_steps;
+            a.curve_step = ctx->curve_step;
+            a.curve_step_dev = ctx->curve_step_dev;
+            a.curve_off = ctx->curve_off;
+        }
     }
-    const unsigned int nb = (unsigned int)((p.T + K - 1) / K);
+    const unsigned int nb = (unsigned int)((p.T + K - 1) / K) + (unsigned int)a.curve_blocks;
     const size_t lds = tile_lds_bytes_fixed<K>() + tile_lds_bytes_per_node() * (size_t)p.tile_max_nodes;
     const ElasticDev e = elastic_consts(p);
     const VPConst c = vp_consts(p.vp);
@@ -1049,31 +1107,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_curves(const float* __restrict
                                                          int32_t ncurve, int64_t curve_steps, int64_t step,
                                                          const int64_t* step_dev, int32_t step_off) {
     const int64_t time = step_dev ? (*step_dev + step_off) : step;
-    if (time < 0 || time >= curve_steps) return;
-    const int64_t k = time * ncurve / curve_steps;
-    if (time > 0 && (time - 1) * ncurve / curve_steps == k) return;  // not the first step of sample k
-    const int cell = blockIdx.x;
-    const nfx_cell cp = cells[cell];
-    const float pos = (float)indenter_pos(cp.tip_start, cp.tip_speed, cp.tip_move_steps, time);
-    double s = 0.0;
-    for (int64_t n = cell_node_off[cell] + threadIdx.x; n < cell_node_off[cell + 1]; n += kBlock) {
-        const V3 p = mk(x[n], x[N + n], x[2 * N + n]);
-        if constexpr (IND == NFX_INDENTER_SPHERE) {
-            s -= (double)sphere_term(p, cp.tip_k, cp.tip_radius, pos).y;
-        } else {
-            s -= (double)plane_term(p.y, cp.tip_k, pos);
-        }
-    }
-    __shared__ double ws[kBlock / 64];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / 64; ++w) t += ws[w];
-        curves[(int64_t)cell * ncurve + k] = t;
-    }
+    batch_curve_sample(x, N, cell_node_off, cells, curves, ncurve, curve_steps, time, blockIdx.x, IND);
 }
 
 static int validate_batch(const nfx_plan* p, const nfx_batch* b) {
@@ -1103,7 +1137,10 @@ static void launch_batch_node_sub(nfx_ctx* ctx, const nfx_plan& p, const BatchNo
 
 static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b, int64_t step, const int64_t* sd,
                              int32_t off, int which = 7) {
-    if ((which & 1) && b.curves) {
+    // a whole step enqueues the curve sample inside the force pass's launch (it reads the same
+    // positions, before the node pass moves them); the curve pass alone keeps its own kernel
+    const bool fuse = (which & 3) == 3 && b.curves;
+    if ((which & 1) && b.curves && !fuse) {
         if (p.indenter == NFX_INDENTER_SPHERE)
             k_batch_curves<1><<<b.ncells, kBlock, 0, ctx->stream>>>(p.points, p.N, b.cell_node_off, b.cells, b.curves,
                                                                     b.ncurve, b.curve_steps, step, sd, off);
@@ -1114,8 +1151,13 @@ static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b
     }
     if (which & 2) {
         ctx->batch = &b;
+        ctx->curve_fused = fuse;
+        ctx->curve_step = step;
+        ctx->curve_step_dev = sd;
+        ctx->curve_off = off;
         const int rc = launch_force_pass(ctx, p, false);
         ctx->batch = nullptr;
+        ctx->curve_fused = false;
         if (rc) return rc;
     }
     if (which & 4) {
