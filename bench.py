@@ -154,17 +154,24 @@ def cpu_baseline(P, T, cfg, material, target_s=12.0):
     }
 
 
-def bench_batched(steps=10000, ncells=512, log=log):
+def bench_batched(steps=10000, ncells=512, form="auto", log=log):
     """The batched workload: 512 seeded cells (seeds 1000+i), 10k steps each, one 256-point curve per
-    cell; tile size, cache policy and cell grouping from the autotuner.  Returns the JSON block."""
-    from nofluidx3d_amd.autotune import tune_batch
+    cell; tile size, cache policy and cell grouping from the autotuner (``form`` "auto"), or fixed:
+    "groups" (cache-sized cell groups, tile 256, default policy) or "single" (one batch, tile 256,
+    nontemporal).  Returns the JSON block."""
+    from nofluidx3d_amd.autotune import group_tetra, tune_batch
     from nofluidx3d_amd.batch import CellGroups, make_meshes, sample_batch
 
     specs, stats = sample_batch(ncells, seed0=1000, steps=steps)
     t0 = time.time()
     meshes = make_meshes(specs)
     tmesh = time.time() - t0
-    best, tuned = tune_batch(specs, meshes, steps, use_cache=False, log=log)
+    if form == "auto":
+        best, tuned = tune_batch(specs, meshes, steps, use_cache=False, log=log)
+    elif form == "groups":
+        best, tuned = {"tile_k": 256, "stream_nt": None, "group_tetra": group_tetra()}, None
+    else:
+        best, tuned = {"tile_k": 256, "stream_nt": 1, "group_tetra": 0}, None
     b = CellGroups(specs, meshes, max_tetra=best.get("group_tetra") or None, steps=steps, ncurve=256,
                    tile_k=best["tile_k"], stream_nt=best["stream_nt"])
     b.advance(100)  # includes the graph capture
@@ -306,6 +313,8 @@ def main():
     ap.add_argument("--only", default=None, choices=[None, "batched", "surrogate"],
                     help="run just one secondary measurement and print its block (profiling runs)")
     ap.add_argument("--batched-steps", type=int, default=10000)
+    ap.add_argument("--batched-form", default="auto", choices=["auto", "groups", "single"],
+                    help="batched population as tuned, as cache-sized cell groups, or as one batch (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is design-only "
@@ -325,7 +334,7 @@ def main():
         torch.cuda.synchronize()
         del a, b
     if args.only == "batched":
-        print(json.dumps(bench_batched(steps=args.batched_steps)), flush=True)
+        print(json.dumps(bench_batched(steps=args.batched_steps, form=args.batched_form)), flush=True)
         return
     if args.only == "surrogate":
         print(json.dumps(bench_surrogate(os.path.join(ROOT, "nofluidx3d_amd", "surrogate", "weights",
