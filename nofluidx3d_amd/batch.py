@@ -431,6 +431,8 @@ class BatchSimulation:
 
         from nofluidx3d_amd import openCL
 
+        if len(specs) != self.ncells:
+            raise ValueError(f"reset needs one spec per cell ({self.ncells}), got {len(specs)}")
         for a, b in zip(specs, self.specs):
             if (a.radius_um, a.spacing_um, a.seed) != (b.radius_um, b.spacing_um, b.seed):
                 raise ValueError("reset keeps the meshes: radius, spacing and seed must not change")
@@ -597,6 +599,8 @@ class CellGroups:
             g.sync()
 
     def reset(self, specs):
+        if len(specs) != self.ncells:
+            raise ValueError(f"reset needs one spec per cell ({self.ncells}), got {len(specs)}")
         for (a, b), g in zip(self.ranges, self.groups):
             g.reset(specs[a:b])
         self._collect()

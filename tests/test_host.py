@@ -203,6 +203,29 @@ def test_config_mesh_written_to_init_vtk_reads_back_through_tetracell(tmp_path):
     assert np.allclose(c.points, (P - P[0]) * scale, rtol=0, atol=1e-9 * cfg["CELL"]["RadiusSIM"])
 
 
+def test_cell_groups_partition_consecutive_cells_under_the_budget():
+    """batch.cell_groups: consecutive ranges covering every cell once, each within the tetra budget
+    unless it is a single cell larger than the budget; a budget above the total gives one group"""
+    from nofluidx3d_amd.autotune import MEMORY_SIDE_CACHE_BYTES, RESIDENT_BYTES_PER_TETRA, group_tetra
+    from nofluidx3d_amd.batch import cell_groups
+
+    rng = np.random.default_rng(8)
+    sizes = rng.integers(100, 5000, size=200)
+    sizes[17] = 20000  # a cell above the budget
+    meshes = [(None, np.zeros((int(t), 4))) for t in sizes]
+    budget = 9000
+    r = cell_groups(meshes, budget)
+    assert r[0][0] == 0 and r[-1][1] == len(sizes)
+    assert all(a < b for a, b in r) and all(r[i][1] == r[i + 1][0] for i in range(len(r) - 1))
+    for a, b in r:
+        tot = int(sizes[a:b].sum())
+        assert tot <= budget or b - a == 1
+        if b < len(sizes):
+            assert tot + int(sizes[b]) > budget  # groups are filled greedily
+    assert cell_groups(meshes, int(sizes.sum())) == [(0, len(sizes))]
+    assert group_tetra() == MEMORY_SIDE_CACHE_BYTES // RESIDENT_BYTES_PER_TETRA
+
+
 def test_kernel_builder_defines_render_like_the_reference():
     from nofluidx3d_amd.KernelBuilder import KernelBuilder
 
