@@ -121,6 +121,81 @@ __global__ __launch_bounds__(256) void k_gemm_nt(nfx_mlp_gemm_args g) {
             }
 }
 
+// The same product for grids that fill the chip (M, N multiples of 128; one pass over K): a
+// workgroup tile of 128 x 128 staged through LDS, each wave a 64 x 64 sub-tile (4 x 4 MFMA tiles,
+// 16 MFMAs per k-step from 8 LDS fragment reads instead of 4 MFMAs per 4 global reads).  Double
+// buffered: the next k-slice goes global -> registers during the MFMAs and registers -> the other
+// LDS buffer after them, one barrier per k-step.  Rows are padded by 8 BF16 against bank
+// conflicts.  Accumulator layout and epilogue as k_gemm_nt.
+constexpr int kLdsTile = 128, kLdsK = 32, kLdsPitch = kLdsK + 8;
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_lds(nfx_mlp_gemm_args g) {
+    __shared__ __align__(16) __bf16 As[2][kLdsTile * kLdsPitch];
+    __shared__ __align__(16) __bf16 Bs[2][kLdsTile * kLdsPitch];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int m0 = blockIdx.y * kLdsTile, n0 = blockIdx.x * kLdsTile;
+    const __bf16* __restrict__ A = static_cast<const __bf16*>(g.A);
+    const __bf16* __restrict__ B = static_cast<const __bf16*>(g.B);
+    const int r16 = lane & 15, kq = 8 * (lane >> 4);
+    // copy assignment: 128 rows x 4 chunks of 8 BF16 per operand, two chunks per thread
+    int crow[2], ck[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int c = tid + 256 * i;
+        crow[i] = c >> 2;
+        ck[i] = (c & 3) * 8;
+    }
+    bf16x8 ra[2], rb[2];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            ra[i] = *reinterpret_cast<const bf16x8*>(A + (int64_t)(m0 + crow[i]) * g.lda + k0 + ck[i]);
+            rb[i] = *reinterpret_cast<const bf16x8*>(B + (int64_t)(n0 + crow[i]) * g.ldb + k0 + ck[i]);
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            *reinterpret_cast<bf16x8*>(&As[buf][crow[i] * kLdsPitch + ck[i]]) = ra[i];
+            *reinterpret_cast<bf16x8*>(&Bs[buf][crow[i] * kLdsPitch + ck[i]]) = rb[i];
+        }
+    };
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < g.K; k0 += kLdsK) {
+        const bool more = k0 + kLdsK < g.K;
+        if (more) gload(k0 + kLdsK);
+        bf16x8 a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            a[i] = *reinterpret_cast<const bf16x8*>(&As[buf][(wm + 16 * i + r16) * kLdsPitch + kq]);
+            b[i] = *reinterpret_cast<const bf16x8*>(&Bs[buf][(wn + 16 * i + r16) * kLdsPitch + kq]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (more) sstore(buf ^ 1);  // nobody reads buf ^ 1 now: the last barrier ended its reads
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                epilogue<EPI>(g, m0 + wm + 16 * i + 4 * (lane >> 4) + r, n0 + wn + 16 * j + r16, acc[i][j][r]);
+}
+
 // split-K finish: one thread per output element sums the split_k workspace planes in order
 template <int EPI>
 __global__ __launch_bounds__(256) void k_splitk_finish(nfx_mlp_gemm_args g) {
@@ -211,7 +286,18 @@ int nfx_mlp_gemm(nfx_ctx* c, const nfx_mlp_gemm_args* g) {
     NFX_REQUIRE(g->split_k >= 0 && S <= 64 && g->K % (32 * S) == 0,
                 "mlp gemm: split_k %d must be at most 64 and divide K / 32 (K %d)", g->split_k, g->K);
     NFX_REQUIRE(S == 1 || g->workspace, "mlp gemm: split_k > 1 needs a workspace of split_k * M * N floats");
-    if (S == 1) {
+    // the LDS-tiled kernel where its 128 x 128 tiles still give a full grid (>= 256 workgroups)
+    const bool lds = S == 1 && g->M % kLdsTile == 0 && g->N % kLdsTile == 0 &&
+                     (int64_t)(g->M / kLdsTile) * (g->N / kLdsTile) >= 256;
+    if (lds) {
+        const dim3 grid(g->N / kLdsTile, g->M / kLdsTile);
+        switch (g->epilogue) {
+            case NFX_MLP_STORE: k_gemm_lds<NFX_MLP_STORE><<<grid, 256, 0, c->stream>>>(*g); break;
+            case NFX_MLP_FWD_GELU: k_gemm_lds<NFX_MLP_FWD_GELU><<<grid, 256, 0, c->stream>>>(*g); break;
+            case NFX_MLP_BWD_GELU: k_gemm_lds<NFX_MLP_BWD_GELU><<<grid, 256, 0, c->stream>>>(*g); break;
+            default: k_gemm_lds<NFX_MLP_ADAM><<<grid, 256, 0, c->stream>>>(*g); break;
+        }
+    } else if (S == 1) {
         const dim3 grid(g->N / 32, g->M / 128);
         switch (g->epilogue) {
             case NFX_MLP_STORE: k_gemm_nt<NFX_MLP_STORE, false><<<grid, 256, 0, c->stream>>>(*g); break;

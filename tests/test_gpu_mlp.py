@@ -38,7 +38,8 @@ def bf_round(a):
     return bf(a).float().numpy().astype(np.float64)
 
 
-@pytest.mark.parametrize("M,N,K", [(128, 32, 32), (256, 96, 64), (512, 512, 512), (256, 64, 4096)])
+@pytest.mark.parametrize("M,N,K", [(128, 32, 32), (256, 96, 64), (512, 512, 512), (256, 64, 4096),
+                                   (16384, 256, 512)])  # the last takes the LDS-tiled kernel (256 tiles of 128 x 128)
 def test_gemm_nt_matches_fp64_product(ctx, M, N, K):
     import torch
 
