@@ -226,6 +226,31 @@ def test_cell_groups_partition_consecutive_cells_under_the_budget():
     assert group_tetra() == MEMORY_SIDE_CACHE_BYTES // RESIDENT_BYTES_PER_TETRA
 
 
+def test_split_k_choice_divides_the_batch_and_fills_the_grid():
+    """surrogate.mlp.split_k: a power of two <= 64 whose slices are whole multiples of 32 rows of K,
+    doubled only while the grid of 128 x 32 tiles stays under 512 workgroups"""
+    from nofluidx3d_amd.surrogate.mlp import split_k
+
+    for M, N, K in [(512, 32, 4096), (512, 512, 4096), (256, 512, 4096), (128, 32, 256), (512, 512, 128),
+                    (4096, 512, 512), (128, 32, 96)]:
+        s = split_k(M, N, K)
+        groups = (M // 128) * (N // 32)
+        assert s & (s - 1) == 0 and 1 <= s <= 64 and K % (32 * s) == 0, (M, N, K, s)
+        if s > 1:
+            assert groups * (s // 2) < 512  # doubled only while the grid was short
+        assert s == 64 or groups * s >= 512 or K % (64 * s) != 0, (M, N, K, s)
+    assert split_k(512, 512, 4096) == 8 and split_k(4096, 512, 512) == 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "one GPU" in r.stderr
+
+
 def test_kernel_builder_defines_render_like_the_reference():
     from nofluidx3d_amd.KernelBuilder import KernelBuilder
 
