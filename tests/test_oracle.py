@@ -6,7 +6,8 @@ transposed operand, a wrong SoA index) fails it:
   energy (central differences), textbook closed forms (hydrostatic, uniaxial NH, simple-shear MR
   including the normal-stress differences), rest state, rigid motions, momentum balance;
 - SVK: energy consistency with Psi = lambda/2 tr(E)^2 + mu tr(E^2); SVKK's polar root against
-  scipy.linalg.polar; SVKK rest state and hydrostatic closed form of its own stress law;
+  scipy.linalg.polar; SVKK rest state and hydrostatic closed form of its own stress law; SVKK and
+  SVK both reduce to Hooke's law at small strain;
 - viscoplastic: equals the MR force when Fp = I, one-step rate known answer from the shear
   closed form, plastic incompressibility, Maxwell relaxation time;
 - contacts: closed forms per node; integrator: the 3-term damped Stormer recurrence and the
@@ -245,6 +246,26 @@ def test_svkk_polar_root_and_hydrostatic():
     V0 = np.linalg.det(Dm) / 6
     rows = -V0 * mu * (1 + (kappa * (3 * s * s - 3) - 1) / (s * s)) * s * np.linalg.inv(Dm)
     assert np.allclose(f[1:], rows, rtol=1e-10, atol=1e-16)
+
+
+def test_svkk_is_linear_elasticity_at_small_strain():
+    """SVKK has no known energy to check against, but at small strain it must reduce to linear
+    elasticity, as SVK does: for F = I + e H (H random, not symmetric) its forces match SVK's to
+    O(e) relative, and both match Hooke's law (virial stress = lambda tr(eps) I + 2 mu eps,
+    eps = e sym(H)) to O(e).  A dropped or mis-signed term of the linear part fails this."""
+    E, nu = 0.01, 0.35
+    mu, lam = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+    for _ in range(4):
+        H = RNG.normal(size=(3, 3))
+        for e in (1e-4, 1e-5):
+            x = deform(np.eye(3) + e * H, REF_TET)
+            fk = forces_of(O.SVKK, E, nu, 1.0, REF_TET, x)
+            fs = forces_of(O.SVK, E, nu, 1.0, REF_TET, x)
+            assert np.max(np.abs(fk - fs)) < 20 * e * np.max(np.abs(fs)), e
+            eps = e * 0.5 * (H + H.T)
+            hooke = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+            sig = virial_stress(REF_TET, x, fk)
+            assert np.max(np.abs(sig - hooke)) < 20 * e * np.max(np.abs(hooke)), e
 
 
 def test_sequential_and_atomic_scatter_agree():
