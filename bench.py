@@ -7,14 +7,18 @@ one Simulation.timeStep of the whole cell (force pass + node pass); steps are re
 captured hipGraph.  Inputs are resident in HBM before the timed region starts.
 
 Printed JSON line (one):
-- value = tetra-updates/s = T * steps / wall seconds of the timed region;
-- hbm_gbs = algorithmic bytes per step / ms_per_step (96 T + 108 N for the viscoplastic step:
-  connectivity 16 B + shape state S = Dm^-1 Fp^-1 read and written 72 B + epsbar read and written
-  8 B per tetra; 56 T + 108 N for the elastic one: connectivity 16 + Dm^-1 36 + V0 4; the batch's
-  linear Maxwell law keeps no epsbar and its state is the symmetric G = S S^T: 64 T + 108 N);
-- roofline: the dominant kernel (the force pass) -- algorithmic bytes per launch (96 T + 24 N)
-  over its HIP-event average duration, against 8 TB/s; ``traffic`` = measured HBM bytes per
-  launch from rocprofv3 FETCH_SIZE + WRITE_SIZE when ``--traffic-json`` names a file holding them;
+- value = tetra-updates/s = T * steps / wall seconds of the timed region (the steps' graphs are
+  captured during the warm-up, before the region; ``graph_captures_timed`` counts any inside it);
+- hbm_gbs = algorithmic bytes per step / ms_per_step, with the per-unit bytes of the hot-path
+  specification: 136 T + 108 N for the viscoplastic step (connectivity 16 B, Dm^-1 36 B, V0 4 B,
+  the plastic state 36 B and epsbar 4 B each read and written, per tetra; x, v, f, f_old per node),
+  56 T + 108 N for the elastic one;
+- roofline: the dominant kernel (the force pass, ``k_tile_force``): those algorithmic bytes per
+  launch (136 T + 24 N viscoplastic, 56 T + 24 N elastic) over its HIP-event average duration,
+  against 8 TB/s; ``kernel_bytes_per_launch`` is what the kernel itself must move (it reads per-
+  corner tile records instead of the connectivity and derives V0 from Dm^-1); ``traffic`` = HBM
+  bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated) in a separate
+  profiling run of this same command, read from ``--traffic-json`` (``traffic_source`` says which);
 - cpu_baseline: the FP64 C oracle (OpenMP, all allowed host threads) on a bounded step sample
   of the same cell;
 - plain_small: the small cell (R 8 um, 0.8 um lattice, ~25k tetra) fused step, for comparison;
@@ -37,7 +41,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-VP_TET_BYTES = 96  # per tetra and step: connectivity 16 + shape state 36 + 36 + epsbar 4 + 4
+# algorithmic bytes per tetra of one force pass, as the hot-path specification counts them
+SPEC_TET_BYTES = {"viscoplastic": 16 + 36 + 4 + 2 * 36 + 2 * 4, "elastic": 16 + 36 + 4}
+# what the tiled force pass reads and writes per tetra: corner records 8 (instead of the 16 B
+# connectivity), Dm^-1 36, and V0 4 (elastic) or the plastic deviation Q 36 + 36 and epsbar 4 + 4
+KERNEL_TET_BYTES = {"viscoplastic": 8 + 36 + 2 * 36 + 2 * 4, "elastic": 8 + 36 + 4}
 
 
 def log(*a):
@@ -85,14 +93,29 @@ def sync(sim):
     torch.cuda.synchronize()
 
 
+def graph_captures():
+    import ctypes
+
+    from nofluidx3d_amd import _lib, openCL
+
+    n = ctypes.c_int64()
+    _lib.call("nfx_graph_captures", openCL.handle(), ctypes.byref(n))
+    return n.value
+
+
 def time_steps(sim, warmup, steps):
-    sim.advance(100)  # capture and instantiate the 100-step graph outside the timed region
+    """ms per step over ``steps`` steps, after a first call of the same length (which captures the
+    graph blocks the timed call replays) and ``warmup`` more steps; returns (s/step, captures in the
+    timed region)"""
+    sim.advance(steps)
     sim.advance(warmup)
     sync(sim)
+    c0 = graph_captures()
     t0 = time.perf_counter()
     sim.advance(steps)
     sync(sim)
-    return (time.perf_counter() - t0) / steps
+    dt = (time.perf_counter() - t0) / steps
+    return dt, graph_captures() - c0
 
 
 def pass_times(sim, nrep):
@@ -174,21 +197,26 @@ def bench_batched(steps=10000, ncells=512, form="auto", log=log):
         best, tuned = {"tile_k": 256, "stream_nt": 1, "group_tetra": 0}, None
     b = CellGroups(specs, meshes, max_tetra=best.get("group_tetra") or None, steps=steps, ncurve=256,
                    tile_k=best["tile_k"], stream_nt=best["stream_nt"])
-    b.advance(100)  # includes the graph capture
+    b.advance(100)  # captures every group's 100-step graph; the groups keep them
     b.sync()
+    c0 = graph_captures()
     t0 = time.perf_counter()
     b.advance(steps - 100)
     b.sync()
     dt = time.perf_counter() - t0
+    captures_timed = graph_captures() - c0
     ms = dt / (steps - 100) * 1e3
     f, n, c = b.time_passes(20)
     curves = b.curves_normalized()
     ok = np.isfinite(curves).all(axis=1)
     tet = np.array([len(m[1]) for m in meshes])
-    # the batch's linear Maxwell law does not read the accumulated plastic strain (not kept), and its
-    # state is the symmetric G = S S^T (6 floats) instead of S (9)
-    tet_bytes = 16 + 2 * (24 if b.sym_state else 36) + (8 if b.eps is not None else 0)
-    step_bytes = tet_bytes * b.T_real + 108 * b.N
+    # algorithmic bytes as the specification counts a viscoplastic step (136 T + 108 N); the batch's
+    # kernels move less: the linear Maxwell law does not read the accumulated plastic strain (not
+    # kept), its plastic state is the symmetric deviation Gd (6 floats) instead of 9, and its node
+    # state is (u, w) without the previous force
+    step_bytes = SPEC_TET_BYTES["viscoplastic"] * b.T_real + 108 * b.N
+    kernel_step_bytes = (8 + 36 + 2 * (24 if b.sym_state else 36) + (8 if b.eps is not None else 0)) * b.T_real + \
+        (2 * 24 + 4 + 4 + 12) * b.N  # per node: u and w read and written, X0.y, node_cell; x read by the tiles
     return {
         "workload": f"{ncells} cells (seeds 1000+i), {b.T_real} tetra, {b.N} nodes, NH + Maxwell viscoplastic, "
                     f"{steps} steps each, 256-point curve per cell",
@@ -203,7 +231,9 @@ def bench_batched(steps=10000, ncells=512, form="auto", log=log):
         "group_tetra": int(best.get("group_tetra") or 0),
         "autotune_ms_per_step": tuned,
         "algorithmic_bytes_per_step": int(step_bytes),
+        "kernel_bytes_per_step": int(kernel_step_bytes),
         "ms_per_step": ms,
+        "graph_captures_timed": captures_timed,
         "value": b.T_real * 1e3 / ms,
         "unit": "tetra-updates/s",
         "hbm_gbs": step_bytes / (ms * 1e-3) / 1e9,
@@ -355,18 +385,25 @@ def main():
         from nofluidx3d_amd.autotune import tune_simulation
 
         _, tuned = tune_simulation(sim, log=log)
-    ms = time_steps(sim, args.warmup, args.steps) * 1e3
+    dt, captures_timed = time_steps(sim, args.warmup, args.steps)
+    ms = dt * 1e3
     ms_force, ms_node = pass_times(sim, min(args.steps, 100))
     vp = material == "viscoplastic"
-    force_bytes = (VP_TET_BYTES if vp else 56) * nT + 24 * N
-    step_bytes = (VP_TET_BYTES if vp else 56) * nT + 108 * N
+    force_bytes = SPEC_TET_BYTES[material] * nT + 24 * N
+    kernel_bytes = KERNEL_TET_BYTES[material] * nT + 24 * N
+    step_bytes = SPEC_TET_BYTES[material] * nT + 108 * N
     achieved = force_bytes / (ms_force * 1e-3) / 1e9
-    traffic = None
+    nt_flag = sim.plan().stream_nt
+    kernel = (f"k_tile_force<{'NFX_VISCOPLASTIC' if vp else 'NFX_MR'}, K={sim.tile_k}, "
+              f"{'nontemporal' if nt_flag else 'default'} cache policy>") if args.scatter == "tiled" else \
+        f"{'k_vp' if vp else 'k_tetra'}<{args.scatter}>"
+    traffic, traffic_source = None, None
     if args.traffic_json and os.path.exists(args.traffic_json):
         tj = json.load(open(args.traffic_json))
-        key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if sim.stream_nt else ''}"
+        key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if nt_flag else ''}"
         if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}):
             traffic = tj["force_pass_bytes"][key]
+            traffic_source = f"{os.path.relpath(args.traffic_json, ROOT)}[{key}]: {tj.get('source', '')}"
     out = {
         "metric": "tetra-updates/s",
         "value": nT * 1e3 / ms,
@@ -375,6 +412,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
+        "graph_captures_timed": captures_timed,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -399,13 +437,16 @@ def main():
         "ms_node_pass": ms_node,
         "roofline": {
             "bound": "hbm",
-            "kernel": "force pass (k_vp / k_tetra)",
+            "kernel": kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": traffic_source,
             "algorithmic_bytes_per_launch": force_bytes,
+            "kernel_bytes_per_launch": kernel_bytes,
+            "ms_per_launch": ms_force,
         },
     }
     del sim
@@ -416,7 +457,7 @@ def main():
             if other == args.scatter:
                 continue
             s2 = make_sim(cfg, P, T, other, material)
-            ms2 = time_steps(s2, args.warmup, min(args.steps, 200)) * 1e3
+            ms2 = time_steps(s2, args.warmup, min(args.steps, 200))[0] * 1e3
             f2, n2 = pass_times(s2, 50)
             out["other_scatter"].append({"scatter": other, "ms_per_step": ms2, "value": nT * 1e3 / ms2,
                                          "ms_force_pass": f2, "ms_node_pass": n2})
@@ -424,7 +465,7 @@ def main():
         small = configs.small_cell()
         Ps, Ts = cached_mesh(small)
         s3 = make_sim(small, Ps, Ts, "tiled", "elastic")
-        ms3 = time_steps(s3, 200, 2000) * 1e3
+        ms3 = time_steps(s3, 200, 2000)[0] * 1e3
         out["plain_small"] = {"workload": f"small_cell {len(Ps)} nodes {len(Ts)} tetra NH", "ms_per_step": ms3,
                               "value": len(Ts) * 1e3 / ms3, "unit": "tetra-updates/s"}
         del s3

@@ -78,27 +78,41 @@ typedef struct {
     double yield_offset;        /* sigma_y >= 0; rate uses max(svm - sigma_y, 0); 0 = reference law */
 } nfx_viscoplastic;
 
-/* Everything one fused step of one cell touches. */
+/* Everything one fused step of one cell touches.
+ *
+ * Position state.  The fused and batched steps keep each node's DISPLACEMENT u = x - X0 from its
+ * reference position X0 (ref, FP32; the positions the FP64 precompute of dminv saw) in disp, and form
+ * the deformation gradient as F = I + [u1-u0 | u2-u0 | u3-u0] Dm^-1.  An FP32 absolute position near
+ * |x| = 50 carries ~4e-6 of rounding, which F = r Dm^-1 would pass on to the strain; a displacement
+ * keeps its rounding relative to its own (small) size.  nfx_points_from_disp / nfx_disp_from_points
+ * convert to and from the absolute positions the per-plugin kernels take (the reference's buffers).
+ *
+ * Viscoplastic state.  vp_state holds the plastic deviation Q = Fp^-1 - I (9T SoA, zero at the
+ * start), or with vp_sym the symmetric deviation Gd = Fp^-1 Fp^-T - I (6T: xx, yy, zz, xy, yz, xz).
+ * Stored as Fp itself in FP32, each step's small plastic increment would be rounded against Fp's
+ * O(1) entries; as a deviation it keeps its own relative precision.  nfx_vp_fp_from_q /
+ * nfx_vp_q_from_fp convert to and from the reference's plasticDeformationTensor. */
 typedef struct {
     int64_t N, T;
     const uint32_t* tetras; /* 4T  SoA */
     const float* dminv;     /* 9T  SoA, inverse reference shape matrix */
-    const float* v0;        /* T   reference volume */
-    float* points;          /* 3N  SoA */
+    const float* v0;        /* T   reference volume (models NFX_SVKK, NFX_SVK, NFX_MR) */
+    const float* ref;       /* 3N  SoA reference positions X0 */
+    float* disp;            /* 3N  SoA displacements u = x - X0 */
     float* vel;             /* 3N */
     float* force;           /* 3N  (atomic scatter target; cleared by the node update) */
     float* fold;            /* 3N */
     int32_t model;          /* NFX_SVKK / NFX_SVK / NFX_MR / NFX_VISCOPLASTIC / legacy laws */
     double E, nu, ratio;
     nfx_viscoplastic vp;
-    float* Fp;        /* 9T, model NFX_VISCOPLASTIC */
+    float* vp_state;  /* model NFX_VISCOPLASTIC: Q (9T) or, with vp_sym, Gd (6T) */
     float* epsbar;    /* T; may be NULL on the tiled path when the law is linear (flow exponent 1,
                          hardening exponent 0: the flow does not read it), and then is not kept */
     float* von_mises; /* T, nullable: when set (with pressure), nfx_run_steps writes the per-tetra von
                          Mises stress and pressure of the last step of the call (viscoplastic and
                          legacy laws) */
     float* pressure;  /* T, nullable */
-    int32_t scatter;  /* NFX_SCATTER_ATOMIC or NFX_SCATTER_CSR */
+    int32_t scatter;  /* NFX_SCATTER_ATOMIC, NFX_SCATTER_CSR or NFX_SCATTER_TILED */
     const uint32_t* csr_offsets; /* N+1, incidences of node n at [off[n], off[n+1]) */
     const uint32_t* csr_slots;   /* 4T, slot = 4*t + corner, ascending per node */
     float* corners;              /* 12T scratch: force of corner a of tetra t at 12*t + 3*a + c */
@@ -119,13 +133,10 @@ typedef struct {
     int32_t indenter; /* NFX_INDENTER_* */
     double ind_k, ind_radius, ind_start, ind_speed, ind_move_steps;
     int32_t fix_top_bottom;
-    float* vp_shape; /* 9T, model NFX_VISCOPLASTIC with NFX_SCATTER_TILED: the state in shape form
-                        S = Dm^-1 Fp^-1 (replaces Fp and dminv on that path; nfx_vp_shape_from_fp) */
     double shear, bulk, shear2, volume_fraction; /* legacy laws (as in nfx_elastic) */
-    int32_t vp_sym;    /* NFX_VISCOPLASTIC with NFX_SCATTER_TILED: 1 = vp_shape holds the symmetric state
-                          G = S S^T (6T: xx, yy, zz, xy, yz, xz).  The stress, flow and forces depend
-                          on G alone, so the run is the same, with a third less state; Fp is not
-                          recoverable from it (nfx_vp_sym_from_shape) */
+    int32_t vp_sym;    /* NFX_VISCOPLASTIC with NFX_SCATTER_TILED: 1 = vp_state holds Gd.  The stress,
+                          flow and forces depend on Gd alone, so the run is the same with a third less
+                          state; Fp is not recoverable from it */
     int32_t stream_nt; /* NFX_SCATTER_TILED with NFX_MR or NFX_VISCOPLASTIC: 1 = the force pass reads
                           and writes the per-tetra state and tile tables with the nontemporal cache
                           policy (faster when they exceed the memory-side cache), 0 = default policy;
@@ -145,10 +156,10 @@ typedef struct {
 
 /* A batch is the concatenation of many cells' meshes, with tiles aligned to cells (a tile never
  * spans two cells, so each cell sums its forces in the same order as when it runs alone).
- * Batched steps keep the damped velocity-Verlet state as (x, u) with u = a v + f/2 of the last
- * step (a = 1 - xi/2) in plan.vel: the same recurrence as VelocityVerlet.cl (x += u,
- * u' = (a/b)(u + f'/2) + f'/2, b = 1 + xi/2) without the previous force; plan.fold is unused.
- * Zero v and f give u = 0. */
+ * Batched steps keep the damped velocity-Verlet state as (u, w) with w = a v + f/2 of the last
+ * step (a = 1 - xi/2) in plan.vel: the same recurrence as VelocityVerlet.cl (u += w,
+ * w' = (a/b)(w + f'/2) + f'/2, b = 1 + xi/2) without the previous force; plan.fold is unused.
+ * Zero v and f give w = 0. */
 typedef struct {
     int32_t ncells;
     const uint16_t* tile_ntet;     /* ntiles: tetra in each tile */
@@ -217,19 +228,21 @@ int nfx_linear_viscoelastic_force(nfx_ctx* ctx, const nfx_viscoelastic* p, const
 int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float* points, const uint32_t* tetras,
                            const float* dminv, int64_t N, int64_t T, float* Fp, float* epsbar, float* von_mises,
                            float* pressure, float* force, float* corners);
-/* The tiled viscoplastic force keeps its state as S = Dm^-1 Fp^-1 (one 9-float tensor instead of
- * Dm^-1 and Fp; Fe = r S).  Conversions between that form and the reference's Fp
- * (plasticDeformationTensor, Interactions.py:279-281), 9T SoA FP32 each: */
-int nfx_vp_shape_from_fp(nfx_ctx* ctx, const float* dminv, const float* Fp, int64_t T, float* shape);
-int nfx_vp_fp_from_shape(nfx_ctx* ctx, const float* dminv, const float* shape, int64_t T, float* Fp);
+/* The fused viscoplastic state Q = Fp^-1 - I and the reference's Fp (plasticDeformationTensor,
+ * Interactions.py:279-281), 9T SoA FP32 each */
+int nfx_vp_q_from_fp(nfx_ctx* ctx, const float* Fp, int64_t T, float* Q);
+int nfx_vp_fp_from_q(nfx_ctx* ctx, const float* Q, int64_t T, float* Fp);
+/* absolute positions x = X0 + u (3N SoA) from displacements, and back (no reference counterpart:
+ * the reference keeps absolute FP64 positions) */
+int nfx_points_from_disp(nfx_ctx* ctx, const float* ref, const float* disp, int64_t N, float* points);
+int nfx_disp_from_points(nfx_ctx* ctx, const float* ref, const float* points, int64_t N, float* disp);
 /* A batch's geometry from its cells' recentred FP64 positions p_rel (3N SoA, micrometres) and per-cell
- * scales (RadiusSIM / |p[1].x|): FP32 positions, Dm^-1 and V0, exactly as the host construction
- * computes them (TetraCell.py:15-41); padding tetra get the identity shape (a reset on the device) */
+ * scales (RadiusSIM / |p[1].x|): the FP32 reference positions X0 = fl32(p_rel * scale) and, from X0 in
+ * FP64, Dm^-1 and V0, exactly as the host construction computes them (TetraCell.py:15-41); padding
+ * tetra get the identity shape (a reset on the device) */
 int nfx_batch_geometry(nfx_ctx* ctx, const double* p_rel, const double* cell_scale, const uint32_t* node_cell,
                        const uint32_t* tetras, const uint16_t* tile_ntet, int32_t tile_k, int64_t N, int64_t T,
-                       float* points, float* dminv, float* v0);
-/* the symmetric state G = S S^T (6T) from the shape state S (9T; S = Dm^-1 at Fp = I) */
-int nfx_vp_sym_from_shape(nfx_ctx* ctx, const float* shape, int64_t T, float* sym);
+                       float* ref, float* dminv, float* v0);
 /* deterministic gather: force[c*N+n] += sum over slots of node n, in slot order */
 int nfx_gather_corners(nfx_ctx* ctx, const uint32_t* csr_offsets, const uint32_t* csr_slots, const float* corners,
                        int64_t N, float* force);
@@ -250,9 +263,11 @@ int nfx_plane_force(nfx_ctx* ctx, float* force, const float* points, int64_t N, 
 int nfx_velocity_verlet(nfx_ctx* ctx, float* points, float* vel, float* force, float* fold, int64_t N,
                         int32_t fix_top_bottom); /* VelocityVerlet.cl */
 /* Simulation.py:146-153 / 198-212 forceSI and "Max dis" in simulation units, reduced in FP64 on the
- * device: out2[0] = indenter force, out2[1] = max penetration (device pointer to 2 doubles). */
-int nfx_tip_force(nfx_ctx* ctx, const float* points, int64_t N, int32_t indenter, double k, double radius,
-                  double pos, double* out2);
+ * device: out2[0] = indenter force, out2[1] = max penetration (device pointer to 2 doubles).  With
+ * ref non-NULL, points holds displacements from ref and the penetrations are taken in FP64 on
+ * x = ref + points. */
+int nfx_tip_force(nfx_ctx* ctx, const float* points, const float* ref, int64_t N, int32_t indenter, double k,
+                  double radius, double pos, double* out2);
 
 /* ---- legacy contacts and integrators (the kernels SimulationLegacy.py:139-330, 550-617 and
  *      884-940 use), restated as written; force/points/vel/fold are 3N SoA FP32 ------------------ */
@@ -287,11 +302,15 @@ int nfx_adhesive_plane_surface_force(nfx_ctx* ctx, float* force, const float* po
 /* ---- fused stepping: Simulation.timeStep (Simulation.py:232-237) for steps
  *      [step0, step0 + nsteps): force pass, then one node pass that applies the substrate and
  *      indenter terms at the device-computed indenter position (Simulation.py:117-132, 173-183)
- *      and the damped velocity-Verlet update.  tip_hist (device, nullable, nsteps doubles)
- *      receives the FP64 indenter force of each step.  use_graph != 0 replays the steps from a
- *      captured hipGraph of up to 100 steps. ------------------------------------------------- */
+ *      and the damped velocity-Verlet update, on the displacement state (plan.ref, plan.disp).
+ *      tip_hist (device, nullable, nsteps doubles) receives the FP64 indenter force of each step.
+ *      use_graph != 0 replays whole blocks of 100 steps from a captured hipGraph (captured once per
+ *      plan and history mode, and reused whatever nsteps is); the remainder runs eagerly. ------ */
 int nfx_run_steps(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int64_t nsteps, double* tip_hist,
                   int32_t use_graph);
+/* number of step blocks the context has captured into graphs so far (a replay of a cached block
+ * does not count) */
+int nfx_graph_captures(nfx_ctx* ctx, int64_t* count);
 /* the node pass alone (for callers that enqueue the force pass themselves) */
 int nfx_node_update(nfx_ctx* ctx, const nfx_plan* plan, int64_t step, double* tip_out);
 

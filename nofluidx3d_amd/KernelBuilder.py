@@ -1,4 +1,5 @@
-"""Kernel registry with the API of reference ``nofluidx3d/KernelBuilder.py:25-62``.
+"""Kernel registry with the API of reference ``nofluidx3d/KernelBuilder.py:25-62``, for plugins
+written against the reference (the built-in plugins bind their libnfx entry points directly).
 
 The reference concatenates process-global ``#define``s with shared headers and a ``.cl`` file and
 JIT-compiles it per plugin.  Here every kernel is precompiled for gfx950 inside libnfx.so;
@@ -49,9 +50,42 @@ def _f(x):
     return float(getattr(x, "value", x))
 
 
+_inverse_cache = {}
+
+
+def _reference_inverse(edges, volumes, T):
+    """(Dm^-1, V0) device pointers for calcTetraForce's reference-geometry arguments.  The
+    reference's plugin passes its FP64 edge vectors (9T, SoA) and volumes (T) (``Tetra.py:49-58``):
+    those are inverted once with the FP64 precompute and the result kept for the buffers' lifetime.
+    FP32 arguments of the same sizes are taken as an inverse already (what ``Tetra`` itself holds).
+    Anything else is refused: read as the wrong quantity, it would give wrong forces silently."""
+    import torch
+
+    if not (isinstance(edges, torch.Tensor) and isinstance(volumes, torch.Tensor)):
+        raise _lib.NfxError("calcTetraForce: reference edge vectors and volumes must be device tensors")
+    if edges.numel() != 9 * T or volumes.numel() != T:
+        raise _lib.NfxError(f"calcTetraForce: expected 9T = {9 * T} edge-vector and T = {T} volume entries, got "
+                            f"{edges.numel()} and {volumes.numel()}")
+    if edges.dtype == torch.float32 and volumes.dtype == torch.float32:
+        return edges.data_ptr(), volumes.data_ptr()
+    if edges.dtype != torch.float64 or volumes.dtype != torch.float64:
+        raise _lib.NfxError(f"calcTetraForce: edge vectors/volumes must both be float64 (the reference's buffers) or "
+                            f"both float32 (an inverse), got {edges.dtype} and {volumes.dtype}")
+    key = (edges.data_ptr(), volumes.data_ptr(), T)
+    hit = _inverse_cache.get(key)
+    if hit is None:
+        dminv = torch.empty(9 * T, dtype=torch.float32, device=edges.device)
+        v0 = torch.empty(T, dtype=torch.float32, device=edges.device)
+        torch.cuda.synchronize(edges.device)
+        _lib.call("nfx_precompute_f64", openCL.handle(), edges.data_ptr(), volumes.data_ptr(), T, dminv.data_ptr(),
+                  v0.data_ptr())
+        hit = _inverse_cache[key] = (dminv, v0, edges, volumes)  # the sources stay referenced with their key
+    return hit[0].data_ptr(), hit[1].data_ptr()
+
+
 def _launch_tetra(k: Kernel):
-    # calcTetraForce(force, points, tetras, referenceEdgeVectors -> Dm^-1, referenceVolumes)
-    force, points, tetras, dminv, v0 = k.args
+    # calcTetraForce(force, points, tetras, referenceEdgeVectors, referenceVolumes)
+    force, points, tetras, edges, volumes = k.args
     if k.values.get("TETRA_SAINT_VENANT_KIRCHHOFF_KELLNBERGER"):
         model = _lib.NFX_SVKK
     elif k.values.get("TETRA_SAINT_VENANT_KIRCHHOFF"):
@@ -63,7 +97,8 @@ def _launch_tetra(k: Kernel):
     mat = _lib.Elastic(model, float(k._v("def_tetraYoungsModulus")), float(k._v("def_tetraPoissonRatio")),
                        float(k._v("def_tetraMooneyRivlinRatio", 1)))
     N, T = int(k._v("def_pointCount")), int(k._v("def_tetraCount"))
-    _lib.call("nfx_tetra_force", openCL.handle(), mat, _p(points), _p(tetras), _p(dminv), _p(v0), N, T, _p(force))
+    dminv, v0 = _reference_inverse(edges, volumes, T)
+    _lib.call("nfx_tetra_force", openCL.handle(), mat, _p(points), _p(tetras), dminv, v0, N, T, _p(force))
 
 
 def _launch_sphere(k: Kernel):

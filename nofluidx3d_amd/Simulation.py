@@ -104,8 +104,17 @@ class Simulation:
         self.interactions = []
         self.time = 0
 
-        # device buffers shared by several kernels (Simulation.py:82-90), FP32 state on the device
-        self.points = Bridge(self.cell.points.reshape(3 * self.numPoints, order="F"), datatype=np.float64)
+        # device buffers shared by several kernels (Simulation.py:82-90), FP32 state on the device.
+        # Positions have two forms: ``points`` holds absolute positions (the reference's buffer,
+        # which the per-plugin kernels advance) and ``displacement`` holds u = x - X0 from the
+        # reference positions ``refPoints`` (the fused step's state, which keeps FP32 rounding
+        # relative to the displacement rather than to |x|).  Whichever was advanced last is
+        # current; the other is rebuilt from it on demand.
+        self.device = openCL.getContext().torch_device
+        self.points = _Positions(self, self.cell.points.reshape(3 * self.numPoints, order="F"))
+        self.refPoints = self.points.buf.clone()
+        self.displacement = self.points.buf.new_zeros(3 * self.numPoints)
+        self._pos_current = {"points", "disp"}
         self.tetras = Bridge(self.cell.tetras.reshape(4 * self.numTetra, order="F"), datatype=np.uint32)
         self.force = Bridge(np.zeros(self.numPoints * 3), datatype=np.float64)
         self.vonMises = Bridge(np.zeros(self.numTetra), datatype=np.float64)
@@ -122,6 +131,23 @@ class Simulation:
         self._csr = None
         self._tiles = None
         self._tip = None
+
+    # ---------------------------------------------------------------------------------------------
+    # the two position forms
+    # ---------------------------------------------------------------------------------------------
+    def _use_points(self):
+        """make the absolute positions current (before per-plugin steps or a read of ``points``)"""
+        if "points" not in self._pos_current:
+            _lib.call("nfx_points_from_disp", openCL.handle(), self.refPoints.data_ptr(), self.displacement.data_ptr(),
+                      self.numPoints, self.points.ptr)
+            self._pos_current.add("points")
+
+    def _use_disp(self):
+        """make the displacements current (before fused steps)"""
+        if "disp" not in self._pos_current:
+            _lib.call("nfx_disp_from_points", openCL.handle(), self.refPoints.data_ptr(), self.points.ptr,
+                      self.numPoints, self.displacement.data_ptr())
+            self._pos_current.add("disp")
 
     def _print(self, *a, **k):
         if self.verbose:
@@ -309,6 +335,7 @@ class Simulation:
     def _record_force(self, inter):
         import torch
 
+        self._use_points()
         saved = self.force.buf.clone()
         self.force.buf.zero_()
         torch.cuda.synchronize()
@@ -401,7 +428,7 @@ class Simulation:
             slots = np.zeros(4 * self.numTetra, dtype=np.uint32)
             _lib.call("nfx_build_node_csr_host", tets.ctypes.data_as(ctypes.c_void_p), self.numPoints, self.numTetra,
                       off.ctypes.data_as(ctypes.c_void_p), slots.ctypes.data_as(ctypes.c_void_p))
-            dev = self.points.buf.device
+            dev = self.device
             self._csr = (
                 torch.from_numpy(off.view(np.int32)).to(dev),
                 torch.from_numpy(slots.view(np.int32)).to(dev),
@@ -438,7 +465,7 @@ class Simulation:
                       ctypes.byref(mx), ctypes.byref(m), count.ctypes.data_as(vp), tile_nodes.ctypes.data_as(vp),
                       tile_lend.ctypes.data_as(vp), tile_corners.ctypes.data_as(vp), tile_dst.ctypes.data_as(vp),
                       node_off.ctypes.data_as(vp))
-            dev = self.points.buf.device
+            dev = self.device
             to = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a.view(np.int16)).to(dev)
             self._tiles = {
                 "tile_k": K,
@@ -460,16 +487,20 @@ class Simulation:
         Plugin state is brought into the form the fused step advances (callers step right after)."""
         p = self._build_plan()
         if p is not None:
+            self._use_disp()
             for inter in self.interactions:
-                inter.state_form(shape=p.scatter == _lib.SCATTER_TILED)
+                inter.state_form(fused=True)
         return p
 
     def sync_forms(self):
-        """Bring every plugin state representation up to date (before a snapshot of the state)."""
+        """Bring every state representation up to date (before a snapshot of the state)."""
+        self._use_points()
+        self._use_disp()
         for inter in self.interactions:
             inter.sync_forms()
 
     def forms_restored(self):
+        self._pos_current = {"points", "disp"}
         for inter in self.interactions:
             inter.forms_restored()
 
@@ -481,7 +512,8 @@ class Simulation:
             return self._plan
         p = _lib.Plan()
         p.N, p.T = self.numPoints, self.numTetra
-        p.tetras, p.points, p.force = self.tetras.ptr, self.points.ptr, self.force.ptr
+        p.tetras, p.force = self.tetras.ptr, self.force.ptr
+        p.ref, p.disp = self.refPoints.data_ptr(), self.displacement.data_ptr()
         ok = len(self.interactions) >= 2 and isinstance(self.interactions[-1], VelocityVerlet)
         for inter in self.interactions:
             ok = ok and inter.contribute(p)
@@ -522,18 +554,22 @@ class Simulation:
             return
         _lib.call("nfx_run_steps", openCL.handle(), p, self.time, nsteps,
                   None if tip_hist is None else tip_hist.data_ptr(), 1 if self.use_graph else 0)
+        self._pos_current = {"disp"}
         self.time += nsteps
 
     def timeStep(self):
         p = self.plan()
         if p is not None:
             _lib.call("nfx_run_steps", openCL.handle(), p, self.time, 1, None, 0)
+            self._pos_current = {"disp"}
             self.time += 1
             return
+        self._use_points()
         for interaction in self.interactions:
             interaction.beforeTimeStep(self.time)
         for interaction in self.interactions:
             interaction.enqueue()
+        self._pos_current = {"points"}
         self.time += 1
 
     def tip_force(self, indenter, k, radius, pos):
@@ -542,28 +578,42 @@ class Simulation:
         import torch
 
         if self._tip is None:
-            self._tip = torch.zeros(2, dtype=torch.float64, device=self.points.buf.device)
-        _lib.call("nfx_tip_force", openCL.handle(), self.points.ptr, self.numPoints, indenter, float(k), float(radius),
+            self._tip = torch.zeros(2, dtype=torch.float64, device=self.device)
+        if "disp" in self._pos_current:
+            x, ref = self.displacement.data_ptr(), self.refPoints.data_ptr()
+        else:
+            x, ref = self.points.ptr, None
+        _lib.call("nfx_tip_force", openCL.handle(), x, ref, self.numPoints, indenter, float(k), float(radius),
                   float(pos), self._tip.data_ptr())
         getCommandQueue().finish()
         out = self._tip.cpu().numpy()
         return float(out[0]), float(out[1])
 
     def state_tensors(self):
-        """Device tensors that carry the simulation state (positions, force, plugin state)."""
-        out = [self.points.buf, self.force.buf, self.vonMises.buf, self.pressure.buf]
+        """Device tensors that carry the simulation state (positions, force, plugin state); call
+        ``sync_forms`` before taking them and ``forms_restored`` after writing them back."""
+        out = [self.points.buf, self.displacement, self.force.buf, self.vonMises.buf, self.pressure.buf]
         for inter in self.interactions:
-            for name in ("velocityB", "forceOldB", "plasticDeformationTensorB", "shapeB", "accumulatedStrainB"):
-                t = getattr(inter, name, None)
-                if t is not None:
-                    out.append(t)
+            out.extend(inter.state_buffers())
         return out
 
     def positions(self) -> np.ndarray:
-        """Current node positions (N, 3) float64."""
+        """Current node positions (N, 3) float64.  From the displacement form they are X0 + u
+        summed in float64 (both are FP32 values), which keeps the displacement's precision."""
+        if "disp" in self._pos_current:
+            getCommandQueue().finish()
+            ref = self.refPoints.cpu().numpy().astype(np.float64)
+            u = self.displacement.cpu().numpy().astype(np.float64)
+            return (ref + u).reshape(3, self.numPoints).T.copy()
         return self.points.readFromGPU().reshape(3, self.numPoints).T.copy()
 
+    def _record(self):
+        return [quantity() for quantity in self.recordedQuantities]
+
     def run(self, writeStress=True, writePressure=True, writeVolumeChange=False):
+        """The reference's run loop (``Simulation.py:239-290``): a frame and a row of recorded
+        quantities at t = 0 and after every ``VTKInterval`` steps, ``VTKFramesTotal`` times;
+        ``simdata.dat`` is rewritten with all rows after each frame."""
         self._print("\n\n")
         self._print("+------------------------------------+")
         self._print("|         Starting Simulation        |")
@@ -574,41 +624,53 @@ class Simulation:
             if isinstance(inter, (FiniteStrainViscoplastic, LegacyElastic, LinearViscoelastic)) and \
                     (writeStress or writePressure):
                 inter.write_stress = True
-                inter.setArgs()
                 self._plan = None
-        starttimes = []
-        endtimes = []
-        numberCycles = self.parameters["VTKFramesTotal"]
+        frames = self.parameters["VTKFramesTotal"]
+        simdata = self.parameters["OutputDir"] + "simdata.dat"
 
-        writeVTK(self, writeStress=writeStress, writePressure=writePressure, writeVolumeChange=writeVolumeChange)
-        newDataPoints = []
-        for quant in self.recordedQuantities:
-            newDataPoints.append(quant())
-        dataPoints = np.array(newDataPoints)
-        np.savetxt(self.parameters["OutputDir"] + "simdata.dat", dataPoints)
+        def frame():
+            writeVTK(self, writeStress=writeStress, writePressure=writePressure, writeVolumeChange=writeVolumeChange)
+            rows.append(self._record())
+            np.savetxt(simdata, np.array(rows))
 
-        for i in range(numberCycles):
-            starttimes.append(time.time())
+        rows = []
+        frame()
+        step_seconds = []  # wall time of each block of VTKInterval steps
+        for done in range(1, frames + 1):
+            t_block = time.time()
             self.advance(self.VTKInterval)
             getCommandQueue().finish()
-            anatime1 = time.time()
-            writeVTK(self, writeStress=writeStress, writePressure=writePressure, writeVolumeChange=writeVolumeChange)
-            newDataPoints = []
-            for quant in self.recordedQuantities:
-                newDataPoints.append(quant())
-            dataPoints = np.vstack((dataPoints, newDataPoints))
-            np.savetxt(self.parameters["OutputDir"] + "simdata.dat", dataPoints)
-            anatime2 = time.time()
-            endtimes.append(time.time())
-            cycleTime = np.average(np.array(endtimes) - np.array(starttimes))
-            vtktime = anatime2 - anatime1
+            t_frame = time.time()
+            frame()
+            step_seconds.append(t_frame - t_block)
+            frame_seconds = time.time() - t_frame
+            mean_block = sum(step_seconds) / len(step_seconds)
+            remaining_min = (frames - done + 1) * (mean_block + frame_seconds) / 60
             self._print(
-                f"| Average # of simulation steps per second: {np.round(self.VTKInterval / cycleTime, 1)} "
-                f"| VTK cycle took {np.round(vtktime, 1)} seconds | estimate of time remaining: "
-                f"{np.round(((numberCycles - i) * (cycleTime + vtktime)) / 60, 1)} min",
+                f"| Average # of simulation steps per second: {np.round(self.VTKInterval / mean_block, 1)} "
+                f"| VTK cycle took {np.round(frame_seconds, 1)} seconds | estimate of time remaining: "
+                f"{np.round(remaining_min, 1)} min",
                 end="\r",
             )
         self._print("\n\n")
         self._print("+------------------------------------+")
         self._print("|  Simulation finished successfully  |")
         self._print("+------------------------------------+")
+
+
+class _Positions(Bridge):
+    """The ``points`` buffer: reading it (``readFromGPU``) first rebuilds the absolute positions
+    when the displacement form was advanced last, and writing it (``writeToGPU``) makes the
+    absolute form the current one."""
+
+    def __init__(self, simulation, host):
+        super().__init__(host, datatype=np.float64)
+        self._simulation = simulation
+
+    def readFromGPU(self):
+        self._simulation._use_points()
+        return super().readFromGPU()
+
+    def writeToGPU(self):
+        super().writeToGPU()
+        self._simulation._pos_current = {"points"}

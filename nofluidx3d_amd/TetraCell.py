@@ -1,9 +1,13 @@
-"""Tetrahedral cell mesh with its FP64 reference quantities (reference ``nofluidx3d/TetraCell.py``).
+"""The reference cell of a simulation (reference ``nofluidx3d/TetraCell.py``): its tetrahedral mesh
+in simulation units and the per-tetra reference quantities the force laws start from.
 
-``TetraCell(path, radius=18.0, recenter=True)`` reads a legacy-VTK unstructured grid (or takes an
-in-memory ``UnstructuredGrid``), translates point 0 to the origin, scales so that point 1's
-x-coordinate has magnitude ``radius`` (``TetraCell.py:15-20``), and computes the edge vectors
-``[p1-p0, p2-p0, p3-p0]`` (T, 9) and the signed volumes (``TetraCell.py:26-41``) in FP64.
+``TetraCell(path, radius=18.0, recenter=True)`` takes a legacy-VTK file (or an in-memory
+``UnstructuredGrid``).  With ``recenter`` the mesh is moved so that node 0 is the origin and scaled
+so that node 1 lies at |x| = ``radius`` (``TetraCell.py:15-20``).  Node coordinates are then held
+as the FP32 values the device keeps (stored in float64 arrays), so that the FP64 precompute, the
+device state and the oracle all start from one set of positions.  ``edgeVectors`` (T, 9: R1, R2,
+R3 with R_k = x_k - x_0) and the signed ``volumes`` R1 . (R2 x R3) / 6 follow
+(``TetraCell.py:26-41``).
 """
 
 from __future__ import annotations
@@ -13,50 +17,67 @@ import numpy as np
 from nofluidx3d_amd.util import vtk
 
 
+def _normalised(grid, radius):
+    """node 0 at the origin, |x of node 1| = radius"""
+    origin = np.array(grid.points[0], dtype=np.float64)
+    grid.translate(-origin)
+    grid.scale(radius / abs(float(grid.points[1][0])), inplace=True)
+
+
+def _device_values(points):
+    """the coordinates as the FP32 device state holds them, in float64"""
+    return np.asarray(points, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
 class TetraCell:
     def __init__(self, path, radius=18.0, recenter=True):
-        if isinstance(path, vtk.UnstructuredGrid):
-            self.mesh = path.copy()
-        else:
-            self.mesh = vtk.read(path)
+        grid = path.copy() if isinstance(path, vtk.UnstructuredGrid) else vtk.read(path)
         if recenter:
-            transvec = -self.mesh.points[0]  # point 0 of the reference mesh marks the origin
-            self.mesh.translate(transvec)
-            radiusMesh = np.abs(self.mesh.points[1][0])  # x-value of point 1
-            self.mesh.scale(radius / radiusMesh, inplace=True)
-        self.points = self.mesh.points
+            _normalised(grid, radius)
+        grid.points = _device_values(grid.points)
+        self.mesh = grid
+        self.points = grid.points  # the reference positions (frames later replace mesh.points)
         self.radius = radius
-        self.tetras = self.mesh.cells_dict[vtk.VTK_TETRA]
-        R1, R2, R3 = self.calcEdgeVectors(self.points)
-        self.edgeVectors = np.stack([R1, R2, R3], 1).reshape(len(R1), 9)
-        self.volumes = self.calcVolumes(R1, R2, R3)
+        self.tetras = grid.cells_dict[vtk.VTK_TETRA]
+        self._set_reference()
 
     @classmethod
     def from_arrays(cls, points, tets, radius=18.0, recenter=True):
         return cls(vtk.from_tetra(points, tets), radius=radius, recenter=recenter)
 
+    def _set_reference(self):
+        edges = self._edges(self.points)  # (T, 3, 3): edge k of tetra t in edges[t, k]
+        self.edgeVectors = edges.reshape(len(edges), 9)
+        self.volumes = np.linalg.det(edges) / 6.0
+
+    def _edges(self, points):
+        corners = np.asarray(points, dtype=np.float64)[self.tetras]  # (T, 4, 3)
+        return corners[:, 1:, :] - corners[:, :1, :]
+
     def calcEdgeVectors(self, points):
-        p0 = points[self.tetras[:, 0]]
-        p1 = points[self.tetras[:, 1]]
-        p2 = points[self.tetras[:, 2]]
-        p3 = points[self.tetras[:, 3]]
-        return p1 - p0, p2 - p0, p3 - p0
+        """(R1, R2, R3), each (T, 3): the edges from node 0 of every tetra at ``points``."""
+        e = self._edges(points)
+        return e[:, 0], e[:, 1], e[:, 2]
 
     def calcVolumes(self, R1, R2, R3):
-        return np.einsum("ij,ij->i", R1, np.cross(R2, R3)) / 6
+        """signed volumes R1 . (R2 x R3) / 6"""
+        return np.linalg.det(np.stack([R1, R2, R3], axis=1)) / 6.0
 
     def scale(self, scaling):
         self.mesh.scale(scaling, inplace=True)
+        self.mesh.points = _device_values(self.mesh.points)
         self.points = self.mesh.points
-        R1, R2, R3 = self.calcEdgeVectors(self.points)
-        self.edgeVectors = np.stack([R1, R2, R3], 1).reshape(len(R1), 9)
-        self.volumes = self.calcVolumes(R1, R2, R3)
+        self._set_reference()
 
     def addTetraData(self, name, dataArray):
-        # pad at the front to the mesh's cell count (non-tetra cells come first), TetraCell.py:53-56
-        dataArrayPadded = np.pad(dataArray, pad_width=(self.mesh.n_cells - dataArray.size, 0))
-        self.mesh.cell_data.set_array(dataArrayPadded, name)
+        """Attach a per-tetra array as cell data; cells before the tetra (the mesh's other cell
+        types come first) get zeros (``TetraCell.py:53-56``)."""
+        values = np.asarray(dataArray)
+        padded = np.zeros(self.mesh.n_cells, dtype=values.dtype)
+        padded[self.mesh.n_cells - values.size:] = values
+        self.mesh.cell_data.set_array(padded, name)
 
     def calcVolumeChange(self):
-        curVolumes = self.calcVolumes(*self.calcEdgeVectors(self.mesh.points))
-        self.addTetraData("volumeChange", curVolumes / self.volumes - 1)
+        """``volumeChange`` = V / V0 - 1 at the mesh's current positions."""
+        current = np.linalg.det(self._edges(self.mesh.points)) / 6.0
+        self.addTetraData("volumeChange", current / self.volumes - 1.0)

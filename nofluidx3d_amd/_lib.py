@@ -54,7 +54,8 @@ class Plan(ctypes.Structure):
         ("tetras", c_vp),
         ("dminv", c_vp),
         ("v0", c_vp),
-        ("points", c_vp),
+        ("ref", c_vp),
+        ("disp", c_vp),
         ("vel", c_vp),
         ("force", c_vp),
         ("fold", c_vp),
@@ -63,7 +64,7 @@ class Plan(ctypes.Structure):
         ("nu", c_f64),
         ("ratio", c_f64),
         ("vp", Viscoplastic),
-        ("Fp", c_vp),
+        ("vp_state", c_vp),
         ("epsbar", c_vp),
         ("von_mises", c_vp),
         ("pressure", c_vp),
@@ -90,7 +91,6 @@ class Plan(ctypes.Structure):
         ("ind_speed", c_f64),
         ("ind_move_steps", c_f64),
         ("fix_top_bottom", c_i32),
-        ("vp_shape", c_vp),
         ("shear", c_f64),
         ("bulk", c_f64),
         ("shear2", c_f64),
@@ -166,7 +166,11 @@ SIGNATURES = {
     "nfx_viscoplastic_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Viscoplastic), c_vp, c_vp, c_vp, c_i64, c_i64,
                                               c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
-    "nfx_vp_shape_from_fp": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_vp_q_from_fp": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "nfx_vp_fp_from_q": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "nfx_points_from_disp": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_disp_from_points": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "nfx_graph_captures": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i64)]),
     "nfx_round_tip_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
     "nfx_linear_viscoelastic_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Viscoelastic), c_vp, c_vp, c_vp, c_i64,
                                                       c_i64, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
@@ -181,8 +185,6 @@ SIGNATURES = {
     "nfx_adhesive_plane_surface_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_f64, c_f64, c_f64]),
     "nfx_legacy_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Elastic), c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp,
                                          c_vp, c_vp]),
-    "nfx_vp_fp_from_shape": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
-    "nfx_vp_sym_from_shape": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
     "nfx_batch_geometry": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_i64, c_vp, c_vp, c_vp]),
     "nfx_build_node_csr_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_vp, c_vp]),
     "nfx_build_tiles_host": (ctypes.c_int, [c_vp, c_i64, c_i64, c_i32, ctypes.POINTER(c_i32), ctypes.POINTER(c_i64),
@@ -191,7 +193,7 @@ SIGNATURES = {
     "nfx_substrate_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
     "nfx_plane_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
     "nfx_velocity_verlet": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i32]),
-    "nfx_tip_force": (ctypes.c_int, [c_vp, c_vp, c_i64, c_i32, c_f64, c_f64, c_f64, c_vp]),
+    "nfx_tip_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i32, c_f64, c_f64, c_f64, c_vp]),
     "nfx_run_steps": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i64, c_vp, c_i32]),
     "nfx_node_update": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_vp]),
     "nfx_time_passes": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i32, ctypes.POINTER(c_f64),

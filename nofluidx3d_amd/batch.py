@@ -164,9 +164,10 @@ def make_meshes(specs, workers=None, cache="/tmp/nfx_cells"):
 
 
 def sim_points(P_um, radius_sim):
-    """TetraCell's recentre and scale (TetraCell.py:15-20) in FP64."""
+    """TetraCell's recentre and scale (TetraCell.py:15-20) in FP64, held at the FP32 values the
+    device keeps as the reference positions (as ``TetraCell`` holds them)."""
     p = np.asarray(P_um, np.float64) - P_um[0]
-    return p * (radius_sim / abs(p[1, 0]))
+    return (p * (radius_sim / abs(p[1, 0]))).astype(np.float32).astype(np.float64)
 
 
 def narrow_tile_arrays(tile_nodes, tile_dst, tile_count, MX):
@@ -312,8 +313,13 @@ class BatchSimulation:
     Options (none changes the physics; each cuts bytes per step):
     ``plastic_strain`` keep the accumulated plastic strain (the linear Maxwell law does not read it);
     ``stream_nt`` cache policy of the force pass's streams (None: nontemporal beyond the 256 MB
-    memory-side cache); ``sym_state`` viscoplastic state G = S S^T (6 floats) instead of S (9);
-    ``narrow`` 16-bit tile-node ids and partial positions (when every cell has < 65536 nodes)."""
+    memory-side cache); ``sym_state`` viscoplastic state Gd = Fp^-1 Fp^-T - I (6 floats) instead of
+    Q = Fp^-1 - I (9);
+    ``narrow`` 16-bit tile-node ids and partial positions (when every cell has < 65536 nodes).
+
+    State on the device: node displacements u from the FP32 reference positions X0 (``ref``), the
+    velocity-Verlet state w = a v + f/2, and for the viscoplastic law the plastic deviation Gd
+    (``sym_state``, 6 per tetra) or Q (9 per tetra), all zero at the start."""
 
     def __init__(self, specs, meshes, steps=10000, ncurve=256, tile_k=256, use_graph=True, device_id=0,
                  plastic_strain=False, stream_nt=None, sym_state=True, narrow=True):
@@ -345,7 +351,8 @@ class BatchSimulation:
         f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(dev)
         u32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.uint32).view(np.int32)).to(dev)
         u16 = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.uint16).view(np.int16)).to(dev)
-        self.x = f32(h["x"].T.reshape(-1))
+        self.ref = f32(h["x"].T.reshape(-1))
+        self.x = torch.zeros(3 * self.N, dtype=torch.float32, device=dev)  # displacements
         self.v = torch.zeros(3 * self.N, dtype=torch.float32, device=dev)
         self.fold = torch.zeros(3 * self.N, dtype=torch.float32, device=dev)
         self.tetras = u32(h["tets"].T.reshape(-1))
@@ -361,19 +368,14 @@ class BatchSimulation:
         del ev, vol
         self.sym_state = bool(sym_state) and self.model == _lib.NFX_VISCOPLASTIC
         if self.model == _lib.NFX_VISCOPLASTIC:
-            # state in shape form S = Dm^-1 Fp^-1 (S = Dm^-1 at Fp = I), or, by default, its symmetric
-            # part G = S S^T: the stress, flow and forces depend on G alone, and the batch never
-            # reports Fp, so a third of the state traffic is saved
-            if self.sym_state:
-                self.shape = torch.empty(6 * self.T, dtype=torch.float32, device=dev)
-                _lib.call("nfx_vp_sym_from_shape", hh, self.dminv.data_ptr(), self.T, self.shape.data_ptr())
-                _lib.call("nfx_stream_sync", hh)
-            else:
-                self.shape = self.dminv.clone()
+            # the plastic deviation, zero while Fp = I: by default its symmetric form Gd (the stress,
+            # flow and forces depend on Gd alone, and the batch never reports Fp, so a third of the
+            # state traffic is saved), else Q
+            self.state = torch.zeros((6 if self.sym_state else 9) * self.T, dtype=torch.float32, device=dev)
             # the linear Maxwell law does not read the accumulated plastic strain: kept on request
             self.eps = torch.zeros(self.T, dtype=torch.float32, device=dev) if plastic_strain else None
         else:
-            self.shape = self.eps = None
+            self.state = self.eps = None
         self.t_count, self.t_nodes, self.t_lend = u32(h["tile_count"]), u32(h["tile_nodes"]), u16(h["tile_lend"])
         self.t_corners, self.t_dst = u16(h["tile_corners"]), u32(h["tile_dst"])
         self.t_ntet, self.t_cell = u16(h["tile_ntet"]), u32(h["tile_cell"])
@@ -396,11 +398,12 @@ class BatchSimulation:
         p = _lib.Plan()
         p.N, p.T = self.N, self.T
         p.tetras, p.dminv, p.v0 = self.tetras.data_ptr(), self.dminv.data_ptr(), self.v0.data_ptr()
-        p.points, p.vel, p.fold = self.x.data_ptr(), self.v.data_ptr(), self.fold.data_ptr()
+        p.ref, p.disp = self.ref.data_ptr(), self.x.data_ptr()
+        p.vel, p.fold = self.v.data_ptr(), self.fold.data_ptr()
         p.force = 0
         p.model = self.model
-        if self.shape is not None:
-            p.vp_shape = self.shape.data_ptr()
+        if self.state is not None:
+            p.vp_state = self.state.data_ptr()
             p.epsbar = self.eps.data_ptr() if self.eps is not None else 0
             p.vp.flow_exponent, p.vp.hardening_exponent = 1.0, 0.0  # every cell: linear Maxwell flow
             p.vp_sym = 1 if self.sym_state else 0
@@ -447,13 +450,12 @@ class BatchSimulation:
         torch.cuda.synchronize(dev)
         hh = openCL.handle()
         _lib.call("nfx_batch_geometry", hh, self.p_rel.data_ptr(), scale.data_ptr(), self.node_cell.data_ptr(),
-                  self.tetras.data_ptr(), self.t_ntet.data_ptr(), self.tile_k, self.N, self.T, self.x.data_ptr(),
+                  self.tetras.data_ptr(), self.t_ntet.data_ptr(), self.tile_k, self.N, self.T, self.ref.data_ptr(),
                   self.dminv.data_ptr(), self.v0.data_ptr())
-        if self.sym_state:
-            _lib.call("nfx_vp_sym_from_shape", hh, self.dminv.data_ptr(), self.T, self.shape.data_ptr())
         _lib.call("nfx_stream_sync", hh)
-        if self.shape is not None and not self.sym_state:
-            self.shape.copy_(self.dminv)
+        if self.state is not None:
+            self.state.zero_()
+        self.x.zero_()
         self.v.zero_()
         self.fold.zero_()
         if self.eps is not None:
@@ -504,9 +506,12 @@ class BatchSimulation:
         return self.curves_sim() / (E_SIM * self.radius_sim[:, None] ** 2)
 
     def positions(self, i):
+        """cell i's node positions X0 + u (N_i, 3), summed in float64"""
         self.sync()
-        X = self.x.cpu().numpy().reshape(3, self.N).T
-        return X[self.node_off[i]:self.node_off[i + 1]].astype(np.float64)
+        sl = slice(self.node_off[i], self.node_off[i + 1])
+        ref = self.ref.cpu().numpy().reshape(3, self.N).T[sl].astype(np.float64)
+        u = self.x.cpu().numpy().reshape(3, self.N).T[sl].astype(np.float64)
+        return ref + u
 
     def single_cell_config(self, i, outdir="/tmp/nfx_batch_cell/"):
         """The reference-format config of cell i (for the single-cell Simulation path)."""

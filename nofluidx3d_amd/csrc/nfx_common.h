@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/nfx.h"
 
@@ -44,6 +45,7 @@ void set_error(const char* fmt, ...);
 // reduction scratch: per-block FP64 partials and a completion counter, allocated once per context
 constexpr int kMaxReduceBlocks = 2048;
 constexpr int kGraphBlock = 100;  // steps per captured graph
+constexpr int kMaxGraphs = 32;    // captured graphs a context keeps
 
 }  // namespace nfx
 
@@ -57,22 +59,24 @@ struct nfx_ctx {
     int64_t* step_dev = nullptr;      // device step counter for graph replays
     double* hist_dev = nullptr;       // kGraphBlock doubles, tip force of the replayed block
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // captured graph cache
-    hipGraphExec_t graph_exec = nullptr;
-    hipGraph_t graph = nullptr;
-    nfx_plan graph_plan{};
-    int64_t graph_steps = 0;
-    bool graph_hist = false;
-    // batched stepping: the batch being enqueued (NULL otherwise) and its graph cache
+    // batched stepping: the batch being enqueued (NULL otherwise)
     const nfx_batch* batch = nullptr;
     // ... and, while its force pass is enqueued with the step's curve sample riding in it, that step
     bool curve_fused = false;
     int64_t curve_step = 0;
     const int64_t* curve_step_dev = nullptr;
     int32_t curve_off = 0;
-    hipGraphExec_t bgraph_exec = nullptr;
-    hipGraph_t bgraph = nullptr;
-    nfx_plan bgraph_plan{};
-    nfx_batch bgraph_batch{};
-    int64_t bgraph_steps = 0;
+    // captured blocks of kGraphBlock steps, one per (plan, batch, history mode), least recently
+    // used evicted past kMaxGraphs
+    struct GraphEntry {
+        nfx_plan plan;
+        nfx_batch batch;
+        bool is_batch, hist;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        uint64_t last_use;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0;
+    int64_t captures = 0;  // graphs captured so far (nfx_graph_captures)
 };

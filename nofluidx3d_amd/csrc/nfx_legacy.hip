@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kB) void k_lve(float* f, float* corners, const floa
         pr[t] = t3;
     }
     const float detr = dot(r1, cross(r2, r3));
-    const Corners fc = cofactor_forces(S, r1, r2, r3, detr < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
+    const Corners fc = cofactor_forces_r(S, r1, r2, r3, detr < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
     const uint32_t id[4] = {n0, n1, n2, n3};
     const V3 fa[4] = {fc.f0, fc.f1, fc.f2, fc.f3};
     if constexpr (CORNERS) {

@@ -1,13 +1,16 @@
 // libnfx: plain (single-cell) path of the explicit tetrahedral-FEM step on gfx950.
 //
 // Passes per step (Simulation.timeStep, Simulation.py:232-237):
-//   force pass  one thread per tetra: gather 4 nodes, F = r * Dm^-1, stress, 4 corner forces;
-//               either float atomics into the node force (the reference's scatter,
-//               atomicAdd.cl:1-11) or a 12-float corner record per tetra (deterministic mode).
-//   node pass   one node per thread (atomic mode) or 4 lanes per node that sum the node's corner
-//               records in CSR order and combine them with a fixed xor butterfly (deterministic
-//               mode); then substrate + indenter terms at the device-computed indenter position,
-//               the damped velocity-Verlet update, and the FP64 indenter-force reduction.
+//   force pass  gather 4 nodes, F = I + [u1-u0|u2-u0|u3-u0] Dm^-1 from node displacements u,
+//               stress, 4 corner forces, reduced to nodes by one of three scatters: float atomics
+//               into the node force (the reference's atomicAdd.cl:1-11), a 12-float corner record
+//               per tetra gathered through the node CSR, or per-tile LDS sums (the default).
+//   node pass   the node's force (atomic sum, CSR gather with 4 lanes per node and a fixed xor
+//               butterfly, or its tile partials in order); substrate + indenter terms in FP64 at
+//               x = X0 + u and the device-computed indenter position; the damped velocity-Verlet
+//               update of u; the FP64 indenter-force reduction.
+// The per-plugin kernels (one per reference kernel) take absolute positions, as the reference's
+// buffers hold them.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -148,10 +151,11 @@ __global__ __launch_bounds__(kBlock) void k_precompute_f64(const double* __restr
 }
 
 // A batch's geometry in simulation units from its cells' recentred positions p (FP64, micrometres)
-// and per-cell scales f (RadiusSIM / |p[1].x|, TetraCell.py:15-20): positions x = p f as FP32, and
-// per tetra the edges R_k = x_k - x_0, V0 = R1 . (R2 x R3) / 6 and Dm^-1, in the operation order of
-// the host construction (no contraction), so a reset rebuilds exactly what construction built.
-// Padding tetra (past tile_ntet in their tile) get the identity shape and V0 = 1/6.
+// and per-cell scales f (RadiusSIM / |p[1].x|, TetraCell.py:15-20): the FP32 reference positions
+// X0 = fl32(p f), and per tetra, from X0 in FP64, the edges R_k = X0_k - X0_0, V0 = R1 . (R2 x R3) / 6
+// and Dm^-1, in the operation order of the host construction (no contraction), so that a reset
+// rebuilds exactly what construction built.  Padding tetra (past tile_ntet in their tile) get the
+// identity shape and V0 = 1/6.
 __global__ __launch_bounds__(kBlock) void k_batch_points(const double* __restrict__ p, const double* __restrict__ f,
                                                          const uint32_t* __restrict__ node_cell, int64_t N,
                                                          float* __restrict__ x) {
@@ -184,12 +188,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_edges(const double* __restrict
         const double s = f[node_cell[n0]];
         double x0[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) x0[c] = p[c * N + n0] * s;
+        for (int c = 0; c < 3; ++c) x0[c] = (double)(float)(p[c * N + n0] * s);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const uint32_t nk = tets[(int64_t)(k + 1) * T + t];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) R[k][c] = p[c * N + nk] * s - x0[c];
+            for (int c = 0; c < 3; ++c) R[k][c] = (double)(float)(p[c * N + nk] * s) - x0[c];
         }
         const double cx = R[1][1] * R[2][2] - R[1][2] * R[2][1];
         const double cy = R[1][2] * R[2][0] - R[1][0] * R[2][2];
@@ -200,21 +204,44 @@ __global__ __launch_bounds__(kBlock) void k_batch_edges(const double* __restrict
     v0[t] = (float)vol;
 }
 
-// ---- force pass, elastic models -------------------------------------------------------------
-template <int MODEL, bool CORNERS>
+// ---- force pass, one thread per tetra (atomic or corner-record scatter) --------------------------
+// DISP: the node array holds displacements from the reference positions (fused steps); otherwise
+// absolute positions (the per-plugin kernels)
+template <bool DISP>
+__device__ __forceinline__ M9 tetra_grad(const TetArgs& a, uint32_t n0, uint32_t n1, uint32_t n2, uint32_t n3,
+                                         const float* D) {
+    const V3 p0 = gather3(a.x, a.N, n0), p1 = gather3(a.x, a.N, n1), p2 = gather3(a.x, a.N, n2),
+             p3 = gather3(a.x, a.N, n3);
+    if constexpr (DISP) return grad_disp(p0, p1, p2, p3, D);
+    else return grad_abs(p0, p1, p2, p3, D);
+}
+
+__device__ __forceinline__ void emit_corners(const TetArgs& a, int64_t t, uint32_t n0, uint32_t n1, uint32_t n2,
+                                             uint32_t n3, const Corners& c, bool corners) {
+    if (corners) {
+        store_corners(a.corners, t, c);
+    } else {
+        scatter_atomic(a.force, a.N, n0, c.f0);
+        scatter_atomic(a.force, a.N, n1, c.f1);
+        scatter_atomic(a.force, a.N, n2, c.f2);
+        scatter_atomic(a.force, a.N, n3, c.f3);
+    }
+}
+
+template <int MODEL, bool CORNERS, bool DISP>
 __global__ __launch_bounds__(kBlock) void k_tetra(TetArgs a, ElasticDev e) {
     const int64_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (t >= a.T) return;
-    const int64_t T = a.T, N = a.N;
+    const int64_t T = a.T;
     const uint32_t n0 = a.tetras[t], n1 = a.tetras[T + t], n2 = a.tetras[2 * T + t], n3 = a.tetras[3 * T + t];
     float D[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) D[k] = a.dminv[k * T + t];
-    const V3 p0 = gather3(a.x, N, n0), p1 = gather3(a.x, N, n1), p2 = gather3(a.x, N, n2), p3 = gather3(a.x, N, n3);
+    const M9 H = tetra_grad<DISP>(a, n0, n1, n2, n3, D);
     Corners c;
     if constexpr (MODEL >= NFX_LINEAR_ELASTIC) {
         float vm, pr;
-        c = legacy_tetra<MODEL>(p0, p1, p2, p3, D, e.L, vm, pr);
+        c = legacy_tetra<MODEL>(H, D, e.L, vm, pr);
         if (a.vm) {
             a.vm[t] = vm;
             a.pr[t] = pr;
@@ -222,90 +249,68 @@ __global__ __launch_bounds__(kBlock) void k_tetra(TetArgs a, ElasticDev e) {
     } else {
         const float v0 = a.v0[t];
         if constexpr (MODEL == NFX_MR)
-            c = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
+            c = mr_tetra(H, D, v0, e.mu1, e.mu2, e.kappa);
         else if constexpr (MODEL == NFX_SVK)
-            c = svk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+            c = svk_tetra(H, D, v0, e.mu, e.kappa_svk);
         else
-            c = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+            c = svkk_tetra(H, D, v0, e.mu, e.kappa_svk);
     }
-    if constexpr (CORNERS) {
-        store_corners(a.corners, t, c);
-    } else {
-        scatter_atomic(a.force, N, n0, c.f0);
-        scatter_atomic(a.force, N, n1, c.f1);
-        scatter_atomic(a.force, N, n2, c.f2);
-        scatter_atomic(a.force, N, n3, c.f3);
-    }
+    emit_corners(a, t, n0, n1, n2, n3, c, CORNERS);
 }
 
 // ---- force pass, viscoplastic -----------------------------------------------------------------
-template <bool CORNERS, bool STRESS>
-__global__ __launch_bounds__(kBlock) void k_vp(TetArgs a, VPConst P, float* __restrict__ Fp, float* __restrict__ eps,
+// FUSED: displacements and the plastic deviation Q = Fp^-1 - I (state); otherwise absolute
+// positions and the reference's Fp (the per-plugin kernel)
+template <bool CORNERS, bool STRESS, bool FUSED>
+__global__ __launch_bounds__(kBlock) void k_vp(TetArgs a, VPConst P, float* __restrict__ state, float* __restrict__ eps,
                                                float* __restrict__ vm, float* __restrict__ pr) {
     const int64_t t = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     if (t >= a.T) return;
-    const int64_t T = a.T, N = a.N;
+    const int64_t T = a.T;
     const uint32_t n0 = a.tetras[t], n1 = a.tetras[T + t], n2 = a.tetras[2 * T + t], n3 = a.tetras[3 * T + t];
-    float D[9], fp[9];
+    float D[9], st[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) D[k] = a.dminv[k * T + t];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) fp[k] = Fp[k * T + t];
-    const float e0 = eps[t];
-    const V3 p0 = gather3(a.x, N, n0), p1 = gather3(a.x, N, n1), p2 = gather3(a.x, N, n2), p3 = gather3(a.x, N, n3);
-    const VPOut o = vp_tetra(p0, p1, p2, p3, D, fp, e0, P);
+    for (int k = 0; k < 9; ++k) st[k] = state[k * T + t];
+    const float e0 = eps ? eps[t] : 0.0f;
+    const M9 H = tetra_grad<FUSED>(a, n0, n1, n2, n3, D);
+    const VPOut o = FUSED ? vp_tetra_q(H, D, st, e0, P) : vp_tetra(H, D, st, e0, P);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Fp[k * T + t] = o.Fp[k];
-    eps[t] = o.epsbar;
+    for (int k = 0; k < 9; ++k) state[k * T + t] = o.Fp[k];
+    if (eps) eps[t] = o.epsbar;
     if constexpr (STRESS) {
         vm[t] = o.von_mises;
         pr[t] = o.pressure;
     }
-    if constexpr (CORNERS) {
-        store_corners(a.corners, t, o.f);
-    } else {
-        scatter_atomic(a.force, N, n0, o.f.f0);
-        scatter_atomic(a.force, N, n1, o.f.f1);
-        scatter_atomic(a.force, N, n2, o.f.f2);
-        scatter_atomic(a.force, N, n3, o.f.f3);
-    }
+    emit_corners(a, t, n0, n1, n2, n3, o.f, CORNERS);
 }
 
-// ---- viscoplastic state conversion: shape state S = Dm^-1 Fp^-1 <-> Fp = S^-1 Dm^-1 -------------
-template <bool TO_SHAPE>
-__global__ __launch_bounds__(kBlock) void k_vp_convert(const float* __restrict__ dminv, const float* __restrict__ in,
-                                                       int64_t T, float* __restrict__ out) {
+// ---- state conversions ---------------------------------------------------------------------------
+// Q = Fp^-1 - I  <->  Fp = (I + Q)^-1, per tetra (9T SoA each)
+template <bool TO_Q>
+__global__ __launch_bounds__(kBlock) void k_vp_convert(const float* __restrict__ in, int64_t T, float* __restrict__ out) {
     const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (t >= T) return;
-    M9 D, X;
+    M9 X;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            D.m[i][j] = dminv[(3 * i + j) * T + t];
-            X.m[i][j] = in[(3 * i + j) * T + t];
-        }
-    const M9 Y = TO_SHAPE ? mul9(D, inv9(X)) : mul9(inv9(X), D);
+        for (int j = 0; j < 3; ++j) X.m[i][j] = in[(3 * i + j) * T + t] + ((!TO_Q && i == j) ? 1.0f : 0.0f);
+    const M9 Y = inv9(X);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) out[(3 * i + j) * T + t] = Y.m[i][j];
+        for (int j = 0; j < 3; ++j) out[(3 * i + j) * T + t] = Y.m[i][j] - ((TO_Q && i == j) ? 1.0f : 0.0f);
 }
 
-// G = S S^T per tetra (the symmetric viscoplastic state; planes xx, yy, zz, xy, yz, xz)
-__global__ __launch_bounds__(kBlock) void k_vp_sym(const float* __restrict__ S, int64_t T, float* __restrict__ G) {
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (t >= T) return;
-    float m[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) m[k] = S[k * T + t];
-    const V3 a = mk(m[0], m[1], m[2]), b = mk(m[3], m[4], m[5]), c = mk(m[6], m[7], m[8]);  // rows of S
-    G[t] = dot(a, a);
-    G[T + t] = dot(b, b);
-    G[2 * T + t] = dot(c, c);
-    G[3 * T + t] = dot(a, b);
-    G[4 * T + t] = dot(b, c);
-    G[5 * T + t] = dot(a, c);
+// x = X0 + u (TO_POINTS) or u = x - X0, component-wise over 3N
+template <bool TO_POINTS>
+__global__ __launch_bounds__(kBlock) void k_disp_convert(const float* __restrict__ ref, const float* __restrict__ in,
+                                                         int64_t n3, float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n3) return;
+    out[i] = TO_POINTS ? ref[i] + in[i] : in[i] - ref[i];
 }
 
 // ---- force pass, tiled deterministic scatter ---------------------------------------------------
@@ -315,17 +320,18 @@ __global__ __launch_bounds__(kBlock) void k_vp_sym(const float* __restrict__ S, 
 // and the position of its partial sum in node order (tile_dst).  Each corner slot 4*(t-t0)+a
 // carries its tile-local node and its rank in that node's run (tile_corners), so its place in the
 // sorted list is run start + rank.  Phases:
-//   0  issue the tetra's streaming loads (Dm^-1 and V0, or the viscoplastic shape state and
-//      epsbar), its corner records and the tile metadata
-//   1  node positions and run starts -> LDS
-//   2  material law on LDS positions; each corner force -> LDS at its place in the sorted list
+//   0  issue the tetra's streaming loads (Dm^-1, and V0 or the viscoplastic state and epsbar), its
+//      corner records and the tile metadata
+//   1  node displacements and run starts -> LDS
+//   2  material law on LDS displacements; each corner force -> LDS at its place in the sorted list
 //   3  one thread per tile-node: sum its contiguous run -> partials[tile_dst]
 // The node pass sums each node's contiguous partials in tile order: no atomics, fixed order.
 struct TileArgs {
     int64_t N, T;
     const float* __restrict__ dminv;
     const float* __restrict__ v0;
-    const float* __restrict__ x;
+    const float* __restrict__ x;    // node displacements u
+    const float* __restrict__ ref;  // reference positions X0
     const uint32_t* __restrict__ tile_count;
     const uint32_t* __restrict__ tile_nodes;
     const uint16_t* __restrict__ tile_lend;
@@ -358,19 +364,25 @@ struct TileArgs {
 // step of each of the ncurve sample intervals, the indenter force on the cell's nodes at the
 // current positions, summed in FP64 in a fixed order (strided per-thread sums, then lanes and
 // waves in order).  Every exit before the barrier is uniform over the workgroup.
-__device__ void batch_curve_sample(const float* __restrict__ x, int64_t N, const uint32_t* __restrict__ cell_node_off,
-                                   const nfx_cell* __restrict__ cells, double* curves, int32_t ncurve,
-                                   int64_t curve_steps, int64_t time, int cell, int ind) {
+__device__ void batch_curve_sample(const float* __restrict__ x, const float* __restrict__ ref, int64_t N,
+                                   const uint32_t* __restrict__ cell_node_off, const nfx_cell* __restrict__ cells,
+                                   double* curves, int32_t ncurve, int64_t curve_steps, int64_t time, int cell,
+                                   int ind) {
     if (time < 0 || time >= curve_steps) return;
     const int64_t k = time * ncurve / curve_steps;
     if (time > 0 && (time - 1) * ncurve / curve_steps == k) return;  // not the first step of sample k
     const nfx_cell cp = cells[cell];
-    const float pos = (float)indenter_pos(cp.tip_start, cp.tip_speed, cp.tip_move_steps, time);
+    const double pos = indenter_pos(cp.tip_start, cp.tip_speed, cp.tip_move_steps, time);
     double s = 0.0;
     for (int64_t n = cell_node_off[cell] + threadIdx.x; n < cell_node_off[cell + 1]; n += blockDim.x) {
-        const V3 p = mk(x[n], x[N + n], x[2 * N + n]);
-        s -= ind == NFX_INDENTER_SPHERE ? (double)sphere_term(p, cp.tip_k, cp.tip_radius, pos).y
-                                        : (double)plane_term(p.y, cp.tip_k, pos);
+        const double y = (double)ref[N + n] + (double)x[N + n];
+        if (ind == NFX_INDENTER_SPHERE) {
+            if (y > pos - (double)cp.tip_radius)
+                s -= sphere_term_d((double)ref[n] + (double)x[n], y, (double)ref[2 * N + n] + (double)x[2 * N + n],
+                                   cp.tip_k, cp.tip_radius, pos).fy;
+        } else {
+            s += (double)cp.tip_k * fmax(y - pos, 0.0);
+        }
     }
     __shared__ double ws[8];
 #pragma unroll
@@ -393,10 +405,10 @@ __host__ __device__ constexpr size_t tile_lds_bytes_fixed() {
 __host__ __device__ constexpr size_t tile_lds_bytes_per_node() { return 12 + 2; }
 
 // NT: cache policy of the streams; EPS: the viscoplastic law keeps its accumulated plastic strain
-// (the linear law does not read it); SYM: its state is the symmetric G = S S^T (6 planes) instead
-// of the shape state S (9 planes); NARROW: 16-bit tile-node metadata (batches)
+// (the linear law does not read it); SYM: its state is the symmetric deviation Gd (6 planes) instead
+// of Q (9 planes); NARROW: 16-bit tile-node metadata (batches)
 template <int MODEL, int K, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false, bool NARROW = false>
-__global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
+__global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS)) ? 6 : 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
     // one thread per tetra of the tile (blockDim.x == K)
@@ -409,7 +421,7 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     const uint32_t ntile_blocks = gridDim.x - (uint32_t)a.curve_blocks;
     if (blockIdx.x >= ntile_blocks) {  // the step's curve sample of cell blockIdx.x - ntile_blocks
         const int64_t time = a.curve_step_dev ? (*a.curve_step_dev + a.curve_off) : a.curve_step;
-        batch_curve_sample(a.x, a.N, a.cell_node_off, a.cells, a.curves, a.ncurve, a.curve_steps, time,
+        batch_curve_sample(a.x, a.ref, a.N, a.cell_node_off, a.cells, a.curves, a.ncurve, a.curve_steps, time,
                            (int)(blockIdx.x - ntile_blocks), a.curve_ind);
         return;
     }
@@ -435,18 +447,17 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     const bool active = tl < nt;
     const uint32_t u = (uint32_t)tl;
 
-    // issue this tetra's streaming loads first: they overlap the staging phases below.  The
-    // viscoplastic law reads its shape state S = Dm^-1 Fp^-1 in place of Dm^-1 and Fp.
-    float D[9], e0 = 0.f, v0 = 0.f;
+    // issue this tetra's streaming loads first: they overlap the staging phases below
+    float D[9], Sv[MODEL == NFX_VISCOPLASTIC ? 9 : 1], e0 = 0.f, v0 = 0.f;
     if (active) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) D[k] = ld_g<NT, float>(uat(a.dminv + t0 + k * T, u));
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
 #pragma unroll
-            for (int k = 0; k < (SYM ? 6 : 9); ++k) D[k] = ld_g<NT, float>(uat((const float*)(Sst + t0 + k * T), u));
+            for (int k = 0; k < (SYM ? 6 : 9); ++k) Sv[k] = ld_g<NT, float>(uat((const float*)(Sst + t0 + k * T), u));
             if constexpr (EPS) e0 = ld_g<NT, float>(uat((const float*)(eps + t0), u));
-        } else {
-#pragma unroll
-            for (int k = 0; k < 9; ++k) D[k] = ld_g<NT, float>(uat(a.dminv + t0 + k * T, u));
-            if constexpr (MODEL < NFX_VISCOPLASTIC) v0 = ld_g<NT, float>(uat(a.v0 + t0, u));
+        } else if constexpr (MODEL < NFX_VISCOPLASTIC) {
+            v0 = ld_g<NT, float>(uat(a.v0 + t0, u));
         }
     }
 
@@ -480,15 +491,16 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
     if (active) {
         const int l0 = (int)(cr & 0xFF), l1 = (int)((cr >> 16) & 0xFF), l2 = (int)((cr >> 32) & 0xFF),
                   l3 = (int)((cr >> 48) & 0xFF);
-        const V3 p0 = mk(s_x[l0], s_x[MX + l0], s_x[2 * MX + l0]);
-        const V3 p1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
-        const V3 p2 = mk(s_x[l2], s_x[MX + l2], s_x[2 * MX + l2]);
-        const V3 p3 = mk(s_x[l3], s_x[MX + l3], s_x[2 * MX + l3]);
+        const V3 u0 = mk(s_x[l0], s_x[MX + l0], s_x[2 * MX + l0]);
+        const V3 u1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
+        const V3 u2 = mk(s_x[l2], s_x[MX + l2], s_x[2 * MX + l2]);
+        const V3 u3 = mk(s_x[l3], s_x[MX + l3], s_x[2 * MX + l3]);
+        const M9 H = grad_disp(u0, u1, u2, u3, D);
         Corners f;
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
             VPOut o;
-            if constexpr (SYM) o = vp_tetra_sym(p0, p1, p2, p3, D, e0, P);
-            else o = vp_tetra_shape(p0, p1, p2, p3, D, e0, P);
+            if constexpr (SYM) o = vp_tetra_gd(H, D, Sv, e0, P);
+            else o = vp_tetra_q(H, D, Sv, e0, P);
 #pragma unroll
             for (int k = 0; k < (SYM ? 6 : 9); ++k) st_g<NT, float>(uat(Sst + t0 + k * T, u), o.Fp[k]);
             if constexpr (EPS) st_g<NT, float>(uat(eps + t0, u), o.epsbar);
@@ -499,17 +511,17 @@ __global__ __launch_bounds__(K, 8) void k_tile_force(TileArgs a, ElasticDev e, V
             f = o.f;
         } else if constexpr (MODEL >= NFX_LINEAR_ELASTIC) {
             float svm, spr;
-            f = legacy_tetra<MODEL>(p0, p1, p2, p3, D, e.L, svm, spr);
+            f = legacy_tetra<MODEL>(H, D, e.L, svm, spr);
             if constexpr (STRESS) {
                 st_s<NT>(vm + t0 + u, svm);
                 st_s<NT>(pr + t0 + u, spr);
             }
         } else if constexpr (MODEL == NFX_MR) {
-            f = mr_tetra(p0, p1, p2, p3, D, v0, e.mu1, e.mu2, e.kappa);
+            f = mr_tetra(H, D, v0, e.mu1, e.mu2, e.kappa);
         } else if constexpr (MODEL == NFX_SVK) {
-            f = svk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+            f = svk_tetra(H, D, v0, e.mu, e.kappa_svk);
         } else {
-            f = svkk_tetra(p0, p1, p2, p3, D, v0, e.mu, e.kappa_svk);
+            f = svkk_tetra(H, D, v0, e.mu, e.kappa_svk);
         }
         // each corner's place in the tile's (node, slot)-sorted corner list: run start + rank
         const int q0 = s_lb[l0] + (int)((cr >> 8) & 0xFF), q1 = s_lb[l1] + (int)((cr >> 24) & 0xFF);
@@ -614,7 +626,8 @@ __device__ __forceinline__ V3 sum_partials(const float* __restrict__ partials, u
 
 struct NodeArgs {
     int64_t N;
-    float* x;
+    float* x;  // displacements u (fused node pass) or absolute positions (per-plugin kernels)
+    const float* __restrict__ ref;
     float* v;
     float* force;
     float* fold;
@@ -623,9 +636,9 @@ struct NodeArgs {
     const float* __restrict__ corners;
     const uint32_t* __restrict__ node_off;
     const float* __restrict__ partials;
-    float sub_k, wall;
+    double sub_k, wall;
     int32_t indenter;
-    float ind_k, ind_radius;
+    double ind_k, ind_radius;
     double ind_start, ind_speed, ind_move;
 };
 
@@ -648,6 +661,32 @@ __device__ __forceinline__ V3 gather_node(const NodeArgs& a, int64_t n, int sub)
     return f;
 }
 
+// The contact terms of a node at x = X0 + u, in FP64: the substrate (Substrate.cl:4-5), the
+// indenter (Sphere.cl:9-16 or Plane.cl:4-5) at the FP64 position pos, added to f; the indenter's
+// own force and the penetration for the tip reduction.  X0.x/z and u.x/z are read only for nodes
+// the sphere can touch (y > pos - R): a few per cent of a cell.
+template <bool SUB, int IND>
+__device__ __forceinline__ void node_contacts(const float* __restrict__ ref, int64_t N, int64_t n, V3 u, double sub_k,
+                                              double wall, double ind_k, double ind_radius, double pos, V3& f,
+                                              double& tip_f, double& tip_dis) {
+    const double y = (double)ref[N + n] + (double)u.y;
+    if constexpr (SUB) f.y += (float)(sub_k * fmax(wall - y, 0.0));
+    if constexpr (IND == NFX_INDENTER_SPHERE) {
+        if (y > pos - ind_radius) {
+            const Contact c = sphere_term_d((double)ref[n] + (double)u.x, y, (double)ref[2 * N + n] + (double)u.z,
+                                            ind_k, ind_radius, pos);
+            f = f + mk((float)c.fx, (float)c.fy, (float)c.fz);
+            tip_f = -c.fy;
+            tip_dis = c.dis;
+        }
+    } else if constexpr (IND == NFX_INDENTER_PLANE) {
+        const double dis = fmax(y - pos, 0.0);
+        f.y -= (float)(ind_k * dis);
+        tip_f = ind_k * dis;
+        tip_dis = dis;
+    }
+}
+
 // GATHER: 0 = read the atomically accumulated force (and clear it), 1 = corner records via the
 // node CSR (4 lanes per node), 2 = tile partials via node_parts (one lane per node)
 template <int GATHER, bool SUB, int IND, bool FIX, bool TIP>
@@ -657,7 +696,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
     constexpr bool CSR = GATHER == 1;
     constexpr int G = CSR ? 4 : 1;
     const int64_t time = step_dev ? (*step_dev + step_off) : step;
-    const float pos = IND ? (float)indenter_pos(a.ind_start, a.ind_speed, a.ind_move, time) : 0.0f;
+    const double pos = IND ? indenter_pos(a.ind_start, a.ind_speed, a.ind_move, time) : 0.0;
     const int64_t N = a.N;
     const int sub = threadIdx.x % G;
     double tsum = 0.0, tmax = 0.0;
@@ -673,30 +712,19 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
         }
         if (sub != 0) continue;
         V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
-        if constexpr (SUB) f.y += substrate_term(x.y, a.sub_k, a.wall);
-        if constexpr (IND == NFX_INDENTER_SPHERE) {
-            const V3 fs = sphere_term(x, a.ind_k, a.ind_radius, pos);
-            f = f + fs;
-            if constexpr (TIP) {
-                tsum -= (double)fs.y;
-                const float r = sqrtf(x.x * x.x + (x.y - pos) * (x.y - pos) + x.z * x.z);
-                tmax = fmax(tmax, (double)fmaxf(a.ind_radius - r, 0.0f));
-            }
-        } else if constexpr (IND == NFX_INDENTER_PLANE) {
-            const float fp = plane_term(x.y, a.ind_k, pos);
-            f.y += fp;
-            if constexpr (TIP) {
-                tsum -= (double)fp;
-                tmax = fmax(tmax, (double)fmaxf(x.y - pos, 0.0f));
-            }
+        double tf = 0.0, td = 0.0;
+        node_contacts<SUB, IND>(a.ref, N, n, x, a.sub_k, a.wall, a.ind_k, a.ind_radius, pos, f, tf, td);
+        if constexpr (TIP) {
+            tsum += tf;
+            tmax = fmax(tmax, td);
         }
         V3 v = mk(a.v[n], a.v[N + n], a.v[2 * N + n]);
         const V3 fo = mk(a.fold[n], a.fold[N + n], a.fold[2 * N + n]);
         vv_update(x, v, f, fo);
         if constexpr (FIX) {
-            if (n == 3 || n == 4) {
-                x.x = 0.0f;
-                x.z = 0.0f;
+            if (n == 3 || n == 4) {  // x = z = 0 (VelocityVerlet.cl:17-22): u = -X0
+                x.x = -a.ref[n];
+                x.z = -a.ref[2 * N + n];
             }
         }
         a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
@@ -764,20 +792,29 @@ __global__ __launch_bounds__(kBlock) void k_vv(float* x, float* v, float* f, flo
     f[n] = 0.0f; f[N + n] = 0.0f; f[2 * N + n] = 0.0f;
 }
 
-__global__ __launch_bounds__(kBlock) void k_tip(const float* __restrict__ x, int64_t N, int32_t ind, float k, float R,
-                                                float pos, double* partials, unsigned int* counter, double* out_sum,
-                                                double* out_max) {
+// The indenter force and maximum penetration at the current positions, in FP64 (Simulation.py:
+// 198-212 evaluates them on the host in FP64).  REF: x holds displacements from ref; otherwise
+// absolute positions.
+template <bool REF>
+__global__ __launch_bounds__(kBlock) void k_tip(const float* __restrict__ x, const float* __restrict__ ref, int64_t N,
+                                                int32_t ind, double k, double R, double pos, double* partials,
+                                                unsigned int* counter, double* out_sum, double* out_max) {
     double s = 0.0, m = 0.0;
     for (int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x; n < N; n += (int64_t)gridDim.x * kBlock) {
-        const V3 p = mk(x[n], x[N + n], x[2 * N + n]);
+        double px = (double)x[n], py = (double)x[N + n], pz = (double)x[2 * N + n];
+        if constexpr (REF) {
+            px += (double)ref[n];
+            py += (double)ref[N + n];
+            pz += (double)ref[2 * N + n];
+        }
         if (ind == NFX_INDENTER_SPHERE) {
-            const V3 fs = sphere_term(p, k, R, pos);
-            s -= (double)fs.y;
-            const float r = sqrtf(p.x * p.x + (p.y - pos) * (p.y - pos) + p.z * p.z);
-            m = fmax(m, (double)fmaxf(R - r, 0.0f));
+            const Contact c = sphere_term_d(px, py, pz, k, R, pos);
+            s -= c.fy;
+            m = fmax(m, c.dis);
         } else {
-            s += (double)(k * fmaxf(p.y - pos, 0.0f));
-            m = fmax(m, (double)fmaxf(p.y - pos, 0.0f));
+            const double dis = fmax(py - pos, 0.0);
+            s += k * dis;
+            m = fmax(m, dis);
         }
     }
     reduce_finish(s, m, partials, counter, out_sum, out_max);
@@ -805,13 +842,14 @@ static int check_sizes(int64_t N, int64_t T) {
 }
 
 static TetArgs tet_args(const nfx_plan& p) {
-    return TetArgs{p.N, p.T, p.tetras, p.dminv, p.v0, p.points, p.force, p.corners, nullptr, nullptr};
+    return TetArgs{p.N, p.T, p.tetras, p.dminv, p.v0, p.disp, p.force, p.corners, nullptr, nullptr};
 }
 
 static NodeArgs node_args(const nfx_plan& p) {
     NodeArgs a;
     a.N = p.N;
-    a.x = p.points;
+    a.x = p.disp;
+    a.ref = p.ref;
     a.v = p.vel;
     a.force = p.force;
     a.fold = p.fold;
@@ -820,11 +858,11 @@ static NodeArgs node_args(const nfx_plan& p) {
     a.corners = p.corners;
     a.node_off = p.node_off;
     a.partials = p.partials;
-    a.sub_k = (float)p.substrate_k;
-    a.wall = (float)p.wall;
+    a.sub_k = p.substrate_k;
+    a.wall = p.wall;
     a.indenter = p.indenter;
-    a.ind_k = (float)p.ind_k;
-    a.ind_radius = (float)p.ind_radius;
+    a.ind_k = p.ind_k;
+    a.ind_radius = p.ind_radius;
     a.ind_start = p.ind_start;
     a.ind_speed = p.ind_speed;
     a.ind_move = p.ind_move_steps;
@@ -835,16 +873,13 @@ static int validate_plan(const nfx_plan* p) {
     NFX_REQUIRE(p != nullptr, "plan is NULL");
     int rc = check_sizes(p->N, p->T);
     if (rc) return rc;
-    NFX_REQUIRE(p->tetras && p->dminv && p->points && p->vel && p->fold, "plan: NULL mesh/state pointer");
+    NFX_REQUIRE(p->tetras && p->dminv && p->ref && p->disp && p->vel && p->fold, "plan: NULL mesh/state pointer");
     NFX_REQUIRE(p->model >= NFX_SVKK && p->model <= NFX_PORO_NEO_HOOKEAN, "plan: unknown model %d", p->model);
     NFX_REQUIRE(p->model > NFX_MR || p->v0, "plan: NULL v0");
     NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->epsbar ||
                     (p->scatter == NFX_SCATTER_TILED && p->vp.flow_exponent == 1.0 && p->vp.hardening_exponent == 0.0),
                 "plan: viscoplastic state epsbar missing (it may be NULL only for the linear law on the tiled path)");
-    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->scatter == NFX_SCATTER_TILED || p->Fp,
-                "plan: viscoplastic state Fp missing");
-    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->scatter != NFX_SCATTER_TILED || p->vp_shape,
-                "plan: the tiled viscoplastic force needs the shape state vp_shape");
+    NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->vp_state, "plan: viscoplastic state vp_state missing");
     NFX_REQUIRE(!p->vp_sym || (p->model == NFX_VISCOPLASTIC && p->scatter == NFX_SCATTER_TILED),
                 "plan: the symmetric state (vp_sym) is for the tiled viscoplastic force only");
     NFX_REQUIRE(p->scatter >= NFX_SCATTER_ATOMIC && p->scatter <= NFX_SCATTER_TILED, "plan: unknown scatter %d",
@@ -884,35 +919,34 @@ static void launch_tile_vp(unsigned int nb, size_t lds, hipStream_t s, const Til
     if constexpr (!STRESS) {  // the batch's form: symmetric state with 16-bit tile-node metadata
         if (p.vp_sym && a.tile_nodes16) {
             if (p.epsbar)
-                k_tile_force<NFX_VISCOPLASTIC, K, false, NT, true, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape,
+                k_tile_force<NFX_VISCOPLASTIC, K, false, NT, true, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
                                                                                                 p.epsbar, vm, pr);
             else
-                k_tile_force<NFX_VISCOPLASTIC, K, false, NT, false, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape,
+                k_tile_force<NFX_VISCOPLASTIC, K, false, NT, false, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
                                                                                                  nullptr, vm, pr);
             return;
         }
     }
     if (p.vp_sym) {
         if (p.epsbar)
-            k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar,
+            k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar,
                                                                                         vm, pr);
         else
-            k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, nullptr,
+            k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr,
                                                                                          vm, pr);
     } else if (p.epsbar) {
-        k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, p.epsbar, vm, pr);
+        k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar, vm, pr);
     } else {
-        k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false><<<nb, K, lds, s>>>(a, e, c, p.vp_shape, nullptr, vm, pr);
+        k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr, vm, pr);
     }
 }
 
 template <int K>
 static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     // ctx->batch is set only while a batched step is being enqueued
-    TileArgs a{p.N,         p.T,          p.dminv,     p.v0,     p.points,         p.tile_count,
-               p.tile_nodes, p.tile_lend,  p.tile_corners, p.tile_dst, p.partials,   p.tile_max_nodes,
-               nullptr,     nullptr,      nullptr,    nullptr,   nullptr,          nullptr,
-               nullptr};
+    TileArgs a{p.N,          p.T,         p.dminv,        p.v0,       p.disp,     p.ref,
+               p.tile_count, p.tile_nodes, p.tile_lend,    p.tile_corners, p.tile_dst, p.partials,
+               p.tile_max_nodes};
     if (ctx->batch) {
         a.tile_ntet = ctx->batch->tile_ntet;
         a.tile_cell = ctx->batch->tile_cell;
@@ -969,48 +1003,59 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     }
 }
 
+// One thread per tetra with the atomic (csr false) or corner-record scatter.  DISP: a.x holds
+// displacements and a viscoplastic state is Q (the fused steps); otherwise absolute positions and Fp
+// (the per-plugin kernels).  a.vm/a.pr select the legacy laws' stress outputs.
+template <bool DISP>
+static void launch_tetra_pass(hipStream_t s, const TetArgs& a, int model, bool csr, const ElasticDev& e,
+                              const VPConst& c, float* state, float* eps, float* vm, float* pr) {
+    const unsigned int nb = nblocks(a.T);
+    if (model == NFX_VISCOPLASTIC) {
+        const bool st = vm && pr;
+        if (csr && st) k_vp<true, true, DISP><<<nb, kBlock, 0, s>>>(a, c, state, eps, vm, pr);
+        else if (csr) k_vp<true, false, DISP><<<nb, kBlock, 0, s>>>(a, c, state, eps, nullptr, nullptr);
+        else if (st) k_vp<false, true, DISP><<<nb, kBlock, 0, s>>>(a, c, state, eps, vm, pr);
+        else k_vp<false, false, DISP><<<nb, kBlock, 0, s>>>(a, c, state, eps, nullptr, nullptr);
+        return;
+    }
+    switch (model * 2 + (csr ? 1 : 0)) {
+        case NFX_MR * 2: k_tetra<NFX_MR, false, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_MR * 2 + 1: k_tetra<NFX_MR, true, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_SVK * 2: k_tetra<NFX_SVK, false, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_SVK * 2 + 1: k_tetra<NFX_SVK, true, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_SVKK * 2: k_tetra<NFX_SVKK, false, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_SVKK * 2 + 1: k_tetra<NFX_SVKK, true, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_LINEAR_ELASTIC * 2: k_tetra<NFX_LINEAR_ELASTIC, false, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_LINEAR_ELASTIC * 2 + 1: k_tetra<NFX_LINEAR_ELASTIC, true, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_LINEAR_DEVIATORIC * 2: k_tetra<NFX_LINEAR_DEVIATORIC, false, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_LINEAR_DEVIATORIC * 2 + 1:
+            k_tetra<NFX_LINEAR_DEVIATORIC, true, DISP><<<nb, kBlock, 0, s>>>(a, e);
+            break;
+        case NFX_NEO_HOOKEAN_2 * 2: k_tetra<NFX_NEO_HOOKEAN_2, false, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_NEO_HOOKEAN_2 * 2 + 1: k_tetra<NFX_NEO_HOOKEAN_2, true, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        case NFX_PORO_NEO_HOOKEAN * 2: k_tetra<NFX_PORO_NEO_HOOKEAN, false, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+        default: k_tetra<NFX_PORO_NEO_HOOKEAN, true, DISP><<<nb, kBlock, 0, s>>>(a, e); break;
+    }
+}
+
+// the fused step's force pass (displacement state)
 static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
+    const bool st = stress && p.von_mises && p.pressure;
     if (p.scatter == NFX_SCATTER_TILED) {
-        const bool st = stress && p.von_mises && p.pressure;
         if (p.tile_k == 256) launch_tiled_k<256>(ctx, p, st);
         else launch_tiled_k<512>(ctx, p, st);
         NFX_CHECK_LAUNCH();
         return NFX_OK;
     }
     TetArgs a = tet_args(p);
-    const unsigned int nb = nblocks(p.T);
-    const bool csr = p.scatter == NFX_SCATTER_CSR;
-    hipStream_t s = ctx->stream;
-    if (p.model == NFX_VISCOPLASTIC) {
-        const VPConst c = vp_consts(p.vp);
-        const bool st = stress && p.von_mises && p.pressure;
-        if (csr && st) k_vp<true, true><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
-        else if (csr) k_vp<true, false><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, nullptr, nullptr);
-        else if (st) k_vp<false, true><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, p.von_mises, p.pressure);
-        else k_vp<false, false><<<nb, kBlock, 0, s>>>(a, c, p.Fp, p.epsbar, nullptr, nullptr);
-    } else {
-        const ElasticDev e = elastic_consts(p);
-        if (is_legacy(p.model) && stress && p.von_mises && p.pressure) {
-            a.vm = p.von_mises;
-            a.pr = p.pressure;
-        }
-        switch (p.model * 2 + (csr ? 1 : 0)) {
-            case NFX_MR * 2: k_tetra<NFX_MR, false><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_MR * 2 + 1: k_tetra<NFX_MR, true><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_SVK * 2: k_tetra<NFX_SVK, false><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_SVK * 2 + 1: k_tetra<NFX_SVK, true><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_SVKK * 2: k_tetra<NFX_SVKK, false><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_SVKK * 2 + 1: k_tetra<NFX_SVKK, true><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_LINEAR_ELASTIC * 2: k_tetra<NFX_LINEAR_ELASTIC, false><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_LINEAR_ELASTIC * 2 + 1: k_tetra<NFX_LINEAR_ELASTIC, true><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_LINEAR_DEVIATORIC * 2: k_tetra<NFX_LINEAR_DEVIATORIC, false><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_LINEAR_DEVIATORIC * 2 + 1: k_tetra<NFX_LINEAR_DEVIATORIC, true><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_NEO_HOOKEAN_2 * 2: k_tetra<NFX_NEO_HOOKEAN_2, false><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_NEO_HOOKEAN_2 * 2 + 1: k_tetra<NFX_NEO_HOOKEAN_2, true><<<nb, kBlock, 0, s>>>(a, e); break;
-            case NFX_PORO_NEO_HOOKEAN * 2: k_tetra<NFX_PORO_NEO_HOOKEAN, false><<<nb, kBlock, 0, s>>>(a, e); break;
-            default: k_tetra<NFX_PORO_NEO_HOOKEAN, true><<<nb, kBlock, 0, s>>>(a, e); break;
-        }
+    const bool legacy_st = is_legacy(p.model) && st;
+    if (legacy_st) {
+        a.vm = p.von_mises;
+        a.pr = p.pressure;
     }
+    launch_tetra_pass<true>(ctx->stream, a, p.model, p.scatter == NFX_SCATTER_CSR, elastic_consts(p),
+                            vp_consts(p.vp), p.vp_state, p.epsbar, st ? p.von_mises : nullptr,
+                            st ? p.pressure : nullptr);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }
@@ -1064,8 +1109,9 @@ static int launch_node_pass(nfx_ctx* ctx, const nfx_plan& p, int64_t step, const
 // ---- batched node pass and curve sampling ----------------------------------------------------
 struct BatchNodeArgs {
     int64_t N;
-    float* x;
-    float* u;  // velocity-Verlet state u = a v + f/2
+    float* x;  // displacements u
+    const float* __restrict__ ref;
+    float* w;  // velocity-Verlet state w = a v + f/2
     const uint32_t* __restrict__ node_off;
     const float* __restrict__ partials;
     const uint32_t* __restrict__ node_cell;
@@ -1073,7 +1119,8 @@ struct BatchNodeArgs {
 };
 
 // Node pass of a batch: each node applies its own cell's substrate and indenter (position from
-// the step index with the cell's law), then the damped velocity-Verlet update in the (x, u) form.
+// the step index with the cell's law, contacts in FP64 at X0 + u), then the damped velocity-Verlet
+// update in the (u, w) form.
 template <bool SUB, int IND>
 __global__ __launch_bounds__(kBlock) void k_batch_node(BatchNodeArgs a, int64_t step, const int64_t* step_dev,
                                                        int32_t step_off) {
@@ -1084,30 +1131,25 @@ __global__ __launch_bounds__(kBlock) void k_batch_node(BatchNodeArgs a, int64_t 
     V3 f = sum_partials(a.partials, a.node_off[n], a.node_off[n + 1]);
     const nfx_cell* cp = a.cells + a.node_cell[n];
     V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
-    if constexpr (SUB) f.y += substrate_term(x.y, cp->substrate_k, cp->wall);
-    if constexpr (IND == NFX_INDENTER_SPHERE) {
-        const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
-        f = f + sphere_term(x, cp->tip_k, cp->tip_radius, pos);
-    } else if constexpr (IND == NFX_INDENTER_PLANE) {
-        const float pos = (float)indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time);
-        f.y += plane_term(x.y, cp->tip_k, pos);
-    }
-    V3 u = mk(a.u[n], a.u[N + n], a.u[2 * N + n]);
-    vv_update_u(x, u, f);
+    const double pos = IND ? indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time) : 0.0;
+    double tf = 0.0, td = 0.0;
+    node_contacts<SUB, IND>(a.ref, N, n, x, cp->substrate_k, cp->wall, cp->tip_k, cp->tip_radius, pos, f, tf, td);
+    V3 w = mk(a.w[n], a.w[N + n], a.w[2 * N + n]);
+    vv_update_w(x, w, f);
     a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
-    a.u[n] = u.x; a.u[N + n] = u.y; a.u[2 * N + n] = u.z;
+    a.w[n] = w.x; a.w[N + n] = w.y; a.w[2 * N + n] = w.z;
 }
 
 // One workgroup per cell: on a sampling step, the cell's indenter force (Simulation.py:198-212 in
 // simulation units) from the positions at the start of the step, summed in FP64 in a fixed order.
 template <int IND>
-__global__ __launch_bounds__(kBlock) void k_batch_curves(const float* __restrict__ x, int64_t N,
-                                                         const uint32_t* __restrict__ cell_node_off,
+__global__ __launch_bounds__(kBlock) void k_batch_curves(const float* __restrict__ x, const float* __restrict__ ref,
+                                                         int64_t N, const uint32_t* __restrict__ cell_node_off,
                                                          const nfx_cell* __restrict__ cells, double* curves,
                                                          int32_t ncurve, int64_t curve_steps, int64_t step,
                                                          const int64_t* step_dev, int32_t step_off) {
     const int64_t time = step_dev ? (*step_dev + step_off) : step;
-    batch_curve_sample(x, N, cell_node_off, cells, curves, ncurve, curve_steps, time, blockIdx.x, IND);
+    batch_curve_sample(x, ref, N, cell_node_off, cells, curves, ncurve, curve_steps, time, blockIdx.x, IND);
 }
 
 static int validate_batch(const nfx_plan* p, const nfx_batch* b) {
@@ -1142,11 +1184,11 @@ static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b
     const bool fuse = (which & 3) == 3 && b.curves;
     if ((which & 1) && b.curves && !fuse) {
         if (p.indenter == NFX_INDENTER_SPHERE)
-            k_batch_curves<1><<<b.ncells, kBlock, 0, ctx->stream>>>(p.points, p.N, b.cell_node_off, b.cells, b.curves,
-                                                                    b.ncurve, b.curve_steps, step, sd, off);
+            k_batch_curves<1><<<b.ncells, kBlock, 0, ctx->stream>>>(p.disp, p.ref, p.N, b.cell_node_off, b.cells,
+                                                                    b.curves, b.ncurve, b.curve_steps, step, sd, off);
         else
-            k_batch_curves<2><<<b.ncells, kBlock, 0, ctx->stream>>>(p.points, p.N, b.cell_node_off, b.cells, b.curves,
-                                                                    b.ncurve, b.curve_steps, step, sd, off);
+            k_batch_curves<2><<<b.ncells, kBlock, 0, ctx->stream>>>(p.disp, p.ref, p.N, b.cell_node_off, b.cells,
+                                                                    b.curves, b.ncurve, b.curve_steps, step, sd, off);
         NFX_CHECK_LAUNCH();
     }
     if (which & 2) {
@@ -1161,7 +1203,7 @@ static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b
         if (rc) return rc;
     }
     if (which & 4) {
-        BatchNodeArgs a{p.N, p.points, p.vel, p.node_off, p.partials, b.node_cell, b.cells};
+        BatchNodeArgs a{p.N, p.disp, p.ref, p.vel, p.node_off, p.partials, b.node_cell, b.cells};
         if (p.substrate) launch_batch_node_sub<true>(ctx, p, a, step, sd, off);
         else launch_batch_node_sub<false>(ctx, p, a, step, sd, off);
         NFX_CHECK_LAUNCH();
@@ -1169,19 +1211,51 @@ static int launch_batch_step(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b
     return NFX_OK;
 }
 
-static void drop_bgraph(nfx_ctx* ctx) {
-    if (ctx->bgraph_exec) (void)hipGraphExecDestroy(ctx->bgraph_exec);
-    if (ctx->bgraph) (void)hipGraphDestroy(ctx->bgraph);
-    ctx->bgraph_exec = nullptr;
-    ctx->bgraph = nullptr;
-    ctx->bgraph_steps = 0;
+// ---- captured step blocks --------------------------------------------------------------------
+// A context keeps the graphs of the last kMaxGraphs (plan, batch, history mode) combinations it
+// stepped, so that callers alternating between several batches (cell groups) or plans replay their
+// own graphs instead of capturing again.
+static void drop_graphs(nfx_ctx* ctx) {
+    for (auto& g : ctx->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    ctx->graphs.clear();
 }
 
-static int capture_bgraph(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b, int64_t steps) {
-    drop_bgraph(ctx);
+static nfx_ctx::GraphEntry* find_graph(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch* b, bool hist) {
+    for (auto& g : ctx->graphs)
+        if (g.is_batch == (b != nullptr) && g.hist == hist && std::memcmp(&g.plan, &p, sizeof(nfx_plan)) == 0 &&
+            (!b || std::memcmp(&g.batch, b, sizeof(nfx_batch)) == 0)) {
+            g.last_use = ++ctx->graph_clock;
+            return &g;
+        }
+    return nullptr;
+}
+
+// capture kGraphBlock steps of the plan (b == NULL) or of the batch; steps read the device step
+// counter ctx->step_dev, so a replay is valid at any step offset
+static int capture_graph(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch* b, bool hist, nfx_ctx::GraphEntry** out) {
+    if ((int)ctx->graphs.size() >= kMaxGraphs) {  // evict the least recently used
+        size_t old = 0;
+        for (size_t i = 1; i < ctx->graphs.size(); ++i)
+            if (ctx->graphs[i].last_use < ctx->graphs[old].last_use) old = i;
+        if (ctx->graphs[old].exec) (void)hipGraphExecDestroy(ctx->graphs[old].exec);
+        if (ctx->graphs[old].graph) (void)hipGraphDestroy(ctx->graphs[old].graph);
+        ctx->graphs.erase(ctx->graphs.begin() + (std::ptrdiff_t)old);
+    }
     NFX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeGlobal));
     int rc = NFX_OK;
-    for (int64_t i = 0; i < steps && rc == NFX_OK; ++i) rc = launch_batch_step(ctx, p, b, 0, ctx->step_dev, (int32_t)i);
+    for (int64_t i = 0; i < kGraphBlock && rc == NFX_OK; ++i) {
+        if (b) {
+            rc = launch_batch_step(ctx, p, *b, 0, ctx->step_dev, (int32_t)i);
+        } else {
+            rc = launch_force_pass(ctx, p, false);
+            if (rc == NFX_OK)
+                rc = launch_node_pass(ctx, p, 0, ctx->step_dev, (int32_t)i, hist ? ctx->hist_dev + i : nullptr,
+                                      nullptr);
+        }
+    }
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
     if (rc != NFX_OK) {
@@ -1192,46 +1266,22 @@ static int capture_bgraph(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch& b, i
         set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
         return NFX_ERR_HIP;
     }
-    ctx->bgraph = g;
-    NFX_HIP(hipGraphInstantiate(&ctx->bgraph_exec, g, nullptr, nullptr, 0));
-    ctx->bgraph_plan = p;
-    ctx->bgraph_batch = b;
-    ctx->bgraph_steps = steps;
-    return NFX_OK;
-}
-
-static void drop_graph(nfx_ctx* ctx) {
-    if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->graph_exec = nullptr;
-    ctx->graph = nullptr;
-    ctx->graph_steps = 0;
-}
-
-static int capture_graph(nfx_ctx* ctx, const nfx_plan& p, int64_t steps, bool hist) {
-    drop_graph(ctx);
-    NFX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeGlobal));
-    int rc = NFX_OK;
-    for (int64_t i = 0; i < steps && rc == NFX_OK; ++i) {
-        rc = launch_force_pass(ctx, p, false);
-        if (rc == NFX_OK)
-            rc = launch_node_pass(ctx, p, 0, ctx->step_dev, (int32_t)i, hist ? ctx->hist_dev + i : nullptr, nullptr);
-    }
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
-    if (rc != NFX_OK) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-    }
-    if (e != hipSuccess) {
-        set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    nfx_ctx::GraphEntry ge{};
+    ge.graph = g;
+    const hipError_t ei = hipGraphInstantiate(&ge.exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        set_error("hipGraphInstantiate failed: %s", hipGetErrorString(ei));
         return NFX_ERR_HIP;
     }
-    ctx->graph = g;
-    NFX_HIP(hipGraphInstantiate(&ctx->graph_exec, g, nullptr, nullptr, 0));
-    ctx->graph_plan = p;
-    ctx->graph_steps = steps;
-    ctx->graph_hist = hist;
+    ge.plan = p;
+    if (b) ge.batch = *b;
+    ge.is_batch = b != nullptr;
+    ge.hist = hist;
+    ge.last_use = ++ctx->graph_clock;
+    ctx->graphs.push_back(ge);
+    ctx->captures++;
+    *out = &ctx->graphs.back();
     return NFX_OK;
 }
 
@@ -1305,8 +1355,7 @@ int nfx_destroy(nfx_ctx* c) {
     if (!c) return NFX_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    nfx::drop_graph(c);
-    nfx::drop_bgraph(c);
+    nfx::drop_graphs(c);
     (void)hipFree(c->partials);
     (void)hipFree(c->counter);
     (void)hipFree(c->step_dev);
@@ -1315,6 +1364,12 @@ int nfx_destroy(nfx_ctx* c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return NFX_OK;
+}
+
+int nfx_graph_captures(nfx_ctx* c, int64_t* count) {
+    NFX_REQUIRE(c && count, "NULL argument");
+    *count = c->captures;
     return NFX_OK;
 }
 
@@ -1334,13 +1389,13 @@ int nfx_device_name(nfx_ctx* c, char* buf, int32_t len) {
 
 int nfx_batch_geometry(nfx_ctx* c, const double* p_rel, const double* cell_scale, const uint32_t* node_cell,
                        const uint32_t* tetras, const uint16_t* tile_ntet, int32_t tile_k, int64_t N, int64_t T,
-                       float* points, float* dminv, float* v0) {
-    NFX_REQUIRE(c && p_rel && cell_scale && node_cell && tetras && tile_ntet && points && dminv && v0, "NULL argument");
+                       float* ref, float* dminv, float* v0) {
+    NFX_REQUIRE(c && p_rel && cell_scale && node_cell && tetras && tile_ntet && ref && dminv && v0, "NULL argument");
     NFX_REQUIRE(tile_k == 256 || tile_k == 512, "tile_k must be 256 or 512");
     int rc = check_sizes(N, T);
     if (rc) return rc;
     NFX_REQUIRE(T % tile_k == 0, "padded tetra count must be a multiple of tile_k");
-    k_batch_points<<<nblocks(N), kBlock, 0, c->stream>>>(p_rel, cell_scale, node_cell, N, points);
+    k_batch_points<<<nblocks(N), kBlock, 0, c->stream>>>(p_rel, cell_scale, node_cell, N, ref);
     NFX_CHECK_LAUNCH();
     k_batch_edges<<<nblocks(T), kBlock, 0, c->stream>>>(p_rel, cell_scale, node_cell, tetras, tile_ntet, tile_k, N, T,
                                                         dminv, v0);
@@ -1357,27 +1412,30 @@ int nfx_precompute_f64(nfx_ctx* c, const double* edge_vectors, const double* vol
     return NFX_OK;
 }
 
+static ElasticDev elastic_of(const nfx_elastic& m) {
+    nfx_plan p{};
+    p.model = m.model;
+    p.E = m.E;
+    p.nu = m.nu;
+    p.ratio = m.ratio;
+    p.shear = m.shear;
+    p.bulk = m.bulk;
+    p.shear2 = m.shear2;
+    p.volume_fraction = m.volume_fraction;
+    return elastic_consts(p);
+}
+
 static int tetra_common(nfx_ctx* c, const nfx_elastic* m, const float* points, const uint32_t* tetras,
                         const float* dminv, const float* v0, int64_t N, int64_t T, float* force, float* corners) {
     NFX_REQUIRE(c && m && points && tetras && dminv && v0 && (force || corners), "NULL argument");
     NFX_REQUIRE(m->model >= NFX_SVKK && m->model <= NFX_MR, "unknown elastic model %d", m->model);
     int rc = check_sizes(N, T);
     if (rc) return rc;
-    nfx_plan p{};
-    p.N = N;
-    p.T = T;
-    p.tetras = tetras;
-    p.dminv = dminv;
-    p.v0 = v0;
-    p.points = const_cast<float*>(points);
-    p.force = force;
-    p.corners = corners;
-    p.model = m->model;
-    p.E = m->E;
-    p.nu = m->nu;
-    p.ratio = m->ratio;
-    p.scatter = corners ? NFX_SCATTER_CSR : NFX_SCATTER_ATOMIC;
-    return launch_force_pass(c, p, false);
+    const TetArgs a{N, T, tetras, dminv, v0, points, force, corners, nullptr, nullptr};
+    launch_tetra_pass<false>(c->stream, a, m->model, corners != nullptr, elastic_of(*m), VPConst{}, nullptr, nullptr,
+                             nullptr, nullptr);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
 }
 
 int nfx_legacy_force(nfx_ctx* c, const nfx_elastic* m, const float* points, const uint32_t* tetras,
@@ -1387,26 +1445,13 @@ int nfx_legacy_force(nfx_ctx* c, const nfx_elastic* m, const float* points, cons
     NFX_REQUIRE(is_legacy(m->model), "not a legacy law: model %d", m->model);
     int rc = check_sizes(N, T);
     if (rc) return rc;
-    nfx_plan p{};
-    p.N = N;
-    p.T = T;
-    p.tetras = tetras;
-    p.dminv = dminv;
-    p.points = const_cast<float*>(points);
-    p.force = force;
-    p.corners = corners;
-    p.model = m->model;
-    p.E = m->E;
-    p.nu = m->nu;
-    p.ratio = m->ratio;
-    p.shear = m->shear;
-    p.bulk = m->bulk;
-    p.shear2 = m->shear2;
-    p.volume_fraction = m->volume_fraction;
-    p.von_mises = von_mises;
-    p.pressure = pressure;
-    p.scatter = corners ? NFX_SCATTER_CSR : NFX_SCATTER_ATOMIC;
-    return launch_force_pass(c, p, true);
+    const bool st = von_mises && pressure;
+    const TetArgs a{N, T, tetras, dminv, nullptr, points, force, corners, st ? von_mises : nullptr,
+                    st ? pressure : nullptr};
+    launch_tetra_pass<false>(c->stream, a, m->model, corners != nullptr, elastic_of(*m), VPConst{}, nullptr, nullptr,
+                             nullptr, nullptr);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
 }
 
 int nfx_tetra_force(nfx_ctx* c, const nfx_elastic* m, const float* points, const uint32_t* tetras, const float* dminv,
@@ -1427,44 +1472,42 @@ int nfx_viscoplastic_force(nfx_ctx* c, const nfx_viscoplastic* vp, const float* 
     NFX_REQUIRE(c && vp && points && tetras && dminv && Fp && epsbar && (force || corners), "NULL argument");
     int rc = check_sizes(N, T);
     if (rc) return rc;
-    nfx_plan p{};
-    p.N = N;
-    p.T = T;
-    p.tetras = tetras;
-    p.dminv = dminv;
-    p.points = const_cast<float*>(points);
-    p.force = force;
-    p.corners = corners;
-    p.model = NFX_VISCOPLASTIC;
-    p.vp = *vp;
-    p.Fp = Fp;
-    p.epsbar = epsbar;
-    p.von_mises = von_mises;
-    p.pressure = pressure;
-    p.scatter = corners ? NFX_SCATTER_CSR : NFX_SCATTER_ATOMIC;
-    return launch_force_pass(c, p, true);
-}
-
-int nfx_vp_shape_from_fp(nfx_ctx* c, const float* dminv, const float* Fp, int64_t T, float* shape) {
-    NFX_REQUIRE(c && dminv && Fp && shape, "NULL argument");
-    NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
-    k_vp_convert<true><<<nblocks(T), kBlock, 0, c->stream>>>(dminv, Fp, T, shape);
+    const bool st = von_mises && pressure;
+    const TetArgs a{N, T, tetras, dminv, nullptr, points, force, corners, nullptr, nullptr};
+    launch_tetra_pass<false>(c->stream, a, NFX_VISCOPLASTIC, corners != nullptr, ElasticDev{}, vp_consts(*vp), Fp,
+                             epsbar, st ? von_mises : nullptr, st ? pressure : nullptr);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }
 
-int nfx_vp_sym_from_shape(nfx_ctx* c, const float* shape, int64_t T, float* sym) {
-    NFX_REQUIRE(c && shape && sym, "NULL argument");
+int nfx_vp_q_from_fp(nfx_ctx* c, const float* Fp, int64_t T, float* Q) {
+    NFX_REQUIRE(c && Fp && Q, "NULL argument");
     NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
-    k_vp_sym<<<nblocks(T), kBlock, 0, c->stream>>>(shape, T, sym);
+    k_vp_convert<true><<<nblocks(T), kBlock, 0, c->stream>>>(Fp, T, Q);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }
 
-int nfx_vp_fp_from_shape(nfx_ctx* c, const float* dminv, const float* shape, int64_t T, float* Fp) {
-    NFX_REQUIRE(c && dminv && Fp && shape, "NULL argument");
+int nfx_vp_fp_from_q(nfx_ctx* c, const float* Q, int64_t T, float* Fp) {
+    NFX_REQUIRE(c && Q && Fp, "NULL argument");
     NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
-    k_vp_convert<false><<<nblocks(T), kBlock, 0, c->stream>>>(dminv, shape, T, Fp);
+    k_vp_convert<false><<<nblocks(T), kBlock, 0, c->stream>>>(Q, T, Fp);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_points_from_disp(nfx_ctx* c, const float* ref, const float* disp, int64_t N, float* points) {
+    NFX_REQUIRE(c && ref && disp && points, "NULL argument");
+    NFX_REQUIRE(N > 0 && N < (int64_t(1) << 31), "node count out of range");
+    k_disp_convert<true><<<nblocks(3 * N), kBlock, 0, c->stream>>>(ref, disp, 3 * N, points);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_disp_from_points(nfx_ctx* c, const float* ref, const float* points, int64_t N, float* disp) {
+    NFX_REQUIRE(c && ref && disp && points, "NULL argument");
+    NFX_REQUIRE(N > 0 && N < (int64_t(1) << 31), "node count out of range");
+    k_disp_convert<false><<<nblocks(3 * N), kBlock, 0, c->stream>>>(ref, points, 3 * N, disp);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }
@@ -1609,13 +1652,17 @@ int nfx_velocity_verlet(nfx_ctx* c, float* points, float* vel, float* force, flo
     return NFX_OK;
 }
 
-int nfx_tip_force(nfx_ctx* c, const float* points, int64_t N, int32_t ind, double k, double radius, double pos,
-                  double* out2) {
+int nfx_tip_force(nfx_ctx* c, const float* points, const float* ref, int64_t N, int32_t ind, double k, double radius,
+                  double pos, double* out2) {
     NFX_REQUIRE(c && points && out2 && N > 0, "bad argument");
     NFX_REQUIRE(ind == NFX_INDENTER_SPHERE || ind == NFX_INDENTER_PLANE, "unknown indenter %d", ind);
     const unsigned int nb = std::min<unsigned int>(nblocks(N), kMaxReduceBlocks);
-    k_tip<<<nb, kBlock, 0, c->stream>>>(points, N, ind, (float)k, (float)radius, (float)pos, c->partials, c->counter,
-                                        out2, out2 + 1);
+    if (ref)
+        k_tip<true><<<nb, kBlock, 0, c->stream>>>(points, ref, N, ind, k, radius, pos, c->partials, c->counter, out2,
+                                                  out2 + 1);
+    else
+        k_tip<false><<<nb, kBlock, 0, c->stream>>>(points, nullptr, N, ind, k, radius, pos, c->partials, c->counter,
+                                                   out2, out2 + 1);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }
@@ -1639,18 +1686,21 @@ int nfx_run_steps(nfx_ctx* c, const nfx_plan* p, int64_t step0, int64_t nsteps, 
     // the per-tetra von Mises stress and pressure (the state a frame written afterwards shows)
     const bool stress = p->von_mises && p->pressure;
     const int64_t ngraph = stress ? nsteps - 1 : nsteps;
-    if (use_graph && ngraph >= 2) {
-        const int64_t B = std::min<int64_t>(ngraph, kGraphBlock);
+    // whole blocks of kGraphBlock steps replay one captured graph, kept for as long as the plan and
+    // the history mode stay the same; a remainder (and a call shorter than a block) runs eagerly,
+    // so no call re-captures for a new step count
+    if (use_graph && ngraph >= kGraphBlock) {
+        const int64_t B = kGraphBlock;
         const bool hist = tip_hist != nullptr;
-        if (!c->graph_exec || c->graph_steps != B || c->graph_hist != hist ||
-            std::memcmp(&c->graph_plan, p, sizeof(nfx_plan)) != 0) {
-            rc = capture_graph(c, *p, B, hist);
+        nfx_ctx::GraphEntry* ge = find_graph(c, *p, nullptr, hist);
+        if (!ge) {
+            rc = capture_graph(c, *p, nullptr, hist, &ge);
             if (rc) return rc;
         }
         while (ngraph - done >= B) {
             k_set_i64<<<1, 1, 0, c->stream>>>(c->step_dev, step0 + done);
             NFX_CHECK_LAUNCH();
-            NFX_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+            NFX_HIP(hipGraphLaunch(ge->exec, c->stream));
             if (hist)
                 NFX_HIP(hipMemcpyAsync(tip_hist + done, c->hist_dev, sizeof(double) * B, hipMemcpyDeviceToDevice,
                                        c->stream));
@@ -1707,17 +1757,17 @@ int nfx_batch_run_steps(nfx_ctx* c, const nfx_plan* p, const nfx_batch* b, int64
     if (rc) return rc;
     NFX_REQUIRE(nsteps >= 0, "nsteps < 0");
     int64_t done = 0;
-    if (use_graph && nsteps >= 2) {
-        const int64_t B = std::min<int64_t>(nsteps, kGraphBlock);
-        if (!c->bgraph_exec || c->bgraph_steps != B || std::memcmp(&c->bgraph_plan, p, sizeof(nfx_plan)) != 0 ||
-            std::memcmp(&c->bgraph_batch, b, sizeof(nfx_batch)) != 0) {
-            rc = capture_bgraph(c, *p, *b, B);
+    if (use_graph && nsteps >= kGraphBlock) {  // as in nfx_run_steps: whole blocks replay, the rest is eager
+        const int64_t B = kGraphBlock;
+        nfx_ctx::GraphEntry* ge = find_graph(c, *p, b, false);
+        if (!ge) {
+            rc = capture_graph(c, *p, b, false, &ge);
             if (rc) return rc;
         }
         while (nsteps - done >= B) {
             k_set_i64<<<1, 1, 0, c->stream>>>(c->step_dev, step0 + done);
             NFX_CHECK_LAUNCH();
-            NFX_HIP(hipGraphLaunch(c->bgraph_exec, c->stream));
+            NFX_HIP(hipGraphLaunch(ge->exec, c->stream));
             done += B;
         }
     }

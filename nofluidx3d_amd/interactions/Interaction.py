@@ -1,13 +1,38 @@
-"""Plugin base class (reference ``nofluidx3d/interactions/Interaction.py:15-40``).
+"""Plugin protocol of the simulation (reference ``nofluidx3d/interactions/Interaction.py:15-40``).
 
-Protocol: ``register(simulation)`` calls ``createBuffers()``, ``build()``, ``setArgs()``;
-``Simulation.timeStep`` calls every plugin's ``beforeTimeStep(globalTime)`` and then every
-plugin's ``enqueue()`` in registration order.  ``enqueue`` launches the plugin's precompiled
-kernel on the context's stream.
+A plugin is registered with ``Simulation.register``, which calls ``register(simulation)``: the
+plugin allocates its device state (``createBuffers``), binds its kernel (``build``, which sets
+``scope`` and ``knl``) and hands the kernel its arguments (``setArgs``).  Each reference time step
+then calls ``beforeTimeStep(globalTime)`` on every plugin and ``enqueue()`` on every plugin, in
+registration order.
 
-``contribute(plan)`` is this implementation's fusion hook: a plugin that can describe its term to
-the fused step (``nfx_run_steps``) fills its fields of the ``nfx_plan`` and returns True.
+The built-in plugins bind ``knl`` to a :class:`DeviceCall` of a libnfx entry point (precompiled
+for gfx950); third-party plugins may still go through ``KernelBuilder``.
+
+Two hooks belong to this implementation:
+- ``contribute(plan)``: a plugin that can describe its term to the fused device step
+  (``nfx_run_steps``) fills its fields of the ``nfx_plan`` and returns True;
+- ``state_form(fused)`` / ``sync_forms()`` / ``forms_restored()``: plugins whose state the fused
+  step keeps in another representation (the viscoplastic law's plastic deviation) convert between
+  the two when the stepping path changes.
 """
+
+from __future__ import annotations
+
+
+class DeviceCall:
+    """A libnfx entry point bound to an argument provider: ``call()`` enqueues it on the active
+    context's stream with the arguments ``args()`` returns at that moment (device pointers are read
+    when the call is made, so buffers swapped after ``build`` are picked up)."""
+
+    def __init__(self, entry: str, args):
+        self.entry = entry
+        self.args = args
+
+    def __call__(self):
+        from nofluidx3d_amd import _lib, openCL
+
+        _lib.call(self.entry, openCL.handle(), *self.args())
 
 
 class Interaction:
@@ -39,12 +64,16 @@ class Interaction:
     def contribute(self, plan) -> bool:
         return False
 
-    def state_form(self, shape: bool):
-        """Called before steps: ``shape`` is True when the tiled fused step will advance the
-        plugin's state (plugins with a second state representation convert here)."""
+    def state_form(self, fused: bool):
+        """Called before steps; ``fused`` is True when the fused device step will advance the
+        plugin's state next."""
 
     def sync_forms(self):
         """Bring every state representation up to date (before a snapshot of the state)."""
 
     def forms_restored(self):
         """Every state representation was just restored from a snapshot taken after sync_forms."""
+
+    def state_buffers(self):
+        """Device tensors holding this plugin's evolving state (for snapshots)."""
+        return []

@@ -1,35 +1,27 @@
-"""Rigid substrate penalty wall pushing along +y (reference ``interactions/Substrate.py:18-37``,
-kernel ``Substrate.cl``)."""
-import ctypes
-from pathlib import Path
+"""Rigid substrate under the cell (reference ``interactions/Substrate.py:18-37``, kernel
+``Interactions/Substrate.cl:1-7``): a penalty force k * max(wall - y, 0) along +y on every node
+below the wall plane y = wallPosition.
 
-from nofluidx3d_amd.interactions.Interaction import Interaction
-from nofluidx3d_amd.KernelBuilder import KernelBuilder
-
-kernels = Path(__file__).parents[1] / "kernels"
+Per-plugin step: ``nfx_substrate_force`` on the absolute positions.  Fused step: the same term is
+evaluated inside the node pass (``contribute``).
+"""
+from nofluidx3d_amd.interactions.Interaction import DeviceCall, Interaction
 
 
 class Substrate(Interaction):
     def __init__(self, wallPosition, forceConst):
+        super().__init__()
         self.wallPosition = wallPosition
         self.forceConst = forceConst
 
     def build(self):
-        self.scope = self.simulation.numPoints
-        KernelBuilder.define(INSERT_NUM_POINTS=self.simulation.numPoints, def_FORCE_CONST=self.forceConst)
-        self.knl = KernelBuilder.build(kernels / "Interactions" / "Substrate.cl", "Interaction_Substrate")
-
-    def setArgs(self):
-        self.knl.set_args(
-            self.simulation.force.buf,
-            self.simulation.points.buf,
-            ctypes.c_double(self.wallPosition),
-        )
+        sim = self.simulation
+        self.scope = sim.numPoints
+        self.knl = DeviceCall("nfx_substrate_force", lambda: (sim.force.ptr, sim.points.ptr, sim.numPoints,
+                                                              float(self.forceConst), float(self.wallPosition)))
 
     def contribute(self, plan) -> bool:
-        if plan.substrate:
+        if plan.substrate:  # one substrate per fused step
             return False
-        plan.substrate = 1
-        plan.substrate_k = float(self.knl.values["def_FORCE_CONST"])
-        plan.wall = float(self.wallPosition)
+        plan.substrate, plan.substrate_k, plan.wall = 1, float(self.forceConst), float(self.wallPosition)
         return True

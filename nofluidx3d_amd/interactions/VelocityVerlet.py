@@ -1,43 +1,44 @@
-"""Damped velocity-Verlet integrator (reference ``interactions/VelocityVerlet.py:22-50``, kernel
-``VelocityVerlet.cl``).  Register it last: it consumes and clears the accumulated force."""
-from pathlib import Path
+"""Damped velocity-Verlet time integration (reference ``interactions/VelocityVerlet.py:22-50``,
+kernel ``Interactions/VelocityVerlet.cl:1-36``): unit node mass, dt = 1, damping xi = 0.005;
+``fixTopBottom`` pins x and z of nodes 3 and 4.  It consumes and clears the accumulated force, so
+it is registered last.
 
+State: the velocity and the previous step's force, 3N FP32 each, starting at zero.  The fused step
+advances the same state in its node pass (``contribute``).
+"""
 import numpy as np
 
-from nofluidx3d_amd.interactions.Interaction import Interaction
-from nofluidx3d_amd.KernelBuilder import KernelBuilder
-
-kernels = Path(__file__).parents[1] / "kernels"
+from nofluidx3d_amd.interactions.Interaction import DeviceCall, Interaction
 
 
 class VelocityVerlet(Interaction):
     def __init__(self, numPoints, fixTopBottom=False):
-        self.fixTopBottom = fixTopBottom
+        super().__init__()
+        self.numPoints = numPoints
+        self.fixTopBottom = bool(fixTopBottom)
 
     def createBuffers(self):
         import torch
 
-        numPoints = self.simulation.numPoints
-        dev = self.simulation.points.buf.device
-        # velocity and old force start at zero (VelocityVerlet.py:29-36)
-        self.velocityB = torch.zeros(3 * numPoints, dtype=torch.float32, device=dev)
-        self.forceOldB = torch.zeros(3 * numPoints, dtype=torch.float32, device=dev)
+        n3 = 3 * self.simulation.numPoints
+        dev = self.simulation.device
+        self.velocityB = torch.zeros(n3, dtype=torch.float32, device=dev)
+        self.forceOldB = torch.zeros(n3, dtype=torch.float32, device=dev)
 
     def build(self):
-        self.scope = self.simulation.numPoints
-        KernelBuilder.define(INSERT_NUM_POINTS=self.simulation.numPoints)
-        if self.fixTopBottom:
-            KernelBuilder.define("FIXED_TOP_BOTTOM")
-        self.knl = KernelBuilder.build(kernels / "Interactions" / "VelocityVerlet.cl", "VelocityVerlet")
-
-    def setArgs(self):
-        self.knl.set_args(self.simulation.points.buf, self.velocityB, self.simulation.force.buf, self.forceOldB)
+        sim = self.simulation
+        self.scope = sim.numPoints
+        self.knl = DeviceCall("nfx_velocity_verlet", lambda: (sim.points.ptr, self.velocityB.data_ptr(), sim.force.ptr,
+                                                              self.forceOldB.data_ptr(), sim.numPoints,
+                                                              int(self.fixTopBottom)))
 
     def contribute(self, plan) -> bool:
-        plan.vel = self.velocityB.data_ptr()
-        plan.fold = self.forceOldB.data_ptr()
-        plan.fix_top_bottom = 1 if "FIXED_TOP_BOTTOM" in self.knl.defines else 0
+        plan.vel, plan.fold = self.velocityB.data_ptr(), self.forceOldB.data_ptr()
+        plan.fix_top_bottom = int(self.fixTopBottom)
         return True
+
+    def state_buffers(self):
+        return [self.velocityB, self.forceOldB]
 
     def velocities(self) -> np.ndarray:
         return self.velocityB.cpu().numpy().astype(np.float64)

@@ -49,7 +49,7 @@ def positions(sim):
 
     _lib.call("nfx_stream_sync", openCL.handle())
     torch.cuda.synchronize()
-    return sim.points.buf.cpu().numpy().copy()
+    return sim.displacement.cpu().numpy().copy()
 
 
 def refused(name, *args, expect):
@@ -77,7 +77,8 @@ PLAN_CASES = [
     ({"N": 0}, "node count 0 out of range"),
     ({"T": 1 << 30}, "tetra count"),
     ({"model": 99}, "unknown model 99"),
-    ({"points": None}, "NULL mesh/state pointer"),
+    ({"disp": None}, "NULL mesh/state pointer"),
+    ({"ref": None}, "NULL mesh/state pointer"),
     ({"vp_sym": 1}, "symmetric state (vp_sym) is for the tiled viscoplastic force only"),
     ({"scatter": 7}, "unknown scatter 7"),
     ({"tile_k": 128}, "tile_k must be 256 or 512"),

@@ -198,14 +198,22 @@ def test_node_kernels_parity(ctx):
         assert np.max(np.abs(host(xd) - xr)) < 4e-6
         assert np.max(np.abs(host(vd) - vr)) < 1e-7
         assert np.array_equal(host(fod), for_) and np.all(host(fd) == 0)
-    # FP64 tip-force reduction vs oracle sum on the same FP32 positions
+    # FP64 tip-force reduction vs oracle sum on the same FP32 positions, from absolute positions and
+    # from displacements u against reference positions X0 (x = X0 + u summed in FP64)
     out = torch.zeros(2, dtype=torch.float64, device="cuda:0")
+    X0 = O.soa((x + 30.0).astype(np.float32).astype(np.float64))
+    u = (xs - X0).astype(np.float32).astype(np.float64)
     for ind, R, pos in ((1, 3.0, 2.0), (2, 0.0, 1.0)):
         xsd = dev(xs, np.float32)
-        call("nfx_tip_force", ctx, xsd.data_ptr(), N, ind, 0.1, R, pos, out.data_ptr())
+        call("nfx_tip_force", ctx, xsd.data_ptr(), None, N, ind, 0.1, R, pos, out.data_ptr())
         f, m = host(out)
         fr, mr = O.tip_force(xs, ind, 0.1, R, pos)
-        assert f == pytest.approx(fr, rel=1e-5) and m == pytest.approx(mr, rel=1e-5)
+        assert f == pytest.approx(fr, rel=1e-12) and m == pytest.approx(mr, rel=1e-12)
+        ud, X0d = dev(u, np.float32), dev(X0, np.float32)
+        call("nfx_tip_force", ctx, ud.data_ptr(), X0d.data_ptr(), N, ind, 0.1, R, pos, out.data_ptr())
+        f, m = host(out)
+        fr, mr = O.tip_force(X0 + u, ind, 0.1, R, pos)
+        assert f == pytest.approx(fr, rel=1e-12) and m == pytest.approx(mr, rel=1e-12)
 
 
 @pytest.mark.parametrize("law", [(1.0, 0.0, 1.0, 0.0005), (1.0, 0.0, 1.0, 0.0), (2.5, 0.3, 0.01, 0.0),
@@ -251,26 +259,57 @@ def test_viscoplastic_kernel_parity(ctx, law):
     assert acc_r.max() > 0  # flow happened
 
 
-def test_vp_shape_state_conversions(ctx):
-    """S = Dm^-1 Fp^-1 (the tiled viscoplastic state) and back, against FP64 numpy on random
-    plastic deformations.  Tolerances: FP32 products and cofactor inverses (cond(Dm) <= ~30)."""
+def test_vp_deviation_conversions(ctx):
+    """Q = Fp^-1 - I (the fused viscoplastic state) from Fp and back, against FP64 numpy on random
+    plastic deformations; and the position forms x = X0 + u, u = x - X0."""
     import torch
 
-    P, T = small_mesh()
-    nT = len(T)
-    dminv, _, _, _ = gpu_precompute(ctx, P, T)
-    D = host(dminv).reshape(9, -1).T.reshape(-1, 3, 3)
+    nT = 5000
     Fp = (np.eye(3) + 0.05 * RNG.normal(size=(nT, 3, 3))).astype(np.float32).astype(np.float64)
     Fpd = dev(Fp.reshape(nT, 9).T.reshape(-1), np.float32)
-    S = torch.empty(9 * nT, dtype=torch.float32, device="cuda:0")
-    call("nfx_vp_shape_from_fp", ctx, dminv.data_ptr(), Fpd.data_ptr(), nT, S.data_ptr())
-    Sg = host(S).reshape(9, -1).T.reshape(-1, 3, 3)
-    Sref = D @ np.linalg.inv(Fp)
-    assert np.max(np.abs(Sg - Sref).max(axis=(1, 2)) / np.abs(Sref).max(axis=(1, 2))) < 2e-6
-    back = torch.empty_like(S)
-    call("nfx_vp_fp_from_shape", ctx, dminv.data_ptr(), S.data_ptr(), nT, back.data_ptr())
+    Q = torch.empty(9 * nT, dtype=torch.float32, device="cuda:0")
+    call("nfx_vp_q_from_fp", ctx, Fpd.data_ptr(), nT, Q.data_ptr())
+    Qg = host(Q).reshape(9, -1).T.reshape(-1, 3, 3)
+    Qref = np.linalg.inv(Fp) - np.eye(3)
+    # an FP32 Fp near I: the inverse's diagonal carries ~ulp(1) = 1.2e-7 before I is subtracted
+    assert np.max(np.abs(Qg - Qref)) < 5e-7
+    back = torch.empty_like(Q)
+    call("nfx_vp_fp_from_q", ctx, Q.data_ptr(), nT, back.data_ptr())
     Fb = host(back).reshape(9, -1).T.reshape(-1, 3, 3)
-    assert np.max(np.abs(Fb - Fp)) < 5e-5
+    assert np.max(np.abs(Fb - Fp)) < 5e-7
+    N = 4000
+    X0 = RNG.normal(scale=40.0, size=3 * N).astype(np.float32)
+    u = RNG.normal(scale=0.5, size=3 * N).astype(np.float32)
+    X0d, ud = dev(X0), dev(u)
+    x = torch.empty_like(X0d)
+    call("nfx_points_from_disp", ctx, X0d.data_ptr(), ud.data_ptr(), N, x.data_ptr())
+    assert np.array_equal(host(x, np.float32), X0 + u)
+    u2 = torch.empty_like(X0d)
+    call("nfx_disp_from_points", ctx, X0d.data_ptr(), x.data_ptr(), N, u2.data_ptr())
+    assert np.array_equal(host(u2, np.float32), (X0 + u) - X0)
+
+
+def test_graph_is_captured_once_per_plan(ctx):
+    """Whole 100-step blocks replay one captured graph whatever the call's step count; the rest of
+    a call runs eagerly (a second call with the same count, or another count, captures nothing)."""
+    from nofluidx3d_amd import _lib
+
+    def captures():
+        n = ctypes.c_int64()
+        _lib.call("nfx_graph_captures", ctx, ctypes.byref(n))
+        return n.value
+
+    sim = build_sim(scatter="tiled")
+    c0 = captures()
+    sim.advance(20)
+    assert captures() == c0  # shorter than a block: eager
+    sim.advance(250)
+    c1 = captures()
+    assert c1 == c0 + 1
+    sim.advance(250)
+    sim.advance(130)
+    sim.advance(20)
+    assert captures() == c1
 
 
 # ------------------------------------------------------------------------------------------------
@@ -331,10 +370,9 @@ def oracle_for(sim, nthreads=1):
             prm.legacy = O.LegacyParams(E=p.E, nu=p.nu, shear=p.shear, bulk=p.bulk, shear2=p.shear2,
                                         phi=p.volume_fraction)
         elif isinstance(inter, Tetra):
-            v = inter.knl.values
             prm.model = O.MR if inter.model_code() == 2 else inter.model_code()
-            prm.E, prm.nu, prm.ratio = (float(v[k]) for k in ("def_tetraYoungsModulus", "def_tetraPoissonRatio",
-                                                                "def_tetraMooneyRivlinRatio"))
+            prm.E, prm.nu, prm.ratio = (float(v) for v in (inter.youngsModulus, inter.poissonRatio,
+                                                             inter.mooneyRivlinRatio))
         elif isinstance(inter, FiniteStrainViscoplastic):
             prm.model = O.VISCOPLASTIC
             p = inter.params
@@ -391,7 +429,7 @@ def test_viscoplastic_fused_run_matches_oracle(ctx, scatter):
 
 @pytest.mark.parametrize("scatter", ["csr", "tiled"])
 def test_viscoplastic_state_matches_oracle(ctx, scatter):
-    """Per-tetra state after 600 steps: Fp (read back from the shape form on the tiled path) and
+    """Per-tetra state after 600 steps: Fp (converted from the fused step's plastic deviation Q) and
     the accumulated plastic strain, against the FP64 oracle's Fp and epsbar."""
     from nofluidx3d_amd.interactions import FiniteStrainViscoplastic
 
@@ -409,8 +447,9 @@ def test_viscoplastic_state_matches_oracle(ctx, scatter):
 
 
 def test_viscoplastic_switching_between_fused_and_per_plugin_steps(ctx):
-    """Tiled fused steps (shape-form state), then per-plugin steps (Fp form), then fused again:
-    the state follows each switch and the run matches the oracle."""
+    """Tiled fused steps (displacements, plastic deviation Q), then per-plugin steps (absolute
+    positions, Fp), then fused again: the state follows each switch and the run matches the
+    oracle."""
     import torch
 
     sim = build_sim(material="VP", scatter="tiled")

@@ -90,7 +90,7 @@ def test_null_arguments_are_errors_not_crashes():
 
     L = _lib.lib()
     assert L.nfx_stream_sync(None) == -1
-    assert L.nfx_tip_force(None, None, 10, 1, 0.1, 1.0, 0.0, None) == -1
+    assert L.nfx_tip_force(None, None, None, 10, 1, 0.1, 1.0, 0.0, None) == -1
     assert L.nfx_run_steps(None, None, 0, 1, None, 0) == -1
 
 
@@ -200,7 +200,9 @@ def test_config_mesh_written_to_init_vtk_reads_back_through_tetracell(tmp_path):
     c = TetraCell(cfg["CELL"]["InitVTK"], radius=cfg["CELL"]["RadiusSIM"])
     assert np.array_equal(c.tetras, T)
     scale = cfg["CELL"]["RadiusSIM"] / abs(P[1, 0] - P[0, 0])
-    assert np.allclose(c.points, (P - P[0]) * scale, rtol=0, atol=1e-9 * cfg["CELL"]["RadiusSIM"])
+    # held at the FP32 values the device keeps: within FP32 rounding of the FP64 recentred mesh
+    assert np.allclose(c.points, (P - P[0]) * scale, rtol=0, atol=1e-7 * cfg["CELL"]["RadiusSIM"])
+    assert np.array_equal(c.points, c.points.astype(np.float32).astype(np.float64))
 
 
 def test_cell_groups_partition_consecutive_cells_under_the_budget():
