@@ -125,11 +125,15 @@ def sample_batch(n=512, seed0=1000, steps=10000, t_range=(2000, 60000), max_rela
     return specs, stats
 
 
+def _mesh_path(R, s, seed, cache):
+    return os.path.join(cache, f"cell_{R:.6f}_{s:.6f}_{seed}.npz") if cache else None
+
+
 def _mesh_job(args):
     from nofluidx3d_amd.mesh import generate_sphere_mesh
 
     R, s, seed, cache = args
-    path = os.path.join(cache, f"cell_{R:.6f}_{s:.6f}_{seed}.npz") if cache else None
+    path = _mesh_path(R, s, seed, cache)
     if path and os.path.exists(path):
         d = np.load(path)
         return d["P"], d["T"]
@@ -142,25 +146,36 @@ def _mesh_job(args):
 
 
 def make_meshes(specs, workers=None, cache="/tmp/nfx_cells"):
-    """Seeded meshes of all cells (a spawned process pool; no GPU use in the workers)."""
+    """Seeded meshes of all cells.  Cached meshes are read in this process; only the missing ones
+    are generated, by a spawned process pool (no GPU use in the workers), so a run whose meshes are
+    all cached starts no worker (and a profiler preloaded into this process reaches none)."""
     import multiprocessing as mp
 
     if cache:
         os.makedirs(cache, exist_ok=True)
     jobs = [(c.radius_um, c.spacing_um, c.seed, cache) for c in specs]
+    missing = [i for i, j in enumerate(jobs) if not (cache and os.path.exists(_mesh_path(*j)))]
+    out = [None] * len(jobs)
+    for i, j in enumerate(jobs):
+        if i not in set(missing):
+            out[i] = _mesh_job(j)
     workers = workers or min(16, os.cpu_count() or 1)
-    if workers <= 1 or len(jobs) < 4:
-        return [_mesh_job(j) for j in jobs]
+    if workers <= 1 or len(missing) < 4:
+        for i in missing:
+            out[i] = _mesh_job(jobs[i])
+        return out
     # close + join: the workers end on their own rather than by SIGTERM (a profiler preloaded into
     # them handles that signal and can stall the teardown)
     pool = mp.get_context("spawn").Pool(workers)
     try:
-        out = pool.map(_mesh_job, jobs, chunksize=4)
+        made = pool.map(_mesh_job, [jobs[i] for i in missing], chunksize=4)
         pool.close()
         pool.join()
-        return out
     finally:
         pool.terminate()
+    for i, m in zip(missing, made):
+        out[i] = m
+    return out
 
 
 def sim_points(P_um, radius_sim):
