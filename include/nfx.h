@@ -298,6 +298,12 @@ int nfx_volume_integral_force(nfx_ctx* ctx, int32_t tip_shape, const float* poin
  * (3 x NT SoA) by the 3-point edge-midpoint rule, y component, float-atomic scatter */
 int nfx_adhesive_plane_surface_force(nfx_ctx* ctx, float* force, const float* points, const uint32_t* triangles,
                                      int64_t N, int64_t NT, double wall_pos, double adhesion, double distance);
+/* InteractionHardAdhesivePlaneSurfaceIntegral.cl (wrapper Interactions.py:778-817, set up by
+ * SimulationLegacy.py:657-692): area density exp(-force_const d) - adhesion exp(-d), d = y - wall_pos,
+ * by the same 3-point rule over the surface triangles, y component, float-atomic scatter */
+int nfx_hard_adhesive_plane_surface_force(nfx_ctx* ctx, float* force, const float* points, const uint32_t* triangles,
+                                          int64_t N, int64_t NT, double wall_pos, double adhesion,
+                                          double force_const);
 
 /* ---- fused stepping: Simulation.timeStep (Simulation.py:232-237) for steps
  *      [step0, step0 + nsteps): force pass, then one node pass that applies the substrate and

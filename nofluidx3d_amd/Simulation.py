@@ -37,8 +37,9 @@ from nofluidx3d_amd.interactions import (FiniteStrainViscoplastic, LegacyElastic
                                         LinearElasticDeviatoric, LinearViscoelastic, Plane, PoroNeoHookean,
                                         SecondOrderNeoHookean, Sphere, Substrate, Tetra, VelocityVerlet)
 from nofluidx3d_amd.interactions.LegacyContacts import (AdhesivePlane, AdhesivePlaneSurfaceIntegral, CalcCOM,
-                                                        COMForce, OverDamped, PointZeroForce, RoundTip,
-                                                        SphereIntegral, TipIntegral, VelocityVerlet2)
+                                                        COMForce, HardAdhesivePlaneSurfaceIntegral, OverDamped,
+                                                        PointZeroForce, RoundTip, SphereIntegral, TipIntegral,
+                                                        VelocityVerlet2)
 from nofluidx3d_amd.openCL import getCommandQueue, initializeOpenCLObjects
 from nofluidx3d_amd.TetraCell import TetraCell
 from nofluidx3d_amd.util.Bridge import Bridge
@@ -389,6 +390,19 @@ class Simulation:
         self.register(AdhesivePlaneSurfaceIntegral(lambda t: wall, self._adhesion_const(),
                                                    self.parameters["AdhesivePlaneDistance"],
                                                    self._surface_triangles()))
+
+    def setInteractionHardAdhesivePlaneNonDimensional(self, isSpherical=True):
+        """SimulationLegacy.py:657-692: the wall as for the adhesive planes; the adhesion constant from
+        the non-dimensional CELL-independent ``AdhesionConstant`` scaled by E R / (p0 L0) (and the
+        legacy ReynoldScaling / YoungsScaling, 1 when absent); k = ``PotentialForceConst``."""
+        wall = self._adhesion_wall(isSpherical)
+        c = self.parameters["CELL"]
+        adhesion = (self.parameters["AdhesionConstant"] * c["YoungsModulusSI"] * c["RadiusSI"] / self.p0 / self.L0 /
+                    self.parameters.get("ReynoldScaling", 1.0) / c.get("YoungsScaling", 1.0))
+        inter = HardAdhesivePlaneSurfaceIntegral(lambda t: wall, adhesion, self.parameters["PotentialForceConst"],
+                                                 self._surface_triangles())
+        self.register(inter)
+        return inter
 
     def setInteractionCOMForce(self):
         calc = CalcCOM()

@@ -183,6 +183,8 @@ SIGNATURES = {
     "nfx_volume_integral_force": (ctypes.c_int, [c_vp, c_i32, c_vp, c_vp, c_i64, c_i64, c_f64, c_f64, c_f64, c_f64,
                                                   c_f64, c_f64, c_vp, c_vp]),
     "nfx_adhesive_plane_surface_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_f64, c_f64, c_f64]),
+    "nfx_hard_adhesive_plane_surface_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_f64, c_f64,
+                                                              c_f64]),
     "nfx_legacy_force": (ctypes.c_int, [c_vp, ctypes.POINTER(Elastic), c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp,
                                          c_vp, c_vp]),
     "nfx_batch_geometry": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i32, c_i64, c_i64, c_vp, c_vp, c_vp]),

@@ -9,8 +9,9 @@
 //   k_volume_quad      InteractionSphereIntegral.cl, InteractionTipIntegral.cl
 //                      exp(-k dis) n body force, Bower's 4-point rule per tetra
 //   k_lve              InteractionLinearViscoelastic.cl   small-strain Prony-series law (node-0 base)
-//   k_surface_adhesion InteractionAdhesivePlaneSurfaceIntegral.cl
-//                      Lennard-Jones area density, 3-point edge-midpoint rule per surface triangle
+//   k_surface_adhesion InteractionAdhesivePlaneSurfaceIntegral.cl (Lennard-Jones area density) and
+//                      InteractionHardAdhesivePlaneSurfaceIntegral.cl (exp(-k d) - A exp(-d)),
+//                      3-point edge-midpoint rule per surface triangle
 // Scatters into the force array use float atomics, as the reference's atomicAdd; the volume
 // integrals can also write per-corner records for the deterministic CSR gather.
 #include <hip/hip_runtime.h>
@@ -209,34 +210,43 @@ __global__ __launch_bounds__(kB) void k_volume_quad(float* f, float* corners, co
     }
 }
 
+// HARD = false: Lennard-Jones density 24 eps/d (2 (sigma/d)^12 - (sigma/d)^6) with (eps, sigma) =
+// (a, b) (InteractionAdhesivePlaneSurfaceIntegral.cl:1-4); HARD = true: exp(-k d) - A exp(-d) with
+// (A, k) = (a, b) (InteractionHardAdhesivePlaneSurfaceIntegral.cl:1-4).  d = y - wall at each point.
+template <bool HARD>
 __global__ __launch_bounds__(kB) void k_surface_adhesion(float* f, const float* __restrict__ x,
                                                          const uint32_t* __restrict__ tri, int64_t N, int64_t NT,
-                                                         float wall, float eps, float sigma) {
+                                                         float wall, float a, float b) {
     const int64_t t = (int64_t)blockIdx.x * kB + threadIdx.x;
     if (t >= NT) return;
     uint32_t id[3];
     V3 p[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        id[a] = tri[a * NT + t];
-        p[a] = load3(x, N, id[a]);
+    for (int c = 0; c < 3; ++c) {
+        id[c] = tri[c * NT + t];
+        p[c] = load3(x, N, id[c]);
     }
-    const V3 c = cross(p[0] - p[2], p[1] - p[2]);
-    const float area = 0.5f * sqrtf(dot(c, c));
+    const V3 cr = cross(p[0] - p[2], p[1] - p[2]);
+    const float area = 0.5f * sqrtf(dot(cr, cr));
     const float xi[3][2] = {{0.5f, 0.0f}, {0.0f, 0.5f}, {0.5f, 0.5f}};
     float F[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int qi = 0; qi < 3; ++qi) {
         const float Nq[3] = {xi[qi][0], xi[qi][1], 1.0f - xi[qi][0] - xi[qi][1]};
         const float d = Nq[0] * p[0].y + Nq[1] * p[1].y + Nq[2] * p[2].y - wall;
-        const float s = sigma / d;
-        const float s6 = s * s * s * s * s * s;
-        const float fq = 24.0f * eps / d * (2.0f * s6 * s6 - s6);
+        float fq;
+        if constexpr (HARD) {
+            fq = expf(-b * d) - a * expf(-d);
+        } else {
+            const float s = b / d;
+            const float s6 = s * s * s * s * s * s;
+            fq = 24.0f * a / d * (2.0f * s6 * s6 - s6);
+        }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) F[a] += fq * Nq[a];
+        for (int c = 0; c < 3; ++c) F[c] += fq * Nq[c];
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&f[N + id[a]], area * (1.0f / 3.0f) * F[a]);
+    for (int c = 0; c < 3; ++c) unsafeAtomicAdd(&f[N + id[c]], area * (1.0f / 3.0f) * F[c]);
 }
 
 // InteractionLinearViscoelastic.cl: small-strain generalized Maxwell (Prony) law, node-0 base
@@ -443,8 +453,20 @@ int nfx_adhesive_plane_surface_force(nfx_ctx* c, float* force, const float* poin
     if (int rc = check_n(N, "node")) return rc;
     if (int rc = check_n(NT, "triangle")) return rc;
     const double sigma = std::pow(0.5, 1.0 / 6.0) * distance;
-    k_surface_adhesion<<<nb_of(NT), kB, 0, c->stream>>>(force, points, triangles, N, NT, (float)wall_pos,
-                                                       (float)adhesion, (float)sigma);
+    k_surface_adhesion<false><<<nb_of(NT), kB, 0, c->stream>>>(force, points, triangles, N, NT, (float)wall_pos,
+                                                              (float)adhesion, (float)sigma);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_hard_adhesive_plane_surface_force(nfx_ctx* c, float* force, const float* points, const uint32_t* triangles,
+                                          int64_t N, int64_t NT, double wall_pos, double adhesion,
+                                          double force_const) {
+    NFX_REQUIRE(c && force && points && triangles, "NULL argument");
+    if (int rc = check_n(N, "node")) return rc;
+    if (int rc = check_n(NT, "triangle")) return rc;
+    k_surface_adhesion<true><<<nb_of(NT), kB, 0, c->stream>>>(force, points, triangles, N, NT, (float)wall_pos,
+                                                             (float)adhesion, (float)force_const);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }

@@ -1,6 +1,6 @@
 """Legacy contacts and integrators behind the current plugin protocol (reference
-``Interactions.py:35-65, 440-481, 601-691, 692-777, 818-860``; set up by
-``SimulationLegacy.py:139-330, 550-617, 884-940``).  Each plugin launches one libnfx kernel that
+``Interactions.py:35-65, 440-481, 601-691, 692-817, 818-860``; set up by
+``SimulationLegacy.py:139-330, 550-617, 657-692, 884-940``).  Each plugin launches one libnfx kernel that
 restates the reference's OpenCL kernel as written (see ``include/nfx.h``):
 
 - ``RoundTip``: spherical tip with a cylindrical shaft, prefactor 0.1 (``RoundTip.cl``);
@@ -8,6 +8,8 @@ restates the reference's OpenCL kernel as written (see ``include/nfx.h``):
   4-point rule (``InteractionSphereIntegral.cl``, ``InteractionTipIntegral.cl``);
 - ``AdhesivePlane`` / ``AdhesivePlaneSurfaceIntegral``: Lennard-Jones wall on the surface points or
   integrated over the surface triangles (``InteractionAdhesivePlane*.cl``);
+  ``HardAdhesivePlaneSurfaceIntegral``: the exp(-k d) - A exp(-d) wall integrated the same way
+  (``InteractionHardAdhesivePlaneSurfaceIntegral.cl``);
 - ``CalcCOM`` + ``COMForce`` and ``PointZeroForce``: springs that hold the cell in place;
 - ``VelocityVerlet2`` (mass-weighted, ``velocityVerlet2.cl``) and ``OverDamped`` (``OverDamped.cl``):
   integrators, registered last.
@@ -112,6 +114,22 @@ class AdhesivePlaneSurfaceIntegral(AdhesivePlane):
         _lib.call("nfx_adhesive_plane_surface_force", openCL.handle(), s.force.ptr, s.points.ptr,
                   self.trianglesB.data_ptr(), s.numPoints, len(self.triangles), float(self.wall),
                   float(self.adhesionConst), float(self.distance))
+
+
+class HardAdhesivePlaneSurfaceIntegral(AdhesivePlaneSurfaceIntegral):
+    """Reference ``InteractionHardAdhesivePlaneSurfaceIntegral`` (``Interactions.py:778-817``, kernel
+    ``InteractionHardAdhesivePlaneSurfaceIntegral.cl``): the area density exp(-k d) - A exp(-d),
+    d = y - wall (k = ``forceConst``, A = ``adhesionConst``), integrated over the surface triangles."""
+
+    def __init__(self, wallFunc, adhesionConst, forceConst, triangles):
+        super().__init__(wallFunc, adhesionConst, 0.0, triangles)
+        self.forceConst = forceConst
+
+    def enqueue(self):
+        s = self.simulation
+        _lib.call("nfx_hard_adhesive_plane_surface_force", openCL.handle(), s.force.ptr, s.points.ptr,
+                  self.trianglesB.data_ptr(), s.numPoints, len(self.triangles), float(self.wall),
+                  float(self.adhesionConst), float(self.forceConst))
 
 
 class CalcCOM(Interaction):

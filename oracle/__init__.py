@@ -133,6 +133,7 @@ def lib():
         L.orc_velocity_verlet2.argtypes = [i64, _dp, _dp, _dp, _dp, _dp, f64]
         L.orc_volume_quadrature.argtypes = [ctypes.c_int, i64, i64, _dp, _dp, _up, f64, f64, f64, f64, f64, f64]
         L.orc_adhesive_plane_surface.argtypes = [i64, i64, _dp, _dp, _up, f64, f64, f64]
+        L.orc_hard_adhesive_plane_surface.argtypes = [i64, i64, _dp, _dp, _up, f64, f64, f64]
         L.orc_substrate.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double]
         L.orc_plane.argtypes = [i64, ctypes.c_double, _dp, _dp, ctypes.c_double]
         L.orc_velocity_verlet.argtypes = [i64, ctypes.c_int, _dp, _dp, _dp, _dp]
@@ -276,6 +277,11 @@ def volume_quadrature(tip_shape, force, points, tetras, radius, half_angle, tip_
 def adhesive_plane_surface(force, points, triangles, wall, adhesion, distance):
     lib().orc_adhesive_plane_surface(points.size // 3, triangles.size // 3, _d(force), _d(points), _u(triangles),
                                      wall, adhesion, distance)
+
+
+def hard_adhesive_plane_surface(force, points, triangles, wall, adhesion, force_const):
+    lib().orc_hard_adhesive_plane_surface(points.size // 3, triangles.size // 3, _d(force), _d(points), _u(triangles),
+                                          wall, adhesion, force_const)
 
 
 def linear_viscoelastic(P: LVEParams, points, tetras, ev, force, devold, transient, vonMises=None, pressure=None):

@@ -15,11 +15,13 @@
  *   orc_legacy_force     kernels/InteractionLinearElastic.cl, InteractionLinearElasticDeviatoric.cl,
  *                        InteractionSecondOrderNeoHookean.cl, InteractionPoroNeoHookean.cl (node-0 base)
  *   orc_round_tip, orc_adhesive_plane, orc_point_zero_force, orc_center_of_mass, orc_com_force,
- *   orc_over_damped, orc_velocity_verlet2, orc_volume_quadrature, orc_adhesive_plane_surface
+ *   orc_over_damped, orc_velocity_verlet2, orc_volume_quadrature, orc_adhesive_plane_surface,
+ *   orc_hard_adhesive_plane_surface
  *                        the legacy contacts and integrators (RoundTip.cl, InteractionAdhesivePlane.cl,
  *                        PointZeroForce.cl, CalcCOM.cl, COMForce.cl, OverDamped.cl, velocityVerlet2.cl,
  *                        InteractionSphereIntegral.cl, InteractionTipIntegral.cl,
- *                        InteractionAdhesivePlaneSurfaceIntegral.cl)
+ *                        InteractionAdhesivePlaneSurfaceIntegral.cl,
+ *                        InteractionHardAdhesivePlaneSurfaceIntegral.cl)
  *   orc_sphere           kernels/Interactions/Sphere.cl:1-17
  *   orc_substrate        kernels/Interactions/Substrate.cl:1-7
  *   orc_plane            kernels/Interactions/Plane.cl:1-7
@@ -753,11 +755,13 @@ void orc_volume_quadrature(int tip_shape, int64_t N, int64_t T, double* force, c
     }
 }
 
-/* InteractionAdhesivePlaneSurfaceIntegral.cl:1-56: the Lennard-Jones area density integrated over
- * each surface triangle with Bower's 3-point (edge mid-point) rule, y component */
-void orc_adhesive_plane_surface(int64_t N, int64_t NT, double* force, const double* points, const uint32_t* tri,
-                                double wallPos, double adhesion, double distance) {
-    const double sigma = pow(0.5, 1.0 / 6.0) * distance;
+/* Bower's 3-point (edge mid-point) rule of an area density over each surface triangle, y component:
+ * node a of triangle t gets area/3 * sum_q f(y_q - wall) N_a(xi_q).  law 0: the Lennard-Jones density
+ * 24 A/d (2 (s/d)^12 - (s/d)^6), s = 2^-1/6 p (InteractionAdhesivePlaneSurfaceIntegral.cl:1-56);
+ * law 1: exp(-p d) - A exp(-d) (InteractionHardAdhesivePlaneSurfaceIntegral.cl:1-55). */
+static void surface_density_rule(int law, int64_t N, int64_t NT, double* force, const double* points,
+                                 const uint32_t* tri, double wallPos, double A, double p_) {
+    const double sigma = pow(0.5, 1.0 / 6.0) * p_;
     const double xi[3][2] = {{0.5, 0.0}, {0.0, 0.5}, {0.5, 0.5}};
     for (int64_t t = 0; t < NT; t++) {
         uint32_t id[3];
@@ -772,11 +776,22 @@ void orc_adhesive_plane_surface(int64_t N, int64_t NT, double* force, const doub
             const double Nq[3] = {xi[q][0], xi[q][1], 1.0 - xi[q][0] - xi[q][1]};
             const double y = Nq[0] * p[0].y + Nq[1] * p[1].y + Nq[2] * p[2].y;
             const double d = y - wallPos;
-            const double fq = 24.0 * adhesion / d * (2 * pow(sigma / d, 12) - pow(sigma / d, 6));
+            const double fq = law == 0 ? 24.0 * A / d * (2 * pow(sigma / d, 12) - pow(sigma / d, 6))
+                                       : exp(-p_ * d) - A * exp(-d);
             for (int a = 0; a < 3; a++) F[a] += fq * Nq[a];
         }
         for (int a = 0; a < 3; a++) force[N + id[a]] += area / 3.0 * F[a];
     }
+}
+
+void orc_adhesive_plane_surface(int64_t N, int64_t NT, double* force, const double* points, const uint32_t* tri,
+                                double wallPos, double adhesion, double distance) {
+    surface_density_rule(0, N, NT, force, points, tri, wallPos, adhesion, distance);
+}
+
+void orc_hard_adhesive_plane_surface(int64_t N, int64_t NT, double* force, const double* points, const uint32_t* tri,
+                                     double wallPos, double adhesion, double forceConst) {
+    surface_density_rule(1, N, NT, force, points, tri, wallPos, adhesion, forceConst);
 }
 
 /* Sphere.cl:1-17.  OpenCL normalize() of a zero vector returns the zero vector. */

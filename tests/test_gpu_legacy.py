@@ -191,6 +191,30 @@ def test_surface_adhesion(ctx, mesh):
     rel_close(host(fd), fr, 1e-4)
 
 
+def test_hard_surface_adhesion(ctx, mesh):
+    """InteractionHardAdhesivePlaneSurfaceIntegral.cl: exp(-k d) - A exp(-d) by the 3-point rule, against
+    the oracle; the wall 0.05 above the lowest node, so that the surface near the wall (nearer than
+    d* = ln(A) / (1 - k) = 0.255) is repelled and the rest attracted."""
+    import torch
+
+    from nofluidx3d_amd.interactions.LegacyContacts import surface_triangles
+
+    P, T, x = mesh
+    N = len(P)
+    tri = surface_triangles(T)
+    wall = x[N:2 * N].min() + 0.05
+    A, k = 0.6, 3.0
+    fr = np.zeros(3 * N)
+    O.hard_adhesive_plane_surface(fr, x, O.soa(tri, np.uint32), wall, A, k)
+    fd = torch.zeros(3 * N, dtype=torch.float32, device="cuda:0")
+    xd, trid = dev(x, np.float32), dev(O.soa(tri, np.uint32))
+    call("nfx_hard_adhesive_plane_surface_force", ctx, fd.data_ptr(), xd.data_ptr(), trid.data_ptr(), N, len(tri),
+         wall, A, k)
+    fy = fr.reshape(3, -1)[1]
+    assert fy.max() > 0 and fy.min() < 0  # repulsion near the wall, adhesion farther up
+    rel_close(host(fd), fr, 1e-4)
+
+
 def test_linear_viscoelastic_steps(ctx, mesh):
     """nfx_linear_viscoelastic_force over 12 steps of a changing deformation (state carried from
     step to step) against the oracle: forces, von Mises stress, pressure and both state arrays."""
@@ -234,3 +258,31 @@ def test_linear_viscoelastic_steps(ctx, mesh):
         rel_close(host(pr_d), pr_r, 1e-4 * max(np.abs(vm_r).max() / np.abs(pr_r).max(), 1.0))
     rel_close(host(do_d), do_r, 1e-4)
     rel_close(host(tr_d), tr_r, 1e-4)
+
+
+def test_hard_adhesive_setter_through_simulation(ctx):
+    """setInteractionHardAdhesivePlaneNonDimensional (SimulationLegacy.py:657-692) on a Simulation: the
+    wall 0.2 below the cell, A = AdhesionConstant E R / (p0 L0), k = PotentialForceConst; one enqueue
+    against the oracle on the cell's surface triangles."""
+    from nofluidx3d_amd import configs
+    from nofluidx3d_amd.interactions.LegacyContacts import surface_triangles
+    from nofluidx3d_amd.Simulation import Simulation
+    from nofluidx3d_amd.util import vtk
+
+    cfg = configs.small_cell(outdir="/tmp/nfx_test_hard/", steps=10, frames=1)
+    cfg["MESH"] = {"radius_um": 3.0, "spacing_um": 0.6, "seed": 5}
+    cfg["CELL"]["RadiusSI"], cfg["CELL"]["RadiusSIM"] = 3e-6, 6.0
+    cfg.update({"AdhesivePlaneDistance": -0.15, "AdhesionConstant": 30.0, "PotentialForceConst": 2.5})
+    P, T = configs.make_mesh(cfg)
+    sim = Simulation(cfg, mesh=vtk.from_tetra(P, T), verbose=False)
+    inter = sim.setInteractionHardAdhesivePlaneNonDimensional()
+    f = sim._record_force(inter)
+    wall = -6.0 + 0.15
+    A = 30.0 * cfg["CELL"]["YoungsModulusSI"] * 3e-6 / sim.p0 / sim.L0
+    assert inter.wall == wall and inter.adhesionConst == pytest.approx(A, rel=1e-14) and inter.forceConst == 2.5
+    fr = np.zeros(3 * sim.numPoints)
+    x = O.soa(sim.cell.points)
+    O.hard_adhesive_plane_surface(fr, x, O.soa(surface_triangles(sim.cell.tetras), np.uint32), wall, A, 2.5)
+    fr = fr.reshape(3, -1).T
+    assert np.abs(fr[:, 1]).max() > 0 and np.all(f[:, [0, 2]] == 0)
+    assert np.max(np.abs(f - fr)) < 1e-4 * np.abs(fr).max()

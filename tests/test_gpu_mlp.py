@@ -114,10 +114,15 @@ def test_split_k_matches_one_pass(ctx, M, N, K, S):
     assert np.max(np.abs(got[0] - got[1])) < 1e-6
 
 
-def small_net():
+SMALL = ((6, 128, 128, 128), 256)
+# the surrogate's network and training batch: 6 -> 512 x4 (GELU) -> 256, batch 4096
+FULL = ((6, 512, 512, 512, 512, 256), 4096)
+
+
+def make_net(sizes):
     from nofluidx3d_amd.surrogate.mlp import MLP
 
-    return MLP(sizes=(6, 128, 128, 128), seed=3)
+    return MLP(sizes=sizes, seed=3)
 
 
 def oracle_params(net):
@@ -130,14 +135,14 @@ def rel(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-@pytest.mark.parametrize("weighted", [False, True])
-def test_forward_and_gradients_match_oracle(ctx, weighted):
+@pytest.mark.parametrize("shape,weighted", [(SMALL, False), (SMALL, True), (FULL, True)])
+def test_forward_and_gradients_match_oracle(ctx, shape, weighted):
     import torch
 
-    net = small_net()
-    B = 256
+    sizes, B = shape
+    net = make_net(sizes)
     X = RNG.normal(size=(B, 6))
-    T = RNG.normal(scale=0.3, size=(B, 128))
+    T = RNG.normal(scale=0.3, size=(B, sizes[-1]))
     w = RNG.uniform(0.1, 4.0, size=B).astype(np.float32) if weighted else None
     net.set_input(X, train=True)
     Y = net.forward_dev(B, train=True)
@@ -151,19 +156,20 @@ def test_forward_and_gradients_match_oracle(ctx, weighted):
     assert rel(Y.cpu().numpy(), Yo) < 3e-2
     loss, gWo, gbo = OM.loss_and_grads(Ws, bs, Xr, T, None if w is None else w.astype(np.float64))
     assert abs(net.loss_dev.item() - loss) < 3e-2 * loss
-    for l in range(3):
+    for l in range(len(sizes) - 1):
         assert rel(gW[l].cpu().numpy()[:, :Ws[l].shape[1]], gWo[l]) < 3e-2, l
         assert rel(gb[l].cpu().numpy(), gbo[l]) < 3e-2, l
         assert np.all(gW[l].cpu().numpy()[:, Ws[l].shape[1]:] == 0)  # padded input columns
 
 
-def test_fused_adam_matches_oracle_adam_on_kernel_gradients(ctx):
+@pytest.mark.parametrize("shape", [SMALL, FULL])
+def test_fused_adam_matches_oracle_adam_on_kernel_gradients(ctx, shape):
     import torch
 
-    a, b = small_net(), small_net()
-    B = 256
+    sizes, B = shape
+    a, b = make_net(sizes), make_net(sizes)
     X = RNG.normal(size=(B, 6))
-    T = RNG.normal(scale=0.3, size=(B, 128))
+    T = RNG.normal(scale=0.3, size=(B, sizes[-1]))
     Td = torch.tensor(T, dtype=torch.float32, device="cuda:0")
     a.set_input(X, train=True)
     a.forward_dev(B, train=True)

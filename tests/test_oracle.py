@@ -670,6 +670,39 @@ def test_adhesive_plane_surface_integral():
     assert g.reshape(3, -1)[1].sum() == pytest.approx(area * lj(ys - wall).mean(), rel=2e-3)
 
 
+def test_hard_adhesive_plane_surface_integral():
+    """exp(-k d) - A exp(-d) by the 3-point rule: a triangle parallel to the wall gets area/3 of the
+    density per node; the total force is -dE/dh of the energy area * U(d), U = exp(-k d)/k - A exp(-d),
+    written independently; the force vanishes at d* = ln(A)/(1 - k); a tilted triangle integrates the
+    density (Monte Carlo)."""
+    A, k, wall = 0.6, 3.0, -1.0
+    dens = lambda d: np.exp(-k * d) - A * np.exp(-d)  # noqa: E731
+    U = lambda d: np.exp(-k * d) / k - A * np.exp(-d)  # noqa: E731
+    tri = O.soa(np.array([[0, 1, 2]]), np.uint32)
+    base = np.array([[0.0, 0.0, 0.0], [1.3, 0.0, 0.0], [0.0, 0.0, 0.7]])
+    area = 0.5 * 1.3 * 0.7
+
+    def fy(P):
+        f = np.zeros(9)
+        O.hard_adhesive_plane_surface(f, O.soa(P), tri, wall, A, k)
+        return f.reshape(3, -1)[1]
+
+    for h in (0.2, 0.9, 2.5):
+        P = base + [0, wall + h, 0]
+        assert np.allclose(fy(P), area / 3 * dens(h), rtol=1e-12)
+        e = 1e-5
+        dE = (area * U(h + e) - area * U(h - e)) / (2 * e)
+        assert fy(P).sum() == pytest.approx(-dE, rel=1e-8)
+    dstar = np.log(A) / (1 - k)
+    assert np.abs(fy(base + [0, wall + dstar, 0])).max() < 1e-14
+    Q = np.array([[0.0, 0.6, 0.0], [1.0, 0.9, 0.0], [0.0, 0.7, 1.0]])
+    g = fy(Q)
+    rng = np.random.default_rng(6)
+    w = rng.dirichlet(np.ones(3), size=400000)
+    area_q = 0.5 * np.linalg.norm(np.cross(Q[0] - Q[2], Q[1] - Q[2]))
+    assert g.sum() == pytest.approx(area_q * dens(w @ Q[:, 1] - wall).mean(), rel=3e-3)
+
+
 # --------------------------------------------------------------------------------------------
 # contacts, integrator, tip force
 # --------------------------------------------------------------------------------------------
