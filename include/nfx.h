@@ -269,6 +269,17 @@ int nfx_velocity_verlet(nfx_ctx* ctx, float* points, float* vel, float* force, f
 int nfx_tip_force(nfx_ctx* ctx, const float* points, const float* ref, int64_t N, int32_t indenter, double k,
                   double radius, double pos, double* out2);
 
+/* nfx_tip_force without a context and without any allocation: the same FP64 reduction (bitwise the
+ * same result) with its scratch -- 2048 per-block (sum, max) pairs and a completion counter, 32.25
+ * KiB -- in a fixed array of the library that each GPU holds for the life of the process.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Enqueued on `stream`
+ * (a hipStream_t, NULL for the null stream) on the current device.  N up to 2^31 - 1 (the grid is
+ * capped at 2048 blocks, so every N in range fits the scratch).  All calls on one GPU share that
+ * scratch: calls that overlap in time on one GPU race. */
+int nfx_tip_force_noalloc(const float* points, const float* ref, int64_t N, int32_t indenter, double k, double radius,
+                          double pos, double* out2, void* stream);
+
 /* ---- legacy contacts and integrators (the kernels SimulationLegacy.py:139-330, 550-617 and
  *      884-940 use), restated as written; force/points/vel/fold are 3N SoA FP32 ------------------ */
 /* Interactions/RoundTip.cl: sphere below its centre, cylinder above; force 0.1 max(|r| - R, 0) r/|r| */

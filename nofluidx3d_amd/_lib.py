@@ -196,6 +196,7 @@ SIGNATURES = {
     "nfx_plane_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_f64, c_f64]),
     "nfx_velocity_verlet": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i32]),
     "nfx_tip_force": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_i32, c_f64, c_f64, c_f64, c_vp]),
+    "nfx_tip_force_noalloc": (ctypes.c_int, [c_vp, c_vp, c_i64, c_i32, c_f64, c_f64, c_f64, c_vp, c_vp]),
     "nfx_run_steps": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i64, c_vp, c_i32]),
     "nfx_node_update": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_vp]),
     "nfx_time_passes": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan), c_i64, c_i32, ctypes.POINTER(c_f64),

@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "nfx_common.h"
@@ -821,6 +822,16 @@ __global__ __launch_bounds__(kBlock) void k_tip(const float* __restrict__ x, con
 }
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
+
+// ---- scratch of the allocation-free tip-force reduction (nfx_tip_force_noalloc) ----------------
+// The reduction's per-block FP64 (sum, max) partials, kMaxReduceBlocks pairs, then its completion
+// counter in the first word of the 32-double tail: 32.25 KiB.  Each GPU the library runs on holds
+// its own copy of this array until the process ends.  In host code the name is not a device
+// address: host code uses only the per-device address hipGetSymbolAddress returns (noalloc_scratch).
+constexpr int kNoAllocPartials = 2 * kMaxReduceBlocks;
+static __device__ __align__(256) double scratch[kNoAllocPartials + 32];
+
+__global__ void k_clear_u32(unsigned int* p) { *p = 0u; }
 
 // STREAM-style copy, the measured HBM ceiling: one 16-byte element per thread, nontemporal loads
 // and stores (the fastest of the copy forms measured on MI355X: grid-stride loops and unrolled
@@ -1663,6 +1674,53 @@ int nfx_tip_force(nfx_ctx* c, const float* points, const float* ref, int64_t N, 
     else
         k_tip<false><<<nb, kBlock, 0, c->stream>>>(points, nullptr, N, ind, k, radius, pos, c->partials, c->counter,
                                                    out2, out2 + 1);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+namespace nfx {
+constexpr int kMaxNoAllocDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_addr[kMaxNoAllocDevices] = {};
+
+// the current device's copy of ``scratch``, looked up once per device
+static int noalloc_scratch(double** out) {
+    int dev = -1;
+    NFX_HIP(hipGetDevice(&dev));
+    NFX_REQUIRE(dev >= 0 && dev < kMaxNoAllocDevices, "device %d: the allocation-free entry points serve devices "
+                "0..%d", dev, kMaxNoAllocDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* a = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !a) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return NFX_ERR_HIP;
+        }
+        g_scratch_addr[dev] = static_cast<double*>(a);
+    }
+    *out = g_scratch_addr[dev];
+    return NFX_OK;
+}
+}  // namespace nfx
+
+int nfx_tip_force_noalloc(const float* points, const float* ref, int64_t N, int32_t ind, double k, double radius,
+                          double pos, double* out2, void* stream) {
+    NFX_REQUIRE(points && out2, "NULL argument");
+    NFX_REQUIRE(N > 0 && N < (int64_t(1) << 31), "node count %lld out of range (1 .. 2^31 - 1)", (long long)N);
+    NFX_REQUIRE(ind == NFX_INDENTER_SPHERE || ind == NFX_INDENTER_PLANE, "unknown indenter %d", ind);
+    double* s = nullptr;
+    const int rc = nfx::noalloc_scratch(&s);
+    if (rc) return rc;
+    // the grid is capped at kMaxReduceBlocks, so its partials always fit the kNoAllocPartials slots
+    const unsigned int nb = std::min<unsigned int>(nblocks(N), kMaxReduceBlocks);
+    unsigned int* counter = reinterpret_cast<unsigned int*>(s + kNoAllocPartials);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    k_clear_u32<<<1, 1, 0, st>>>(counter);  // a kernel, not a memset: callers may capture this call
+    if (ref)
+        k_tip<true><<<nb, kBlock, 0, st>>>(points, ref, N, ind, k, radius, pos, s, counter, out2, out2 + 1);
+    else
+        k_tip<false><<<nb, kBlock, 0, st>>>(points, nullptr, N, ind, k, radius, pos, s, counter, out2, out2 + 1);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }
