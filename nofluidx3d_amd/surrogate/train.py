@@ -174,31 +174,77 @@ def load(path):
     return net, Normaliser(d["x_mean"], d["x_std"], float(d["y_scale"]))
 
 
-def main(argv=None):
+def _population(pool_size, mesh_seed0, steps, tile_k):
     from nofluidx3d_amd.autotune import group_tetra
     from nofluidx3d_amd.batch import CellGroups, make_meshes, sample_batch
 
+    pool, _ = sample_batch(pool_size, seed0=mesh_seed0, steps=steps)
+    return pool, CellGroups(pool, make_meshes(pool), max_tetra=group_tetra(), steps=steps, ncurve=256, tile_k=tile_k)
+
+
+def write_part(prefix, X, Y, meta):
+    """One slice of training data: inputs (float32), normalised curves (float16: 11 bits, far below
+    the per-cent errors the network is judged at) and what produced them."""
+    np.save(prefix + "_X.npy", X.astype(np.float32))
+    np.save(prefix + "_Y.npy", Y.astype(np.float16))
+    json.dump(meta, open(prefix + "_meta.json", "w"), indent=1)
+
+
+def read_parts(prefixes):
+    X = np.concatenate([np.load(p + "_X.npy") for p in prefixes])
+    Y = np.concatenate([np.load(p + "_Y.npy").astype(np.float32) for p in prefixes])
+    return X, Y
+
+
+def main(argv=None):
+    """``all`` (default): generate, train and evaluate in one run.  For data sets larger than one
+    run can generate, ``generate`` writes slices (``--first-chunk``/``--chunks`` of a pool) and
+    ``fit`` trains on slices and evaluates on a held-out slice."""
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", required=True)
+    ap.add_argument("mode", nargs="?", default="all", choices=["all", "generate", "fit"])
+    ap.add_argument("--out", required=True, help="output directory (all, fit) or file prefix (generate)")
     ap.add_argument("--chunks", type=int, default=130)
+    ap.add_argument("--first-chunk", type=int, default=0)
+    ap.add_argument("--seed0", type=int, default=100000, help="physics seeds: seed0 + chunk * pool + i")
+    ap.add_argument("--mesh-seed0", type=int, default=1000, help="meshes: the batched workload's cells from here")
     ap.add_argument("--heldout-chunks", type=int, default=6)
     ap.add_argument("--newmesh-chunks", type=int, default=4)
     ap.add_argument("--pool", type=int, default=512)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--iters", type=int, default=12000)
     ap.add_argument("--tile-k", type=int, default=256)
+    ap.add_argument("--train", nargs="*", default=[], help="fit: training slice prefixes")
+    ap.add_argument("--heldout", nargs="*", default=[], help="fit: held-out slice prefixes (new cells)")
     args = ap.parse_args(argv)
-    os.makedirs(args.out, exist_ok=True)
-    log = lambda *a, **k: print(*a, **k, file=sys.stderr)
+    log = lambda *a, **k: print(*a, **k, file=sys.stderr)  # noqa: E731
     t0 = time.time()
-    pool, _ = sample_batch(args.pool, seed0=1000, steps=args.steps)
-    bs = CellGroups(pool, make_meshes(pool), max_tetra=group_tetra(), steps=args.steps, ncurve=256, tile_k=args.tile_k)
-    Xt, Yt = generate(bs, pool, args.chunks, seed0=100000, log=log)
+    if args.mode == "generate":
+        pool, bs = _population(args.pool, args.mesh_seed0, args.steps, args.tile_k)
+        X, Y = generate(bs, pool, args.chunks, seed0=args.seed0 + args.first_chunk * len(pool), log=log)
+        write_part(args.out, X, Y, {"pool": args.pool, "mesh_seed0": args.mesh_seed0, "seed0": args.seed0,
+                                    "first_chunk": args.first_chunk, "chunks": args.chunks, "steps": args.steps,
+                                    "curves": int(len(X)), "seconds": time.time() - t0})
+        print(json.dumps({"curves": int(len(X)), "seconds": time.time() - t0}))
+        return
+    os.makedirs(args.out, exist_ok=True)
+    if args.mode == "fit":
+        Xt, Yt = read_parts(args.train)
+        Xn, Yn = read_parts(args.heldout)
+        split = len(Xn) // 2  # the first half reports progress during training, all of it is scored
+        t1 = time.time()
+        net, norm, hist = train(Xt, Yt, iters=args.iters, log=log, Xv=Xn[:split], Yv=Yn[:split])
+        save(os.path.join(args.out, "surrogate.npz"), net, norm)
+        rep = {"train_curves": int(len(Xt)), "train_slices": args.train, "heldout_slices": args.heldout,
+               "training_s": time.time() - t1, "iters": args.iters, "history": hist,
+               "heldout_cells_rel_l2": error_summary(curve_errors(net, norm, Xn, Yn))}
+        json.dump(rep, open(os.path.join(args.out, "report.json"), "w"), indent=1)
+        print(json.dumps({"train_curves": rep["train_curves"], "heldout_cells_rel_l2": rep["heldout_cells_rel_l2"]}))
+        return
+    pool, bs = _population(args.pool, args.mesh_seed0, args.steps, args.tile_k)
+    Xt, Yt = generate(bs, pool, args.chunks, seed0=args.seed0, log=log)
     Xv, Yv = generate(bs, pool, args.heldout_chunks, seed0=900000, log=log)
     del bs
-    pool2, _ = sample_batch(args.pool, seed0=5000, steps=args.steps)
-    bs2 = CellGroups(pool2, make_meshes(pool2), max_tetra=group_tetra(), steps=args.steps, ncurve=256,
-                     tile_k=args.tile_k)
+    pool2, bs2 = _population(args.pool, 5000, args.steps, args.tile_k)
     Xn, Yn = generate(bs2, pool2, args.newmesh_chunks, seed0=950000, log=log)
     del bs2
     tgen = time.time() - t0
