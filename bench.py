@@ -44,8 +44,14 @@ HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per tetra of one force pass, as the hot-path specification counts them
 SPEC_TET_BYTES = {"viscoplastic": 16 + 36 + 4 + 2 * 36 + 2 * 4, "elastic": 16 + 36 + 4}
 # what the tiled force pass reads and writes per tetra: corner records 8 (instead of the 16 B
-# connectivity), Dm^-1 36, and V0 4 (elastic) or the plastic deviation Q 36 + 36 and epsbar 4 + 4
-KERNEL_TET_BYTES = {"viscoplastic": 8 + 36 + 2 * 36 + 2 * 4, "elastic": 8 + 36 + 4}
+# connectivity), the plastic deviation Q 36 + 36 and epsbar 4 + 4 (viscoplastic), and, when it
+# streams the reference shape instead of forming it from the reference positions, Dm^-1 36 and V0 4
+# (elastic; the viscoplastic law takes V0 from det Dm^-1)
+KERNEL_TET_BYTES = {"viscoplastic": 8 + 2 * 36 + 2 * 4, "elastic": 8}
+STREAMED_SHAPE_BYTES = {"viscoplastic": 36, "elastic": 36 + 4}
+# per node: u read 12, partial written 12, and the reference position X0 12 when the shape is formed
+# from it (each node counted once; tile halos re-read some)
+KERNEL_NODE_BYTES = 24
 
 
 def log(*a):
@@ -69,15 +75,16 @@ def cached_mesh(cfg):
     return P, T
 
 
-def make_sim(cfg, P, T, scatter, material, tile_k=512):
+def make_sim(cfg, P, T, scatter, material, tile_k=512, shape_from_ref=True, compact_state=False):
     from nofluidx3d_amd.Simulation import Simulation
     from nofluidx3d_amd.util import vtk
 
-    sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False, tile_k=tile_k)
+    sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False, tile_k=tile_k,
+                     shape_from_ref=shape_from_ref)
     sim.setInteractionSubstrate()
     sim.setInteractionSphere()
     if material == "viscoplastic":
-        sim.setInteractionFiniteStrainViscoplastic()
+        sim.setInteractionFiniteStrainViscoplastic(compactState=compact_state)
     else:
         sim.setInteractionTetra()
     sim.setInteractionVelocityVerlet()
@@ -215,8 +222,11 @@ def bench_batched(steps=10000, ncells=512, form="auto", log=log):
     # kept), its plastic state is the symmetric deviation Gd (6 floats) instead of 9, and its node
     # state is (u, w) without the previous force
     step_bytes = SPEC_TET_BYTES["viscoplastic"] * b.T_real + 108 * b.N
-    kernel_step_bytes = (8 + 36 + 2 * (24 if b.sym_state else 36) + (8 if b.eps is not None else 0)) * b.T_real + \
-        (2 * 24 + 4 + 4 + 12) * b.N  # per node: u and w read and written, X0.y, node_cell; x read by the tiles
+    geo = b.shape_from_ref
+    kernel_step_bytes = (8 + (0 if geo else 36) + 2 * (24 if b.sym_state else 36) + (8 if b.plastic_strain else 0)) * \
+        b.T_real + (2 * 24 + 4 + 4 + 12 + (12 if geo else 0)) * b.N
+    # per node: u and w read and written, X0.y, node_cell; u (and X0 when the shape is formed from it)
+    # read by the tiles
     return {
         "workload": f"{ncells} cells (seeds 1000+i), {b.T_real} tetra, {b.N} nodes, NH + Maxwell viscoplastic, "
                     f"{steps} steps each, 256-point curve per cell",
@@ -337,6 +347,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 512-cell measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
     ap.add_argument("--tile-k", type=int, default=0, choices=[0, 256, 512], help="0: autotune")
+    ap.add_argument("--compact-state", action="store_true",
+                    help="viscoplastic: the fused tiled step keeps the symmetric plastic deviation only "
+                         "(same forces; no Fp)")
+    ap.add_argument("--streamed-shape", action="store_true",
+                    help="the tiled force pass reads the precomputed Dm^-1 and V0 instead of forming them "
+                         "from the reference positions")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_large_cell.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (HBM-counter calibration under rocprofv3)")
@@ -378,7 +394,8 @@ def main():
         material = args.material or "elastic"
     P, T = cached_mesh(cfg)
     N, nT = len(P), len(T)
-    sim = make_sim(cfg, P, T, args.scatter, material, args.tile_k or 512)
+    compact = args.compact_state and material == "viscoplastic" and args.scatter == "tiled"
+    sim = make_sim(cfg, P, T, args.scatter, material, args.tile_k or 512, not args.streamed_shape, compact)
     assert sim.plan() is not None
     tuned = None
     if args.scatter == "tiled" and args.tile_k == 0:
@@ -390,17 +407,22 @@ def main():
     ms_force, ms_node = pass_times(sim, min(args.steps, 100))
     vp = material == "viscoplastic"
     force_bytes = SPEC_TET_BYTES[material] * nT + 24 * N
-    kernel_bytes = KERNEL_TET_BYTES[material] * nT + 24 * N
+    geo = bool(sim.plan().shape_from_ref) and args.scatter == "tiled"
+    tet_bytes = KERNEL_TET_BYTES[material] + (0 if geo else STREAMED_SHAPE_BYTES[material])
+    if compact:  # Gd 24 + 24 instead of Q 36 + 36; epsbar only when the flow reads it
+        tet_bytes -= 2 * 12 + (0 if sim.plan().epsbar else 8)
+    kernel_bytes = tet_bytes * nT + (KERNEL_NODE_BYTES + (12 if geo else 0)) * N
     step_bytes = SPEC_TET_BYTES[material] * nT + 108 * N
     achieved = force_bytes / (ms_force * 1e-3) / 1e9
     nt_flag = sim.plan().stream_nt
     kernel = (f"k_tile_force<{'NFX_VISCOPLASTIC' if vp else 'NFX_MR'}, K={sim.tile_k}, "
+              f"{'shape from reference positions' if geo else 'streamed shape'}, "
               f"{'nontemporal' if nt_flag else 'default'} cache policy>") if args.scatter == "tiled" else \
         f"{'k_vp' if vp else 'k_tetra'}<{args.scatter}>"
     traffic, traffic_source = None, None
     if args.traffic_json and os.path.exists(args.traffic_json):
         tj = json.load(open(args.traffic_json))
-        key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if nt_flag else ''}"
+        key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if nt_flag else ''}{'_shape_from_ref' if geo else ''}"
         if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}):
             traffic = tj["force_pass_bytes"][key]
             traffic_source = f"{os.path.relpath(args.traffic_json, ROOT)}[{key}]: {tj.get('source', '')}"
@@ -428,6 +450,9 @@ def main():
             "scatter": args.scatter,
             "tile_k": sim.tile_k,
             "stream_nt": sim.stream_nt,
+            "shape_from_ref": geo,
+            "viscoplastic_state": ("compact: symmetric deviation Gd" + ("" if sim.plan().epsbar else ", no plastic strain")
+                                   if compact else "Q = Fp^-1 - I and plastic strain") if vp else None,
             "autotune_ms_per_step": tuned,
             "parallelism": "single GPU",
         },
@@ -449,6 +474,7 @@ def main():
             "ms_per_launch": ms_force,
         },
     }
+    sim_tile_k, sim_stream_nt = sim.tile_k, sim.stream_nt
     del sim
     torch.cuda.empty_cache()
     if not args.no_extra:
@@ -462,6 +488,15 @@ def main():
             out["other_scatter"].append({"scatter": other, "ms_per_step": ms2, "value": nT * 1e3 / ms2,
                                          "ms_force_pass": f2, "ms_node_pass": n2})
             del s2
+        if vp and args.scatter == "tiled":  # the other viscoplastic state form, same tile shape
+            s4 = make_sim(cfg, P, T, "tiled", material, sim_tile_k, True, not compact)
+            s4.stream_nt = sim_stream_nt
+            ms4 = time_steps(s4, args.warmup, min(args.steps, 200))[0] * 1e3
+            f4, n4 = pass_times(s4, 50)
+            out["other_state_form"] = {"viscoplastic_state": "Q = Fp^-1 - I and plastic strain" if compact else
+                                       "compact: symmetric deviation Gd", "ms_per_step": ms4,
+                                       "value": nT * 1e3 / ms4, "ms_force_pass": f4, "ms_node_pass": n4}
+            del s4
         small = configs.small_cell()
         Ps, Ts = cached_mesh(small)
         s3 = make_sim(small, Ps, Ts, "tiled", "elastic")

@@ -141,6 +141,11 @@ typedef struct {
                           and writes the per-tetra state and tile tables with the nontemporal cache
                           policy (faster when they exceed the memory-side cache), 0 = default policy;
                           an autotuner choice, results are identical */
+    int32_t shape_from_ref; /* NFX_SCATTER_TILED: 1 = the force pass forms each tetra's Dm^-1 and V0 from the
+                               reference positions ref of its nodes (FP32 cross products) instead of reading
+                               dminv and v0: 40 B per tetra less traffic for ~60 FLOPs.  Both describe the
+                               same reference shape; they differ by FP32 rounding (~1e-7 relative).
+                               dminv must still be set (the other scatter modes read it) */
 } nfx_plan;
 
 /* Per-cell parameters of a batch, in that cell's simulation units.  The batched material is the

@@ -20,6 +20,8 @@ Extensions (keyword arguments):
 ``tile_k`` tetra per tile for "tiled" (256 or 512);
 ``stream_nt`` cache policy of the tiled force pass's per-tetra streams (1 nontemporal, 0 default,
   None: nontemporal once they exceed the 256 MB memory-side cache; ``autotune`` times both);
+``shape_from_ref`` the tiled force pass forms each tetra's Dm^-1 and V0 from the FP32 reference
+  positions instead of streaming them (40 B per tetra less; the same shape to FP32 rounding);
 ``fused`` and ``use_graph`` switch the fused step and the graph replay off.
 """
 
@@ -53,7 +55,7 @@ def _law(fn, start, speed, move_steps):
 
 class Simulation:
     def __init__(self, con, scatter="tiled", fused=True, use_graph=True, mesh=None, verbose=True, tile_k=512,
-                 stream_nt=None):
+                 stream_nt=None, shape_from_ref=True):
         if isinstance(con, dict):
             self.parameters = dict(con)
         else:
@@ -64,6 +66,7 @@ class Simulation:
         self.scatter = scatter
         self.tile_k = tile_k
         self.stream_nt = stream_nt
+        self.shape_from_ref = shape_from_ref
         self.fused = fused
         self.use_graph = use_graph
         self.verbose = verbose
@@ -229,11 +232,12 @@ class Simulation:
         model = Tetra.materialModel[self.parameters["CELL"]["model"]]
         self.register(Tetra(self.cell, youngsModulus, poissonRatio, mooneyRivlinRatio, model))
 
-    def setInteractionFiniteStrainViscoplastic(self):
+    def setInteractionFiniteStrainViscoplastic(self, compactState=False):
         """Legacy parameter mapping of SimulationLegacy.py:842-882 (ReynoldScaling and YoungsScaling
         default to 1 when absent).  Config-2 style keys ``ViscositySI``/``YieldStressSI`` in CELL
         select the Maxwell-type mapping m = 1, n = 0, Y = 3*eta*(1/s), rate = T0 [1/s per step],
-        yield offset = YieldStressSI."""
+        yield offset = YieldStressSI.  ``compactState``: the fused tiled step keeps the symmetric
+        plastic deviation only (same forces; no Fp, no per-plugin step; see FiniteStrainViscoplastic)."""
         c = self.parameters["CELL"]
         rs = self.parameters.get("ReynoldScaling", 1.0)
         ys = c.get("YoungsScaling", 1.0)
@@ -256,7 +260,7 @@ class Simulation:
             yieldOffset = 0.0
         self.register(FiniteStrainViscoplastic(self.cell, ratio * shear, (1 - ratio) * shear, bulk, flowExponent,
                                                yieldStress, flowRate, hardeningThreshhold, hardeningExponent,
-                                               yieldOffset))
+                                               yieldOffset, compactState=compactState))
 
     # legacy stress laws (SimulationLegacy.py:696-840); ReynoldScaling and YoungsScaling default to 1
     def _legacy_youngs(self):
@@ -549,6 +553,7 @@ class Simulation:
             p.scatter = _lib.SCATTER_TILED
             p.tile_k, p.tile_max_nodes = t["tile_k"], t["max_nodes"]
             p.stream_nt = default_stream_nt(self.numTetra) if self.stream_nt is None else int(self.stream_nt)
+            p.shape_from_ref = 1 if self.shape_from_ref else 0
             for k in ("tile_count", "tile_nodes", "tile_lend", "tile_corners", "tile_dst", "node_off", "partials"):
                 setattr(p, k, t[k].data_ptr())
         else:

@@ -97,6 +97,7 @@ class Plan(ctypes.Structure):
         ("volume_fraction", c_f64),
         ("vp_sym", c_i32),
         ("stream_nt", c_i32),
+        ("shape_from_ref", c_i32),
     ]
 
 

@@ -330,14 +330,15 @@ class BatchSimulation:
     ``stream_nt`` cache policy of the force pass's streams (None: nontemporal beyond the 256 MB
     memory-side cache); ``sym_state`` viscoplastic state Gd = Fp^-1 Fp^-T - I (6 floats) instead of
     Q = Fp^-1 - I (9);
-    ``narrow`` 16-bit tile-node ids and partial positions (when every cell has < 65536 nodes).
+    ``narrow`` 16-bit tile-node ids and partial positions (when every cell has < 65536 nodes);
+    ``shape_from_ref`` the force pass forms Dm^-1 from the reference positions instead of streaming it.
 
     State on the device: node displacements u from the FP32 reference positions X0 (``ref``), the
     velocity-Verlet state w = a v + f/2, and for the viscoplastic law the plastic deviation Gd
     (``sym_state``, 6 per tetra) or Q (9 per tetra), all zero at the start."""
 
     def __init__(self, specs, meshes, steps=10000, ncurve=256, tile_k=256, use_graph=True, device_id=0,
-                 plastic_strain=False, stream_nt=None, sym_state=True, narrow=True):
+                 plastic_strain=False, stream_nt=None, sym_state=True, narrow=True, shape_from_ref=True):
         import torch
 
         from nofluidx3d_amd import openCL
@@ -389,8 +390,10 @@ class BatchSimulation:
             self.state = torch.zeros((6 if self.sym_state else 9) * self.T, dtype=torch.float32, device=dev)
             # the linear Maxwell law does not read the accumulated plastic strain: kept on request
             self.eps = torch.zeros(self.T, dtype=torch.float32, device=dev) if plastic_strain else None
+            self.plastic_strain = self.eps is not None
         else:
             self.state = self.eps = None
+            self.plastic_strain = False
         self.t_count, self.t_nodes, self.t_lend = u32(h["tile_count"]), u32(h["tile_nodes"]), u16(h["tile_lend"])
         self.t_corners, self.t_dst = u16(h["tile_corners"]), u32(h["tile_dst"])
         self.t_ntet, self.t_cell = u16(h["tile_ntet"]), u32(h["tile_cell"])
@@ -423,6 +426,8 @@ class BatchSimulation:
             p.vp.flow_exponent, p.vp.hardening_exponent = 1.0, 0.0  # every cell: linear Maxwell flow
             p.vp_sym = 1 if self.sym_state else 0
         p.stream_nt = default_stream_nt(self.T) if stream_nt is None else int(stream_nt)
+        self.shape_from_ref = bool(shape_from_ref)
+        p.shape_from_ref = 1 if shape_from_ref else 0
         p.scatter = _lib.SCATTER_TILED
         p.tile_k, p.tile_max_nodes = K, MX
         p.tile_count, p.tile_nodes, p.tile_lend = (t.data_ptr() for t in (self.t_count, self.t_nodes, self.t_lend))
@@ -584,6 +589,7 @@ class CellGroups:
         self.groups = [BatchSimulation(specs[a:b], meshes[a:b], **kw) for a, b in self.ranges]
         g0 = self.groups[0]
         self.steps, self.ncurve, self.tile_k, self.sym_state = g0.steps, g0.ncurve, g0.tile_k, g0.sym_state
+        self.shape_from_ref, self.plastic_strain = g0.shape_from_ref, g0.plastic_strain
         self.ncells = len(specs)
         self.T_real = sum(g.T_real for g in self.groups)
         self.N = sum(g.N for g in self.groups)

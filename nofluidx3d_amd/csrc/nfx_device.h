@@ -114,6 +114,20 @@ __device__ __forceinline__ M9 deformation_gradient(V3 r1, V3 r2, V3 r3, const fl
     return F;
 }
 
+// Dm^-1 of a tetra from its reference edges R_k = X0_k - X0_0 (the columns of Dm), in FP32: rows
+// (R2 x R3, R3 x R1, R1 x R2) / det with det = R1 . (R2 x R3) = 6 V0, returned.  The edges of FP32
+// reference positions are correctly rounded differences (relative error 2^-24), so D differs from the
+// FP64-computed, FP32-rounded Dm^-1 by a few units in the last place relative to |D|.
+__device__ __forceinline__ float dminv_from_ref(V3 r1, V3 r2, V3 r3, float* D) {
+    const V3 g1 = cross(r2, r3), g2 = cross(r3, r1), g3 = cross(r1, r2);
+    const float det = dot(r1, g1);
+    const float id = frcp(det);
+    D[0] = g1.x * id; D[1] = g1.y * id; D[2] = g1.z * id;
+    D[3] = g2.x * id; D[4] = g2.y * id; D[5] = g2.z * id;
+    D[6] = g3.x * id; D[7] = g3.y * id; D[8] = g3.z * id;
+    return det;
+}
+
 // H = F - I from the four node displacements (the fused and batched steps)
 __device__ __forceinline__ M9 grad_disp(V3 u0, V3 u1, V3 u2, V3 u3, const float* D) {
     return deformation_gradient(u1 - u0, u2 - u0, u3 - u0, D);

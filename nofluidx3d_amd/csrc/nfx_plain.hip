@@ -321,11 +321,14 @@ __global__ __launch_bounds__(kBlock) void k_disp_convert(const float* __restrict
 // and the position of its partial sum in node order (tile_dst).  Each corner slot 4*(t-t0)+a
 // carries its tile-local node and its rank in that node's run (tile_corners), so its place in the
 // sorted list is run start + rank.  Phases:
-//   0  issue the tetra's streaming loads (Dm^-1, and V0 or the viscoplastic state and epsbar), its
-//      corner records and the tile metadata
-//   1  node displacements and run starts -> LDS
-//   2  material law on LDS displacements; each corner force -> LDS at its place in the sorted list
+//   0  issue the tetra's streaming loads (Dm^-1 and V0 unless GEO, the viscoplastic state and epsbar),
+//      its corner records and the tile metadata
+//   1  node displacements (GEO: and reference positions) and run starts -> LDS
+//   2  (GEO: Dm^-1 and V0 from the reference edges) material law on LDS displacements; each corner
+//      force -> LDS at its place in the sorted list
 //   3  one thread per tile-node: sum its contiguous run -> partials[tile_dst]
+// GEO trades the 36 + 4 B per tetra of streamed Dm^-1 and V0 for 12 B per tile-node of reference
+// positions (~4 B per tetra) and ~60 FLOPs per tetra.
 // The node pass sums each node's contiguous partials in tile order: no atomics, fixed order.
 struct TileArgs {
     int64_t N, T;
@@ -397,19 +400,21 @@ __device__ void batch_curve_sample(const float* __restrict__ x, const float* __r
     }
 }
 
-// LDS of a tile: corner forces 12K floats, then per tile-node its position (3 floats) and run
-// start (u16)
+// LDS of a tile: corner forces 12K floats, then per tile-node its displacement (3 floats), with GEO
+// its reference position (3 floats), and its run start (u16)
 template <int K>
 __host__ __device__ constexpr size_t tile_lds_bytes_fixed() {
     return 4 * 12 * K;
 }
-__host__ __device__ constexpr size_t tile_lds_bytes_per_node() { return 12 + 2; }
+__host__ __device__ constexpr size_t tile_lds_bytes_per_node(bool geo) { return 12 + (geo ? 12 : 0) + 2; }
 
-// NT: cache policy of the streams; EPS: the viscoplastic law keeps its accumulated plastic strain
-// (the linear law does not read it); SYM: its state is the symmetric deviation Gd (6 planes) instead
-// of Q (9 planes); NARROW: 16-bit tile-node metadata (batches)
-template <int MODEL, int K, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false, bool NARROW = false>
-__global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS)) ? 6 : 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
+// GEO: Dm^-1 and V0 from the staged reference positions; NT: cache policy of the streams; EPS: the
+// viscoplastic law keeps its accumulated plastic strain (the linear law does not read it); SYM: its
+// state is the symmetric deviation Gd (6 planes) instead of Q (9 planes); NARROW: 16-bit tile-node
+// metadata (batches)
+template <int MODEL, int K, bool GEO, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false,
+          bool NARROW = false>
+__global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS || STRESS)) ? 6 : 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
     // one thread per tetra of the tile (blockDim.x == K)
@@ -417,7 +422,8 @@ __global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS)) ? 6
     float* s_cf = reinterpret_cast<float*>(smem);  // [component][4K], sorted-corner order
     const int MX = a.max_nodes;
     float* s_x = s_cf + 12 * K;                    // [component][MX]
-    uint16_t* s_lb = reinterpret_cast<uint16_t*>(s_x + 3 * MX);
+    float* s_r = s_x + 3 * MX;                     // GEO: [component][MX] reference positions
+    uint16_t* s_lb = reinterpret_cast<uint16_t*>(s_r + (GEO ? 3 * MX : 0));
 
     const uint32_t ntile_blocks = gridDim.x - (uint32_t)a.curve_blocks;
     if (blockIdx.x >= ntile_blocks) {  // the step's curve sample of cell blockIdx.x - ntile_blocks
@@ -451,13 +457,15 @@ __global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS)) ? 6
     // issue this tetra's streaming loads first: they overlap the staging phases below
     float D[9], Sv[MODEL == NFX_VISCOPLASTIC ? 9 : 1], e0 = 0.f, v0 = 0.f;
     if (active) {
+        if constexpr (!GEO) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) D[k] = ld_g<NT, float>(uat(a.dminv + t0 + k * T, u));
+            for (int k = 0; k < 9; ++k) D[k] = ld_g<NT, float>(uat(a.dminv + t0 + k * T, u));
+        }
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
 #pragma unroll
             for (int k = 0; k < (SYM ? 6 : 9); ++k) Sv[k] = ld_g<NT, float>(uat((const float*)(Sst + t0 + k * T), u));
             if constexpr (EPS) e0 = ld_g<NT, float>(uat((const float*)(eps + t0), u));
-        } else if constexpr (MODEL < NFX_VISCOPLASTIC) {
+        } else if constexpr (MODEL < NFX_VISCOPLASTIC && !GEO) {
             v0 = ld_g<NT, float>(uat(a.v0 + t0, u));
         }
     }
@@ -486,12 +494,24 @@ __global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS)) ? 6
         s_x[tl] = a.x[nid];
         s_x[MX + tl] = a.x[N + nid];
         s_x[2 * MX + tl] = a.x[2 * N + nid];
+        if constexpr (GEO) {
+            s_r[tl] = a.ref[nid];
+            s_r[MX + tl] = a.ref[N + nid];
+            s_r[2 * MX + tl] = a.ref[2 * N + nid];
+        }
         s_lb[tl] = (uint16_t)lb;
     }
     __syncthreads();
     if (active) {
         const int l0 = (int)(cr & 0xFF), l1 = (int)((cr >> 16) & 0xFF), l2 = (int)((cr >> 32) & 0xFF),
                   l3 = (int)((cr >> 48) & 0xFF);
+        if constexpr (GEO) {
+            const V3 r0 = mk(s_r[l0], s_r[MX + l0], s_r[2 * MX + l0]);
+            const V3 r1 = mk(s_r[l1], s_r[MX + l1], s_r[2 * MX + l1]);
+            const V3 r2 = mk(s_r[l2], s_r[MX + l2], s_r[2 * MX + l2]);
+            const V3 r3 = mk(s_r[l3], s_r[MX + l3], s_r[2 * MX + l3]);
+            v0 = dminv_from_ref(r1 - r0, r2 - r0, r3 - r0, D) * (1.0f / 6.0f);
+        }
         const V3 u0 = mk(s_x[l0], s_x[MX + l0], s_x[2 * MX + l0]);
         const V3 u1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
         const V3 u2 = mk(s_x[l2], s_x[MX + l2], s_x[2 * MX + l2]);
@@ -903,7 +923,7 @@ static int validate_plan(const nfx_plan* p) {
         NFX_REQUIRE(p->tile_max_nodes > 0 && p->tile_max_nodes <= p->tile_k && p->tile_max_nodes <= 256,
                     "plan: tile_max_nodes %d exceeds tile_k %d or 256 (use another tile size or scatter mode)",
                     p->tile_max_nodes, p->tile_k);
-        const size_t lds = (size_t)p->tile_k * 48 + tile_lds_bytes_per_node() * (size_t)p->tile_max_nodes;
+        const size_t lds = (size_t)p->tile_k * 48 + tile_lds_bytes_per_node(p->shape_from_ref != 0) * (size_t)p->tile_max_nodes;
         NFX_REQUIRE(lds <= 65536, "plan: tile needs %zu B of LDS (max 65536); use a smaller tile_k", lds);
     }
     NFX_REQUIRE(p->scatter != NFX_SCATTER_ATOMIC || p->force, "plan: atomic scatter needs force");
@@ -913,16 +933,16 @@ static int validate_plan(const nfx_plan* p) {
     return NFX_OK;
 }
 
-template <int MODEL, int K>
+template <int MODEL, int K, bool GEO>
 static void launch_tile_legacy(unsigned int nb, size_t lds, hipStream_t s, const TileArgs& a, const ElasticDev& e,
                                const VPConst& c, float* vm, float* pr) {
     if (vm && pr)
-        k_tile_force<MODEL, K, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, vm, pr);
+        k_tile_force<MODEL, K, GEO, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, vm, pr);
     else
-        k_tile_force<MODEL, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
+        k_tile_force<MODEL, K, GEO, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
 }
 
-template <int K, bool STRESS, bool NT>
+template <int K, bool GEO, bool STRESS, bool NT>
 static void launch_tile_vp(unsigned int nb, size_t lds, hipStream_t s, const TileArgs& a, const ElasticDev& e,
                            const VPConst& c, const nfx_plan& p) {
     float* vm = STRESS ? p.von_mises : nullptr;
@@ -930,29 +950,29 @@ static void launch_tile_vp(unsigned int nb, size_t lds, hipStream_t s, const Til
     if constexpr (!STRESS) {  // the batch's form: symmetric state with 16-bit tile-node metadata
         if (p.vp_sym && a.tile_nodes16) {
             if (p.epsbar)
-                k_tile_force<NFX_VISCOPLASTIC, K, false, NT, true, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
+                k_tile_force<NFX_VISCOPLASTIC, K, GEO, false, NT, true, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
                                                                                                 p.epsbar, vm, pr);
             else
-                k_tile_force<NFX_VISCOPLASTIC, K, false, NT, false, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
+                k_tile_force<NFX_VISCOPLASTIC, K, GEO, false, NT, false, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
                                                                                                  nullptr, vm, pr);
             return;
         }
     }
     if (p.vp_sym) {
         if (p.epsbar)
-            k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar,
+            k_tile_force<NFX_VISCOPLASTIC, K, GEO, STRESS, NT, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar,
                                                                                         vm, pr);
         else
-            k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr,
+            k_tile_force<NFX_VISCOPLASTIC, K, GEO, STRESS, NT, false, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr,
                                                                                          vm, pr);
     } else if (p.epsbar) {
-        k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar, vm, pr);
+        k_tile_force<NFX_VISCOPLASTIC, K, GEO, STRESS, NT, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar, vm, pr);
     } else {
-        k_tile_force<NFX_VISCOPLASTIC, K, STRESS, NT, false><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr, vm, pr);
+        k_tile_force<NFX_VISCOPLASTIC, K, GEO, STRESS, NT, false><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr, vm, pr);
     }
 }
 
-template <int K>
+template <int K, bool GEO>
 static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     // ctx->batch is set only while a batched step is being enqueued
     TileArgs a{p.N,          p.T,         p.dminv,        p.v0,       p.disp,     p.ref,
@@ -981,7 +1001,7 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
         }
     }
     const unsigned int nb = (unsigned int)((p.T + K - 1) / K) + (unsigned int)a.curve_blocks;
-    const size_t lds = tile_lds_bytes_fixed<K>() + tile_lds_bytes_per_node() * (size_t)p.tile_max_nodes;
+    const size_t lds = tile_lds_bytes_fixed<K>() + tile_lds_bytes_per_node(GEO) * (size_t)p.tile_max_nodes;
     const ElasticDev e = elastic_consts(p);
     const VPConst c = vp_consts(p.vp);
     hipStream_t s = ctx->stream;
@@ -991,26 +1011,26 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     switch (p.model) {
         case NFX_MR:
             if (a.tile_nodes16 && nt)
-                k_tile_force<NFX_MR, K, false, true, true, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr,
+                k_tile_force<NFX_MR, K, GEO, false, true, true, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr,
                                                                                          nullptr, nullptr);
             else if (a.tile_nodes16)
-                k_tile_force<NFX_MR, K, false, false, true, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr,
+                k_tile_force<NFX_MR, K, GEO, false, false, true, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr,
                                                                                           nullptr, nullptr);
             else if (nt)
-                k_tile_force<NFX_MR, K, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
-            else k_tile_force<NFX_MR, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
+                k_tile_force<NFX_MR, K, GEO, false, true><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
+            else k_tile_force<NFX_MR, K, GEO, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
             break;
-        case NFX_SVK: k_tile_force<NFX_SVK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
-        case NFX_SVKK: k_tile_force<NFX_SVKK, K, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
-        case NFX_LINEAR_ELASTIC: launch_tile_legacy<NFX_LINEAR_ELASTIC, K>(nb, lds, s, a, e, c, vm, pr); break;
-        case NFX_LINEAR_DEVIATORIC: launch_tile_legacy<NFX_LINEAR_DEVIATORIC, K>(nb, lds, s, a, e, c, vm, pr); break;
-        case NFX_NEO_HOOKEAN_2: launch_tile_legacy<NFX_NEO_HOOKEAN_2, K>(nb, lds, s, a, e, c, vm, pr); break;
-        case NFX_PORO_NEO_HOOKEAN: launch_tile_legacy<NFX_PORO_NEO_HOOKEAN, K>(nb, lds, s, a, e, c, vm, pr); break;
+        case NFX_SVK: k_tile_force<NFX_SVK, K, GEO, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_SVKK: k_tile_force<NFX_SVKK, K, GEO, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr); break;
+        case NFX_LINEAR_ELASTIC: launch_tile_legacy<NFX_LINEAR_ELASTIC, K, GEO>(nb, lds, s, a, e, c, vm, pr); break;
+        case NFX_LINEAR_DEVIATORIC: launch_tile_legacy<NFX_LINEAR_DEVIATORIC, K, GEO>(nb, lds, s, a, e, c, vm, pr); break;
+        case NFX_NEO_HOOKEAN_2: launch_tile_legacy<NFX_NEO_HOOKEAN_2, K, GEO>(nb, lds, s, a, e, c, vm, pr); break;
+        case NFX_PORO_NEO_HOOKEAN: launch_tile_legacy<NFX_PORO_NEO_HOOKEAN, K, GEO>(nb, lds, s, a, e, c, vm, pr); break;
         default:
-            if (st && nt) launch_tile_vp<K, true, true>(nb, lds, s, a, e, c, p);
-            else if (st) launch_tile_vp<K, true, false>(nb, lds, s, a, e, c, p);
-            else if (nt) launch_tile_vp<K, false, true>(nb, lds, s, a, e, c, p);
-            else launch_tile_vp<K, false, false>(nb, lds, s, a, e, c, p);
+            if (st && nt) launch_tile_vp<K, GEO, true, true>(nb, lds, s, a, e, c, p);
+            else if (st) launch_tile_vp<K, GEO, true, false>(nb, lds, s, a, e, c, p);
+            else if (nt) launch_tile_vp<K, GEO, false, true>(nb, lds, s, a, e, c, p);
+            else launch_tile_vp<K, GEO, false, false>(nb, lds, s, a, e, c, p);
     }
 }
 
@@ -1053,8 +1073,9 @@ static void launch_tetra_pass(hipStream_t s, const TetArgs& a, int model, bool c
 static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
     const bool st = stress && p.von_mises && p.pressure;
     if (p.scatter == NFX_SCATTER_TILED) {
-        if (p.tile_k == 256) launch_tiled_k<256>(ctx, p, st);
-        else launch_tiled_k<512>(ctx, p, st);
+        const bool geo = p.shape_from_ref != 0;
+        if (p.tile_k == 256) geo ? launch_tiled_k<256, true>(ctx, p, st) : launch_tiled_k<256, false>(ctx, p, st);
+        else geo ? launch_tiled_k<512, true>(ctx, p, st) : launch_tiled_k<512, false>(ctx, p, st);
         NFX_CHECK_LAUNCH();
         return NFX_OK;
     }

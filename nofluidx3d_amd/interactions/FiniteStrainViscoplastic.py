@@ -15,6 +15,13 @@ State per tetra: the accumulated plastic strain, and the plastic deformation in 
   every step; Q carries the same information with its own relative precision.
 ``state_form`` converts when the stepping path changes; ``plastic_deformation()`` returns an
 up-to-date Fp.  ``vonMises`` and ``pressure`` are written when the plugin's ``write_stress`` is on.
+
+``compactState=True`` keeps, for the fused tiled step only, the symmetric deviation
+Gd = Fp^-1 Fp^-T - I (6 SoA, ``plasticDeviationSymB``) instead of Fp and Q, and the accumulated plastic
+strain only when the flow reads it (flow exponent != 1 or hardening exponent != 0).  The stress, the
+flow and the forces depend on Gd alone, so positions and forces are the same to FP32 rounding with 36
+B per tetra and step less traffic (the Maxwell-type law: 44 B); Fp is not recoverable from Gd, so
+the per-plugin step and ``plastic_deformation()`` are unavailable.
 """
 import numpy as np
 
@@ -36,6 +43,7 @@ class FiniteStrainViscoplastic(Interaction):
         hardeningThreshhold,
         hardeningExponent,
         yieldOffset=0.0,
+        compactState=False,
     ):
         super().__init__()
         self.referenceCell = referenceCell
@@ -44,6 +52,10 @@ class FiniteStrainViscoplastic(Interaction):
             float(viscoplasticYieldStress), float(viscoplasticFlowRate), float(hardeningThreshhold),
             float(hardeningExponent), float(yieldOffset))
         self.write_stress = False
+        self.compact = bool(compactState)
+
+    def _reads_strain(self):
+        return self.params.flow_exponent != 1.0 or self.params.hardening_exponent != 0.0
 
     def createBuffers(self):
         import torch
@@ -53,6 +65,11 @@ class FiniteStrainViscoplastic(Interaction):
         T = self.simulation.numTetra
         dev = self.simulation.device
         self.accumulatedStrainB = torch.zeros(T, dtype=torch.float32, device=dev)
+        if self.compact:
+            self.plasticDeviationSymB = torch.zeros(6 * T, dtype=torch.float32, device=dev)
+            torch.cuda.synchronize(dev)
+            self._current = {"gd"}
+            return
         eye_soa = np.repeat(np.eye(3, dtype=np.float32).reshape(9), T)  # entry k of tetra t at k*T + t
         self.plasticDeformationTensorB = torch.from_numpy(eye_soa).to(dev)
         self.plasticDeviationB = torch.zeros(9 * T, dtype=torch.float32, device=dev)
@@ -60,7 +77,15 @@ class FiniteStrainViscoplastic(Interaction):
         self._current = {"fp", "q"}
 
     # ---- the two state forms --------------------------------------------------------------------
+    def _no_fp(self, what):
+        raise RuntimeError(f"{what} needs Fp, which the compact viscoplastic state (Gd) does not hold: "
+                           "use compactState=False")
+
     def state_form(self, fused: bool):
+        if self.compact:
+            if not fused:
+                self._no_fp("the per-plugin viscoplastic step")
+            return
         want = "q" if fused else "fp"
         if want not in self._current:
             self._convert(want)
@@ -77,18 +102,24 @@ class FiniteStrainViscoplastic(Interaction):
         self._current.add(want)
 
     def sync_forms(self):
+        if self.compact:
+            return
         for want in ("fp", "q"):
             if want not in self._current:
                 self._convert(want)
 
     def forms_restored(self):
-        self._current = {"fp", "q"}
+        self._current = {"gd"} if self.compact else {"fp", "q"}
 
     def state_buffers(self):
+        if self.compact:
+            return [self.plasticDeviationSymB, self.accumulatedStrainB]
         return [self.plasticDeformationTensorB, self.plasticDeviationB, self.accumulatedStrainB]
 
     def plastic_deformation(self):
         """The plastic deformation gradient Fp (9T SoA device tensor), up to date."""
+        if self.compact:
+            self._no_fp("plastic_deformation()")
         self.sync_forms()
         openCL.getCommandQueue().finish()
         return self.plasticDeformationTensorB
@@ -120,8 +151,14 @@ class FiniteStrainViscoplastic(Interaction):
         plan.model = _lib.NFX_VISCOPLASTIC
         plan.vp = self.params
         plan.dminv, plan.v0 = self.dminvB.data_ptr(), self.volumesB.data_ptr()
-        plan.vp_state = self.plasticDeviationB.data_ptr()
-        plan.epsbar = self.accumulatedStrainB.data_ptr()
+        if self.compact:
+            if getattr(self.simulation, "scatter", "atomic") != "tiled":
+                self._no_fp(f"the {self.simulation.scatter} scatter's viscoplastic step")
+            plan.vp_state, plan.vp_sym = self.plasticDeviationSymB.data_ptr(), 1
+            plan.epsbar = self.accumulatedStrainB.data_ptr() if self._reads_strain() else 0
+        else:
+            plan.vp_state = self.plasticDeviationB.data_ptr()
+            plan.epsbar = self.accumulatedStrainB.data_ptr()
         if self.write_stress:
             plan.von_mises, plan.pressure = self.simulation.vonMises.ptr, self.simulation.pressure.ptr
         return True

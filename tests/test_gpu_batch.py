@@ -182,6 +182,21 @@ def test_symmetric_deviation_matches_full_deviation(ctx):
         assert np.max(np.abs(a.positions(i) - b.positions(i))) < 1e-5 * a.radius_sim[i]
 
 
+def test_shape_from_reference_positions_matches_streamed_shape(ctx):
+    """The force pass forming Dm^-1 from the FP32 reference positions against the one streaming the
+    FP64-precomputed Dm^-1: the same reference shape, FP32 rounding apart."""
+    a, _ = make_batch(True, shape_from_ref=True)
+    a.advance(STEPS)
+    b, _ = make_batch(True, shape_from_ref=False)
+    b.advance(STEPS)
+    assert a.plan.shape_from_ref == 1 and b.plan.shape_from_ref == 0
+    ca, cb = a.curves_sim(), b.curves_sim()
+    assert np.max(np.abs(cb)) > 0
+    assert np.max(np.abs(ca - cb)) < 1e-5 * np.max(np.abs(cb))
+    for i in range(a.ncells):
+        assert np.max(np.abs(a.positions(i) - b.positions(i))) < 1e-5 * a.radius_sim[i]
+
+
 def test_narrow_tile_metadata_changes_no_bit(ctx):
     """16-bit tile-node ids and partial positions against the 32-bit arrays: the same run, bitwise."""
     a, _ = make_batch(True)

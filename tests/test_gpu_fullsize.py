@@ -5,7 +5,8 @@ maximum (the force is zero before contact, so a pointwise relative error is unde
 - The large cell (R 10 um, 0.2 um lattice, seed 2: 523 305 nodes, 3 377 961 tetra; Mooney-Rivlin +
   Maxwell viscoplastic, substrate, spherical tip):
   - developed contact: the tip driven at 8x the protocol's speed (40 um/s) for 2000 steps, 0.8 um
-    into the cell, tiled scatter with hipGraph replay; a second run is bitwise identical;
+    into the cell, tiled scatter with hipGraph replay; a second run is bitwise identical; the same
+    window with the compact viscoplastic state;
   - the three scatter modes over the first 600 steps of that run: float atomics and the CSR gather
     against the oracle and against the tiled run;
   - the whole protocol (3 um at 5 um/s, then a 2 s hold: 260 000 steps) on the GPU alone, checked
@@ -56,7 +57,7 @@ def large_mesh(cfg):
     return P, T
 
 
-def large_cell_sim(scatter="tiled", speed=1.0):
+def large_cell_sim(scatter="tiled", speed=1.0, compact=False):
     from nofluidx3d_amd import configs
     from nofluidx3d_amd.Simulation import Simulation
     from nofluidx3d_amd.util import vtk
@@ -68,7 +69,7 @@ def large_cell_sim(scatter="tiled", speed=1.0):
     sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False, tile_k=512)
     sim.setInteractionSubstrate()
     sim.setInteractionSphere()
-    sim.setInteractionFiniteStrainViscoplastic()
+    sim.setInteractionFiniteStrainViscoplastic(compactState=compact)
     sim.setInteractionVelocityVerlet()
     return sim
 
@@ -117,6 +118,17 @@ def test_large_cell_developed_contact_matches_oracle(large_oracle):
     again = large_cell_sim(speed=FAST)
     gx2, gf2 = run(again, WINDOW)
     assert np.array_equal(gx2, gx) and np.array_equal(gf2, gf)
+
+
+def test_large_cell_compact_state_matches_oracle(large_oracle):
+    """The compact viscoplastic state (Gd, no plastic strain) over the developed-contact window."""
+    ref = large_oracle
+    sim = large_cell_sim(speed=FAST, compact=True)
+    gx, gf = run(sim, WINDOW)
+    assert sim.plan().vp_sym == 1
+    pe, fe = errors(gx, gf, ref["x"], ref["f"], ref["R"])
+    print(f"compact state, {WINDOW} steps: pos_err {pe:.2e} of R, force_err {fe:.2e}")
+    assert pe < 1e-4 and fe < 1e-4
 
 
 @pytest.mark.parametrize("scatter", ["atomic", "csr"])

@@ -193,8 +193,8 @@ def test_surface_adhesion(ctx, mesh):
 
 def test_hard_surface_adhesion(ctx, mesh):
     """InteractionHardAdhesivePlaneSurfaceIntegral.cl: exp(-k d) - A exp(-d) by the 3-point rule, against
-    the oracle; the wall 0.05 above the lowest node, so that the surface near the wall (nearer than
-    d* = ln(A) / (1 - k) = 0.255) is repelled and the rest attracted."""
+    the oracle.  The rule samples the edge midpoints; the wall lies 0.1 above the lowest of them, so
+    that the surface there (below d* = ln(A) / (1 - k) = 0.255) is repelled and the rest attracted."""
     import torch
 
     from nofluidx3d_amd.interactions.LegacyContacts import surface_triangles
@@ -202,7 +202,9 @@ def test_hard_surface_adhesion(ctx, mesh):
     P, T, x = mesh
     N = len(P)
     tri = surface_triangles(T)
-    wall = x[N:2 * N].min() + 0.05
+    y = x[N:2 * N]
+    mids = np.concatenate([(y[tri[:, a]] + y[tri[:, b]]) / 2 for a, b in ((0, 2), (1, 2), (0, 1))])
+    wall = mids.min() + 0.1
     A, k = 0.6, 3.0
     fr = np.zeros(3 * N)
     O.hard_adhesive_plane_surface(fr, x, O.soa(tri, np.uint32), wall, A, k)
