@@ -257,13 +257,16 @@ def bench_batched(steps=10000, ncells=512, form="auto", log=log):
     }
 
 
-def bench_surrogate(weights, n_targets=512, log=log):
-    """The learned part: MLP inference and training-step throughput on the MFMA kernels, and a
-    surrogate-assisted fit of ``n_targets`` new cells (batched workload generator, seeds 7000+i,
-    1% noise) confirmed by the plain simulator."""
+def bench_surrogate(weights, n_targets=4096, n_nm=16, nm_evals=300, single=16, single_nm=1, log=log):
+    """The learned part: MLP inference and training-step throughput on the MFMA kernels, then the
+    fitting benchmark (``surrogate.fit_bench``): ``n_targets`` new cells (batched workload generator,
+    seeds 7000+i, 1% noise) fitted together on the surrogate path with the plain simulator refining
+    and confirming, simulation-only Nelder-Mead on the first ``n_nm`` of them for ``nm_evals``
+    evaluations (the evaluations it needs for the surrogate path's median error are reported), and
+    the p50/p99 latency of one-target fits."""
     import torch
 
-    from nofluidx3d_amd.surrogate import fit, fit_bench, train
+    from nofluidx3d_amd.surrogate import fit_bench, train
 
     net, norm = train.load(weights)
     sizes = net.sizes
@@ -294,15 +297,9 @@ def bench_surrogate(weights, n_targets=512, log=log):
            "inference_rows_per_s": rows / t_inf, "inference_tflops": flops_row * rows / t_inf / 1e12,
            "train_step_ms_batch4096": t_train * 1e3, "train_tflops": 3 * flops_row * B / t_train / 1e12,
            "mfma_peak_tflops_bf16_dense": 2500.0}
-    cells, meshes, bs, clean, noisy = fit_bench.make_targets(n_targets)
-    r = fit.fit_surrogate(net, norm, noisy, fit_bench.hidden_blind(cells), bs, ntop=1, corrections=1)
-    out["fit"] = {"targets": n_targets, "fits_per_s": n_targets / r["total_s"], "batch_wall_s": r["total_s"],
-                  "simulations_per_target": r["simulations_per_target"],
-                  "confirmed_rel_l2": fit_bench.err_stats(r["sim_err"]),
-                  "confirmed_rel_l2_vs_noise_free_curve": fit_bench.err_stats(fit.rel_err(r["curve"], clean)),
-                  "noise_floor_rel_l2_median": float(np.median(fit.rel_err(clean, noisy))),
-                  "param_errors": fit_bench.param_errors(r["theta"], fit_bench.theta_true(cells))}
-    del bs
+    args = fit_bench.options(["--weights", weights, "--n", str(n_targets), "--n-nm", str(n_nm), "--nm-evals",
+                              str(nm_evals), "--single", str(single), "--single-nm", str(single_nm)])
+    out["fit"] = fit_bench.run(net, norm, args, log=log)
     return out
 
 

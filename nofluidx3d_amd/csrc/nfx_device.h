@@ -503,6 +503,7 @@ struct VPOut {
     float Fp[9];  // the updated state: Fp (vp_tetra), Q = Fp^-1 - I (vp_tetra_q) or Gd (vp_tetra_gd)
     float epsbar;
     float von_mises, pressure;
+    bool flows;  // Dp != 0: the plastic state changes this step
 };
 
 struct VPConst {
@@ -556,6 +557,8 @@ __device__ __forceinline__ S6 vp_stress_flow(const S6& Bed, float epsbar, const 
         o.epsbar = epsbar + rate;
         g = (svm != 0.0f) ? rate * 1.5f * frcp(svm) : 0.0f;
     }
+    // Dp = 0 (no overstress, or no deviatoric stress) gives N = 0: the state update adds exact zeros
+    o.flows = g != 0.0f && svm != 0.0f;
     return S6{g * dv.xx, g * dv.yy, g * dv.zz, g * dv.xy, g * dv.yz, g * dv.xz};
 }
 
@@ -608,7 +611,9 @@ __device__ __forceinline__ VPOut vp_tetra(const M9& H, const float* D, const flo
 // matrix C the forces already use.  Stored as Fp itself in FP32, a small plastic increment would
 // be rounded against the O(1) entries of Fp every step (a drift of ~1e-7 of Fp per step); stored
 // as Q it keeps its own relative precision.
-__device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, const float* Q, float epsbar,
+// s = det Dm (= 6 V0).  A tetra that does not flow keeps Q bit for bit; a wave whose tetra all do not
+// flow (the far field under a yield offset) skips the update.
+__device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, float s, const float* Q, float epsbar,
                                             const VPConst& P) {
     VPOut o;
     M9 He;
@@ -620,10 +625,14 @@ __device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, const f
                          (H.m[i][0] * Q[0 * 3 + j] + H.m[i][1] * Q[1 * 3 + j] + H.m[i][2] * Q[2 * 3 + j]);
     S6 sig;
     const S6 Dp = vp_stress_flow(cauchy_green_m1(He), epsbar, P, o, sig);
-    const float s = det_dm(D);
     const float Jm1 = det_ipH_m1(H);
     const M9 C = cof_ipH(H);
     o.f = cofactor_forces(sig, edge_cofactors(C, D, s), s * (1.0f + Jm1) < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
+    if (!o.flows) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) o.Fp[k] = Q[k];
+        return o;
+    }
     // W = N Fe = N (I + He)
     const S6 N = flow_n(Dp);
     const V3 n0 = mk(N.xx, N.xy, N.xz), n1 = mk(N.xy, N.yy, N.yz), n2 = mk(N.xz, N.yz, N.zz);
@@ -656,7 +665,7 @@ __device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, const f
 //     Gd' = Gd + F^-1 X F^-T,   X = -N Be - Be N + N Be N = (N^2 - 2N) Be,
 // (N is a function of the deviatoric stress of an isotropic law, a polynomial in Be, so it commutes
 // with Be), an increment that is zero when nothing flows.  Fp is not recoverable from Gd.
-__device__ __forceinline__ VPOut vp_tetra_gd(const M9& H, const float* D, const float* G6, float epsbar,
+__device__ __forceinline__ VPOut vp_tetra_gd(const M9& H, const float* D, float s, const float* G6, float epsbar,
                                              const VPConst& P) {
     VPOut o;
     const S6 G{G6[0], G6[1], G6[2], G6[3], G6[4], G6[5]};
@@ -673,10 +682,14 @@ __device__ __forceinline__ VPOut vp_tetra_gd(const M9& H, const float* D, const 
     Bed.xz += dot(tx, fz);
     S6 sig;
     const S6 Dp = vp_stress_flow(Bed, epsbar, P, o, sig);
-    const float s = det_dm(D);
     const float Jm1 = det_ipH_m1(H);
     const M9 C = cof_ipH(H);
     o.f = cofactor_forces(sig, edge_cofactors(C, D, s), s * (1.0f + Jm1) < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
+    if (!o.flows) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) o.Fp[k] = G6[k];
+        return o;
+    }
     const S6 N = flow_n(Dp);
     // X = (N^2 - 2N)(I + Bed)
     const V3 n0 = mk(N.xx, N.xy, N.xz), n1 = mk(N.xy, N.yy, N.yz), n2 = mk(N.xz, N.yz, N.zz);

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_vp(TetArgs a, VPConst P, float* __re
     for (int k = 0; k < 9; ++k) st[k] = state[k * T + t];
     const float e0 = eps ? eps[t] : 0.0f;
     const M9 H = tetra_grad<FUSED>(a, n0, n1, n2, n3, D);
-    const VPOut o = FUSED ? vp_tetra_q(H, D, st, e0, P) : vp_tetra(H, D, st, e0, P);
+    const VPOut o = FUSED ? vp_tetra_q(H, D, det_dm(D), st, e0, P) : vp_tetra(H, D, st, e0, P);
 #pragma unroll
     for (int k = 0; k < 9; ++k) state[k * T + t] = o.Fp[k];
     if (eps) eps[t] = o.epsbar;
@@ -411,10 +411,11 @@ __host__ __device__ constexpr size_t tile_lds_bytes_per_node(bool geo) { return 
 // GEO: Dm^-1 and V0 from the staged reference positions; NT: cache policy of the streams; EPS: the
 // viscoplastic law keeps its accumulated plastic strain (the linear law does not read it); SYM: its
 // state is the symmetric deviation Gd (6 planes) instead of Q (9 planes); NARROW: 16-bit tile-node
-// metadata (batches)
+// metadata (batches).  Waves per SIMD: the viscoplastic forms need 70-77 VGPRs (6 waves) except the
+// symmetric one without epsbar (64, 8 waves; 7 with GEO, whose 64-register bound spilled)
 template <int MODEL, int K, bool GEO, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false,
           bool NARROW = false>
-__global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS || STRESS)) ? 6 : 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
+__global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || STRESS) ? 6 : GEO ? 7 : 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
     // one thread per tetra of the tile (blockDim.x == K)
@@ -455,7 +456,7 @@ __global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS || ST
     const uint32_t u = (uint32_t)tl;
 
     // issue this tetra's streaming loads first: they overlap the staging phases below
-    float D[9], Sv[MODEL == NFX_VISCOPLASTIC ? 9 : 1], e0 = 0.f, v0 = 0.f;
+    float D[9], Sv[MODEL == NFX_VISCOPLASTIC ? 9 : 1], e0 = 0.f, v0 = 0.f, sdm = 0.f;
     if (active) {
         if constexpr (!GEO) {
 #pragma unroll
@@ -510,7 +511,10 @@ __global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS || ST
             const V3 r1 = mk(s_r[l1], s_r[MX + l1], s_r[2 * MX + l1]);
             const V3 r2 = mk(s_r[l2], s_r[MX + l2], s_r[2 * MX + l2]);
             const V3 r3 = mk(s_r[l3], s_r[MX + l3], s_r[2 * MX + l3]);
-            v0 = dminv_from_ref(r1 - r0, r2 - r0, r3 - r0, D) * (1.0f / 6.0f);
+            sdm = dminv_from_ref(r1 - r0, r2 - r0, r3 - r0, D);
+            v0 = sdm * (1.0f / 6.0f);
+        } else if constexpr (MODEL == NFX_VISCOPLASTIC) {
+            sdm = det_dm(D);
         }
         const V3 u0 = mk(s_x[l0], s_x[MX + l0], s_x[2 * MX + l0]);
         const V3 u1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
@@ -520,8 +524,8 @@ __global__ __launch_bounds__(K, (MODEL == NFX_VISCOPLASTIC && (!SYM || EPS || ST
         Corners f;
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
             VPOut o;
-            if constexpr (SYM) o = vp_tetra_gd(H, D, Sv, e0, P);
-            else o = vp_tetra_q(H, D, Sv, e0, P);
+            if constexpr (SYM) o = vp_tetra_gd(H, D, sdm, Sv, e0, P);
+            else o = vp_tetra_q(H, D, sdm, Sv, e0, P);
 #pragma unroll
             for (int k = 0; k < (SYM ? 6 : 9); ++k) st_g<NT, float>(uat(Sst + t0 + k * T, u), o.Fp[k]);
             if constexpr (EPS) st_g<NT, float>(uat(eps + t0, u), o.epsbar);

@@ -81,26 +81,14 @@ def err_stats(e):
             "frac_le_2pct": float(np.mean(e <= 0.02))}
 
 
-def main(argv=None):
+def run(net, norm, args, log=log):
+    """The benchmark with the settings of ``args`` (the fields of ``main``'s options); returns the
+    report as a dict."""
     from nofluidx3d_amd.autotune import group_tetra
     from nofluidx3d_amd.batch import BatchSimulation, CellGroups
-    from nofluidx3d_amd.surrogate import fit, train
+    from nofluidx3d_amd.surrogate import fit
 
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--weights", required=True)
-    ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--n-nm", type=int, default=512)
-    ap.add_argument("--nm-evals", type=int, default=60)
-    ap.add_argument("--ntop", type=int, default=1)
-    ap.add_argument("--corrections", type=int, default=1)
-    ap.add_argument("--variants", default="",
-                    help="more fitter settings on the same targets, 'ntop:corrections,...' (reported, not timed "
-                         "against Nelder-Mead)")
-    ap.add_argument("--single", type=int, default=16)
-    ap.add_argument("--steps", type=int, default=10000)
-    args = ap.parse_args(argv)
-
-    net, norm = train.load(args.weights)
+    t_start = time.time()
     cells, meshes, bs, clean, noisy = make_targets(args.n, steps=args.steps)
     blind = hidden_blind(cells)
     truth = theta_true(cells)
@@ -179,17 +167,44 @@ def main(argv=None):
         fit.fit_surrogate(net, norm, noisy[i:i + 1], blind[i:i + 1], b1, ntop=args.ntop,
                           corrections=args.corrections, confirm=False)
         lat_s.append(time.perf_counter() - t0)
-        if i < 2:
+        if i < args.single_nm:
             t0 = time.perf_counter()
             fit.fit_nelder_mead(noisy[i:i + 1], blind[i:i + 1], b1, max_evals=args.nm_evals)
             lat_n.append(time.perf_counter() - t0)
         del b1
     if lat_s:
         out["single"] = {"surrogate_p50_s": float(np.percentile(lat_s, 50)),
-                         "surrogate_p99_s": float(np.percentile(lat_s, 99)),
-                         "nelder_mead_p50_s": float(np.percentile(lat_n, 50)),
-                         "nelder_mead_p99_s": float(np.percentile(lat_n, 99)), "samples": [len(lat_s), len(lat_n)]}
-    print(json.dumps(out), flush=True)
+                         "surrogate_p99_s": float(np.percentile(lat_s, 99)), "samples": [len(lat_s), len(lat_n)]}
+        if lat_n:
+            out["single"].update({"nelder_mead_p50_s": float(np.percentile(lat_n, 50)),
+                                  "nelder_mead_p99_s": float(np.percentile(lat_n, 99))})
+    out["wall_s"] = time.time() - t_start
+    return out
+
+
+def options(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--weights", required=True)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--n-nm", type=int, default=512)
+    ap.add_argument("--nm-evals", type=int, default=60)
+    ap.add_argument("--ntop", type=int, default=1)
+    ap.add_argument("--corrections", type=int, default=1)
+    ap.add_argument("--variants", default="",
+                    help="more fitter settings on the same targets, 'ntop:corrections,...' (reported, not timed "
+                         "against Nelder-Mead)")
+    ap.add_argument("--single", type=int, default=16, help="one-target surrogate fits timed for p50/p99")
+    ap.add_argument("--single-nm", type=int, default=2, help="one-target Nelder-Mead fits timed")
+    ap.add_argument("--steps", type=int, default=10000)
+    return ap.parse_args(argv)
+
+
+def main(argv=None):
+    from nofluidx3d_amd.surrogate import train
+
+    args = options(argv)
+    net, norm = train.load(args.weights)
+    print(json.dumps(run(net, norm, args)), flush=True)
 
 
 if __name__ == "__main__":
