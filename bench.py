@@ -344,6 +344,15 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 512-cell measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
     ap.add_argument("--tile-k", type=int, default=0, choices=[0, 256, 512], help="0: autotune")
+    ap.add_argument("--start-step", type=int, default=-1,
+                    help="protocol step the timed window starts from (untimed steps before it); -1: the end of "
+                         "the indentation for the large cell (the deepest contact, the largest plastic zone), 0 "
+                         "for the small cell")
+    ap.add_argument("--no-full-protocol", action="store_true", help="skip timing the whole protocol")
+    ap.add_argument("--save-state", default=None, metavar="NPZ",
+                    help="advance to the start step, save the simulation state there and exit")
+    ap.add_argument("--load-state", default=None, metavar="NPZ",
+                    help="start the timed window from a state --save-state wrote (same workload)")
     ap.add_argument("--compact-state", action="store_true",
                     help="viscoplastic: the fused tiled step keeps the symmetric plastic deviation only "
                          "(same forces; no Fp)")
@@ -394,8 +403,23 @@ def main():
     compact = args.compact_state and material == "viscoplastic" and args.scatter == "tiled"
     sim = make_sim(cfg, P, T, args.scatter, material, args.tile_k or 512, not args.streamed_shape, compact)
     assert sim.plan() is not None
+    move_steps = int(round(cfg["MoveTimeSI"] / sim.T0))
+    start = (move_steps if args.workload == "large" else 0) if args.start_step < 0 else args.start_step
+    if args.load_state:  # the protocol state a previous run saved (a profiler then sees the window only)
+        sim.load_state(args.load_state)
+        start = sim.time
+        log(f"loaded step {start} from {args.load_state}")
+    elif start > 0:  # untimed: bring the cell to the protocol state the timed window starts from
+        t0 = time.perf_counter()
+        sim.advance(start)
+        sync(sim)
+        log(f"advanced to step {start} in {time.perf_counter() - t0:.1f}s")
+    if args.save_state:
+        sim.save_state(args.save_state)
+        print(json.dumps({"saved_state": args.save_state, "step": sim.time}), flush=True)
+        return
     tuned = None
-    if args.scatter == "tiled" and args.tile_k == 0:
+    if args.scatter == "tiled" and args.tile_k == 0:  # tuned in the state the window starts from
         from nofluidx3d_amd.autotune import tune_simulation
 
         _, tuned = tune_simulation(sim, log=log)
@@ -403,6 +427,10 @@ def main():
     ms = dt * 1e3
     ms_force, ms_node = pass_times(sim, min(args.steps, 100))
     vp = material == "viscoplastic"
+    plastic_tiles = None
+    if sim._vp_active is not None and sim.plan().vp_active:
+        fl = sim._vp_active.cpu().numpy()
+        plastic_tiles = float((fl != 0).mean())
     force_bytes = SPEC_TET_BYTES[material] * nT + 24 * N
     geo = bool(sim.plan().shape_from_ref) and args.scatter == "tiled"
     tet_bytes = KERNEL_TET_BYTES[material] + (0 if geo else STREAMED_SHAPE_BYTES[material])
@@ -441,12 +469,15 @@ def main():
             "workload": f"{args.workload}_cell: R {cfg['MESH']['radius_um']} um, lattice {cfg['MESH']['spacing_um']} "
                         f"um, seed {cfg['MESH']['seed']}, {N} nodes, {nT} tetra, "
                         f"{'Mooney-Rivlin + viscoplastic' if vp else cfg['CELL']['model']}, substrate + sphere tip, "
-                        f"fused step, {args.scatter} scatter, hipGraph replay",
+                        f"fused step, {args.scatter} scatter, hipGraph replay, timed from protocol step {start}"
+                        + (" (end of the 3 um indentation)" if start == move_steps and args.workload == "large" else ""),
             "nodes": N,
             "tetra": nT,
             "scatter": args.scatter,
             "tile_k": sim.tile_k,
             "stream_nt": sim.stream_nt,
+            "timed_from_step": start,
+            "protocol_steps": int(sim.Nsteps),
             "shape_from_ref": geo,
             "viscoplastic_state": ("compact: symmetric deviation Gd" + ("" if sim.plan().epsbar else ", no plastic strain")
                                    if compact else "Q = Fp^-1 - I and plastic strain") if vp else None,
@@ -468,6 +499,7 @@ def main():
             "traffic_source": traffic_source,
             "algorithmic_bytes_per_launch": force_bytes,
             "kernel_bytes_per_launch": kernel_bytes,
+            "plastic_tile_fraction": plastic_tiles,
             "ms_per_launch": ms_force,
         },
     }
@@ -485,9 +517,11 @@ def main():
             out["other_scatter"].append({"scatter": other, "ms_per_step": ms2, "value": nT * 1e3 / ms2,
                                          "ms_force_pass": f2, "ms_node_pass": n2})
             del s2
-        if vp and args.scatter == "tiled":  # the other viscoplastic state form, same tile shape
+        if vp and args.scatter == "tiled":  # the other viscoplastic state form, same tile shape and window
             s4 = make_sim(cfg, P, T, "tiled", material, sim_tile_k, True, not compact)
             s4.stream_nt = sim_stream_nt
+            if start > 0:
+                s4.advance(start)
             ms4 = time_steps(s4, args.warmup, min(args.steps, 200))[0] * 1e3
             f4, n4 = pass_times(s4, 50)
             out["other_state_form"] = {"viscoplastic_state": "Q = Fp^-1 - I and plastic strain" if compact else
@@ -501,6 +535,21 @@ def main():
         out["plain_small"] = {"workload": f"small_cell {len(Ps)} nodes {len(Ts)} tetra NH", "ms_per_step": ms3,
                               "value": len(Ts) * 1e3 / ms3, "unit": "tetra-updates/s"}
         del s3
+    if not args.no_extra and not args.no_full_protocol and args.scatter == "tiled":
+        # the whole protocol from rest on a fresh cell, graph captures included (one per distinct
+        # block length, milliseconds against seconds)
+        s5 = make_sim(cfg, P, T, "tiled", material, sim_tile_k, not args.streamed_shape, compact)
+        s5.stream_nt = sim_stream_nt
+        n5 = int(s5.Nsteps)
+        sync(s5)
+        t0 = time.perf_counter()
+        s5.advance(n5)
+        sync(s5)
+        dt5 = time.perf_counter() - t0
+        out["full_protocol"] = {"steps": n5, "seconds": dt5, "ms_per_step": dt5 / n5 * 1e3,
+                                "value": nT * n5 / dt5, "unit": "tetra-updates/s",
+                                "finite": bool(np.isfinite(s5.positions()).all())}
+        del s5
     if not args.no_extra:
         sc = stream_copy_gbs()
         out["stream_copy"] = sc

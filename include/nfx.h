@@ -146,6 +146,13 @@ typedef struct {
                                dminv and v0: 40 B per tetra less traffic for ~60 FLOPs.  Both describe the
                                same reference shape; they differ by FP32 rounding (~1e-7 relative).
                                dminv must still be set (the other scatter modes read it) */
+    uint32_t* vp_active;    /* NFX_VISCOPLASTIC with NFX_SCATTER_TILED, nullable: per tile (ceil(T/tile_k)), 0 =
+                               every tetra of the tile has a zero plastic state (vp_state and epsbar).  The
+                               force pass then reads no state for the tile and sets the flag once one of its
+                               tetra flows.  A caller that writes vp_state or epsbar itself recomputes the
+                               flags with nfx_vp_tile_flags.  Independently of the flags, a tetra writes its
+                               state back only in a step in which it flows (the state is unchanged otherwise).
+                               Results are bitwise those without flags */
 } nfx_plan;
 
 /* Per-cell parameters of a batch, in that cell's simulation units.  The batched material is the
@@ -237,6 +244,9 @@ int nfx_viscoplastic_force(nfx_ctx* ctx, const nfx_viscoplastic* vp, const float
  * Interactions.py:279-281), 9T SoA FP32 each */
 int nfx_vp_q_from_fp(nfx_ctx* ctx, const float* Fp, int64_t T, float* Q);
 int nfx_vp_fp_from_q(nfx_ctx* ctx, const float* Q, int64_t T, float* Fp);
+/* plan.vp_active from plan.vp_state and plan.epsbar: 1 for a tile with any nonzero entry (no
+ * reference counterpart: the reference updates every tetra's state every step) */
+int nfx_vp_tile_flags(nfx_ctx* ctx, const nfx_plan* plan);
 /* absolute positions x = X0 + u (3N SoA) from displacements, and back (no reference counterpart:
  * the reference keeps absolute FP64 positions) */
 int nfx_points_from_disp(nfx_ctx* ctx, const float* ref, const float* disp, int64_t N, float* points);

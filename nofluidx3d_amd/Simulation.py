@@ -135,6 +135,9 @@ class Simulation:
         self._csr = None
         self._tiles = None
         self._tip = None
+        self._vp_active = None  # per-tile plastic-activity flags of the tiled viscoplastic step
+        self._vp_flags_stale = True
+        self.use_plastic_flags = True
 
     # ---------------------------------------------------------------------------------------------
     # the two position forms
@@ -508,7 +511,15 @@ class Simulation:
             self._use_disp()
             for inter in self.interactions:
                 inter.state_form(fused=True)
+            if p.vp_active and self._vp_flags_stale:
+                _lib.call("nfx_vp_tile_flags", openCL.handle(), p)
+                self._vp_flags_stale = False
         return p
+
+    def plastic_state_changed(self):
+        """The viscoplastic state was written outside the fused tiled step (a conversion, a restore,
+        a caller's own write): the per-tile activity flags are recomputed before the next fused step."""
+        self._vp_flags_stale = True
 
     def sync_forms(self):
         """Bring every state representation up to date (before a snapshot of the state)."""
@@ -519,6 +530,7 @@ class Simulation:
 
     def forms_restored(self):
         self._pos_current = {"points", "disp"}
+        self._vp_flags_stale = True
         for inter in self.interactions:
             inter.forms_restored()
 
@@ -554,6 +566,14 @@ class Simulation:
             p.tile_k, p.tile_max_nodes = t["tile_k"], t["max_nodes"]
             p.stream_nt = default_stream_nt(self.numTetra) if self.stream_nt is None else int(self.stream_nt)
             p.shape_from_ref = 1 if self.shape_from_ref else 0
+            if p.model == _lib.NFX_VISCOPLASTIC and self.use_plastic_flags:
+                import torch
+
+                ntiles = -(-self.numTetra // t["tile_k"])
+                if self._vp_active is None or self._vp_active.numel() != ntiles:
+                    self._vp_active = torch.ones(ntiles, dtype=torch.int32, device=self.device)
+                p.vp_active = self._vp_active.data_ptr()
+                self._vp_flags_stale = True
             for k in ("tile_count", "tile_nodes", "tile_lend", "tile_corners", "tile_dst", "node_off", "partials"):
                 setattr(p, k, t[k].data_ptr())
         else:
@@ -607,6 +627,36 @@ class Simulation:
         getCommandQueue().finish()
         out = self._tip.cpu().numpy()
         return float(out[0]), float(out[1])
+
+    def save_state(self, path):
+        """Checkpoint to an ``.npz`` file: the step counter and every state tensor (node positions,
+        displacements, forces, stress outputs, each plugin's state).  ``load_state`` restores it into
+        a Simulation built the same way (same config, mesh and plugin chain)."""
+        import torch
+
+        self.sync_forms()
+        openCL.getCommandQueue().finish()
+        torch.cuda.synchronize(self.device)
+        arrays = {f"t{i}": t.detach().cpu().numpy() for i, t in enumerate(self.state_tensors())}
+        np.savez(path, time=np.int64(self.time), count=np.int64(len(arrays)), **arrays)
+
+    def load_state(self, path):
+        import torch
+
+        d = np.load(path)
+        tensors = self.state_tensors()
+        if int(d["count"]) != len(tensors):
+            raise ValueError(f"{path}: {int(d['count'])} state tensors, this simulation has {len(tensors)}")
+        for i, t in enumerate(tensors):
+            a = d[f"t{i}"]
+            if a.shape != tuple(t.shape) or a.dtype != torch.empty(0, dtype=t.dtype).numpy().dtype:
+                raise ValueError(f"{path}: tensor {i} is {a.dtype}{a.shape}, expected {tuple(t.shape)}")
+        openCL.getCommandQueue().finish()
+        for i, t in enumerate(tensors):
+            t.copy_(torch.from_numpy(d[f"t{i}"]))
+        torch.cuda.synchronize(self.device)
+        self.forms_restored()
+        self.time = int(d["time"])
 
     def state_tensors(self):
         """Device tensors that carry the simulation state (positions, force, plugin state); call

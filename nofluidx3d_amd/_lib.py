@@ -98,6 +98,7 @@ class Plan(ctypes.Structure):
         ("vp_sym", c_i32),
         ("stream_nt", c_i32),
         ("shape_from_ref", c_i32),
+        ("vp_active", c_vp),
     ]
 
 
@@ -169,6 +170,7 @@ SIGNATURES = {
     "nfx_gather_corners": (ctypes.c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_vp_q_from_fp": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
     "nfx_vp_fp_from_q": (ctypes.c_int, [c_vp, c_vp, c_i64, c_vp]),
+    "nfx_vp_tile_flags": (ctypes.c_int, [c_vp, ctypes.POINTER(Plan)]),
     "nfx_points_from_disp": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_disp_from_points": (ctypes.c_int, [c_vp, c_vp, c_vp, c_i64, c_vp]),
     "nfx_graph_captures": (ctypes.c_int, [c_vp, ctypes.POINTER(c_i64)]),

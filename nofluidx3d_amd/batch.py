@@ -397,6 +397,8 @@ class BatchSimulation:
         self.t_count, self.t_nodes, self.t_lend = u32(h["tile_count"]), u32(h["tile_nodes"]), u16(h["tile_lend"])
         self.t_corners, self.t_dst = u16(h["tile_corners"]), u32(h["tile_dst"])
         self.t_ntet, self.t_cell = u16(h["tile_ntet"]), u32(h["tile_cell"])
+        # per-tile plastic-activity flags: the state starts at zero everywhere
+        self.t_active = torch.zeros(len(h["tile_ntet"]), dtype=torch.int32, device=dev) if self.state is not None else None
         # 16-bit tile-node metadata (absent when a cell has 65536 nodes or more)
         self.t_narrow = (u16(h["tile_nodes16"]), u16(h["tile_dst16"]), u32(h["tile_node_base"]),
                          u32(h["tile_dst_base"])) if narrow and "tile_nodes16" in h else None
@@ -425,6 +427,9 @@ class BatchSimulation:
             p.epsbar = self.eps.data_ptr() if self.eps is not None else 0
             p.vp.flow_exponent, p.vp.hardening_exponent = 1.0, 0.0  # every cell: linear Maxwell flow
             p.vp_sym = 1 if self.sym_state else 0
+            # the flags pay where cells have a yield offset (tiles below it never flow); with none,
+            # every stressed tetra flows from the first steps and the flags are overhead
+            p.vp_active = self.t_active.data_ptr() if (self.cells_host["vp_offset"] > 0).any() else 0
         p.stream_nt = default_stream_nt(self.T) if stream_nt is None else int(stream_nt)
         self.shape_from_ref = bool(shape_from_ref)
         p.shape_from_ref = 1 if shape_from_ref else 0
@@ -475,6 +480,7 @@ class BatchSimulation:
         _lib.call("nfx_stream_sync", hh)
         if self.state is not None:
             self.state.zero_()
+            self.t_active.zero_()
         self.x.zero_()
         self.v.zero_()
         self.fold.zero_()

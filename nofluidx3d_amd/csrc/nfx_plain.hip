@@ -305,6 +305,23 @@ __global__ __launch_bounds__(kBlock) void k_vp_convert(const float* __restrict__
         for (int j = 0; j < 3; ++j) out[(3 * i + j) * T + t] = Y.m[i][j] - ((TO_Q && i == j) ? 1.0f : 0.0f);
 }
 
+// The plastic-activity flag of each tile from the state itself: 1 when any of its tetra has a
+// nonzero plastic deviation or plastic strain.  One workgroup per tile of K tetra.
+__global__ __launch_bounds__(kBlock) void k_vp_tile_flags(const float* __restrict__ state, int planes,
+                                                          const float* __restrict__ eps, int64_t T, int K,
+                                                          uint32_t* __restrict__ flags) {
+    const int64_t t0 = (int64_t)blockIdx.x * K;
+    int nz = 0;
+    for (int i = threadIdx.x; i < K; i += kBlock) {
+        const int64_t t = t0 + i;
+        if (t >= T) break;
+        for (int k = 0; k < planes; ++k) nz |= state[k * T + t] != 0.0f;
+        if (eps) nz |= eps[t] != 0.0f;
+    }
+    nz = __syncthreads_or(nz);
+    if (threadIdx.x == 0) flags[blockIdx.x] = nz ? 1u : 0u;
+}
+
 // x = X0 + u (TO_POINTS) or u = x - X0, component-wise over 3N
 template <bool TO_POINTS>
 __global__ __launch_bounds__(kBlock) void k_disp_convert(const float* __restrict__ ref, const float* __restrict__ in,
@@ -327,6 +344,10 @@ __global__ __launch_bounds__(kBlock) void k_disp_convert(const float* __restrict
 //   2  (GEO: Dm^-1 and V0 from the reference edges) material law on LDS displacements; each corner
 //      force -> LDS at its place in the sorted list
 //   3  one thread per tile-node: sum its contiguous run -> partials[tile_dst]
+// Viscoplastic state: a tetra writes its state back only when it flows this step (otherwise the
+// update is the identity and memory already holds it), and a tile whose vp_active flag is 0 reads
+// none (its tetra have never flowed: Q = 0, epsbar = 0, which memory also holds).  Far from the
+// contact, below the yield offset, a tile costs its corner records and node data only.
 // GEO trades the 36 + 4 B per tetra of streamed Dm^-1 and V0 for 12 B per tile-node of reference
 // positions (~4 B per tetra) and ~60 FLOPs per tetra.
 // The node pass sums each node's contiguous partials in tile order: no atomics, fixed order.
@@ -343,6 +364,9 @@ struct TileArgs {
     const uint32_t* __restrict__ tile_dst;
     float* __restrict__ partials;
     int32_t max_nodes;
+    // viscoplastic, nullable: per tile, 0 = every tetra of the tile has a zero plastic state (its
+    // state is neither read nor written until one of them flows; the tile then sets its flag)
+    uint32_t* __restrict__ vp_active;
     // batches: cell-aligned tiles with per-cell material constants (NULL for a single cell)
     const uint16_t* __restrict__ tile_ntet;
     const uint32_t* __restrict__ tile_cell;
@@ -454,6 +478,11 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
     const int tl = threadIdx.x;
     const bool active = tl < nt;
     const uint32_t u = (uint32_t)tl;
+    // the tile's plastic-activity flag (uniform): without it every tile reads its state
+    bool plastic = true;
+    if constexpr (MODEL == NFX_VISCOPLASTIC) {
+        if (a.vp_active) plastic = a.vp_active[c] != 0u;
+    }
 
     // issue this tetra's streaming loads first: they overlap the staging phases below
     float D[9], Sv[MODEL == NFX_VISCOPLASTIC ? 9 : 1], e0 = 0.f, v0 = 0.f, sdm = 0.f;
@@ -463,9 +492,15 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
             for (int k = 0; k < 9; ++k) D[k] = ld_g<NT, float>(uat(a.dminv + t0 + k * T, u));
         }
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
+            if (plastic) {
 #pragma unroll
-            for (int k = 0; k < (SYM ? 6 : 9); ++k) Sv[k] = ld_g<NT, float>(uat((const float*)(Sst + t0 + k * T), u));
-            if constexpr (EPS) e0 = ld_g<NT, float>(uat((const float*)(eps + t0), u));
+                for (int k = 0; k < (SYM ? 6 : 9); ++k)
+                    Sv[k] = ld_g<NT, float>(uat((const float*)(Sst + t0 + k * T), u));
+                if constexpr (EPS) e0 = ld_g<NT, float>(uat((const float*)(eps + t0), u));
+            } else {
+#pragma unroll
+                for (int k = 0; k < (SYM ? 6 : 9); ++k) Sv[k] = 0.0f;
+            }
         } else if constexpr (MODEL < NFX_VISCOPLASTIC && !GEO) {
             v0 = ld_g<NT, float>(uat(a.v0 + t0, u));
         }
@@ -503,6 +538,7 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         s_lb[tl] = (uint16_t)lb;
     }
     __syncthreads();
+    int flows = 0;
     if (active) {
         const int l0 = (int)(cr & 0xFF), l1 = (int)((cr >> 16) & 0xFF), l2 = (int)((cr >> 32) & 0xFF),
                   l3 = (int)((cr >> 48) & 0xFF);
@@ -526,9 +562,12 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
             VPOut o;
             if constexpr (SYM) o = vp_tetra_gd(H, D, sdm, Sv, e0, P);
             else o = vp_tetra_q(H, D, sdm, Sv, e0, P);
+            if (o.flows) {
+                flows = 1;
 #pragma unroll
-            for (int k = 0; k < (SYM ? 6 : 9); ++k) st_g<NT, float>(uat(Sst + t0 + k * T, u), o.Fp[k]);
-            if constexpr (EPS) st_g<NT, float>(uat(eps + t0, u), o.epsbar);
+                for (int k = 0; k < (SYM ? 6 : 9); ++k) st_g<NT, float>(uat(Sst + t0 + k * T, u), o.Fp[k]);
+                if constexpr (EPS) st_g<NT, float>(uat(eps + t0, u), o.epsbar);
+            }
             if constexpr (STRESS) {
                 st_s<NT>(vm + t0 + u, o.von_mises);
                 st_s<NT>(pr + t0 + u, o.pressure);
@@ -556,7 +595,12 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         s_cf[q2] = f.f2.x; s_cf[4 * K + q2] = f.f2.y; s_cf[8 * K + q2] = f.f2.z;
         s_cf[q3] = f.f3.x; s_cf[4 * K + q3] = f.f3.y; s_cf[8 * K + q3] = f.f3.z;
     }
-    __syncthreads();
+    if (MODEL == NFX_VISCOPLASTIC && a.vp_active) {  // uniform
+        const int any = __syncthreads_or(flows);
+        if (!plastic && any && tl == 0) a.vp_active[c] = 1u;  // sticky
+    } else {
+        __syncthreads();
+    }
     if (tl < M) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
         for (int q = lb; q < le; ++q) {  // the run in slot order
@@ -917,6 +961,8 @@ static int validate_plan(const nfx_plan* p) {
     NFX_REQUIRE(p->model != NFX_VISCOPLASTIC || p->vp_state, "plan: viscoplastic state vp_state missing");
     NFX_REQUIRE(!p->vp_sym || (p->model == NFX_VISCOPLASTIC && p->scatter == NFX_SCATTER_TILED),
                 "plan: the symmetric state (vp_sym) is for the tiled viscoplastic force only");
+    NFX_REQUIRE(!p->vp_active || (p->model == NFX_VISCOPLASTIC && p->scatter == NFX_SCATTER_TILED),
+                "plan: the plastic-activity flags (vp_active) are for the tiled viscoplastic force only");
     NFX_REQUIRE(p->scatter >= NFX_SCATTER_ATOMIC && p->scatter <= NFX_SCATTER_TILED, "plan: unknown scatter %d",
                 p->scatter);
     if (p->scatter == NFX_SCATTER_TILED) {
@@ -982,6 +1028,7 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     TileArgs a{p.N,          p.T,         p.dminv,        p.v0,       p.disp,     p.ref,
                p.tile_count, p.tile_nodes, p.tile_lend,    p.tile_corners, p.tile_dst, p.partials,
                p.tile_max_nodes};
+    if (p.model == NFX_VISCOPLASTIC) a.vp_active = p.vp_active;
     if (ctx->batch) {
         a.tile_ntet = ctx->batch->tile_ntet;
         a.tile_cell = ctx->batch->tile_cell;
@@ -1528,6 +1575,19 @@ int nfx_vp_fp_from_q(nfx_ctx* c, const float* Q, int64_t T, float* Fp) {
     NFX_REQUIRE(c && Q && Fp, "NULL argument");
     NFX_REQUIRE(T > 0 && T < (int64_t(1) << 30), "tetra count %lld out of range", (long long)T);
     k_vp_convert<false><<<nblocks(T), kBlock, 0, c->stream>>>(Q, T, Fp);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
+int nfx_vp_tile_flags(nfx_ctx* c, const nfx_plan* p) {
+    NFX_REQUIRE(c && p, "NULL argument");
+    const int rc = validate_plan(p);
+    if (rc) return rc;
+    NFX_REQUIRE(p->model == NFX_VISCOPLASTIC && p->scatter == NFX_SCATTER_TILED && p->vp_active,
+                "plan: plastic-activity flags need the tiled viscoplastic force and vp_active");
+    const int64_t ntiles = (p->T + p->tile_k - 1) / p->tile_k;
+    k_vp_tile_flags<<<(unsigned int)ntiles, kBlock, 0, c->stream>>>(p->vp_state, p->vp_sym ? 6 : 9, p->epsbar, p->T,
+                                                                     p->tile_k, p->vp_active);
     NFX_CHECK_LAUNCH();
     return NFX_OK;
 }

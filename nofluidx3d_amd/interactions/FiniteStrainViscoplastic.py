@@ -96,6 +96,7 @@ class FiniteStrainViscoplastic(Interaction):
         if want == "q":
             _lib.call("nfx_vp_q_from_fp", openCL.handle(), self.plasticDeformationTensorB.data_ptr(), T,
                       self.plasticDeviationB.data_ptr())
+            self.simulation.plastic_state_changed()
         else:
             _lib.call("nfx_vp_fp_from_q", openCL.handle(), self.plasticDeviationB.data_ptr(), T,
                       self.plasticDeformationTensorB.data_ptr())

@@ -196,6 +196,16 @@ def read_parts(prefixes):
     return X, Y
 
 
+def read_metas(prefixes):
+    out = []
+    for p in prefixes:
+        try:
+            out.append(json.load(open(p + "_meta.json")))
+        except (OSError, ValueError):
+            out.append(None)
+    return out
+
+
 def main(argv=None):
     """``all`` (default): generate, train and evaluate in one run.  For data sets larger than one
     run can generate, ``generate`` writes slices (``--first-chunk``/``--chunks`` of a pool) and
@@ -215,6 +225,8 @@ def main(argv=None):
     ap.add_argument("--tile-k", type=int, default=256)
     ap.add_argument("--train", nargs="*", default=[], help="fit: training slice prefixes")
     ap.add_argument("--heldout", nargs="*", default=[], help="fit: held-out slice prefixes (new cells)")
+    ap.add_argument("--heldout-physics", nargs="*", default=[],
+                    help="fit: held-out slice prefixes (new physics on the training pool's meshes)")
     args = ap.parse_args(argv)
     log = lambda *a, **k: print(*a, **k, file=sys.stderr)  # noqa: E731
     t0 = time.time()
@@ -234,11 +246,18 @@ def main(argv=None):
         t1 = time.time()
         net, norm, hist = train(Xt, Yt, iters=args.iters, log=log, Xv=Xn[:split], Yv=Yn[:split])
         save(os.path.join(args.out, "surrogate.npz"), net, norm)
+        metas = read_metas(args.train + args.heldout + args.heldout_physics)
         rep = {"train_curves": int(len(Xt)), "train_slices": args.train, "heldout_slices": args.heldout,
-               "training_s": time.time() - t1, "iters": args.iters, "history": hist,
+               "heldout_physics_slices": args.heldout_physics,
+               "data_generation_s": float(sum(m.get("seconds", 0.0) for m in metas if m)),
+               "slices": metas, "training_s": time.time() - t1, "iters": args.iters, "history": hist,
                "heldout_cells_rel_l2": error_summary(curve_errors(net, norm, Xn, Yn))}
+        if args.heldout_physics:
+            Xp, Yp = read_parts(args.heldout_physics)
+            rep["heldout_physics_rel_l2"] = error_summary(curve_errors(net, norm, Xp, Yp))
         json.dump(rep, open(os.path.join(args.out, "report.json"), "w"), indent=1)
-        print(json.dumps({"train_curves": rep["train_curves"], "heldout_cells_rel_l2": rep["heldout_cells_rel_l2"]}))
+        print(json.dumps({k: rep[k] for k in ("train_curves", "heldout_cells_rel_l2", "heldout_physics_rel_l2")
+                          if k in rep}))
         return
     pool, bs = _population(args.pool, args.mesh_seed0, args.steps, args.tile_k)
     Xt, Yt = generate(bs, pool, args.chunks, seed0=args.seed0, log=log)

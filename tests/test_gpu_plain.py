@@ -322,7 +322,7 @@ LEGACY_SETTERS = {"LE": "setInteractionLinearElastic", "LD": "setInteractionLine
 
 
 def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, mesh=(4.0, 0.8, 1), material="NH",
-              indenter="sphere", steps=400, compact=False):
+              indenter="sphere", steps=400, compact=False, yield_si=5.0):
     from nofluidx3d_amd import configs
     from nofluidx3d_amd.Simulation import Simulation
     from nofluidx3d_amd.util import vtk
@@ -336,7 +336,7 @@ def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, 
     cfg.update(cfg_overrides or {})
     if material == "VP":
         cfg["CELL"].update({"MooneyRivlinRatio": 0.75, "model": "MOONEY_RIVLIN", "ViscositySI": 10.0,
-                            "YieldStressSI": 5.0})
+                            "YieldStressSI": yield_si})
     P, T = configs.make_mesh(cfg)
     tile_k = 256 if scatter == "tiled256" else 512
     scatter = "tiled" if scatter == "tiled256" else scatter
@@ -664,6 +664,57 @@ def test_compact_viscoplastic_state_matches_oracle_and_full_state(ctx):
     eager = build_sim(material="VP", scatter="tiled", compact=True, fused=False)
     with pytest.raises(RuntimeError, match="compact"):
         eager.timeStep()
+
+
+def test_plastic_activity_flags_change_no_bit(ctx):
+    """The tiled viscoplastic step with per-tile plastic-activity flags (tiles whose tetra never
+    flowed read and write no state) against the same step without flags: bitwise equal, with both
+    kinds of tile present; the flags the kernel kept equal those recomputed from the state."""
+    import torch
+
+    from nofluidx3d_amd import _lib, openCL
+
+    runs = []
+    for flags in (True, False):
+        sim = build_sim(material="VP", scatter="tiled256", mesh=(6.0, 0.6, 1), yield_si=60.0)
+        sim.use_plastic_flags = flags
+        runs.append(run_gpu(sim, 600))
+        assert bool(sim.plan().vp_active) == flags
+        if flags:
+            kept = host(sim._vp_active, np.int64)
+            _lib.call("nfx_vp_tile_flags", openCL.handle(), sim.plan())
+            assert np.array_equal(host(sim._vp_active, np.int64), kept)
+            print(f"{int(kept.sum())} of {len(kept)} tiles have flowed")
+            assert 0 < kept.sum() < len(kept), kept
+            q = host(sim.interactions[2].plasticDeviationB).reshape(9, -1)
+            assert np.any(q != 0)
+    (xa, fa), (xb, fb) = runs
+    assert np.array_equal(xa, xb) and np.array_equal(fa, fb)
+    torch.cuda.synchronize()
+
+
+def test_checkpoint_restart_is_bitwise(ctx, tmp_path):
+    """save_state after 300 steps, then 200 more steps, against a fresh Simulation that loads the
+    checkpoint and runs the same 200: bitwise equal positions, tip force and plastic state (the
+    restored run recomputes its plastic-activity flags)."""
+    import torch
+
+    a = build_sim(material="VP", scatter="tiled")
+    run_gpu(a, 300)
+    path = str(tmp_path / "state.npz")
+    a.save_state(path)
+    xa, fa = run_gpu(a, 200)
+    b = build_sim(material="VP", scatter="tiled")
+    b.load_state(path)
+    assert b.time == 300
+    xb, fb = run_gpu(b, 200)
+    assert np.array_equal(xa, xb) and np.array_equal(fa, fb)
+    qa, qb = (host(s.interactions[2].plastic_deformation()) for s in (a, b))
+    assert np.array_equal(qa, qb)
+    c = build_sim(material="NH", scatter="tiled")
+    with pytest.raises(ValueError):
+        c.load_state(path)
+    torch.cuda.synchronize()
 
 
 def test_device_selection_before_any_torch_use(ctx):
