@@ -127,6 +127,16 @@ __device__ __forceinline__ float dminv_from_ref(V3 r1, V3 r2, V3 r3, float* D) {
     D[6] = g3.x * id; D[7] = g3.y * id; D[8] = g3.z * id;
     return det;
 }
+// The same rows unscaled, G = det(Dm) Dm^-1 = rows (R2 x R3, R3 x R1, R1 x R2); returns det.  The laws
+// need Dm^-1 only as H = dU Dm^-1 = (dU G) / det and as the edge cofactors c_k = cof(F) g_k, so a
+// caller holding G scales H once and passes G with scale 1 (no division of the nine entries).
+__device__ __forceinline__ float shape_rows(V3 r1, V3 r2, V3 r3, float* G) {
+    const V3 g1 = cross(r2, r3), g2 = cross(r3, r1), g3 = cross(r1, r2);
+    G[0] = g1.x; G[1] = g1.y; G[2] = g1.z;
+    G[3] = g2.x; G[4] = g2.y; G[5] = g2.z;
+    G[6] = g3.x; G[7] = g3.y; G[8] = g3.z;
+    return dot(r1, g1);
+}
 
 // H = F - I from the four node displacements (the fused and batched steps)
 __device__ __forceinline__ M9 grad_disp(V3 u0, V3 u1, V3 u2, V3 u3, const float* D) {
@@ -208,6 +218,7 @@ __device__ __forceinline__ float det_dm(const float* D) {
 struct Cof3 {
     V3 c1, c2, c3;
 };
+// c_k = C (s d_k) for rows d_k of D (D = Dm^-1 with s = det Dm, or D = G with s = 1)
 __device__ __forceinline__ Cof3 edge_cofactors(const M9& C, const float* D, float s) {
     Cof3 o;
     V3* c[3] = {&o.c1, &o.c2, &o.c3};
@@ -613,8 +624,9 @@ __device__ __forceinline__ VPOut vp_tetra(const M9& H, const float* D, const flo
 // as Q it keeps its own relative precision.
 // s = det Dm (= 6 V0).  A tetra that does not flow keeps Q bit for bit; a wave whose tetra all do not
 // flow (the far field under a yield offset) skips the update.
-__device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, float s, const float* Q, float epsbar,
-                                            const VPConst& P) {
+// D with scale gs gives the edge cofactors (D = Dm^-1, gs = s; or D = G, gs = 1); s = det Dm.
+__device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, float gs, float s, const float* Q,
+                                            float epsbar, const VPConst& P) {
     VPOut o;
     M9 He;
 #pragma unroll
@@ -625,9 +637,11 @@ __device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, float s
                          (H.m[i][0] * Q[0 * 3 + j] + H.m[i][1] * Q[1 * 3 + j] + H.m[i][2] * Q[2 * 3 + j]);
     S6 sig;
     const S6 Dp = vp_stress_flow(cauchy_green_m1(He), epsbar, P, o, sig);
-    const float Jm1 = det_ipH_m1(H);
     const M9 C = cof_ipH(H);
-    o.f = cofactor_forces(sig, edge_cofactors(C, D, s), s * (1.0f + Jm1) < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
+    // det F from the cofactor row the forces need anyway: only its sign and 1/det F enter below (the
+    // elastic volume change is taken from det Be), so the cancellation in det F - 1 does not matter
+    const float J = (1.0f + H.m[0][0]) * C.m[0][0] + H.m[0][1] * C.m[0][1] + H.m[0][2] * C.m[0][2];
+    o.f = cofactor_forces(sig, edge_cofactors(C, D, gs), s * J < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
     if (!o.flows) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) o.Fp[k] = Q[k];
@@ -646,7 +660,7 @@ __device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, float s
         W.m[2][j] = (j == 0 ? N.xz : j == 1 ? N.yz : N.zz) + nh.z;
     }
     // Q' = Q - C^T W / det F
-    const float idet = frcp(1.0f + Jm1);
+    const float idet = frcp(J);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const V3 ci = mk(C.m[0][i], C.m[1][i], C.m[2][i]);  // column i of C = row i of C^T
@@ -665,8 +679,8 @@ __device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, float s
 //     Gd' = Gd + F^-1 X F^-T,   X = -N Be - Be N + N Be N = (N^2 - 2N) Be,
 // (N is a function of the deviatoric stress of an isotropic law, a polynomial in Be, so it commutes
 // with Be), an increment that is zero when nothing flows.  Fp is not recoverable from Gd.
-__device__ __forceinline__ VPOut vp_tetra_gd(const M9& H, const float* D, float s, const float* G6, float epsbar,
-                                             const VPConst& P) {
+__device__ __forceinline__ VPOut vp_tetra_gd(const M9& H, const float* D, float gs, float s, const float* G6,
+                                             float epsbar, const VPConst& P) {
     VPOut o;
     const S6 G{G6[0], G6[1], G6[2], G6[3], G6[4], G6[5]};
     const M9 F = ipH(H);
@@ -682,9 +696,11 @@ __device__ __forceinline__ VPOut vp_tetra_gd(const M9& H, const float* D, float 
     Bed.xz += dot(tx, fz);
     S6 sig;
     const S6 Dp = vp_stress_flow(Bed, epsbar, P, o, sig);
-    const float Jm1 = det_ipH_m1(H);
     const M9 C = cof_ipH(H);
-    o.f = cofactor_forces(sig, edge_cofactors(C, D, s), s * (1.0f + Jm1) < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
+    // det F from the cofactor row the forces need anyway: only its sign and 1/det F enter below (the
+    // elastic volume change is taken from det Be), so the cancellation in det F - 1 does not matter
+    const float J = (1.0f + H.m[0][0]) * C.m[0][0] + H.m[0][1] * C.m[0][1] + H.m[0][2] * C.m[0][2];
+    o.f = cofactor_forces(sig, edge_cofactors(C, D, gs), s * J < 0.0f ? (1.0f / 6.0f) : (-1.0f / 6.0f));
     if (!o.flows) {
 #pragma unroll
         for (int k = 0; k < 6; ++k) o.Fp[k] = G6[k];
@@ -700,7 +716,7 @@ __device__ __forceinline__ VPOut vp_tetra_gd(const M9& H, const float* D, float 
     // F^-1 X F^-T = C^T X C / det F^2, with the columns k_i of C
     const V3 k0 = mk(C.m[0][0], C.m[1][0], C.m[2][0]), k1 = mk(C.m[0][1], C.m[1][1], C.m[2][1]),
              k2 = mk(C.m[0][2], C.m[1][2], C.m[2][2]);
-    const float idet = frcp(1.0f + Jm1);
+    const float idet = frcp(J);
     const float id2 = idet * idet;
     const V3 u0 = smul(X, k0), u1 = smul(X, k1), u2 = smul(X, k2);
     o.Fp[0] = G.xx + dot(k0, u0) * id2;

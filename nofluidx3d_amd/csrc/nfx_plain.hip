@@ -276,7 +276,9 @@ __global__ __launch_bounds__(kBlock) void k_vp(TetArgs a, VPConst P, float* __re
     for (int k = 0; k < 9; ++k) st[k] = state[k * T + t];
     const float e0 = eps ? eps[t] : 0.0f;
     const M9 H = tetra_grad<FUSED>(a, n0, n1, n2, n3, D);
-    const VPOut o = FUSED ? vp_tetra_q(H, D, det_dm(D), st, e0, P) : vp_tetra(H, D, st, e0, P);
+    float sdm = 0.f;
+    if constexpr (FUSED) sdm = det_dm(D);
+    const VPOut o = FUSED ? vp_tetra_q(H, D, sdm, sdm, st, e0, P) : vp_tetra(H, D, st, e0, P);
 #pragma unroll
     for (int k = 0; k < 9; ++k) state[k * T + t] = o.Fp[k];
     if (eps) eps[t] = o.epsbar;
@@ -542,13 +544,20 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
     if (active) {
         const int l0 = (int)(cr & 0xFF), l1 = (int)((cr >> 16) & 0xFF), l2 = (int)((cr >> 32) & 0xFF),
                   l3 = (int)((cr >> 48) & 0xFF);
+        // ROWS: D holds the unscaled rows G = det(Dm) Dm^-1 (the laws whose forces are linear in
+        // V0 Dm^-1 and whose cofactors are cof(F) g_k); the legacy laws take Dm^-1 itself
+        constexpr bool ROWS = GEO && MODEL <= NFX_VISCOPLASTIC;
         if constexpr (GEO) {
             const V3 r0 = mk(s_r[l0], s_r[MX + l0], s_r[2 * MX + l0]);
             const V3 r1 = mk(s_r[l1], s_r[MX + l1], s_r[2 * MX + l1]);
             const V3 r2 = mk(s_r[l2], s_r[MX + l2], s_r[2 * MX + l2]);
             const V3 r3 = mk(s_r[l3], s_r[MX + l3], s_r[2 * MX + l3]);
-            sdm = dminv_from_ref(r1 - r0, r2 - r0, r3 - r0, D);
-            v0 = sdm * (1.0f / 6.0f);
+            if constexpr (ROWS) {
+                sdm = shape_rows(r1 - r0, r2 - r0, r3 - r0, D);
+            } else {
+                sdm = dminv_from_ref(r1 - r0, r2 - r0, r3 - r0, D);
+                v0 = sdm * (1.0f / 6.0f);
+            }
         } else if constexpr (MODEL == NFX_VISCOPLASTIC) {
             sdm = det_dm(D);
         }
@@ -556,12 +565,22 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         const V3 u1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
         const V3 u2 = mk(s_x[l2], s_x[MX + l2], s_x[2 * MX + l2]);
         const V3 u3 = mk(s_x[l3], s_x[MX + l3], s_x[2 * MX + l3]);
-        const M9 H = grad_disp(u0, u1, u2, u3, D);
+        M9 H = grad_disp(u0, u1, u2, u3, D);
+        if constexpr (ROWS) {
+            const float id = frcp(sdm);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) H.m[i][j] *= id;
+        }
+        // with ROWS, V0 Dm^-1 = G / 6 and the edge cofactors take G with scale 1
+        const float vs = ROWS ? (1.0f / 6.0f) : v0;
+        const float gs = ROWS ? 1.0f : sdm;
         Corners f;
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
             VPOut o;
-            if constexpr (SYM) o = vp_tetra_gd(H, D, sdm, Sv, e0, P);
-            else o = vp_tetra_q(H, D, sdm, Sv, e0, P);
+            if constexpr (SYM) o = vp_tetra_gd(H, D, gs, sdm, Sv, e0, P);
+            else o = vp_tetra_q(H, D, gs, sdm, Sv, e0, P);
             if (o.flows) {
                 flows = 1;
 #pragma unroll
@@ -581,11 +600,11 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
                 st_s<NT>(pr + t0 + u, spr);
             }
         } else if constexpr (MODEL == NFX_MR) {
-            f = mr_tetra(H, D, v0, e.mu1, e.mu2, e.kappa);
+            f = mr_tetra(H, D, vs, e.mu1, e.mu2, e.kappa);
         } else if constexpr (MODEL == NFX_SVK) {
-            f = svk_tetra(H, D, v0, e.mu, e.kappa_svk);
+            f = svk_tetra(H, D, vs, e.mu, e.kappa_svk);
         } else {
-            f = svkk_tetra(H, D, v0, e.mu, e.kappa_svk);
+            f = svkk_tetra(H, D, vs, e.mu, e.kappa_svk);
         }
         // each corner's place in the tile's (node, slot)-sorted corner list: run start + rank
         const int q0 = s_lb[l0] + (int)((cr >> 8) & 0xFF), q1 = s_lb[l1] + (int)((cr >> 24) & 0xFF);
