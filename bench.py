@@ -500,6 +500,10 @@ def main():
             "algorithmic_bytes_per_launch": force_bytes,
             "kernel_bytes_per_launch": kernel_bytes,
             "plastic_tile_fraction": plastic_tiles,
+            "note": ("achieved counts the specification's bytes per launch (136 B/tetra + 24 B/node); the kernel "
+                     "forms Dm^-1 from the reference positions and touches the plastic state only in tiles that "
+                     "have yielded and tetra that flow, so it moves far less (traffic, measured) and is bound by "
+                     "VALU issue (profiles/instructions_large_cell_viscoplastic_tiled.txt)") if vp and geo else None,
             "ms_per_launch": ms_force,
         },
     }
