@@ -449,6 +449,7 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
     float* s_cf = reinterpret_cast<float*>(smem);  // [component][4K], sorted-corner order
     const int MX = a.max_nodes;
     float* s_x = s_cf + 12 * K;                    // [component][MX]
+    __shared__ int s_flow;                         // viscoplastic: a tetra of the tile flowed this step
     float* s_r = s_x + 3 * MX;                     // GEO: [component][MX] reference positions
     uint16_t* s_lb = reinterpret_cast<uint16_t*>(s_r + (GEO ? 3 * MX : 0));
 
@@ -539,6 +540,7 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         }
         s_lb[tl] = (uint16_t)lb;
     }
+    if (MODEL == NFX_VISCOPLASTIC && tl == 0) s_flow = 0;
     __syncthreads();
     int flows = 0;
     if (active) {
@@ -614,15 +616,21 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         s_cf[q2] = f.f2.x; s_cf[4 * K + q2] = f.f2.y; s_cf[8 * K + q2] = f.f2.z;
         s_cf[q3] = f.f3.x; s_cf[4 * K + q3] = f.f3.y; s_cf[8 * K + q3] = f.f3.z;
     }
-    if (MODEL == NFX_VISCOPLASTIC && a.vp_active) {  // uniform
-        const int any = __syncthreads_or(flows);
-        if (!plastic && any && tl == 0) a.vp_active[c] = 1u;  // sticky
-    } else {
-        __syncthreads();
-    }
+    if (MODEL == NFX_VISCOPLASTIC && flows) s_flow = 1;  // any flowing lane; all write the same value
+    __syncthreads();
+    if (MODEL == NFX_VISCOPLASTIC && a.vp_active && !plastic && tl == 0 && s_flow) a.vp_active[c] = 1u;  // sticky
     if (tl < M) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        for (int q = lb; q < le; ++q) {  // the run in slot order
+        int q = lb;
+        for (; q + 4 <= le; q += 4) {  // the run in slot order, four corners per trip
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                fx += s_cf[q + i];
+                fy += s_cf[4 * K + q + i];
+                fz += s_cf[8 * K + q + i];
+            }
+        }
+        for (; q < le; ++q) {
             fx += s_cf[q];
             fy += s_cf[4 * K + q];
             fz += s_cf[8 * K + q];
