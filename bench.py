@@ -41,6 +41,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# FP32 vector peak of the VALUs without packed FP32 instructions (which this build does not emit):
+# 256 CUs x 4 SIMDs x 16 lanes x 2 FLOP (FMA) x 2.4 GHz; AMD's 157.3 TFLOP/s figure counts packed FP32
+VALU_FP32_PEAK_TFLOPS = 78.6
 # algorithmic bytes per tetra of one force pass, as the hot-path specification counts them
 SPEC_TET_BYTES = {"viscoplastic": 16 + 36 + 4 + 2 * 36 + 2 * 4, "elastic": 16 + 36 + 4}
 # what the tiled force pass reads and writes per tetra: corner records 8 (instead of the 16 B
@@ -360,6 +363,7 @@ def main():
                     help="the tiled force pass reads the precomputed Dm^-1 and V0 instead of forming them "
                          "from the reference positions")
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "traffic_large_cell.json"))
+    ap.add_argument("--valu-json", default=os.path.join(ROOT, "profiles", "valu_large_cell.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (HBM-counter calibration under rocprofv3)")
     ap.add_argument("--only", default=None, choices=[None, "batched", "surrogate"],
@@ -451,6 +455,17 @@ def main():
         if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}):
             traffic = tj["force_pass_bytes"][key]
             traffic_source = f"{os.path.relpath(args.traffic_json, ROOT)}[{key}]: {tj.get('source', '')}"
+    compute = None
+    if args.valu_json and os.path.exists(args.valu_json):
+        vj = json.load(open(args.valu_json))
+        key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if nt_flag else ''}{'_shape_from_ref' if geo else ''}"
+        if vj.get("tetra") == nT and key in vj.get("force_pass", {}) and start == vj.get("from_step"):
+            fl = vj["force_pass"][key]["fp32_flops"]
+            compute = {"bound": "valu", "flops_per_launch": fl, "achieved": fl / (ms_force * 1e-3) / 1e12,
+                       "peak": VALU_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                       "frac": fl / (ms_force * 1e-3) / 1e12 / VALU_FP32_PEAK_TFLOPS,
+                       "valu_insts_per_wave": vj["force_pass"][key].get("valu_insts_per_wave"),
+                       "source": f"{os.path.relpath(args.valu_json, ROOT)}[{key}]: {vj.get('source', '')}"}
     out = {
         "metric": "tetra-updates/s",
         "value": nT * 1e3 / ms,
@@ -505,6 +520,7 @@ def main():
                      "have yielded and tetra that flow, so it moves far less (traffic, measured) and is bound by "
                      "VALU issue (profiles/instructions_large_cell_viscoplastic_tiled.txt)") if vp and geo else None,
             "ms_per_launch": ms_force,
+            "compute": compute,
         },
     }
     sim_tile_k, sim_stream_nt = sim.tile_k, sim.stream_nt

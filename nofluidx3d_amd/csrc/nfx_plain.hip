@@ -426,13 +426,13 @@ __device__ void batch_curve_sample(const float* __restrict__ x, const float* __r
     }
 }
 
-// LDS of a tile: corner forces 12K floats, then per tile-node its displacement (3 floats), with GEO
-// its reference position (3 floats), and its run start (u16)
+// LDS of a tile: corner forces 12K floats, then per tile-node its displacement and X0.x (16 B), with
+// GEO X0.y and X0.z (8 B), and its run start (u16)
 template <int K>
 __host__ __device__ constexpr size_t tile_lds_bytes_fixed() {
     return 4 * 12 * K;
 }
-__host__ __device__ constexpr size_t tile_lds_bytes_per_node(bool geo) { return 12 + (geo ? 12 : 0) + 2; }
+__host__ __device__ constexpr size_t tile_lds_bytes_per_node(bool geo) { return 16 + (geo ? 8 : 0) + 2; }
 
 // GEO: Dm^-1 and V0 from the staged reference positions; NT: cache policy of the streams; EPS: the
 // viscoplastic law keeps its accumulated plastic strain (the linear law does not read it); SYM: its
@@ -448,10 +448,12 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
     extern __shared__ __align__(16) unsigned char smem[];
     float* s_cf = reinterpret_cast<float*>(smem);  // [component][4K], sorted-corner order
     const int MX = a.max_nodes;
-    float* s_x = s_cf + 12 * K;                    // [component][MX]
+    // per tile-node, node-major: (ux, uy, uz, X0x) as one 16-byte LDS word and, with GEO, (X0y, X0z)
+    // as one 8-byte word, so a corner's node data is one or two LDS reads at one address
+    float4* s_n = reinterpret_cast<float4*>(s_cf + 12 * K);
     __shared__ int s_flow;                         // viscoplastic: a tetra of the tile flowed this step
-    float* s_r = s_x + 3 * MX;                     // GEO: [component][MX] reference positions
-    uint16_t* s_lb = reinterpret_cast<uint16_t*>(s_r + (GEO ? 3 * MX : 0));
+    float2* s_r = reinterpret_cast<float2*>(s_n + MX);
+    uint16_t* s_lb = reinterpret_cast<uint16_t*>(s_r + (GEO ? MX : 0));
 
     const uint32_t ntile_blocks = gridDim.x - (uint32_t)a.curve_blocks;
     if (blockIdx.x >= ntile_blocks) {  // the step's curve sample of cell blockIdx.x - ntile_blocks
@@ -530,13 +532,12 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
     uint64_t cr = 0;
     if (active) cr = ld_g<NT, uint64_t>(uat(reinterpret_cast<const uint64_t*>(a.tile_corners + 4 * K * c), u));
     if (owner) {
-        s_x[tl] = a.x[nid];
-        s_x[MX + tl] = a.x[N + nid];
-        s_x[2 * MX + tl] = a.x[2 * N + nid];
+        const float ux = a.x[nid], uy = a.x[N + nid], uz = a.x[2 * N + nid];
         if constexpr (GEO) {
-            s_r[tl] = a.ref[nid];
-            s_r[MX + tl] = a.ref[N + nid];
-            s_r[2 * MX + tl] = a.ref[2 * N + nid];
+            s_n[tl] = make_float4(ux, uy, uz, a.ref[nid]);
+            s_r[tl] = make_float2(a.ref[N + nid], a.ref[2 * N + nid]);
+        } else {
+            s_n[tl] = make_float4(ux, uy, uz, 0.0f);
         }
         s_lb[tl] = (uint16_t)lb;
     }
@@ -550,10 +551,10 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         // V0 Dm^-1 and whose cofactors are cof(F) g_k); the legacy laws take Dm^-1 itself
         constexpr bool ROWS = GEO && MODEL <= NFX_VISCOPLASTIC;
         if constexpr (GEO) {
-            const V3 r0 = mk(s_r[l0], s_r[MX + l0], s_r[2 * MX + l0]);
-            const V3 r1 = mk(s_r[l1], s_r[MX + l1], s_r[2 * MX + l1]);
-            const V3 r2 = mk(s_r[l2], s_r[MX + l2], s_r[2 * MX + l2]);
-            const V3 r3 = mk(s_r[l3], s_r[MX + l3], s_r[2 * MX + l3]);
+            const float4 n0 = s_n[l0], n1 = s_n[l1], n2 = s_n[l2], n3 = s_n[l3];
+            const float2 m0 = s_r[l0], m1 = s_r[l1], m2 = s_r[l2], m3 = s_r[l3];
+            const V3 r0 = mk(n0.w, m0.x, m0.y), r1 = mk(n1.w, m1.x, m1.y), r2 = mk(n2.w, m2.x, m2.y),
+                     r3 = mk(n3.w, m3.x, m3.y);
             if constexpr (ROWS) {
                 sdm = shape_rows(r1 - r0, r2 - r0, r3 - r0, D);
             } else {
@@ -563,10 +564,9 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         } else if constexpr (MODEL == NFX_VISCOPLASTIC) {
             sdm = det_dm(D);
         }
-        const V3 u0 = mk(s_x[l0], s_x[MX + l0], s_x[2 * MX + l0]);
-        const V3 u1 = mk(s_x[l1], s_x[MX + l1], s_x[2 * MX + l1]);
-        const V3 u2 = mk(s_x[l2], s_x[MX + l2], s_x[2 * MX + l2]);
-        const V3 u3 = mk(s_x[l3], s_x[MX + l3], s_x[2 * MX + l3]);
+        const float4 w0 = s_n[l0], w1 = s_n[l1], w2 = s_n[l2], w3 = s_n[l3];
+        const V3 u0 = mk(w0.x, w0.y, w0.z), u1 = mk(w1.x, w1.y, w1.z), u2 = mk(w2.x, w2.y, w2.z),
+                 u3 = mk(w3.x, w3.y, w3.z);
         M9 H = grad_disp(u0, u1, u2, u3, D);
         if constexpr (ROWS) {
             const float id = frcp(sdm);
