@@ -551,9 +551,14 @@ def main():
         small = configs.small_cell()
         Ps, Ts = cached_mesh(small)
         s3 = make_sim(small, Ps, Ts, "tiled", "elastic")
+        from nofluidx3d_amd.autotune import tune_simulation
+
+        (k3, nt3), tuned3 = tune_simulation(s3)
         ms3 = time_steps(s3, 200, 2000)[0] * 1e3
+        f3, n3 = pass_times(s3, 200)
         out["plain_small"] = {"workload": f"small_cell {len(Ps)} nodes {len(Ts)} tetra NH", "ms_per_step": ms3,
-                              "value": len(Ts) * 1e3 / ms3, "unit": "tetra-updates/s"}
+                              "value": len(Ts) * 1e3 / ms3, "unit": "tetra-updates/s", "tile_k": k3,
+                              "ms_force_pass": f3, "ms_node_pass": n3, "autotune_ms_per_step": tuned3}
         del s3
     if not args.no_extra and not args.no_full_protocol and args.scatter == "tiled":
         # the whole protocol from rest on a fresh cell, graph captures included (one per distinct

@@ -116,7 +116,8 @@ typedef struct {
     const uint32_t* csr_offsets; /* N+1, incidences of node n at [off[n], off[n+1]) */
     const uint32_t* csr_slots;   /* 4T, slot = 4*t + corner, ascending per node */
     float* corners;              /* 12T scratch: force of corner a of tetra t at 12*t + 3*a + c */
-    int32_t tile_k;              /* NFX_SCATTER_TILED: tetra per tile (256 or 512) */
+    int32_t tile_k;              /* NFX_SCATTER_TILED: tetra per tile (128, 256 or 512; 128 fills the chip
+                                    for small meshes) */
     int32_t tile_max_nodes;      /* MX: tile-node capacity per tile (largest distinct-node count,
                                     at most 256) */
     const uint32_t* tile_count;  /* ntiles: distinct nodes of tile c (<= MX) */

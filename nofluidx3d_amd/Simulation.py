@@ -17,7 +17,7 @@ Extensions (keyword arguments):
   "csr" deterministic gather of per-tetra corner records through the node CSR;
   "atomic" float atomics in arbitrary order, as the reference's atomicAdd.
   The physics is the same; only the floating-point summation order differs.
-``tile_k`` tetra per tile for "tiled" (256 or 512);
+``tile_k`` tetra per tile for "tiled" (128, 256 or 512);
 ``stream_nt`` cache policy of the tiled force pass's per-tetra streams (1 nontemporal, 0 default,
   None: nontemporal once they exceed the 256 MB memory-side cache; ``autotune`` times both);
 ``shape_from_ref`` the tiled force pass forms each tetra's Dm^-1 and V0 from the FP32 reference

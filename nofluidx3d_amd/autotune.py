@@ -6,7 +6,8 @@ Candidates are timed over a few steps (a single cell: the force and node passes 
 physics; tile size only changes the floating-point summation order, by rounding.
 
 Tunables:
-- ``tile_k``: tetra per tile = threads per workgroup of the force pass (256 or 512);
+- ``tile_k``: tetra per tile = threads per workgroup of the force pass (128, 256 or 512; 128 gives a
+  small mesh enough workgroups to fill the chip, at the cost of more tile-node halo);
 - ``stream_nt``: cache policy of the force pass's per-tetra streams (state, tile tables): 1 =
   nontemporal, 0 = default.  A state that fits MI355X's 256 MB memory-side cache is served from it
   from step to step under the default policy; a larger one streams faster past it;
@@ -23,6 +24,8 @@ import os
 import time
 
 TILE_CANDIDATES = (256, 512)
+# a single cell also tries 128-tetra tiles when 512-tetra tiles would leave the chip's 256 CUs idle
+SMALL_MESH_TETRA = 256 * 512
 STREAM_CANDIDATES = (0, 1)
 MEMORY_SIDE_CACHE_BYTES = 256 << 20
 STATE_BYTES_PER_TETRA = 48  # force pass streams: 9-float state, plastic strain or volume, 4 slots
@@ -151,7 +154,8 @@ def tune_simulation(sim, nrep=20, log=None):
     openCL.getCommandQueue().finish()
     snapshot = [t.clone() for t in sim.state_tensors()]  # timing runs real steps: restore afterwards
     times, choice = {}, {}
-    for K in TILE_CANDIDATES:
+    cands = ((128,) if sim.numTetra < SMALL_MESH_TETRA else ()) + TILE_CANDIDATES
+    for K in cands:
         sim.tile_k = K
         sim._tiles = None
         for nt in STREAM_CANDIDATES:

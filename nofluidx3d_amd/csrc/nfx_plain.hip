@@ -993,7 +993,7 @@ static int validate_plan(const nfx_plan* p) {
     NFX_REQUIRE(p->scatter >= NFX_SCATTER_ATOMIC && p->scatter <= NFX_SCATTER_TILED, "plan: unknown scatter %d",
                 p->scatter);
     if (p->scatter == NFX_SCATTER_TILED) {
-        NFX_REQUIRE(p->tile_k == 256 || p->tile_k == 512, "plan: tile_k must be 256 or 512");
+        NFX_REQUIRE(p->tile_k == 128 || p->tile_k == 256 || p->tile_k == 512, "plan: tile_k must be 128, 256 or 512");
         NFX_REQUIRE(p->tile_count && p->tile_nodes && p->tile_lend && p->tile_corners && p->tile_dst && p->node_off &&
                         p->partials,
                     "plan: tiled scatter needs the tile arrays");
@@ -1152,7 +1152,8 @@ static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
     const bool st = stress && p.von_mises && p.pressure;
     if (p.scatter == NFX_SCATTER_TILED) {
         const bool geo = p.shape_from_ref != 0;
-        if (p.tile_k == 256) geo ? launch_tiled_k<256, true>(ctx, p, st) : launch_tiled_k<256, false>(ctx, p, st);
+        if (p.tile_k == 128) geo ? launch_tiled_k<128, true>(ctx, p, st) : launch_tiled_k<128, false>(ctx, p, st);
+        else if (p.tile_k == 256) geo ? launch_tiled_k<256, true>(ctx, p, st) : launch_tiled_k<256, false>(ctx, p, st);
         else geo ? launch_tiled_k<512, true>(ctx, p, st) : launch_tiled_k<512, false>(ctx, p, st);
         NFX_CHECK_LAUNCH();
         return NFX_OK;
@@ -1501,7 +1502,7 @@ int nfx_batch_geometry(nfx_ctx* c, const double* p_rel, const double* cell_scale
                        const uint32_t* tetras, const uint16_t* tile_ntet, int32_t tile_k, int64_t N, int64_t T,
                        float* ref, float* dminv, float* v0) {
     NFX_REQUIRE(c && p_rel && cell_scale && node_cell && tetras && tile_ntet && ref && dminv && v0, "NULL argument");
-    NFX_REQUIRE(tile_k == 256 || tile_k == 512, "tile_k must be 256 or 512");
+    NFX_REQUIRE(tile_k == 128 || tile_k == 256 || tile_k == 512, "tile_k must be 128, 256 or 512");
     int rc = check_sizes(N, T);
     if (rc) return rc;
     NFX_REQUIRE(T % tile_k == 0, "padded tetra count must be a multiple of tile_k");
@@ -1675,7 +1676,7 @@ int nfx_build_tiles_host(const uint32_t* tet, int64_t N, int64_t T, int32_t K, i
                          uint32_t* tile_count, uint32_t* tile_nodes, uint16_t* tile_lend, uint16_t* tile_corners,
                          uint32_t* tile_dst, uint32_t* node_off) {
     NFX_REQUIRE(tet && max_nodes && m_total && tile_count, "NULL argument");
-    NFX_REQUIRE(K == 256 || K == 512, "tile_k must be 256 or 512");
+    NFX_REQUIRE(K == 128 || K == 256 || K == 512, "tile_k must be 128, 256 or 512");
     int rc = check_sizes(N, T);
     if (rc) return rc;
     const bool fill = tile_nodes != nullptr;

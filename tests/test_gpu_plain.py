@@ -338,8 +338,8 @@ def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, 
         cfg["CELL"].update({"MooneyRivlinRatio": 0.75, "model": "MOONEY_RIVLIN", "ViscositySI": 10.0,
                             "YieldStressSI": yield_si})
     P, T = configs.make_mesh(cfg)
-    tile_k = 256 if scatter == "tiled256" else 512
-    scatter = "tiled" if scatter == "tiled256" else scatter
+    tile_k = {"tiled256": 256, "tiled128": 128}.get(scatter, 512)
+    scatter = "tiled" if scatter.startswith("tiled") else scatter
     sim = Simulation(cfg, scatter=scatter, fused=fused, use_graph=use_graph, mesh=vtk.from_tetra(P, T),
                      verbose=False, tile_k=tile_k)
     sim.setInteractionSubstrate()
@@ -407,10 +407,12 @@ def assert_parity(sim, gx, gf, ox, of):
     return pos_err, f_err
 
 
-@pytest.mark.parametrize("scatter", ["atomic", "csr", "tiled", "tiled256"])
+@pytest.mark.parametrize("scatter", ["atomic", "csr", "tiled", "tiled256", "tiled128"])
 @pytest.mark.parametrize("indenter", ["sphere", "plane"])
 def test_fused_run_matches_oracle(ctx, scatter, indenter):
     sim = build_sim(scatter=scatter, indenter=indenter)
+    if scatter == "tiled128":
+        assert sim.plan().tile_k == 128 and sim.scatter == "tiled"
     gx, gf = run_gpu(sim, 600)
     ref = oracle_for(sim)
     of = ref.run(600)
