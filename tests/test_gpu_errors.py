@@ -81,7 +81,7 @@ PLAN_CASES = [
     ({"ref": None}, "NULL mesh/state pointer"),
     ({"vp_sym": 1}, "symmetric state (vp_sym) is for the tiled viscoplastic force only"),
     ({"scatter": 7}, "unknown scatter 7"),
-    ({"tile_k": 128}, "tile_k must be 256 or 512"),
+    ({"tile_k": 100}, "tile_k must be 128, 256 or 512"),
     ({"tile_max_nodes": 300}, "tile_max_nodes 300 exceeds"),
     ({"tile_corners": None}, "tiled scatter needs the tile arrays"),
     ({"indenter": 3}, "unknown indenter 3"),
@@ -116,8 +116,8 @@ def test_bad_sizes_and_options_of_other_entry_points(ctx):
     buf = torch.zeros(4096, dtype=torch.float64, device="cuda:0")
     ptr = buf.data_ptr()
     refused("nfx_precompute_f64", ctx, ptr, ptr, 0, ptr, ptr, expect="tetra count 0 out of range")
-    refused("nfx_batch_geometry", ctx, ptr, ptr, ptr, ptr, ptr, 128, 100, 512, ptr, ptr, ptr,
-            expect="tile_k must be 256 or 512")
+    refused("nfx_batch_geometry", ctx, ptr, ptr, ptr, ptr, ptr, 100, 100, 500, ptr, ptr, ptr,
+            expect="tile_k must be 128, 256 or 512")
     refused("nfx_batch_geometry", ctx, ptr, ptr, ptr, ptr, ptr, 256, 100, 1000, ptr, ptr, ptr,
             expect="multiple of tile_k")
     refused("nfx_batch_geometry", ctx, ptr, ptr, ptr, ptr, ptr, 256, 0, 512, ptr, ptr, ptr,
