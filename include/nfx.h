@@ -338,13 +338,18 @@ int nfx_hard_adhesive_plane_surface_force(nfx_ctx* ctx, float* force, const floa
  *      and the damped velocity-Verlet update, on the displacement state (plan.ref, plan.disp).
  *      tip_hist (device, nullable, nsteps doubles) receives the FP64 indenter force of each step.
  *      use_graph != 0 replays whole blocks of 100 steps from a captured hipGraph (captured once per
- *      plan and history mode, and reused whatever nsteps is); the remainder runs eagerly. ------ */
+ *      plan and history mode, and reused whatever nsteps is); the remainder runs eagerly.
+ *      Between calls the velocity-Verlet state is the reference's (vel, fold).  Inside a call of
+ *      n >= 2 steps the first step turns it into w = a v + f_old/2 (kept in vel), the middle steps
+ *      advance (disp, w) without a previous force (24 B per node less), and the last step turns w
+ *      back into (vel, fold): the same recurrence, with rounding that depends on how a run is cut
+ *      into calls (the same cuts give the same bits). ------ */
 int nfx_run_steps(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int64_t nsteps, double* tip_hist,
                   int32_t use_graph);
 /* number of step blocks the context has captured into graphs so far (a replay of a cached block
  * does not count) */
 int nfx_graph_captures(nfx_ctx* ctx, int64_t* count);
-/* the node pass alone (for callers that enqueue the force pass themselves) */
+/* the node pass alone (for callers that enqueue the force pass themselves), on (vel, fold) */
 int nfx_node_update(nfx_ctx* ctx, const nfx_plan* plan, int64_t step, double* tip_out);
 
 /* ---- batched stepping (no reference counterpart; the reference simulates one cell per process):
@@ -358,8 +363,9 @@ int nfx_batch_run_steps(nfx_ctx* ctx, const nfx_plan* plan, const nfx_batch* bat
 int nfx_batch_time_passes(nfx_ctx* ctx, const nfx_plan* plan, const nfx_batch* batch, int64_t step0, int32_t nrep,
                           double* ms_force, double* ms_node, double* ms_curve);
 
-/* ---- kernel timing for the benchmark: average device time of the force pass and of the node
- *      pass over nrep steps, measured with HIP events on the context's stream (ms). ---------- */
+/* ---- kernel timing for the benchmark: average device time of the force pass over nrep steps and
+ *      of the node pass over the middle steps of those nrep (the (disp, w) form; all of them when
+ *      nrep < 3), measured with HIP events on the context's stream (ms). ---------- */
 int nfx_time_passes(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int32_t nrep, double* ms_force,
                     double* ms_node);
 

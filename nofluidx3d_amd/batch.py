@@ -156,8 +156,9 @@ def make_meshes(specs, workers=None, cache="/tmp/nfx_cells"):
     jobs = [(c.radius_um, c.spacing_um, c.seed, cache) for c in specs]
     missing = [i for i, j in enumerate(jobs) if not (cache and os.path.exists(_mesh_path(*j)))]
     out = [None] * len(jobs)
+    absent = set(missing)
     for i, j in enumerate(jobs):
-        if i not in set(missing):
+        if i not in absent:
             out[i] = _mesh_job(j)
     workers = workers or min(16, os.cpu_count() or 1)
     if workers <= 1 or len(missing) < 4:

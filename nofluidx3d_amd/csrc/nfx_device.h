@@ -782,6 +782,19 @@ __device__ __forceinline__ void vv_update_w(V3& x, V3& w, V3 f) {
     w = (w + h) * VV::a_over_b + h;
     x = x + w;
 }
+// Entering the (x, w) form: the step's velocity from (v, f_old) as vv_update has it, then
+// w' = a v' + f'/2 (the caller adds it to x).
+__device__ __forceinline__ V3 vv_w_from_v(V3 v, V3 f, V3 fo) {
+    const V3 vn = (v * VV::a + (f + fo) * 0.5f) * VV::inv_b;
+    return vn * VV::a + f * 0.5f;
+}
+// Leaving it: v' = (w + f'/2)/b, x' = x + a v' + f'/2; w is replaced by v' (f' becomes f_old).
+__device__ __forceinline__ void vv_v_from_w(V3& x, V3& w, V3 f) {
+    const V3 h = f * 0.5f;
+    const V3 vn = (w + h) * VV::inv_b;
+    x = x + (vn * VV::a + h);
+    w = vn;
+}
 
 // Simulation.py:173-183 / 117-132, in double
 __device__ __forceinline__ double indenter_pos(double start, double speed, double move_steps, int64_t time) {

@@ -784,8 +784,16 @@ __device__ __forceinline__ void node_contacts(const float* __restrict__ ref, int
 }
 
 // GATHER: 0 = read the atomically accumulated force (and clear it), 1 = corner records via the
-// node CSR (4 lanes per node), 2 = tile partials via node_parts (one lane per node)
-template <int GATHER, bool SUB, int IND, bool FIX, bool TIP>
+// node CSR (4 lanes per node), 2 = tile partials via node_parts (one lane per node).
+// VVM, the velocity-Verlet state this step reads and writes (the reference's (v, f_old) between
+// calls, the (u, w) form of the batch inside a call, w = a v + f_old/2, which needs no previous force
+// and so moves 24 B per node less):
+//   VV_PLAIN   reads v, f_old and writes v, f_old (a one-step call)
+//   VV_FIRST   reads v, f_old and writes w (into v's buffer)
+//   VV_MIDDLE  reads w and writes w
+//   VV_LAST    reads w and writes v, f_old
+enum { VV_PLAIN = 0, VV_FIRST = 1, VV_MIDDLE = 2, VV_LAST = 3 };
+template <int GATHER, bool SUB, int IND, bool FIX, bool TIP, int VVM = VV_PLAIN>
 __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step, const int64_t* step_dev, int32_t step_off,
                                                         double* partials, unsigned int* counter, double* tip_sum,
                                                         double* tip_max) {
@@ -814,9 +822,17 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
             tsum += tf;
             tmax = fmax(tmax, td);
         }
-        V3 v = mk(a.v[n], a.v[N + n], a.v[2 * N + n]);
-        const V3 fo = mk(a.fold[n], a.fold[N + n], a.fold[2 * N + n]);
-        vv_update(x, v, f, fo);
+        V3 v = mk(a.v[n], a.v[N + n], a.v[2 * N + n]);  // v, or w inside a call
+        if constexpr (VVM == VV_PLAIN) {
+            vv_update(x, v, f, mk(a.fold[n], a.fold[N + n], a.fold[2 * N + n]));
+        } else if constexpr (VVM == VV_FIRST) {
+            v = vv_w_from_v(v, f, mk(a.fold[n], a.fold[N + n], a.fold[2 * N + n]));
+            x = x + v;
+        } else if constexpr (VVM == VV_MIDDLE) {
+            vv_update_w(x, v, f);
+        } else {
+            vv_v_from_w(x, v, f);
+        }
         if constexpr (FIX) {
             if (n == 3 || n == 4) {  // x = z = 0 (VelocityVerlet.cl:17-22): u = -X0
                 x.x = -a.ref[n];
@@ -825,7 +841,9 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
         }
         a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
         a.v[n] = v.x; a.v[N + n] = v.y; a.v[2 * N + n] = v.z;
-        a.fold[n] = f.x; a.fold[N + n] = f.y; a.fold[2 * N + n] = f.z;
+        if constexpr (VVM == VV_PLAIN || VVM == VV_LAST) {
+            a.fold[n] = f.x; a.fold[N + n] = f.y; a.fold[2 * N + n] = f.z;
+        }
         if constexpr (GATHER == 0) {
             a.force[n] = 0.0f; a.force[N + n] = 0.0f; a.force[2 * N + n] = 0.0f;
         }
@@ -1171,47 +1189,59 @@ static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
     return NFX_OK;
 }
 
-template <int CSR, bool SUB, int IND, bool FIX>
-static void launch_node_t(nfx_ctx* ctx, const NodeArgs& a, int64_t step, const int64_t* step_dev, int32_t off,
+template <int CSR, bool SUB, int IND, bool FIX, int VVM>
+static void launch_node_m(nfx_ctx* ctx, const NodeArgs& a, int64_t step, const int64_t* step_dev, int32_t off,
                           double* tip_sum, double* tip_max) {
     constexpr int G = CSR == 1 ? 4 : 1;
     const bool tip = (tip_sum || tip_max) && IND != NFX_INDENTER_NONE;
     unsigned int nb = nblocks(a.N * G);
     if (tip) {
         nb = std::min<unsigned int>(nb, kMaxReduceBlocks);
-        k_node_update<CSR, SUB, IND, FIX, (IND != NFX_INDENTER_NONE)><<<nb, kBlock, 0, ctx->stream>>>(
+        k_node_update<CSR, SUB, IND, FIX, (IND != NFX_INDENTER_NONE), VVM><<<nb, kBlock, 0, ctx->stream>>>(
             a, step, step_dev, off, ctx->partials, ctx->counter, tip_sum, tip_max);
     } else {
-        k_node_update<CSR, SUB, IND, FIX, false><<<nb, kBlock, 0, ctx->stream>>>(a, step, step_dev, off, nullptr,
-                                                                                nullptr, nullptr, nullptr);
+        k_node_update<CSR, SUB, IND, FIX, false, VVM><<<nb, kBlock, 0, ctx->stream>>>(a, step, step_dev, off, nullptr,
+                                                                                     nullptr, nullptr, nullptr);
+    }
+}
+
+template <int CSR, bool SUB, int IND, bool FIX>
+static void launch_node_t(nfx_ctx* ctx, const NodeArgs& a, int64_t step, const int64_t* step_dev, int32_t off,
+                          double* tip_sum, double* tip_max, int vvm) {
+    switch (vvm) {
+        case VV_FIRST: launch_node_m<CSR, SUB, IND, FIX, VV_FIRST>(ctx, a, step, step_dev, off, tip_sum, tip_max); break;
+        case VV_MIDDLE: launch_node_m<CSR, SUB, IND, FIX, VV_MIDDLE>(ctx, a, step, step_dev, off, tip_sum, tip_max); break;
+        case VV_LAST: launch_node_m<CSR, SUB, IND, FIX, VV_LAST>(ctx, a, step, step_dev, off, tip_sum, tip_max); break;
+        default: launch_node_m<CSR, SUB, IND, FIX, VV_PLAIN>(ctx, a, step, step_dev, off, tip_sum, tip_max);
     }
 }
 
 template <int CSR, bool SUB>
 static void launch_node_ind(nfx_ctx* ctx, const nfx_plan& p, const NodeArgs& a, int64_t step, const int64_t* sd,
-                            int32_t off, double* ts, double* tm) {
+                            int32_t off, double* ts, double* tm, int vvm) {
     const bool fix = p.fix_top_bottom != 0;
     switch (p.indenter * 2 + (fix ? 1 : 0)) {
-        case 0: launch_node_t<CSR, SUB, 0, false>(ctx, a, step, sd, off, ts, tm); break;
-        case 1: launch_node_t<CSR, SUB, 0, true>(ctx, a, step, sd, off, ts, tm); break;
-        case 2: launch_node_t<CSR, SUB, 1, false>(ctx, a, step, sd, off, ts, tm); break;
-        case 3: launch_node_t<CSR, SUB, 1, true>(ctx, a, step, sd, off, ts, tm); break;
-        case 4: launch_node_t<CSR, SUB, 2, false>(ctx, a, step, sd, off, ts, tm); break;
-        default: launch_node_t<CSR, SUB, 2, true>(ctx, a, step, sd, off, ts, tm); break;
+        case 0: launch_node_t<CSR, SUB, 0, false>(ctx, a, step, sd, off, ts, tm, vvm); break;
+        case 1: launch_node_t<CSR, SUB, 0, true>(ctx, a, step, sd, off, ts, tm, vvm); break;
+        case 2: launch_node_t<CSR, SUB, 1, false>(ctx, a, step, sd, off, ts, tm, vvm); break;
+        case 3: launch_node_t<CSR, SUB, 1, true>(ctx, a, step, sd, off, ts, tm, vvm); break;
+        case 4: launch_node_t<CSR, SUB, 2, false>(ctx, a, step, sd, off, ts, tm, vvm); break;
+        default: launch_node_t<CSR, SUB, 2, true>(ctx, a, step, sd, off, ts, tm, vvm); break;
     }
 }
 
+// vvm: VV_PLAIN for a one-step call; a call of n >= 2 steps runs VV_FIRST, VV_MIDDLE x (n - 2), VV_LAST
 static int launch_node_pass(nfx_ctx* ctx, const nfx_plan& p, int64_t step, const int64_t* step_dev, int32_t off,
-                            double* tip_sum, double* tip_max) {
+                            double* tip_sum, double* tip_max, int vvm = VV_PLAIN) {
     const NodeArgs a = node_args(p);
     const bool sub = p.substrate != 0;
     switch (p.scatter * 2 + (sub ? 1 : 0)) {
-        case 0: launch_node_ind<0, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
-        case 1: launch_node_ind<0, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
-        case 2: launch_node_ind<1, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
-        case 3: launch_node_ind<1, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
-        case 4: launch_node_ind<2, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
-        default: launch_node_ind<2, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max); break;
+        case 0: launch_node_ind<0, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max, vvm); break;
+        case 1: launch_node_ind<0, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max, vvm); break;
+        case 2: launch_node_ind<1, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max, vvm); break;
+        case 3: launch_node_ind<1, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max, vvm); break;
+        case 4: launch_node_ind<2, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max, vvm); break;
+        default: launch_node_ind<2, true>(ctx, p, a, step, step_dev, off, tip_sum, tip_max, vvm); break;
     }
     NFX_CHECK_LAUNCH();
     return NFX_OK;
@@ -1364,7 +1394,7 @@ static int capture_graph(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch* b, bo
             rc = launch_force_pass(ctx, p, false);
             if (rc == NFX_OK)
                 rc = launch_node_pass(ctx, p, 0, ctx->step_dev, (int32_t)i, hist ? ctx->hist_dev + i : nullptr,
-                                      nullptr);
+                                      nullptr, VV_MIDDLE);
         }
     }
     hipGraph_t g = nullptr;
@@ -1852,15 +1882,26 @@ int nfx_run_steps(nfx_ctx* c, const nfx_plan* p, int64_t step0, int64_t nsteps, 
     if (rc) return rc;
     NFX_REQUIRE(nsteps >= 0, "nsteps < 0");
     NFX_REQUIRE(!tip_hist || p->indenter != NFX_INDENTER_NONE, "tip force history needs an indenter");
-    int64_t done = 0;
-    // with stress outputs in the plan, the last step of the call is enqueued eagerly and writes
-    // the per-tetra von Mises stress and pressure (the state a frame written afterwards shows)
+    if (nsteps == 0) return NFX_OK;
+    // with stress outputs in the plan, the last step of the call writes the per-tetra von Mises
+    // stress and pressure (the state a frame written afterwards shows)
     const bool stress = p->von_mises && p->pressure;
-    const int64_t ngraph = stress ? nsteps - 1 : nsteps;
-    // whole blocks of kGraphBlock steps replay one captured graph, kept for as long as the plan and
-    // the history mode stay the same; a remainder (and a call shorter than a block) runs eagerly,
-    // so no call re-captures for a new step count
-    if (use_graph && ngraph >= kGraphBlock) {
+    if (nsteps == 1) {
+        rc = launch_force_pass(c, *p, stress);
+        if (rc) return rc;
+        return launch_node_pass(c, *p, step0, nullptr, 0, tip_hist, nullptr, VV_PLAIN);
+    }
+    // n >= 2: the first step turns (v, f_old) into w, the middle steps advance (u, w), the last turns
+    // w back into (v, f_old); the first and last steps run eagerly, the middle ones in whole blocks of
+    // kGraphBlock steps that replay one captured graph (kept for as long as the plan and the history
+    // mode stay the same) and an eager remainder, so no call re-captures for a new step count
+    rc = launch_force_pass(c, *p, false);
+    if (rc) return rc;
+    rc = launch_node_pass(c, *p, step0, nullptr, 0, tip_hist, nullptr, VV_FIRST);
+    if (rc) return rc;
+    int64_t done = 1;
+    const int64_t ngraph = nsteps - 1;  // middle steps run while done < ngraph
+    if (use_graph && ngraph - done >= kGraphBlock) {
         const int64_t B = kGraphBlock;
         const bool hist = tip_hist != nullptr;
         nfx_ctx::GraphEntry* ge = find_graph(c, *p, nullptr, hist);
@@ -1879,9 +1920,11 @@ int nfx_run_steps(nfx_ctx* c, const nfx_plan* p, int64_t step0, int64_t nsteps, 
         }
     }
     for (; done < nsteps; ++done) {
-        rc = launch_force_pass(c, *p, stress && done == nsteps - 1);
+        const bool last = done == nsteps - 1;
+        rc = launch_force_pass(c, *p, stress && last);
         if (rc) return rc;
-        rc = launch_node_pass(c, *p, step0 + done, nullptr, 0, tip_hist ? tip_hist + done : nullptr, nullptr);
+        rc = launch_node_pass(c, *p, step0 + done, nullptr, 0, tip_hist ? tip_hist + done : nullptr, nullptr,
+                              last ? VV_LAST : VV_MIDDLE);
         if (rc) return rc;
     }
     return NFX_OK;
@@ -1898,21 +1941,28 @@ int nfx_time_passes(nfx_ctx* c, const nfx_plan* p, int64_t step0, int32_t nrep, 
         rc = launch_force_pass(c, *p, false);
         if (rc) break;
         NFX_HIP(hipEventRecord(ev[3 * i + 1], c->stream));
-        rc = launch_node_pass(c, *p, step0 + i, nullptr, 0, nullptr, nullptr);
+        const int vvm = nrep == 1 ? VV_PLAIN : i == 0 ? VV_FIRST : i == nrep - 1 ? VV_LAST : VV_MIDDLE;
+        rc = launch_node_pass(c, *p, step0 + i, nullptr, 0, nullptr, nullptr, vvm);
         NFX_HIP(hipEventRecord(ev[3 * i + 2], c->stream));
     }
     if (rc == NFX_OK) {
         NFX_HIP(hipEventSynchronize(ev[3 * (size_t)nrep - 1]));
+        // the node pass is averaged over the middle steps (the (u, w) form a long call runs) when
+        // there are any
         double sf = 0, sn = 0;
+        int nn = 0;
         for (int i = 0; i < nrep; ++i) {
             float a = 0, b = 0;
             NFX_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
             NFX_HIP(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
             sf += a;
-            sn += b;
+            if (nrep < 3 || (i > 0 && i < nrep - 1)) {
+                sn += b;
+                ++nn;
+            }
         }
         *ms_force = sf / nrep;
-        *ms_node = sn / nrep;
+        *ms_node = sn / nn;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     return rc;
