@@ -346,7 +346,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the secondary measurements")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 512-cell measurement")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
-    ap.add_argument("--tile-k", type=int, default=0, choices=[0, 256, 512], help="0: autotune")
+    ap.add_argument("--tile-k", type=int, default=0, choices=[0, 128, 256, 512], help="0: autotune")
     ap.add_argument("--start-step", type=int, default=-1,
                     help="protocol step the timed window starts from (untimed steps before it); -1: the end of "
                          "the indentation for the large cell (the deepest contact, the largest plastic zone), 0 "
@@ -431,10 +431,7 @@ def main():
     ms = dt * 1e3
     ms_force, ms_node = pass_times(sim, min(args.steps, 100))
     vp = material == "viscoplastic"
-    plastic_tiles = None
-    if sim._vp_active is not None and sim.plan().vp_active:
-        fl = sim._vp_active.cpu().numpy()
-        plastic_tiles = float((fl != 0).mean())
+    plastic_tiles = sim.plastic_tile_fraction()
     force_bytes = SPEC_TET_BYTES[material] * nT + 24 * N
     geo = bool(sim.plan().shape_from_ref) and args.scatter == "tiled"
     tet_bytes = KERNEL_TET_BYTES[material] + (0 if geo else STREAMED_SHAPE_BYTES[material])
@@ -452,7 +449,7 @@ def main():
     if args.traffic_json and os.path.exists(args.traffic_json):
         tj = json.load(open(args.traffic_json))
         key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if nt_flag else ''}{'_shape_from_ref' if geo else ''}"
-        if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}):
+        if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}) and start == tj.get("from_step"):
             traffic = tj["force_pass_bytes"][key]
             traffic_source = f"{os.path.relpath(args.traffic_json, ROOT)}[{key}]: {tj.get('source', '')}"
     compute = None

@@ -516,6 +516,14 @@ class Simulation:
                 self._vp_flags_stale = False
         return p
 
+    def plastic_tile_fraction(self):
+        """Fraction of the tiled viscoplastic step's tiles whose tetra have flowed (None without flags)."""
+        p = self._plan
+        if self._vp_active is None or p is None or not p.vp_active:
+            return None
+        openCL.getCommandQueue().finish()
+        return float((self._vp_active.cpu().numpy() != 0).mean())
+
     def plastic_state_changed(self):
         """The viscoplastic state was written outside the fused tiled step (a conversion, a restore,
         a caller's own write): the per-tile activity flags are recomputed before the next fused step."""
