@@ -419,6 +419,32 @@ def test_fused_run_matches_oracle(ctx, scatter, indenter):
     assert_parity(sim, gx, gf, ref.positions(), of)
 
 
+@pytest.mark.parametrize("material", ["NH", "VP"])
+def test_fused_calls_of_every_length_match_oracle(ctx, material):
+    """A multi-step call carries (u, w) with w = a*v + f_old/2 between its first and last step; a
+    one-step call keeps (v, f_old). Calls of 1, 2, 3 and more steps (eager and graph-replayed), in
+    any order, hand (v, f_old) to the next call, as the reference's per-step loop does."""
+    import torch
+
+    sim = build_sim(material=material, scatter="tiled")
+    vv = sim.interactions[-1]
+    chunks = [1, 2, 1, 3, 7, 1, 150, 2, 1, 32]
+    hist = torch.zeros(sum(chunks), dtype=torch.float64, device="cuda:0")
+    ref = oracle_for(sim)
+    t, of = 0, []
+    for n in chunks:
+        sim.advance(n, tip_hist=hist[t:t + n])
+        of.append(ref.run(n).copy())
+        t += n
+        gv = vv.velocities()
+        gfo = vv.forceOldB.cpu().numpy().astype(np.float64)
+        vs = max(np.max(np.abs(ref.vel)), 1e-30)
+        fs = max(np.max(np.abs(ref.fold)), 1e-30)
+        assert np.max(np.abs(gv - ref.vel)) / vs < 1e-3, (n, t)
+        assert np.max(np.abs(gfo - ref.fold)) / fs < 1e-3, (n, t)
+    assert_parity(sim, sim.positions(), host(hist), ref.positions(), np.concatenate(of))
+
+
 @pytest.mark.parametrize("scatter", ["csr", "tiled"])
 def test_viscoplastic_fused_run_matches_oracle(ctx, scatter):
     sim = build_sim(material="VP", scatter=scatter)
