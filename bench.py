@@ -78,12 +78,12 @@ def cached_mesh(cfg):
     return P, T
 
 
-def make_sim(cfg, P, T, scatter, material, tile_k=512, shape_from_ref=True, compact_state=False):
+def make_sim(cfg, P, T, scatter, material, tile_k=512, shape_from_ref=True, compact_state=False, node_fused=None):
     from nofluidx3d_amd.Simulation import Simulation
     from nofluidx3d_amd.util import vtk
 
     sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False, tile_k=tile_k,
-                     shape_from_ref=shape_from_ref)
+                     shape_from_ref=shape_from_ref, fuse_node_pass=node_fused)
     sim.setInteractionSubstrate()
     sim.setInteractionSphere()
     if material == "viscoplastic":
@@ -359,6 +359,10 @@ def main():
     ap.add_argument("--compact-state", action="store_true",
                     help="viscoplastic: the fused tiled step keeps the symmetric plastic deviation only "
                          "(same forces; no Fp)")
+    ap.add_argument("--step-form", choices=("auto", "fused", "two"), default="auto",
+                    help="fused: the node update of each step rides in the next force pass (one launch per "
+                         "step); two: a force pass and a node pass; auto: the autotuner's choice, or by mesh "
+                         "size with a fixed --tile-k")
     ap.add_argument("--streamed-shape", action="store_true",
                     help="the tiled force pass reads the precomputed Dm^-1 and V0 instead of forming them "
                          "from the reference positions")
@@ -405,7 +409,8 @@ def main():
     P, T = cached_mesh(cfg)
     N, nT = len(P), len(T)
     compact = args.compact_state and material == "viscoplastic" and args.scatter == "tiled"
-    sim = make_sim(cfg, P, T, args.scatter, material, args.tile_k or 512, not args.streamed_shape, compact)
+    sim = make_sim(cfg, P, T, args.scatter, material, args.tile_k or 512, not args.streamed_shape, compact,
+                   {"auto": None, "fused": True, "two": False}[args.step_form])
     assert sim.plan() is not None
     move_steps = int(round(cfg["MoveTimeSI"] / sim.T0))
     start = (move_steps if args.workload == "large" else 0) if args.start_step < 0 else args.start_step
@@ -432,30 +437,39 @@ def main():
     ms_force, ms_node = pass_times(sim, min(args.steps, 100))
     vp = material == "viscoplastic"
     plastic_tiles = sim.plastic_tile_fraction()
-    force_bytes = SPEC_TET_BYTES[material] * nT + 24 * N
+    # node-fused: the dominant kernel is the force pass carrying the previous step's node update, so
+    # one launch moves a whole step's algorithmic bytes
+    node_fused = bool(sim.plan().disp_b)
+    step_bytes = SPEC_TET_BYTES[material] * nT + 108 * N
+    force_bytes = step_bytes if node_fused else SPEC_TET_BYTES[material] * nT + 24 * N
     geo = bool(sim.plan().shape_from_ref) and args.scatter == "tiled"
     tet_bytes = KERNEL_TET_BYTES[material] + (0 if geo else STREAMED_SHAPE_BYTES[material])
     if compact:  # Gd 24 + 24 instead of Q 36 + 36; epsbar only when the flow reads it
         tet_bytes -= 2 * 12 + (0 if sim.plan().epsbar else 8)
     kernel_bytes = tet_bytes * nT + (KERNEL_NODE_BYTES + (12 if geo else 0)) * N
-    step_bytes = SPEC_TET_BYTES[material] * nT + 108 * N
+    if node_fused:  # the node update: the previous partials (12 B each, M of them), node_off 4, w read,
+        # u and w written (12 each) per node
+        kernel_bytes += 12 * sim.node_tiles()["m_total"] + (4 + 3 * 12) * N
     achieved = force_bytes / (ms_force * 1e-3) / 1e9
     nt_flag = sim.plan().stream_nt
     kernel = (f"k_tile_force<{'NFX_VISCOPLASTIC' if vp else 'NFX_MR'}, K={sim.tile_k}, "
               f"{'shape from reference positions' if geo else 'streamed shape'}, "
-              f"{'nontemporal' if nt_flag else 'default'} cache policy>") if args.scatter == "tiled" else \
-        f"{'k_vp' if vp else 'k_tetra'}<{args.scatter}>"
+              f"{'nontemporal' if nt_flag else 'default'} cache policy"
+              f"{', node-fused (carries the previous step node update)' if node_fused else ''}>") \
+        if args.scatter == "tiled" else f"{'k_vp' if vp else 'k_tetra'}<{args.scatter}>"
+    prof_key = (f"{material}_{args.scatter}_{sim.tile_k}{'nt' if nt_flag else ''}{'_shape_from_ref' if geo else ''}"
+                f"{'_node_fused' if node_fused else ''}")
     traffic, traffic_source = None, None
     if args.traffic_json and os.path.exists(args.traffic_json):
         tj = json.load(open(args.traffic_json))
-        key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if nt_flag else ''}{'_shape_from_ref' if geo else ''}"
+        key = prof_key
         if tj.get("tetra") == nT and key in tj.get("force_pass_bytes", {}) and start == tj.get("from_step"):
             traffic = tj["force_pass_bytes"][key]
             traffic_source = f"{os.path.relpath(args.traffic_json, ROOT)}[{key}]: {tj.get('source', '')}"
     compute = None
     if args.valu_json and os.path.exists(args.valu_json):
         vj = json.load(open(args.valu_json))
-        key = f"{material}_{args.scatter}_{sim.tile_k}{'nt' if nt_flag else ''}{'_shape_from_ref' if geo else ''}"
+        key = prof_key
         if vj.get("tetra") == nT and key in vj.get("force_pass", {}) and start == vj.get("from_step"):
             fl = vj["force_pass"][key]["fp32_flops"]
             compute = {"bound": "valu", "flops_per_launch": fl, "achieved": fl / (ms_force * 1e-3) / 1e12,
@@ -491,6 +505,7 @@ def main():
             "timed_from_step": start,
             "protocol_steps": int(sim.Nsteps),
             "shape_from_ref": geo,
+            "step_form": "node-fused (one launch per step)" if node_fused else "two launches per step",
             "viscoplastic_state": ("compact: symmetric deviation Gd" + ("" if sim.plan().epsbar else ", no plastic strain")
                                    if compact else "Q = Fp^-1 - I and plastic strain") if vp else None,
             "autotune_ms_per_step": tuned,
@@ -510,9 +525,11 @@ def main():
             "traffic": traffic,
             "traffic_source": traffic_source,
             "algorithmic_bytes_per_launch": force_bytes,
+            "node_fused": node_fused,
             "kernel_bytes_per_launch": kernel_bytes,
             "plastic_tile_fraction": plastic_tiles,
-            "note": ("achieved counts the specification's bytes per launch (136 B/tetra + 24 B/node); the kernel "
+            "note": (f"achieved counts the specification's bytes per launch (136 B/tetra + "
+                     f"{108 if node_fused else 24} B/node{': the launch carries the previous step node update, so a step is one launch' if node_fused else ''}); the kernel "
                      "forms Dm^-1 from the reference positions and touches the plastic state only in tiles that "
                      "have yielded and tetra that flow, so it moves far less (traffic, measured) and is bound by "
                      "VALU issue (profiles/instructions_large_cell_viscoplastic_tiled.txt)") if vp and geo else None,
@@ -520,6 +537,15 @@ def main():
             "compute": compute,
         },
     }
+    if args.scatter == "tiled" and not args.no_extra:  # the same cell and window in the other step form
+        sim.fuse_node_pass = not node_fused
+        if bool(sim.plan().disp_b) != node_fused:
+            ms6 = time_steps(sim, args.warmup, min(args.steps, 1000))[0] * 1e3
+            f6, n6 = pass_times(sim, 50)
+            out["other_step_form"] = {"step": "two launches per step" if node_fused else
+                                      "node-fused (one launch per step)", "ms_per_step": ms6,
+                                      "value": nT * 1e3 / ms6, "ms_force_pass": f6, "ms_node_pass": n6}
+        sim.fuse_node_pass = node_fused
     sim_tile_k, sim_stream_nt = sim.tile_k, sim.stream_nt
     del sim
     torch.cuda.empty_cache()
@@ -535,7 +561,7 @@ def main():
                                          "ms_force_pass": f2, "ms_node_pass": n2})
             del s2
         if vp and args.scatter == "tiled":  # the other viscoplastic state form, same tile shape and window
-            s4 = make_sim(cfg, P, T, "tiled", material, sim_tile_k, True, not compact)
+            s4 = make_sim(cfg, P, T, "tiled", material, sim_tile_k, True, not compact, node_fused)
             s4.stream_nt = sim_stream_nt
             if start > 0:
                 s4.advance(start)
@@ -555,12 +581,13 @@ def main():
         f3, n3 = pass_times(s3, 200)
         out["plain_small"] = {"workload": f"small_cell {len(Ps)} nodes {len(Ts)} tetra NH", "ms_per_step": ms3,
                               "value": len(Ts) * 1e3 / ms3, "unit": "tetra-updates/s", "tile_k": k3,
+                              "node_fused": bool(s3.plan().disp_b),
                               "ms_force_pass": f3, "ms_node_pass": n3, "autotune_ms_per_step": tuned3}
         del s3
     if not args.no_extra and not args.no_full_protocol and args.scatter == "tiled":
         # the whole protocol from rest on a fresh cell, graph captures included (one per distinct
         # block length, milliseconds against seconds)
-        s5 = make_sim(cfg, P, T, "tiled", material, sim_tile_k, not args.streamed_shape, compact)
+        s5 = make_sim(cfg, P, T, "tiled", material, sim_tile_k, not args.streamed_shape, compact, node_fused)
         s5.stream_nt = sim_stream_nt
         n5 = int(s5.Nsteps)
         sync(s5)

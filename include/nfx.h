@@ -154,6 +154,19 @@ typedef struct {
                                flags with nfx_vp_tile_flags.  Independently of the flags, a tetra writes its
                                state back only in a step in which it flows (the state is unchanged otherwise).
                                Results are bitwise those without flags */
+    /* Node-fused steps (NFX_SCATTER_TILED with NFX_MR, NFX_SVK, NFX_SVKK or NFX_VISCOPLASTIC; the four
+       below set or all NULL; every node must be a corner of some tetra).  With them, a call of n >= 2 steps
+       runs n - 1 of its node passes inside the next step's force pass: each tile first updates the
+       nodes it touches from the previous pass's partials (every tile that touches a node computes
+       the same values; the node's first tile stores them) and then computes its forces from the new
+       displacements.  A step is then one launch instead of two.  The launches alternate between
+       (disp, vel, partials) and these second buffers; between calls the state is in disp, vel and
+       fold as without them.  Scratch: their contents between calls mean nothing. */
+    float* disp_b;     /* 3N */
+    float* vel_b;      /* 3N */
+    float* partials_b; /* 3*M, as partials */
+    const uint32_t* tile_range; /* node-fused steps: ntiles*MX*2, for tile-node j of tile c the range
+                                   [node_off[n], node_off[n+1]) of its node n's partials at 2*(c*MX + j) */
 } nfx_plan;
 
 /* Per-cell parameters of a batch, in that cell's simulation units.  The batched material is the
@@ -343,7 +356,11 @@ int nfx_hard_adhesive_plane_surface_force(nfx_ctx* ctx, float* force, const floa
  *      n >= 2 steps the first step turns it into w = a v + f_old/2 (kept in vel), the middle steps
  *      advance (disp, w) without a previous force (24 B per node less), and the last step turns w
  *      back into (vel, fold): the same recurrence, with rounding that depends on how a run is cut
- *      into calls (the same cuts give the same bits). ------ */
+ *      into calls (the same cuts give the same bits).  With the node-fused buffers in the plan
+ *      (disp_b, vel_b, partials_b), steps 1 .. n-1 of the call are one launch each (the node update
+ *      of the step before rides in the force pass) and a last node pass ends the call; the first
+ *      call with a tip history allocates the context's per-tile indenter-force rows (ceil(T/tile_k)
+ *      x 100 doubles), so a caller that captures this call makes one such call beforehand. ------ */
 int nfx_run_steps(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int64_t nsteps, double* tip_hist,
                   int32_t use_graph);
 /* number of step blocks the context has captured into graphs so far (a replay of a cached block
@@ -365,7 +382,9 @@ int nfx_batch_time_passes(nfx_ctx* ctx, const nfx_plan* plan, const nfx_batch* b
 
 /* ---- kernel timing for the benchmark: average device time of the force pass over nrep steps and
  *      of the node pass over the middle steps of those nrep (the (disp, w) form; all of them when
- *      nrep < 3), measured with HIP events on the context's stream (ms). ---------- */
+ *      nrep < 3), measured with HIP events on the context's stream (ms).  For a plan with the
+ *      node-fused buffers and nrep >= 3, ms_force is the average node-fused launch over the middle
+ *      steps (node update of the step before + force pass) and ms_node the call's last node pass. */
 int nfx_time_passes(nfx_ctx* ctx, const nfx_plan* plan, int64_t step0, int32_t nrep, double* ms_force,
                     double* ms_node);
 

@@ -55,7 +55,7 @@ def _law(fn, start, speed, move_steps):
 
 class Simulation:
     def __init__(self, con, scatter="tiled", fused=True, use_graph=True, mesh=None, verbose=True, tile_k=512,
-                 stream_nt=None, shape_from_ref=True):
+                 stream_nt=None, shape_from_ref=True, fuse_node_pass=None):
         if isinstance(con, dict):
             self.parameters = dict(con)
         else:
@@ -67,6 +67,10 @@ class Simulation:
         self.tile_k = tile_k
         self.stream_nt = stream_nt
         self.shape_from_ref = shape_from_ref
+        # tiled MR/SVK/SVKK/viscoplastic steps: each node update rides in the next step's force pass
+        # (one launch per step instead of two; needs every node to be a corner of some tetra).  None:
+        # by mesh size (autotune.default_node_fused), True/False: forced; the autotuner times both
+        self.fuse_node_pass = fuse_node_pass
         self.fused = fused
         self.use_graph = use_graph
         self.verbose = verbose
@@ -138,6 +142,7 @@ class Simulation:
         self._vp_active = None  # per-tile plastic-activity flags of the tiled viscoplastic step
         self._vp_flags_stale = True
         self.use_plastic_flags = True
+        self._node_b = None  # second (disp, vel) buffers of the node-fused step
 
     # ---------------------------------------------------------------------------------------------
     # the two position forms
@@ -488,6 +493,13 @@ class Simulation:
                       node_off.ctypes.data_as(vp))
             dev = self.device
             to = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a.view(np.int16)).to(dev)
+            # per tile-node, the range [node_off[n], node_off[n+1]) of its node's partials (the node-fused
+            # step reads them with the tile metadata); zero for the padding past a tile's count
+            tn = tile_nodes.reshape(ntiles, MX).astype(np.int64)
+            valid = np.arange(MX)[None, :] < count[:, None]
+            tile_range = np.zeros((ntiles, MX, 2), np.uint32)
+            tile_range[..., 0] = np.where(valid, node_off[tn], 0)
+            tile_range[..., 1] = np.where(valid, node_off[tn + 1], 0)
             self._tiles = {
                 "tile_k": K,
                 "max_nodes": MX,
@@ -498,7 +510,11 @@ class Simulation:
                 "tile_corners": to(tile_corners),
                 "tile_dst": to(tile_dst),
                 "node_off": to(node_off),
+                "tile_range": to(tile_range.reshape(-1)),
                 "partials": torch.zeros(3 * M, dtype=torch.float32, device=dev),
+                # every node is a corner of some tetra (has a partial): the node-fused step updates
+                # nodes only through the tiles that touch them
+                "every_node": bool(np.all(node_off[1:] > node_off[:-1])),
             }
             torch.cuda.synchronize(dev)
         return self._tiles
@@ -545,7 +561,7 @@ class Simulation:
     def _build_plan(self):
         if not self.fused:
             return None
-        key = tuple(id(i) for i in self.interactions) + (self.scatter,)
+        key = tuple(id(i) for i in self.interactions) + (self.scatter, self.node_fused_wanted())
         if self._plan is not None and self._plan_key == key:
             return self._plan
         p = _lib.Plan()
@@ -584,10 +600,30 @@ class Simulation:
                 self._vp_flags_stale = True
             for k in ("tile_count", "tile_nodes", "tile_lend", "tile_corners", "tile_dst", "node_off", "partials"):
                 setattr(p, k, t[k].data_ptr())
+            if self.node_fused_wanted() and t["every_node"] and p.model in (_lib.NFX_SVKK, _lib.NFX_SVK, _lib.NFX_MR,
+                                                                        _lib.NFX_VISCOPLASTIC):
+                import torch
+
+                if self._node_b is None:
+                    self._node_b = torch.zeros(2, 3 * self.numPoints, dtype=torch.float32, device=self.device)
+                if "partials_b" not in t:
+                    t["partials_b"] = torch.zeros_like(t["partials"])
+                p.disp_b, p.vel_b = self._node_b[0].data_ptr(), self._node_b[1].data_ptr()
+                p.partials_b = t["partials_b"].data_ptr()
+                p.tile_range = t["tile_range"].data_ptr()
         else:
             p.scatter = _lib.SCATTER_ATOMIC
         self._plan, self._plan_key = p, key
         return p
+
+    def node_fused_wanted(self):
+        """whether the fused tiled step should take the node-fused form (fuse_node_pass, or by mesh
+        size when it is None)"""
+        if self.fuse_node_pass is None:
+            from nofluidx3d_amd.autotune import default_node_fused
+
+            return default_node_fused(self.numTetra)
+        return bool(self.fuse_node_pass)
 
     def advance(self, nsteps, tip_hist=None):
         """Advance ``nsteps`` steps (fused when possible).  ``tip_hist``: optional float64 device

@@ -99,6 +99,10 @@ class Plan(ctypes.Structure):
         ("stream_nt", c_i32),
         ("shape_from_ref", c_i32),
         ("vp_active", c_vp),
+        ("disp_b", c_vp),
+        ("vel_b", c_vp),
+        ("partials_b", c_vp),
+        ("tile_range", c_vp),
     ]
 
 

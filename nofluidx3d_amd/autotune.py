@@ -1,7 +1,8 @@
 """Launch-shape autotuner for the tiled force pass (single cells and batches).
 
-Candidates are timed over a few steps (a single cell: the force and node passes with HIP events,
-``nfx_time_passes``; a batch: graph-replayed steps, launch costs included); the fastest wins and is cached per problem signature in a JSON file
+Candidates are timed over graph-replayed steps, launch costs included (a single cell: each tile
+size, cache policy and step form; a batch: each tile size and cache policy, and cell groups); the
+fastest wins; batch choices are cached per problem signature in a JSON file
 (``$NFX_TUNE_CACHE`` or ``~/.cache/nofluidx3d_amd/tune.json``).  Every candidate computes the same
 physics; tile size only changes the floating-point summation order, by rounding.
 
@@ -11,6 +12,8 @@ Tunables:
 - ``stream_nt``: cache policy of the force pass's per-tetra streams (state, tile tables): 1 =
   nontemporal, 0 = default.  A state that fits MI355X's 256 MB memory-side cache is served from it
   from step to step under the default policy; a larger one streams faster past it;
+- the step form (single cells): node-fused (the node update of each step rides in the next force
+  pass: one launch per step) or two launches per step;
 - ``graph_steps``: steps per captured hipGraph replay block (the library captures at most 100);
 - ``group_tetra`` (batches): run the cells as consecutive groups of at most this many tetra, each
   group all its steps before the next (``batch.CellGroups``), so that a group's working set stays
@@ -38,6 +41,18 @@ RESIDENT_BYTES_PER_TETRA = 56
 def default_stream_nt(tetra):
     """the cache policy to use untuned: nontemporal once the streamed state exceeds the cache"""
     return int(STATE_BYTES_PER_TETRA * int(tetra) > MEMORY_SIDE_CACHE_BYTES)
+
+
+# the node-fused step (one launch per step) wins where launches dominate a step; on a mesh of
+# millions of tetra its force pass, which then also updates every node it touches (~1.7 times per
+# node), is longer than the two passes apart (measured on MI355X: 25k tetra 5.6 against 7.2 us per
+# step, 3.4M tetra 58 against 50 us)
+NODE_FUSED_MAX_TETRA = SMALL_MESH_TETRA
+
+
+def default_node_fused(tetra):
+    """the step form to use untuned: node-fused for meshes of at most NODE_FUSED_MAX_TETRA tetra"""
+    return int(tetra) <= NODE_FUSED_MAX_TETRA
 
 
 def group_tetra():
@@ -138,11 +153,14 @@ def tune_batch(specs, meshes, steps, ncurve=256, nsteps=300, use_cache=True, log
     return best, times
 
 
-def tune_simulation(sim, nrep=20, log=None):
-    """Pick ``tile_k`` and ``stream_nt`` for a single-cell Simulation with the tiled scatter (sets
-    ``sim.tile_k`` and ``sim.stream_nt``).  Returns ((tile_k, stream_nt), {label: ms_per_step})."""
-    import ctypes
-
+def tune_simulation(sim, nsteps=300, log=None):
+    """Pick ``tile_k``, ``stream_nt`` and the step form for a single-cell Simulation with the tiled
+    scatter: each candidate runs ``nsteps`` steps to capture its graph, then ``nsteps`` graph-replayed
+    steps timed on the host clock, launch costs included (what a run of the cell pays per step).
+    The step forms are the node-fused one (one launch per step, when the plan admits it and
+    ``sim.fuse_node_pass`` is not False) and two launches per step.  The state is restored afterwards.
+    Sets ``sim.tile_k``, ``sim.stream_nt`` and ``sim.fuse_node_pass``; returns ((tile_k, stream_nt),
+    {label: ms_per_step}) with labels "256", "256nt", ... and an "f" suffix for the node-fused form."""
     import torch
 
     from nofluidx3d_amd import _lib, openCL
@@ -153,33 +171,39 @@ def tune_simulation(sim, nrep=20, log=None):
     sim.sync_forms()
     openCL.getCommandQueue().finish()
     snapshot = [t.clone() for t in sim.state_tensors()]  # timing runs real steps: restore afterwards
+    forms = (False,) if sim.fuse_node_pass is False else (True, False)
     times, choice = {}, {}
     cands = ((128,) if sim.numTetra < SMALL_MESH_TETRA else ()) + TILE_CANDIDATES
     for K in cands:
         sim.tile_k = K
         sim._tiles = None
         for nt in STREAM_CANDIDATES:
-            sim.stream_nt = nt
-            sim._plan = None
-            p = sim.plan()
-            if p is None or sim.scatter != "tiled":
-                sim.scatter = "tiled"
-                break
-            mf, mn = ctypes.c_double(), ctypes.c_double()
-            _lib.call("nfx_time_passes", openCL.handle(), p, sim.time, nrep, ctypes.byref(mf), ctypes.byref(mn))
-            sim.time += nrep
-            lab = _label(K, nt)
-            times[lab], choice[lab] = mf.value + mn.value, (K, nt)
-            if log:
-                log(f"{lab}: force {mf.value:.4f} ms, node {mn.value:.4f} ms")
+            for fused in forms:
+                sim.stream_nt, sim.fuse_node_pass = nt, fused
+                sim._plan = None
+                p = sim.plan()
+                if p is None or sim.scatter != "tiled":
+                    sim.scatter = "tiled"
+                    break
+                if fused and not p.disp_b:  # the plan cannot take the node-fused form
+                    continue
+                sim.advance(nsteps)
+                _lib.call("nfx_stream_sync", openCL.handle())
+                t0 = time.perf_counter()
+                sim.advance(nsteps)
+                _lib.call("nfx_stream_sync", openCL.handle())
+                lab = _label(K, nt) + ("f" if fused else "")
+                times[lab], choice[lab] = (time.perf_counter() - t0) / nsteps * 1e3, (K, nt, fused)
+                if log:
+                    log(f"{lab}: {times[lab]:.4f} ms/step")
     openCL.getCommandQueue().finish()
     for t, s0 in zip(sim.state_tensors(), snapshot):
         t.copy_(s0)
     torch.cuda.synchronize()
     sim.forms_restored()
     sim.time = saved_time
-    best = choice[min(times, key=times.get)]
-    sim.tile_k, sim.stream_nt = best
+    K, nt, fused = choice[min(times, key=times.get)]
+    sim.tile_k, sim.stream_nt, sim.fuse_node_pass = K, nt, fused
     sim._tiles = None
     sim._plan = None
-    return best, times
+    return (K, nt), times

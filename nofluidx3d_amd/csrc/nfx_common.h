@@ -58,6 +58,8 @@ struct nfx_ctx {
     unsigned int* counter = nullptr;  // 1 counter (reset by the last block of each reduction)
     int64_t* step_dev = nullptr;      // device step counter for graph replays
     double* hist_dev = nullptr;       // kGraphBlock doubles, tip force of the replayed block
+    double* tip_rows = nullptr;       // node-fused steps: per tile and step of a block, the indenter force
+    int64_t tip_rows_cap = 0;         // doubles allocated at tip_rows
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // batched stepping: the batch being enqueued (NULL otherwise)
     const nfx_batch* batch = nullptr;

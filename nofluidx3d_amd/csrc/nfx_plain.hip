@@ -333,6 +333,133 @@ __global__ __launch_bounds__(kBlock) void k_disp_convert(const float* __restrict
     out[i] = TO_POINTS ? ref[i] + in[i] : in[i] - ref[i];
 }
 
+// ---- node helpers shared by the node pass and the node-fused force pass -----------------------
+// Sum of a node's tile partials [b, e) in order, four records' loads in flight at a time (zeros past
+// the end, which leave the sum unchanged): a node has 1-8 partials, and a loop of dependent single
+// loads would wait a full memory latency per partial.
+__device__ __forceinline__ V3 sum_partials(const float* __restrict__ partials, uint32_t b, uint32_t e) {
+    V3 f = mk(0.f, 0.f, 0.f);
+    for (uint32_t j = b; j < e; j += 4) {
+        V3 q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float* p = partials + 3 * (int64_t)(j + i);
+            q[i] = (j + i < e) ? mk(p[0], p[1], p[2]) : mk(0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) f = f + q[i];
+    }
+    return f;
+}
+
+// The same sum with the first eight records' loads issued together (one memory round for nodes of
+// up to eight partials, most of them), then four at a time; the additions run in the same order
+__device__ __forceinline__ V3 sum_partials8(const float* __restrict__ partials, uint32_t b, uint32_t e) {
+    V3 q[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float* p = partials + 3 * (int64_t)(b + i);
+        q[i] = (b + i < e) ? mk(p[0], p[1], p[2]) : mk(0.f, 0.f, 0.f);
+    }
+    V3 f = mk(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) f = f + q[i];
+    for (uint32_t j = b + 8; j < e; j += 4) {
+        V3 r[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float* p = partials + 3 * (int64_t)(j + i);
+            r[i] = (j + i < e) ? mk(p[0], p[1], p[2]) : mk(0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) f = f + r[i];
+    }
+    return f;
+}
+
+// The contact terms of a node at x = X0 + u, in FP64: the substrate (Substrate.cl:4-5), the
+// indenter (Sphere.cl:9-16 or Plane.cl:4-5) at the FP64 position pos, added to f; the indenter's
+// own force and the penetration for the tip reduction.  X0.x/z and u.x/z are read only for nodes
+// the sphere can touch (y > pos - R): a few per cent of a cell.
+template <bool SUB, int IND>
+__device__ __forceinline__ void node_contacts(const float* __restrict__ ref, int64_t N, int64_t n, V3 u, double sub_k,
+                                              double wall, double ind_k, double ind_radius, double pos, V3& f,
+                                              double& tip_f, double& tip_dis) {
+    const double y = (double)ref[N + n] + (double)u.y;
+    if constexpr (SUB) f.y += (float)(sub_k * fmax(wall - y, 0.0));
+    if constexpr (IND == NFX_INDENTER_SPHERE) {
+        if (y > pos - ind_radius) {
+            const Contact c = sphere_term_d((double)ref[n] + (double)u.x, y, (double)ref[2 * N + n] + (double)u.z,
+                                            ind_k, ind_radius, pos);
+            f = f + mk((float)c.fx, (float)c.fy, (float)c.fz);
+            tip_f = -c.fy;
+            tip_dis = c.dis;
+        }
+    } else if constexpr (IND == NFX_INDENTER_PLANE) {
+        const double dis = fmax(y - pos, 0.0);
+        f.y -= (float)(ind_k * dis);
+        tip_f = ind_k * dis;
+        tip_dis = dis;
+    }
+}
+
+// The previous step's node update riding in a node-fused force pass: every tile that touches node n
+// updates it from the same inputs with the same operations (so the copies agree bitwise) and the
+// node's first tile (its partial is the node's first, dst == node_off[n]) stores the result.
+struct NodeStep {
+    const float* __restrict__ prev_partials;  // the previous force pass's partials, node order
+    const uint2* __restrict__ range;          // per tile-node, its node's partials [b, e)
+    const float* __restrict__ w;     // w = a v + f_old/2, or v when vv_first
+    const float* __restrict__ fold;  // vv_first: f_old
+    float* __restrict__ x_next;      // the displacements after the step
+    float* __restrict__ w_next;
+    double* __restrict__ tip_row;  // nullable: per tile, the indenter force on the nodes it stores
+    int32_t substrate, indenter, fix, vv_first;
+    double sub_k, wall, ind_k, ind_radius, ind_start, ind_speed, ind_move;
+    int64_t step;  // the step being completed (step_dev + step_off when step_dev is set)
+    const int64_t* step_dev;
+    int32_t step_off;
+};
+
+// node n's update (k_node_update's arithmetic, the substrate/indenter/fix choices uniform at run
+// time): x holds u before the step on entry and after it on return; tip receives the node's indenter
+// force when this tile stores the node
+__device__ __forceinline__ void node_step(const NodeStep& s, const float* __restrict__ ref, int64_t N, uint32_t n,
+                                          uint32_t dst, uint2 rg, double pos, V3& x, double& tip) {
+    const uint32_t b = rg.x, e = rg.y;
+    V3 f = sum_partials8(s.prev_partials, b, e);
+    double tf = 0.0, td = 0.0;
+    if (s.substrate) {
+        if (s.indenter == NFX_INDENTER_SPHERE)
+            node_contacts<true, NFX_INDENTER_SPHERE>(ref, N, n, x, s.sub_k, s.wall, s.ind_k, s.ind_radius, pos, f, tf, td);
+        else if (s.indenter == NFX_INDENTER_PLANE)
+            node_contacts<true, NFX_INDENTER_PLANE>(ref, N, n, x, s.sub_k, s.wall, s.ind_k, s.ind_radius, pos, f, tf, td);
+        else
+            node_contacts<true, NFX_INDENTER_NONE>(ref, N, n, x, s.sub_k, s.wall, s.ind_k, s.ind_radius, pos, f, tf, td);
+    } else {
+        if (s.indenter == NFX_INDENTER_SPHERE)
+            node_contacts<false, NFX_INDENTER_SPHERE>(ref, N, n, x, s.sub_k, s.wall, s.ind_k, s.ind_radius, pos, f, tf, td);
+        else if (s.indenter == NFX_INDENTER_PLANE)
+            node_contacts<false, NFX_INDENTER_PLANE>(ref, N, n, x, s.sub_k, s.wall, s.ind_k, s.ind_radius, pos, f, tf, td);
+    }
+    V3 w = mk(s.w[n], s.w[N + n], s.w[2 * N + n]);
+    if (s.vv_first) {
+        w = vv_w_from_v(w, f, mk(s.fold[n], s.fold[N + n], s.fold[2 * N + n]));
+        x = x + w;
+    } else {
+        vv_update_w(x, w, f);
+    }
+    if (s.fix && (n == 3 || n == 4)) {  // x = z = 0 (VelocityVerlet.cl:17-22): u = -X0
+        x.x = -ref[n];
+        x.z = -ref[2 * N + n];
+    }
+    if (dst == b) {
+        s.x_next[n] = x.x; s.x_next[N + n] = x.y; s.x_next[2 * N + n] = x.z;
+        s.w_next[n] = w.x; s.w_next[N + n] = w.y; s.w_next[2 * N + n] = w.z;
+        tip = tf;
+    }
+}
+
 // ---- force pass, tiled deterministic scatter ---------------------------------------------------
 // One workgroup per tile of K consecutive (Morton-ordered) tetra, one thread per tetra.  The
 // tile's distinct nodes are listed on the host (tile_nodes, ascending, MX <= 256 entries per tile)
@@ -388,6 +515,8 @@ struct TileArgs {
     int64_t curve_steps, curve_step;
     const int64_t* curve_step_dev;
     int32_t curve_off;
+    // NODE: the previous step's node update (x holds the displacements before it)
+    NodeStep ns;
 };
 
 // One cell's sample of the batched curves, by one workgroup of any size (<= 512): on the first
@@ -440,7 +569,7 @@ __host__ __device__ constexpr size_t tile_lds_bytes_per_node(bool geo) { return 
 // metadata (batches).  Waves per SIMD: the viscoplastic forms need 70-77 VGPRs (6 waves) except the
 // symmetric one without epsbar (64, 8 waves; 7 with GEO, whose 64-register bound spilled)
 template <int MODEL, int K, bool GEO, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false,
-          bool NARROW = false>
+          bool NARROW = false, bool NODE = false>
 __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || STRESS) ? 6 : GEO ? 7 : 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
@@ -452,6 +581,7 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
     // as one 8-byte word, so a corner's node data is one or two LDS reads at one address
     float4* s_n = reinterpret_cast<float4*>(s_cf + 12 * K);
     __shared__ int s_flow;                         // viscoplastic: a tetra of the tile flowed this step
+    __shared__ double s_tip[NODE ? K / 64 : 1];    // NODE: per wave, the indenter force of the previous step
     float2* s_r = reinterpret_cast<float2*>(s_n + MX);
     uint16_t* s_lb = reinterpret_cast<uint16_t*>(s_r + (GEO ? MX : 0));
 
@@ -517,7 +647,9 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
     const bool owner = tl < MX;
     uint32_t nid = 0, dst = 0;
     int lb = 0, le = 0;
+    uint2 rg = make_uint2(0u, 0u);  // NODE: the node's partials of the previous force pass
     if (owner) {
+        if constexpr (NODE) rg = a.ns.range[j0 + tl];
         if constexpr (NARROW) {
             nid = a.tile_node_base[c] + (uint32_t)ld_g<NT, uint16_t>(uat(a.tile_nodes16 + j0, u));
             dst = a.tile_dst_base[c] + (uint32_t)ld_g<NT, uint16_t>(uat(a.tile_dst16 + j0, u));
@@ -531,8 +663,22 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
     // this tetra's 4 corner records (tile-node | rank << 8), as one 8-byte load
     uint64_t cr = 0;
     if (active) cr = ld_g<NT, uint64_t>(uat(reinterpret_cast<const uint64_t*>(a.tile_corners + 4 * K * c), u));
+    double tip = 0.0;  // NODE: the indenter force on the node this thread stores
     if (owner) {
-        const float ux = a.x[nid], uy = a.x[N + nid], uz = a.x[2 * N + nid];
+        float ux = a.x[nid], uy = a.x[N + nid], uz = a.x[2 * N + nid];
+        if (NODE && tl < M) {  // complete the previous step at this node (padding entries past M are
+                               // not nodes of the tile); the forces below see the result
+            double pos = 0.0;
+            if (a.ns.indenter) {
+                const int64_t time = a.ns.step_dev ? (*a.ns.step_dev + a.ns.step_off) : a.ns.step;
+                pos = indenter_pos(a.ns.ind_start, a.ns.ind_speed, a.ns.ind_move, time);
+            }
+            V3 xn = mk(ux, uy, uz);
+            node_step(a.ns, a.ref, N, nid, dst, rg, pos, xn, tip);
+            ux = xn.x;
+            uy = xn.y;
+            uz = xn.z;
+        }
         if constexpr (GEO) {
             s_n[tl] = make_float4(ux, uy, uz, a.ref[nid]);
             s_r[tl] = make_float2(a.ref[N + nid], a.ref[2 * N + nid]);
@@ -542,7 +688,22 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         s_lb[tl] = (uint16_t)lb;
     }
     if (MODEL == NFX_VISCOPLASTIC && tl == 0) s_flow = 0;
+    if constexpr (NODE) {
+        if (a.ns.tip_row) {  // the tile's share of the indenter force: lanes by a fixed butterfly, then waves
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) tip += __shfl_xor(tip, off);
+            if ((tl & 63) == 0) s_tip[tl >> 6] = tip;
+        }
+    }
     __syncthreads();
+    if constexpr (NODE) {
+        if (a.ns.tip_row && tl == 0) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < K / 64; ++w) t += s_tip[w];
+            a.ns.tip_row[c] = t;
+        }
+    }
     int flows = 0;
     if (active) {
         const int l0 = (int)(cr & 0xFF), l1 = (int)((cr >> 16) & 0xFF), l2 = (int)((cr >> 32) & 0xFF),
@@ -702,29 +863,13 @@ __device__ void reduce_finish(double s, double m, double* partials, unsigned int
 }
 
 // ---- node pass ------------------------------------------------------------------------------
-// Sum of a node's tile partials [b, e) in order, four records' loads in flight at a time (zeros past
-// the end, which leave the sum unchanged): a node has 1-8 partials, and a loop of dependent single
-// loads would wait a full memory latency per partial.
-__device__ __forceinline__ V3 sum_partials(const float* __restrict__ partials, uint32_t b, uint32_t e) {
-    V3 f = mk(0.f, 0.f, 0.f);
-    for (uint32_t j = b; j < e; j += 4) {
-        V3 q[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float* p = partials + 3 * (int64_t)(j + i);
-            q[i] = (j + i < e) ? mk(p[0], p[1], p[2]) : mk(0.f, 0.f, 0.f);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) f = f + q[i];
-    }
-    return f;
-}
-
 struct NodeArgs {
     int64_t N;
     float* x;  // displacements u (fused node pass) or absolute positions (per-plugin kernels)
     const float* __restrict__ ref;
     float* v;
+    float* xo;  // where the node pass stores x and v (x and v themselves, or the other buffers of a
+    float* vo;  // node-fused call's last step)
     float* force;
     float* fold;
     const uint32_t* __restrict__ off;
@@ -755,32 +900,6 @@ __device__ __forceinline__ V3 gather_node(const NodeArgs& a, int64_t n, int sub)
         f.z += __shfl_xor(f.z, o);
     }
     return f;
-}
-
-// The contact terms of a node at x = X0 + u, in FP64: the substrate (Substrate.cl:4-5), the
-// indenter (Sphere.cl:9-16 or Plane.cl:4-5) at the FP64 position pos, added to f; the indenter's
-// own force and the penetration for the tip reduction.  X0.x/z and u.x/z are read only for nodes
-// the sphere can touch (y > pos - R): a few per cent of a cell.
-template <bool SUB, int IND>
-__device__ __forceinline__ void node_contacts(const float* __restrict__ ref, int64_t N, int64_t n, V3 u, double sub_k,
-                                              double wall, double ind_k, double ind_radius, double pos, V3& f,
-                                              double& tip_f, double& tip_dis) {
-    const double y = (double)ref[N + n] + (double)u.y;
-    if constexpr (SUB) f.y += (float)(sub_k * fmax(wall - y, 0.0));
-    if constexpr (IND == NFX_INDENTER_SPHERE) {
-        if (y > pos - ind_radius) {
-            const Contact c = sphere_term_d((double)ref[n] + (double)u.x, y, (double)ref[2 * N + n] + (double)u.z,
-                                            ind_k, ind_radius, pos);
-            f = f + mk((float)c.fx, (float)c.fy, (float)c.fz);
-            tip_f = -c.fy;
-            tip_dis = c.dis;
-        }
-    } else if constexpr (IND == NFX_INDENTER_PLANE) {
-        const double dis = fmax(y - pos, 0.0);
-        f.y -= (float)(ind_k * dis);
-        tip_f = ind_k * dis;
-        tip_dis = dis;
-    }
 }
 
 // GATHER: 0 = read the atomically accumulated force (and clear it), 1 = corner records via the
@@ -839,8 +958,8 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
                 x.z = -a.ref[2 * N + n];
             }
         }
-        a.x[n] = x.x; a.x[N + n] = x.y; a.x[2 * N + n] = x.z;
-        a.v[n] = v.x; a.v[N + n] = v.y; a.v[2 * N + n] = v.z;
+        a.xo[n] = x.x; a.xo[N + n] = x.y; a.xo[2 * N + n] = x.z;
+        a.vo[n] = v.x; a.vo[N + n] = v.y; a.vo[2 * N + n] = v.z;
         if constexpr (VVM == VV_PLAIN || VVM == VV_LAST) {
             a.fold[n] = f.x; a.fold[N + n] = f.y; a.fold[2 * N + n] = f.z;
         }
@@ -936,6 +1055,25 @@ __global__ __launch_bounds__(kBlock) void k_tip(const float* __restrict__ x, con
 
 __global__ void k_set_i64(int64_t* p, int64_t v) { *p = v; }
 
+// A node-fused step's indenter force: row blockIdx.x of rows (one FP64 value per tile) summed in a
+// fixed order (strided per-thread sums, lanes by a fixed butterfly, waves in order) into out.
+__global__ __launch_bounds__(kBlock) void k_tip_rows(const double* __restrict__ rows, int64_t ntiles,
+                                                     double* __restrict__ out) {
+    const double* r = rows + (int64_t)blockIdx.x * ntiles;
+    double t = 0.0;
+    for (int64_t i = threadIdx.x; i < ntiles; i += kBlock) t += r[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off);
+    __shared__ double ws[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) x += ws[w];
+        out[blockIdx.x] = x;
+    }
+}
+
 // ---- scratch of the allocation-free tip-force reduction (nfx_tip_force_noalloc) ----------------
 // The reduction's per-block FP64 (sum, max) partials, kMaxReduceBlocks pairs, then its completion
 // counter in the first word of the 32-double tail: 32.25 KiB.  Each GPU the library runs on holds
@@ -975,6 +1113,8 @@ static NodeArgs node_args(const nfx_plan& p) {
     a.x = p.disp;
     a.ref = p.ref;
     a.v = p.vel;
+    a.xo = p.disp;
+    a.vo = p.vel;
     a.force = p.force;
     a.fold = p.fold;
     a.off = p.csr_offsets;
@@ -1025,6 +1165,14 @@ static int validate_plan(const nfx_plan* p) {
     NFX_REQUIRE(p->scatter != NFX_SCATTER_CSR || (p->csr_offsets && p->csr_slots && p->corners),
                 "plan: CSR scatter needs offsets, slots and corners");
     NFX_REQUIRE(p->indenter >= 0 && p->indenter <= 2, "plan: unknown indenter %d", p->indenter);
+    const int nb = (p->disp_b != nullptr) + (p->vel_b != nullptr) + (p->partials_b != nullptr) +
+                   (p->tile_range != nullptr);
+    NFX_REQUIRE(nb == 0 || nb == 4,
+                "plan: the node-fused arrays disp_b, vel_b, partials_b and tile_range are set together");
+    NFX_REQUIRE(nb == 0 || (p->scatter == NFX_SCATTER_TILED && p->model >= NFX_SVKK && p->model <= NFX_VISCOPLASTIC),
+                "plan: node-fused steps need the tiled scatter and model NFX_SVKK, NFX_SVK, NFX_MR or NFX_VISCOPLASTIC");
+    NFX_REQUIRE(nb == 0 || (p->disp_b != p->disp && p->vel_b != p->vel && p->partials_b != p->partials),
+                "plan: the node-fused buffers must differ from disp, vel and partials");
     return NFX_OK;
 }
 
@@ -1037,12 +1185,12 @@ static void launch_tile_legacy(unsigned int nb, size_t lds, hipStream_t s, const
         k_tile_force<MODEL, K, GEO, false><<<nb, K, lds, s>>>(a, e, c, nullptr, nullptr, nullptr, nullptr);
 }
 
-template <int K, bool GEO, bool STRESS, bool NT>
+template <int K, bool GEO, bool STRESS, bool NT, bool NODE>
 static void launch_tile_vp(unsigned int nb, size_t lds, hipStream_t s, const TileArgs& a, const ElasticDev& e,
                            const VPConst& c, const nfx_plan& p) {
     float* vm = STRESS ? p.von_mises : nullptr;
     float* pr = STRESS ? p.pressure : nullptr;
-    if constexpr (!STRESS) {  // the batch's form: symmetric state with 16-bit tile-node metadata
+    if constexpr (!STRESS && !NODE) {  // the batch's form: symmetric state with 16-bit tile-node metadata
         if (p.vp_sym && a.tile_nodes16) {
             if (p.epsbar)
                 k_tile_force<NFX_VISCOPLASTIC, K, GEO, false, NT, true, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
@@ -1053,27 +1201,72 @@ static void launch_tile_vp(unsigned int nb, size_t lds, hipStream_t s, const Til
             return;
         }
     }
+    constexpr int VP = NFX_VISCOPLASTIC;
     if (p.vp_sym) {
         if (p.epsbar)
-            k_tile_force<NFX_VISCOPLASTIC, K, GEO, STRESS, NT, true, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar,
-                                                                                        vm, pr);
+            k_tile_force<VP, K, GEO, STRESS, NT, true, true, false, NODE><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
+                                                                                       p.epsbar, vm, pr);
         else
-            k_tile_force<NFX_VISCOPLASTIC, K, GEO, STRESS, NT, false, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr,
-                                                                                         vm, pr);
+            k_tile_force<VP, K, GEO, STRESS, NT, false, true, false, NODE><<<nb, K, lds, s>>>(a, e, c, p.vp_state,
+                                                                                        nullptr, vm, pr);
     } else if (p.epsbar) {
-        k_tile_force<NFX_VISCOPLASTIC, K, GEO, STRESS, NT, true><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar, vm, pr);
+        k_tile_force<VP, K, GEO, STRESS, NT, true, false, false, NODE><<<nb, K, lds, s>>>(a, e, c, p.vp_state, p.epsbar,
+                                                                                    vm, pr);
     } else {
-        k_tile_force<NFX_VISCOPLASTIC, K, GEO, STRESS, NT, false><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr, vm, pr);
+        k_tile_force<VP, K, GEO, STRESS, NT, false, false, false, NODE><<<nb, K, lds, s>>>(a, e, c, p.vp_state, nullptr,
+                                                                                     vm, pr);
     }
 }
 
-template <int K, bool GEO>
-static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
+// the buffers of one node-fused launch (launch k >= 1 of a call completes step k-1 at the nodes
+// and runs the force pass of step k)
+struct FusedIO {
+    float* x;       // displacements before the node step (read)
+    float* x_next;  // ... after it
+    float* w;       // w, or v on the call's first launch
+    float* w_next;
+    float* prev_partials;
+    float* partials;
+    double* tip_row;  // nullable
+    int32_t vv_first;
+    int64_t step;
+    const int64_t* step_dev;
+    int32_t step_off;
+};
+
+template <int K, bool GEO, bool NODE = false>
+static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st, const FusedIO* io = nullptr) {
     // ctx->batch is set only while a batched step is being enqueued
     TileArgs a{p.N,          p.T,         p.dminv,        p.v0,       p.disp,     p.ref,
                p.tile_count, p.tile_nodes, p.tile_lend,    p.tile_corners, p.tile_dst, p.partials,
                p.tile_max_nodes};
     if (p.model == NFX_VISCOPLASTIC) a.vp_active = p.vp_active;
+    if constexpr (NODE) {
+        a.x = io->x;
+        a.partials = io->partials;
+        NodeStep& n = a.ns;
+        n.prev_partials = io->prev_partials;
+        n.range = reinterpret_cast<const uint2*>(p.tile_range);
+        n.w = io->w;
+        n.fold = p.fold;
+        n.x_next = io->x_next;
+        n.w_next = io->w_next;
+        n.tip_row = io->tip_row;
+        n.substrate = p.substrate;
+        n.indenter = p.indenter;
+        n.fix = p.fix_top_bottom;
+        n.vv_first = io->vv_first;
+        n.sub_k = p.substrate_k;
+        n.wall = p.wall;
+        n.ind_k = p.ind_k;
+        n.ind_radius = p.ind_radius;
+        n.ind_start = p.ind_start;
+        n.ind_speed = p.ind_speed;
+        n.ind_move = p.ind_move_steps;
+        n.step = io->step;
+        n.step_dev = io->step_dev;
+        n.step_off = io->step_off;
+    }
     if (ctx->batch) {
         a.tile_ntet = ctx->batch->tile_ntet;
         a.tile_cell = ctx->batch->tile_cell;
@@ -1104,6 +1297,33 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
     float* vm = st ? p.von_mises : nullptr;
     float* pr = st ? p.pressure : nullptr;
     const bool nt = p.stream_nt != 0;
+    if constexpr (NODE) {  // the single-cell laws of a node-fused step (validate_plan admits no other)
+        constexpr bool N_ = true;
+        switch (p.model) {
+            case NFX_MR:
+                if (nt)
+                    k_tile_force<NFX_MR, K, GEO, false, true, true, false, false, N_><<<nb, K, lds, s>>>(a, e, c, nullptr,
+                                                                                             nullptr, nullptr, nullptr);
+                else
+                    k_tile_force<NFX_MR, K, GEO, false, false, true, false, false, N_><<<nb, K, lds, s>>>(a, e, c, nullptr,
+                                                                                              nullptr, nullptr, nullptr);
+                break;
+            case NFX_SVK:
+                k_tile_force<NFX_SVK, K, GEO, false, false, true, false, false, N_><<<nb, K, lds, s>>>(a, e, c, nullptr,
+                                                                                           nullptr, nullptr, nullptr);
+                break;
+            case NFX_SVKK:
+                k_tile_force<NFX_SVKK, K, GEO, false, false, true, false, false, N_><<<nb, K, lds, s>>>(a, e, c, nullptr,
+                                                                                            nullptr, nullptr, nullptr);
+                break;
+            default:
+                if (st && nt) launch_tile_vp<K, GEO, true, true, true>(nb, lds, s, a, e, c, p);
+                else if (st) launch_tile_vp<K, GEO, true, false, true>(nb, lds, s, a, e, c, p);
+                else if (nt) launch_tile_vp<K, GEO, false, true, true>(nb, lds, s, a, e, c, p);
+                else launch_tile_vp<K, GEO, false, false, true>(nb, lds, s, a, e, c, p);
+        }
+        return;
+    }
     switch (p.model) {
         case NFX_MR:
             if (a.tile_nodes16 && nt)
@@ -1123,10 +1343,10 @@ static void launch_tiled_k(nfx_ctx* ctx, const nfx_plan& p, bool st) {
         case NFX_NEO_HOOKEAN_2: launch_tile_legacy<NFX_NEO_HOOKEAN_2, K, GEO>(nb, lds, s, a, e, c, vm, pr); break;
         case NFX_PORO_NEO_HOOKEAN: launch_tile_legacy<NFX_PORO_NEO_HOOKEAN, K, GEO>(nb, lds, s, a, e, c, vm, pr); break;
         default:
-            if (st && nt) launch_tile_vp<K, GEO, true, true>(nb, lds, s, a, e, c, p);
-            else if (st) launch_tile_vp<K, GEO, true, false>(nb, lds, s, a, e, c, p);
-            else if (nt) launch_tile_vp<K, GEO, false, true>(nb, lds, s, a, e, c, p);
-            else launch_tile_vp<K, GEO, false, false>(nb, lds, s, a, e, c, p);
+            if (st && nt) launch_tile_vp<K, GEO, true, true, false>(nb, lds, s, a, e, c, p);
+            else if (st) launch_tile_vp<K, GEO, true, false, false>(nb, lds, s, a, e, c, p);
+            else if (nt) launch_tile_vp<K, GEO, false, true, false>(nb, lds, s, a, e, c, p);
+            else launch_tile_vp<K, GEO, false, false, false>(nb, lds, s, a, e, c, p);
     }
 }
 
@@ -1189,6 +1409,43 @@ static int launch_force_pass(nfx_ctx* ctx, const nfx_plan& p, bool stress) {
     return NFX_OK;
 }
 
+static bool node_fused(const nfx_plan& p) { return p.disp_b != nullptr; }  // validate_plan checked the rest
+
+// launch k (k >= 1) of a node-fused call starting at step0: it completes step step0 + k - 1 at the
+// nodes and runs the force pass of step step0 + k.  Launch k reads the buffers of parity (k - 1) & 1 and
+// writes those of parity k & 1, where parity 0 is the plan's (disp, vel, partials) and 1 the second
+// buffers; the call's first force pass (k = 0) writes the plan's partials.
+static FusedIO fused_io(const nfx_plan& p, int64_t k, double* tip_row, int64_t step, const int64_t* step_dev,
+                        int32_t step_off) {
+    float* X[2] = {p.disp, p.disp_b};
+    float* W[2] = {p.vel, p.vel_b};
+    float* P[2] = {p.partials, p.partials_b};
+    const int r = (int)((k - 1) & 1), w = (int)(k & 1);
+    FusedIO io;
+    io.x = X[r];
+    io.x_next = X[w];
+    io.w = W[r];
+    io.w_next = W[w];
+    io.prev_partials = P[r];
+    io.partials = P[w];
+    io.tip_row = tip_row;
+    io.vv_first = k == 1;
+    io.step = step;
+    io.step_dev = step_dev;
+    io.step_off = step_off;
+    return io;
+}
+
+static int launch_fused(nfx_ctx* ctx, const nfx_plan& p, bool stress, const FusedIO& io) {
+    const bool st = stress && p.von_mises && p.pressure;
+    const bool geo = p.shape_from_ref != 0;
+    if (p.tile_k == 128) geo ? launch_tiled_k<128, true, true>(ctx, p, st, &io) : launch_tiled_k<128, false, true>(ctx, p, st, &io);
+    else if (p.tile_k == 256) geo ? launch_tiled_k<256, true, true>(ctx, p, st, &io) : launch_tiled_k<256, false, true>(ctx, p, st, &io);
+    else geo ? launch_tiled_k<512, true, true>(ctx, p, st, &io) : launch_tiled_k<512, false, true>(ctx, p, st, &io);
+    NFX_CHECK_LAUNCH();
+    return NFX_OK;
+}
+
 template <int CSR, bool SUB, int IND, bool FIX, int VVM>
 static void launch_node_m(nfx_ctx* ctx, const NodeArgs& a, int64_t step, const int64_t* step_dev, int32_t off,
                           double* tip_sum, double* tip_max) {
@@ -1230,10 +1487,33 @@ static void launch_node_ind(nfx_ctx* ctx, const nfx_plan& p, const NodeArgs& a, 
     }
 }
 
+// the buffers of a node pass that reads and stores different ones (a node-fused call's last step)
+struct NodeIO {
+    float* x;
+    float* xo;
+    float* v;
+    float* vo;
+    const float* partials;
+};
+
+// the node pass ending a node-fused call of n steps: step n-1 from the buffers of parity (n - 1) & 1,
+// stored into the plan's disp and vel (and fold)
+static NodeIO fused_last_io(const nfx_plan& p, int64_t n) {
+    const int r = (int)((n - 1) & 1);
+    return NodeIO{r ? p.disp_b : p.disp, p.disp, r ? p.vel_b : p.vel, p.vel, r ? p.partials_b : p.partials};
+}
+
 // vvm: VV_PLAIN for a one-step call; a call of n >= 2 steps runs VV_FIRST, VV_MIDDLE x (n - 2), VV_LAST
 static int launch_node_pass(nfx_ctx* ctx, const nfx_plan& p, int64_t step, const int64_t* step_dev, int32_t off,
-                            double* tip_sum, double* tip_max, int vvm = VV_PLAIN) {
-    const NodeArgs a = node_args(p);
+                            double* tip_sum, double* tip_max, int vvm = VV_PLAIN, const NodeIO* io = nullptr) {
+    NodeArgs a = node_args(p);
+    if (io) {
+        a.x = io->x;
+        a.xo = io->xo;
+        a.v = io->v;
+        a.vo = io->vo;
+        a.partials = io->partials;
+    }
     const bool sub = p.substrate != 0;
     switch (p.scatter * 2 + (sub ? 1 : 0)) {
         case 0: launch_node_ind<0, false>(ctx, p, a, step, step_dev, off, tip_sum, tip_max, vvm); break;
@@ -1390,11 +1670,25 @@ static int capture_graph(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch* b, bo
     for (int64_t i = 0; i < kGraphBlock && rc == NFX_OK; ++i) {
         if (b) {
             rc = launch_batch_step(ctx, p, *b, 0, ctx->step_dev, (int32_t)i);
+        } else if (node_fused(p)) {
+            // launch i of a block is launch k0 + i of its call with k0 even (run_fused starts blocks at
+            // k = 2, 102, ...), so its buffers are those of launch 2 + i
+            const int64_t ntiles = (p.T + p.tile_k - 1) / p.tile_k;
+            rc = launch_fused(ctx, p, false,
+                              fused_io(p, 2 + i, hist ? ctx->tip_rows + i * ntiles : nullptr, 0, ctx->step_dev, (int32_t)i));
         } else {
             rc = launch_force_pass(ctx, p, false);
             if (rc == NFX_OK)
                 rc = launch_node_pass(ctx, p, 0, ctx->step_dev, (int32_t)i, hist ? ctx->hist_dev + i : nullptr,
                                       nullptr, VV_MIDDLE);
+        }
+    }
+    if (rc == NFX_OK && !b && hist && node_fused(p)) {
+        k_tip_rows<<<kGraphBlock, kBlock, 0, ctx->stream>>>(ctx->tip_rows, (p.T + p.tile_k - 1) / p.tile_k, ctx->hist_dev);
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess) {
+            set_error("kernel launch failed: %s", hipGetErrorString(le));
+            rc = NFX_ERR_HIP;
         }
     }
     hipGraph_t g = nullptr;
@@ -1424,6 +1718,76 @@ static int capture_graph(nfx_ctx* ctx, const nfx_plan& p, const nfx_batch* b, bo
     ctx->captures++;
     *out = &ctx->graphs.back();
     return NFX_OK;
+}
+
+// the context's per-tile indenter-force rows for kGraphBlock steps of a plan with ntiles tiles;
+// growing them drops the captured graphs (they hold the old address)
+static int ensure_tip_rows(nfx_ctx* ctx, int64_t ntiles) {
+    const int64_t need = ntiles * kGraphBlock;
+    if (ctx->tip_rows_cap >= need) return NFX_OK;
+    NFX_HIP(hipStreamSynchronize(ctx->stream));
+    drop_graphs(ctx);
+    if (ctx->tip_rows) NFX_HIP(hipFree(ctx->tip_rows));
+    ctx->tip_rows = nullptr;
+    ctx->tip_rows_cap = 0;
+    NFX_HIP(hipMalloc(&ctx->tip_rows, sizeof(double) * (size_t)need));
+    ctx->tip_rows_cap = need;
+    return NFX_OK;
+}
+
+// A node-fused call of n >= 2 steps: the force pass of step 0, launches k = 1 .. n-1 (each completes
+// step k-1 at the nodes and runs the force pass of step k; launch 1 turns (v, f_old) into w), then
+// the node pass of step n-1, which turns w back into (v, f_old) in the plan's buffers.  Launches
+// 2 .. n-2 replay captured blocks of kGraphBlock where whole blocks fit (every block starts at an even
+// k, so one captured graph serves them all); the last launch stays eager so that it can write the
+// stress outputs.  With a tip history each launch writes one value per tile, reduced in a fixed
+// order per step (k_tip_rows).
+static int run_fused(nfx_ctx* c, const nfx_plan& p, int64_t step0, int64_t n, double* tip_hist, bool use_graph,
+                     bool stress) {
+    const int64_t ntiles = (p.T + p.tile_k - 1) / p.tile_k;
+    int rc = NFX_OK;
+    if (tip_hist) {
+        rc = ensure_tip_rows(c, ntiles);
+        if (rc) return rc;
+    }
+    rc = launch_force_pass(c, p, false);
+    if (rc) return rc;
+    auto eager = [&](int64_t k) -> int {
+        int r = launch_fused(c, p, stress && k == n - 1,
+                             fused_io(p, k, tip_hist ? c->tip_rows : nullptr, step0 + k - 1, nullptr, 0));
+        if (r == NFX_OK && tip_hist) {
+            k_tip_rows<<<1, kBlock, 0, c->stream>>>(c->tip_rows, ntiles, tip_hist + k - 1);
+            NFX_CHECK_LAUNCH();
+        }
+        return r;
+    };
+    rc = eager(1);
+    if (rc) return rc;
+    int64_t k = 2;
+    if (use_graph && n - 1 - k >= kGraphBlock) {
+        const bool hist = tip_hist != nullptr;
+        nfx_ctx::GraphEntry* ge = find_graph(c, p, nullptr, hist);
+        if (!ge) {
+            rc = capture_graph(c, p, nullptr, hist, &ge);
+            if (rc) return rc;
+        }
+        while (n - 1 - k >= kGraphBlock) {
+            k_set_i64<<<1, 1, 0, c->stream>>>(c->step_dev, step0 + k - 1);
+            NFX_CHECK_LAUNCH();
+            NFX_HIP(hipGraphLaunch(ge->exec, c->stream));
+            if (hist)
+                NFX_HIP(hipMemcpyAsync(tip_hist + k - 1, c->hist_dev, sizeof(double) * kGraphBlock,
+                                       hipMemcpyDeviceToDevice, c->stream));
+            k += kGraphBlock;
+        }
+    }
+    for (; k < n; ++k) {
+        rc = eager(k);
+        if (rc) return rc;
+    }
+    const NodeIO io = fused_last_io(p, n);
+    return launch_node_pass(c, p, step0 + n - 1, nullptr, 0, tip_hist ? tip_hist + n - 1 : nullptr, nullptr, VV_LAST,
+                            &io);
 }
 
 }  // namespace nfx
@@ -1501,6 +1865,7 @@ int nfx_destroy(nfx_ctx* c) {
     (void)hipFree(c->counter);
     (void)hipFree(c->step_dev);
     (void)hipFree(c->hist_dev);
+    if (c->tip_rows) (void)hipFree(c->tip_rows);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->own_stream) (void)hipStreamDestroy(c->stream);
@@ -1891,6 +2256,7 @@ int nfx_run_steps(nfx_ctx* c, const nfx_plan* p, int64_t step0, int64_t nsteps, 
         if (rc) return rc;
         return launch_node_pass(c, *p, step0, nullptr, 0, tip_hist, nullptr, VV_PLAIN);
     }
+    if (node_fused(*p)) return run_fused(c, *p, step0, nsteps, tip_hist, use_graph != 0, stress);
     // n >= 2: the first step turns (v, f_old) into w, the middle steps advance (u, w), the last turns
     // w back into (v, f_old); the first and last steps run eagerly, the middle ones in whole blocks of
     // kGraphBlock steps that replay one captured graph (kept for as long as the plan and the history
@@ -1936,6 +2302,37 @@ int nfx_time_passes(nfx_ctx* c, const nfx_plan* p, int64_t step0, int32_t nrep, 
     if (rc) return rc;
     std::vector<hipEvent_t> ev(3 * (size_t)nrep);
     for (auto& e : ev) NFX_HIP(hipEventCreate(&e));
+    if (node_fused(*p) && nrep >= 3) {
+        // the force pass of step 0 (ev 0-1), node-fused launches 1 .. nrep-1 (ev[i] - ev[i+1]), the
+        // last node pass (ev nrep - ev nrep+1)
+        NFX_HIP(hipEventRecord(ev[0], c->stream));
+        rc = launch_force_pass(c, *p, false);
+        for (int k = 1; k < nrep && rc == NFX_OK; ++k) {
+            NFX_HIP(hipEventRecord(ev[k], c->stream));
+            rc = launch_fused(c, *p, false, fused_io(*p, k, nullptr, step0 + k - 1, nullptr, 0));
+        }
+        if (rc == NFX_OK) {
+            NFX_HIP(hipEventRecord(ev[nrep], c->stream));
+            const NodeIO io = fused_last_io(*p, nrep);
+            rc = launch_node_pass(c, *p, step0 + nrep - 1, nullptr, 0, nullptr, nullptr, VV_LAST, &io);
+            NFX_HIP(hipEventRecord(ev[nrep + 1], c->stream));
+        }
+        if (rc == NFX_OK) {
+            NFX_HIP(hipEventSynchronize(ev[nrep + 1]));
+            double sf = 0;
+            for (int k = 2; k < nrep; ++k) {
+                float a = 0;
+                NFX_HIP(hipEventElapsedTime(&a, ev[k], ev[k + 1]));
+                sf += a;
+            }
+            float b = 0;
+            NFX_HIP(hipEventElapsedTime(&b, ev[nrep], ev[nrep + 1]));
+            *ms_force = sf / (nrep - 2);
+            *ms_node = b;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        return rc;
+    }
     for (int i = 0; i < nrep && rc == NFX_OK; ++i) {
         NFX_HIP(hipEventRecord(ev[3 * i], c->stream));
         rc = launch_force_pass(c, *p, false);

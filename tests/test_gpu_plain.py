@@ -321,8 +321,24 @@ LEGACY_SETTERS = {"LE": "setInteractionLinearElastic", "LD": "setInteractionLine
                   "NH2": "setInteractionSecondOrderNeoHookean", "PNH": "setInteractionPoroNeoHookean"}
 
 
+def with_poles(P, T):
+    """The mesh relabelled so that nodes 3 and 4 are its top and bottom poles on the y axis (moved
+    onto x = z = 0 from a few hundredths of the lattice spacing), the nodes fixTopBottom pins."""
+    r = np.hypot(P[:, 0], P[:, 2])
+    near = np.flatnonzero((r < 0.2) & (np.arange(len(P)) >= 2))
+    top, bottom = near[np.argmax(P[near, 1])], near[np.argmin(P[near, 1])]
+    assert len({top, bottom, 3, 4}) == 4
+    m = np.arange(len(P))
+    m[[3, top]] = m[[top, 3]]
+    m[[4, bottom]] = m[[bottom, 4]]
+    P = P[m].copy()
+    P[3:5, [0, 2]] = 0.0
+    inv = np.argsort(m)
+    return P, inv[T]
+
+
 def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, mesh=(4.0, 0.8, 1), material="NH",
-              indenter="sphere", steps=400, compact=False, yield_si=5.0):
+              indenter="sphere", steps=400, compact=False, yield_si=5.0, node_fused=True, fix=False):
     from nofluidx3d_amd import configs
     from nofluidx3d_amd.Simulation import Simulation
     from nofluidx3d_amd.util import vtk
@@ -338,10 +354,12 @@ def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, 
         cfg["CELL"].update({"MooneyRivlinRatio": 0.75, "model": "MOONEY_RIVLIN", "ViscositySI": 10.0,
                             "YieldStressSI": yield_si})
     P, T = configs.make_mesh(cfg)
+    if fix:
+        P, T = with_poles(P, T)
     tile_k = {"tiled256": 256, "tiled128": 128}.get(scatter, 512)
     scatter = "tiled" if scatter.startswith("tiled") else scatter
     sim = Simulation(cfg, scatter=scatter, fused=fused, use_graph=use_graph, mesh=vtk.from_tetra(P, T),
-                     verbose=False, tile_k=tile_k)
+                     verbose=False, tile_k=tile_k, fuse_node_pass=node_fused)
     sim.setInteractionSubstrate()
     if indenter == "sphere":
         sim.setInteractionSphere()
@@ -354,7 +372,7 @@ def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, 
         getattr(sim, LEGACY_SETTERS[material])()
     else:
         sim.setInteractionTetra()
-    sim.setInteractionVelocityVerlet()
+    sim.setInteractionVelocityVerlet(fixTopBottom=fix)
     return sim
 
 
@@ -386,6 +404,8 @@ def oracle_for(sim, nthreads=1):
         elif hasattr(inter, "wallFunc"):
             prm.indenter, prm.ind_k, prm.ind_radius = 2, inter.forceConst, 0.0
             prm.ind_start, prm.ind_speed, prm.ind_move_steps = inter.wallFunc.law
+        elif hasattr(inter, "fixTopBottom"):
+            prm.fix_top_bottom = int(inter.fixTopBottom)
     return O.OracleState(sim.cell.points, sim.cell.tetras, prm)
 
 
@@ -419,14 +439,17 @@ def test_fused_run_matches_oracle(ctx, scatter, indenter):
     assert_parity(sim, gx, gf, ref.positions(), of)
 
 
+@pytest.mark.parametrize("node_fused", [True, False], ids=["one_launch", "two_launches"])
 @pytest.mark.parametrize("material", ["NH", "VP"])
-def test_fused_calls_of_every_length_match_oracle(ctx, material):
+def test_fused_calls_of_every_length_match_oracle(ctx, material, node_fused):
     """A multi-step call carries (u, w) with w = a*v + f_old/2 between its first and last step; a
     one-step call keeps (v, f_old). Calls of 1, 2, 3 and more steps (eager and graph-replayed), in
-    any order, hand (v, f_old) to the next call, as the reference's per-step loop does."""
+    any order, hand (v, f_old) to the next call, as the reference's per-step loop does; with the
+    node-fused step each call also begins and ends in the plan's own buffers."""
     import torch
 
-    sim = build_sim(material=material, scatter="tiled")
+    sim = build_sim(material=material, scatter="tiled", node_fused=node_fused)
+    assert bool(sim.plan().disp_b) == node_fused
     vv = sim.interactions[-1]
     chunks = [1, 2, 1, 3, 7, 1, 150, 2, 1, 32]
     hist = torch.zeros(sum(chunks), dtype=torch.float64, device="cuda:0")
@@ -443,6 +466,30 @@ def test_fused_calls_of_every_length_match_oracle(ctx, material):
         assert np.max(np.abs(gv - ref.vel)) / vs < 1e-3, (n, t)
         assert np.max(np.abs(gfo - ref.fold)) / fs < 1e-3, (n, t)
     assert_parity(sim, sim.positions(), host(hist), ref.positions(), np.concatenate(of))
+
+
+@pytest.mark.parametrize("fix", [False, True], ids=["free", "fix_top_bottom"])
+@pytest.mark.parametrize("material", ["NH", "VP"])
+def test_node_fused_step_matches_two_launch_step(ctx, material, fix):
+    """The node-fused step (each node update carried by the next force pass; every tile that
+    touches a node computes it, its first tile stores it) against the same step as two launches:
+    the same recurrence in the same operation order, equal to within FP32 rounding of the compiled
+    forms, and both at the oracle's 1e-4."""
+    runs = {}
+    for nf in (True, False):
+        sim = build_sim(material=material, scatter="tiled256", node_fused=nf, fix=fix)
+        assert bool(sim.plan().disp_b) == nf
+        runs[nf] = (sim,) + run_gpu(sim, 600)
+    sim, gx, gf = runs[True]
+    _, ux, uf = runs[False]
+    R = sim.parameters["CELL"]["RadiusSIM"]
+    assert np.max(np.abs(gx - ux)) / R < 1e-6
+    assert np.max(np.abs(gf - uf)) / np.max(np.abs(uf)) < 1e-6
+    if fix:
+        assert np.all(gx[3:5, [0, 2]] == 0.0)  # x = z = 0 at nodes 3 and 4 (VelocityVerlet.cl:17-22)
+    ref = oracle_for(sim)
+    of = ref.run(600)
+    assert_parity(sim, gx, gf, ref.positions(), of)
 
 
 @pytest.mark.parametrize("scatter", ["csr", "tiled"])
