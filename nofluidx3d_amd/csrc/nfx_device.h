@@ -625,16 +625,19 @@ __device__ __forceinline__ VPOut vp_tetra(const M9& H, const float* D, const flo
 // s = det Dm (= 6 V0).  A tetra that does not flow keeps Q bit for bit; a wave whose tetra all do not
 // flow (the far field under a yield offset) skips the update.
 // D with scale gs gives the edge cofactors (D = Dm^-1, gs = s; or D = G, gs = 1); s = det Dm.
+// zero_state (uniform over the caller's workgroup): Q is known to be zero, so Fe = F and He = H
 __device__ __forceinline__ VPOut vp_tetra_q(const M9& H, const float* D, float gs, float s, const float* Q,
-                                            float epsbar, const VPConst& P) {
+                                            float epsbar, const VPConst& P, bool zero_state = false) {
     VPOut o;
-    M9 He;
+    M9 He = H;
+    if (!zero_state) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-            He.m[i][j] = H.m[i][j] + Q[3 * i + j] +
-                         (H.m[i][0] * Q[0 * 3 + j] + H.m[i][1] * Q[1 * 3 + j] + H.m[i][2] * Q[2 * 3 + j]);
+            for (int j = 0; j < 3; ++j)
+                He.m[i][j] = H.m[i][j] + Q[3 * i + j] +
+                             (H.m[i][0] * Q[0 * 3 + j] + H.m[i][1] * Q[1 * 3 + j] + H.m[i][2] * Q[2 * 3 + j]);
+    }
     S6 sig;
     const S6 Dp = vp_stress_flow(cauchy_green_m1(He), epsbar, P, o, sig);
     const M9 C = cof_ipH(H);

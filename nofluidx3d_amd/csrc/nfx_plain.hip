@@ -743,7 +743,7 @@ __global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (!SYM || EPS || 
         if constexpr (MODEL == NFX_VISCOPLASTIC) {
             VPOut o;
             if constexpr (SYM) o = vp_tetra_gd(H, D, gs, sdm, Sv, e0, P);
-            else o = vp_tetra_q(H, D, gs, sdm, Sv, e0, P);
+            else o = vp_tetra_q(H, D, gs, sdm, Sv, e0, P, !plastic);
             if (o.flows) {
                 flows = 1;
 #pragma unroll
