@@ -319,6 +319,9 @@ def test_graph_is_captured_once_per_plan(ctx):
 
 LEGACY_SETTERS = {"LE": "setInteractionLinearElastic", "LD": "setInteractionLinearPoroElastic",
                   "NH2": "setInteractionSecondOrderNeoHookean", "PNH": "setInteractionPoroNeoHookean"}
+# the Tetra plugin's other laws (config CELL keys)
+ELASTIC_LAWS = {"SVK": {"model": "SAINT_VENANT_KIRCHHOFF"}, "SVKK": {"model": "SAINT_VENANT_KIRCHHOFF_KELLNBERGER"},
+                "MR": {"model": "MOONEY_RIVLIN", "MooneyRivlinRatio": 0.75}}
 
 
 def with_poles(P, T):
@@ -353,6 +356,7 @@ def build_sim(cfg_overrides=None, scatter="atomic", fused=True, use_graph=True, 
     if material == "VP":
         cfg["CELL"].update({"MooneyRivlinRatio": 0.75, "model": "MOONEY_RIVLIN", "ViscositySI": 10.0,
                             "YieldStressSI": yield_si})
+    cfg["CELL"].update(ELASTIC_LAWS.get(material, {}))
     P, T = configs.make_mesh(cfg)
     if fix:
         P, T = with_poles(P, T)
@@ -589,6 +593,22 @@ def test_legacy_law_fused_run_matches_oracle(ctx, material, scatter):
     gx, gf = run_gpu(sim, 200)
     ref = oracle_for(sim)
     of = ref.run(200)
+    assert_parity(sim, gx, gf, ref.positions(), of)
+
+
+@pytest.mark.parametrize("node_fused", [True, False], ids=["one_launch", "two_launches"])
+@pytest.mark.parametrize("material", ["SVK", "SVKK", "MR"])
+def test_elastic_laws_tiled_run_matches_oracle(ctx, material, node_fused):
+    """The Tetra plugin's SVK, SVKK and MR (ratio 0.75) laws through the tiled step in both step
+    forms, 300 steps with the tip starting 0.2 above the cell (the SVK laws soften in compression and
+    are run short of the deep indentation)."""
+    sim = build_sim(material=material, scatter="tiled256", node_fused=node_fused,
+                    cfg_overrides={"InitialDistance": 0.2, "SubstrateDistance": 0.3})
+    assert sim.plan().model == {"SVK": O.SVK, "SVKK": O.SVKK, "MR": O.MR}[material]
+    assert bool(sim.plan().disp_b) == node_fused
+    gx, gf = run_gpu(sim, 300)
+    ref = oracle_for(sim)
+    of = ref.run(300)
     assert_parity(sim, gx, gf, ref.positions(), of)
 
 
