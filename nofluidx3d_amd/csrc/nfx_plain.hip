@@ -929,7 +929,7 @@ __global__ __launch_bounds__(kBlock) void k_node_update(NodeArgs a, int64_t step
         if constexpr (GATHER == 1) {
             f = gather_node<G>(a, n, sub);
         } else if constexpr (GATHER == 2) {
-            f = sum_partials(a.partials, a.node_off[n], a.node_off[n + 1]);
+            f = sum_partials8(a.partials, a.node_off[n], a.node_off[n + 1]);
         } else {
             f = mk(a.force[n], a.force[N + n], a.force[2 * N + n]);
         }
@@ -1549,7 +1549,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_node(BatchNodeArgs a, int64_t 
     const int64_t N = a.N;
     const int64_t n = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (n >= N) return;
-    V3 f = sum_partials(a.partials, a.node_off[n], a.node_off[n + 1]);
+    V3 f = sum_partials8(a.partials, a.node_off[n], a.node_off[n + 1]);
     const nfx_cell* cp = a.cells + a.node_cell[n];
     V3 x = mk(a.x[n], a.x[N + n], a.x[2 * N + n]);
     const double pos = IND ? indenter_pos(cp->tip_start, cp->tip_speed, cp->tip_move_steps, time) : 0.0;

@@ -635,13 +635,15 @@ def test_fused_run_writes_stress_of_the_last_step(ctx):
     assert np.max(np.abs(gpr - pr)) < 1e-3 * np.max(np.abs(pr))
 
 
-@pytest.mark.parametrize("scatter", ["csr", "tiled"])
+@pytest.mark.parametrize("scatter", ["csr", "tiled", "tiled_two_launches"])
 def test_deterministic_modes_are_bitwise_reproducible_and_graph_equals_eager(ctx, scatter):
-    a = build_sim(scatter=scatter, use_graph=True)
+    nf = scatter == "tiled"  # the tiled step in the node-fused form, or as two launches per step
+    scatter = "tiled" if scatter.startswith("tiled") else scatter
+    a = build_sim(scatter=scatter, use_graph=True, node_fused=nf)
     xa, fa = run_gpu(a, 250)
-    b = build_sim(scatter=scatter, use_graph=True)
+    b = build_sim(scatter=scatter, use_graph=True, node_fused=nf)
     xb, fb = run_gpu(b, 250)
-    c = build_sim(scatter=scatter, use_graph=False)
+    c = build_sim(scatter=scatter, use_graph=False, node_fused=nf)
     xc, fc = run_gpu(c, 250)
     assert np.array_equal(xa, xb) and np.array_equal(fa, fb)
     assert np.array_equal(xa, xc) and np.array_equal(fa, fc)
