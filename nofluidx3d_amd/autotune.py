@@ -44,7 +44,7 @@ def default_stream_nt(tetra):
 
 
 # the node-fused step (one launch per step) wins where launches dominate a step; on a mesh of
-# millions of tetra its force pass, which then also updates every node it touches (~1.7 times per
+# millions of tetra its force pass, which then also updates every node it touches (~2.9 times per
 # node), is longer than the two passes apart (measured on MI355X: 25k tetra 5.6 against 7.2 us per
 # step, 3.4M tetra 58 against 50 us)
 NODE_FUSED_MAX_TETRA = SMALL_MESH_TETRA
