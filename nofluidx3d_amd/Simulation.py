@@ -53,6 +53,19 @@ def _law(fn, start, speed, move_steps):
     return fn
 
 
+def tile_ranges(tile_nodes, count, node_off, max_nodes):
+    """Per tile-node (ntiles x max_nodes, padded), the range [node_off[n], node_off[n+1]) of its node
+    n's partials, two u32 each (nfx_plan.tile_range: the node-fused step reads it with the tile
+    metadata); zero for the padding past a tile's count."""
+    ntiles = len(count)
+    tn = np.asarray(tile_nodes).reshape(ntiles, max_nodes).astype(np.int64)
+    valid = np.arange(max_nodes)[None, :] < np.asarray(count)[:, None]
+    out = np.zeros((ntiles, max_nodes, 2), np.uint32)
+    out[..., 0] = np.where(valid, node_off[tn], 0)
+    out[..., 1] = np.where(valid, node_off[tn + 1], 0)
+    return out.reshape(-1)
+
+
 class Simulation:
     def __init__(self, con, scatter="tiled", fused=True, use_graph=True, mesh=None, verbose=True, tile_k=512,
                  stream_nt=None, shape_from_ref=True, fuse_node_pass=None):
@@ -493,13 +506,7 @@ class Simulation:
                       node_off.ctypes.data_as(vp))
             dev = self.device
             to = lambda a: torch.from_numpy(a.view(np.int32) if a.dtype == np.uint32 else a.view(np.int16)).to(dev)
-            # per tile-node, the range [node_off[n], node_off[n+1]) of its node's partials (the node-fused
-            # step reads them with the tile metadata); zero for the padding past a tile's count
-            tn = tile_nodes.reshape(ntiles, MX).astype(np.int64)
-            valid = np.arange(MX)[None, :] < count[:, None]
-            tile_range = np.zeros((ntiles, MX, 2), np.uint32)
-            tile_range[..., 0] = np.where(valid, node_off[tn], 0)
-            tile_range[..., 1] = np.where(valid, node_off[tn + 1], 0)
+            tile_range = tile_ranges(tile_nodes, count, node_off, MX)
             self._tiles = {
                 "tile_k": K,
                 "max_nodes": MX,
@@ -510,7 +517,7 @@ class Simulation:
                 "tile_corners": to(tile_corners),
                 "tile_dst": to(tile_dst),
                 "node_off": to(node_off),
-                "tile_range": to(tile_range.reshape(-1)),
+                "tile_range": to(tile_range),
                 "partials": torch.zeros(3 * M, dtype=torch.float32, device=dev),
                 # every node is a corner of some tetra (has a partial): the node-fused step updates
                 # nodes only through the tiles that touch them

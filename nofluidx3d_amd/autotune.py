@@ -45,9 +45,10 @@ def default_stream_nt(tetra):
 
 # the node-fused step (one launch per step) wins where launches dominate a step; on a mesh of
 # millions of tetra its force pass, which then also updates every node it touches (~2.9 times per
-# node), is longer than the two passes apart (measured on MI355X: 25k tetra 5.6 against 7.2 us per
-# step, 3.4M tetra 58 against 50 us)
-NODE_FUSED_MAX_TETRA = SMALL_MESH_TETRA
+# node), is longer than the two passes apart.  Measured on MI355X (NH cells, best tile shape of each
+# form, us per graph-replayed step, fused against two launches): 25k tetra 5.5 / 7.2, 106k 6.3 / 7.9,
+# 211k 7.5 / 8.8, 526k 11.7 / 12.5, 878k 16.6 / 16.5; the 3.4M-tetra viscoplastic cell 58 / 50
+NODE_FUSED_MAX_TETRA = 600_000
 
 
 def default_node_fused(tetra):

@@ -335,6 +335,29 @@ def test_tile_builder_invariants(K):
     # within a node, partial positions follow tile order
     for n in range(N):
         assert np.all(np.diff(owner[no[n]:no[n + 1]]) > 0)
+    # the node-fused step's per-tile-node partial ranges, and its storing tile: for every node exactly
+    # one tile-node, the node's first tile, has its partial at the start of the range
+    from nofluidx3d_amd.Simulation import tile_ranges
+
+    rg = tile_ranges(tn, count, no, MX).reshape(ntiles, MX, 2)
+    stores = np.zeros(N, np.int64)
+    for c in range(ntiles):
+        for i in range(MX):
+            if i >= count[c]:
+                assert rg[c, i, 0] == 0 and rg[c, i, 1] == 0
+                continue
+            n = tn[c * MX + i]
+            assert rg[c, i, 0] == no[n] and rg[c, i, 1] == no[n + 1]
+            stores[n] += td[c * MX + i] == rg[c, i, 0]
+    assert np.all(stores == 1) and np.all(np.diff(no) > 0)  # every node has a partial (node-fused form)
+
+
+def test_step_form_default_by_mesh_size():
+    from nofluidx3d_amd import autotune
+
+    assert autotune.default_node_fused(25_342) and autotune.default_node_fused(autotune.NODE_FUSED_MAX_TETRA)
+    assert not autotune.default_node_fused(autotune.NODE_FUSED_MAX_TETRA + 1)
+    assert not autotune.default_node_fused(3_377_961)
 
 
 def test_cell_table_layout_matches_header():
