@@ -569,10 +569,10 @@ __host__ __device__ constexpr size_t tile_lds_bytes_per_node(bool geo) { return 
 // metadata (batches); NODE: the previous step's node update rides in this pass.  Waves per SIMD: the
 // viscoplastic forms fit 72 VGPRs (7 waves; measured 5 % faster than 6, while the 64-register bound
 // of 8 waves spills and is slower) except with stress outputs or NODE (6), and the symmetric one
-// without epsbar fits 64 (8 waves; 7 with GEO, whose 64-register bound spilled)
+// without epsbar fits 64 (8 waves)
 template <int MODEL, int K, bool GEO, bool STRESS, bool NT = false, bool EPS = true, bool SYM = false,
           bool NARROW = false, bool NODE = false>
-__global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (STRESS || NODE) ? 6 : (!SYM || EPS || GEO) ? 7 : 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
+__global__ __launch_bounds__(K, MODEL != NFX_VISCOPLASTIC ? 8 : (STRESS || NODE) ? 6 : (!SYM || EPS) ? 7 : 8) void k_tile_force(TileArgs a, ElasticDev e, VPConst P, float* __restrict__ Sst,
                                                   float* __restrict__ eps, float* __restrict__ vm,
                                                   float* __restrict__ pr) {
     // one thread per tetra of the tile (blockDim.x == K)
