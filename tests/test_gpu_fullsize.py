@@ -57,7 +57,7 @@ def large_mesh(cfg):
     return P, T
 
 
-def large_cell_sim(scatter="tiled", speed=1.0, compact=False):
+def large_cell_sim(scatter="tiled", speed=1.0, compact=False, node_fused=None):
     from nofluidx3d_amd import configs
     from nofluidx3d_amd.Simulation import Simulation
     from nofluidx3d_amd.util import vtk
@@ -66,7 +66,8 @@ def large_cell_sim(scatter="tiled", speed=1.0, compact=False):
     cfg["VelocitySI"] *= speed
     cfg["MoveTimeSI"] /= speed
     P, T = large_mesh(cfg)
-    sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False, tile_k=512)
+    sim = Simulation(cfg, scatter=scatter, mesh=vtk.from_tetra(P, T), verbose=False, tile_k=512,
+                     fuse_node_pass=node_fused)
     sim.setInteractionSubstrate()
     sim.setInteractionSphere()
     sim.setInteractionFiniteStrainViscoplastic(compactState=compact)
@@ -128,6 +129,18 @@ def test_large_cell_compact_state_matches_oracle(large_oracle):
     assert sim.plan().vp_sym == 1
     pe, fe = errors(gx, gf, ref["x"], ref["f"], ref["R"])
     print(f"compact state, {WINDOW} steps: pos_err {pe:.2e} of R, force_err {fe:.2e}")
+    assert pe < 1e-4 and fe < 1e-4
+
+
+def test_large_cell_node_fused_step_matches_oracle(large_oracle):
+    """The node-fused step (one launch per step; not the size default here) over the developed-contact
+    window at full size."""
+    ref = large_oracle
+    sim = large_cell_sim(speed=FAST, node_fused=True)
+    gx, gf = run(sim, WINDOW)
+    assert sim.plan().disp_b
+    pe, fe = errors(gx, gf, ref["x"], ref["f"], ref["R"])
+    print(f"node-fused step, {WINDOW} steps: pos_err {pe:.2e} of R, force_err {fe:.2e}")
     assert pe < 1e-4 and fe < 1e-4
 
 
