@@ -3,8 +3,10 @@
 
 Workload (default): the large single cell -- R 10 um, lattice 0.2 um, seed 2 (~0.52M nodes,
 ~3.3M tetra), Mooney-Rivlin + viscoplastic, substrate and spherical tip, FP32 state.  A step is
-one Simulation.timeStep of the whole cell (force pass + node pass); steps are replayed from a
-captured hipGraph.  Inputs are resident in HBM before the timed region starts.
+one Simulation.timeStep of the whole cell (force pass + node pass, as two launches or, in the
+node-fused form, as one launch carrying the previous step's node update: the autotuner picks the
+form, the tile size and the cache policy); steps are replayed from a captured hipGraph.  Inputs are
+resident in HBM before the timed region starts.
 
 Printed JSON line (one):
 - value = tetra-updates/s = T * steps / wall seconds of the timed region (the steps' graphs are
@@ -14,13 +16,15 @@ Printed JSON line (one):
   the plastic state 36 B and epsbar 4 B each read and written, per tetra; x, v, f, f_old per node),
   56 T + 108 N for the elastic one;
 - roofline: the dominant kernel (the force pass, ``k_tile_force``): those algorithmic bytes per
-  launch (136 T + 24 N viscoplastic, 56 T + 24 N elastic) over its HIP-event average duration,
+  launch (136 T + 24 N viscoplastic, 56 T + 24 N elastic; the whole step's bytes when the launch is
+  node-fused) over its HIP-event average duration,
   against 8 TB/s; ``kernel_bytes_per_launch`` is what the kernel itself must move (it reads per-
   corner tile records instead of the connectivity and derives V0 from Dm^-1); ``traffic`` = HBM
   bytes per launch measured by rocprofv3 (FETCH_SIZE + WRITE_SIZE, calibrated) in a separate
   profiling run of this same command, read from ``--traffic-json`` (``traffic_source`` says which);
 - cpu_baseline: the FP64 C oracle (OpenMP, all allowed host threads) on a bounded step sample
   of the same cell;
+- other_step_form: the same window in the step form the tuner did not pick;
 - plain_small: the small cell (R 8 um, 0.8 um lattice, ~25k tetra) fused step, for comparison;
 - batched: the 512-cell workload (10k steps each, 256-point curve per cell);
 - surrogate: MLP inference / training-step throughput on the MFMA kernels and a surrogate-assisted
