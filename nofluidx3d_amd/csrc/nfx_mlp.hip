@@ -126,10 +126,12 @@ __global__ __launch_bounds__(256) void k_gemm_nt(nfx_mlp_gemm_args g) {
 // 16 MFMAs per k-step from 8 LDS fragment reads instead of 4 MFMAs per 4 global reads).  Double
 // buffered: the next k-slice goes global -> registers during the MFMAs and registers -> the other
 // LDS buffer after them, one barrier per k-step.  Rows are padded by 8 BF16 against bank
-// conflicts.  Accumulator layout and epilogue as k_gemm_nt.
+// conflicts.  Accumulator layout and epilogue as k_gemm_nt.  Four waves per SIMD (the accumulators
+// then live in VGPRs, 124 of them; the 40 KB of LDS per workgroup fit four workgroups per CU): 13-30 %
+// faster than the three the unbounded allocation gives.
 constexpr int kLdsTile = 128, kLdsK = 32, kLdsPitch = kLdsK + 8;
 template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_lds(nfx_mlp_gemm_args g) {
+__global__ __launch_bounds__(256, 4) void k_gemm_lds(nfx_mlp_gemm_args g) {
     __shared__ __align__(16) __bf16 As[2][kLdsTile * kLdsPitch];
     __shared__ __align__(16) __bf16 Bs[2][kLdsTile * kLdsPitch];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
