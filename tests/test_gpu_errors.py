@@ -102,6 +102,14 @@ def test_bad_plans_are_refused_and_leave_the_state(ctx):
         for use_graph in (0, 1):
             refused("nfx_run_steps", ctx, ctypes.byref(q), 5, 3, None, use_graph, expect=expect)
     refused("nfx_run_steps", ctx, ctypes.byref(p), 5, -1, None, 0, expect="nsteps < 0")
+    # the node-fused arrays (this small cell takes the node-fused form): incomplete, aliasing the
+    # plan's own buffers, or with a law the fused form does not cover
+    assert p.disp_b and p.tile_range
+    for changes, expect in (({"tile_range": None}, "are set together"), ({"vel_b": None}, "are set together"),
+                            ({"disp_b": p.disp}, "must differ from disp, vel and partials"),
+                            ({"model": _lib.NFX_LINEAR_ELASTIC}, "node-fused steps need the tiled scatter")):
+        q = copy_plan(p, **changes)
+        refused("nfx_run_steps", ctx, ctypes.byref(q), 5, 3, None, 1, expect=expect)
     assert np.array_equal(positions(a), before)
     a.advance(7)
     b.advance(7)
